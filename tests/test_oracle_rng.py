@@ -1,0 +1,79 @@
+"""Pins the RNG oracle: Random123 known answers and the values printed in mocat.pdf p.125-126."""
+import numpy as np
+
+from oracle import rng
+
+
+def test_threefry_random123_kat():
+    # Random123 kat_vectors for threefry2x32_20 (also jax's own threefry test vectors)
+    cases = [
+        ((0, 0), (0, 0), (0x6B200159, 0x99BA4EFE)),
+        ((0xFFFFFFFF, 0xFFFFFFFF), (0xFFFFFFFF, 0xFFFFFFFF), (0x1CB996FC, 0xBB002BE7)),
+        ((0x13198A2E, 0x03707344), (0x243F6A88, 0x85A308D3), (0xC4923A9C, 0x483DF7A0)),
+    ]
+    for key, ctr, want in cases:
+        a, b = rng.threefry2x32(key[0], key[1], np.array([ctr[0]]), np.array([ctr[1]]))
+        assert (int(a[0]), int(b[0])) == want
+
+
+def test_prngkey():
+    assert rng.PRNGKey(0).tolist() == [0, 0]
+    assert rng.PRNGKey(7).tolist() == [0, 7]
+
+
+# mocat.pdf p.125: normal(PRNGKey(0)); split(PRNGKey(0), 3) -> normal(s1), normal(s2)
+PDF_SCALARS = (-0.20584235, 0.5781487, 0.85355157)
+# mocat.pdf p.126: x = -5 + 0.1 * normal(s1, (4, 2)); momenta = normal(s2, (4, 2))
+PDF_X = np.array([[-5.1619368, -4.871614], [-5.1136875, -5.048856],
+                  [-5.0195227, -4.8458843], [-5.037027, -5.017855]], np.float32)
+PDF_P = np.array([[-0.38537452, -1.4707391], [0.5467919, 2.095505],
+                  [1.1165614, 0.16117463], [-0.5371375, -0.89213836]], np.float32)
+
+
+def test_pdf_golden_values():
+    k = rng.PRNGKey(0)
+    _, s1, s2 = rng.split(k, 3)
+    got = [float(rng.normal(k)), float(rng.normal(s1)), float(rng.normal(s2))]
+    np.testing.assert_allclose(got, PDF_SCALARS, rtol=5e-7)
+    x = np.float32(-5) + np.float32(0.1) * rng.normal(s1, (4, 2))
+    # printed to 8 significant digits at magnitude 5 -> exact to the f32 grid there
+    np.testing.assert_array_equal(x, PDF_X)
+    np.testing.assert_allclose(rng.normal(s2, (4, 2)), PDF_P, rtol=5e-7)
+    np.testing.assert_allclose(rng.normal_exact(s2, (4, 2)), PDF_P, rtol=5e-7)
+
+
+def test_split_layout_is_halves():
+    # split(k, 2) uses counters (0, 2) and (1, 3): a plain interleaved layout would differ
+    k = rng.PRNGKey(0)
+    s = rng.split(k, 2)
+    a0, a1 = rng.threefry2x32(0, 0, np.array([0, 1]), np.array([2, 3]))
+    assert s.tolist() == [[int(a0[0]), int(a0[1])], [int(a1[0]), int(a1[1])]]
+
+
+def test_odd_size_padding():
+    k = rng.PRNGKey(3)
+    b = rng.bits(k, 5)
+    # halves of [0,1,2,3,4,0]: pairs (0,3), (1,4), (2,0)
+    a, c = rng.threefry2x32(0, 3, np.array([0, 1, 2]), np.array([3, 4, 0]))
+    assert b.tolist() == [int(v) for v in np.concatenate([a, c[:2]])]
+
+
+def test_uniform_range_and_erfinv_against_scipy():
+    from scipy.special import erfinv
+    k = rng.PRNGKey(11)
+    u = rng.uniform(k, (100000,))
+    assert u.min() >= 0 and u.max() < 1
+    x = rng.uniform(k, (100000,), rng.NORMAL_LO, 1.0)
+    ref = erfinv(x.astype(np.float64))
+    got = rng.erfinv_f32(x).astype(np.float64)
+    rel = np.abs(got - ref) / np.maximum(np.abs(ref), 1e-30)
+    # Giles reports ~1e-6 relative in the central branch; the tail branch (|x| -> 1) is looser
+    assert np.median(rel) < 2e-7 and rel.max() < 1e-5
+
+
+def test_categorical_matches_bruteforce_argmax():
+    k = rng.PRNGKey(5)
+    logits = np.log(np.array([0.1, 0.2, 0.3, 0.4]))
+    idx = rng.categorical(k, logits, 2000)
+    freq = np.bincount(idx, minlength=4) / 2000
+    np.testing.assert_allclose(freq, [0.1, 0.2, 0.3, 0.4], atol=0.04)
