@@ -40,6 +40,41 @@ int mocat_random_uniform(const uint32_t* keys, int64_t n_keys, int64_t size, flo
 /* normal(keys[i], (size,)) -> out[i][size] (f32). */
 int mocat_random_normal(const uint32_t* keys, int64_t n_keys, int64_t size, float* out, hipStream_t stream);
 
+/* ------------------------------------------------------------------------ logistic regression
+ * Fused potential + adjoint of the Bayesian logistic-regression Scenario (user Scenario pattern,
+ * README.md:35-49; gradients replace jax.grad in mocat/src/core.py:204-230):
+ *   prior = 0.5*prec*|t|^2 + 0.5*d*(log 2pi - log prec), lik = sum softplus(x.t) - y*x.t,
+ *   grad_lik = X^T (sigmoid(X t) - y). X is [n][d] f32 row-major, y [n] f32, d <= 128.
+ * precision: 0 = FP32 vector FMAs (reads X, y), 1 = bf16x3 MFMA (reads the packed buffer). */
+
+/* Bytes needed for the packed (bf16 hi/lo, MFMA operand order) copy of X and y. */
+int mocat_lr_packed_bytes(int64_t n, int64_t d, int64_t* bytes);
+/* Writes the packed copy of X, y into `packed` (device, mocat_lr_packed_bytes bytes). */
+int mocat_lr_pack(const float* X, const float* y, int64_t n, int64_t d, void* packed, hipStream_t stream);
+/* For C chains theta[C][d]: prior_pot[C], lik_pot[C], grad_prior[C][d], grad_lik[C][d]
+ * (any output may be NULL). Replaces vmap(Scenario.*_potential_and_grad), mocat/src/core.py:204-230. */
+int mocat_lr_potential_grad(const float* X, const float* y, const void* packed, int64_t n, int64_t d,
+                            int precision, float prior_prec, float temperature, const float* theta, int64_t C,
+                            float* prior_pot, float* lik_pot, float* grad_prior, float* grad_lik,
+                            hipStream_t stream);
+
+/* n_iter iterations of MCMCSampler.update for C independent chains of Underdamped (HMC when
+ * friction = inf; MALA when also leapfrog_steps = 1) on the logistic-regression scenario:
+ * always (mocat/src/mcmc/standard_mcmc.py:107-123), leapfrog (mocat/src/utils.py:108-146),
+ * proposal/acceptance (standard_mcmc.py:125-153), correction (mocat/src/mcmc/metropolis.py:48-161).
+ * correction: 0 uncorrected, 1 Metropolis, 2 RMMetropolis (log Robbins-Monro, step iter^-exp*scale).
+ * State arrays (in/out): theta, momenta, grad [C][d]; potential, prior_pot, lik_pot, stepsize,
+ * alpha [C]; keys [C][2]. iter0 = extra.iter before the first iteration. Optional traces
+ * (NULL to skip): trace_value/trace_momenta [n_iter][C][d], trace_potential/alpha/stepsize
+ * [n_iter][C]. */
+int mocat_hmc_lr_iterate(const float* X, const float* y, const void* packed, int64_t n, int64_t d, int precision,
+                         float prior_prec, float temperature, int64_t C, float* theta, float* momenta, float* grad,
+                         float* potential, float* prior_pot, float* lik_pot, float* stepsize, float* alpha,
+                         uint32_t* keys, int64_t iter0, int n_iter, int leapfrog_steps, float friction,
+                         int correction, float target, float rm_scale, float rm_neg_exponent, double adapt_cut_off,
+                         float* trace_value, float* trace_potential, float* trace_alpha, float* trace_stepsize,
+                         float* trace_momenta, hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

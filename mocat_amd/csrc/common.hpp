@@ -93,9 +93,12 @@ MOCAT_DEV float unit_float(uint32_t bits) {
 }
 
 // uniform(key, minval, maxval): max(minval, f * (maxval - minval) + minval), rounded step by step.
+// HIP compiles with fp-contract=fast; the pragma keeps multiply and add separately rounded.
 MOCAT_DEV float uniform_from_bits(uint32_t bits, float lo, float hi) {
+#pragma clang fp contract(off)
   const float f = unit_float(bits);
-  const float v = __fadd_rn(__fmul_rn(f, __fsub_rn(hi, lo)), lo);
+  const float span = hi - lo;
+  const float v = f * span + lo;
   return fmaxf(lo, v);
 }
 

@@ -1,0 +1,696 @@
+// Bayesian logistic regression on the GPU: fused potential + hand-written adjoint, and the fused
+// HMC / MALA / underdamped-Langevin iteration built on it.
+//
+// Model (a user Scenario in the reference's README pattern, README.md:35-49):
+//   prior_potential(theta)      = 0.5 * prec * |theta|^2 + 0.5 * d * (log 2pi - log prec)
+//   likelihood_potential(theta) = sum_i softplus(x_i . theta) - y_i * (x_i . theta)
+//   grad_likelihood             = X^T (sigmoid(X theta) - y)
+// Chains/particles are processed in tiles of 32: the contraction over the data rows is the hot loop.
+//
+// Two engines evaluate (U, grad U) for a tile:
+//   Fp32Engine - data rows staged through LDS in chunks, FP32 FMAs on the vector ALU.
+//   Bf16Engine - MFMA v_mfma_f32_32x32x16_bf16 with split precision: every operand is
+//                hi + lo bf16 and each product is hi*hi + hi*lo + lo*hi in FP32 accumulation.
+//                Z = X Theta^T is accumulated with rows of X on the M side and the 32 chains on
+//                N, so each lane's accumulator column is one chain. sigmoid(Z) - y is formed in
+//                registers and reused directly as the A operand (chains x rows) of
+//                G = R X: the K slots of that operand are the rows the lane already holds, and
+//                X's second packed layout lists its rows in the same order.
+//
+// The iteration kernel follows mocat's Underdamped + Metropolis/RMMetropolis update
+// (mocat/src/mcmc/standard_mcmc.py:107-153, mocat/src/utils.py:108-146,
+//  mocat/src/mcmc/metropolis.py:48-161), one chain per (tile, slot), key tree per chain as mocat's
+// single-chain run (mocat/src/mcmc/sampler.py:92-111).
+#include "common.hpp"
+
+using namespace mocat;
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+namespace {
+
+constexpr int CT = 32;  // chains per tile
+constexpr float kLog2Pi = 1.83787706640934548356f;
+
+MOCAT_DEV int acc_row(int r, int lane) { return 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3); }
+
+// --- per-row likelihood pieces -------------------------------------------------------------
+// r = sigmoid(z) - y ; c = softplus(z) - y z. Rows with y < 0 are padding: r = c = 0.
+MOCAT_DEV void lr_row(float z, float y, float& r, float& c) {
+  if (y < 0.f) {
+    r = 0.f;
+    c = 0.f;
+    return;
+  }
+  const float e = __expf(-fabsf(z));
+  const float inv = __frcp_rn(1.f + e);
+  const float sig = z >= 0.f ? inv : e * inv;
+  r = sig - y;
+  c = fmaxf(z, 0.f) + log1pf(e) - y * z;
+}
+
+// ------------------------------------------------------------------------------------------
+// FP32 engine.  LDS: X chunk [RC][DP+4], R chunk [CT][RC+4].
+template <int DP, int NT>
+struct Fp32Engine {
+  static constexpr int RC = 64;
+  static constexpr int XS = DP + 4;
+  static constexpr int RS = RC + 4;
+  struct alignas(16) Smem {
+    alignas(16) float x[RC * XS];
+    alignas(16) float r[CT * RS];
+    float y[RC];
+    float red[NT / kWave][CT];
+  };
+
+  // theta: LDS [CT][DP] (zero beyond d). Outputs: lik[CT], glik[CT][DP] in LDS.
+  static MOCAT_DEV void eval(Smem& s, const float* __restrict__ X, const float* __restrict__ y, const void*,
+                             int n, int d, const float* theta, float* lik, float* glik) {
+    const int t = threadIdx.x;
+    const int lane = t & (kWave - 1), w = t / kWave;
+    constexpr int NW = NT / kWave;
+    constexpr int P1 = CT / NW;         // chains per thread in phase 1 (row = lane)
+    constexpr int KPT = DP / kWave;     // k columns per thread in phase 2 (1 or 2)
+    float gacc[P1][KPT];
+    float cacc[P1];
+#pragma unroll
+    for (int m = 0; m < P1; ++m) {
+      cacc[m] = 0.f;
+#pragma unroll
+      for (int q = 0; q < KPT; ++q) gacc[m][q] = 0.f;
+    }
+    for (int i0 = 0; i0 < n; i0 += RC) {
+      __syncthreads();
+      // stage X rows [i0, i0+RC) and y
+      for (int e = t; e < RC * DP; e += NT) {
+        const int i = e / DP, k = e - i * DP;
+        const int gi = i0 + i;
+        s.x[i * XS + k] = (gi < n && k < d) ? X[(int64_t)gi * d + k] : 0.f;
+      }
+      for (int i = t; i < RC; i += NT) s.y[i] = (i0 + i < n) ? y[i0 + i] : -1.f;
+      __syncthreads();
+      // phase 1: z[c][i] for row i = lane, chains c = w + NW*m
+      {
+        float z[P1];
+#pragma unroll
+        for (int m = 0; m < P1; ++m) z[m] = 0.f;
+        const float* xr = &s.x[lane * XS];
+#pragma unroll 4
+        for (int k = 0; k < DP; k += 4) {
+          const floatx4 xv = *reinterpret_cast<const floatx4*>(xr + k);
+#pragma unroll
+          for (int m = 0; m < P1; ++m) {
+            const floatx4 tv = *reinterpret_cast<const floatx4*>(theta + (w + NW * m) * DP + k);
+            z[m] += xv.x * tv.x + xv.y * tv.y + xv.z * tv.z + xv.w * tv.w;
+          }
+        }
+        const float yv = s.y[lane];
+#pragma unroll
+        for (int m = 0; m < P1; ++m) {
+          float r, c;
+          lr_row(z[m], yv, r, c);
+          cacc[m] += c;
+          s.r[(w + NW * m) * RS + lane] = r;
+        }
+      }
+      __syncthreads();
+      // phase 2: g[c][k] += sum_i r[c][i] x[i][k], k = lane (+64), chains c = w + NW*m
+#pragma unroll 2
+      for (int i = 0; i < RC; i += 4) {
+        float xv[4][KPT];
+#pragma unroll
+        for (int ii = 0; ii < 4; ++ii)
+#pragma unroll
+          for (int q = 0; q < KPT; ++q) xv[ii][q] = s.x[(i + ii) * XS + lane + kWave * q];
+#pragma unroll
+        for (int m = 0; m < P1; ++m) {
+          const floatx4 rv = *reinterpret_cast<const floatx4*>(&s.r[(w + NW * m) * RS + i]);
+#pragma unroll
+          for (int q = 0; q < KPT; ++q)
+            gacc[m][q] += rv.x * xv[0][q] + rv.y * xv[1][q] + rv.z * xv[2][q] + rv.w * xv[3][q];
+        }
+      }
+    }
+    // outputs: each (chain, k) owned by exactly one thread; potentials reduced over lanes
+#pragma unroll
+    for (int m = 0; m < P1; ++m) {
+      const int c = w + NW * m;
+#pragma unroll
+      for (int q = 0; q < KPT; ++q) glik[c * DP + lane + kWave * q] = gacc[m][q];
+      const float cs = wave_sum(cacc[m]);
+      if (lane == 0) lik[c] = cs;
+    }
+    __syncthreads();
+  }
+};
+
+// ------------------------------------------------------------------------------------------
+// BF16 split-precision MFMA engine.
+// Packed data per block of 32 rows (rb):
+//   A layout [2 parts][KB][64 lanes][8]   lane l: row l%32, columns kb*16 + (l/32)*8 + s
+//   B layout [2 parts][2][NB][64][8]      lane l: column nb*32 + l%32, rows acc_row(kb2*8+s, l)
+//   y layout [2][16] f32                  g = l/32: y[acc_row(r, l)], -1 on padding rows
+template <int DP>
+struct PackLayout {
+  static constexpr int KB = DP / 16;
+  static constexpr int NB = DP / 32;
+  static constexpr int A_ELEMS = 2 * KB * 512;
+  static constexpr int B_ELEMS = 2 * 2 * NB * 512;
+  static constexpr int BLOCK_BF16 = A_ELEMS + B_ELEMS;
+  static constexpr int BLOCK_BYTES = BLOCK_BF16 * 2 + 32 * 4;
+};
+
+MOCAT_DEV void split_bf16(float v, __bf16& hi, __bf16& lo) {
+  hi = (__bf16)v;
+  lo = (__bf16)(v - (float)hi);
+}
+
+template <int DP, int NT>
+struct Bf16Engine {
+  using L = PackLayout<DP>;
+  static constexpr int KB = L::KB, NB = L::NB, NW = NT / kWave, HW = NW / 2;
+  static_assert(NW % 2 == 0, "Bf16Engine needs an even wave count");
+  struct alignas(16) Smem {
+    alignas(16) float gred[HW][CT * DP];
+    float ured[NW][2][CT];
+  };
+
+  static MOCAT_DEV void eval(Smem& s, const float*, const float*, const void* packed, int n, int d,
+                             const float* theta, float* lik, float* glik) {
+    const int t = threadIdx.x;
+    const int lane = t & (kWave - 1), w = t / kWave, g = lane >> 5, col = lane & 31;
+    const int nrb = (n + 31) / 32;
+    const char* base = reinterpret_cast<const char*>(packed);
+
+    // theta^T as B operand: lane holds chain `col`, columns kb*16 + g*8 + s
+    bf16x8 th[KB], tl[KB];
+#pragma unroll
+    for (int kb = 0; kb < KB; ++kb) {
+      const float* src = theta + col * DP + kb * 16 + g * 8;
+#pragma unroll
+      for (int s2 = 0; s2 < 8; ++s2) {
+        __bf16 h, l;
+        split_bf16(src[s2], h, l);
+        th[kb][s2] = h;
+        tl[kb][s2] = l;
+      }
+    }
+    floatx16 gacc[NB];
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) gacc[nb][r] = 0.f;
+    float upart = 0.f;
+
+    for (int rb = w; rb < nrb; rb += NW) {
+      const char* blk = base + (int64_t)rb * L::BLOCK_BYTES;
+      const bf16x8* A = reinterpret_cast<const bf16x8*>(blk);
+      const bf16x8* B = A + L::A_ELEMS / 8;
+      const float* Y = reinterpret_cast<const float*>(blk + L::BLOCK_BF16 * 2);
+      floatx16 z;
+#pragma unroll
+      for (int r = 0; r < 16; ++r) z[r] = 0.f;
+#pragma unroll
+      for (int kb = 0; kb < KB; ++kb) {
+        const bf16x8 ah = A[(0 * KB + kb) * 64 + lane];
+        const bf16x8 al = A[(1 * KB + kb) * 64 + lane];
+        z = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, th[kb], z, 0, 0, 0);
+        z = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, tl[kb], z, 0, 0, 0);
+        z = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, th[kb], z, 0, 0, 0);
+      }
+      const floatx4* yv4 = reinterpret_cast<const floatx4*>(Y + g * 16);
+      float yv[16];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const floatx4 v = yv4[q];
+        yv[4 * q] = v.x;
+        yv[4 * q + 1] = v.y;
+        yv[4 * q + 2] = v.z;
+        yv[4 * q + 3] = v.w;
+      }
+      bf16x8 rh[2], rl[2];
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        float rr, cc;
+        lr_row(z[r], yv[r], rr, cc);
+        upart += cc;
+        __bf16 h, l;
+        split_bf16(rr, h, l);
+        rh[r >> 3][r & 7] = h;
+        rl[r >> 3][r & 7] = l;
+      }
+#pragma unroll
+      for (int kb2 = 0; kb2 < 2; ++kb2) {
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) {
+          const bf16x8 xh = B[((0 * 2 + kb2) * NB + nb) * 64 + lane];
+          const bf16x8 xl = B[((1 * 2 + kb2) * NB + nb) * 64 + lane];
+          gacc[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(rh[kb2], xh, gacc[nb], 0, 0, 0);
+          gacc[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(rh[kb2], xl, gacc[nb], 0, 0, 0);
+          gacc[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(rl[kb2], xh, gacc[nb], 0, 0, 0);
+        }
+      }
+    }
+    // deterministic cross-wave reduction: upper half adds into lower half, then ordered sum
+    s.ured[w][g][col] = upart;
+    __syncthreads();
+    if (w >= HW) {
+#pragma unroll
+      for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) s.gred[w - HW][acc_row(r, lane) * DP + nb * 32 + col] = gacc[nb][r];
+    }
+    __syncthreads();
+    if (w < HW) {
+#pragma unroll
+      for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          float* p = &s.gred[w][acc_row(r, lane) * DP + nb * 32 + col];
+          *p = gacc[nb][r] + *p;
+        }
+    }
+    __syncthreads();
+    for (int e = t; e < CT * DP; e += NT) {
+      float acc = s.gred[0][e];
+#pragma unroll
+      for (int q = 1; q < HW; ++q) acc += s.gred[q][e];
+      glik[e] = acc;
+    }
+    if (t < CT) {
+      float acc = 0.f;
+      for (int q = 0; q < NW; ++q) acc += s.ured[q][0][t] + s.ured[q][1][t];
+      lik[t] = acc;
+    }
+    __syncthreads();
+  }
+};
+
+// Pack X (n x d, row-major f32) and y into the Bf16Engine layout, zero padded to DP columns and a
+// multiple of 32 rows. One thread per packed element.
+template <int DP>
+__global__ void lr_pack_kernel(const float* __restrict__ X, const float* __restrict__ y, int n, int d,
+                               char* __restrict__ out) {
+  using L = PackLayout<DP>;
+  const int nrb = (n + 31) / 32;
+  const int64_t per = L::BLOCK_BF16 + 32;
+  const int64_t total = (int64_t)nrb * per;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int rb = (int)(e / per);
+    const int q = (int)(e - rb * per);
+    char* blk = out + (int64_t)rb * L::BLOCK_BYTES;
+    if (q < L::A_ELEMS) {
+      const int s = q & 7, lane = (q >> 3) & 63, pk = q >> 9;  // pk = part*KB + kb
+      const int part = pk / L::KB, kb = pk - part * L::KB;
+      const int row = rb * 32 + (lane & 31), colm = kb * 16 + (lane >> 5) * 8 + s;
+      const float v = (row < n && colm < d) ? X[(int64_t)row * d + colm] : 0.f;
+      __bf16 h, l;
+      split_bf16(v, h, l);
+      reinterpret_cast<__bf16*>(blk)[q] = part ? l : h;
+    } else if (q < L::BLOCK_BF16) {
+      const int qq = q - L::A_ELEMS;
+      const int s = qq & 7, lane = (qq >> 3) & 63, rest = qq >> 9;  // rest = (part*2 + kb2)*NB + nb
+      const int nb = rest % L::NB, pk2 = rest / L::NB, kb2 = pk2 & 1, part = pk2 >> 1;
+      const int row = rb * 32 + acc_row(kb2 * 8 + s, lane), colm = nb * 32 + (lane & 31);
+      const float v = (row < n && colm < d) ? X[(int64_t)row * d + colm] : 0.f;
+      __bf16 h, l;
+      split_bf16(v, h, l);
+      reinterpret_cast<__bf16*>(blk)[q] = part ? l : h;
+    } else {
+      const int qq = q - L::BLOCK_BF16;  // 0..31 = g*16 + r
+      const int gg = qq >> 4, r = qq & 15;
+      const int row = rb * 32 + 8 * (r >> 2) + 4 * gg + (r & 3);
+      reinterpret_cast<float*>(blk + L::BLOCK_BF16 * 2)[qq] = row < n ? y[row] : -1.f;
+    }
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+// Tile state shared by the evaluation and iteration kernels.
+template <int DP>
+struct alignas(16) TileState {
+  alignas(16) float theta[CT * DP];
+  float mom[CT * DP];
+  float grad[CT * DP];  // grad of the tempered potential
+  float mom0[CT * DP];  // momenta after refresh (kept on rejection)
+  float glik[CT * DP];
+  float lik[CT];
+  float pot[CT];
+  float prior[CT];
+  float likv[CT];
+  float pot0[CT];
+  float prior0[CT];
+  float lik0[CT];
+  float step[CT];
+  float alpha[CT];
+  float kin0[CT];
+  uint32_t key[CT][2];
+};
+
+struct LrArgs {
+  const float* X;
+  const float* y;
+  const void* packed;
+  int n, d;
+  float prior_prec, temperature;
+};
+
+// Evaluates prior/likelihood potentials and gradients of the tile's theta (in LDS).
+template <class Engine, int DP, int NT>
+MOCAT_DEV void tile_eval(typename Engine::Smem& es, TileState<DP>& ts, const LrArgs& a) {
+  Engine::eval(es, a.X, a.y, a.packed, a.n, a.d, ts.theta, ts.lik, ts.glik);
+  const int t = threadIdx.x;
+  for (int e = t; e < CT * DP; e += NT) ts.grad[e] = a.prior_prec * ts.theta[e] + a.temperature * ts.glik[e];
+  if (t < CT) {
+    float sq = 0.f;
+    for (int k = 0; k < a.d; ++k) sq += ts.theta[t * DP + k] * ts.theta[t * DP + k];
+    const float pr = 0.5f * a.prior_prec * sq + 0.5f * (float)a.d * (kLog2Pi - logf(a.prior_prec));
+    ts.prior[t] = pr;
+    ts.likv[t] = ts.lik[t];
+    ts.pot[t] = pr + a.temperature * ts.lik[t];
+  }
+  __syncthreads();
+}
+
+template <class Engine, int DP, int NT>
+__global__ __launch_bounds__(NT) void lr_eval_kernel(LrArgs a, const float* __restrict__ theta, int64_t C,
+                                                     float* prior_pot, float* lik_pot, float* grad_prior,
+                                                     float* grad_lik) {
+  __shared__ TileState<DP> ts;
+  __shared__ typename Engine::Smem es;
+  const int t = threadIdx.x;
+  const int64_t c0 = (int64_t)blockIdx.x * CT;
+  for (int e = t; e < CT * DP; e += NT) {
+    const int c = e / DP, k = e - c * DP;
+    ts.theta[e] = (c0 + c < C && k < a.d) ? theta[(c0 + c) * a.d + k] : 0.f;
+  }
+  __syncthreads();
+  tile_eval<Engine, DP, NT>(es, ts, a);
+  for (int e = t; e < CT * DP; e += NT) {
+    const int c = e / DP, k = e - c * DP;
+    if (c0 + c < C && k < a.d) {
+      if (grad_lik) grad_lik[(c0 + c) * a.d + k] = ts.glik[e];
+      if (grad_prior) grad_prior[(c0 + c) * a.d + k] = a.prior_prec * ts.theta[e];
+    }
+  }
+  if (t < CT && c0 + t < C) {
+    if (prior_pot) prior_pot[c0 + t] = ts.prior[t];
+    if (lik_pot) lik_pot[c0 + t] = ts.likv[t];
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+struct HmcArgs {
+  LrArgs lr;
+  int64_t C;
+  float* theta;
+  float* mom;
+  float* grad;
+  float* pot;
+  float* prior;
+  float* lik;
+  float* step;
+  float* alpha;
+  uint32_t* keys;
+  int64_t iter0;
+  int n_iter;
+  int leapfrog_steps;
+  float friction;
+  int correction;  // 0 none, 1 Metropolis, 2 Robbins-Monro-adapted Metropolis (log update)
+  float target;
+  float rm_scale, rm_neg_exponent;
+  double adapt_cut_off;
+  float* tr_value;
+  float* tr_pot;
+  float* tr_alpha;
+  float* tr_step;
+  float* tr_mom;
+};
+
+template <class Engine, int DP, int NT>
+__global__ __launch_bounds__(NT) void hmc_lr_kernel(HmcArgs h) {
+  __shared__ TileState<DP> ts;
+  __shared__ typename Engine::Smem es;
+  const int t = threadIdx.x;
+  const int d = h.lr.d;
+  const int64_t c0 = (int64_t)blockIdx.x * CT;
+  const int nvalid = (int)((h.C - c0) < CT ? (h.C - c0) : CT);
+
+  // load the tile (padding chains are zero and never written back)
+  for (int e = t; e < CT * DP; e += NT) {
+    const int c = e / DP, k = e - c * DP;
+    const bool v = c < nvalid && k < d;
+    ts.theta[e] = v ? h.theta[(c0 + c) * d + k] : 0.f;
+    ts.mom[e] = v ? h.mom[(c0 + c) * d + k] : 0.f;
+    ts.grad[e] = v ? h.grad[(c0 + c) * d + k] : 0.f;
+  }
+  if (t < CT) {
+    const bool v = t < nvalid;
+    ts.pot[t] = v ? h.pot[c0 + t] : 0.f;
+    ts.prior[t] = v ? h.prior[c0 + t] : 0.f;
+    ts.likv[t] = v ? h.lik[c0 + t] : 0.f;
+    ts.step[t] = v ? h.step[c0 + t] : 1.f;  // padding chains: finite dummy stepsize
+    ts.alpha[t] = v ? h.alpha[c0 + t] : 0.f;
+    ts.key[t][0] = v ? h.keys[2 * (c0 + t)] : 0u;
+    ts.key[t][1] = v ? h.keys[2 * (c0 + t) + 1] : 0u;
+  }
+  __syncthreads();
+
+  const uint32_t hd = (uint32_t)(d + 1) >> 1;  // threefry blocks per normal(sub, (d,))
+  for (int it = 0; it < h.n_iter; ++it) {
+    const int64_t iter = h.iter0 + it + 1;
+    // ---- always: p <- -p ; (key, sub) = split(key) ; OU refresh (standard_mcmc.py:107-123)
+    __shared__ uint32_t sub[CT][2];
+    if (t < CT) {
+      Key k{ts.key[t][0], ts.key[t][1]};
+      Key nk = split_at(k, 2, 0), sk = split_at(k, 2, 1);
+      ts.key[t][0] = nk.a;
+      ts.key[t][1] = nk.b;
+      sub[t][0] = sk.a;
+      sub[t][1] = sk.b;
+      // keys = split(key, L+1); key = keys[0] (proposal, standard_mcmc.py:129-131)
+      Key pk = split_at(nk, (uint32_t)h.leapfrog_steps + 1u, 0);
+      ts.key[t][0] = pk.a;
+      ts.key[t][1] = pk.b;
+    }
+    __syncthreads();
+    for (int e = t; e < CT * (int)hd; e += NT) {
+      const int c = e / hd;
+      const uint32_t i = (uint32_t)(e - c * hd);
+      uint32_t w0, w1;
+      bits_pair(Key{sub[c][0], sub[c][1]}, (uint32_t)d, i, w0, w1);
+      const float hstep = ts.step[c];
+      const float decay = __expf(-h.friction * hstep);
+      const float noise = sqrtf(1.f - __expf(-2.f * h.friction * hstep));
+      float* p = &ts.mom[c * DP];
+      p[i] = -p[i] * decay + noise * normal_from_bits(w0);
+      if (i + hd < (uint32_t)d) p[i + hd] = -p[i + hd] * decay + noise * normal_from_bits(w1);
+    }
+    __syncthreads();
+    // keep the pre-proposal state for rejection
+    for (int e = t; e < CT * DP; e += NT) ts.mom0[e] = ts.mom[e];
+    if (t < CT) {
+      ts.pot0[t] = ts.pot[t];
+      ts.prior0[t] = ts.prior[t];
+      ts.lik0[t] = ts.likv[t];
+      float kk = 0.f;
+      for (int k = 0; k < d; ++k) kk += ts.mom[t * DP + k] * ts.mom[t * DP + k];
+      ts.kin0[t] = 0.5f * kk + 0.5f * (float)d * kLog2Pi;
+    }
+    // the pre-proposal position/gradient stay in global memory (unchanged until accepted)
+    __syncthreads();
+    // ---- leapfrog (utils.py:117-146)
+    for (int s = 0; s < h.leapfrog_steps; ++s) {
+      for (int e = t; e < CT * DP; e += NT) {
+        const int c = e / DP;
+        const float hh = ts.step[c];
+        const float ph = ts.mom[e] - (hh / 2.f) * ts.grad[e];
+        ts.mom[e] = ph;
+        ts.theta[e] = ((e - c * DP) < d) ? ts.theta[e] + hh * ph : 0.f;
+      }
+      __syncthreads();
+      tile_eval<Engine, DP, NT>(es, ts, h.lr);
+      for (int e = t; e < CT * DP; e += NT) {
+        const int c = e / DP;
+        ts.mom[e] = ts.mom[e] - (ts.step[c] / 2.f) * ts.grad[e];
+      }
+      __syncthreads();
+    }
+    // ---- correction (metropolis.py:48-70, 118-161)
+    __shared__ int accept[CT];
+    if (t < CT) {
+      float kk = 0.f;
+      for (int k = 0; k < d; ++k) kk += ts.mom[t * DP + k] * ts.mom[t * DP + k];
+      const float kin1 = 0.5f * kk + 0.5f * (float)d * kLog2Pi;  // |-p|^2 == |p|^2
+      float a = 1.f;
+      int acc = 1;
+      if (h.correction > 0) {
+        a = fminf(1.f, __expf(-ts.pot[t] + ts.pot0[t] - kin1 + ts.kin0[t]));
+        a = isnan(a) ? 0.f : a;
+        Key k{ts.key[t][0], ts.key[t][1]};
+        Key nk = split_at(k, 2, 0), sk = split_at(k, 2, 1);
+        ts.key[t][0] = nk.a;
+        ts.key[t][1] = nk.b;
+        const float u = uniform_from_bits(bits_at(sk, 1, 0), 0.f, 1.f);
+        acc = u < a;
+      }
+      accept[t] = acc;
+      ts.alpha[t] = a;
+      if (!acc) {
+        ts.pot[t] = ts.pot0[t];
+        ts.prior[t] = ts.prior0[t];
+        ts.likv[t] = ts.lik0[t];
+      }
+      if (h.correction == 2 && (double)iter <= h.adapt_cut_off) {
+        const float rm = h.rm_scale * powf((float)iter, -h.rm_neg_exponent);
+        ts.step[t] = expf(logf(ts.step[t]) + rm * (a - h.target));
+      }
+    }
+    __syncthreads();
+    // accepted: momenta <- -p'; rejected: restore position/grad from global, momenta = refreshed
+    for (int e = t; e < CT * DP; e += NT) {
+      const int c = e / DP, k = e - c * DP;
+      if (accept[c]) {
+        ts.mom[e] = -ts.mom[e];
+        if (c < nvalid && k < d) {
+          h.theta[(c0 + c) * d + k] = ts.theta[e];
+          h.grad[(c0 + c) * d + k] = ts.grad[e];
+        }
+      } else {
+        ts.mom[e] = ts.mom0[e];
+        if (c < nvalid && k < d) {
+          ts.theta[e] = h.theta[(c0 + c) * d + k];
+          ts.grad[e] = h.grad[(c0 + c) * d + k];
+        } else {
+          ts.theta[e] = 0.f;
+          ts.grad[e] = 0.f;
+        }
+      }
+      if (c < nvalid && k < d) {
+        if (h.tr_value) h.tr_value[((int64_t)it * h.C + c0 + c) * d + k] = ts.theta[e];
+        if (h.tr_mom) h.tr_mom[((int64_t)it * h.C + c0 + c) * d + k] = ts.mom[e];
+      }
+    }
+    if (t < nvalid) {
+      const int64_t o = (int64_t)it * h.C + c0 + t;
+      if (h.tr_pot) h.tr_pot[o] = ts.pot[t];
+      if (h.tr_alpha) h.tr_alpha[o] = ts.alpha[t];
+      if (h.tr_step) h.tr_step[o] = ts.step[t];
+    }
+    __syncthreads();
+  }
+  // write back the final per-chain state
+  for (int e = t; e < CT * DP; e += NT) {
+    const int c = e / DP, k = e - c * DP;
+    if (c < nvalid && k < d) h.mom[(c0 + c) * d + k] = ts.mom[e];
+  }
+  if (t < nvalid) {
+    h.pot[c0 + t] = ts.pot[t];
+    h.prior[c0 + t] = ts.prior[t];
+    h.lik[c0 + t] = ts.likv[t];
+    h.step[c0 + t] = ts.step[t];
+    h.alpha[c0 + t] = ts.alpha[t];
+    h.keys[2 * (c0 + t)] = ts.key[t][0];
+    h.keys[2 * (c0 + t) + 1] = ts.key[t][1];
+  }
+}
+
+constexpr int kThreads = 512;
+
+template <template <int, int> class E>
+int dispatch_eval(const LrArgs& a, const float* theta, int64_t C, float* pp, float* lp, float* gp, float* gl,
+                  hipStream_t st) {
+  const unsigned blocks = (unsigned)((C + CT - 1) / CT);
+  if (a.d <= 64)
+    hipLaunchKernelGGL((lr_eval_kernel<E<64, kThreads>, 64, kThreads>), dim3(blocks), dim3(kThreads), 0, st, a,
+                       theta, C, pp, lp, gp, gl);
+  else
+    hipLaunchKernelGGL((lr_eval_kernel<E<128, kThreads>, 128, kThreads>), dim3(blocks), dim3(kThreads), 0, st, a,
+                       theta, C, pp, lp, gp, gl);
+  MOCAT_LAUNCH_CHECK();
+  return MOCAT_OK;
+}
+
+template <template <int, int> class E>
+int dispatch_hmc(const HmcArgs& h, hipStream_t st) {
+  const unsigned blocks = (unsigned)((h.C + CT - 1) / CT);
+  if (h.lr.d <= 64)
+    hipLaunchKernelGGL((hmc_lr_kernel<E<64, kThreads>, 64, kThreads>), dim3(blocks), dim3(kThreads), 0, st, h);
+  else
+    hipLaunchKernelGGL((hmc_lr_kernel<E<128, kThreads>, 128, kThreads>), dim3(blocks), dim3(kThreads), 0, st, h);
+  MOCAT_LAUNCH_CHECK();
+  return MOCAT_OK;
+}
+
+int check_lr(const float* X, const float* y, const void* packed, int64_t n, int64_t d, int precision) {
+  if (n <= 0 || d <= 0 || d > 128 || n > (1ll << 30)) return MOCAT_ERR_ARG;
+  if (precision == 0) return (X && y) ? MOCAT_OK : MOCAT_ERR_ARG;
+  if (precision == 1) return packed ? MOCAT_OK : MOCAT_ERR_ARG;
+  return MOCAT_ERR_ARG;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mocat_lr_packed_bytes(int64_t n, int64_t d, int64_t* bytes) {
+  if (n <= 0 || d <= 0 || d > 128 || !bytes) return MOCAT_ERR_ARG;
+  const int64_t nrb = (n + 31) / 32;
+  *bytes = nrb * (d <= 64 ? PackLayout<64>::BLOCK_BYTES : PackLayout<128>::BLOCK_BYTES);
+  return MOCAT_OK;
+}
+
+int mocat_lr_pack(const float* X, const float* y, int64_t n, int64_t d, void* packed, hipStream_t stream) {
+  if (!X || !y || !packed || n <= 0 || d <= 0 || d > 128 || n > (1ll << 30)) return MOCAT_ERR_ARG;
+  const int64_t nrb = (n + 31) / 32;
+  const int threads = 256;
+  if (d <= 64) {
+    const int64_t total = nrb * (PackLayout<64>::BLOCK_BF16 + 32);
+    const unsigned blocks = (unsigned)std::min<int64_t>((total + threads - 1) / threads, 65535);
+    hipLaunchKernelGGL(lr_pack_kernel<64>, dim3(blocks), dim3(threads), 0, stream, X, y, (int)n, (int)d,
+                       (char*)packed);
+  } else {
+    const int64_t total = nrb * (PackLayout<128>::BLOCK_BF16 + 32);
+    const unsigned blocks = (unsigned)std::min<int64_t>((total + threads - 1) / threads, 65535);
+    hipLaunchKernelGGL(lr_pack_kernel<128>, dim3(blocks), dim3(threads), 0, stream, X, y, (int)n, (int)d,
+                       (char*)packed);
+  }
+  MOCAT_LAUNCH_CHECK();
+  return MOCAT_OK;
+}
+
+int mocat_lr_potential_grad(const float* X, const float* y, const void* packed, int64_t n, int64_t d, int precision,
+                            float prior_prec, float temperature, const float* theta, int64_t C, float* prior_pot,
+                            float* lik_pot, float* grad_prior, float* grad_lik, hipStream_t stream) {
+  int rc = check_lr(X, y, packed, n, d, precision);
+  if (rc) return rc;
+  if (C < 0 || (C > 0 && !theta)) return MOCAT_ERR_ARG;
+  if (C == 0) return MOCAT_OK;
+  LrArgs a{X, y, packed, (int)n, (int)d, prior_prec, temperature};
+  return precision == 0 ? dispatch_eval<Fp32Engine>(a, theta, C, prior_pot, lik_pot, grad_prior, grad_lik, stream)
+                        : dispatch_eval<Bf16Engine>(a, theta, C, prior_pot, lik_pot, grad_prior, grad_lik, stream);
+}
+
+int mocat_hmc_lr_iterate(const float* X, const float* y, const void* packed, int64_t n, int64_t d, int precision,
+                         float prior_prec, float temperature, int64_t C, float* theta, float* momenta, float* grad,
+                         float* potential, float* prior_pot, float* lik_pot, float* stepsize, float* alpha,
+                         uint32_t* keys, int64_t iter0, int n_iter, int leapfrog_steps, float friction,
+                         int correction, float target, float rm_scale, float rm_neg_exponent, double adapt_cut_off,
+                         float* trace_value, float* trace_potential, float* trace_alpha, float* trace_stepsize,
+                         float* trace_momenta, hipStream_t stream) {
+  int rc = check_lr(X, y, packed, n, d, precision);
+  if (rc) return rc;
+  if (C < 0 || n_iter < 0 || leapfrog_steps < 1 || correction < 0 || correction > 2) return MOCAT_ERR_ARG;
+  if (C == 0 || n_iter == 0) return MOCAT_OK;
+  if (!theta || !momenta || !grad || !potential || !prior_pot || !lik_pot || !stepsize || !alpha || !keys)
+    return MOCAT_ERR_ARG;
+  HmcArgs h{{X, y, packed, (int)n, (int)d, prior_prec, temperature},
+            C, theta, momenta, grad, potential, prior_pot, lik_pot, stepsize, alpha, keys, iter0, n_iter,
+            leapfrog_steps, friction, correction, target, rm_scale, rm_neg_exponent, adapt_cut_off,
+            trace_value, trace_potential, trace_alpha, trace_stepsize, trace_momenta};
+  return precision == 0 ? dispatch_hmc<Fp32Engine>(h, stream) : dispatch_hmc<Bf16Engine>(h, stream);
+}
+
+}  // extern "C"

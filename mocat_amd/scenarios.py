@@ -1,0 +1,109 @@
+"""Scenarios with fused GPU potential + gradient kernels.
+
+LogisticRegression: Bayesian logistic regression with a N(0, prior_variance I) prior. The reference
+has no such scenario: it is the user Scenario pattern of README.md:35-49 (prior + likelihood
+potentials, gradients by jax.grad via mocat/src/core.py:204-230), here with a hand-written adjoint
+(grad = X^T (sigmoid(X theta) - y) + theta / prior_variance) evaluated by libmocat_amd.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import torch
+
+from . import _lib
+from . import random as mr
+from .core import Scenario
+
+PRECISIONS = {"fp32": 0, "bf16x3": 1}
+
+_lib.register("mocat_lr_packed_bytes", [_lib.c_i64, _lib.c_i64, _lib.c_ptr])
+_lib.register("mocat_lr_pack", [_lib.c_ptr, _lib.c_ptr, _lib.c_i64, _lib.c_i64, _lib.c_ptr, _lib.c_ptr])
+_lib.register("mocat_lr_potential_grad",
+              [_lib.c_ptr, _lib.c_ptr, _lib.c_ptr, _lib.c_i64, _lib.c_i64, _lib.c_i32, _lib.c_f32, _lib.c_f32,
+               _lib.c_ptr, _lib.c_i64, _lib.c_ptr, _lib.c_ptr, _lib.c_ptr, _lib.c_ptr, _lib.c_ptr])
+
+
+def _dev_tensor(a, dtype=torch.float32):
+    if isinstance(a, torch.Tensor):
+        t = a.detach().to(dtype)
+    else:
+        t = torch.as_tensor(np.asarray(a), dtype=dtype)
+    if not t.is_cuda:
+        t = t.to(mr.default_device())
+    return t.contiguous()
+
+
+class LogisticRegression(Scenario):
+    """Bayesian logistic regression: y_i ~ Bernoulli(sigmoid(x_i . theta)), theta ~ N(0, v I).
+
+    precision: "bf16x3" (MFMA, hi/lo split, FP32 accumulation) or "fp32" (vector FMAs).
+    Supports d <= 128.
+    """
+    name = "Logistic Regression"
+
+    def __init__(self, X, y, prior_variance: float = 10.0, precision: str = "bf16x3", name: str = None):
+        super().__init__(name=name)
+        if precision not in PRECISIONS:
+            raise ValueError(f"precision must be one of {list(PRECISIONS)}")
+        self.X = _dev_tensor(X)
+        self.y = _dev_tensor(y).reshape(-1)
+        if self.X.dim() != 2 or self.X.shape[0] != self.y.shape[0]:
+            raise ValueError("X must be (n, d) and y (n,)")
+        self.n, self.dim = int(self.X.shape[0]), int(self.X.shape[1])
+        if self.dim > 128:
+            raise ValueError("LogisticRegression kernels support d <= 128")
+        self.prior_variance = float(prior_variance)
+        self.prior_precision = 1.0 / self.prior_variance
+        self.precision = precision
+        self.packed = None
+        if precision == "bf16x3":
+            nbytes = _lib.ctypes.c_int64(0)
+            _lib.call("mocat_lr_packed_bytes", self.n, self.dim, _lib.ctypes.byref(nbytes))
+            self.packed = torch.empty(int(nbytes.value), dtype=torch.uint8, device=self.X.device)
+            _lib.call("mocat_lr_pack", _lib.ptr(self.X), _lib.ptr(self.y), self.n, self.dim, _lib.ptr(self.packed),
+                      _lib.stream_handle())
+
+    # kernel arguments shared with the fused samplers
+    def kernel_args(self):
+        return (_lib.ptr(self.X), _lib.ptr(self.y), _lib.ptr(self.packed), self.n, self.dim,
+                PRECISIONS[self.precision], float(self.prior_precision))
+
+    def evaluate(self, x: torch.Tensor, temperature: float = 1.0):
+        """(prior_potential, likelihood_potential, grad_prior, grad_likelihood) for x (..., d)."""
+        x = _dev_tensor(x)
+        if x.shape[-1] != self.dim:
+            raise ValueError(f"expected trailing dimension {self.dim}, got {tuple(x.shape)}")
+        batch = x.shape[:-1]
+        C = int(math.prod(batch)) if len(batch) else 1
+        pp = torch.empty(batch, dtype=torch.float32, device=x.device)
+        lp = torch.empty_like(pp)
+        gp = torch.empty_like(x)
+        gl = torch.empty_like(x)
+        X, y, packed, n, d, prec, pprec = self.kernel_args()
+        _lib.call("mocat_lr_potential_grad", X, y, packed, n, d, prec, pprec, float(temperature), _lib.ptr(x), C,
+                  _lib.ptr(pp), _lib.ptr(lp), _lib.ptr(gp), _lib.ptr(gl), _lib.stream_handle())
+        return pp, lp, gp, gl
+
+    def prior_potential(self, x, random_key=None):
+        return self.evaluate(x)[0]
+
+    def likelihood_potential(self, x, random_key=None):
+        return self.evaluate(x)[1]
+
+    def prior_potential_and_grad(self, x, random_key=None):
+        pp, _, gp, _ = self.evaluate(x)
+        return pp, gp
+
+    def likelihood_potential_and_grad(self, x, random_key=None):
+        _, lp, _, gl = self.evaluate(x)
+        return lp, gl
+
+    def tempered_potential_and_grad(self, x, temperature, random_key=None):
+        pp, lp, gp, gl = self.evaluate(x, temperature)
+        return pp + temperature * lp, gp + temperature * gl
+
+    def prior_sample(self, random_key):
+        """sqrt(v) * normal(key, (d,)), batched over keys (..., 2)."""
+        return math.sqrt(self.prior_variance) * mr.normal(random_key, (self.dim,))

@@ -36,12 +36,13 @@ def _rotl(x, r):
 
 
 def threefry2x32(k0, k1, x0, x1):
-    """Vectorised threefry2x32-20 over counter arrays x0, x1 with one key (k0, k1)."""
-    k0 = U32(k0)
-    k1 = U32(k1)
-    x0 = np.array(x0, dtype=U32, copy=True)
-    x1 = np.array(x1, dtype=U32, copy=True)
-    ks = (k0, k1, U32(k0 ^ k1 ^ U32(0x1BD11BDA)))
+    """Vectorised threefry2x32-20; keys and counters broadcast against each other."""
+    k0 = np.asarray(k0, dtype=U32)
+    k1 = np.asarray(k1, dtype=U32)
+    shape = np.broadcast_shapes(k0.shape, k1.shape, np.shape(x0), np.shape(x1))
+    x0 = np.array(np.broadcast_to(np.asarray(x0, U32), shape), dtype=U32, copy=True)
+    x1 = np.array(np.broadcast_to(np.asarray(x1, U32), shape), dtype=U32, copy=True)
+    ks = (k0, k1, (k0 ^ k1 ^ U32(0x1BD11BDA)).astype(U32))
     with np.errstate(over="ignore"):
         x0 += ks[0]
         x1 += ks[1]
@@ -146,3 +147,28 @@ def categorical(key, logits, n):
     logits = np.asarray(logits, np.float64)
     g = gumbel(key, (n, logits.shape[-1]))
     return np.argmax(g + logits[None, :], axis=-1).astype(np.int32)
+
+
+def bits_batch(keys, size):
+    """bits(keys[i], (size,)) for a batch of keys (C, 2) -> (C, size), vectorised over keys."""
+    keys = np.asarray(keys, U32).reshape(-1, 2)
+    S = int(size)
+    h = (S + 1) // 2
+    i = np.arange(h, dtype=U32)
+    c1 = np.where(i + h < S, i + h, 0).astype(U32)
+    a, b = threefry2x32(keys[:, 0:1], keys[:, 1:2], i[None, :], c1[None, :])
+    out = np.concatenate([a, b], axis=1)[:, :S]
+    return out
+
+
+def split_batch(keys, num=2):
+    return bits_batch(keys, 2 * num).reshape(-1, num, 2)
+
+
+def uniform_batch(keys, size, minval=0.0, maxval=1.0):
+    return uniform_from_bits(bits_batch(keys, size), minval, maxval)
+
+
+def normal_batch(keys, size):
+    u = uniform_batch(keys, size, NORMAL_LO, 1.0)
+    return (F32(np.sqrt(2)) * erfinv_f32(u)).astype(F32)

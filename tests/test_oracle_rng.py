@@ -77,3 +77,12 @@ def test_categorical_matches_bruteforce_argmax():
     idx = rng.categorical(k, logits, 2000)
     freq = np.bincount(idx, minlength=4) / 2000
     np.testing.assert_allclose(freq, [0.1, 0.2, 0.3, 0.4], atol=0.04)
+
+
+def test_batched_helpers_match_scalar_path():
+    keys = rng.split(rng.PRNGKey(4), 17)
+    for size in (1, 2, 5, 50):
+        want = np.stack([rng.bits(k, size) for k in keys])
+        np.testing.assert_array_equal(rng.bits_batch(keys, size), want)
+    np.testing.assert_array_equal(rng.split_batch(keys, 3), np.stack([rng.split(k, 3) for k in keys]))
+    np.testing.assert_array_equal(rng.normal_batch(keys, 7), np.stack([rng.normal(k, (7,)) for k in keys]))
