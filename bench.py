@@ -1,0 +1,230 @@
+#!/usr/bin/env python3
+"""Benchmark: batched HMC on Bayesian logistic regression, one MI355X.
+
+Workload: X 2000 x 50 ~ N(0,1), theta* ~ N(0, I), y ~ Bernoulli(sigmoid(X theta*)), prior N(0, 10 I);
+8192 independent chains of Underdamped(leapfrog_steps=10, friction=inf) = HMC with Robbins-Monro
+stepsize adaptation to 0.651 acceptance; FP32 state, bf16x3 MFMA contractions.
+A step is one HMC iteration of all chains (one launch of the fused iteration kernel).
+
+value = ESS/s: per chain, mocat's ess_autocorrelation of the potential trace over the second half
+of the timed iterations, summed over chains, divided by the wall time of all timed iterations.
+
+Also reported: roofline of the fused kernel (algorithmic FLOPs 4*n*d*L per chain-iteration over its
+measured average launch time, against the 2.5 PFLOP/s dense bf16 MFMA peak), a CPU baseline (the
+oracle's NumPy port of the same chains on the host cores, bounded sample) and the unfused FP32
+torch path on the same GPU.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+N_ROWS, DIM, CHAINS, LEAPFROG = 2000, 50, 8192, 10
+PEAK_TFLOPS = {"bf16x3": 2500.0, "fp32": 157.3}
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hmc_lr_bf16x3_c8192_n2000_d50_traffic.json")
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--chains", type=int, default=CHAINS)
+    ap.add_argument("--precision", default="bf16x3", choices=["bf16x3", "fp32"])
+    ap.add_argument("--cpu-baseline", dest="cpu", action="store_true", default=True)
+    ap.add_argument("--no-cpu-baseline", dest="cpu", action="store_false")
+    ap.add_argument("--cpu-chains", type=int, default=64)
+    ap.add_argument("--cpu-iters", type=int, default=120)
+    ap.add_argument("--plain-steps", type=int, default=30, help="iterations timed on the unfused FP32 path (0: skip)")
+    return ap.parse_args()
+
+
+def clone(c):
+    import torch
+    from mocat_amd import cdict
+    out = cdict()
+    for k, v in c.__dict__.items():
+        if isinstance(v, torch.Tensor):
+            out.__dict__[k] = v.clone()
+        elif isinstance(v, cdict):
+            out.__dict__[k] = clone(v)
+        else:
+            out.__dict__[k] = v
+    return out
+
+
+def plain_fp32_ms_per_iter(X, y, state, extra, steps, prec=0.1):
+    """The same HMC iteration as unfused FP32 torch ops (GEMM, elementwise, GEMM per leapfrog step)."""
+    import torch
+    import mocat_amd.random as mr
+    Xt = torch.as_tensor(X, device="cuda")
+    yt = torch.as_tensor(y, device="cuda")
+    th = state.value.clone()
+    g = state.grad_potential.clone()
+    U = state.potential.clone()
+    h = state.stepsize.clone()
+    keys = extra.random_key.clone()
+    C, d = th.shape
+
+    def pot_grad(t):
+        z = t @ Xt.T
+        r = torch.sigmoid(z) - yt
+        lik = (torch.nn.functional.softplus(z) - yt * z).sum(1)
+        gr = prec * t + r @ Xt
+        return 0.5 * prec * (t * t).sum(1) + lik, gr
+
+    def one(it):
+        nonlocal th, g, U, h, keys
+        s = mr.split(keys, 2)
+        p = mr.normal(s[:, 1].contiguous(), (d,))
+        k = mr.split(s[:, 0].contiguous(), LEAPFROG + 1)[:, 0].contiguous()
+        x, gg, pp = th, g, p
+        K0 = 0.5 * (p * p).sum(1)
+        for _ in range(LEAPFROG):
+            ph = pp - (h[:, None] / 2) * gg
+            x = x + h[:, None] * ph
+            U1, gg = pot_grad(x)
+            pp = ph - (h[:, None] / 2) * gg
+        a = torch.clamp(torch.exp(-U1 + U - 0.5 * (pp * pp).sum(1) + K0), max=1.0).nan_to_num(0.0)
+        s = mr.split(k, 2)
+        keys = s[:, 0].contiguous()
+        u = mr.uniform(s[:, 1].contiguous(), (1,))[:, 0]
+        acc = u < a
+        th = torch.where(acc[:, None], x, th)
+        g = torch.where(acc[:, None], gg, g)
+        U = torch.where(acc, U1, U)
+        h = torch.exp(torch.log(h) + it ** (-2 / 3) * (a - 0.651))
+
+    for it in range(1, 4):
+        one(it)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for it in range(4, 4 + steps):
+        one(it)
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps * 1e3
+
+
+def cpu_baseline(X, y, chains, iters, ess_per_chain_iter):
+    """Oracle NumPy port of the same HMC chains on the host cores; bounded sample."""
+    import numpy as np
+    from oracle import mcmc as omcmc, rng as orng
+    try:
+        from threadpoolctl import threadpool_info
+        threads = max([i.get("num_threads", 1) for i in threadpool_info()] or [1])
+    except Exception:
+        threads = len(os.sched_getaffinity(0))
+    pg = omcmc.lr_pot_grad_fn(X, y, 0.1)
+    st, keys = omcmc.startup(orng.PRNGKey(0), chains, DIM, lambda k: np.sqrt(10.0) * orng.normal(k, (DIM,)), pg)
+    t0 = time.perf_counter()
+    for it in range(1, iters + 1):
+        st, keys, a, acc = omcmc.iterate(st, keys, it, pg, LEAPFROG, np.inf, "rm", 0.651)
+    dt = time.perf_counter() - t0
+    rate = chains * iters / dt  # chain-iterations / s
+    return {"value": rate * ess_per_chain_iter, "unit": "ESS/s", "cores": int(threads), "kind": "port",
+            "sample": f"{chains} chains x {iters} iterations of the NumPy oracle ({dt:.1f} s); ESS/s = "
+                      f"chain-iterations/s ({rate:.1f}) x ESS per chain-iteration of the GPU run (same algorithm)",
+            "chain_iterations_per_s": rate}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print("bench.py measures one GPU only: multi-GPU runs are not part of this build "
+              "(chains shard trivially across GPUs; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+    import mocat_amd as m
+    from mocat_amd import datasets
+    if not torch.cuda.is_available():
+        print("bench.py needs a ROCm GPU", file=sys.stderr)
+        sys.exit(1)
+
+    X, y, _ = datasets.logistic_regression(0, N_ROWS, DIM)
+    sc = m.LogisticRegression(X, y, prior_variance=10.0, precision=a.precision)
+    sampler = m.Underdamped(leapfrog_steps=LEAPFROG)
+    extra = m.cdict(random_key=m.random.PRNGKey(0))
+    state, extra = sampler.startup(sc, a.steps + 1, None, extra, n_chains=a.chains, correction=m.RMMetropolis())
+
+    if a.warmup > 0:
+        ws, we = clone(state), clone(extra)
+        for _ in range(a.warmup):
+            sampler.iterate(sc, ws, we, 1)
+        torch.cuda.synchronize()
+        del ws, we
+
+    K = a.steps
+    trace_pot = torch.empty((K, a.chains), dtype=torch.float32, device="cuda")
+    trace_alpha = torch.empty((K, a.chains), dtype=torch.float32, device="cuda")
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ev0.record()
+    for i in range(K):
+        sampler.iterate(sc, state, extra, 1, {"potential": trace_pot[i:i + 1], "alpha": trace_alpha[i:i + 1]})
+    ev1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kernel_ms = ev0.elapsed_time(ev1) / K
+
+    half = trace_pot[K // 2:]
+    ess = m.metrics.ess_autocorrelation(half)
+    ess_total = float(ess.sum().item())
+    value = ess_total / wall
+    acc = float(trace_alpha[K // 2:].mean().item())
+
+    flop_per_launch = 4.0 * N_ROWS * DIM * LEAPFROG * a.chains
+    achieved = flop_per_launch / (kernel_ms * 1e-3) / 1e12
+    peak = PEAK_TFLOPS[a.precision]
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE):
+        with open(TRAFFIC_FILE) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+
+    out = {
+        "metric": "ESS/s (batched HMC, Bayesian logistic regression)",
+        "value": value,
+        "unit": "ESS/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": a.warmup,
+        "ms_per_step": wall / K * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16x3+f32" if a.precision == "bf16x3" else "f32",
+        "data": "synthetic (X ~ N(0,1), seed 0)",
+        "config": {"workload": "batched HMC on Bayesian logistic regression", "n": N_ROWS, "d": DIM,
+                   "chains": a.chains, "leapfrog_steps": LEAPFROG, "friction": "inf",
+                   "correction": "RMMetropolis(target=0.651)", "precision": a.precision,
+                   "ess_window": f"iterations {K // 2 + 1}-{K}"},
+        "ess_total": ess_total,
+        "acceptance": acc,
+        "chain_iterations_per_s": a.chains * K / wall,
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                     "frac": achieved / peak, "traffic": traffic,
+                     "kernel": "hmc_lr_kernel<Bf16Engine<64,512>>" if a.precision == "bf16x3"
+                     else "hmc_lr_kernel<Fp32Engine<64,512>>",
+                     "kernel_ms": kernel_ms, "flop_per_launch": flop_per_launch,
+                     "issued_mfma_flop_per_launch": (3 * 4.0 * 64 * (math.ceil(N_ROWS / 32) * 32) * LEAPFROG
+                                                     * a.chains) if a.precision == "bf16x3" else None},
+    }
+    if a.plain_steps > 0:
+        ms_plain = plain_fp32_ms_per_iter(X, y, state, extra, a.plain_steps)
+        out["plain_fp32_gpu"] = {"ms_per_step": ms_plain, "speedup": ms_plain / (wall / K * 1e3),
+                                 "note": "unfused FP32 torch GEMM+elementwise HMC iteration, same GPU"}
+    if a.cpu:
+        ess_per_ci = ess_total / (a.chains * K)  # same ESS-window-over-total-time convention as value
+        out["cpu_baseline"] = cpu_baseline(X, y, a.cpu_chains, a.cpu_iters, ess_per_ci)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

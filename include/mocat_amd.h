@@ -40,6 +40,12 @@ int mocat_random_uniform(const uint32_t* keys, int64_t n_keys, int64_t size, flo
 /* normal(keys[i], (size,)) -> out[i][size] (f32). */
 int mocat_random_normal(const uint32_t* keys, int64_t n_keys, int64_t size, float* out, hipStream_t stream);
 
+/* --------------------------------------------------------------------------------- reductions
+ * logsumexp over rows of w[batch][n] (f32 in, FP64 accumulation and results): lse[b] =
+ * logsumexp(w[b]), lse2[b] = logsumexp(2 w[b]); either output may be NULL. log ESS = 2 lse - lse2
+ * (mocat/src/metrics.py:69-78; jax.scipy.special.logsumexp at mocat/src/transport/smc.py:160,212-215). */
+int mocat_logsumexp(const float* w, int64_t batch, int64_t n, double* lse, double* lse2, hipStream_t stream);
+
 /* ------------------------------------------------------------------------ logistic regression
  * Fused potential + adjoint of the Bayesian logistic-regression Scenario (user Scenario pattern,
  * README.md:35-49; gradients replace jax.grad in mocat/src/core.py:204-230):

@@ -75,21 +75,29 @@ def test_hmc_iteration_matches_oracle(precision):
             ("value", "momenta", "potential", "grad_potential", "prior_potential", "likelihood_potential",
              "stepsize")}
     keys0 = extra.random_key.cpu().numpy().copy()
+    prev = state.value.cpu().numpy().copy()
     for it in range(1, 4):
         s.update(sc, state, extra)
         o, okeys2, a, acc = omcmc.iterate(host, keys0, it, omcmc.lr_pot_grad_fn(X, y, 0.1), L, np.inf, "rm")
+        # key tree is integer work: bit-exact
         np.testing.assert_array_equal(extra.random_key.cpu().numpy(), okeys2)
         ga = state.alpha.cpu().numpy()
         gv = state.value.cpu().numpy()
         tol = 2e-4 if precision == "fp32" else 2e-3
-        # alpha agrees except where exp() of a large energy error saturates differently
         np.testing.assert_allclose(ga, a, rtol=tol, atol=tol)
-        # chains whose accept decision is not within rounding of the uniform draw agree exactly
-        same = np.all(np.isclose(gv, o["value"], rtol=1e-4, atol=1e-4), axis=1)
-        assert same.mean() > 0.95, same.mean()
+        # accept decisions agree wherever the uniform is not within rounding of alpha
+        s2 = orng.split_batch(orng.split_batch(keys0, 2)[:, 0], L + 1)[:, 0]
+        u = orng.uniform_batch(orng.split_batch(s2, 2)[:, 1], 1)[:, 0]
+        g_acc = ~np.all(gv == prev, axis=1)
+        clear = np.abs(u - a) > 5 * tol
+        np.testing.assert_array_equal(g_acc[clear], acc[clear])
+        # accepted proposals agree to the precision of the gradient engine
+        both = clear & acc
+        np.testing.assert_allclose(gv[both], o["value"][both], rtol=10 * tol, atol=10 * tol)
         # continue both from the GPU state so that rare near-threshold decisions do not compound
         host = {k: getattr(state, k).cpu().numpy().copy() for k in host}
         keys0 = extra.random_key.cpu().numpy().copy()
+        prev = gv.copy()
 
 
 def test_hmc_run_api_and_posterior():
