@@ -11,6 +11,6 @@ from .core import Scenario, cdict, static_cdict, save_cdict, load_cdict, is_impl
 from .sample import Sampler, run
 from .mcmc import MCMCSampler, Underdamped, Correction, Uncorrected, Metropolis, RMMetropolis
 from .scenarios import LogisticRegression
-from . import reduce, metrics
+from . import reduce, metrics, ssm
 
 __version__ = "0.1.0"

@@ -39,7 +39,7 @@ typedef struct {
 
 namespace {
 
-constexpr int NT = 1024;
+constexpr int NT = 512;
 constexpr int NW = NT / kWave;
 constexpr double kLog2PiD = 1.83787706640934548356;
 
@@ -52,30 +52,40 @@ struct Lgssm {
   static constexpr int NPARAM = D + 3 * D * D + DY * D + DY * DY + 1;
   float m0[D], L0[D * D], A[D * D], LQ[D * D], H[DY * D], SR[DY * DY], nlz;
   MOCAT_DEV void load(const float* p) {
+    #pragma unroll
     for (int i = 0; i < D; ++i) m0[i] = p[i];
     p += D;
+    #pragma unroll
     for (int i = 0; i < D * D; ++i) L0[i] = p[i];
     p += D * D;
+    #pragma unroll
     for (int i = 0; i < D * D; ++i) A[i] = p[i];
     p += D * D;
+    #pragma unroll
     for (int i = 0; i < D * D; ++i) LQ[i] = p[i];
     p += D * D;
+    #pragma unroll
     for (int i = 0; i < DY * D; ++i) H[i] = p[i];
     p += DY * D;
+    #pragma unroll
     for (int i = 0; i < DY * DY; ++i) SR[i] = p[i];
     p += DY * DY;
     nlz = p[0];
   }
   MOCAT_DEV void init(const float* eps, float* x) const {
+    #pragma unroll
     for (int i = 0; i < D; ++i) {
       float a = 0.f;
+      #pragma unroll
       for (int k = 0; k < D; ++k) a += L0[i * D + k] * eps[k];
       x[i] = a + m0[i];
     }
   }
   MOCAT_DEV void transition(const float* xp, const float* eps, float* x) const {
+    #pragma unroll
     for (int i = 0; i < D; ++i) {
       float a = 0.f, b = 0.f;
+      #pragma unroll
       for (int k = 0; k < D; ++k) {
         a += A[i * D + k] * xp[k];
         b += LQ[i * D + k] * eps[k];
@@ -85,14 +95,18 @@ struct Lgssm {
   }
   MOCAT_DEV float lik(const float* x, const float* y) const {
     float diff[DY];
+    #pragma unroll
     for (int j = 0; j < DY; ++j) {
       float hx = 0.f;
+      #pragma unroll
       for (int k = 0; k < D; ++k) hx += H[j * D + k] * x[k];
       diff[j] = y[j] - hx;
     }
     float s = 0.f;
+    #pragma unroll
     for (int k = 0; k < DY; ++k) {
       float v = 0.f;
+      #pragma unroll
       for (int j = 0; j < DY; ++j) v += diff[j] * SR[j * DY + k];
       s += v * v;
     }
@@ -207,7 +221,7 @@ MOCAT_DEV uint64_t block_exclusive_scan(uint64_t v, BlockScratch& s, uint64_t& t
   __syncthreads();
   s.u64[threadIdx.x] = v;
   __syncthreads();
-  // Hillis-Steele in LDS (NT = 1024: 10 rounds)
+  // Hillis-Steele in LDS (log2 NT rounds)
   for (int o = 1; o < NT; o <<= 1) {
     const uint64_t add = threadIdx.x >= o ? s.u64[threadIdx.x - o] : 0ull;
     __syncthreads();
@@ -347,13 +361,14 @@ __global__ __launch_bounds__(NT) void pf_kernel(PfArgs a) {
     const float* yf = a.y + F.y_offset;
     const Key key{F.key[0], F.key[1]};
     const Key rkey = split_at(key, 2, 0), sub = split_at(key, 2, 1);
-    const uint32_t hD = (D + 1) / 2;
+    constexpr uint32_t hD = (D + 1) / 2;
 
     // ---- t = 0: initiate_particles
     LseAcc acc{-INFINITY, 0.0, 0.0};
     for (int64_t i = tid; i < n; i += NT) {
       const Key ki = split_at(sub, (uint32_t)n, (uint32_t)i);
       float eps[D], x[D];
+      #pragma unroll
       for (uint32_t q = 0; q < hD; ++q) {
         uint32_t w0, w1;
         bits_pair(ki, D, q, w0, w1);
@@ -362,10 +377,12 @@ __global__ __launch_bounds__(NT) void pf_kernel(PfArgs a) {
       }
       model.init(eps, x);
       const float l = -model.lik(x, yf);
+      #pragma unroll
       for (int q = 0; q < D; ++q) xa[i * D + q] = x[q];
       lwa[i] = l;
       lse_add(acc, l);
       if (a.out_value && F.out_offset >= 0)
+        #pragma unroll
         for (int q = 0; q < D; ++q) a.out_value[(F.out_offset + i) * D + q] = x[q];
       if (a.out_lw && F.out_offset >= 0) a.out_lw[F.out_offset + i] = l;
       if (a.out_anc && F.out_offset >= 0) a.out_anc[F.out_offset + i] = (int32_t)i;
@@ -402,21 +419,25 @@ __global__ __launch_bounds__(NT) void pf_kernel(PfArgs a) {
         const int64_t src = res ? (int64_t)anc[i] : i;
         const Key ki = split_at(ik, (uint32_t)n, (uint32_t)i);
         float eps[D], xp[D], x[D];
+        #pragma unroll
         for (uint32_t q = 0; q < hD; ++q) {
           uint32_t w0, w1;
           bits_pair(ki, D, q, w0, w1);
           eps[q] = normal_from_bits(w0);
           if (q + hD < (uint32_t)D) eps[q + hD] = normal_from_bits(w1);
         }
+        #pragma unroll
         for (int q = 0; q < D; ++q) xp[q] = xa[src * D + q];
         model.transition(xp, eps, x);
         const float l = (res ? 0.f : lwa[i]) - model.lik(x, ycur);
+        #pragma unroll
         for (int q = 0; q < D; ++q) xb[i * D + q] = x[q];
         lwb[i] = l;
         lse_add(acc, l);
         if (F.out_offset >= 0) {
           const int64_t o = F.out_offset + t * n + i;
           if (a.out_value)
+            #pragma unroll
             for (int q = 0; q < D; ++q) a.out_value[o * D + q] = x[q];
           if (a.out_lw) a.out_lw[o] = l;
           if (a.out_anc) a.out_anc[o] = (int32_t)src;
@@ -441,6 +462,7 @@ __global__ __launch_bounds__(NT) void pf_kernel(PfArgs a) {
     // final particles live in xa/lwa; expose them in slot 0 buffers for the host
     if (xa != a.x0 + F.state_offset * D) {
       for (int64_t i = tid; i < n; i += NT) {
+        #pragma unroll
         for (int q = 0; q < D; ++q) (a.x0 + F.state_offset * D)[i * D + q] = xa[i * D + q];
         (a.lw0 + F.state_offset)[i] = lwa[i];
       }

@@ -1,0 +1,4 @@
+"""mocat.ssm surface (mocat/ssm.py): models, bootstrap filter, filtering drivers."""
+from .models import StateSpaceModel, TimeHomogenousLinearGaussian, StochasticVolatility
+from .filtering import (ParticleFilter, BootstrapFilter, run_particle_filter_for_marginals, run_particle_filters,
+                        resample_indices, categorical, run_kalman_filter_for_marginals)

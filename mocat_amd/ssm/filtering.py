@@ -1,0 +1,212 @@
+"""Particle filtering (mocat/src/ssm/filtering.py) on the persistent HIP filter kernel.
+
+run_particle_filter_for_marginals keeps mocat's signature and key tree
+(mocat/src/ssm/filtering.py:255-324) and output cdict(value (T, n, d), log_weight (T, n), y, t,
+ess (T,)), and adds: `resampling` ("categorical" = mocat's Gumbel-max, the default; "systematic";
+"multinomial"), `log_likelihood` (FP64 estimate of log p(y_0:T)) and `ancestors` (T, n).
+
+run_particle_filters runs many independent filters of uneven particle counts and lengths in one
+launch (one workgroup per filter, largest n*T first) and returns their FP64 log-likelihoods.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from .. import _lib
+from .. import random as mr
+from ..core import cdict
+
+RESAMPLING = {"systematic": 0, "multinomial": 1, "categorical": 2}
+
+_P = _lib.c_ptr
+_lib.register("mocat_pf_param_count", [_lib.c_i32, _lib.c_i32, _lib.c_i32, _P])
+_lib.register("mocat_pf_run", [_lib.c_i32, _lib.c_i32, _lib.c_i32, _P, _lib.c_i32, _P, _P, _P, _P, _P, _P, _P, _P,
+                               _lib.c_i32, _lib.c_f32, _P, _P, _P, _P, _P, _P, _P])
+_lib.register("mocat_resample", [_P, _lib.c_i64, _lib.c_i32, _P, _P, _P, _P])
+_lib.register("mocat_resample_categorical", [_P, _lib.c_i64, _lib.c_i64, _P, _P, _P])
+
+FILTER_DTYPE = np.dtype([("n", "<i8"), ("T", "<i8"), ("y_offset", "<i8"), ("state_offset", "<i8"),
+                         ("out_offset", "<i8"), ("t_offset", "<i8"), ("key", "<u4", (2,)),
+                         ("param_offset", "<i4"), ("reserved", "<i4")])
+assert FILTER_DTYPE.itemsize == 64
+
+
+class ParticleFilter:
+    name = "Particle Filter"
+
+    def __init__(self, name: str = None):
+        if name is not None:
+            self.name = name
+
+    def __repr__(self):
+        return f"mocat_amd.ParticleFilter.{self.__class__.__name__}"
+
+
+class BootstrapFilter(ParticleFilter):
+    """Proposal = transition, weight = likelihood (mocat/src/ssm/filtering.py:142-170)."""
+    name = "Bootstrap Filter"
+
+
+def resample_indices(weights: torch.Tensor, random_key, method: str = "systematic") -> torch.Tensor:
+    """Ancestor indices from non-negative weights: exact fixed-point cumulative sums, uniforms from
+    `random_key` (systematic: uniform(key); multinomial: uniform(key, (n,)))."""
+    w = weights.to(torch.float32).contiguous()
+    n = int(w.shape[0])
+    key = mr.as_key(random_key).reshape(2).contiguous()
+    ws = torch.empty(n, dtype=torch.int64, device=w.device)
+    idx = torch.empty(n, dtype=torch.int32, device=w.device)
+    _lib.call("mocat_resample", _lib.ptr(w), n, {"systematic": 0, "multinomial": 1}[method], _lib.ptr(key),
+              _lib.ptr(ws), _lib.ptr(idx), _lib.stream_handle())
+    return idx
+
+
+def categorical(random_key, log_weight: torch.Tensor, n_out: int = None) -> torch.Tensor:
+    """random.categorical(key, log_weight, shape=(n_out,)) by Gumbel-max, as mocat's _resample."""
+    lw = log_weight.to(torch.float32).contiguous()
+    n = int(lw.shape[0])
+    n_out = n if n_out is None else int(n_out)
+    key = mr.as_key(random_key).reshape(2).contiguous()
+    idx = torch.empty(n_out, dtype=torch.int32, device=lw.device)
+    _lib.call("mocat_resample_categorical", _lib.ptr(lw), n, n_out, _lib.ptr(key), _lib.ptr(idx),
+              _lib.stream_handle())
+    return idx
+
+
+def _key_np(k):
+    if isinstance(k, torch.Tensor):
+        return k.detach().cpu().numpy().astype(np.uint32).reshape(2)
+    return np.asarray(k, np.uint32).reshape(2)
+
+
+def _launch(models, ys, keys, ns, ess_threshold, resampling, store):
+    """Runs len(models) filters (same model class and dims) in one kernel launch."""
+    m0 = models[0]
+    d, dy, kind = m0.dim, m0.dim_obs, m0.kernel_model
+    for mdl in models:
+        if (mdl.kernel_model, mdl.dim, mdl.dim_obs) != (kind, d, dy):
+            raise ValueError("filters of one launch must share the model class and dimensions")
+    F = len(models)
+    Ts = [int(np.asarray(y).shape[0]) for y in ys]
+    order = sorted(range(F), key=lambda i: -(ns[i] * Ts[i]))  # longest first
+    desc = np.zeros(F, FILTER_DTYPE)
+    params, yflat = [], []
+    poff = yoff = soff = ooff = toff = 0
+    for slot, i in enumerate(order):
+        p = models[i].kernel_params()
+        yi = np.asarray(ys[i], np.float32).reshape(Ts[i], dy)
+        desc[slot] = (ns[i], Ts[i], yoff, soff, ooff if store else -1, toff, _key_np(keys[i]), poff, 0)
+        params.append(p)
+        yflat.append(yi.ravel())
+        poff += p.size
+        yoff += yi.size
+        soff += ns[i]
+        ooff += ns[i] * Ts[i]
+        toff += Ts[i]
+    dev = mr.default_device()
+    f32 = dict(dtype=torch.float32, device=dev)
+    desc_t = torch.from_numpy(desc.view(np.uint8).copy()).to(dev)
+    params_t = torch.from_numpy(np.concatenate(params)).to(dev)
+    y_t = torch.from_numpy(np.concatenate(yflat).astype(np.float32)).to(dev)
+    x0 = torch.empty(soff * d, **f32)
+    x1 = torch.empty(soff * d, **f32)
+    lw0 = torch.empty(soff, **f32)
+    lw1 = torch.empty(soff, **f32)
+    cs = torch.empty(soff, dtype=torch.int64, device=dev)
+    anc = torch.empty(soff, dtype=torch.int32, device=dev)
+    out_value = torch.empty(ooff * d, **f32) if store else None
+    out_lw = torch.empty(ooff, **f32) if store else None
+    out_anc = torch.empty(ooff, dtype=torch.int32, device=dev) if store else None
+    out_ess = torch.empty(toff, **f32)
+    out_llt = torch.empty(toff, dtype=torch.float64, device=dev)
+    out_ll = torch.empty(F, dtype=torch.float64, device=dev)
+    _lib.call("mocat_pf_run", kind, d, dy, _lib.ptr(desc_t), F, _lib.ptr(params_t), _lib.ptr(y_t), _lib.ptr(x0),
+              _lib.ptr(x1), _lib.ptr(lw0), _lib.ptr(lw1), _lib.ptr(cs), _lib.ptr(anc), RESAMPLING[resampling],
+              float(ess_threshold), _lib.ptr(out_value), _lib.ptr(out_lw), _lib.ptr(out_anc), _lib.ptr(out_ess),
+              _lib.ptr(out_llt), _lib.ptr(out_ll), _lib.stream_handle())
+    res = [None] * F
+    for slot, i in enumerate(order):
+        r = dict(desc=desc[slot])
+        n, T = ns[i], Ts[i]
+        s0, o0, t0 = int(desc[slot]["state_offset"]), int(desc[slot]["out_offset"]), int(desc[slot]["t_offset"])
+        r["log_likelihood"] = out_ll[slot]
+        r["ess"] = out_ess[t0:t0 + T]
+        r["log_likelihood_t"] = out_llt[t0:t0 + T]
+        r["final_value"] = x0[s0 * d:(s0 + n) * d].reshape(n, d)
+        r["final_log_weight"] = lw0[s0:s0 + n]
+        if store:
+            r["value"] = out_value[o0 * d:(o0 + n * T) * d].reshape(T, n, d)
+            r["log_weight"] = out_lw[o0:o0 + n * T].reshape(T, n)
+            r["ancestors"] = out_anc[o0:o0 + n * T].reshape(T, n)
+        res[i] = r
+    return res
+
+
+def run_particle_filter_for_marginals(ssm_scenario, particle_filter, y, t, random_key, n: int = None,
+                                      initial_sample: cdict = None, ess_threshold: float = 0.5,
+                                      resampling: str = "categorical") -> cdict:
+    if not isinstance(particle_filter, BootstrapFilter) and not (
+            isinstance(particle_filter, type) and issubclass(particle_filter, BootstrapFilter)):
+        raise NotImplementedError("the fused filter kernel implements the bootstrap filter")
+    if initial_sample is not None:
+        raise NotImplementedError("resuming from initial_sample is not supported by the fused filter kernel")
+    if resampling not in RESAMPLING:
+        raise ValueError(f"resampling must be one of {list(RESAMPLING)}")
+    if n is None:
+        raise ValueError("n (number of particles) is required")
+    yn = y.detach().cpu().numpy() if isinstance(y, torch.Tensor) else np.asarray(y)
+    if yn.ndim == 1:
+        yn = yn[:, None]
+    tn = t.detach().cpu().numpy() if isinstance(t, torch.Tensor) else np.asarray(t)
+    if getattr(ssm_scenario, "kernel_model", None) == 1 and len(tn) > 1 and not np.allclose(np.diff(tn), 1.0):
+        raise NotImplementedError("the stochastic-volatility kernel assumes unit time steps")
+    if resampling == "categorical" and n * n >= 0xFFFFFFFF:
+        raise ValueError("categorical resampling draws an (n, n) Gumbel matrix: n * n must be < 2^32 - 1")
+    r = _launch([ssm_scenario], [yn], [random_key], [int(n)], ess_threshold, resampling, True)[0]
+    dev = r["value"].device
+    out = cdict(value=r["value"], log_weight=r["log_weight"], t=torch.as_tensor(tn, device=dev),
+                y=torch.as_tensor(yn, dtype=torch.float32, device=dev), ess=r["ess"],
+                log_likelihood=r["log_likelihood"], ancestors=r["ancestors"])
+    return out
+
+
+def run_particle_filters(ssm_scenarios, ys, random_keys, ns, ess_threshold: float = 0.5,
+                         resampling: str = "systematic", store: bool = False):
+    """Independent bootstrap filters of uneven sizes in one launch. Returns a list of cdicts with
+    log_likelihood (FP64), ess (T,), final_value, final_log_weight (and value/log_weight/ancestors when
+    store=True)."""
+    ys = [y.detach().cpu().numpy() if isinstance(y, torch.Tensor) else np.asarray(y) for y in ys]
+    if resampling == "categorical" and any(int(v) * int(v) >= 0xFFFFFFFF for v in ns):
+        raise ValueError("categorical resampling draws an (n, n) Gumbel matrix: n * n must be < 2^32 - 1")
+    res = _launch(list(ssm_scenarios), ys, list(random_keys), [int(v) for v in ns], ess_threshold, resampling, store)
+    return [cdict(**{k: v for k, v in r.items() if k != "desc"}) for r in res]
+
+
+def run_kalman_filter_for_marginals(lgssm, y, t=None):
+    """Kalman filter means/covariances (mocat/src/ssm/linear_gaussian/kalman.py:16-57) plus the
+    log-likelihood, with the initial covariance (not its square root) at t = 0. FP64 torch on the GPU."""
+    dev = mr.default_device()
+    f = lambda a: torch.as_tensor(np.asarray(a), dtype=torch.float64, device=dev)
+    A, Q, H, R = f(lgssm.transition_matrix), f(lgssm.transition_covariance), f(lgssm.likelihood_matrix), \
+        f(lgssm.likelihood_covariance)
+    m, P = f(lgssm.initial_mean), f(lgssm.initial_covariance)
+    yy = torch.as_tensor(y, dtype=torch.float64, device=dev).reshape(len(y), -1)
+    ll = torch.zeros((), dtype=torch.float64, device=dev)
+    means, covs = [], []
+    for k in range(yy.shape[0]):
+        if k > 0:
+            m = A @ m
+            P = A @ P @ A.T + Q
+        S = H @ P @ H.T + R
+        r = yy[k] - H @ m
+        L = torch.linalg.cholesky(S)
+        sol = torch.cholesky_solve(r[:, None], L)[:, 0]
+        ll = ll - 0.5 * (r @ sol + 2 * torch.log(torch.diagonal(L)).sum() + r.shape[0] * np.log(2 * np.pi))
+        K = P @ H.T @ torch.cholesky_inverse(L)
+        m = m + K @ r
+        P = P - K @ H @ P
+        means.append(m)
+        covs.append(P)
+    return cdict(value=torch.stack(means), cov=torch.stack(covs), log_likelihood=ll)

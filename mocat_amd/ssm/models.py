@@ -1,0 +1,139 @@
+"""State-space models with fused particle-filter kernels (mocat/src/ssm/ssm.py,
+mocat/src/ssm/linear_gaussian/linear_gaussian.py, mocat/src/ssm/scenarios/stochastic_volatility.py).
+
+Parameters are factored on the host in FP64 exactly as mocat's reset_covariance does
+(mocat/src/utils.py:247-262: covariance_sqrt = cholesky(C), precision_sqrt = inv(cholesky(C)),
+precision_det = 1 / det(C)) and flattened into the kernel's parameter layout.
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from .. import random as mr
+
+LOG2PI = float(np.log(2 * np.pi))
+
+
+def _np(a):
+    if isinstance(a, torch.Tensor):
+        return a.detach().cpu().numpy().astype(np.float64)
+    return np.asarray(a, np.float64)
+
+
+class StateSpaceModel:
+    name = None
+    dim = None
+    dim_obs = None
+    kernel_model = None  # 0 linear Gaussian, 1 stochastic volatility
+
+    def __repr__(self):
+        return f"mocat_amd.StateSpaceModel.{self.__class__.__name__}"
+
+    def kernel_params(self) -> np.ndarray:
+        raise NotImplementedError
+
+    def simulate(self, t_all, random_key):
+        """StateSpaceModel.simulate (mocat/src/ssm/ssm.py:138-163): keys split(key, 2T); latent path
+        from the first T keys, observations from the last T. Returns cdict(x, y, t)."""
+        from ..core import cdict
+        t_all = _np(t_all)
+        T = len(t_all)
+        keys = mr.split(random_key, 2 * T)
+        eps_x = mr.normal(keys[:T].contiguous(), (self.dim,)).double().cpu().numpy()
+        eps_y = mr.normal(keys[T:].contiguous(), (self.dim_obs,)).double().cpu().numpy()
+        xs = [self._init_from_eps(eps_x[0])]
+        for t in range(1, T):
+            xs.append(self._transition_from_eps(xs[-1], eps_x[t], t_all[t] - t_all[t - 1]))
+        xs = np.stack(xs)
+        ys = np.stack([self._observe_from_eps(xs[t], eps_y[t]) for t in range(T)])
+        dev = mr.default_device()
+        return cdict(x=torch.tensor(xs, dtype=torch.float32, device=dev),
+                     y=torch.tensor(ys, dtype=torch.float32, device=dev),
+                     t=torch.tensor(t_all, dtype=torch.float32, device=dev), name=f"{self.name} simulation")
+
+
+class TimeHomogenousLinearGaussian(StateSpaceModel):
+    """x_t ~ N(A x_{t-1}, Q), y_t ~ N(H x_t, R), x_0 ~ N(m0, P0); 2-D covariance matrices."""
+    name = "Time-homogenous Linear Gaussian"
+    kernel_model = 0
+
+    def __init__(self, initial_mean=None, initial_covariance=None, transition_matrix=None,
+                 transition_covariance=None, likelihood_matrix=None, likelihood_covariance=None, name=None):
+        if name is not None:
+            self.name = name
+        dim = None
+        for v in (initial_mean, initial_covariance, transition_covariance, likelihood_matrix):
+            if v is not None:
+                dim = _np(v).shape[-1]
+                break
+        if dim is None:
+            raise AttributeError("Could not find dimension for TimeHomogenousLinearGaussian")
+        self.dim = int(dim)
+        self.initial_mean = np.zeros(dim) if initial_mean is None else _np(initial_mean)
+        self.initial_covariance = np.eye(dim) if initial_covariance is None else _np(initial_covariance)
+        self.transition_matrix = np.eye(dim) if transition_matrix is None else _np(transition_matrix)
+        self.transition_covariance = np.eye(dim) if transition_covariance is None else _np(transition_covariance)
+        self.likelihood_matrix = np.eye(dim) if likelihood_matrix is None else _np(likelihood_matrix)
+        self.dim_obs = int(self.likelihood_matrix.shape[0])
+        self.likelihood_covariance = np.eye(self.dim_obs) if likelihood_covariance is None else \
+            _np(likelihood_covariance)
+        for c in (self.initial_covariance, self.transition_covariance, self.likelihood_covariance):
+            if c.ndim != 2:
+                raise ValueError("covariances must be 2-D matrices")
+        self.initial_covariance_sqrt = np.linalg.cholesky(self.initial_covariance)
+        self.transition_covariance_sqrt = np.linalg.cholesky(self.transition_covariance)
+        self.likelihood_covariance_sqrt = np.linalg.cholesky(self.likelihood_covariance)
+        self.likelihood_precision_sqrt = np.linalg.inv(self.likelihood_covariance_sqrt)
+        self.likelihood_precision_det = 1.0 / np.linalg.det(self.likelihood_covariance)
+
+    def kernel_params(self):
+        nlz = (self.dim_obs * LOG2PI - np.log(self.likelihood_precision_det)) / 2
+        return np.concatenate([self.initial_mean.ravel(), self.initial_covariance_sqrt.ravel(),
+                               self.transition_matrix.ravel(), self.transition_covariance_sqrt.ravel(),
+                               self.likelihood_matrix.ravel(), self.likelihood_precision_sqrt.ravel(),
+                               [nlz]]).astype(np.float32)
+
+    def _init_from_eps(self, e):
+        return self.initial_covariance_sqrt @ e + self.initial_mean
+
+    def _transition_from_eps(self, x, e, dt):
+        return self.transition_matrix @ x + self.transition_covariance_sqrt @ e
+
+    def _observe_from_eps(self, x, e):
+        return self.likelihood_matrix @ x + self.likelihood_covariance_sqrt @ e
+
+
+class StochasticVolatility(StateSpaceModel):
+    """1-D stochastic volatility: x_t = phi x_{t-1} + sigma eps, y_t = beta exp(x_t / 2) eta.
+    Arguments as mocat's (mocat/src/ssm/scenarios/stochastic_volatility.py:24-57): transition_matrix
+    = phi, transition_covariance = sigma^2, likelihood_covariance_diag = beta^2; unit time steps."""
+    name = "Stochastic Volatility"
+    kernel_model = 1
+
+    def __init__(self, dim: int = 1, initial_mean=None, initial_covariance=None, transition_matrix=None,
+                 transition_covariance=None, likelihood_covariance_diag=None, name=None):
+        if dim != 1:
+            raise NotImplementedError("the fused kernel covers the 1-D stochastic volatility model")
+        if name is not None:
+            self.name = name
+        self.dim = self.dim_obs = 1
+        f = lambda v, default: default if v is None else float(np.asarray(_np(v)).ravel()[0])
+        self.initial_mean = f(initial_mean, 0.0)
+        self.initial_covariance = f(initial_covariance, 1.0)
+        self.phi = f(transition_matrix, 1.0)
+        self.sigma = float(np.sqrt(f(transition_covariance, 1.0)))
+        self.beta = float(np.sqrt(f(likelihood_covariance_diag, 1.0)))
+
+    def kernel_params(self):
+        return np.array([self.initial_mean, np.sqrt(self.initial_covariance), self.phi, self.sigma,
+                         1.0 / self.beta], np.float32)
+
+    def _init_from_eps(self, e):
+        return np.sqrt(self.initial_covariance) * e + self.initial_mean
+
+    def _transition_from_eps(self, x, e, dt):
+        return x * self.phi + e * self.sigma * np.sqrt(dt)
+
+    def _observe_from_eps(self, x, e):
+        return self.beta * np.exp(0.5 * x) * e

@@ -1,0 +1,204 @@
+"""Bootstrap particle filter and resampling, NumPy restatement (test infrastructure only).
+
+Follows mocat's run_particle_filter_for_marginals (mocat/src/ssm/filtering.py:255-324) key by key:
+  (key, sub) = split(key)                             :267
+  initiate_particles: split(sub, n), initial_sample, -likelihood_potential     :173-193
+  int_keys = split(key, T - 1)                        :278
+  resample iff ESS_{t-1} < threshold * n, lw reset to 0                        :286-292
+  split(int_key, n) -> transition_sample keys; lw += -likelihood_potential     :294-303
+Models:
+  TimeHomogenousLinearGaussian  mocat/src/ssm/linear_gaussian/linear_gaussian.py:46-51,86-94,118-128,
+                                covariances factored as reset_covariance (mocat/src/utils.py:247-262)
+  StochasticVolatility (1-D)    mocat/src/ssm/scenarios/stochastic_volatility.py:61-108
+The log-likelihood (absent in mocat) is log Z = sum_t [lse(lw_t) - lse(lw_res_{t-1})] (the SMC
+log-normaliser recursion of mocat/src/transport/smc.py:212-215), t = 0 term lse(lw_0) - log n.
+
+Resampling from given non-negative f32 weights (new in this build; mocat only has the Gumbel-max
+categorical): integer weights q_i = floor(w_i * 2^s) with s = 52 - bitlen(n) - exponent(max w)
+(exact scaling), C = cumsum(q) exactly, target T_j: systematic (j + u) * (W / n), multinomial
+u_j * W (FP64, IEEE), ancestor = first i with C_i > T_j. Pinned by brute-force counting on small
+inputs, offspring-count bounds (systematic: floor/ceil of n w_i / W) and the FP64 ESS/lse closed
+forms (tests/test_oracle_pf.py). The PF log-likelihood is pinned against the exact Kalman
+log-likelihood (oracle.kalman) statistically.
+"""
+import numpy as np
+
+from . import rng
+
+LOG2PI = np.log(2 * np.pi)
+
+
+def lgssm_params(initial_mean, initial_covariance, transition_matrix, transition_covariance,
+                 likelihood_matrix, likelihood_covariance):
+    """Kernel parameter layout [m0, L0, A, LQ, H, SR, nlz] with mocat's covariance factorisation."""
+    m0 = np.asarray(initial_mean, np.float64)
+    P0 = np.asarray(initial_covariance, np.float64)
+    A = np.asarray(transition_matrix, np.float64)
+    Q = np.asarray(transition_covariance, np.float64)
+    H = np.asarray(likelihood_matrix, np.float64)
+    R = np.asarray(likelihood_covariance, np.float64)
+    L0 = np.linalg.cholesky(P0)
+    LQ = np.linalg.cholesky(Q)
+    LR = np.linalg.cholesky(R)
+    SR = np.linalg.inv(LR)                 # likelihood_precision_sqrt
+    det_prec = 1.0 / np.linalg.det(R)      # likelihood_precision_det
+    dy = H.shape[0]
+    nlz = (dy * LOG2PI - np.log(det_prec)) / 2
+    return dict(m0=m0, L0=L0, A=A, LQ=LQ, H=H, SR=SR, nlz=nlz, d=m0.shape[0], dy=dy)
+
+
+def lgssm_flat(p):
+    return np.concatenate([p["m0"].ravel(), p["L0"].ravel(), p["A"].ravel(), p["LQ"].ravel(), p["H"].ravel(),
+                           p["SR"].ravel(), [p["nlz"]]]).astype(np.float32)
+
+
+def sv_params(phi, sigma, beta, m0=0.0, var0=1.0):
+    return dict(m0=np.array([m0]), L0=np.array([[np.sqrt(var0)]]), phi=float(phi), sigma=float(sigma),
+                inv_beta=1.0 / float(beta), d=1, dy=1, sv=True)
+
+
+def sv_flat(p):
+    return np.array([p["m0"][0], p["L0"][0, 0], p["phi"], p["sigma"], p["inv_beta"]], np.float32)
+
+
+def init_state(p, eps):
+    if p.get("sv"):
+        return eps * p["L0"][0, 0] + p["m0"][0]
+    return eps @ p["L0"].T + p["m0"]
+
+
+def transition(p, x, eps):
+    if p.get("sv"):
+        return x * p["phi"] + eps * p["sigma"]
+    return x @ p["A"].T + eps @ p["LQ"].T
+
+
+def lik_potential(p, x, y):
+    if p.get("sv"):
+        s = p["inv_beta"] / np.exp(0.5 * x[:, 0])
+        return 0.5 * (y[0] * s) ** 2 + 0.5 * LOG2PI - np.log(s)
+    diff = y[None, :] - x @ p["H"].T
+    v = diff @ p["SR"]
+    return 0.5 * np.sum(v * v, axis=1) + p["nlz"]
+
+
+def lse(w):
+    w = np.asarray(w, np.float64)
+    m = np.max(w)
+    if not np.isfinite(m):
+        return m
+    return m + np.log(np.sum(np.exp(w - m)))
+
+
+def log_ess(w):
+    w = np.asarray(w, np.float64)
+    return 2 * lse(w) - lse(2 * w)
+
+
+# ----------------------------------------------------------------------------- resampling
+def quantise(w, n=None):
+    w = np.asarray(w, np.float32)
+    n = w.shape[0] if n is None else n
+    wmax = np.float32(np.max(np.where(w > 0, w, 0)))
+    _, e = np.frexp(wmax)
+    shift = (52 - int(n).bit_length()) - int(e)
+    wd = np.where(w > 0, w, 0).astype(np.float64)
+    return np.floor(np.ldexp(wd, shift)).astype(np.uint64)
+
+
+def _search(C, T):
+    idx = np.searchsorted(C.astype(np.float64), T, side="right")
+    return np.minimum(idx, C.shape[0] - 1).astype(np.int32)
+
+
+def systematic(w, u):
+    q = quantise(w)
+    C = np.cumsum(q, dtype=np.uint64)
+    n = q.shape[0]
+    W = float(C[-1])
+    step = W / n
+    T = (np.arange(n, dtype=np.float64) + np.float64(u)) * step
+    return _search(C, T)
+
+
+def multinomial(w, us):
+    q = quantise(w)
+    C = np.cumsum(q, dtype=np.uint64)
+    W = float(C[-1])
+    T = np.asarray(us, np.float32).astype(np.float64) * W
+    return _search(C, T)
+
+
+def resample_indices(w, key, method):
+    """Ancestors for weights w with the uniforms the kernels draw from `key`."""
+    n = np.asarray(w).shape[0]
+    if method == "systematic":
+        return systematic(w, rng.uniform(key))
+    if method == "multinomial":
+        return multinomial(w, rng.uniform(key, (n,)))
+    raise ValueError(method)
+
+
+# ----------------------------------------------------------------------------- filter
+def run_pf(p, y, key, n, ess_threshold=0.5, resampling="systematic", dtype=np.float64):
+    """Returns dict(value (T, n, d), log_weight (T, n), ess (T,), log_likelihood, ancestors (T, n))."""
+    y = np.asarray(y, np.float64).reshape(len(y), -1)
+    T = y.shape[0]
+    d = p["d"]
+    key = np.asarray(key, np.uint32)
+    rkey, sub = rng.split(key, 2)
+    eps = rng.normal_batch(rng.split(sub, n), d).astype(dtype)
+    x = init_state(p, eps).reshape(n, d)
+    lw = -lik_potential(p, x, y[0])
+    values, lws, esss, ancs = [x], [lw], [], [np.arange(n, dtype=np.int32)]
+    l0, l2 = lse(lw), lse(2 * lw)
+    esss.append(np.exp(2 * l0 - l2))
+    loglik = l0 - np.log(n)
+    int_keys = rng.split(rkey, T - 1) if T > 1 else np.zeros((0, 2), np.uint32)
+    for t in range(1, T):
+        ik = int_keys[t - 1]
+        res = esss[-1] < ess_threshold * n
+        if res:
+            w = np.exp(np.asarray(lw, np.float64) - l0).astype(np.float32)
+            a = resample_indices(w, ik, resampling)
+            lw_res = np.zeros(n)
+            lres = np.log(n)
+        else:
+            a = np.arange(n, dtype=np.int32)
+            lw_res = lw
+            lres = l0
+        eps = rng.normal_batch(rng.split(ik, n), d).astype(dtype)
+        x = transition(p, x[a], eps).reshape(n, d)
+        lw = lw_res - lik_potential(p, x, y[t])
+        l0, l2 = lse(lw), lse(2 * lw)
+        loglik += l0 - lres
+        esss.append(np.exp(2 * l0 - l2))
+        values.append(x)
+        lws.append(lw)
+        ancs.append(a)
+    return dict(value=np.stack(values), log_weight=np.stack(lws), ess=np.array(esss), log_likelihood=loglik,
+                ancestors=np.stack(ancs))
+
+
+def simulate(p, T, key):
+    """StateSpaceModel.simulate (mocat/src/ssm/ssm.py:138-163): keys split(key, 2T), latent from
+    the first T keys (initial_sample then transition_sample), observations likelihood_sample with the
+    last T keys."""
+    keys = rng.split(np.asarray(key, np.uint32), 2 * T)
+    d, dy = p["d"], p["dy"]
+    xs = []
+    x = init_state(p, rng.normal(keys[0], (d,)).astype(np.float64)[None, :]).reshape(1, d)
+    xs.append(x[0])
+    for t in range(1, T):
+        x = transition(p, x, rng.normal(keys[t], (d,)).astype(np.float64)[None, :]).reshape(1, d)
+        xs.append(x[0])
+    xs = np.stack(xs)
+    ys = []
+    for t in range(T):
+        e = rng.normal(keys[T + t], (dy,)).astype(np.float64)
+        if p.get("sv"):
+            ys.append((1.0 / p["inv_beta"]) * np.exp(0.5 * xs[t]) * e)
+        else:
+            LR = np.linalg.inv(p["SR"])
+            ys.append(p["H"] @ xs[t] + LR @ e)
+    return xs, np.stack(ys)

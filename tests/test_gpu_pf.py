@@ -1,0 +1,108 @@
+"""GPU particle filters and resampling vs the oracle (same keys => same random path)."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import kalman as okal
+from oracle import pf as opf
+from oracle import rng as orng
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def m():
+    import mocat_amd
+    return mocat_amd
+
+
+@pytest.mark.parametrize("method", ["systematic", "multinomial"])
+@pytest.mark.parametrize("n", [1, 2, 7, 1000, 4096, 65539])
+def test_resampling_bit_exact(m, method, n):
+    rs = np.random.default_rng(n)
+    w = (rs.uniform(0, 1, n) ** 3 * (rs.uniform(size=n) > 0.2)).astype(np.float32)
+    w[rs.integers(0, n)] = 1.0
+    key = orng.PRNGKey(n + 1)
+    got = m.ssm.resample_indices(torch.tensor(w, device="cuda"), m.random.PRNGKey(n + 1), method).cpu().numpy()
+    want = opf.resample_indices(w, key, method)
+    np.testing.assert_array_equal(got, want)
+
+
+def test_resampling_edge_cases(m):
+    # all weight on one particle; tiny and zero weights; all-zero weights select the last index
+    for w in (np.array([0, 0, 5, 0], np.float32), np.array([1e-30, 1.0, 1e-38, 0], np.float32),
+              np.zeros(5, np.float32)):
+        for method in ("systematic", "multinomial"):
+            got = m.ssm.resample_indices(torch.tensor(w, device="cuda"), m.random.PRNGKey(9), method).cpu().numpy()
+            np.testing.assert_array_equal(got, opf.resample_indices(w, orng.PRNGKey(9), method))
+
+
+def test_categorical_matches_gumbel_max(m):
+    lw = np.log(np.random.default_rng(0).uniform(0.01, 1, 64)).astype(np.float32)
+    got = m.ssm.categorical(m.random.PRNGKey(3), torch.tensor(lw, device="cuda")).cpu().numpy()
+    want = orng.categorical(orng.PRNGKey(3), lw, 64)
+    assert (got == want).mean() >= 0.95  # argmax ties within f32 log rounding may differ
+
+
+def _lgssm_np(model):
+    return opf.lgssm_params(model.initial_mean, model.initial_covariance, model.transition_matrix,
+                            model.transition_covariance, model.likelihood_matrix, model.likelihood_covariance)
+
+
+@pytest.mark.parametrize("resampling,thr", [("systematic", 1.0), ("multinomial", 0.5), ("systematic", 0.0)])
+def test_lgssm_filter_same_path_as_fp64_oracle(m, resampling, thr):
+    from mocat_amd import datasets
+    model, y, t = datasets.lgssm_config(0, 4, 30)
+    out = m.ssm.run_particle_filter_for_marginals(model, m.ssm.BootstrapFilter(), y, t, m.random.PRNGKey(1),
+                                                  n=1024, ess_threshold=thr, resampling=resampling)
+    o = opf.run_pf(_lgssm_np(model), y.cpu().numpy(), orng.PRNGKey(1), 1024, thr, resampling)
+    assert out.value.shape == (30, 1024, 4) and out.log_weight.shape == (30, 1024) and out.ess.shape == (30,)
+    np.testing.assert_allclose(out.value[0].cpu().numpy(), o["value"][0], rtol=1e-5, atol=1e-5)
+    ll = out.log_likelihood.item()
+    assert abs(ll - o["log_likelihood"]) <= 1e-4 * abs(o["log_likelihood"]), (ll, o["log_likelihood"])
+    same_anc = (out.ancestors.cpu().numpy() == o["ancestors"]).mean()
+    assert same_anc > 0.99
+    np.testing.assert_allclose(out.ess.cpu().numpy(), o["ess"], rtol=2e-3)
+
+
+def test_config1_loglik_vs_kalman(m):
+    from mocat_amd import datasets
+    model, y, t = datasets.lgssm_config(0, 4, 100)
+    R = 32
+    keys = m.random.split(m.random.PRNGKey(7), R)
+    res = m.ssm.run_particle_filters([model] * R, [y.cpu().numpy()] * R, [k for k in keys], [4096] * R,
+                                     ess_threshold=1.0, resampling="systematic")
+    lls = np.array([r.log_likelihood.item() for r in res])
+    p = _lgssm_np(model)
+    kll, _, _ = okal.loglik(p["m0"], np.eye(4), p["A"], p["LQ"] @ p["LQ"].T, p["H"],
+                            np.linalg.inv(p["SR"] @ p["SR"].T), y.cpu().numpy().astype(np.float64))
+    gk = m.ssm.run_kalman_filter_for_marginals(model, y).log_likelihood.item()
+    assert abs(gk - kll) < 1e-8 * abs(kll)
+    mean, se = lls.mean(), lls.std(ddof=1) / np.sqrt(R)
+    # bootstrap-filter log-likelihoods are biased low by ~Var/2; allow 4 SE plus that bias
+    assert abs(mean - kll) < 4 * se + 0.5 * lls.var(), (mean, se, kll)
+
+
+def test_sv_batched_uneven_same_path(m):
+    from mocat_amd import datasets
+    models, ys, ns = datasets.sv_batch_config(3, n_models=6, n_min=64, n_max=3000, t_min=5, t_max=60)
+    keys = m.random.split(m.random.PRNGKey(11), 6)
+    res = m.ssm.run_particle_filters(models, ys, [k for k in keys], ns, ess_threshold=1.0,
+                                     resampling="systematic", store=True)
+    for i in range(6):
+        p = opf.sv_params(models[i].phi, models[i].sigma, models[i].beta)
+        o = opf.run_pf(p, ys[i], keys[i].cpu().numpy(), ns[i], 1.0, "systematic")
+        ll = res[i].log_likelihood.item()
+        assert abs(ll - o["log_likelihood"]) <= 1e-4 * abs(o["log_likelihood"]) + 1e-4, (i, ll, o["log_likelihood"])
+        assert res[i].value.shape == (len(ys[i]), ns[i], 1)
+
+
+def test_categorical_resampling_filter_runs(m):
+    from mocat_amd import datasets
+    model, y, t = datasets.lgssm_config(0, 4, 10)
+    out = m.ssm.run_particle_filter_for_marginals(model, m.ssm.BootstrapFilter(), y, t, m.random.PRNGKey(2),
+                                                  n=256, ess_threshold=0.5)
+    assert torch.isfinite(out.log_likelihood)
+    o = opf.run_pf(_lgssm_np(model), y.cpu().numpy(), orng.PRNGKey(2), 256, 0.5, "systematic")
+    # different resampler, same model: log-likelihoods agree to Monte Carlo error
+    assert abs(out.log_likelihood.item() - o["log_likelihood"]) < 5.0

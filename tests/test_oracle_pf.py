@@ -1,0 +1,89 @@
+"""Pins the particle-filter, resampling and Kalman oracles."""
+import numpy as np
+import pytest
+
+from oracle import kalman, pf, rng
+
+
+def _lgssm(seed=0, d=4):
+    rs = np.random.default_rng(seed)
+    A = 0.9 * np.eye(d) + np.where(np.eye(d) > 0, 0, rs.normal(0, 0.1, (d, d)))
+    H = rs.normal(0, 0.5, (d, d))
+    return pf.lgssm_params(np.zeros(d), np.eye(d), A, 0.1 * np.eye(d), H, 0.5 * np.eye(d))
+
+
+def test_kalman_matches_bruteforce_joint_gaussian():
+    p = _lgssm(1, 3)
+    _, y = pf.simulate(p, 6, rng.PRNGKey(2))
+    P0 = np.diag([1.0, 2.0, 0.5])  # non-identity P0: exercises the corrected t = 0 covariance
+    Q, R = p["LQ"] @ p["LQ"].T, np.linalg.inv(p["SR"] @ p["SR"].T)
+    ll, _, _ = kalman.loglik(p["m0"], P0, p["A"], Q, p["H"], R, y)
+    bf = kalman.joint_loglik_bruteforce(p["m0"], P0, p["A"], Q, p["H"], R, y)
+    assert abs(ll - bf) < 1e-9 * abs(bf)
+
+
+def test_lgssm_likelihood_is_gaussian_logpdf():
+    from scipy.stats import multivariate_normal
+    p = _lgssm(3)
+    x = np.random.default_rng(0).standard_normal((5, 4))
+    y = np.random.default_rng(1).standard_normal(4)
+    R = np.linalg.inv(p["SR"] @ p["SR"].T)
+    want = -multivariate_normal(mean=np.zeros(4), cov=R).logpdf(y[None, :] - x @ p["H"].T)
+    np.testing.assert_allclose(pf.lik_potential(p, x, y), want, rtol=1e-10)
+
+
+def test_sv_likelihood_is_gaussian_logpdf():
+    from scipy.stats import norm
+    p = pf.sv_params(0.95, 0.2, 0.7)
+    x = np.array([[-1.0], [0.3], [2.0]])
+    want = -norm(0, 0.7 * np.exp(0.5 * x[:, 0])).logpdf(0.4)
+    np.testing.assert_allclose(pf.lik_potential(p, x, np.array([0.4])), want, rtol=1e-12)
+
+
+@pytest.mark.parametrize("n", [1, 2, 7, 100, 1000])
+def test_systematic_offspring_bounds_and_bruteforce(n):
+    rs = np.random.default_rng(n)
+    w = rs.uniform(0, 1, n).astype(np.float32) * (rs.uniform(size=n) > 0.3)
+    if w.sum() == 0:
+        w[0] = 1
+    for u in (0.0, 0.37, 0.999):
+        a = pf.systematic(w, u)
+        assert np.all(np.diff(a) >= 0)
+        q = pf.quantise(w).astype(np.float64)
+        # brute force: count j with threshold in [C_{i-1}, C_i)
+        C = np.cumsum(q)
+        T = (np.arange(n) + u) * (C[-1] / n)
+        bf = np.array([np.argmax(C > t) if np.any(C > t) else n - 1 for t in T])
+        np.testing.assert_array_equal(a, bf)
+        counts = np.bincount(a, minlength=n)
+        expect = n * q / q.sum()
+        assert np.all(counts >= np.floor(expect) - 1) and np.all(counts <= np.ceil(expect) + 1)
+        assert np.all(counts[q == 0] == 0)
+
+
+def test_multinomial_frequencies():
+    w = np.array([0.1, 0.0, 0.5, 0.4], np.float32)
+    a = pf.resample_indices(w, rng.PRNGKey(3), "multinomial")
+    assert a.shape == (4,)
+    a = pf.multinomial(w, rng.uniform(rng.PRNGKey(4), (20000,)))
+    freq = np.bincount(a, minlength=4) / 20000
+    np.testing.assert_allclose(freq, w / w.sum(), atol=0.015)
+
+
+def test_quantise_is_exact_scaling():
+    w = np.array([1.0, 0.5, 0.25, 3e-8, 0.0], np.float32)
+    q = pf.quantise(w)
+    # max weight maps to 2^(52 - bitlen(n) - 1): scaling by a power of two keeps ratios exact
+    assert q[0] == 2 * q[1] == 4 * q[2]
+    assert q[4] == 0
+
+
+def test_pf_loglik_close_to_kalman_statistically():
+    p = _lgssm(0, 2)
+    _, y = pf.simulate(p, 30, rng.PRNGKey(0))
+    Q, R = p["LQ"] @ p["LQ"].T, np.linalg.inv(p["SR"] @ p["SR"].T)
+    ll, _, _ = kalman.loglik(p["m0"], p["L0"] @ p["L0"].T, p["A"], Q, p["H"], R, y)
+    lls = [pf.run_pf(p, y, rng.PRNGKey(s), 512, 1.0, "systematic")["log_likelihood"] for s in range(12)]
+    m, se = np.mean(lls), np.std(lls) / np.sqrt(len(lls))
+    assert abs(m - ll) < 4 * se + 0.05, (m, se, ll)
+
