@@ -10,7 +10,9 @@ from . import random
 from .core import Scenario, cdict, static_cdict, save_cdict, load_cdict, is_implemented, impl_checkable
 from .sample import Sampler, run
 from .mcmc import MCMCSampler, Underdamped, Correction, Uncorrected, Metropolis, RMMetropolis
-from .scenarios import LogisticRegression
+from .scenarios import LogisticRegression, DiagonalGaussian
+from .transport import (TransportSampler, SMCSampler, TemperedSMCSampler, MetropolisedSMCSampler,
+                        RMMetropolisedSMCSampler)
 from . import reduce, metrics, ssm
 
 __version__ = "0.1.0"

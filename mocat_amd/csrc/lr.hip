@@ -328,6 +328,39 @@ __global__ void lr_pack_kernel(const float* __restrict__ X, const float* __restr
 }
 
 // ------------------------------------------------------------------------------------------
+// Diagonal Gaussian likelihood engine (mocat/src/scenarios/toy_examples.py:19-51 with a diagonal
+// covariance): lik = 0.5 * sum_k lam_k (theta_k - mu_k)^2, grad = lam * (theta - mu).
+// Data: X -> mu[d], y -> lam[d] (precisions); n unused.
+template <int DP, int NT>
+struct GaussEngine {
+  struct alignas(16) Smem {
+    float part[CT * DP];
+  };
+  static MOCAT_DEV void eval(Smem& s, const float* __restrict__ mu, const float* __restrict__ lam, const void*, int,
+                             int d, const float* theta, float* lik, float* glik) {
+    const int t = threadIdx.x;
+    for (int e = t; e < CT * DP; e += NT) {
+      const int k = e % DP;
+      float g = 0.f, c = 0.f;
+      if (k < d) {
+        const float df = theta[e] - mu[k];
+        g = lam[k] * df;
+        c = 0.5f * lam[k] * df * df;
+      }
+      glik[e] = g;
+      s.part[e] = c;
+    }
+    __syncthreads();
+    if (t < CT) {
+      float acc = 0.f;
+      for (int k = 0; k < d; ++k) acc += s.part[t * DP + k];
+      lik[t] = acc;
+    }
+    __syncthreads();
+  }
+};
+
+// ------------------------------------------------------------------------------------------
 // Tile state shared by the evaluation and iteration kernels.
 template <int DP>
 struct alignas(16) TileState {
@@ -366,7 +399,9 @@ MOCAT_DEV void tile_eval(typename Engine::Smem& es, TileState<DP>& ts, const LrA
   if (t < CT) {
     float sq = 0.f;
     for (int k = 0; k < a.d; ++k) sq += ts.theta[t * DP + k] * ts.theta[t * DP + k];
-    const float pr = 0.5f * a.prior_prec * sq + 0.5f * (float)a.d * (kLog2Pi - logf(a.prior_prec));
+    // prec == 0: improper flat prior, potential 0 (mocat's default uniform prior, core.py:183-189)
+    const float pr =
+        a.prior_prec > 0.f ? 0.5f * a.prior_prec * sq + 0.5f * (float)a.d * (kLog2Pi - logf(a.prior_prec)) : 0.f;
     ts.prior[t] = pr;
     ts.likv[t] = ts.lik[t];
     ts.pot[t] = pr + a.temperature * ts.lik[t];
@@ -626,7 +661,7 @@ int dispatch_hmc(const HmcArgs& h, hipStream_t st) {
 
 int check_lr(const float* X, const float* y, const void* packed, int64_t n, int64_t d, int precision) {
   if (n <= 0 || d <= 0 || d > 128 || n > (1ll << 30)) return MOCAT_ERR_ARG;
-  if (precision == 0) return (X && y) ? MOCAT_OK : MOCAT_ERR_ARG;
+  if (precision == 0 || precision == 2) return (X && y) ? MOCAT_OK : MOCAT_ERR_ARG;
   if (precision == 1) return packed ? MOCAT_OK : MOCAT_ERR_ARG;
   return MOCAT_ERR_ARG;
 }
@@ -669,6 +704,8 @@ int mocat_lr_potential_grad(const float* X, const float* y, const void* packed, 
   if (C < 0 || (C > 0 && !theta)) return MOCAT_ERR_ARG;
   if (C == 0) return MOCAT_OK;
   LrArgs a{X, y, packed, (int)n, (int)d, prior_prec, temperature};
+  if (precision == 2)
+    return dispatch_eval<GaussEngine>(a, theta, C, prior_pot, lik_pot, grad_prior, grad_lik, stream);
   return precision == 0 ? dispatch_eval<Fp32Engine>(a, theta, C, prior_pot, lik_pot, grad_prior, grad_lik, stream)
                         : dispatch_eval<Bf16Engine>(a, theta, C, prior_pot, lik_pot, grad_prior, grad_lik, stream);
 }
@@ -690,6 +727,7 @@ int mocat_hmc_lr_iterate(const float* X, const float* y, const void* packed, int
             C, theta, momenta, grad, potential, prior_pot, lik_pot, stepsize, alpha, keys, iter0, n_iter,
             leapfrog_steps, friction, correction, target, rm_scale, rm_neg_exponent, adapt_cut_off,
             trace_value, trace_potential, trace_alpha, trace_stepsize, trace_momenta};
+  if (precision == 2) return dispatch_hmc<GaussEngine>(h, stream);
   return precision == 0 ? dispatch_hmc<Fp32Engine>(h, stream) : dispatch_hmc<Bf16Engine>(h, stream);
 }
 

@@ -473,12 +473,20 @@ __global__ __launch_bounds__(NT) void pf_kernel(PfArgs a) {
 }
 
 // ---------------------------------------------------------------- standalone resampling
+// from_log: w holds log-weights, the weights used are exp(w - max w) (f32).
 __global__ __launch_bounds__(NT) void resample_kernel(const float* __restrict__ w, int64_t n, const uint32_t* key,
-                                                      int method, uint64_t* csum, int32_t* idx) {
+                                                      int method, int from_log, uint64_t* csum, int32_t* idx) {
   __shared__ BlockScratch s;
   const Key k{key[0], key[1]};
+  float m = 0.f;
+  if (from_log) {
+    float mm = -INFINITY;
+    for (int64_t i = threadIdx.x; i < n; i += NT) mm = fmaxf(mm, w[i]);
+    m = block_max(mm, s.fm);
+    __syncthreads();
+  }
   auto wfn = [&](int64_t i) {
-    const float v = w[i];
+    const float v = from_log ? __expf(w[i] - m) : w[i];
     return v > 0.f ? v : 0.f;
   };
   resample_block(wfn, n, k, method, csum, idx, s);
@@ -538,12 +546,12 @@ int mocat_pf_run(int model, int d, int dy, const mocat_pf_filter* filters, int n
   return MOCAT_ERR_ARG;
 }
 
-int mocat_resample(const float* weights, int64_t n, int method, const uint32_t* key, void* workspace, int32_t* idx,
-                   hipStream_t stream) {
+int mocat_resample(const float* weights, int64_t n, int method, int from_log, const uint32_t* key, void* workspace,
+                   int32_t* idx, hipStream_t stream) {
   if (!weights || !key || !workspace || !idx || n <= 0 || n > 0x7FFFFFFFll || method < 0 || method > 1)
     return MOCAT_ERR_ARG;
-  hipLaunchKernelGGL(resample_kernel, dim3(1), dim3(NT), 0, stream, weights, n, key, method, (uint64_t*)workspace,
-                     idx);
+  hipLaunchKernelGGL(resample_kernel, dim3(1), dim3(NT), 0, stream, weights, n, key, method, from_log,
+                     (uint64_t*)workspace, idx);
   MOCAT_LAUNCH_CHECK();
   return MOCAT_OK;
 }

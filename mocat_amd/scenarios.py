@@ -17,6 +17,7 @@ from . import random as mr
 from .core import Scenario
 
 PRECISIONS = {"fp32": 0, "bf16x3": 1}
+ENGINE_GAUSSIAN = 2
 
 _lib.register("mocat_lr_packed_bytes", [_lib.c_i64, _lib.c_i64, _lib.c_ptr])
 _lib.register("mocat_lr_pack", [_lib.c_ptr, _lib.c_ptr, _lib.c_i64, _lib.c_i64, _lib.c_ptr, _lib.c_ptr])
@@ -106,4 +107,39 @@ class LogisticRegression(Scenario):
 
     def prior_sample(self, random_key):
         """sqrt(v) * normal(key, (d,)), batched over keys (..., 2)."""
+        return math.sqrt(self.prior_variance) * mr.normal(random_key, (self.dim,))
+
+
+class DiagonalGaussian(LogisticRegression):
+    """Gaussian likelihood with diagonal covariance and a N(0, prior_variance I) prior.
+
+    likelihood_potential(x) = 0.5 * sum_k (x_k - mean_k)^2 / covariance_k (no normalising constant, as
+    mocat's toy Gaussian, mocat/src/scenarios/toy_examples.py:19-51); prior_variance = inf gives mocat's
+    improper flat prior (potential 0). Evaluated by the same fused kernels (elementwise engine).
+    """
+    name = "Diagonal Gaussian"
+
+    def __init__(self, mean, covariance, prior_variance: float = 10.0, name: str = None):
+        Scenario.__init__(self, name=name)
+        self.mean_t = _dev_tensor(mean).reshape(-1)
+        cov = _dev_tensor(covariance).reshape(-1)
+        if cov.shape != self.mean_t.shape:
+            raise ValueError("mean and covariance (diagonal) must have the same length")
+        self.lam = (1.0 / cov).contiguous()
+        self.dim = int(self.mean_t.shape[0])
+        if self.dim > 128:
+            raise ValueError("the fused kernels support d <= 128")
+        self.n = 1
+        self.prior_variance = float(prior_variance)
+        self.prior_precision = 0.0 if math.isinf(self.prior_variance) else 1.0 / self.prior_variance
+        self.precision = "fp32"
+        self.packed = None
+
+    def kernel_args(self):
+        return (_lib.ptr(self.mean_t), _lib.ptr(self.lam), _lib.ptr(None), 1, self.dim, ENGINE_GAUSSIAN,
+                float(self.prior_precision))
+
+    def prior_sample(self, random_key):
+        if math.isinf(self.prior_variance):
+            raise AttributeError("flat prior: no prior_sample")
         return math.sqrt(self.prior_variance) * mr.normal(random_key, (self.dim,))

@@ -25,7 +25,7 @@ _P = _lib.c_ptr
 _lib.register("mocat_pf_param_count", [_lib.c_i32, _lib.c_i32, _lib.c_i32, _P])
 _lib.register("mocat_pf_run", [_lib.c_i32, _lib.c_i32, _lib.c_i32, _P, _lib.c_i32, _P, _P, _P, _P, _P, _P, _P, _P,
                                _lib.c_i32, _lib.c_f32, _P, _P, _P, _P, _P, _P, _P])
-_lib.register("mocat_resample", [_P, _lib.c_i64, _lib.c_i32, _P, _P, _P, _P])
+_lib.register("mocat_resample", [_P, _lib.c_i64, _lib.c_i32, _lib.c_i32, _P, _P, _P, _P])
 _lib.register("mocat_resample_categorical", [_P, _lib.c_i64, _lib.c_i64, _P, _P, _P])
 
 FILTER_DTYPE = np.dtype([("n", "<i8"), ("T", "<i8"), ("y_offset", "<i8"), ("state_offset", "<i8"),
@@ -50,15 +50,18 @@ class BootstrapFilter(ParticleFilter):
     name = "Bootstrap Filter"
 
 
-def resample_indices(weights: torch.Tensor, random_key, method: str = "systematic") -> torch.Tensor:
-    """Ancestor indices from non-negative weights: exact fixed-point cumulative sums, uniforms from
-    `random_key` (systematic: uniform(key); multinomial: uniform(key, (n,)))."""
+def resample_indices(weights: torch.Tensor, random_key, method: str = "systematic",
+                     log_weights: bool = False) -> torch.Tensor:
+    """Ancestor indices from non-negative weights (or log-weights, exponentiated as exp(w - max w)):
+    exact fixed-point cumulative sums, uniforms from `random_key` (systematic: uniform(key);
+    multinomial: uniform(key, (n,)))."""
     w = weights.to(torch.float32).contiguous()
     n = int(w.shape[0])
     key = mr.as_key(random_key).reshape(2).contiguous()
     ws = torch.empty(n, dtype=torch.int64, device=w.device)
     idx = torch.empty(n, dtype=torch.int32, device=w.device)
-    _lib.call("mocat_resample", _lib.ptr(w), n, {"systematic": 0, "multinomial": 1}[method], _lib.ptr(key),
+    _lib.call("mocat_resample", _lib.ptr(w), n, {"systematic": 0, "multinomial": 1}[method], int(log_weights),
+              _lib.ptr(key),
               _lib.ptr(ws), _lib.ptr(idx), _lib.stream_handle())
     return idx
 
