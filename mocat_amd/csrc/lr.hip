@@ -38,17 +38,15 @@ MOCAT_DEV int acc_row(int r, int lane) { return 8 * (r >> 2) + 4 * (lane >> 5) +
 
 // --- per-row likelihood pieces -------------------------------------------------------------
 // r = sigmoid(z) - y ; c = softplus(z) - y z. Rows with y < 0 are padding: r = c = 0.
+// Hardware exp2/log2/rcp (v_exp_f32, v_log_f32, v_rcp_f32): |error| of c per row <= ~1e-7 absolute.
 MOCAT_DEV void lr_row(float z, float y, float& r, float& c) {
-  if (y < 0.f) {
-    r = 0.f;
-    c = 0.f;
-    return;
-  }
   const float e = __expf(-fabsf(z));
-  const float inv = __frcp_rn(1.f + e);
+  const float ope = 1.f + e;
+  const float inv = __builtin_amdgcn_rcpf(ope);
   const float sig = z >= 0.f ? inv : e * inv;
-  r = sig - y;
-  c = fmaxf(z, 0.f) + log1pf(e) - y * z;
+  const bool valid = y >= 0.f;
+  r = valid ? sig - y : 0.f;
+  c = valid ? fmaxf(z, 0.f) + __logf(ope) - y * z : 0.f;
 }
 
 // ------------------------------------------------------------------------------------------
@@ -172,8 +170,10 @@ struct Bf16Engine {
   using L = PackLayout<DP>;
   static constexpr int KB = L::KB, NB = L::NB, NW = NT / kWave, HW = NW / 2;
   static_assert(NW % 2 == 0, "Bf16Engine needs an even wave count");
+  static constexpr bool kPrefetch = DP <= 64;
   struct alignas(16) Smem {
     alignas(16) float gred[HW][CT * DP];
+    alignas(16) bf16x8 thl[2][KB][64];  // theta^T operand, hi and lo parts, per lane
     float ured[NW][2][CT];
   };
 
@@ -184,19 +184,22 @@ struct Bf16Engine {
     const int nrb = (n + 31) / 32;
     const char* base = reinterpret_cast<const char*>(packed);
 
-    // theta^T as B operand: lane holds chain `col`, columns kb*16 + g*8 + s
-    bf16x8 th[KB], tl[KB];
-#pragma unroll
-    for (int kb = 0; kb < KB; ++kb) {
-      const float* src = theta + col * DP + kb * 16 + g * 8;
+    // theta^T as B operand: lane holds chain `col`, columns kb*16 + g*8 + s; split once into LDS
+    for (int e = t; e < KB * 64; e += NT) {
+      const int kb = e / 64, ln = e % 64;
+      const float* src = theta + (ln & 31) * DP + kb * 16 + (ln >> 5) * 8;
+      bf16x8 h8, l8;
 #pragma unroll
       for (int s2 = 0; s2 < 8; ++s2) {
         __bf16 h, l;
         split_bf16(src[s2], h, l);
-        th[kb][s2] = h;
-        tl[kb][s2] = l;
+        h8[s2] = h;
+        l8[s2] = l;
       }
+      s.thl[0][kb][ln] = h8;
+      s.thl[1][kb][ln] = l8;
     }
+    __syncthreads();
     floatx16 gacc[NB];
 #pragma unroll
     for (int nb = 0; nb < NB; ++nb)
@@ -204,32 +207,55 @@ struct Bf16Engine {
       for (int r = 0; r < 16; ++r) gacc[nb][r] = 0.f;
     float upart = 0.f;
 
-    for (int rb = w; rb < nrb; rb += NW) {
+    // software pipeline: the A operands (and y) of the next row block are loaded while the current
+    // block runs its MFMAs and elementwise work
+    bf16x8 ah[KB], al[KB];
+    floatx4 yq[4];
+    auto load_a = [&](int rb) {
       const char* blk = base + (int64_t)rb * L::BLOCK_BYTES;
       const bf16x8* A = reinterpret_cast<const bf16x8*>(blk);
-      const bf16x8* B = A + L::A_ELEMS / 8;
-      const float* Y = reinterpret_cast<const float*>(blk + L::BLOCK_BF16 * 2);
+#pragma unroll
+      for (int kb = 0; kb < KB; ++kb) {
+        ah[kb] = A[(0 * KB + kb) * 64 + lane];
+        al[kb] = A[(1 * KB + kb) * 64 + lane];
+      }
+    };
+    if (kPrefetch && w < nrb) load_a(w);
+    for (int rb = w; rb < nrb; rb += NW) {
+      if (!kPrefetch) load_a(rb);
+      const char* blk = base + (int64_t)rb * L::BLOCK_BYTES;
+      const bf16x8* B = reinterpret_cast<const bf16x8*>(blk) + L::A_ELEMS / 8;
       floatx16 z;
 #pragma unroll
       for (int r = 0; r < 16; ++r) z[r] = 0.f;
 #pragma unroll
       for (int kb = 0; kb < KB; ++kb) {
-        const bf16x8 ah = A[(0 * KB + kb) * 64 + lane];
-        const bf16x8 al = A[(1 * KB + kb) * 64 + lane];
-        z = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, th[kb], z, 0, 0, 0);
-        z = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, tl[kb], z, 0, 0, 0);
-        z = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, th[kb], z, 0, 0, 0);
+        const bf16x8 th = s.thl[0][kb][lane], tl = s.thl[1][kb][lane];
+        z = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[kb], th, z, 0, 0, 0);
+        z = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[kb], tl, z, 0, 0, 0);
+        z = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[kb], th, z, 0, 0, 0);
       }
-      const floatx4* yv4 = reinterpret_cast<const floatx4*>(Y + g * 16);
+      const floatx4* yv4 = reinterpret_cast<const floatx4*>(blk + L::BLOCK_BF16 * 2 + g * 64);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) yq[q] = yv4[q];
       float yv[16];
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        const floatx4 v = yv4[q];
-        yv[4 * q] = v.x;
-        yv[4 * q + 1] = v.y;
-        yv[4 * q + 2] = v.z;
-        yv[4 * q + 3] = v.w;
+        yv[4 * q] = yq[q].x;
+        yv[4 * q + 1] = yq[q].y;
+        yv[4 * q + 2] = yq[q].z;
+        yv[4 * q + 3] = yq[q].w;
       }
+      // B operands of this block, then the next block's A operands
+      bf16x8 xbh[2][NB], xbl[2][NB];
+#pragma unroll
+      for (int kb2 = 0; kb2 < 2; ++kb2)
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) {
+          xbh[kb2][nb] = B[((0 * 2 + kb2) * NB + nb) * 64 + lane];
+          xbl[kb2][nb] = B[((1 * 2 + kb2) * NB + nb) * 64 + lane];
+        }
+      if (kPrefetch && rb + NW < nrb) load_a(rb + NW);
       bf16x8 rh[2], rl[2];
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
@@ -245,11 +271,9 @@ struct Bf16Engine {
       for (int kb2 = 0; kb2 < 2; ++kb2) {
 #pragma unroll
         for (int nb = 0; nb < NB; ++nb) {
-          const bf16x8 xh = B[((0 * 2 + kb2) * NB + nb) * 64 + lane];
-          const bf16x8 xl = B[((1 * 2 + kb2) * NB + nb) * 64 + lane];
-          gacc[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(rh[kb2], xh, gacc[nb], 0, 0, 0);
-          gacc[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(rh[kb2], xl, gacc[nb], 0, 0, 0);
-          gacc[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(rl[kb2], xh, gacc[nb], 0, 0, 0);
+          gacc[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(rh[kb2], xbh[kb2][nb], gacc[nb], 0, 0, 0);
+          gacc[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(rh[kb2], xbl[kb2][nb], gacc[nb], 0, 0, 0);
+          gacc[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(rl[kb2], xbh[kb2][nb], gacc[nb], 0, 0, 0);
         }
       }
     }
@@ -367,7 +391,6 @@ struct alignas(16) TileState {
   alignas(16) float theta[CT * DP];
   float mom[CT * DP];
   float grad[CT * DP];  // grad of the tempered potential
-  float mom0[CT * DP];  // momenta after refresh (kept on rejection)
   float glik[CT * DP];
   float lik[CT];
   float pot[CT];
@@ -379,8 +402,24 @@ struct alignas(16) TileState {
   float step[CT];
   float alpha[CT];
   float kin0[CT];
+  float sq[CT];
   uint32_t key[CT][2];
 };
+
+// out[c] = sum_k v[c][k]^2 for the CT chains of a tile (one wave per chain group), then a barrier.
+template <int DP, int NT>
+MOCAT_DEV void chain_sqnorm(const float* v, int d, float* out) {
+  constexpr int NW = NT / kWave;
+  const int t = threadIdx.x, lane = t & (kWave - 1), w = t / kWave;
+  __syncthreads();
+  for (int c = w; c < CT; c += NW) {
+    float a = 0.f;
+    for (int k = lane; k < d; k += kWave) a += v[c * DP + k] * v[c * DP + k];
+    a = wave_sum(a);
+    if (lane == 0) out[c] = a;
+  }
+  __syncthreads();
+}
 
 struct LrArgs {
   const float* X;
@@ -524,15 +563,18 @@ __global__ __launch_bounds__(NT) void hmc_lr_kernel(HmcArgs h) {
       if (i + hd < (uint32_t)d) p[i + hd] = -p[i + hd] * decay + noise * normal_from_bits(w1);
     }
     __syncthreads();
-    // keep the pre-proposal state for rejection
-    for (int e = t; e < CT * DP; e += NT) ts.mom0[e] = ts.mom[e];
+    // keep the refreshed momenta for rejection in the global momenta array (position and gradient of
+    // the pre-proposal state stay in global memory until a proposal is accepted)
+    for (int e = t; e < CT * DP; e += NT) {
+      const int c = e / DP, k = e - c * DP;
+      if (c < nvalid && k < d) h.mom[(c0 + c) * d + k] = ts.mom[e];
+    }
+    chain_sqnorm<DP, NT>(ts.mom, d, ts.sq);
     if (t < CT) {
       ts.pot0[t] = ts.pot[t];
       ts.prior0[t] = ts.prior[t];
       ts.lik0[t] = ts.likv[t];
-      float kk = 0.f;
-      for (int k = 0; k < d; ++k) kk += ts.mom[t * DP + k] * ts.mom[t * DP + k];
-      ts.kin0[t] = 0.5f * kk + 0.5f * (float)d * kLog2Pi;
+      ts.kin0[t] = 0.5f * ts.sq[t] + 0.5f * (float)d * kLog2Pi;
     }
     // the pre-proposal position/gradient stay in global memory (unchanged until accepted)
     __syncthreads();
@@ -555,10 +597,9 @@ __global__ __launch_bounds__(NT) void hmc_lr_kernel(HmcArgs h) {
     }
     // ---- correction (metropolis.py:48-70, 118-161)
     __shared__ int accept[CT];
+    chain_sqnorm<DP, NT>(ts.mom, d, ts.sq);
     if (t < CT) {
-      float kk = 0.f;
-      for (int k = 0; k < d; ++k) kk += ts.mom[t * DP + k] * ts.mom[t * DP + k];
-      const float kin1 = 0.5f * kk + 0.5f * (float)d * kLog2Pi;  // |-p|^2 == |p|^2
+      const float kin1 = 0.5f * ts.sq[t] + 0.5f * (float)d * kLog2Pi;  // |-p|^2 == |p|^2
       float a = 1.f;
       int acc = 1;
       if (h.correction > 0) {
@@ -594,7 +635,7 @@ __global__ __launch_bounds__(NT) void hmc_lr_kernel(HmcArgs h) {
           h.grad[(c0 + c) * d + k] = ts.grad[e];
         }
       } else {
-        ts.mom[e] = ts.mom0[e];
+        ts.mom[e] = (c < nvalid && k < d) ? h.mom[(c0 + c) * d + k] : 0.f;
         if (c < nvalid && k < d) {
           ts.theta[e] = h.theta[(c0 + c) * d + k];
           ts.grad[e] = h.grad[(c0 + c) * d + k];

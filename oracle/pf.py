@@ -140,8 +140,12 @@ def resample_indices(w, key, method):
 
 
 # ----------------------------------------------------------------------------- filter
-def run_pf(p, y, key, n, ess_threshold=0.5, resampling="systematic", dtype=np.float64):
-    """Returns dict(value (T, n, d), log_weight (T, n), ess (T,), log_likelihood, ancestors (T, n))."""
+def run_pf(p, y, key, n, ess_threshold=0.5, resampling="systematic", dtype=np.float64, ancestors=None,
+           resample_flags=None):
+    """Returns dict(value (T, n, d), log_weight (T, n), ess (T,), log_likelihood, ancestors (T, n)).
+
+    ancestors / resample_flags (optional, (T, n) and (T,)): replay given resampling decisions and
+    indices instead of drawing them (the fixed-resampling-index path of another run)."""
     y = np.asarray(y, np.float64).reshape(len(y), -1)
     T = y.shape[0]
     d = p["d"]
@@ -157,10 +161,13 @@ def run_pf(p, y, key, n, ess_threshold=0.5, resampling="systematic", dtype=np.fl
     int_keys = rng.split(rkey, T - 1) if T > 1 else np.zeros((0, 2), np.uint32)
     for t in range(1, T):
         ik = int_keys[t - 1]
-        res = esss[-1] < ess_threshold * n
+        res = esss[-1] < ess_threshold * n if resample_flags is None else bool(resample_flags[t])
         if res:
-            w = np.exp(np.asarray(lw, np.float64) - l0).astype(np.float32)
-            a = resample_indices(w, ik, resampling)
+            if ancestors is not None:
+                a = np.asarray(ancestors[t], np.int64)
+            else:
+                w = np.exp(np.asarray(lw, np.float64) - l0).astype(np.float32)
+                a = resample_indices(w, ik, resampling)
             lw_res = np.zeros(n)
             lres = np.log(n)
         else:
