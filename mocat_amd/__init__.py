@@ -12,7 +12,9 @@ from .sample import Sampler, run
 from .mcmc import MCMCSampler, Underdamped, Correction, Uncorrected, Metropolis, RMMetropolis
 from .scenarios import LogisticRegression, DiagonalGaussian
 from .transport import (TransportSampler, SMCSampler, TemperedSMCSampler, MetropolisedSMCSampler,
-                        RMMetropolisedSMCSampler)
+                        RMMetropolisedSMCSampler, SVGD)
+from .scenarios import GaussianMixture
+from . import kernels
 from . import reduce, metrics, ssm
 
 __version__ = "0.1.0"

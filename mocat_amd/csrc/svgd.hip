@@ -1,0 +1,535 @@
+// Stein variational gradient descent (mocat/src/transport/svgd.py) with the Gaussian RBF kernel
+// (mocat/src/kernels.py:82-116) and a diagonal Gaussian-mixture target.
+//
+// phi_i = (1/N) sum_j [ -k_ij gradU_j + (x_i - x_j) k_ij / h^2 ],  k_ij = exp(-|x_i - x_j|^2 / (2 h^2))
+//       = (1/N) [ -(K G)_i + (s_i x_i - (K X)_i) / h^2 ],  s_i = sum_j k_ij        (svgd.py:18-32)
+// computed flash-attention style per tile of 32 particles i: S = X_J X_I^T on MFMA (bf16 hi/lo
+// split, FP32 accumulation) for 32 particles j at a time, k_ij formed in registers from
+// |x_i|^2 + |x_j|^2 - 2 S (particles centred first: distances are translation invariant and the
+// centring keeps the cancellation small), then K [G | X]_J accumulated on MFMA with the k values as
+// the A operand (chains of the accumulator rows = the j rows of the packed B layout).
+// The j range is split over workgroups; partial sums are combined by the update kernel, which also
+// applies jax.example_libraries.optimizers.adagrad(step, momentum=0.9) with gradient -phi
+// (svgd.py:104-106, 138-139):  G += g^2; m = 0.1 g / sqrt(G) + 0.9 m; x -= step m.
+//
+// Bandwidth: median (or mean) of all N^2 pairwise distances, diagonal zeros included, divided by
+// sqrt(2 log N) (kernels.py:220-229). Squared distances are written once (N x N f32) and the two
+// middle order statistics found exactly by an 8-bit radix select over their bit patterns.
+#include "common.hpp"
+
+using namespace mocat;
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+namespace {
+
+constexpr int CT = 32;
+
+MOCAT_DEV int acc_row(int r, int lane) { return 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3); }
+
+MOCAT_DEV void split_bf16(float v, __bf16& hi, __bf16& lo) {
+  hi = (__bf16)v;
+  lo = (__bf16)(v - (float)hi);
+}
+
+template <int DP>
+struct SvgdLayout {
+  static constexpr int KB = DP / 16;       // K blocks of the distance GEMM
+  static constexpr int NB = (2 * DP) / 32;  // N blocks of [G | X] (width 2 DP)
+  static constexpr int A_ELEMS = 2 * KB * 512;
+  static constexpr int B_ELEMS = 2 * 2 * NB * 512;
+  static constexpr int BLOCK_BF16 = A_ELEMS + B_ELEMS;
+  static constexpr int BLOCK_BYTES = BLOCK_BF16 * 2 + 32 * 4;
+};
+
+// Column means of x (one thread per column, FP64 accumulation).
+__global__ void colmean_kernel(const float* __restrict__ x, int n, int d, float* __restrict__ mean) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= d) return;
+  double s = 0.0;
+  for (int i = 0; i < n; ++i) s += (double)x[(int64_t)i * d + k];
+  mean[k] = (float)(s / (double)n);
+}
+
+// Pack the centred particles (A layout), [grad | centred x] (B layout) and |x_j|^2 per 32-row block.
+template <int DP>
+__global__ void svgd_pack_kernel(const float* __restrict__ x, const float* __restrict__ g,
+                                 const float* __restrict__ mean, int n, int d, char* __restrict__ out) {
+  using L = SvgdLayout<DP>;
+  const int nrb = (n + 31) / 32;
+  const int64_t per = L::BLOCK_BF16 + 32;
+  const int64_t total = (int64_t)nrb * per;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int rb = (int)(e / per);
+    const int q = (int)(e - rb * per);
+    char* blk = out + (int64_t)rb * L::BLOCK_BYTES;
+    if (q < L::A_ELEMS) {
+      const int s = q & 7, lane = (q >> 3) & 63, pk = q >> 9;
+      const int part = pk / L::KB, kb = pk - part * L::KB;
+      const int row = rb * 32 + (lane & 31), col = kb * 16 + (lane >> 5) * 8 + s;
+      const float v = (row < n && col < d) ? x[(int64_t)row * d + col] - mean[col] : 0.f;
+      __bf16 h, l;
+      split_bf16(v, h, l);
+      reinterpret_cast<__bf16*>(blk)[q] = part ? l : h;
+    } else if (q < L::BLOCK_BF16) {
+      const int qq = q - L::A_ELEMS;
+      const int s = qq & 7, lane = (qq >> 3) & 63, rest = qq >> 9;
+      const int nb = rest % L::NB, pk2 = rest / L::NB, kb2 = pk2 & 1, part = pk2 >> 1;
+      const int row = rb * 32 + acc_row(kb2 * 8 + s, lane), col = nb * 32 + (lane & 31);
+      float v = 0.f;
+      if (row < n) {
+        if (col < DP) {
+          if (col < d) v = g[(int64_t)row * d + col];
+        } else if (col - DP < d) {
+          v = x[(int64_t)row * d + (col - DP)] - mean[col - DP];
+        }
+      }
+      __bf16 h, l;
+      split_bf16(v, h, l);
+      reinterpret_cast<__bf16*>(blk)[q] = part ? l : h;
+    } else {
+      const int qq = q - L::BLOCK_BF16;
+      const int gg = qq >> 4, r = qq & 15;
+      const int row = rb * 32 + 8 * (r >> 2) + 4 * gg + (r & 3);
+      float nrm = -1.f;  // padding rows
+      if (row < n) {
+        nrm = 0.f;
+        for (int k = 0; k < d; ++k) {
+          const float v = x[(int64_t)row * d + k] - mean[k];
+          nrm += v * v;
+        }
+      }
+      reinterpret_cast<float*>(blk + L::BLOCK_BF16 * 2)[qq] = nrm;
+    }
+  }
+}
+
+// Tile of 32 particles i: centred coordinates in LDS, their squared norms, and the bf16 hi/lo
+// B operand of the distance GEMM (lane: particle l%32, columns kb*16 + (l/32)*8 + s).
+template <int DP>
+struct ITile {
+  alignas(16) float xi[CT * DP];
+  alignas(16) bf16x8 op[2][DP / 16][64];
+  float ni[CT];
+};
+
+template <int DP, int NT>
+MOCAT_DEV void load_itile(ITile<DP>& it, const float* __restrict__ x, const float* __restrict__ mean, int n, int d,
+                          int i0) {
+  const int t = threadIdx.x;
+  for (int e = t; e < CT * DP; e += NT) {
+    const int c = e / DP, k = e - c * DP;
+    it.xi[e] = (i0 + c < n && k < d) ? x[(int64_t)(i0 + c) * d + k] - mean[k] : 0.f;
+  }
+  __syncthreads();
+  for (int e = t; e < (DP / 16) * 64; e += NT) {
+    const int kb = e / 64, ln = e % 64;
+    const float* src = it.xi + (ln & 31) * DP + kb * 16 + (ln >> 5) * 8;
+    bf16x8 h8, l8;
+#pragma unroll
+    for (int s2 = 0; s2 < 8; ++s2) {
+      __bf16 h, l;
+      split_bf16(src[s2], h, l);
+      h8[s2] = h;
+      l8[s2] = l;
+    }
+    it.op[0][kb][ln] = h8;
+    it.op[1][kb][ln] = l8;
+  }
+  if (t < CT) {
+    float a = 0.f;
+    for (int k = 0; k < d; ++k) a += it.xi[t * DP + k] * it.xi[t * DP + k];
+    it.ni[t] = a;
+  }
+  __syncthreads();
+}
+
+// S = X_J X_I^T for one 32-row block of j (accumulator: lane column = particle i, rows = j).
+template <int DP>
+MOCAT_DEV floatx16 gram_block(const bf16x8* __restrict__ A, const ITile<DP>& it, int lane) {
+  constexpr int KB = DP / 16;
+  floatx16 z;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) z[r] = 0.f;
+#pragma unroll
+  for (int kb = 0; kb < KB; ++kb) {
+    const bf16x8 ah = A[(0 * KB + kb) * 64 + lane];
+    const bf16x8 al = A[(1 * KB + kb) * 64 + lane];
+    const bf16x8 bh = it.op[0][kb][lane], bl = it.op[1][kb][lane];
+    z = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh, z, 0, 0, 0);
+    z = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl, z, 0, 0, 0);
+    z = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh, z, 0, 0, 0);
+  }
+  return z;
+}
+
+// ---------------------------------------------------------------- phi (partial over a j range)
+template <int DP, int NT>
+__global__ __launch_bounds__(NT) void svgd_phi_kernel(const char* __restrict__ packed, const float* __restrict__ x,
+                                                      const float* __restrict__ mean, int n, int d, float inv2h2,
+                                                      int js_count, float* __restrict__ part_o,
+                                                      float* __restrict__ part_s) {
+  using L = SvgdLayout<DP>;
+  constexpr int NB = L::NB, NW = NT / kWave, HW = NW / 2;
+  static_assert(NW % 2 == 0, "even wave count");
+  __shared__ ITile<DP> it;
+  __shared__ alignas(16) float gred[HW][CT * 2 * DP];
+  __shared__ float sred[NW][2][CT];
+  const int t = threadIdx.x, lane = t & (kWave - 1), w = t / kWave, g = lane >> 5, col = lane & 31;
+  const int i0 = blockIdx.x * CT;
+  const int nrb = (n + 31) / 32;
+  const int js = blockIdx.y;
+  const int rb0 = (int)((int64_t)nrb * js / js_count), rb1 = (int)((int64_t)nrb * (js + 1) / js_count);
+  load_itile<DP, NT>(it, x, mean, n, d, i0);
+  const float ni = it.ni[col];
+  floatx16 gacc[NB];
+#pragma unroll
+  for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+    for (int r = 0; r < 16; ++r) gacc[nb][r] = 0.f;
+  float spart = 0.f;
+  for (int rb = rb0 + w; rb < rb1; rb += NW) {
+    const char* blk = packed + (int64_t)rb * L::BLOCK_BYTES;
+    const bf16x8* A = reinterpret_cast<const bf16x8*>(blk);
+    const bf16x8* B = A + L::A_ELEMS / 8;
+    const floatx16 z = gram_block<DP>(A, it, lane);
+    const floatx4* nj4 = reinterpret_cast<const floatx4*>(blk + L::BLOCK_BF16 * 2 + g * 64);
+    float nj[16];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const floatx4 v = nj4[q];
+      nj[4 * q] = v.x;
+      nj[4 * q + 1] = v.y;
+      nj[4 * q + 2] = v.z;
+      nj[4 * q + 3] = v.w;
+    }
+    bf16x8 kh[2], kl[2];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const float D = fmaxf(ni + nj[r] - 2.f * z[r], 0.f);
+      const float kv = nj[r] >= 0.f ? __expf(-D * inv2h2) : 0.f;
+      spart += kv;
+      __bf16 h, l;
+      split_bf16(kv, h, l);
+      kh[r >> 3][r & 7] = h;
+      kl[r >> 3][r & 7] = l;
+    }
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb) {
+#pragma unroll
+      for (int kb2 = 0; kb2 < 2; ++kb2) {
+        const bf16x8 xh = B[((0 * 2 + kb2) * NB + nb) * 64 + lane];
+        const bf16x8 xl = B[((1 * 2 + kb2) * NB + nb) * 64 + lane];
+        gacc[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(kh[kb2], xh, gacc[nb], 0, 0, 0);
+        gacc[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(kh[kb2], xl, gacc[nb], 0, 0, 0);
+        gacc[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(kl[kb2], xh, gacc[nb], 0, 0, 0);
+      }
+    }
+  }
+  // deterministic cross-wave reduction; accumulator rows = particles i (acc_row), columns = features
+  sred[w][g][col] = spart;
+  __syncthreads();
+  if (w >= HW) {
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) gred[w - HW][acc_row(r, lane) * 2 * DP + nb * 32 + col] = gacc[nb][r];
+  }
+  __syncthreads();
+  if (w < HW) {
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        float* p = &gred[w][acc_row(r, lane) * 2 * DP + nb * 32 + col];
+        *p = gacc[nb][r] + *p;
+      }
+  }
+  __syncthreads();
+  for (int e = t; e < CT * 2 * DP; e += NT) {
+    const int c = e / (2 * DP);
+    float acc = gred[0][e];
+#pragma unroll
+    for (int q = 1; q < HW; ++q) acc += gred[q][e];
+    if (i0 + c < n) part_o[((int64_t)js * n + i0 + c) * 2 * DP + (e - c * 2 * DP)] = acc;
+  }
+  if (t < CT && i0 + t < n) {
+    float acc = 0.f;
+    for (int q = 0; q < NW; ++q) acc += sred[q][0][t] + sred[q][1][t];
+    part_s[(int64_t)js * n + i0 + t] = acc;
+  }
+}
+
+// phi from the partial sums, then the adagrad step (x, G, m updated in place); phi_out optional.
+__global__ void svgd_update_kernel(const float* __restrict__ part_o, const float* __restrict__ part_s, int js_count,
+                                   int n, int d, int DP, const float* __restrict__ mean, float inv_n, float inv_h2,
+                                   float step, float momentum, float* __restrict__ x, float* __restrict__ gsq,
+                                   float* __restrict__ mom, float* __restrict__ phi_out) {
+  const int64_t total = (int64_t)n * d;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = e / d;
+    const int k = (int)(e - i * d);
+    float kg = 0.f, kx = 0.f, s = 0.f;
+    for (int js = 0; js < js_count; ++js) {
+      const float* o = part_o + ((int64_t)js * n + i) * 2 * DP;
+      kg += o[k];
+      kx += o[DP + k];
+      s += part_s[(int64_t)js * n + i];
+    }
+    const float xc = x[e] - mean[k];
+    const float phi = inv_n * (-kg + (s * xc - kx) * inv_h2);
+    if (phi_out) phi_out[e] = phi;
+    const float gr = -phi;
+    const float G = gsq[e] + gr * gr;
+    gsq[e] = G;
+    const float ginv = G > 0.f ? 1.f / sqrtf(G) : 0.f;
+    const float m = (1.f - momentum) * (gr * ginv) + momentum * mom[e];
+    mom[e] = m;
+    x[e] = x[e] - step * m;
+  }
+}
+
+// ---------------------------------------------------------------- Gaussian mixture (diagonal)
+// U(x) = -logsumexp_k [ logc_k - 0.5 sum_j lam_kj (x_j - mu_kj)^2 ],  grad = sum_k r_k lam_k (x - mu_k)
+// (mocat/src/scenarios/toy_examples.py:54-116, evaluated in log-sum-exp form). One wave per particle.
+__global__ __launch_bounds__(256) void gmm_kernel(const float* __restrict__ x, int n, int d, int K,
+                                                  const float* __restrict__ mu, const float* __restrict__ lam,
+                                                  const float* __restrict__ logc, float* __restrict__ pot,
+                                                  float* __restrict__ grad) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int64_t i = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+  if (i >= n) return;
+  const float* xi = x + i * d;
+  float q[8];
+  float qmax = -INFINITY;
+  for (int k = 0; k < K; ++k) {
+    float a = 0.f;
+    for (int j = lane; j < d; j += kWave) {
+      const float df = xi[j] - mu[k * d + j];
+      a += lam[k * d + j] * df * df;
+    }
+    a = wave_sum(a);
+    q[k] = logc[k] - 0.5f * a;
+    qmax = fmaxf(qmax, q[k]);
+  }
+  float ssum = 0.f;
+  for (int k = 0; k < K; ++k) ssum += __expf(q[k] - qmax);
+  if (lane == 0) pot[i] = -(qmax + __logf(ssum));
+  for (int j = lane; j < d; j += kWave) {
+    float gsum = 0.f;
+    for (int k = 0; k < K; ++k) gsum += __expf(q[k] - qmax) / ssum * lam[k * d + j] * (xi[j] - mu[k * d + j]);
+    grad[i * d + j] = gsum;
+  }
+}
+
+// ---------------------------------------------------------------- pairwise squared distances
+template <int DP, int NT>
+__global__ __launch_bounds__(NT) void sqdist_kernel(const char* __restrict__ packed, const float* __restrict__ x,
+                                                    const float* __restrict__ mean, int n, int d,
+                                                    float* __restrict__ d2) {
+  using L = SvgdLayout<DP>;
+  constexpr int NW = NT / kWave;
+  __shared__ ITile<DP> it;
+  const int t = threadIdx.x, lane = t & (kWave - 1), w = t / kWave, g = lane >> 5, col = lane & 31;
+  const int i0 = blockIdx.x * CT;
+  const int nrb = (n + 31) / 32;
+  load_itile<DP, NT>(it, x, mean, n, d, i0);
+  const float ni = it.ni[col];
+  const int i = i0 + col;
+  for (int rb = blockIdx.y * NW + w; rb < nrb; rb += gridDim.y * NW) {
+    const char* blk = packed + (int64_t)rb * L::BLOCK_BYTES;
+    const floatx16 z = gram_block<DP>(reinterpret_cast<const bf16x8*>(blk), it, lane);
+    const float* nj = reinterpret_cast<const float*>(blk + L::BLOCK_BF16 * 2) + g * 16;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int j = rb * 32 + acc_row(r, lane);
+      if (i < n && j < n) d2[(int64_t)j * n + i] = (i == j) ? 0.f : fmaxf(ni + nj[r] - 2.f * z[r], 0.f);
+    }
+  }
+}
+
+// ---------------------------------------------------------------- exact radix select (2 ranks)
+// state[0..1] prefix bits, state[2..3] remaining rank, hist[2][256]
+__global__ __launch_bounds__(256) void radix_hist_kernel(const uint32_t* __restrict__ v, int64_t m, int shift,
+                                                         const uint32_t* __restrict__ state,
+                                                         uint32_t* __restrict__ hist) {
+  __shared__ uint32_t lh[2][256];
+  const uint32_t mask = shift == 24 ? 0u : (0xFFFFFFFFu << (shift + 8));
+  const uint32_t p0 = state[0] & mask, p1 = state[1] & mask;
+  for (int b = threadIdx.x; b < 512; b += 256) (&lh[0][0])[b] = 0u;
+  __syncthreads();
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < m; e += (int64_t)gridDim.x * blockDim.x) {
+    const uint32_t u = v[e];
+    const uint32_t dig = (u >> shift) & 255u;
+    if ((u & mask) == p0) atomicAdd(&lh[0][dig], 1u);
+    if ((u & mask) == p1) atomicAdd(&lh[1][dig], 1u);
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < 512; b += 256) {
+    const uint32_t c = (&lh[0][0])[b];
+    if (c) atomicAdd(&hist[b], c);
+  }
+}
+
+__global__ void radix_choose_kernel(int shift, uint32_t* __restrict__ state, uint32_t* __restrict__ hist) {
+  const int tsel = threadIdx.x;  // 0, 1
+  if (tsel < 2) {
+    uint64_t k = state[2 + tsel];
+    uint32_t b = 0;
+    for (; b < 256; ++b) {
+      const uint32_t c = hist[tsel * 256 + b];
+      if (k < c) break;
+      k -= c;
+    }
+    if (b > 255) b = 255;
+    state[tsel] = (state[tsel] & ~(255u << shift)) | (b << shift);
+    state[2 + tsel] = (uint32_t)k;
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < 512; b += blockDim.x) hist[b] = 0u;
+}
+
+__global__ void select_init_kernel(uint32_t* __restrict__ ws, uint32_t rank0, uint32_t rank1) {
+  for (int b = threadIdx.x; b < 4 + 512; b += blockDim.x) ws[b] = 0u;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    ws[2] = rank0;
+    ws[3] = rank1;
+  }
+}
+
+// mean of sqrt(d2) over all entries, FP64 accumulation (one value per block, summed on the host side
+// by a second single-block pass)
+__global__ __launch_bounds__(256) void sqrt_sum_kernel(const float* __restrict__ d2, int64_t m,
+                                                       double* __restrict__ partial) {
+  __shared__ double sc[4];
+  double s = 0.0;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < m; e += (int64_t)gridDim.x * blockDim.x)
+    s += (double)sqrtf(d2[e]);
+  s = block_sum(s, sc);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+constexpr int kPhiThreads = 256;
+constexpr int kDistThreads = 256;
+
+template <int DP>
+int svgd_step_dp(const float* x_in, const float* grad, int n, int d, float bandwidth, int js_count, void* packed,
+                 float* mean, float* part_o, float* part_s, float step, float momentum, float* x, float* gsq,
+                 float* mom, float* phi_out, hipStream_t st) {
+  using L = SvgdLayout<DP>;
+  hipLaunchKernelGGL(colmean_kernel, dim3((d + 127) / 128), dim3(128), 0, st, x_in, n, d, mean);
+  const int nrb = (n + 31) / 32;
+  const int64_t tot = (int64_t)nrb * (L::BLOCK_BF16 + 32);
+  const unsigned pb = (unsigned)std::min<int64_t>((tot + 255) / 256, 65535);
+  hipLaunchKernelGGL(svgd_pack_kernel<DP>, dim3(pb), dim3(256), 0, st, x_in, grad, mean, n, d, (char*)packed);
+  const float h2 = bandwidth * bandwidth;
+  hipLaunchKernelGGL((svgd_phi_kernel<DP, kPhiThreads>), dim3((unsigned)nrb, (unsigned)js_count), dim3(kPhiThreads),
+                     0, st, (const char*)packed, x_in, mean, n, d, 0.5f / h2, js_count, part_o, part_s);
+  const int64_t tot2 = (int64_t)n * d;
+  const unsigned ub = (unsigned)std::min<int64_t>((tot2 + 255) / 256, 65535);
+  hipLaunchKernelGGL(svgd_update_kernel, dim3(ub), dim3(256), 0, st, part_o, part_s, js_count, n, d, DP, mean,
+                     1.f / (float)n, 1.f / h2, step, momentum, x, gsq, mom, phi_out);
+  MOCAT_LAUNCH_CHECK();
+  return MOCAT_OK;
+}
+
+template <int DP>
+int sqdist_dp(const float* x, int n, int d, void* packed, float* mean, float* d2, hipStream_t st) {
+  using L = SvgdLayout<DP>;
+  hipLaunchKernelGGL(colmean_kernel, dim3((d + 127) / 128), dim3(128), 0, st, x, n, d, mean);
+  const int nrb = (n + 31) / 32;
+  const int64_t tot = (int64_t)nrb * (L::BLOCK_BF16 + 32);
+  const unsigned pb = (unsigned)std::min<int64_t>((tot + 255) / 256, 65535);
+  // the packed B layout needs a gradient: pass x again (only the A layout and norms are read here)
+  hipLaunchKernelGGL(svgd_pack_kernel<DP>, dim3(pb), dim3(256), 0, st, x, x, mean, n, d, (char*)packed);
+  const int gy = std::max(1, std::min(64, nrb / (kDistThreads / kWave)));
+  hipLaunchKernelGGL((sqdist_kernel<DP, kDistThreads>), dim3((unsigned)nrb, (unsigned)gy), dim3(kDistThreads), 0, st,
+                     (const char*)packed, x, mean, n, d, d2);
+  MOCAT_LAUNCH_CHECK();
+  return MOCAT_OK;
+}
+
+int64_t packed_bytes(int n, int d) {
+  const int64_t nrb = (n + 31) / 32;
+  return nrb * (d <= 64 ? SvgdLayout<64>::BLOCK_BYTES : SvgdLayout<128>::BLOCK_BYTES);
+}
+
+}  // namespace
+
+extern "C" {
+
+int mocat_svgd_workspace_bytes(int64_t n, int64_t d, int js_count, int64_t* bytes) {
+  if (n <= 0 || d <= 0 || d > 128 || js_count <= 0 || !bytes) return MOCAT_ERR_ARG;
+  const int DP = d <= 64 ? 64 : 128;
+  const int64_t pk = (packed_bytes((int)n, (int)d) + 255) / 256 * 256;
+  const int64_t part = (int64_t)js_count * n * (2 * DP + 1) * 4;
+  *bytes = pk + 512 + part;  // packed | mean (<= 128 floats) | partial sums
+  return MOCAT_OK;
+}
+
+int mocat_svgd_step(const float* x_in, const float* grad, int64_t n, int64_t d, float bandwidth, int js_count,
+                    void* workspace, float step, float momentum, float* x, float* gsq, float* mom, float* phi_out,
+                    hipStream_t stream) {
+  if (!x_in || !grad || !workspace || !x || !gsq || !mom || n <= 0 || n > (1 << 26) || d <= 0 || d > 128 ||
+      js_count <= 0 || js_count > 64 || !(bandwidth > 0.f))
+    return MOCAT_ERR_ARG;
+  const int DP = d <= 64 ? 64 : 128;
+  char* ws = (char*)workspace;
+  const int64_t pk = (packed_bytes((int)n, (int)d) + 255) / 256 * 256;
+  float* mean = (float*)(ws + pk);
+  float* part_o = (float*)(ws + pk + 512);
+  float* part_s = part_o + (int64_t)js_count * n * 2 * DP;
+  if (DP == 64)
+    return svgd_step_dp<64>(x_in, grad, (int)n, (int)d, bandwidth, js_count, ws, mean, part_o, part_s, step, momentum,
+                            x, gsq, mom, phi_out, stream);
+  return svgd_step_dp<128>(x_in, grad, (int)n, (int)d, bandwidth, js_count, ws, mean, part_o, part_s, step, momentum,
+                           x, gsq, mom, phi_out, stream);
+}
+
+int mocat_gmm_potential_grad(const float* x, int64_t n, int64_t d, int K, const float* means, const float* lam,
+                             const float* logc, float* potential, float* grad, hipStream_t stream) {
+  if (!x || !means || !lam || !logc || !potential || !grad || n <= 0 || d <= 0 || K <= 0 || K > 8) return MOCAT_ERR_ARG;
+  const int per = 256 / kWave;
+  hipLaunchKernelGGL(gmm_kernel, dim3((unsigned)((n + per - 1) / per)), dim3(256), 0, stream, x, (int)n, (int)d, K,
+                     means, lam, logc, potential, grad);
+  MOCAT_LAUNCH_CHECK();
+  return MOCAT_OK;
+}
+
+int mocat_pairwise_sqdist(const float* x, int64_t n, int64_t d, void* workspace, float* d2, hipStream_t stream) {
+  if (!x || !workspace || !d2 || n <= 0 || n > (1 << 16) || d <= 0 || d > 128) return MOCAT_ERR_ARG;
+  char* ws = (char*)workspace;
+  const int64_t pk = (packed_bytes((int)n, (int)d) + 255) / 256 * 256;
+  float* mean = (float*)(ws + pk);
+  return d <= 64 ? sqdist_dp<64>(x, (int)n, (int)d, ws, mean, d2, stream)
+                 : sqdist_dp<128>(x, (int)n, (int)d, ws, mean, d2, stream);
+}
+
+// Exact order statistics of m non-negative floats (bit patterns ordered as unsigned ints):
+// out_bits[t] = bit pattern of the rank[t]-th smallest value (0-based), t = 0, 1.
+// workspace: 4 + 512 uint32 (state: prefixes, remaining ranks; then two 256-bin histograms).
+int mocat_select2(const float* v, int64_t m, int64_t rank0, int64_t rank1, uint32_t* workspace, hipStream_t stream) {
+  if (!v || !workspace || m <= 0 || rank0 < 0 || rank1 < 0 || rank0 >= m || rank1 >= m || m > 0xFFFFFFFFll)
+    return MOCAT_ERR_ARG;
+  hipLaunchKernelGGL(select_init_kernel, dim3(1), dim3(256), 0, stream, workspace, (uint32_t)rank0, (uint32_t)rank1);
+  const int64_t blocks = std::min<int64_t>((m + 255) / 256, 2048);
+  for (int shift = 24; shift >= 0; shift -= 8) {
+    hipLaunchKernelGGL(radix_hist_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, (const uint32_t*)v, m, shift,
+                       workspace, workspace + 4);
+    hipLaunchKernelGGL(radix_choose_kernel, dim3(1), dim3(256), 0, stream, shift, workspace, workspace + 4);
+  }
+  MOCAT_LAUNCH_CHECK();
+  return MOCAT_OK;
+}
+
+int mocat_sqrt_mean(const float* d2, int64_t m, double* partial, int n_partial, hipStream_t stream) {
+  if (!d2 || !partial || m <= 0 || n_partial <= 0 || n_partial > 4096) return MOCAT_ERR_ARG;
+  hipLaunchKernelGGL(sqrt_sum_kernel, dim3((unsigned)n_partial), dim3(256), 0, stream, d2, m, partial);
+  MOCAT_LAUNCH_CHECK();
+  return MOCAT_OK;
+}
+
+}  // extern "C"

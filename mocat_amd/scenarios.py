@@ -19,6 +19,8 @@ from .core import Scenario
 PRECISIONS = {"fp32": 0, "bf16x3": 1}
 ENGINE_GAUSSIAN = 2
 
+_lib.register("mocat_gmm_potential_grad", [_lib.c_ptr, _lib.c_i64, _lib.c_i64, _lib.c_i32, _lib.c_ptr, _lib.c_ptr,
+                                           _lib.c_ptr, _lib.c_ptr, _lib.c_ptr, _lib.c_ptr])
 _lib.register("mocat_lr_packed_bytes", [_lib.c_i64, _lib.c_i64, _lib.c_ptr])
 _lib.register("mocat_lr_pack", [_lib.c_ptr, _lib.c_ptr, _lib.c_i64, _lib.c_i64, _lib.c_ptr, _lib.c_ptr])
 _lib.register("mocat_lr_potential_grad",
@@ -143,3 +145,75 @@ class DiagonalGaussian(LogisticRegression):
         if math.isinf(self.prior_variance):
             raise AttributeError("flat prior: no prior_sample")
         return math.sqrt(self.prior_variance) * mr.normal(random_key, (self.dim,))
+
+
+class GaussianMixture(Scenario):
+    """Gaussian mixture with diagonal covariances (mocat/src/scenarios/toy_examples.py:54-116).
+
+    potential(x) = -log sum_k w_k c_k exp(-0.5 sum_j (x_j - mu_kj)^2 / var_kj), evaluated in
+    log-sum-exp form. det_convention="textbook" uses c_k = sqrt(det P_k) (2 pi)^(-d/2) (a normalised
+    mixture); "mocat" uses det(P_k) (2 pi)^(-d/2), the reference's weighting (toy_examples.py:101-103).
+    No prior: prior_potential = 0 (mocat warns and assumes a uniform prior, core.py:183-189).
+    """
+    name = "Gaussian Mixture"
+
+    def __init__(self, means, covariances=None, weights=None, det_convention: str = "textbook", name: str = None):
+        super().__init__(name=name)
+        mu = np.asarray(means.detach().cpu() if isinstance(means, torch.Tensor) else means, np.float64)
+        if mu.ndim == 1:
+            mu = mu[:, None]
+        K, d = mu.shape
+        if K > 8:
+            raise ValueError("at most 8 components")
+        if covariances is None:
+            var = np.ones((K, d))
+        else:
+            c = np.asarray(covariances.detach().cpu() if isinstance(covariances, torch.Tensor) else covariances,
+                           np.float64)
+            if c.ndim == 3:
+                if not np.allclose(c, np.einsum("kij,ij->kij", c, np.eye(d))):
+                    raise NotImplementedError("the fused mixture kernel takes diagonal covariances")
+                var = np.einsum("kii->ki", c)
+            elif c.ndim == 2 and c.shape == (d, d):
+                var = np.repeat(np.diag(c)[None], K, 0)
+            else:
+                var = np.broadcast_to(c, (K, d)).astype(np.float64)
+        w = np.ones(K) / K if weights is None else np.asarray(weights, np.float64)
+        lam = 1.0 / var
+        factor = {"textbook": 0.5, "mocat": 1.0}[det_convention]
+        logc = np.log(w) + factor * np.log(lam).sum(1) - 0.5 * d * np.log(2 * np.pi)
+        dev = mr.default_device()
+        self.dim, self.n_components = int(d), int(K)
+        self.means = torch.tensor(mu, dtype=torch.float32, device=dev).contiguous()
+        self.lam = torch.tensor(lam, dtype=torch.float32, device=dev).contiguous()
+        self.logc = torch.tensor(logc, dtype=torch.float32, device=dev).contiguous()
+        self.weights = w
+        self.variances = var
+
+    def evaluate(self, x, temperature: float = 1.0):
+        x = _dev_tensor(x)
+        batch = x.shape[:-1]
+        n = int(math.prod(batch)) if len(batch) else 1
+        pot = torch.empty(batch, dtype=torch.float32, device=x.device)
+        grad = torch.empty_like(x)
+        _lib.call("mocat_gmm_potential_grad", _lib.ptr(x), n, self.dim, self.n_components, _lib.ptr(self.means),
+                  _lib.ptr(self.lam), _lib.ptr(self.logc), _lib.ptr(pot), _lib.ptr(grad), _lib.stream_handle())
+        zeros = torch.zeros_like(pot)
+        return zeros, pot, torch.zeros_like(grad), grad
+
+    def prior_potential(self, x, random_key=None):
+        return torch.zeros(x.shape[:-1], dtype=torch.float32, device=x.device)
+
+    def likelihood_potential(self, x, random_key=None):
+        return self.evaluate(x)[1]
+
+    def prior_potential_and_grad(self, x, random_key=None):
+        return self.prior_potential(x), torch.zeros_like(x)
+
+    def likelihood_potential_and_grad(self, x, random_key=None):
+        _, lp, _, gl = self.evaluate(x)
+        return lp, gl
+
+    def potential_and_grad(self, x, random_key=None):
+        _, lp, _, gl = self.evaluate(x)
+        return self.temperature * lp, self.temperature * gl

@@ -247,3 +247,108 @@ class RMMetropolisedSMCSampler(MetropolisedSMCSampler):
                          ess_threshold_retain, ess_threshold_resample, bisection_tol, max_bisection_iter,
                          resampling, **kwargs)
         self.parameters.rm_stepsize = rm_stepsize
+
+
+_lib.register("mocat_svgd_step", [_P, _P, _lib.c_i64, _lib.c_i64, _lib.c_f32, _lib.c_i32, _P, _lib.c_f32,
+                                  _lib.c_f32, _P, _P, _P, _P, _P])
+
+
+class SVGD(TransportSampler):
+    """Stein variational gradient descent (mocat/src/transport/svgd.py:42-146), full ensemble batch,
+    Gaussian RBF kernel, adagrad(step_size, momentum=0.9) as mocat's default optimiser.
+
+    bandwidth_update: "mean" (mocat's default when no kernel is given, svgd.py:59-61), "median"
+    (kernels.median_bandwidth_update), or None (fixed kernel bandwidth).
+    j_splits: workgroups sharing the pairwise sum of one particle tile (partial sums combined in the
+    update kernel).
+    """
+    name = "SVGD"
+
+    def __init__(self, stepsize, max_iter: int = 1000, kernel=None, kernel_params: cdict = None,
+                 ensemble_batchsize: int = None, optimiser: str = "adagrad", momentum: float = 0.9,
+                 bandwidth_update: str = "default", j_splits: int = 4, **kwargs):
+        super().__init__(max_iter=max_iter, **kwargs)
+        from .kernels import Gaussian
+        if bandwidth_update == "default":
+            bandwidth_update = "mean" if kernel is None else None
+        if kernel is None:
+            kernel = Gaussian()
+        if not isinstance(kernel, Gaussian):
+            raise NotImplementedError("the fused SVGD kernel implements the Gaussian RBF kernel")
+        if optimiser != "adagrad":
+            raise NotImplementedError("the fused SVGD update implements adagrad (mocat's default optimiser)")
+        if ensemble_batchsize is not None:
+            raise NotImplementedError("the fused SVGD kernel uses the full ensemble (ensemble_batchsize=None)")
+        self.kernel = kernel
+        self.parameters.stepsize = stepsize
+        self.parameters.kernel_params = kernel_params if kernel_params is not None else kernel.parameters
+        self.parameters.momentum = momentum
+        self.bandwidth_update = bandwidth_update
+        self.j_splits = int(j_splits)
+
+    def _adapt(self, x):
+        from .kernels import mean_bandwidth_update, median_bandwidth_update
+        if self.bandwidth_update == "median":
+            return float(median_bandwidth_update(x).item())
+        if self.bandwidth_update == "mean":
+            return float(mean_bandwidth_update(x).item())
+        return float(self.parameters.kernel_params.bandwidth)
+
+    def step(self, state: cdict, extra: cdict, scenario, workspace=None):
+        """One SVGD.update (svgd.py:122-146) in place on the device state (value, grad_potential,
+        potential, adagrad g_sq / m); returns phi."""
+        x, g = state.value, state.grad_potential
+        n, d = x.shape
+        extra.iter += 1
+        ks = mr.split(extra.random_key, n + 2)
+        extra.random_key = ks[-2].contiguous()
+        step = self.parameters.stepsize
+        step = float(step(extra.iter)) if callable(step) else float(step)
+        if workspace is None:
+            from .kernels import _workspace
+            workspace = _workspace(n, d, self.j_splits, x.device)
+        x_in = x.clone()
+        phi = torch.empty_like(x)
+        _lib.call("mocat_svgd_step", _lib.ptr(x_in), _lib.ptr(g), n, d, float(extra.bandwidth), self.j_splits,
+                  _lib.ptr(workspace), step, float(self.parameters.momentum), _lib.ptr(x), _lib.ptr(extra.g_sq),
+                  _lib.ptr(extra.m), _lib.ptr(phi), _lib.stream_handle())
+        T = float(scenario.temperature)
+        _, lp, _, gl = scenario.evaluate(x, T)
+        state.potential = T * lp
+        state.grad_potential = (T * gl).contiguous()
+        extra.bandwidth = self._adapt(x)
+        return phi
+
+    def run_fused(self, scenario, n, initial_state, initial_extra, n_chains=None, stack=(), **kwargs):
+        if not hasattr(scenario, "evaluate"):
+            raise NotImplementedError(f"{scenario.__class__.__name__} has no fused HIP kernel")
+        summary = self.summary(scenario, initial_state, initial_extra)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        state, key = self.startup_particles(scenario, n, initial_state, initial_extra.random_key)
+        x = state.value.to(torch.float32).contiguous()
+        key = mr.split(key, n + 1)[-1].contiguous()
+        T = float(scenario.temperature)
+        _, lp, _, gl = scenario.evaluate(x, T)
+        state = cdict(value=x, potential=T * lp, grad_potential=(T * gl).contiguous())
+        extra = cdict(random_key=key, iter=0, g_sq=torch.zeros_like(x), m=torch.zeros_like(x))
+        extra.bandwidth = float(self.parameters.kernel_params.bandwidth)
+        extra.bandwidth = self._adapt(x)
+        from .kernels import _workspace
+        ws = _workspace(n, x.shape[1], self.j_splits, x.device)
+        stacks = {k: [getattr(state, k).clone()] for k in stack}
+        bws = [extra.bandwidth]
+        for _ in range(self.max_iter):
+            self.step(state, extra, scenario, ws)
+            bws.append(extra.bandwidth)
+            for k in stack:
+                stacks[k].append(getattr(state, k).clone())
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - t0
+        chain = cdict()
+        for k in stack:
+            chain.__dict__[k] = torch.stack(stacks[k])
+        chain.bandwidth = torch.tensor(bws, dtype=torch.float32, device=x.device)
+        chain.final_state = state
+        chain.final_extra = extra
+        return chain, elapsed, summary

@@ -1,0 +1,103 @@
+"""GPU SVGD pieces vs the FP64 oracle: GMM potential/adjoint, pairwise distances, exact median,
+one Stein step with adagrad, and an end-to-end run on a Gaussian target (mocat's SVGD test pattern)."""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import svgd as osv
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def m():
+    import mocat_amd
+    return mocat_amd
+
+
+def _gmm_case(seed, K=4, d=100):
+    rs = np.random.default_rng(seed)
+    return rs.normal(0, 3, (K, d)), rs.uniform(0.5, 2, (K, d)), rs.dirichlet(np.ones(K))
+
+
+@pytest.mark.parametrize("conv", ["textbook", "mocat"])
+def test_gmm_potential_grad(m, conv):
+    mu, var, w = _gmm_case(0)
+    sc = m.GaussianMixture(mu, var, w, det_convention=conv)
+    x = np.random.default_rng(1).normal(0, 3, (300, 100)).astype(np.float32)
+    _, pot, _, grad = sc.evaluate(torch.tensor(x, device="cuda"))
+    opot, ograd = osv.gmm_potential_grad(x, mu, var, w, conv)
+    np.testing.assert_allclose(pot.cpu().numpy(), opot, rtol=2e-5, atol=1e-3)
+    np.testing.assert_allclose(grad.cpu().numpy(), ograd, rtol=1e-4, atol=1e-3)
+
+
+def test_gmm_reference_kats_on_gpu(m):
+    sc = m.GaussianMixture(np.arange(15.0).reshape(5, 3))
+    x = torch.tensor([[0.0, 0, 0], [2.0, 2, 2]], device="cuda")
+    pot, grad = sc.potential_and_grad(x)
+    np.testing.assert_allclose(pot.cpu().numpy(), [6.8662534, 6.8552055], rtol=1e-6)
+    np.testing.assert_allclose(grad.cpu().numpy(), [[0.0, -1.0, -2.0], [1.967039, 0.967039, -0.032961]], atol=2e-6)
+
+
+@pytest.mark.parametrize("n,d", [(4096, 100), (300, 7), (33, 64), (1, 5)])
+def test_pairwise_sqdist_vs_fp64(m, n, d):
+    x = np.random.default_rng(n).normal(0, 3, (n, d)).astype(np.float32) + 5.0
+    d2 = m.kernels.pairwise_sqdist(torch.tensor(x, device="cuda")).cpu().numpy().astype(np.float64)
+    if n <= 300:
+        ref = osv.distance_matrix(x) ** 2
+    else:
+        xs = x.astype(np.float64)
+        sq = (xs * xs).sum(1)
+        ref = np.maximum(sq[:, None] + sq[None, :] - 2 * xs @ xs.T, 0)
+        np.fill_diagonal(ref, 0)
+    assert np.all(np.diag(d2) == 0)
+    scale = ref.max() + 1e-30
+    assert np.max(np.abs(d2 - ref)) / scale < 2e-5
+
+
+@pytest.mark.parametrize("m_", [1, 2, 1000, 1 << 20])
+def test_select_exact(m, m_):
+    v = np.random.default_rng(m_).exponential(size=m_).astype(np.float32)
+    v[: m_ // 10] = 0.0
+    r0, r1 = (m_ - 1) // 2, m_ // 2
+    got = m.kernels.select_pair(torch.tensor(v, device="cuda"), r0, r1).cpu().numpy()
+    s = np.sort(v)
+    assert got[0] == s[r0] and got[1] == s[r1]
+
+
+def test_median_and_mean_bandwidth(m):
+    x = np.random.default_rng(3).normal(0, 2, (700, 30)).astype(np.float32)
+    xt = torch.tensor(x, device="cuda")
+    np.testing.assert_allclose(m.kernels.median_bandwidth_update(xt).item(), osv.median_bandwidth(x), rtol=2e-5)
+    np.testing.assert_allclose(m.kernels.mean_bandwidth_update(xt).item(), osv.mean_bandwidth(x), rtol=2e-5)
+
+
+@pytest.mark.parametrize("n,d", [(4096, 100), (500, 10)])
+def test_svgd_step_vs_fp64(m, n, d):
+    rs = np.random.default_rng(d)
+    x = rs.normal(0, 2, (n, d)).astype(np.float32)
+    g = rs.normal(0, 1, (n, d)).astype(np.float32)
+    h = float(osv.median_bandwidth(x)) if n <= 500 else 3.0
+    smp = m.SVGD(stepsize=0.01, bandwidth_update=None, j_splits=4)
+    st = m.cdict(value=torch.tensor(x, device="cuda"), grad_potential=torch.tensor(g, device="cuda"))
+    ex = m.cdict(random_key=m.random.PRNGKey(0), iter=0, bandwidth=h, g_sq=torch.zeros((n, d), device="cuda"),
+                 m=torch.zeros((n, d), device="cuda"))
+    sc = m.DiagonalGaussian(np.zeros(d), np.ones(d), prior_variance=math.inf)
+    phi = smp.step(st, ex, sc).cpu().numpy().astype(np.float64)
+    ophi = osv.phi_gemm(x, g, h)
+    err = np.max(np.abs(phi - ophi)) / np.max(np.abs(ophi))
+    assert err < 1e-4, err
+    ox, gsq, mm = osv.adagrad(x.astype(np.float64), np.zeros((n, d)), np.zeros((n, d)), -ophi, 0.01)
+    np.testing.assert_allclose(st.value.cpu().numpy(), ox, atol=1e-4)
+
+
+def test_svgd_run_gaussian_moments(m):
+    # mocat/src/tests/test_transport.py:65-89 pattern: moments of the final ensemble
+    sc = m.DiagonalGaussian(np.zeros(2), np.array([1.0, 2.0]), prior_variance=math.inf)
+    for upd in ("median", "mean"):
+        chain = m.run(sc, m.SVGD(stepsize=1.0, max_iter=500, bandwidth_update=upd), 256, m.random.PRNGKey(0))
+        x = chain.final_state.value.cpu().numpy()
+        np.testing.assert_allclose(x.mean(0), [0, 0], atol=0.1)
+        np.testing.assert_allclose(np.var(x, 0), [1.0, 2.0], rtol=0.2)

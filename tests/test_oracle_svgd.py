@@ -1,0 +1,55 @@
+"""Pins the SVGD/GMM oracle with mocat's own known answers and a brute-force phi."""
+import numpy as np
+
+from oracle import svgd
+
+
+def test_gaussian_kernel_reference_kat():
+    # mocat/src/tests/test_kernels.py:19-21 (bandwidth 1)
+    np.testing.assert_allclose(svgd.gaussian_kernel(np.zeros(5), np.zeros(5), 1.0), 1.0)
+    np.testing.assert_allclose(svgd.gaussian_kernel(np.zeros(5), np.ones(5), 1.0), 0.082085006, rtol=1e-7)
+
+
+def test_gmm_reference_kats():
+    # 1-D: mocat/src/tests/test_toy_scenarios.py:59-75 (means [0, 1], identity covariances)
+    pot, grad = svgd.gmm_potential_grad(np.array([[5.0]]), [[0.0], [1.0]], np.ones((2, 1)), [0.5, 0.5], "mocat")
+    np.testing.assert_allclose(pot, [9.601038], rtol=1e-6)
+    np.testing.assert_allclose(grad[0], [4.010987], rtol=1e-6)
+    # N-D: :86-106 (means arange(15).reshape(5, 3))
+    mu = np.arange(15.0).reshape(5, 3)
+    for conv in ("mocat", "textbook"):  # identity covariances: both conventions coincide
+        pot, grad = svgd.gmm_potential_grad(np.array([[0.0, 0, 0], [2.0, 2, 2]]), mu, np.ones((5, 3)),
+                                            np.ones(5) / 5, conv)
+        np.testing.assert_allclose(pot, [6.8662534, 6.8552055], rtol=1e-6)
+        np.testing.assert_allclose(grad[0], [0.0, -1.0, -2.0], atol=1e-6)
+        np.testing.assert_allclose(grad[1], [1.967039, 0.967039, -0.032961], atol=2e-6)
+
+
+def test_gmm_textbook_is_normalised_mixture_logpdf():
+    from scipy.stats import multivariate_normal
+    rs = np.random.default_rng(0)
+    mu = rs.normal(0, 3, (3, 4))
+    var = rs.uniform(0.5, 2, (3, 4))
+    w = rs.dirichlet(np.ones(3))
+    x = rs.normal(0, 3, (6, 4))
+    dens = sum(w[k] * multivariate_normal(mu[k], np.diag(var[k])).pdf(x) for k in range(3))
+    pot, _ = svgd.gmm_potential_grad(x, mu, var, w, "textbook")
+    np.testing.assert_allclose(pot, -np.log(dens), rtol=1e-10)
+
+
+def test_phi_gemm_form_equals_pairwise_sum():
+    rs = np.random.default_rng(1)
+    x = rs.normal(size=(40, 5))
+    g = rs.normal(size=(40, 5))
+    np.testing.assert_allclose(svgd.phi_gemm(x, g, 0.7), svgd.phi_pairwise(x, g, 0.7), rtol=1e-10, atol=1e-12)
+
+
+def test_gmm_grad_central_differences():
+    rs = np.random.default_rng(2)
+    mu, var, w = rs.normal(0, 2, (4, 3)), rs.uniform(0.5, 2, (4, 3)), rs.dirichlet(np.ones(4))
+    x = rs.normal(size=(1, 3))
+    _, g = svgd.gmm_potential_grad(x, mu, var, w)
+    eps = 1e-6
+    fd = [(svgd.gmm_potential_grad(x + eps * e, mu, var, w)[0] - svgd.gmm_potential_grad(x - eps * e, mu, var, w)[0])
+          / (2 * eps) for e in np.eye(3)]
+    np.testing.assert_allclose(g[0], np.array(fd)[:, 0], rtol=1e-6, atol=1e-8)
