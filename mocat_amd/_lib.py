@@ -40,16 +40,26 @@ class MocatKernelError(RuntimeError):
     pass
 
 
+_BOUND = {}
+
+
 def register(name: str, argtypes):
     SIGNATURES[name] = argtypes
-    if _LIB is not None:
-        _bind(_LIB, name)
+    _BOUND.pop(name, None)
 
 
 def _bind(lib, name):
-    fn = getattr(lib, name)
-    fn.argtypes = SIGNATURES[name]
-    fn.restype = ctypes.c_int
+    """Resolve and type one entry point (on first use): a missing symbol fails that call only."""
+    fn = _BOUND.get(name)
+    if fn is None:
+        try:
+            fn = getattr(lib, name)
+        except AttributeError as e:
+            raise MocatKernelError(f"{name} is not exported by {LIB_PATH}: rebuild the library") from e
+        fn.argtypes = SIGNATURES[name]
+        fn.restype = ctypes.c_int
+        _BOUND[name] = fn
+    return fn
 
 
 def load(path: os.PathLike | None = None):
@@ -63,9 +73,7 @@ def load(path: os.PathLike | None = None):
             f"{p} not found: build the HIP kernels first (python -m mocat_amd.csrc.build). "
             "mocat_amd has no CPU fallback.")
     lib = ctypes.CDLL(str(p), mode=ctypes.RTLD_GLOBAL)
-    for name in SIGNATURES:
-        _bind(lib, name)
-    v = lib.mocat_abi_version()
+    v = _bind(lib, "mocat_abi_version")()
     if v != ABI_VERSION:
         raise ImportError(f"libmocat_amd ABI version {v} != expected {ABI_VERSION}; rebuild the library")
     _LIB = lib
@@ -95,7 +103,7 @@ def ptr(t: torch.Tensor | None) -> ctypes.c_void_p:
 def call(name: str, *args):
     """Call an entry point (stream appended by the caller); raise on a non-zero status."""
     lib = load()
-    rc = getattr(lib, name)(*args)
+    rc = _bind(lib, name)(*args)
     if rc != 0:
         raise MocatKernelError(f"{name} failed with status {rc} ({STATUS.get(rc, 'unknown')})")
     return rc
