@@ -81,6 +81,88 @@ int mocat_hmc_lr_iterate(const float* X, const float* y, const void* packed, int
                          float* trace_value, float* trace_potential, float* trace_alpha, float* trace_stepsize,
                          float* trace_momenta, hipStream_t stream);
 
+/* ------------------------------------------------------------------------- particle filters
+ * run_particle_filter_for_marginals with a BootstrapFilter (mocat/src/ssm/filtering.py:255-324,
+ * 142-170) for many independent filters of uneven n and T in one launch (one workgroup per filter).
+ * model 0: TimeHomogenousLinearGaussian (linear_gaussian.py:46-51,86-94,118-128), params
+ *   [m0(d), L0(dxd), A(dxd), LQ(dxd), H(dyxd), SR(dyxdy), nlz] with SR = inv(chol(R)) and
+ *   nlz = (dy log 2pi - log det R^-1)/2 as mocat's reset_covariance (mocat/src/utils.py:247-262);
+ * model 1: 1-D StochasticVolatility (stochastic_volatility.py:61-108), params
+ *   [m0, sqrt(P0), phi, sigma, 1/beta], unit time steps.
+ * Key tree per filter: (key, sub) = split(key); particles from split(sub, n); step t uses
+ * split(key, T-1)[t-1] for resampling and split(., n) for the transitions (filtering.py:264-303).
+ * resampling: 0 systematic, 1 multinomial, 2 categorical (mocat's Gumbel-max, n*n < 2^32 - 1);
+ * resample when ESS_{t-1} < ess_threshold * n. Outputs (optional unless noted): trajectories at
+ * out_offset ([T][n][d], [T][n], ancestors [T][n]), ess / cumulative log-likelihood at t_offset [T],
+ * out_loglik[n_filters] (required, FP64). Workspaces are sized by the sum of n over filters. */
+typedef struct {
+  int64_t n;             /* particles */
+  int64_t T;             /* observations y_0..y_{T-1} */
+  int64_t y_offset;      /* floats into y */
+  int64_t state_offset;  /* particle slots into the workspaces */
+  int64_t out_offset;    /* particle-steps into the trajectory outputs, or -1 */
+  int64_t t_offset;      /* steps into out_ess / out_loglik_t */
+  uint32_t key[2];       /* random_key of the filter */
+  int32_t param_offset;  /* floats into params */
+  int32_t reserved;
+} mocat_pf_filter;
+
+int mocat_pf_param_count(int model, int d, int dy, int64_t* count);
+int mocat_pf_run(int model, int d, int dy, const mocat_pf_filter* filters, int n_filters, const float* params,
+                 const float* y, float* x_ws0, float* x_ws1, float* lw_ws0, float* lw_ws1, uint64_t* csum_ws,
+                 int32_t* anc_ws, int resampling, float ess_threshold, float* out_value, float* out_lw,
+                 int32_t* out_anc, float* out_ess, double* out_loglik_t, double* out_loglik, hipStream_t stream);
+
+/* Ancestor indices idx[n] from non-negative weights (from_log = 0) or log-weights (from_log = 1,
+ * weights exp(w - max w)): method 0 systematic (u = uniform(key)), 1 multinomial
+ * (u = uniform(key, (n,))); exact fixed-point cumulative weights. key is a device uint32[2];
+ * workspace holds n uint64. New resamplers beside mocat's _resample (filtering.py:196-199). */
+int mocat_resample(const float* weights, int64_t n, int method, int from_log, const uint32_t* key, void* workspace,
+                   int32_t* idx, hipStream_t stream);
+/* random.categorical(key, log_weight, shape=(n_out,)) by Gumbel-max: mocat's _resample
+ * (mocat/src/ssm/filtering.py:196-199) and SMCSampler.resample (mocat/src/transport/smc.py:61-71). */
+int mocat_resample_categorical(const float* log_weight, int64_t n, int64_t n_out, const uint32_t* key, int32_t* idx,
+                               hipStream_t stream);
+
+/* -------------------------------------------------------------------------------- tempered SMC
+ * dst[j][:] = src[idx[j]][:] for 4-byte elements, width per row: ensemble_state[resampled_indices]
+ * (mocat/src/transport/smc.py:68). */
+int mocat_gather_rows(const void* src, const int32_t* idx, int64_t n_out, int64_t width, void* dst,
+                      hipStream_t stream);
+/* One adapt step of (RM)MetropolisedSMCSampler in FP64 (mocat/src/transport/smc.py:193-217,
+ * 303-326, 406-421; regula falsi of mocat/src/utils.py:205-237): next temperature (adaptive = 1,
+ * ESS target ess * retain) or next_beta; log_weight_out = log_weight - (b' - beta) lik; out[8] =
+ * {b', ESS', lse(new), lse(old), new log stepsize, weighted mean alpha, iterations, min |f|}.
+ * alpha: [k_alpha][n] acceptance probabilities of the moves (NULL: no RM update). */
+int mocat_smc_temper(const float* log_weight, const float* lik, int64_t n, double beta, double ess, double retain,
+                     double max_temperature, double tol, int max_iter, int adaptive, double next_beta,
+                     const float* alpha, int k_alpha, double log_stepsize, double rm_stepsize, double target,
+                     float* log_weight_out, double* out, hipStream_t stream);
+
+/* ------------------------------------------------------------------------------------- SVGD
+ * One SVGD.update with the Gaussian kernel and adagrad(step, momentum)
+ * (mocat/src/transport/svgd.py:18-32,122-146; kernels.py:89-102): x (in/out), gsq and mom
+ * (adagrad state, in/out) [n][d]; x_in and grad are the positions and potential gradients the Stein
+ * direction is computed from; phi_out optional. d <= 128. workspace: mocat_svgd_workspace_bytes. */
+int mocat_svgd_workspace_bytes(int64_t n, int64_t d, int js_count, int64_t* bytes);
+int mocat_svgd_step(const float* x_in, const float* grad, int64_t n, int64_t d, float bandwidth, int js_count,
+                    void* workspace, float step, float momentum, float* x, float* gsq, float* mom, float* phi_out,
+                    hipStream_t stream);
+/* Diagonal Gaussian mixture: potential[n] = -logsumexp_k(logc_k - 0.5 sum_j lam_kj (x_j - mu_kj)^2),
+ * grad[n][d] (GaussianMixture, mocat/src/scenarios/toy_examples.py:54-116). K <= 8. */
+int mocat_gmm_potential_grad(const float* x, int64_t n, int64_t d, int K, const float* means, const float* lam,
+                             const float* logc, float* potential, float* grad, hipStream_t stream);
+/* d2[n][n] = |x_i - x_j|^2 (diagonal 0), the distance matrix of l2_distance_matrix
+ * (mocat/src/utils.py:437-439) squared. workspace: mocat_svgd_workspace_bytes(n, d, 1). */
+int mocat_pairwise_sqdist(const float* x, int64_t n, int64_t d, void* workspace, float* d2, hipStream_t stream);
+/* Exact rank0-th and rank1-th smallest of m non-negative floats (radix select): the median of
+ * median_bandwidth_update (mocat/src/kernels.py:220-224). Results (bit patterns) in workspace[0..1];
+ * workspace: 516 uint32. */
+int mocat_select2(const float* v, int64_t m, int64_t rank0, int64_t rank1, uint32_t* workspace, hipStream_t stream);
+/* partial[b] = sum of sqrt(d2) over a grid-stride share of the m entries (FP64), for
+ * mean_bandwidth_update (mocat/src/kernels.py:227-229). */
+int mocat_sqrt_mean(const float* d2, int64_t m, double* partial, int n_partial, hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif
