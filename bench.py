@@ -41,7 +41,20 @@ def parse():
     ap.add_argument("--cpu-chains", type=int, default=64)
     ap.add_argument("--cpu-iters", type=int, default=120)
     ap.add_argument("--plain-steps", type=int, default=30, help="iterations timed on the unfused FP32 path (0: skip)")
+    ap.add_argument("--mode", default="hmc", choices=["hmc", "copy"],
+                    help="copy: a plain 256 MiB device copy, the calibration run for HBM byte counters")
     return ap.parse_args()
+
+
+def copy_calibration():
+    """Five device-to-device copies of 256 MiB (read 256 MiB + write 256 MiB each)."""
+    import torch
+    src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"mode": "copy", "bytes_read_per_copy": src.numel() * 4, "bytes_written_per_copy": dst.numel() * 4}))
 
 
 def clone(c):
@@ -140,11 +153,14 @@ def main():
         sys.exit(2)
     import numpy as np
     import torch
-    import mocat_amd as m
-    from mocat_amd import datasets
     if not torch.cuda.is_available():
         print("bench.py needs a ROCm GPU", file=sys.stderr)
         sys.exit(1)
+    if a.mode == "copy":
+        copy_calibration()
+        return
+    import mocat_amd as m
+    from mocat_amd import datasets
 
     X, y, _ = datasets.logistic_regression(0, N_ROWS, DIM)
     sc = m.LogisticRegression(X, y, prior_variance=10.0, precision=a.precision)

@@ -206,9 +206,11 @@ __global__ __launch_bounds__(NT) void svgd_phi_kernel(const char* __restrict__ p
       nj[4 * q + 3] = v.w;
     }
     bf16x8 kh[2], kl[2];
+    const int jbase = rb * 32;
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
-      const float D = fmaxf(ni + nj[r] - 2.f * z[r], 0.f);
+      // the self-distance is exactly 0 (the norms-minus-Gram form leaves cancellation residue there)
+      const float D = (jbase + acc_row(r, lane) == i0 + col) ? 0.f : fmaxf(ni + nj[r] - 2.f * z[r], 0.f);
       const float kv = nj[r] >= 0.f ? __expf(-D * inv2h2) : 0.f;
       spart += kv;
       __bf16 h, l;

@@ -136,14 +136,15 @@ class DiagonalGaussian(LogisticRegression):
         self.prior_precision = 0.0 if math.isinf(self.prior_variance) else 1.0 / self.prior_variance
         self.precision = "fp32"
         self.packed = None
+        if math.isinf(self.prior_variance):
+            # flat prior: no prior_sample (samplers then start from N(0, I), as mocat does)
+            self.prior_sample = Scenario.prior_sample.__get__(self)
 
     def kernel_args(self):
         return (_lib.ptr(self.mean_t), _lib.ptr(self.lam), _lib.ptr(None), 1, self.dim, ENGINE_GAUSSIAN,
                 float(self.prior_precision))
 
     def prior_sample(self, random_key):
-        if math.isinf(self.prior_variance):
-            raise AttributeError("flat prior: no prior_sample")
         return math.sqrt(self.prior_variance) * mr.normal(random_key, (self.dim,))
 
 
