@@ -49,6 +49,20 @@ MOCAT_DEV void lr_row(float z, float y, float& r, float& c) {
   c = valid ? fmaxf(z, 0.f) + __logf(ope) - y * z : 0.f;
 }
 
+// MFMA-engine rows, one form for both signs of z: t = exp(-z), sigmoid = 1 / (1 + t) (t = inf gives
+// 0), softplus = max(z + log(1 + t), 0) with t clamped below overflow. Padding rows hold x = 0,
+// y = 0.5, so r = 0 there; their potential terms are masked by the caller.
+MOCAT_DEV float lr_row_grad(float z, float y) {
+  const float t = __expf(-z);
+  return __builtin_amdgcn_rcpf(1.f + t) - y;
+}
+MOCAT_DEV float lr_row_full(float z, float y, float& c) {
+  const float t = __expf(fminf(-z, 87.f));
+  const float ope = 1.f + t;
+  c = fmaxf(z + __logf(ope), 0.f) - y * z;
+  return __builtin_amdgcn_rcpf(ope) - y;
+}
+
 // ------------------------------------------------------------------------------------------
 // FP32 engine.  LDS: X chunk [RC][DP+4], R chunk [CT][RC+4].
 template <int DP, int NT>
@@ -65,7 +79,7 @@ struct Fp32Engine {
 
   // theta: LDS [CT][DP] (zero beyond d). Outputs: lik[CT], glik[CT][DP] in LDS.
   static MOCAT_DEV void eval(Smem& s, const float* __restrict__ X, const float* __restrict__ y, const void*,
-                             int n, int d, const float* theta, float* lik, float* glik) {
+                             int n, int d, const float* theta, float* lik, float* glik, bool /*want_pot*/) {
     const int t = threadIdx.x;
     const int lane = t & (kWave - 1), w = t / kWave;
     constexpr int NW = NT / kWave;
@@ -149,7 +163,7 @@ struct Fp32Engine {
 // Packed data per block of 32 rows (rb):
 //   A layout [2 parts][KB][64 lanes][8]   lane l: row l%32, columns kb*16 + (l/32)*8 + s
 //   B layout [2 parts][2][NB][64][8]      lane l: column nb*32 + l%32, rows acc_row(kb2*8+s, l)
-//   y layout [2][16] f32                  g = l/32: y[acc_row(r, l)], -1 on padding rows
+//   y layout [2][16] f32                  g = l/32: y[acc_row(r, l)], 0.5 on padding rows (x = 0 there)
 template <int DP>
 struct PackLayout {
   static constexpr int KB = DP / 16;
@@ -177,8 +191,9 @@ struct Bf16Engine {
     float ured[NW][2][CT];
   };
 
+  // want_pot = false: gradient only (the leapfrog needs the potential only after its last step)
   static MOCAT_DEV void eval(Smem& s, const float*, const float*, const void* packed, int n, int d,
-                             const float* theta, float* lik, float* glik) {
+                             const float* theta, float* lik, float* glik, bool want_pot) {
     const int t = threadIdx.x;
     const int lane = t & (kWave - 1), w = t / kWave, g = lane >> 5, col = lane & 31;
     const int nrb = (n + 31) / 32;
@@ -257,15 +272,26 @@ struct Bf16Engine {
         }
       if (kPrefetch && rb + NW < nrb) load_a(rb + NW);
       bf16x8 rh[2], rl[2];
+      if (want_pot) {
 #pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        float rr, cc;
-        lr_row(z[r], yv[r], rr, cc);
-        upart += cc;
-        __bf16 h, l;
-        split_bf16(rr, h, l);
-        rh[r >> 3][r & 7] = h;
-        rl[r >> 3][r & 7] = l;
+        for (int r = 0; r < 16; ++r) {
+          float cc;
+          const float rr = lr_row_full(z[r], yv[r], cc);
+          upart += (rb * 32 + acc_row(r, lane) < n) ? cc : 0.f;
+          __bf16 h, l;
+          split_bf16(rr, h, l);
+          rh[r >> 3][r & 7] = h;
+          rl[r >> 3][r & 7] = l;
+        }
+      } else {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const float rr = lr_row_grad(z[r], yv[r]);
+          __bf16 h, l;
+          split_bf16(rr, h, l);
+          rh[r >> 3][r & 7] = h;
+          rl[r >> 3][r & 7] = l;
+        }
       }
 #pragma unroll
       for (int kb2 = 0; kb2 < 2; ++kb2) {
@@ -346,7 +372,7 @@ __global__ void lr_pack_kernel(const float* __restrict__ X, const float* __restr
       const int qq = q - L::BLOCK_BF16;  // 0..31 = g*16 + r
       const int gg = qq >> 4, r = qq & 15;
       const int row = rb * 32 + 8 * (r >> 2) + 4 * gg + (r & 3);
-      reinterpret_cast<float*>(blk + L::BLOCK_BF16 * 2)[qq] = row < n ? y[row] : -1.f;
+      reinterpret_cast<float*>(blk + L::BLOCK_BF16 * 2)[qq] = row < n ? y[row] : 0.5f;
     }
   }
 }
@@ -361,7 +387,7 @@ struct GaussEngine {
     float part[CT * DP];
   };
   static MOCAT_DEV void eval(Smem& s, const float* __restrict__ mu, const float* __restrict__ lam, const void*, int,
-                             int d, const float* theta, float* lik, float* glik) {
+                             int d, const float* theta, float* lik, float* glik, bool /*want_pot*/) {
     const int t = threadIdx.x;
     for (int e = t; e < CT * DP; e += NT) {
       const int k = e % DP;
@@ -431,8 +457,8 @@ struct LrArgs {
 
 // Evaluates prior/likelihood potentials and gradients of the tile's theta (in LDS).
 template <class Engine, int DP, int NT>
-MOCAT_DEV void tile_eval(typename Engine::Smem& es, TileState<DP>& ts, const LrArgs& a) {
-  Engine::eval(es, a.X, a.y, a.packed, a.n, a.d, ts.theta, ts.lik, ts.glik);
+MOCAT_DEV void tile_eval(typename Engine::Smem& es, TileState<DP>& ts, const LrArgs& a, bool want_pot = true) {
+  Engine::eval(es, a.X, a.y, a.packed, a.n, a.d, ts.theta, ts.lik, ts.glik, want_pot);
   const int t = threadIdx.x;
   for (int e = t; e < CT * DP; e += NT) ts.grad[e] = a.prior_prec * ts.theta[e] + a.temperature * ts.glik[e];
   if (t < CT) {
@@ -588,7 +614,7 @@ __global__ __launch_bounds__(NT) void hmc_lr_kernel(HmcArgs h) {
         ts.theta[e] = ((e - c * DP) < d) ? ts.theta[e] + hh * ph : 0.f;
       }
       __syncthreads();
-      tile_eval<Engine, DP, NT>(es, ts, h.lr);
+      tile_eval<Engine, DP, NT>(es, ts, h.lr, s == h.leapfrog_steps - 1);
       for (int e = t; e < CT * DP; e += NT) {
         const int c = e / DP;
         ts.mom[e] = ts.mom[e] - (ts.step[c] / 2.f) * ts.grad[e];
