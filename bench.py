@@ -41,8 +41,9 @@ def parse():
     ap.add_argument("--cpu-chains", type=int, default=64)
     ap.add_argument("--cpu-iters", type=int, default=120)
     ap.add_argument("--plain-steps", type=int, default=30, help="iterations timed on the unfused FP32 path (0: skip)")
-    ap.add_argument("--mode", default="hmc", choices=["hmc", "copy"],
-                    help="copy: a plain 256 MiB device copy, the calibration run for HBM byte counters")
+    ap.add_argument("--mode", default="hmc", choices=["hmc", "copy", "pf1", "pf5", "smc3", "svgd4"],
+                    help="hmc: the headline workload. copy: a plain 256 MiB device copy (calibration run for HBM "
+                         "byte counters). pf1 / pf5 / smc3 / svgd4: the other workloads, one JSON line each")
     return ap.parse_args()
 
 
@@ -145,6 +146,86 @@ def cpu_baseline(X, y, chains, iters, ess_per_chain_iter):
             "chain_iterations_per_s": rate}
 
 
+def _timed(fn):
+    import torch
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    out = fn()
+    torch.cuda.synchronize()
+    return out, time.perf_counter() - t0
+
+
+def bench_pf1():
+    """Linear-Gaussian SSM d = 4, T = 100, N = 4096, systematic resampling every step; also 256
+    independent filters in one launch. Unit: particle-steps/s."""
+    import mocat_amd as m
+    from mocat_amd import datasets
+    model, y, t = datasets.lgssm_config(0, 4, 100)
+    pf = m.ssm.BootstrapFilter()
+    run1 = lambda: m.ssm.run_particle_filter_for_marginals(model, pf, y, t, m.random.PRNGKey(0), n=4096,
+                                                           ess_threshold=1.0, resampling="systematic")
+    run1()
+    out, dt = _timed(run1)
+    keys = m.random.split(m.random.PRNGKey(1), 256)
+    yn = y.cpu().numpy()
+    runb = lambda: m.ssm.run_particle_filters([model] * 256, [yn] * 256, [k for k in keys], [4096] * 256, 1.0,
+                                              "systematic")
+    runb()
+    res, dtb = _timed(runb)
+    print(json.dumps({"mode": "pf1", "metric": "particle-steps/s", "single_filter": 4096 * 100 / dt,
+                      "single_filter_ms": dt * 1e3, "batch256": 256 * 4096 * 100 / dtb, "batch256_ms": dtb * 1e3,
+                      "log_likelihood": out.log_likelihood.item(), "store_trajectory": True}))
+
+
+def bench_pf5():
+    """256 stochastic-volatility filters, particles log-uniform 256..65536, lengths 50..1000, one launch."""
+    import mocat_amd as m
+    from mocat_amd import datasets
+    models, ys, ns = datasets.sv_batch_config(0)
+    keys = m.random.split(m.random.PRNGKey(0), len(models))
+    run = lambda: m.ssm.run_particle_filters(models, ys, [k for k in keys], ns, 1.0, "systematic")
+    run()
+    res, dt = _timed(run)
+    E = sum(n * len(y) for n, y in zip(ns, ys))
+    big = max(n * len(y) for n, y in zip(ns, ys))
+    print(json.dumps({"mode": "pf5", "metric": "particle-steps/s", "value": E / dt, "ms": dt * 1e3,
+                      "particle_steps": E, "largest_filter_particle_steps": big,
+                      "bytes_per_particle_step": 24, "achieved_GBps_algorithmic": 24 * E / dt / 1e9}))
+
+
+def bench_smc3(n_particles=65536):
+    """Tempered SMC on Bayesian logistic regression, X 20000 x 100, 5 MALA moves per temperature."""
+    import mocat_amd as m
+    from mocat_amd import datasets
+    X, y, _ = datasets.logistic_regression(0, 20000, 100)
+    sc = m.LogisticRegression(X, y, prior_variance=10.0, precision="bf16x3")
+    smp = m.RMMetropolisedSMCSampler(m.Underdamped(stepsize=0.02), mcmc_steps=5, ess_threshold_retain=0.5,
+                                     ess_threshold_resample=1.0, resampling="systematic")
+    chain, dt = _timed(lambda: m.run(sc, smp, n_particles, m.random.PRNGKey(0), stack=()))
+    n_temp = int(chain.temperature.shape[0]) - 1
+    evals = 6 * n_particles * n_temp
+    print(json.dumps({"mode": "smc3", "metric": "particle-moves/s", "value": 5 * n_particles * n_temp / dt,
+                      "seconds": dt, "temperatures": n_temp, "log_evidence": chain.log_norm_constant[-1].item(),
+                      "gradient_evals_per_s": evals / dt, "algorithmic_TFLOPs": evals * 4 * 20000 * 100 / dt / 1e12}))
+
+
+def bench_svgd4(n=4096, d=100, iters=500):
+    """SVGD, 4-component diagonal Gaussian mixture in d = 100, median bandwidth, adagrad 1e-2."""
+    import numpy as np
+    import mocat_amd as m
+    rs = np.random.default_rng(0)
+    means = rs.normal(0, 3, (4, d))
+    var = rs.uniform(0.5, 2.0, (4, d))
+    w = rs.dirichlet(np.ones(4))
+    sc = m.GaussianMixture(means, var, w)
+    smp = m.SVGD(stepsize=1e-2, max_iter=iters, bandwidth_update="median")
+    chain, dt = _timed(lambda: m.run(sc, smp, n, m.random.PRNGKey(0)))
+    flop = 6.0 * n * n * d * iters
+    print(json.dumps({"mode": "svgd4", "metric": "iterations/s", "value": iters / dt, "seconds": dt,
+                      "stein_algorithmic_TFLOPs_incl_bandwidth_and_potential": flop / dt / 1e12,
+                      "final_bandwidth": chain.bandwidth[-1].item()}))
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -158,6 +239,9 @@ def main():
         sys.exit(1)
     if a.mode == "copy":
         copy_calibration()
+        return
+    if a.mode != "hmc":
+        {"pf1": bench_pf1, "pf5": bench_pf5, "smc3": bench_smc3, "svgd4": bench_svgd4}[a.mode]()
         return
     import mocat_amd as m
     from mocat_amd import datasets
