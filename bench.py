@@ -188,7 +188,11 @@ def bench_pf5():
     res, dt = _timed(run)
     E = sum(n * len(y) for n, y in zip(ns, ys))
     big = max(n * len(y) for n, y in zip(ns, ys))
+    rung = lambda: m.ssm.run_particle_filters(models, ys, [k for k in keys], ns, 1.0, "systematic", grad=True)
+    rung()
+    _, dtg = _timed(rung)
     print(json.dumps({"mode": "pf5", "metric": "particle-steps/s", "value": E / dt, "ms": dt * 1e3,
+                      "with_gradient": E / dtg, "with_gradient_ms": dtg * 1e3,
                       "particle_steps": E, "largest_filter_particle_steps": big,
                       "bytes_per_particle_step": 24, "achieved_GBps_algorithmic": 24 * E / dt / 1e9}))
 

@@ -94,7 +94,9 @@ int mocat_hmc_lr_iterate(const float* X, const float* y, const void* packed, int
  * resampling: 0 systematic, 1 multinomial, 2 categorical (mocat's Gumbel-max, n*n < 2^32 - 1);
  * resample when ESS_{t-1} < ess_threshold * n. Outputs (optional unless noted): trajectories at
  * out_offset ([T][n][d], [T][n], ancestors [T][n]), ess / cumulative log-likelihood at t_offset [T],
- * out_loglik[n_filters] (required, FP64). Workspaces are sized by the sum of n over filters. */
+ * out_loglik[n_filters] (required, FP64). Workspaces are sized by the sum of n over filters.
+ * out_grad (model 1 only, else NULL): [n_filters][3] d log Z / d(phi, sigma, beta) of the
+ * fixed-resampling-index estimator by forward tangents (workspaces tan_ws0/1: [sum n][5] floats). */
 typedef struct {
   int64_t n;             /* particles */
   int64_t T;             /* observations y_0..y_{T-1} */
@@ -111,7 +113,8 @@ int mocat_pf_param_count(int model, int d, int dy, int64_t* count);
 int mocat_pf_run(int model, int d, int dy, const mocat_pf_filter* filters, int n_filters, const float* params,
                  const float* y, float* x_ws0, float* x_ws1, float* lw_ws0, float* lw_ws1, uint64_t* csum_ws,
                  int32_t* anc_ws, int resampling, float ess_threshold, float* out_value, float* out_lw,
-                 int32_t* out_anc, float* out_ess, double* out_loglik_t, double* out_loglik, hipStream_t stream);
+                 int32_t* out_anc, float* out_ess, double* out_loglik_t, double* out_loglik, float* tan_ws0,
+                 float* tan_ws1, double* out_grad, hipStream_t stream);
 
 /* Ancestor indices idx[n] from non-negative weights (from_log = 0) or log-weights (from_log = 1,
  * weights exp(w - max w)): method 0 systematic (u = uniform(key)), 1 multinomial

@@ -136,6 +136,15 @@ struct Sv {
     const float ys = y[0] * s;
     return 0.5f * ys * ys + 0.5f * (float)kLog2PiD - __logf(s);
   }
+  // d lik / dx = 0.5 - 0.5 (y s)^2 ; d lik / dbeta = (1 - (y s)^2) * inv_beta
+  MOCAT_DEV float lik_grad(const float* x, const float* y, float& dldx, float& dldb) const {
+    const float s = inv_beta / __expf(0.5f * x[0]);
+    const float ys = y[0] * s;
+    const float s2 = ys * ys;
+    dldx = 0.5f - 0.5f * s2;
+    dldb = (1.f - s2) * inv_beta;
+    return 0.5f * s2 + 0.5f * (float)kLog2PiD - __logf(s);
+  }
 };
 
 // ---------------------------------------------------------------- block helpers
@@ -176,10 +185,29 @@ MOCAT_DEV void lse_add(LseAcc& a, float v) {
   }
 }
 
+struct GradAcc {
+  float m;
+  double s1, g0, g1, g2;
+};
+
+MOCAT_DEV GradAcc grad_combine(GradAcc a, GradAcc b) {
+  GradAcc r;
+  r.m = fmaxf(a.m, b.m);
+  if (r.m == -INFINITY) return GradAcc{-INFINITY, 0.0, 0.0, 0.0, 0.0};
+  const double fa = a.m == -INFINITY ? 0.0 : exp((double)a.m - (double)r.m);
+  const double fb = b.m == -INFINITY ? 0.0 : exp((double)b.m - (double)r.m);
+  r.s1 = a.s1 * fa + b.s1 * fb;
+  r.g0 = a.g0 * fa + b.g0 * fb;
+  r.g1 = a.g1 * fa + b.g1 * fb;
+  r.g2 = a.g2 * fa + b.g2 * fb;
+  return r;
+}
+
 struct BlockScratch {
   float fm[NW];
   double d1[NW], d2[NW];
   uint64_t u64[NT];
+  double g0[NW], g1[NW], g2[NW];
   float bcast_f[4];
   double bcast_d[4];
   uint32_t bcast_u[4];
@@ -213,6 +241,52 @@ MOCAT_DEV void block_lse(LseAcc a, BlockScratch& s, double& lse, double& lse2) {
   } else {
     lse = (double)r.m + log(r.s1);
     lse2 = 2.0 * (double)r.m + log(r.s2);
+  }
+}
+
+// Weighted means sum_i w_i t_i / sum_i w_i of the three tangents, w_i = exp(lw_i - max), over the block.
+MOCAT_DEV void block_grad(GradAcc a, BlockScratch& s, double out[3]) {
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  for (int o = kWave / 2; o > 0; o >>= 1) {
+    GradAcc b;
+    b.m = __shfl_xor(a.m, o, kWave);
+    b.s1 = __shfl_xor(a.s1, o, kWave);
+    b.g0 = __shfl_xor(a.g0, o, kWave);
+    b.g1 = __shfl_xor(a.g1, o, kWave);
+    b.g2 = __shfl_xor(a.g2, o, kWave);
+    a = grad_combine(a, b);
+  }
+  __syncthreads();
+  if (lane == 0) {
+    s.fm[w] = a.m;
+    s.d1[w] = a.s1;
+    s.g0[w] = a.g0;
+    s.g1[w] = a.g1;
+    s.g2[w] = a.g2;
+  }
+  __syncthreads();
+  GradAcc r{s.fm[0], s.d1[0], s.g0[0], s.g1[0], s.g2[0]};
+  for (int i = 1; i < NW; ++i) r = grad_combine(r, GradAcc{s.fm[i], s.d1[i], s.g0[i], s.g1[i], s.g2[i]});
+  out[0] = r.g0 / r.s1;
+  out[1] = r.g1 / r.s1;
+  out[2] = r.g2 / r.s1;
+}
+
+MOCAT_DEV void grad_add(GradAcc& a, float v, float t0, float t1, float t2) {
+  if (!(v > -INFINITY)) return;
+  if (v > a.m) {
+    const double f = a.m == -INFINITY ? 0.0 : exp((double)a.m - (double)v);
+    a.s1 = a.s1 * f + 1.0;
+    a.g0 = a.g0 * f + t0;
+    a.g1 = a.g1 * f + t1;
+    a.g2 = a.g2 * f + t2;
+    a.m = v;
+  } else {
+    const double e = exp((double)v - (double)a.m);
+    a.s1 += e;
+    a.g0 += e * t0;
+    a.g1 += e * t1;
+    a.g2 += e * t2;
   }
 }
 
@@ -339,9 +413,15 @@ struct PfArgs {
   float* out_ess;     // [T] at t_offset, or NULL
   double* out_loglik_t;  // [T] at t_offset: cumulative log-likelihood after step t, or NULL
   double* out_loglik;    // [n_filters]
+  float* tan0;           // [sum n][5] tangents (dx/dphi, dx/dsigma, dlw/dphi, dlw/dsigma, dlw/dbeta), ping
+  float* tan1;           // pong
+  double* out_grad;      // [n_filters][3] d log Z / d(phi, sigma, beta) (stochastic volatility only)
 };
 
-template <class Model, int D, int DY>
+// GRAD (stochastic volatility only): forward tangents of the fixed-resampling-index estimator,
+// d log Z / d(phi, sigma, beta): per particle dx/dphi, dx/dsigma (gathered by ancestor like x) and the
+// log-weight tangents; d log Z = sum_t [avg_w(dlw_t) - avg_w(dlw_res)] in FP64.
+template <class Model, int D, int DY, bool GRAD>
 __global__ __launch_bounds__(NT) void pf_kernel(PfArgs a) {
   __shared__ BlockScratch s;
   __shared__ Model model;
@@ -362,6 +442,10 @@ __global__ __launch_bounds__(NT) void pf_kernel(PfArgs a) {
     const Key key{F.key[0], F.key[1]};
     const Key rkey = split_at(key, 2, 0), sub = split_at(key, 2, 1);
     constexpr uint32_t hD = (D + 1) / 2;
+    float* ta = GRAD ? a.tan0 + F.state_offset * 5 : nullptr;
+    float* tb = GRAD ? a.tan1 + F.state_offset * 5 : nullptr;
+    GradAcc gacc{-INFINITY, 0.0, 0.0, 0.0, 0.0};
+    double gtot[3] = {0.0, 0.0, 0.0}, gprev[3] = {0.0, 0.0, 0.0};
 
     // ---- t = 0: initiate_particles
     LseAcc acc{-INFINITY, 0.0, 0.0};
@@ -376,7 +460,20 @@ __global__ __launch_bounds__(NT) void pf_kernel(PfArgs a) {
         if (q + hD < (uint32_t)D) eps[q + hD] = normal_from_bits(w1);
       }
       model.init(eps, x);
-      const float l = -model.lik(x, yf);
+      float l;
+      if constexpr (GRAD) {
+        float dldx, dldb;
+        l = -model.lik_grad(x, yf, dldx, dldb);
+        // the initial state does not depend on (phi, sigma)
+        ta[i * 5 + 0] = 0.f;
+        ta[i * 5 + 1] = 0.f;
+        ta[i * 5 + 2] = 0.f;
+        ta[i * 5 + 3] = 0.f;
+        ta[i * 5 + 4] = -dldb;
+        grad_add(gacc, l, 0.f, 0.f, -dldb);
+      } else {
+        l = -model.lik(x, yf);
+      }
       #pragma unroll
       for (int q = 0; q < D; ++q) xa[i * D + q] = x[q];
       lwa[i] = l;
@@ -389,6 +486,10 @@ __global__ __launch_bounds__(NT) void pf_kernel(PfArgs a) {
     }
     double lse, lse2;
     block_lse(acc, s, lse, lse2);
+    if constexpr (GRAD) {
+      block_grad(gacc, s, gprev);
+      for (int k = 0; k < 3; ++k) gtot[k] = gprev[k];
+    }
     const double logn = log((double)n);
     double loglik = lse - logn;
     double ess = exp(2.0 * lse - lse2);
@@ -415,6 +516,7 @@ __global__ __launch_bounds__(NT) void pf_kernel(PfArgs a) {
         }
       }
       acc = LseAcc{-INFINITY, 0.0, 0.0};
+      gacc = GradAcc{-INFINITY, 0.0, 0.0, 0.0, 0.0};
       for (int64_t i = tid; i < n; i += NT) {
         const int64_t src = res ? (int64_t)anc[i] : i;
         const Key ki = split_at(ik, (uint32_t)n, (uint32_t)i);
@@ -429,7 +531,24 @@ __global__ __launch_bounds__(NT) void pf_kernel(PfArgs a) {
         #pragma unroll
         for (int q = 0; q < D; ++q) xp[q] = xa[src * D + q];
         model.transition(xp, eps, x);
-        const float l = (res ? 0.f : lwa[i]) - model.lik(x, ycur);
+        float l;
+        if constexpr (GRAD) {
+          float dldx, dldb;
+          l = (res ? 0.f : lwa[i]) - model.lik_grad(x, ycur, dldx, dldb);
+          const float dxp = xp[0] + model.phi * ta[src * 5 + 0];
+          const float dxs = eps[0] + model.phi * ta[src * 5 + 1];
+          const float d0 = (res ? 0.f : ta[i * 5 + 2]) - dldx * dxp;
+          const float d1 = (res ? 0.f : ta[i * 5 + 3]) - dldx * dxs;
+          const float d2 = (res ? 0.f : ta[i * 5 + 4]) - dldb;
+          tb[i * 5 + 0] = dxp;
+          tb[i * 5 + 1] = dxs;
+          tb[i * 5 + 2] = d0;
+          tb[i * 5 + 3] = d1;
+          tb[i * 5 + 4] = d2;
+          grad_add(gacc, l, d0, d1, d2);
+        } else {
+          l = (res ? 0.f : lwa[i]) - model.lik(x, ycur);
+        }
         #pragma unroll
         for (int q = 0; q < D; ++q) xb[i * D + q] = x[q];
         lwb[i] = l;
@@ -445,6 +564,17 @@ __global__ __launch_bounds__(NT) void pf_kernel(PfArgs a) {
       }
       block_lse(acc, s, lse, lse2);
       loglik += lse - lse_res;
+      if constexpr (GRAD) {
+        double gnow[3];
+        block_grad(gacc, s, gnow);
+        for (int k = 0; k < 3; ++k) {
+          gtot[k] += gnow[k] - (res ? 0.0 : gprev[k]);
+          gprev[k] = gnow[k];
+        }
+        float* tt = ta;
+        ta = tb;
+        tb = tt;
+      }
       ess = exp(2.0 * lse - lse2);
       if (tid == 0) {
         if (a.out_ess) a.out_ess[F.t_offset + t] = (float)ess;
@@ -467,7 +597,11 @@ __global__ __launch_bounds__(NT) void pf_kernel(PfArgs a) {
         (a.lw0 + F.state_offset)[i] = lwa[i];
       }
     }
-    if (tid == 0) a.out_loglik[f] = loglik;
+    if (tid == 0) {
+      a.out_loglik[f] = loglik;
+      if (GRAD)
+        for (int k = 0; k < 3; ++k) a.out_grad[(int64_t)f * 3 + k] = gtot[k];
+    }
     __syncthreads();
   }
 }
@@ -500,10 +634,10 @@ __global__ __launch_bounds__(256) void categorical_kernel(const float* __restric
     idx[j] = categorical_row(lw, n, k, S, j);
 }
 
-template <class Model, int D, int DY>
+template <class Model, int D, int DY, bool GRAD = false>
 int launch_pf(const PfArgs& a, hipStream_t st) {
   const int grid = a.n_filters < 4096 ? a.n_filters : 4096;
-  hipLaunchKernelGGL((pf_kernel<Model, D, DY>), dim3(grid), dim3(NT), 0, st, a);
+  hipLaunchKernelGGL((pf_kernel<Model, D, DY, GRAD>), dim3(grid), dim3(NT), 0, st, a);
   MOCAT_LAUNCH_CHECK();
   return MOCAT_OK;
 }
@@ -526,16 +660,18 @@ int mocat_pf_param_count(int model, int d, int dy, int64_t* count) {
 int mocat_pf_run(int model, int d, int dy, const mocat_pf_filter* filters, int n_filters, const float* params,
                  const float* y, float* x_ws0, float* x_ws1, float* lw_ws0, float* lw_ws1, uint64_t* csum_ws,
                  int32_t* anc_ws, int resampling, float ess_threshold, float* out_value, float* out_lw,
-                 int32_t* out_anc, float* out_ess, double* out_loglik_t, double* out_loglik, hipStream_t stream) {
+                 int32_t* out_anc, float* out_ess, double* out_loglik_t, double* out_loglik, float* tan_ws0,
+                 float* tan_ws1, double* out_grad, hipStream_t stream) {
   if (n_filters < 0 || !filters || !params || !y || !x_ws0 || !x_ws1 || !lw_ws0 || !lw_ws1 || !csum_ws || !anc_ws ||
       !out_loglik || resampling < 0 || resampling > 2)
     return MOCAT_ERR_ARG;
+  if (out_grad && (model != 1 || !tan_ws0 || !tan_ws1)) return MOCAT_ERR_ARG;
   if (n_filters == 0) return MOCAT_OK;
   PfArgs a{filters, n_filters, params, y, x_ws0, x_ws1, lw_ws0, lw_ws1, csum_ws, anc_ws, resampling,
-           ess_threshold, out_value, out_lw, out_anc, out_ess, out_loglik_t, out_loglik};
+           ess_threshold, out_value, out_lw, out_anc, out_ess, out_loglik_t, out_loglik, tan_ws0, tan_ws1, out_grad};
   if (model == 1) {
     if (d != 1 || dy != 1) return MOCAT_ERR_ARG;
-    return launch_pf<Sv, 1, 1>(a, stream);
+    return out_grad ? launch_pf<Sv, 1, 1, true>(a, stream) : launch_pf<Sv, 1, 1>(a, stream);
   }
   if (model != 0) return MOCAT_ERR_ARG;
 #define MOCAT_PF_CASE(DD, YY) \

@@ -24,7 +24,7 @@ RESAMPLING = {"systematic": 0, "multinomial": 1, "categorical": 2}
 _P = _lib.c_ptr
 _lib.register("mocat_pf_param_count", [_lib.c_i32, _lib.c_i32, _lib.c_i32, _P])
 _lib.register("mocat_pf_run", [_lib.c_i32, _lib.c_i32, _lib.c_i32, _P, _lib.c_i32, _P, _P, _P, _P, _P, _P, _P, _P,
-                               _lib.c_i32, _lib.c_f32, _P, _P, _P, _P, _P, _P, _P])
+                               _lib.c_i32, _lib.c_f32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P])
 _lib.register("mocat_resample", [_P, _lib.c_i64, _lib.c_i32, _lib.c_i32, _P, _P, _P, _P])
 _lib.register("mocat_resample_categorical", [_P, _lib.c_i64, _lib.c_i64, _P, _P, _P])
 
@@ -84,7 +84,7 @@ def _key_np(k):
     return np.asarray(k, np.uint32).reshape(2)
 
 
-def _launch(models, ys, keys, ns, ess_threshold, resampling, store):
+def _launch(models, ys, keys, ns, ess_threshold, resampling, store, grad=False):
     """Runs len(models) filters (same model class and dims) in one kernel launch."""
     m0 = models[0]
     d, dy, kind = m0.dim, m0.dim_obs, m0.kernel_model
@@ -125,16 +125,24 @@ def _launch(models, ys, keys, ns, ess_threshold, resampling, store):
     out_ess = torch.empty(toff, **f32)
     out_llt = torch.empty(toff, dtype=torch.float64, device=dev)
     out_ll = torch.empty(F, dtype=torch.float64, device=dev)
+    if grad and kind != 1:
+        raise NotImplementedError("log-likelihood gradients are implemented for the stochastic-volatility model")
+    tan0 = torch.empty(soff * 5, **f32) if grad else None
+    tan1 = torch.empty(soff * 5, **f32) if grad else None
+    out_grad = torch.empty((F, 3), dtype=torch.float64, device=dev) if grad else None
     _lib.call("mocat_pf_run", kind, d, dy, _lib.ptr(desc_t), F, _lib.ptr(params_t), _lib.ptr(y_t), _lib.ptr(x0),
               _lib.ptr(x1), _lib.ptr(lw0), _lib.ptr(lw1), _lib.ptr(cs), _lib.ptr(anc), RESAMPLING[resampling],
               float(ess_threshold), _lib.ptr(out_value), _lib.ptr(out_lw), _lib.ptr(out_anc), _lib.ptr(out_ess),
-              _lib.ptr(out_llt), _lib.ptr(out_ll), _lib.stream_handle())
+              _lib.ptr(out_llt), _lib.ptr(out_ll), _lib.ptr(tan0), _lib.ptr(tan1), _lib.ptr(out_grad),
+              _lib.stream_handle())
     res = [None] * F
     for slot, i in enumerate(order):
         r = dict(desc=desc[slot])
         n, T = ns[i], Ts[i]
         s0, o0, t0 = int(desc[slot]["state_offset"]), int(desc[slot]["out_offset"]), int(desc[slot]["t_offset"])
         r["log_likelihood"] = out_ll[slot]
+        if grad:
+            r["log_likelihood_grad"] = out_grad[slot]
         r["ess"] = out_ess[t0:t0 + T]
         r["log_likelihood_t"] = out_llt[t0:t0 + T]
         r["final_value"] = x0[s0 * d:(s0 + n) * d].reshape(n, d)
@@ -176,14 +184,16 @@ def run_particle_filter_for_marginals(ssm_scenario, particle_filter, y, t, rando
 
 
 def run_particle_filters(ssm_scenarios, ys, random_keys, ns, ess_threshold: float = 0.5,
-                         resampling: str = "systematic", store: bool = False):
+                         resampling: str = "systematic", store: bool = False, grad: bool = False):
     """Independent bootstrap filters of uneven sizes in one launch. Returns a list of cdicts with
     log_likelihood (FP64), ess (T,), final_value, final_log_weight (and value/log_weight/ancestors when
-    store=True)."""
+    store=True). grad=True (stochastic volatility): log_likelihood_grad = d log Z / d(phi, sigma, beta)
+    of the fixed-resampling-index estimator."""
     ys = [y.detach().cpu().numpy() if isinstance(y, torch.Tensor) else np.asarray(y) for y in ys]
     if resampling == "categorical" and any(int(v) * int(v) >= 0xFFFFFFFF for v in ns):
         raise ValueError("categorical resampling draws an (n, n) Gumbel matrix: n * n must be < 2^32 - 1")
-    res = _launch(list(ssm_scenarios), ys, list(random_keys), [int(v) for v in ns], ess_threshold, resampling, store)
+    res = _launch(list(ssm_scenarios), ys, list(random_keys), [int(v) for v in ns], ess_threshold, resampling, store,
+                  grad)
     return [cdict(**{k: v for k, v in r.items() if k != "desc"}) for r in res]
 
 

@@ -209,3 +209,53 @@ def simulate(p, T, key):
             LR = np.linalg.inv(p["SR"])
             ys.append(p["H"] @ xs[t] + LR @ e)
     return xs, np.stack(ys)
+
+
+def sv_loglik_grad(phi, sigma, beta, y, key, n, ancestors, resample_flags, m0=0.0, var0=1.0):
+    """d log Z / d(phi, sigma, beta) of the fixed-resampling-index estimator (ancestors and noise held
+    fixed) for the 1-D stochastic-volatility filter, by forward tangents in FP64:
+      x_t = phi x_{a} + sigma eps,  dx/dphi = x_a + phi dx_a/dphi,  dx/dsigma = eps + phi dx_a/dsigma
+      lik = 0.5 (y s)^2 + 0.5 log 2pi - log s, s = exp(-x/2) / beta:
+          dlik/dx = 0.5 - 0.5 (y s)^2,  dlik/dbeta = (1 - (y s)^2) / beta
+      d log Z = sum_t [avg_{w_t}(dlw_t) - avg_{w_res}(dlw_res)]
+    Pinned by central finite differences of run_pf on the same fixed path (tests/test_oracle_pf.py)."""
+    y = np.asarray(y, np.float64).reshape(-1)
+    T = y.shape[0]
+    rkey, sub = rng.split(np.asarray(key, np.uint32), 2)
+    eps = rng.normal_batch(rng.split(sub, n), 1).astype(np.float64)[:, 0]
+    x = eps * np.sqrt(var0) + m0
+    dxp = np.zeros(n)
+    dxs = np.zeros(n)
+
+    def lik_parts(x, yt):
+        s2 = (yt * yt) * np.exp(-x) / (beta * beta)
+        lik = 0.5 * s2 + 0.5 * LOG2PI + np.log(beta) + 0.5 * x
+        return lik, 0.5 - 0.5 * s2, (1.0 - s2) / beta
+
+    lik, dldx, dldb = lik_parts(x, y[0])
+    lw = -lik
+    dlw = np.stack([-dldx * dxp, -dldx * dxs, -dldb], 1)
+    w = np.exp(lw - lse(lw))
+    grad = w @ dlw
+    int_keys = rng.split(rkey, T - 1) if T > 1 else np.zeros((0, 2), np.uint32)
+    for t in range(1, T):
+        ik = int_keys[t - 1]
+        if resample_flags[t]:
+            a = np.asarray(ancestors[t], np.int64)
+            lw_res, dlw_res = np.zeros(n), np.zeros((n, 3))
+            res_term = np.zeros(3)
+        else:
+            a = np.arange(n)
+            lw_res, dlw_res = lw, dlw
+            res_term = np.exp(lw - lse(lw)) @ dlw
+        e = rng.normal_batch(rng.split(ik, n), 1).astype(np.float64)[:, 0]
+        xa, dpa, dsa = x[a], dxp[a], dxs[a]
+        x = phi * xa + sigma * e
+        dxp = xa + phi * dpa
+        dxs = e + phi * dsa
+        lik, dldx, dldb = lik_parts(x, y[t])
+        lw = lw_res - lik
+        dlw = dlw_res - np.stack([dldx * dxp, dldx * dxs, dldb], 1)
+        w = np.exp(lw - lse(lw))
+        grad = grad + w @ dlw - res_term
+    return grad
