@@ -41,6 +41,7 @@ def parse():
     ap.add_argument("--cpu-chains", type=int, default=64)
     ap.add_argument("--cpu-iters", type=int, default=120)
     ap.add_argument("--plain-steps", type=int, default=30, help="iterations timed on the unfused FP32 path (0: skip)")
+    ap.add_argument("--iters", type=int, default=500, help="iterations of the svgd4 workload")
     ap.add_argument("--mode", default="hmc", choices=["hmc", "copy", "pf1", "pf5", "smc3", "svgd4"],
                     help="hmc: the headline workload. copy: a plain 256 MiB device copy (calibration run for HBM "
                          "byte counters). pf1 / pf5 / smc3 / svgd4: the other workloads, one JSON line each")
@@ -245,7 +246,10 @@ def main():
         copy_calibration()
         return
     if a.mode != "hmc":
-        {"pf1": bench_pf1, "pf5": bench_pf5, "smc3": bench_smc3, "svgd4": bench_svgd4}[a.mode]()
+        if a.mode == "svgd4":
+            bench_svgd4(iters=a.iters)
+        else:
+            {"pf1": bench_pf1, "pf5": bench_pf5, "smc3": bench_smc3}[a.mode]()
         return
     import mocat_amd as m
     from mocat_amd import datasets
