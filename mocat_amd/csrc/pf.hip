@@ -131,10 +131,10 @@ struct Sv {
   }
   MOCAT_DEV void init(const float* eps, float* x) const { x[0] = eps[0] * L0 + m0; }
   MOCAT_DEV void transition(const float* xp, const float* eps, float* x) const { x[0] = xp[0] * phi + eps[0] * sigma; }
+  // one expression for both paths, so that computing the gradient leaves the log-weights bit-identical
   MOCAT_DEV float lik(const float* x, const float* y) const {
-    const float s = inv_beta / __expf(0.5f * x[0]);
-    const float ys = y[0] * s;
-    return 0.5f * ys * ys + 0.5f * (float)kLog2PiD - __logf(s);
+    float dldx, dldb;
+    return lik_grad(x, y, dldx, dldb);
   }
   // d lik / dx = 0.5 - 0.5 (y s)^2 ; d lik / dbeta = (1 - (y s)^2) * inv_beta
   MOCAT_DEV float lik_grad(const float* x, const float* y, float& dldx, float& dldb) const {

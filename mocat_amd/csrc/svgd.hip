@@ -44,13 +44,37 @@ struct SvgdLayout {
   static constexpr int BLOCK_BYTES = BLOCK_BF16 * 2 + 32 * 4;
 };
 
-// Column means of x (one thread per column, FP64 accumulation).
-__global__ void colmean_kernel(const float* __restrict__ x, int n, int d, float* __restrict__ mean) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= d) return;
+// Column means, FP64, in a fixed summation order. One workgroup per 64 columns: lane = column
+// (coalesced row segments), wave w sums rows w, w + 16, ... in 8 interleaved accumulators, and the
+// 16 wave sums are added in wave order.
+constexpr int kMeanThreads = 1024;
+__global__ __launch_bounds__(kMeanThreads) void colmean_kernel(const float* __restrict__ x, int n, int d,
+                                                               float* __restrict__ mean) {
+  constexpr int NW = kMeanThreads / kWave, U = 8;
+  __shared__ double red[NW][kWave];
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  const int k = blockIdx.x * kWave + lane;
+  double acc[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) acc[u] = 0.0;
+  if (k < d) {
+    int i = w;
+    for (; i + (U - 1) * NW < n; i += U * NW) {
+#pragma unroll
+      for (int u = 0; u < U; ++u) acc[u] += (double)x[(int64_t)(i + u * NW) * d + k];
+    }
+    for (; i < n; i += NW) acc[0] += (double)x[(int64_t)i * d + k];
+  }
   double s = 0.0;
-  for (int i = 0; i < n; ++i) s += (double)x[(int64_t)i * d + k];
-  mean[k] = (float)(s / (double)n);
+#pragma unroll
+  for (int u = 0; u < U; ++u) s += acc[u];
+  red[w][lane] = s;
+  __syncthreads();
+  if (w == 0 && k < d) {
+    double t = 0.0;
+    for (int q = 0; q < NW; ++q) t += red[q][lane];
+    mean[k] = (float)(t / (double)n);
+  }
 }
 
 // Pack the centred particles (A layout), [grad | centred x] (B layout) and |x_j|^2 per 32-row block.
@@ -354,43 +378,82 @@ __global__ __launch_bounds__(NT) void sqdist_kernel(const char* __restrict__ pac
 
 // ---------------------------------------------------------------- exact radix select (2 ranks)
 // state[0..1] prefix bits, state[2..3] remaining rank, hist[2][256]
+// Adds, for every distinct digit among the wave's active lanes, that digit's lane count to h (one LDS
+// atomic per distinct digit): the top-byte pass sees very few distinct digits, which would otherwise
+// serialise on one LDS address. Wave-uniform control flow is required (ballots).
+MOCAT_DEV void wave_count(bool act, uint32_t dig, uint32_t* h, int lane) {
+  uint64_t pending = __ballot(act);
+  while (pending) {
+    const int leader = __ffsll((unsigned long long)pending) - 1;
+    const uint32_t ld = (uint32_t)__shfl((int)dig, leader, kWave);
+    const uint64_t same = __ballot(((pending >> lane) & 1ull) && dig == ld);
+    if (lane == leader) atomicAdd(&h[ld], (uint32_t)__popcll(same));
+    pending &= ~same;
+  }
+}
+
+// One radix pass of the two-rank selection: histogram of the byte at `shift` over the values whose
+// higher bytes equal each rank's prefix so far. When both ranks share the prefix (always on the top
+// byte, and usually after it for adjacent ranks) one histogram is counted and written to both.
 __global__ __launch_bounds__(256) void radix_hist_kernel(const uint32_t* __restrict__ v, int64_t m, int shift,
                                                          const uint32_t* __restrict__ state,
                                                          uint32_t* __restrict__ hist) {
   __shared__ uint32_t lh[2][256];
   const uint32_t mask = shift == 24 ? 0u : (0xFFFFFFFFu << (shift + 8));
   const uint32_t p0 = state[0] & mask, p1 = state[1] & mask;
+  const bool shared_prefix = p0 == p1;
+  const int lane = threadIdx.x & (kWave - 1);
   for (int b = threadIdx.x; b < 512; b += 256) (&lh[0][0])[b] = 0u;
   __syncthreads();
-  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < m; e += (int64_t)gridDim.x * blockDim.x) {
-    const uint32_t u = v[e];
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  // block-uniform trip count, so every lane of a wave reaches the ballots
+  for (int64_t base = blockIdx.x * (int64_t)blockDim.x; base < m; base += stride) {
+    const int64_t e = base + threadIdx.x;
+    const bool valid = e < m;
+    const uint32_t u = valid ? v[e] : 0u;
     const uint32_t dig = (u >> shift) & 255u;
-    if ((u & mask) == p0) atomicAdd(&lh[0][dig], 1u);
-    if ((u & mask) == p1) atomicAdd(&lh[1][dig], 1u);
+    const bool a0 = valid && (u & mask) == p0;
+    if (shift == 24) {
+      wave_count(a0, dig, lh[0], lane);
+    } else {
+      if (a0) atomicAdd(&lh[0][dig], 1u);
+      if (!shared_prefix && valid && (u & mask) == p1) atomicAdd(&lh[1][dig], 1u);
+    }
   }
   __syncthreads();
   for (int b = threadIdx.x; b < 512; b += 256) {
-    const uint32_t c = (&lh[0][0])[b];
+    const uint32_t c = shared_prefix ? lh[0][b & 255] : (&lh[0][0])[b];
     if (c) atomicAdd(&hist[b], c);
   }
 }
 
-__global__ void radix_choose_kernel(int shift, uint32_t* __restrict__ state, uint32_t* __restrict__ hist) {
-  const int tsel = threadIdx.x;  // 0, 1
-  if (tsel < 2) {
-    uint64_t k = state[2 + tsel];
-    uint32_t b = 0;
-    for (; b < 256; ++b) {
-      const uint32_t c = hist[tsel * 256 + b];
-      if (k < c) break;
-      k -= c;
-    }
-    if (b > 255) b = 255;
-    state[tsel] = (state[tsel] & ~(255u << shift)) | (b << shift);
-    state[2 + tsel] = (uint32_t)k;
+// Picks each rank's digit from the pass histogram (block prefix sums over the 256 bins), records it in
+// the rank's prefix, reduces the remaining rank, and clears the histogram for the next pass.
+__global__ __launch_bounds__(256) void radix_choose_kernel(int shift, uint32_t* __restrict__ state,
+                                                           uint32_t* __restrict__ hist) {
+  __shared__ uint32_t wtot[2][4];
+  const int b = threadIdx.x, lane = b & (kWave - 1), w = b / kWave;
+  uint32_t k[2], c[2], inc[2];
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    k[t] = state[2 + t];
+    c[t] = hist[t * 256 + b];
+    inc[t] = wave_inclusive_scan(c[t]);
+    if (lane == kWave - 1) wtot[t][w] = inc[t];
   }
   __syncthreads();
-  for (int b = threadIdx.x; b < 512; b += blockDim.x) hist[b] = 0u;
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    uint32_t off = 0;
+    for (int q = 0; q < w; ++q) off += wtot[t][q];
+    const uint32_t lo = off + inc[t] - c[t], hi = off + inc[t];
+    // exactly one bin holds rank k (the counts sum to the number of candidates, which exceeds k)
+    if (k[t] >= lo && k[t] < hi) {
+      state[t] = (state[t] & ~(255u << shift)) | ((uint32_t)b << shift);
+      state[2 + t] = k[t] - lo;
+    }
+    hist[t * 256 + b] = 0u;
+  }
 }
 
 __global__ void select_init_kernel(uint32_t* __restrict__ ws, uint32_t rank0, uint32_t rank1) {
@@ -422,7 +485,7 @@ int svgd_step_dp(const float* x_in, const float* grad, int n, int d, float bandw
                  float* mean, float* part_o, float* part_s, float step, float momentum, float* x, float* gsq,
                  float* mom, float* phi_out, hipStream_t st) {
   using L = SvgdLayout<DP>;
-  hipLaunchKernelGGL(colmean_kernel, dim3((d + 127) / 128), dim3(128), 0, st, x_in, n, d, mean);
+  hipLaunchKernelGGL(colmean_kernel, dim3((d + kWave - 1) / kWave), dim3(kMeanThreads), 0, st, x_in, n, d, mean);
   const int nrb = (n + 31) / 32;
   const int64_t tot = (int64_t)nrb * (L::BLOCK_BF16 + 32);
   const unsigned pb = (unsigned)std::min<int64_t>((tot + 255) / 256, 65535);
@@ -441,7 +504,7 @@ int svgd_step_dp(const float* x_in, const float* grad, int n, int d, float bandw
 template <int DP>
 int sqdist_dp(const float* x, int n, int d, void* packed, float* mean, float* d2, hipStream_t st) {
   using L = SvgdLayout<DP>;
-  hipLaunchKernelGGL(colmean_kernel, dim3((d + 127) / 128), dim3(128), 0, st, x, n, d, mean);
+  hipLaunchKernelGGL(colmean_kernel, dim3((d + kWave - 1) / kWave), dim3(kMeanThreads), 0, st, x, n, d, mean);
   const int nrb = (n + 31) / 32;
   const int64_t tot = (int64_t)nrb * (L::BLOCK_BF16 + 32);
   const unsigned pb = (unsigned)std::min<int64_t>((tot + 255) / 256, 65535);

@@ -67,6 +67,26 @@ def test_select_exact(m, m_):
     assert got[0] == s[r0] and got[1] == s[r1]
 
 
+@pytest.mark.parametrize("case", ["extremes", "thirds", "constant", "two_values", "narrow"])
+def test_select_exact_rank_pairs(m, case):
+    """Ranks whose prefixes part at the first byte, and inputs whose values share most bytes (every
+    lane of a wave on one digit), against a full sort."""
+    rs = np.random.default_rng(5)
+    m_ = 300_001
+    if case == "constant":
+        v = np.full(m_, 3.5, np.float32)
+    elif case == "two_values":
+        v = np.where(rs.random(m_) < 0.5, 1.0, 1e6).astype(np.float32)
+    elif case == "narrow":
+        v = (100.0 + rs.random(m_) * 1e-3).astype(np.float32)
+    else:
+        v = rs.lognormal(0, 4, m_).astype(np.float32)
+    r0, r1 = {"extremes": (0, m_ - 1), "thirds": (m_ // 3, 2 * m_ // 3)}.get(case, (m_ // 2 - 1, m_ // 2 + 7))
+    got = m.kernels.select_pair(torch.tensor(v, device="cuda"), r0, r1).cpu().numpy()
+    s = np.sort(v)
+    assert got[0] == s[r0] and got[1] == s[r1]
+
+
 def test_median_and_mean_bandwidth(m):
     x = np.random.default_rng(3).normal(0, 2, (700, 30)).astype(np.float32)
     xt = torch.tensor(x, device="cuda")
