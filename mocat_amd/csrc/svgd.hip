@@ -190,35 +190,90 @@ MOCAT_DEV floatx16 gram_block(const bf16x8* __restrict__ A, const ITile<DP>& it,
 }
 
 // ---------------------------------------------------------------- phi (partial over a j range)
-template <int DP, int NT>
-__global__ __launch_bounds__(NT) void svgd_phi_kernel(const char* __restrict__ packed, const float* __restrict__ x,
-                                                      const float* __restrict__ mean, int n, int d, float inv2h2,
-                                                      int js_count, float* __restrict__ part_o,
-                                                      float* __restrict__ part_s) {
+// One workgroup = 4 waves = 128 particles i, 32 per wave, whose distance-GEMM operands (bf16 hi/lo
+// of the centred coordinates) stay in registers. The packed 32-particle j blocks of the workgroup's
+// j range are staged through LDS, double buffered: each block is read from L2 once per workgroup and
+// used by all four waves. Per j block and wave: S = X_J X_I^T (MFMA), K = exp(-|x_i - x_j|^2 / 2h^2)
+// in registers (the accumulator reused as the next A operand), then [K G_J | K X_J] += on MFMA.
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+constexpr int kPhiWaves = 4, kPhiThreads = kPhiWaves * kWave, kPhiI = kPhiWaves * 32;
+
+template <int DP>
+__global__ __launch_bounds__(kPhiThreads) void svgd_phi_kernel(const char* __restrict__ packed,
+                                                               const float* __restrict__ x,
+                                                               const float* __restrict__ mean, int n, int d,
+                                                               float inv2h2, int js_count,
+                                                               float* __restrict__ part_o,
+                                                               float* __restrict__ part_s) {
   using L = SvgdLayout<DP>;
-  constexpr int NB = L::NB, NW = NT / kWave, HW = NW / 2;
-  static_assert(NW % 2 == 0, "even wave count");
-  __shared__ ITile<DP> it;
-  __shared__ alignas(16) float gred[HW][CT * 2 * DP];
-  __shared__ float sred[NW][2][CT];
+  constexpr int KB = L::KB, NB = L::NB;
+  static_assert(L::BLOCK_BYTES % 16 == 0, "packed blocks are copied in 16-byte chunks");
+  constexpr int CHUNKS = L::BLOCK_BYTES / 16, PER = (CHUNKS + kPhiThreads - 1) / kPhiThreads;
+  __shared__ u32x4 stage[2][CHUNKS];
   const int t = threadIdx.x, lane = t & (kWave - 1), w = t / kWave, g = lane >> 5, col = lane & 31;
-  const int i0 = blockIdx.x * CT;
+  const int i0 = blockIdx.x * kPhiI + w * 32;
   const int nrb = (n + 31) / 32;
   const int js = blockIdx.y;
   const int rb0 = (int)((int64_t)nrb * js / js_count), rb1 = (int)((int64_t)nrb * (js + 1) / js_count);
-  load_itile<DP, NT>(it, x, mean, n, d, i0);
-  const float ni = it.ni[col];
+
+  // B operand of the distance GEMM: lane holds particle i0 + col, columns kb*16 + g*8 + s
+  bf16x8 bih[KB], bil[KB];
+  float nih = 0.f;
+  {
+    const int i = i0 + col;
+#pragma unroll
+    for (int kb = 0; kb < KB; ++kb)
+#pragma unroll
+      for (int s2 = 0; s2 < 8; ++s2) {
+        const int k = kb * 16 + g * 8 + s2;
+        const float v = (i < n && k < d) ? x[(int64_t)i * d + k] - mean[k] : 0.f;
+        nih += v * v;
+        __bf16 h, l;
+        split_bf16(v, h, l);
+        bih[kb][s2] = h;
+        bil[kb][s2] = l;
+      }
+  }
+  const float ni = nih + __shfl_xor(nih, 32, kWave);
+
   floatx16 gacc[NB];
 #pragma unroll
   for (int nb = 0; nb < NB; ++nb)
 #pragma unroll
     for (int r = 0; r < 16; ++r) gacc[nb][r] = 0.f;
   float spart = 0.f;
-  for (int rb = rb0 + w; rb < rb1; rb += NW) {
-    const char* blk = packed + (int64_t)rb * L::BLOCK_BYTES;
+
+  // staging registers for the next block (static indices only, so they stay in VGPRs)
+  u32x4 regs[PER];
+  const int nsteps = rb1 - rb0;
+  if (nsteps > 0) {
+    const u32x4* src = reinterpret_cast<const u32x4*>(packed + (int64_t)rb0 * L::BLOCK_BYTES);
+#pragma unroll
+    for (int p = 0; p < PER; ++p)
+      if (t + p * kPhiThreads < CHUNKS) stage[0][t + p * kPhiThreads] = src[t + p * kPhiThreads];
+  }
+  __syncthreads();
+  for (int st = 0; st < nsteps; ++st) {
+    if (st + 1 < nsteps) {
+      const u32x4* src = reinterpret_cast<const u32x4*>(packed + (int64_t)(rb0 + st + 1) * L::BLOCK_BYTES);
+#pragma unroll
+      for (int p = 0; p < PER; ++p)
+        if (t + p * kPhiThreads < CHUNKS) regs[p] = src[t + p * kPhiThreads];
+    }
+    const char* blk = reinterpret_cast<const char*>(stage[st & 1]);
     const bf16x8* A = reinterpret_cast<const bf16x8*>(blk);
     const bf16x8* B = A + L::A_ELEMS / 8;
-    const floatx16 z = gram_block<DP>(A, it, lane);
+    floatx16 z;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) z[r] = 0.f;
+#pragma unroll
+    for (int kb = 0; kb < KB; ++kb) {
+      const bf16x8 ah = A[(0 * KB + kb) * 64 + lane];
+      const bf16x8 al = A[(1 * KB + kb) * 64 + lane];
+      z = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bih[kb], z, 0, 0, 0);
+      z = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bil[kb], z, 0, 0, 0);
+      z = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bih[kb], z, 0, 0, 0);
+    }
     const floatx4* nj4 = reinterpret_cast<const floatx4*>(blk + L::BLOCK_BF16 * 2 + g * 64);
     float nj[16];
 #pragma unroll
@@ -230,7 +285,7 @@ __global__ __launch_bounds__(NT) void svgd_phi_kernel(const char* __restrict__ p
       nj[4 * q + 3] = v.w;
     }
     bf16x8 kh[2], kl[2];
-    const int jbase = rb * 32;
+    const int jbase = (rb0 + st) * 32;
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
       // the self-distance is exactly 0 (the norms-minus-Gram form leaves cancellation residue there)
@@ -253,39 +308,24 @@ __global__ __launch_bounds__(NT) void svgd_phi_kernel(const char* __restrict__ p
         gacc[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(kl[kb2], xh, gacc[nb], 0, 0, 0);
       }
     }
-  }
-  // deterministic cross-wave reduction; accumulator rows = particles i (acc_row), columns = features
-  sred[w][g][col] = spart;
-  __syncthreads();
-  if (w >= HW) {
+    if (st + 1 < nsteps) {
+      u32x4* dst = stage[(st + 1) & 1];
 #pragma unroll
-    for (int nb = 0; nb < NB; ++nb)
-#pragma unroll
-      for (int r = 0; r < 16; ++r) gred[w - HW][acc_row(r, lane) * 2 * DP + nb * 32 + col] = gacc[nb][r];
+      for (int p = 0; p < PER; ++p)
+        if (t + p * kPhiThreads < CHUNKS) dst[t + p * kPhiThreads] = regs[p];
+    }
+    __syncthreads();
   }
-  __syncthreads();
-  if (w < HW) {
+  // accumulator rows = particles i (acc_row), columns = features; row sums of K from both lane halves
+  spart += __shfl_xor(spart, 32, kWave);
+  if (g == 0 && i0 + col < n) part_s[(int64_t)js * n + i0 + col] = spart;
 #pragma unroll
-    for (int nb = 0; nb < NB; ++nb)
+  for (int nb = 0; nb < NB; ++nb)
 #pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        float* p = &gred[w][acc_row(r, lane) * 2 * DP + nb * 32 + col];
-        *p = gacc[nb][r] + *p;
-      }
-  }
-  __syncthreads();
-  for (int e = t; e < CT * 2 * DP; e += NT) {
-    const int c = e / (2 * DP);
-    float acc = gred[0][e];
-#pragma unroll
-    for (int q = 1; q < HW; ++q) acc += gred[q][e];
-    if (i0 + c < n) part_o[((int64_t)js * n + i0 + c) * 2 * DP + (e - c * 2 * DP)] = acc;
-  }
-  if (t < CT && i0 + t < n) {
-    float acc = 0.f;
-    for (int q = 0; q < NW; ++q) acc += sred[q][0][t] + sred[q][1][t];
-    part_s[(int64_t)js * n + i0 + t] = acc;
-  }
+    for (int r = 0; r < 16; ++r) {
+      const int i = i0 + acc_row(r, lane);
+      if (i < n) part_o[((int64_t)js * n + i) * 2 * DP + nb * 32 + col] = gacc[nb][r];
+    }
 }
 
 // phi from the partial sums, then the adagrad step (x, G, m updated in place); phi_out optional.
@@ -477,7 +517,6 @@ __global__ __launch_bounds__(256) void sqrt_sum_kernel(const float* __restrict__
   if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
-constexpr int kPhiThreads = 256;
 constexpr int kDistThreads = 256;
 
 template <int DP>
@@ -491,8 +530,9 @@ int svgd_step_dp(const float* x_in, const float* grad, int n, int d, float bandw
   const unsigned pb = (unsigned)std::min<int64_t>((tot + 255) / 256, 65535);
   hipLaunchKernelGGL(svgd_pack_kernel<DP>, dim3(pb), dim3(256), 0, st, x_in, grad, mean, n, d, (char*)packed);
   const float h2 = bandwidth * bandwidth;
-  hipLaunchKernelGGL((svgd_phi_kernel<DP, kPhiThreads>), dim3((unsigned)nrb, (unsigned)js_count), dim3(kPhiThreads),
-                     0, st, (const char*)packed, x_in, mean, n, d, 0.5f / h2, js_count, part_o, part_s);
+  hipLaunchKernelGGL((svgd_phi_kernel<DP>), dim3((unsigned)((n + kPhiI - 1) / kPhiI), (unsigned)js_count),
+                     dim3(kPhiThreads), 0, st, (const char*)packed, x_in, mean, n, d, 0.5f / h2, js_count, part_o,
+                     part_s);
   const int64_t tot2 = (int64_t)n * d;
   const unsigned ub = (unsigned)std::min<int64_t>((tot2 + 255) / 256, 65535);
   hipLaunchKernelGGL(svgd_update_kernel, dim3(ub), dim3(256), 0, st, part_o, part_s, js_count, n, d, DP, mean,

@@ -259,14 +259,14 @@ class SVGD(TransportSampler):
 
     bandwidth_update: "mean" (mocat's default when no kernel is given, svgd.py:59-61), "median"
     (kernels.median_bandwidth_update), or None (fixed kernel bandwidth).
-    j_splits: workgroups sharing the pairwise sum of one particle tile (partial sums combined in the
-    update kernel).
+    j_splits: workgroups sharing the pairwise sum of one 128-particle tile (partial sums combined in
+    the update kernel); None picks enough to give every compute unit one workgroup.
     """
     name = "SVGD"
 
     def __init__(self, stepsize, max_iter: int = 1000, kernel=None, kernel_params: cdict = None,
                  ensemble_batchsize: int = None, optimiser: str = "adagrad", momentum: float = 0.9,
-                 bandwidth_update: str = "default", j_splits: int = 4, **kwargs):
+                 bandwidth_update: str = "default", j_splits: int = None, **kwargs):
         super().__init__(max_iter=max_iter, **kwargs)
         from .kernels import Gaussian
         if bandwidth_update == "default":
@@ -284,7 +284,13 @@ class SVGD(TransportSampler):
         self.parameters.kernel_params = kernel_params if kernel_params is not None else kernel.parameters
         self.parameters.momentum = momentum
         self.bandwidth_update = bandwidth_update
-        self.j_splits = int(j_splits)
+        self.j_splits = None if j_splits is None else int(j_splits)
+
+    def _js(self, n: int) -> int:
+        if self.j_splits is not None:
+            return self.j_splits
+        tiles, blocks = -(-n // 128), -(-n // 32)
+        return max(1, min(32, blocks, -(-256 // tiles)))
 
     def _adapt(self, x):
         from .kernels import mean_bandwidth_update, median_bandwidth_update
@@ -306,10 +312,10 @@ class SVGD(TransportSampler):
         step = float(step(extra.iter)) if callable(step) else float(step)
         if workspace is None:
             from .kernels import _workspace
-            workspace = _workspace(n, d, self.j_splits, x.device)
+            workspace = _workspace(n, d, self._js(n), x.device)
         x_in = x.clone()
         phi = torch.empty_like(x)
-        _lib.call("mocat_svgd_step", _lib.ptr(x_in), _lib.ptr(g), n, d, float(extra.bandwidth), self.j_splits,
+        _lib.call("mocat_svgd_step", _lib.ptr(x_in), _lib.ptr(g), n, d, float(extra.bandwidth), self._js(n),
                   _lib.ptr(workspace), step, float(self.parameters.momentum), _lib.ptr(x), _lib.ptr(extra.g_sq),
                   _lib.ptr(extra.m), _lib.ptr(phi), _lib.stream_handle())
         T = float(scenario.temperature)
@@ -335,7 +341,7 @@ class SVGD(TransportSampler):
         extra.bandwidth = float(self.parameters.kernel_params.bandwidth)
         extra.bandwidth = self._adapt(x)
         from .kernels import _workspace
-        ws = _workspace(n, x.shape[1], self.j_splits, x.device)
+        ws = _workspace(n, x.shape[1], self._js(n), x.device)
         stacks = {k: [getattr(state, k).clone()] for k in stack}
         bws = [extra.bandwidth]
         for _ in range(self.max_iter):

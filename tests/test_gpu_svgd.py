@@ -94,13 +94,15 @@ def test_median_and_mean_bandwidth(m):
     np.testing.assert_allclose(m.kernels.mean_bandwidth_update(xt).item(), osv.mean_bandwidth(x), rtol=2e-5)
 
 
-@pytest.mark.parametrize("n,d", [(4096, 100), (500, 10)])
-def test_svgd_step_vs_fp64(m, n, d):
+@pytest.mark.parametrize("n,d,js", [(4096, 100, None), (500, 10, 4), (130, 65, None), (33, 3, 1), (300, 128, 64)])
+def test_svgd_step_vs_fp64(m, n, d, js):
+    """Ragged particle tiles (n not a multiple of 32 or 128), both packed widths, and j splits from
+    one to more than there are j blocks (empty ranges contribute zero partial sums)."""
     rs = np.random.default_rng(d)
     x = rs.normal(0, 2, (n, d)).astype(np.float32)
     g = rs.normal(0, 1, (n, d)).astype(np.float32)
     h = float(osv.median_bandwidth(x)) if n <= 500 else 3.0
-    smp = m.SVGD(stepsize=0.01, bandwidth_update=None, j_splits=4)
+    smp = m.SVGD(stepsize=0.01, bandwidth_update=None, j_splits=js)
     st = m.cdict(value=torch.tensor(x, device="cuda"), grad_potential=torch.tensor(g, device="cuda"))
     ex = m.cdict(random_key=m.random.PRNGKey(0), iter=0, bandwidth=h, g_sq=torch.zeros((n, d), device="cuda"),
                  m=torch.zeros((n, d), device="cuda"))
