@@ -166,6 +166,16 @@ int mocat_select2(const float* v, int64_t m, int64_t rank0, int64_t rank1, uint3
  * mean_bandwidth_update (mocat/src/kernels.py:227-229). */
 int mocat_sqrt_mean(const float* d2, int64_t m, double* partial, int n_partial, hipStream_t stream);
 
+/* ---- Kernel Stein discrepancy (replaces mocat/src/metrics.py:88-130, ksd with
+ * ensemble_batchsize=None, for kernels.Gaussian (kernel 0) and kernels.IMQ (kernel 1) with mocat's
+ * k_0 of mocat/src/metrics.py:116-124 and the kernel methods of mocat/src/kernels.py:83-111,
+ * 164-215). part[js][i] = sum over the js-th share of 64-particle j tiles of
+ * weights[j] * k_0(x_i, x_j) (FP64); weights may be NULL (all 1). The caller forms
+ * sqrt(sum_i w_i sum_js part[js][i]) / sum(w). c and beta are used by the IMQ kernel only. */
+int mocat_ksd_rows(const float* x, const float* grad_potential, const float* weights, int64_t n, int64_t d,
+                   int kernel, float bandwidth, float c, float beta, int js_count, double* part,
+                   hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

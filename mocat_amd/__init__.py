@@ -16,5 +16,6 @@ from .transport import (TransportSampler, SMCSampler, TemperedSMCSampler, Metrop
 from .scenarios import GaussianMixture
 from . import kernels
 from . import reduce, metrics, ssm
+from .metrics import ksd
 
 __version__ = "0.1.0"

@@ -1,6 +1,7 @@
 """Kernels and bandwidth updates (mocat/src/kernels.py).
 
 Gaussian(bandwidth): k(x, y) = exp(-|x - y|^2 / (2 h^2)) (kernels.py:82-116).
+IMQ(c, beta, bandwidth): k(x, y) = (c + |x - y|^2 / (2 h^2))^beta (kernels.py:164-215).
 median_bandwidth_update / mean_bandwidth_update: median / mean of all n^2 pairwise distances
 (diagonal zeros included) divided by sqrt(2 log n) (kernels.py:220-229). Distances come from the
 MFMA Gram kernel; the median is an exact radix select of the two middle order statistics.
@@ -40,6 +41,23 @@ class Gaussian(Kernel):
         h = self.parameters.bandwidth if bandwidth is None else bandwidth
         diff = (x - y) / h
         return torch.exp(-0.5 * torch.sum(diff * diff, dim=-1))
+
+
+class IMQ(Kernel):
+    """Inverse multiquadric k(x, y) = (c + |x - y|^2 / (2 h^2))^beta (mocat/src/kernels.py:164-215)."""
+
+    def __init__(self, c: float = 1.0, beta: float = -0.5, bandwidth: float = 1.0):
+        super().__init__()
+        self.parameters.c = c
+        self.parameters.beta = beta
+        self.parameters.bandwidth = bandwidth
+
+    def __call__(self, x, y, c=None, beta=None, bandwidth=None):
+        c = self.parameters.c if c is None else c
+        beta = self.parameters.beta if beta is None else beta
+        h = self.parameters.bandwidth if bandwidth is None else bandwidth
+        diff = (x - y) / h
+        return torch.pow(c + 0.5 * torch.sum(diff * diff, dim=-1), beta)
 
 
 def _workspace(n, d, js=1, device=None):

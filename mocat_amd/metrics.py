@@ -8,10 +8,19 @@ They are measurement utilities (used to report ESS/s), computed with torch FFTs 
 
 log_ess_log_weight / ess_log_weight follow mocat/src/metrics.py:69-78:
 2 * logsumexp(w) - logsumexp(2 w), reduced in FP64 by the reduction kernel of libmocat_amd.
+
+ksd follows mocat/src/metrics.py:88-130 for the Gaussian and IMQ kernels over the full ensemble: the
+weighted k_0 row sums come from the tiled pair kernel of libmocat_amd (FP32 coordinate sums, FP64
+row sums), the final weighting and square root are FP64.
 """
 from __future__ import annotations
 
 import torch
+
+from . import _lib as _l
+
+_l.register("mocat_ksd_rows", [_l.c_ptr, _l.c_ptr, _l.c_ptr, _l.c_i64, _l.c_i64, _l.c_i32, _l.c_f32, _l.c_f32,
+                               _l.c_f32, _l.c_i32, _l.c_ptr, _l.c_ptr])
 
 
 def _next_pow_two(n: int) -> int:
@@ -64,3 +73,46 @@ def log_ess_log_weight(log_weight) -> torch.Tensor:
 
 def ess_log_weight(log_weight) -> torch.Tensor:
     return torch.exp(log_ess_log_weight(log_weight))
+
+
+_KSD_KERNELS = {"Gaussian": 0, "IMQ": 1}
+
+
+def ksd(sample, kernel, grad_potential: torch.Tensor = None, log_weight: torch.Tensor = None,
+        ensemble_batchsize: int = None, random_key=None, **kernel_params) -> float:
+    """Kernel Stein discrepancy of a weighted ensemble (mocat/src/metrics.py:88-130).
+
+    sample: (n, d) values or a cdict with .value (and .grad_potential when grad_potential is None).
+    kernel: mocat_amd.kernels.Gaussian or IMQ; kernel_params override its parameters. As in mocat,
+    k_0 takes the potential gradient where a Stein kernel takes the score."""
+    from . import _lib
+    vals = sample.value if hasattr(sample, "value") and not isinstance(sample, torch.Tensor) else sample
+    if grad_potential is None and hasattr(sample, "grad_potential") and not isinstance(sample, torch.Tensor):
+        grad_potential = sample.grad_potential
+    elif grad_potential is None:
+        raise TypeError("grad_potential not found")
+    if ensemble_batchsize is not None:
+        raise NotImplementedError("the KSD kernel sums over the full ensemble (ensemble_batchsize=None)")
+    name = kernel.__class__.__name__
+    if name not in _KSD_KERNELS:
+        raise NotImplementedError(f"KSD is implemented for the Gaussian and IMQ kernels, not {name}")
+    params = dict(kernel.parameters.__dict__)
+    params.update(kernel_params)
+    _lib.require_gpu()
+    x = torch.as_tensor(vals, dtype=torch.float32, device="cuda")
+    x = x.reshape(x.shape[0], -1).contiguous()
+    g = torch.as_tensor(grad_potential, dtype=torch.float32, device="cuda").reshape(x.shape).contiguous()
+    n, d = x.shape
+    if log_weight is not None:
+        w64 = torch.exp(torch.as_tensor(log_weight, device="cuda").to(torch.float64).reshape(n))
+        w = w64.to(torch.float32).contiguous()
+    else:
+        w64, w = torch.ones(n, dtype=torch.float64, device="cuda"), None
+    tiles = -(-n // 64)
+    js = max(1, min(tiles, 64))
+    part = torch.empty((js, n), dtype=torch.float64, device="cuda")
+    _lib.call("mocat_ksd_rows", _lib.ptr(x), _lib.ptr(g), _lib.ptr(w), n, d, _KSD_KERNELS[name],
+              float(params["bandwidth"]), float(params.get("c", 1.0)), float(params.get("beta", -0.5)), js,
+              _lib.ptr(part), _lib.stream_handle())
+    s = w64.sum()
+    return float(torch.sqrt((part.sum(0) * w64).sum() / (s * s)).item())

@@ -1,0 +1,45 @@
+"""GPU kernel Stein discrepancy vs the FP64 oracle (mocat/src/metrics.py:88-130)."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import ksd as ok
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def m():
+    import mocat_amd
+    return mocat_amd
+
+
+@pytest.mark.parametrize("n,d", [(1, 4), (63, 3), (65, 1), (500, 40), (300, 100)])
+@pytest.mark.parametrize("kern", ["gaussian", "imq"])
+@pytest.mark.parametrize("weighted", [False, True])
+def test_ksd_vs_oracle(m, n, d, kern, weighted):
+    rs = np.random.default_rng(n * 7 + d)
+    x = rs.normal(0, 1.5, (n, d)).astype(np.float32)
+    g = (x + rs.normal(0, 0.3, (n, d))).astype(np.float32)
+    lw = rs.normal(0, 1, n).astype(np.float32) if weighted else None
+    if kern == "gaussian":
+        kernel, prm = m.kernels.Gaussian(bandwidth=1.3 * np.sqrt(d)), dict(bandwidth=1.3 * np.sqrt(d))
+    else:
+        kernel, prm = m.kernels.IMQ(c=0.8, beta=-0.6, bandwidth=2.0), dict(c=0.8, beta=-0.6, bandwidth=2.0)
+    got = m.ksd(m.cdict(value=torch.tensor(x), grad_potential=torch.tensor(g)), kernel,
+                log_weight=None if lw is None else torch.tensor(lw))
+    want = ok.ksd(x, g, kern, log_weight=lw, **prm)
+    assert got == pytest.approx(want, rel=2e-4), (got, want)
+
+
+def test_ksd_kernel_params_override_and_reference_pattern(m):
+    """mocat/src/tests/test_metrics.py:102-115: larger ensembles give a smaller KSD, and keyword
+    parameters override the kernel's own."""
+    rs = np.random.default_rng(0)
+    small, large = rs.normal(size=(10, 2)).astype(np.float32), rs.normal(size=(1000, 2)).astype(np.float32)
+    k = m.kernels.Gaussian(bandwidth=1.0)
+    for h in (1.0, 10.0):
+        a = m.ksd(torch.tensor(small), k, grad_potential=torch.tensor(small), bandwidth=h)
+        b = m.ksd(torch.tensor(large), k, grad_potential=torch.tensor(large), bandwidth=h)
+        assert b < a
+        assert b == pytest.approx(ok.ksd(large, large, "gaussian", bandwidth=h), rel=2e-4)
