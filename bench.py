@@ -318,7 +318,7 @@ def main():
         "chain_iterations_per_s": a.chains * K / wall,
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak, "traffic": traffic,
-                     "kernel": "hmc_lr_kernel<Bf16Engine<64,256>>" if a.precision == "bf16x3"
+                     "kernel": "hmc_lr_kernel<Bf16Engine<64,512>>" if a.precision == "bf16x3"
                      else "hmc_lr_kernel<Fp32Engine<64,512>>",
                      "kernel_ms": kernel_ms, "flop_per_launch": flop_per_launch,
                      "issued_mfma_flop_per_launch": (3 * 4.0 * 64 * (math.ceil(N_ROWS / 32) * 32) * LEAPFROG

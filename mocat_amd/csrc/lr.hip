@@ -184,107 +184,12 @@ struct Bf16Engine {
   using L = PackLayout<DP>;
   static constexpr int KB = L::KB, NB = L::NB, NW = NT / kWave, HW = NW / 2;
   static_assert(NW % 2 == 0, "Bf16Engine needs an even wave count");
-  static constexpr bool kPipelined = DP <= 64;
+  static constexpr bool kPrefetch = DP <= 64;
   struct alignas(16) Smem {
     alignas(16) float gred[HW][CT * DP];
     alignas(16) bf16x8 thl[2][KB][64];  // theta^T operand, hi and lo parts, per lane
     float ured[NW][2][CT];
   };
-
-  // Operands of one 32-row block: A (X rows, hi/lo) with y in accumulator-row order; B (X in the
-  // second GEMM's layout, hi/lo).
-  struct ABlk {
-    bf16x8 ah[KB], al[KB];
-    floatx4 yq[4];
-  };
-  struct BBlk {
-    bf16x8 xbh[2][NB], xbl[2][NB];
-  };
-  static MOCAT_DEV void load_ablk(ABlk& a, const char* base, int rb, int lane, int g) {
-    const char* blk = base + (int64_t)rb * L::BLOCK_BYTES;
-    const bf16x8* A = reinterpret_cast<const bf16x8*>(blk);
-#pragma unroll
-    for (int kb = 0; kb < KB; ++kb) {
-      a.ah[kb] = A[kb * 64 + lane];
-      a.al[kb] = A[(KB + kb) * 64 + lane];
-    }
-    const floatx4* yv4 = reinterpret_cast<const floatx4*>(blk + L::BLOCK_BF16 * 2 + g * 64);
-#pragma unroll
-    for (int q = 0; q < 4; ++q) a.yq[q] = yv4[q];
-  }
-  static MOCAT_DEV void load_bblk(BBlk& b, const char* base, int rb, int lane) {
-    const bf16x8* B = reinterpret_cast<const bf16x8*>(base + (int64_t)rb * L::BLOCK_BYTES) + L::A_ELEMS / 8;
-#pragma unroll
-    for (int kb2 = 0; kb2 < 2; ++kb2)
-#pragma unroll
-      for (int nb = 0; nb < NB; ++nb) {
-        b.xbh[kb2][nb] = B[((0 * 2 + kb2) * NB + nb) * 64 + lane];
-        b.xbl[kb2][nb] = B[((1 * 2 + kb2) * NB + nb) * 64 + lane];
-      }
-  }
-  // Z = X_block Theta^T (hi*hi + hi*lo + lo*hi); theta operands from registers
-  static MOCAT_DEV floatx16 gemm1(const ABlk& a, const bf16x8* thh, const bf16x8* thlo) {
-    floatx16 z;
-#pragma unroll
-    for (int r = 0; r < 16; ++r) z[r] = 0.f;
-#pragma unroll
-    for (int kb = 0; kb < KB; ++kb) {
-      z = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a.ah[kb], thh[kb], z, 0, 0, 0);
-      z = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a.ah[kb], thlo[kb], z, 0, 0, 0);
-      z = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a.al[kb], thh[kb], z, 0, 0, 0);
-    }
-    return z;
-  }
-  // the same with theta operands read from LDS per use (large DP: not enough registers to hold them)
-  static MOCAT_DEV floatx16 gemm1(const ABlk& a, const bf16x8 (*thh)[64], const bf16x8 (*thlo)[64], int lane) {
-    floatx16 z;
-#pragma unroll
-    for (int r = 0; r < 16; ++r) z[r] = 0.f;
-#pragma unroll
-    for (int kb = 0; kb < KB; ++kb) {
-      const bf16x8 th = thh[kb][lane], tl = thlo[kb][lane];
-      z = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a.ah[kb], th, z, 0, 0, 0);
-      z = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a.ah[kb], tl, z, 0, 0, 0);
-      z = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a.al[kb], th, z, 0, 0, 0);
-    }
-    return z;
-  }
-  // R = sigmoid(Z) - y (and the potential rows when asked), split to bf16 hi/lo in the A-operand
-  // order of the second GEMM, then G += R^T X_block. row0: first data row of the block (rows >= n
-  // are padding; their potential terms are masked, their residuals are 0 by the packing).
-  static MOCAT_DEV void elem_gemm2(const floatx16& z, const floatx4* yq, const BBlk& b, floatx16* gacc,
-                                   bool want_pot, int row0, int n, int lane, float& upart) {
-    bf16x8 rh[2], rl[2];
-    if (want_pot) {
-#pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        float cc;
-        const float rr = lr_row_full(z[r], yq[r >> 2][r & 3], cc);
-        upart += (row0 + acc_row(r, lane) < n) ? cc : 0.f;
-        __bf16 h, l;
-        split_bf16(rr, h, l);
-        rh[r >> 3][r & 7] = h;
-        rl[r >> 3][r & 7] = l;
-      }
-    } else {
-#pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        const float rr = lr_row_grad(z[r], yq[r >> 2][r & 3]);
-        __bf16 h, l;
-        split_bf16(rr, h, l);
-        rh[r >> 3][r & 7] = h;
-        rl[r >> 3][r & 7] = l;
-      }
-    }
-#pragma unroll
-    for (int kb2 = 0; kb2 < 2; ++kb2)
-#pragma unroll
-      for (int nb = 0; nb < NB; ++nb) {
-        gacc[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(rh[kb2], b.xbh[kb2][nb], gacc[nb], 0, 0, 0);
-        gacc[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(rh[kb2], b.xbl[kb2][nb], gacc[nb], 0, 0, 0);
-        gacc[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(rl[kb2], b.xbh[kb2][nb], gacc[nb], 0, 0, 0);
-      }
-  }
 
   // want_pot = false: gradient only (the leapfrog needs the potential only after its last step)
   static MOCAT_DEV void eval(Smem& s, const float*, const float*, const void* packed, int n, int d,
@@ -317,59 +222,85 @@ struct Bf16Engine {
       for (int r = 0; r < 16; ++r) gacc[nb][r] = 0.f;
     float upart = 0.f;
 
-    if constexpr (kPipelined) {
-      // Software pipeline over this wave's row blocks b_0, b_1, ... (rows of block i: (w + i NW) * 32):
-      // Z of block i+1 (first GEMM) is issued before the elementwise step of block i, so the matrix
-      // core works on it while the vector ALU forms sigmoid(Z_i) - y_i; A operands and y are loaded
-      // two blocks ahead (two register sets), B operands one block ahead.
-      bf16x8 thh[KB], thlo[KB];
+    // software pipeline: the A operands (and y) of the next row block are loaded while the current
+    // block runs its MFMAs and elementwise work
+    bf16x8 ah[KB], al[KB];
+    floatx4 yq[4];
+    auto load_a = [&](int rb) {
+      const char* blk = base + (int64_t)rb * L::BLOCK_BYTES;
+      const bf16x8* A = reinterpret_cast<const bf16x8*>(blk);
 #pragma unroll
       for (int kb = 0; kb < KB; ++kb) {
-        thh[kb] = s.thl[0][kb][lane];
-        thlo[kb] = s.thl[1][kb][lane];
+        ah[kb] = A[(0 * KB + kb) * 64 + lane];
+        al[kb] = A[(1 * KB + kb) * 64 + lane];
       }
-      const int nmine = w < nrb ? (nrb - 1 - w) / NW + 1 : 0;
-      ABlk a0, a1;
-      BBlk bb;
-      floatx16 zc, zn;
-      floatx4 yc[4], yn[4];
-      if (nmine > 0) {
-        load_ablk(a0, base, w, lane, g);
-        load_bblk(bb, base, w, lane);
-      }
-      if (nmine > 1) load_ablk(a1, base, w + NW, lane, g);
-      if (nmine > 0) {
-        zc = gemm1(a0, thh, thlo);
+    };
+    if (kPrefetch && w < nrb) load_a(w);
+    for (int rb = w; rb < nrb; rb += NW) {
+      if (!kPrefetch) load_a(rb);
+      const char* blk = base + (int64_t)rb * L::BLOCK_BYTES;
+      const bf16x8* B = reinterpret_cast<const bf16x8*>(blk) + L::A_ELEMS / 8;
+      floatx16 z;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) yc[q] = a0.yq[q];
-      }
-      for (int i = 0; i < nmine; i += 2) {
-        if (i + 2 < nmine) load_ablk(a0, base, w + (i + 2) * NW, lane, g);
-        if (i + 1 < nmine) {
-          zn = gemm1(a1, thh, thlo);
+      for (int r = 0; r < 16; ++r) z[r] = 0.f;
 #pragma unroll
-          for (int q = 0; q < 4; ++q) yn[q] = a1.yq[q];
+      for (int kb = 0; kb < KB; ++kb) {
+        const bf16x8 th = s.thl[0][kb][lane], tl = s.thl[1][kb][lane];
+        z = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[kb], th, z, 0, 0, 0);
+        z = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[kb], tl, z, 0, 0, 0);
+        z = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[kb], th, z, 0, 0, 0);
+      }
+      const floatx4* yv4 = reinterpret_cast<const floatx4*>(blk + L::BLOCK_BF16 * 2 + g * 64);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) yq[q] = yv4[q];
+      float yv[16];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        yv[4 * q] = yq[q].x;
+        yv[4 * q + 1] = yq[q].y;
+        yv[4 * q + 2] = yq[q].z;
+        yv[4 * q + 3] = yq[q].w;
+      }
+      // B operands of this block, then the next block's A operands
+      bf16x8 xbh[2][NB], xbl[2][NB];
+#pragma unroll
+      for (int kb2 = 0; kb2 < 2; ++kb2)
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) {
+          xbh[kb2][nb] = B[((0 * 2 + kb2) * NB + nb) * 64 + lane];
+          xbl[kb2][nb] = B[((1 * 2 + kb2) * NB + nb) * 64 + lane];
         }
-        elem_gemm2(zc, yc, bb, gacc, want_pot, (w + i * NW) * 32, n, lane, upart);
-        if (i + 1 >= nmine) break;
-        load_bblk(bb, base, w + (i + 1) * NW, lane);
-        if (i + 3 < nmine) load_ablk(a1, base, w + (i + 3) * NW, lane, g);
-        if (i + 2 < nmine) {
-          zc = gemm1(a0, thh, thlo);
+      if (kPrefetch && rb + NW < nrb) load_a(rb + NW);
+      bf16x8 rh[2], rl[2];
+      if (want_pot) {
 #pragma unroll
-          for (int q = 0; q < 4; ++q) yc[q] = a0.yq[q];
+        for (int r = 0; r < 16; ++r) {
+          float cc;
+          const float rr = lr_row_full(z[r], yv[r], cc);
+          upart += (rb * 32 + acc_row(r, lane) < n) ? cc : 0.f;
+          __bf16 h, l;
+          split_bf16(rr, h, l);
+          rh[r >> 3][r & 7] = h;
+          rl[r >> 3][r & 7] = l;
         }
-        elem_gemm2(zn, yn, bb, gacc, want_pot, (w + (i + 1) * NW) * 32, n, lane, upart);
-        if (i + 2 < nmine) load_bblk(bb, base, w + (i + 2) * NW, lane);
+      } else {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const float rr = lr_row_grad(z[r], yv[r]);
+          __bf16 h, l;
+          split_bf16(rr, h, l);
+          rh[r >> 3][r & 7] = h;
+          rl[r >> 3][r & 7] = l;
+        }
       }
-    } else {
-      ABlk a;
-      BBlk bb;
-      for (int rb = w; rb < nrb; rb += NW) {
-        load_ablk(a, base, rb, lane, g);
-        const floatx16 z = gemm1(a, s.thl[0], s.thl[1], lane);
-        load_bblk(bb, base, rb, lane);
-        elem_gemm2(z, a.yq, bb, gacc, want_pot, rb * 32, n, lane, upart);
+#pragma unroll
+      for (int kb2 = 0; kb2 < 2; ++kb2) {
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) {
+          gacc[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(rh[kb2], xbh[kb2][nb], gacc[nb], 0, 0, 0);
+          gacc[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(rh[kb2], xbl[kb2][nb], gacc[nb], 0, 0, 0);
+          gacc[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(rl[kb2], xbh[kb2][nb], gacc[nb], 0, 0, 0);
+        }
       }
     }
     // deterministic cross-wave reduction: upper half adds into lower half, then ordered sum
@@ -768,32 +699,29 @@ __global__ __launch_bounds__(NT) void hmc_lr_kernel(HmcArgs h) {
   }
 }
 
-// Threads per tile: the vector-ALU engines use 8 waves; the MFMA engine 4 (one wave per SIMD, so
-// each wave has the full register file for its two in-flight row blocks).
 constexpr int kThreads = 512;
-constexpr int kMfmaThreads = 256;
 
-template <template <int, int> class E, int NT>
+template <template <int, int> class E>
 int dispatch_eval(const LrArgs& a, const float* theta, int64_t C, float* pp, float* lp, float* gp, float* gl,
                   hipStream_t st) {
   const unsigned blocks = (unsigned)((C + CT - 1) / CT);
   if (a.d <= 64)
-    hipLaunchKernelGGL((lr_eval_kernel<E<64, NT>, 64, NT>), dim3(blocks), dim3(NT), 0, st, a, theta, C, pp, lp, gp,
-                       gl);
+    hipLaunchKernelGGL((lr_eval_kernel<E<64, kThreads>, 64, kThreads>), dim3(blocks), dim3(kThreads), 0, st, a,
+                       theta, C, pp, lp, gp, gl);
   else
-    hipLaunchKernelGGL((lr_eval_kernel<E<128, NT>, 128, NT>), dim3(blocks), dim3(NT), 0, st, a, theta, C, pp, lp, gp,
-                       gl);
+    hipLaunchKernelGGL((lr_eval_kernel<E<128, kThreads>, 128, kThreads>), dim3(blocks), dim3(kThreads), 0, st, a,
+                       theta, C, pp, lp, gp, gl);
   MOCAT_LAUNCH_CHECK();
   return MOCAT_OK;
 }
 
-template <template <int, int> class E, int NT>
+template <template <int, int> class E>
 int dispatch_hmc(const HmcArgs& h, hipStream_t st) {
   const unsigned blocks = (unsigned)((h.C + CT - 1) / CT);
   if (h.lr.d <= 64)
-    hipLaunchKernelGGL((hmc_lr_kernel<E<64, NT>, 64, NT>), dim3(blocks), dim3(NT), 0, st, h);
+    hipLaunchKernelGGL((hmc_lr_kernel<E<64, kThreads>, 64, kThreads>), dim3(blocks), dim3(kThreads), 0, st, h);
   else
-    hipLaunchKernelGGL((hmc_lr_kernel<E<128, NT>, 128, NT>), dim3(blocks), dim3(NT), 0, st, h);
+    hipLaunchKernelGGL((hmc_lr_kernel<E<128, kThreads>, 128, kThreads>), dim3(blocks), dim3(kThreads), 0, st, h);
   MOCAT_LAUNCH_CHECK();
   return MOCAT_OK;
 }
@@ -844,10 +772,9 @@ int mocat_lr_potential_grad(const float* X, const float* y, const void* packed, 
   if (C == 0) return MOCAT_OK;
   LrArgs a{X, y, packed, (int)n, (int)d, prior_prec, temperature};
   if (precision == 2)
-    return dispatch_eval<GaussEngine, kThreads>(a, theta, C, prior_pot, lik_pot, grad_prior, grad_lik, stream);
-  return precision == 0
-             ? dispatch_eval<Fp32Engine, kThreads>(a, theta, C, prior_pot, lik_pot, grad_prior, grad_lik, stream)
-             : dispatch_eval<Bf16Engine, kMfmaThreads>(a, theta, C, prior_pot, lik_pot, grad_prior, grad_lik, stream);
+    return dispatch_eval<GaussEngine>(a, theta, C, prior_pot, lik_pot, grad_prior, grad_lik, stream);
+  return precision == 0 ? dispatch_eval<Fp32Engine>(a, theta, C, prior_pot, lik_pot, grad_prior, grad_lik, stream)
+                        : dispatch_eval<Bf16Engine>(a, theta, C, prior_pot, lik_pot, grad_prior, grad_lik, stream);
 }
 
 int mocat_hmc_lr_iterate(const float* X, const float* y, const void* packed, int64_t n, int64_t d, int precision,
@@ -867,9 +794,8 @@ int mocat_hmc_lr_iterate(const float* X, const float* y, const void* packed, int
             C, theta, momenta, grad, potential, prior_pot, lik_pot, stepsize, alpha, keys, iter0, n_iter,
             leapfrog_steps, friction, correction, target, rm_scale, rm_neg_exponent, adapt_cut_off,
             trace_value, trace_potential, trace_alpha, trace_stepsize, trace_momenta};
-  if (precision == 2) return dispatch_hmc<GaussEngine, kThreads>(h, stream);
-  return precision == 0 ? dispatch_hmc<Fp32Engine, kThreads>(h, stream)
-                        : dispatch_hmc<Bf16Engine, kMfmaThreads>(h, stream);
+  if (precision == 2) return dispatch_hmc<GaussEngine>(h, stream);
+  return precision == 0 ? dispatch_hmc<Fp32Engine>(h, stream) : dispatch_hmc<Bf16Engine>(h, stream);
 }
 
 }  // extern "C"
