@@ -106,8 +106,10 @@ typedef struct {
   int64_t t_offset;      /* steps into out_ess / out_loglik_t */
   uint32_t key[2];       /* random_key of the filter */
   int32_t param_offset;  /* floats into params */
-  int32_t reserved;
+  int32_t flags;         /* bit 0 (MOCAT_PF_LARGE): run by mocat_pf_run_large, skipped by mocat_pf_run */
 } mocat_pf_filter;
+
+#define MOCAT_PF_LARGE 1
 
 int mocat_pf_param_count(int model, int d, int dy, int64_t* count);
 int mocat_pf_run(int model, int d, int dy, const mocat_pf_filter* filters, int n_filters, const float* params,
@@ -115,6 +117,22 @@ int mocat_pf_run(int model, int d, int dy, const mocat_pf_filter* filters, int n
                  int32_t* anc_ws, int resampling, float ess_threshold, float* out_value, float* out_lw,
                  int32_t* out_anc, float* out_ess, double* out_loglik_t, double* out_loglik, float* tan_ws0,
                  float* tan_ws1, double* out_grad, hipStream_t stream);
+
+/* The same filters (same run_particle_filter_for_marginals semantics, outputs and workspaces as
+ * mocat_pf_run) for those flagged MOCAT_PF_LARGE, spread over the whole GPU: per time step four
+ * grid-wide launches over 256-particle chunks, all flagged filters batched in each (statistics and
+ * resampling decision; quantised weight totals; exact prefix sums; ancestor search, propagation and
+ * weighting). For the same log-weights and key the ancestors equal mocat_pf_run's. host_filters is a
+ * host array (the launch sequence depends on n and T); workspace holds
+ * mocat_pf_large_workspace_bytes bytes. Calls on one stream may follow mocat_pf_run with the same
+ * arguments: the two paths touch disjoint filters. */
+int mocat_pf_large_workspace_bytes(const mocat_pf_filter* host_filters, int n_filters, int64_t* bytes);
+int mocat_pf_run_large(int model, int d, int dy, const mocat_pf_filter* host_filters, int n_filters,
+                       const float* params,
+                       const float* y, float* x_ws0, float* x_ws1, float* lw_ws0, float* lw_ws1, uint64_t* csum_ws,
+                       int resampling, float ess_threshold, float* out_value, float* out_lw, int32_t* out_anc,
+                       float* out_ess, double* out_loglik_t, double* out_loglik, float* tan_ws0, float* tan_ws1,
+                       double* out_grad, void* workspace, int64_t workspace_bytes, hipStream_t stream);
 
 /* Ancestor indices idx[n] from non-negative weights (from_log = 0) or log-weights (from_log = 1,
  * weights exp(w - max w)): method 0 systematic (u = uniform(key)), 1 multinomial

@@ -19,6 +19,9 @@
 //   multinomial  T_j = u_j * W               u = uniform(key, (n,))
 // The reference's resampler, random.categorical = Gumbel-max over an (n, n) Gumbel matrix
 // (filtering.py:196-199), is also provided (O(n^2)).
+#include <algorithm>
+#include <vector>
+
 #include "common.hpp"
 
 using namespace mocat;
@@ -33,7 +36,7 @@ typedef struct {
   int64_t t_offset;
   uint32_t key[2];
   int32_t param_offset;
-  int32_t reserved;
+  int32_t flags;
 } mocat_pf_filter;
 }
 
@@ -41,6 +44,7 @@ namespace {
 
 constexpr int NT = 512;
 constexpr int NW = NT / kWave;
+constexpr int32_t kPfLarge = 1;  // mocat_pf_filter.flags: run by mocat_pf_run_large, skipped by mocat_pf_run
 constexpr double kLog2PiD = 1.83787706640934548356;
 
 // ---------------------------------------------------------------- models
@@ -428,6 +432,7 @@ __global__ __launch_bounds__(NT) void pf_kernel(PfArgs a) {
   const int tid = threadIdx.x;
   for (int f = blockIdx.x; f < a.n_filters; f += gridDim.x) {
     const mocat_pf_filter F = a.filters[f];
+    if (F.flags & kPfLarge) continue;
     const int64_t n = F.n, T = F.T;
     __syncthreads();
     if (tid == 0) model.load(a.params + F.param_offset);
@@ -634,12 +639,493 @@ __global__ __launch_bounds__(256) void categorical_kernel(const float* __restric
     idx[j] = categorical_row(lw, n, k, S, j);
 }
 
+// ---------------------------------------------------------------- large filters: grid-wide launches per step
+// One workgroup per filter leaves a large filter running long after the others have finished. Filters
+// flagged kPfLarge instead advance one time step per sequence of four launches over 256-particle
+// chunks (all flagged filters batched in every launch):
+//   lf_stats   one workgroup per filter: combine the chunk accumulators of step t in chunk order,
+//              log-likelihood and ESS outputs, the resampling decision and weight shift for step t+1;
+//   lf_totals  per chunk: sum of the quantised weights (resampling steps only);
+//   lf_csum    per chunk: offset = sum of the earlier chunks' totals, plus an inclusive block scan;
+//   lf_step    per chunk: ancestor (binary search in the exact prefix sums, or Gumbel-max), then
+//              propagate, weight, outputs and the chunk accumulators of step t+1.
+// The weights are shifted by float(lse) and quantised with the shift of the largest weight exactly as
+// in pf_kernel, so for the same log-weights and key the ancestors are the same.
+constexpr int LT = 256;
+constexpr int LW = LT / kWave;
+
+struct LfDesc {
+  int64_t n, T, y_offset, state_offset, out_offset, t_offset, chunk0;
+  uint32_t key[2];
+  int32_t param_offset, f;
+};
+
+struct LfState {
+  double lse_res;  // lse of the weights the next increments add to (log n after resampling)
+  double loglik;
+  double gprev[3], gtot[3];
+  float m;    // weight shift, float(lse of the last step)
+  int res;    // resample at the next step
+  int shift;  // quantisation shift of the next resampling
+  int pad;
+};
+
+struct LfArgs {
+  const LfDesc* desc;
+  const int32_t* chunk_filter;  // chunk -> index into desc
+  int n_large;
+  int64_t n_chunks;
+  const float* params;
+  const float* y;
+  float* x0;
+  float* x1;
+  float* lw0;
+  float* lw1;
+  uint64_t* csum;
+  float* tan0;
+  float* tan1;
+  int resampling;
+  float ess_threshold;
+  float* out_value;
+  float* out_lw;
+  int32_t* out_anc;
+  float* out_ess;
+  double* out_loglik_t;
+  double* out_loglik;
+  double* out_grad;
+  float* part_m;     // [n_chunks] chunk LSE accumulators (max, sums in FP64)
+  double* part_s1;   // [n_chunks]
+  double* part_s2;   // [n_chunks]
+  double* part_g;    // [n_chunks][3] tangent sums (GRAD)
+  uint64_t* ctot;    // [n_chunks] quantised weight totals
+  LfState* st;       // [n_large]
+};
+
+struct LfScratch {
+  float fm[LW];
+  double d1[LW], d2[LW], g[LW][3];
+  uint64_t u[LW];
+};
+
+// chunk-level reductions in a fixed order (wave butterflies, then waves in order)
+MOCAT_DEV LseAcc lf_block_lse(LseAcc a, LfScratch& s) {
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  for (int o = kWave / 2; o > 0; o >>= 1) {
+    LseAcc b;
+    b.m = __shfl_xor(a.m, o, kWave);
+    b.s1 = __shfl_xor(a.s1, o, kWave);
+    b.s2 = __shfl_xor(a.s2, o, kWave);
+    a = lse_combine(a, b);
+  }
+  __syncthreads();
+  if (lane == 0) {
+    s.fm[w] = a.m;
+    s.d1[w] = a.s1;
+    s.d2[w] = a.s2;
+  }
+  __syncthreads();
+  LseAcc r{s.fm[0], s.d1[0], s.d2[0]};
+  for (int i = 1; i < LW; ++i) r = lse_combine(r, LseAcc{s.fm[i], s.d1[i], s.d2[i]});
+  return r;
+}
+
+MOCAT_DEV GradAcc lf_block_grad(GradAcc a, LfScratch& s) {
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  for (int o = kWave / 2; o > 0; o >>= 1) {
+    GradAcc b;
+    b.m = __shfl_xor(a.m, o, kWave);
+    b.s1 = __shfl_xor(a.s1, o, kWave);
+    b.g0 = __shfl_xor(a.g0, o, kWave);
+    b.g1 = __shfl_xor(a.g1, o, kWave);
+    b.g2 = __shfl_xor(a.g2, o, kWave);
+    a = grad_combine(a, b);
+  }
+  __syncthreads();
+  if (lane == 0) {
+    s.fm[w] = a.m;
+    s.d1[w] = a.s1;
+    s.g[w][0] = a.g0;
+    s.g[w][1] = a.g1;
+    s.g[w][2] = a.g2;
+  }
+  __syncthreads();
+  GradAcc r{s.fm[0], s.d1[0], s.g[0][0], s.g[0][1], s.g[0][2]};
+  for (int i = 1; i < LW; ++i) r = grad_combine(r, GradAcc{s.fm[i], s.d1[i], s.g[i][0], s.g[i][1], s.g[i][2]});
+  return r;
+}
+
+MOCAT_DEV uint64_t lf_block_sum(uint64_t v, LfScratch& s) {
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  v = wave_sum(v);
+  __syncthreads();
+  if (lane == 0) s.u[w] = v;
+  __syncthreads();
+  uint64_t r = 0;
+  for (int i = 0; i < LW; ++i) r += s.u[i];
+  return r;
+}
+
+MOCAT_DEV uint64_t lf_block_inclusive_scan(uint64_t v, LfScratch& s) {
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  const uint64_t incl = wave_inclusive_scan(v);
+  __syncthreads();
+  if (lane == kWave - 1) s.u[w] = incl;
+  __syncthreads();
+  uint64_t off = 0;
+  for (int i = 0; i < w; ++i) off += s.u[i];
+  return off + incl;
+}
+
+// Stores a chunk's accumulators (LSE and, with GRAD, the weighted tangent sums relative to its max).
+template <bool GRAD>
+MOCAT_DEV void lf_store_partials(const LfArgs& a, int64_t c, LseAcc acc, GradAcc gacc, LfScratch& s) {
+  const LseAcc r = lf_block_lse(acc, s);
+  GradAcc gr{};
+  if constexpr (GRAD) gr = lf_block_grad(gacc, s);
+  if (threadIdx.x == 0) {
+    a.part_m[c] = r.m;
+    a.part_s1[c] = r.s1;
+    a.part_s2[c] = r.s2;
+    if constexpr (GRAD) {
+      // rescale the tangent sums to the LSE accumulator's max (both track the same weights)
+      const double f = (gr.m == -INFINITY || r.m == -INFINITY) ? 0.0 : exp((double)gr.m - (double)r.m);
+      a.part_g[c * 3 + 0] = gr.g0 * f;
+      a.part_g[c * 3 + 1] = gr.g1 * f;
+      a.part_g[c * 3 + 2] = gr.g2 * f;
+    }
+  }
+}
+
+template <class Model, int D, int DY, bool GRAD>
+__global__ __launch_bounds__(LT) void lf_init_kernel(LfArgs a) {
+  __shared__ LfScratch s;
+  __shared__ Model model;
+  const int64_t c = blockIdx.x;
+  const LfDesc F = a.desc[a.chunk_filter[c]];
+  const int tid = threadIdx.x;
+  if (tid == 0) model.load(a.params + F.param_offset);
+  __syncthreads();
+  const int64_t n = F.n, i = (c - F.chunk0) * LT + tid;
+  const Key key{F.key[0], F.key[1]};
+  const Key sub = split_at(key, 2, 1);
+  constexpr uint32_t hD = (D + 1) / 2;
+  LseAcc acc{-INFINITY, 0.0, 0.0};
+  GradAcc gacc{-INFINITY, 0.0, 0.0, 0.0, 0.0};
+  if (i < n) {
+    const Key ki = split_at(sub, (uint32_t)n, (uint32_t)i);
+    float eps[D], x[D];
+#pragma unroll
+    for (uint32_t q = 0; q < hD; ++q) {
+      uint32_t w0, w1;
+      bits_pair(ki, D, q, w0, w1);
+      eps[q] = normal_from_bits(w0);
+      if (q + hD < (uint32_t)D) eps[q + hD] = normal_from_bits(w1);
+    }
+    model.init(eps, x);
+    float l;
+    if constexpr (GRAD) {
+      float dldx, dldb;
+      l = -model.lik_grad(x, a.y + F.y_offset, dldx, dldb);
+      float* ta = a.tan0 + F.state_offset * 5;
+      ta[i * 5 + 0] = 0.f;
+      ta[i * 5 + 1] = 0.f;
+      ta[i * 5 + 2] = 0.f;
+      ta[i * 5 + 3] = 0.f;
+      ta[i * 5 + 4] = -dldb;
+      grad_add(gacc, l, 0.f, 0.f, -dldb);
+    } else {
+      l = -model.lik(x, a.y + F.y_offset);
+    }
+    float* xa = a.x0 + F.state_offset * D;
+#pragma unroll
+    for (int q = 0; q < D; ++q) xa[i * D + q] = x[q];
+    a.lw0[F.state_offset + i] = l;
+    lse_add(acc, l);
+    if (F.out_offset >= 0) {
+      if (a.out_value)
+#pragma unroll
+        for (int q = 0; q < D; ++q) a.out_value[(F.out_offset + i) * D + q] = x[q];
+      if (a.out_lw) a.out_lw[F.out_offset + i] = l;
+      if (a.out_anc) a.out_anc[F.out_offset + i] = (int32_t)i;
+    }
+  }
+  lf_store_partials<GRAD>(a, c, acc, gacc, s);
+}
+
+// Step t's statistics (t = 0 after lf_init, else after lf_step(t)).
+template <bool GRAD>
+__global__ __launch_bounds__(LT) void lf_stats_kernel(LfArgs a, int64_t t) {
+  __shared__ LfScratch s;
+  const int fi = blockIdx.x;
+  const LfDesc F = a.desc[fi];
+  if (t >= F.T) return;
+  const int64_t n = F.n, nchunk = (n + LT - 1) / LT;
+  LseAcc acc{-INFINITY, 0.0, 0.0};
+  double g[3] = {0.0, 0.0, 0.0};
+  for (int64_t k = threadIdx.x; k < nchunk; k += LT) {
+    const int64_t c = F.chunk0 + k;
+    const LseAcc p{a.part_m[c], a.part_s1[c], a.part_s2[c]};
+    if constexpr (GRAD) {
+      // the tangent sums follow the LSE rescaling of their chunk
+      const float mnew = fmaxf(acc.m, p.m);
+      const double fo = acc.m == -INFINITY ? 0.0 : exp((double)acc.m - (double)mnew);
+      const double fp = p.m == -INFINITY ? 0.0 : exp((double)p.m - (double)mnew);
+      for (int q = 0; q < 3; ++q) g[q] = g[q] * fo + a.part_g[c * 3 + q] * fp;
+    }
+    acc = lse_combine(acc, p);
+  }
+  const LseAcc r = lf_block_lse(acc, s);
+  double gm[3] = {0.0, 0.0, 0.0};
+  if constexpr (GRAD) {
+    // bring every thread's tangent sums to the block max, then add in thread order
+    const double f = acc.m == -INFINITY ? 0.0 : exp((double)acc.m - (double)r.m);
+    __shared__ double gs[LT][3];
+    for (int q = 0; q < 3; ++q) gs[threadIdx.x][q] = g[q] * f;
+    __syncthreads();
+    for (int k = 0; k < LT; ++k)
+      for (int q = 0; q < 3; ++q) gm[q] += gs[k][q];
+    for (int q = 0; q < 3; ++q) gm[q] /= r.s1;
+  }
+  if (threadIdx.x != 0) return;
+  double lse, lse2;
+  if (isnan(r.m)) {
+    lse = NAN;
+    lse2 = NAN;
+  } else if (r.m == -INFINITY) {
+    lse = -INFINITY;
+    lse2 = -INFINITY;
+  } else {
+    lse = (double)r.m + log(r.s1);
+    lse2 = 2.0 * (double)r.m + log(r.s2);
+  }
+  LfState st = a.st[fi];
+  const double logn = log((double)n);
+  st.loglik = t == 0 ? lse - logn : st.loglik + lse - st.lse_res;
+  const double ess = exp(2.0 * lse - lse2);
+  if (a.out_ess) a.out_ess[F.t_offset + t] = (float)ess;
+  if (a.out_loglik_t) a.out_loglik_t[F.t_offset + t] = st.loglik;
+  if constexpr (GRAD) {
+    for (int q = 0; q < 3; ++q) {
+      st.gtot[q] = t == 0 ? gm[q] : st.gtot[q] + gm[q] - (st.res ? 0.0 : st.gprev[q]);
+      st.gprev[q] = gm[q];
+    }
+  }
+  // decision and weight shift for step t + 1 (pf_kernel: m = float(lse), shift from the largest
+  // weight exp(lw - m), i.e. exp(max lw - m): the hardware exponential is monotone)
+  st.res = ess < (double)a.ess_threshold * (double)n ? 1 : 0;
+  st.lse_res = st.res ? logn : lse;
+  st.m = (float)lse;
+  st.shift = quant_shift(__expf(r.m - st.m), n);
+  a.st[fi] = st;
+  if (t == F.T - 1) {
+    a.out_loglik[F.f] = st.loglik;
+    if constexpr (GRAD)
+      for (int q = 0; q < 3; ++q) a.out_grad[(int64_t)F.f * 3 + q] = st.gtot[q];
+  }
+}
+
+MOCAT_DEV const float* lf_lw_prev(const LfArgs& a, int64_t t) { return ((t - 1) & 1) ? a.lw1 : a.lw0; }
+
+__global__ __launch_bounds__(LT) void lf_totals_kernel(LfArgs a, int64_t t) {
+  __shared__ LfScratch s;
+  const int64_t c = blockIdx.x;
+  const int fi = a.chunk_filter[c];
+  const LfDesc F = a.desc[fi];
+  if (t >= F.T) return;
+  const LfState st = a.st[fi];
+  if (!st.res || a.resampling == 2) return;
+  const int64_t i = (c - F.chunk0) * LT + threadIdx.x;
+  const float* lw = lf_lw_prev(a, t) + F.state_offset;
+  const uint64_t q = i < F.n ? quantise(__expf(lw[i] - st.m), st.shift) : 0ull;
+  const uint64_t tot = lf_block_sum(q, s);
+  if (threadIdx.x == 0) a.ctot[c] = tot;
+}
+
+__global__ __launch_bounds__(LT) void lf_csum_kernel(LfArgs a, int64_t t) {
+  __shared__ LfScratch s;
+  const int64_t c = blockIdx.x;
+  const int fi = a.chunk_filter[c];
+  const LfDesc F = a.desc[fi];
+  if (t >= F.T) return;
+  const LfState st = a.st[fi];
+  if (!st.res || a.resampling == 2) return;
+  const int64_t local = c - F.chunk0;
+  uint64_t off = 0;
+  for (int64_t k = threadIdx.x; k < local; k += LT) off += a.ctot[F.chunk0 + k];
+  off = lf_block_sum(off, s);
+  const int64_t i = local * LT + threadIdx.x;
+  const float* lw = lf_lw_prev(a, t) + F.state_offset;
+  const uint64_t q = i < F.n ? quantise(__expf(lw[i] - st.m), st.shift) : 0ull;
+  const uint64_t incl = lf_block_inclusive_scan(q, s);
+  if (i < F.n) a.csum[F.state_offset + i] = off + incl;
+}
+
+template <class Model, int D, int DY, bool GRAD>
+__global__ __launch_bounds__(LT) void lf_step_kernel(LfArgs a, int64_t t) {
+  __shared__ LfScratch s;
+  __shared__ Model model;
+  const int64_t c = blockIdx.x;
+  const int fi = a.chunk_filter[c];
+  const LfDesc F = a.desc[fi];
+  if (t >= F.T) return;
+  const int tid = threadIdx.x;
+  if (tid == 0) model.load(a.params + F.param_offset);
+  __syncthreads();
+  const LfState st = a.st[fi];
+  const int64_t n = F.n, T = F.T, i = (c - F.chunk0) * LT + tid;
+  const bool res = st.res != 0;
+  const Key key{F.key[0], F.key[1]};
+  const Key rkey = split_at(key, 2, 0);
+  const Key ik = split_at(rkey, (uint32_t)(T - 1), (uint32_t)(t - 1));
+  const bool odd = (t - 1) & 1;
+  const float* xa = (odd ? a.x1 : a.x0) + F.state_offset * D;
+  float* xb = (odd ? a.x0 : a.x1) + F.state_offset * D;
+  const float* lwa = (odd ? a.lw1 : a.lw0) + F.state_offset;
+  float* lwb = (odd ? a.lw0 : a.lw1) + F.state_offset;
+  const float* ta = GRAD ? (odd ? a.tan1 : a.tan0) + F.state_offset * 5 : nullptr;
+  float* tb = GRAD ? (odd ? a.tan0 : a.tan1) + F.state_offset * 5 : nullptr;
+  const float* ycur = a.y + F.y_offset + t * DY;
+  constexpr uint32_t hD = (D + 1) / 2;
+  LseAcc acc{-INFINITY, 0.0, 0.0};
+  GradAcc gacc{-INFINITY, 0.0, 0.0, 0.0, 0.0};
+  if (i < n) {
+    int64_t src = i;
+    if (res) {
+      if (a.resampling == 2) {
+        src = categorical_row(lwa, n, ik, (uint32_t)(n * n), i);
+      } else {
+        const uint64_t* C = a.csum + F.state_offset;
+        const double W = (double)C[n - 1];
+        double target;
+        if (a.resampling == 0) {
+          const double u = (double)uniform_from_bits(bits_at(ik, 1, 0), 0.f, 1.f);
+          target = ((double)i + u) * (W / (double)n);
+        } else {
+          const double u = (double)uniform_from_bits(bits_at(ik, (uint32_t)n, (uint32_t)i), 0.f, 1.f);
+          target = u * W;
+        }
+        src = search_csum(C, n, target);
+      }
+    }
+    const Key ki = split_at(ik, (uint32_t)n, (uint32_t)i);
+    float eps[D], xp[D], x[D];
+#pragma unroll
+    for (uint32_t q = 0; q < hD; ++q) {
+      uint32_t w0, w1;
+      bits_pair(ki, D, q, w0, w1);
+      eps[q] = normal_from_bits(w0);
+      if (q + hD < (uint32_t)D) eps[q + hD] = normal_from_bits(w1);
+    }
+#pragma unroll
+    for (int q = 0; q < D; ++q) xp[q] = xa[src * D + q];
+    model.transition(xp, eps, x);
+    float l;
+    if constexpr (GRAD) {
+      float dldx, dldb;
+      l = (res ? 0.f : lwa[i]) - model.lik_grad(x, ycur, dldx, dldb);
+      const float dxp = xp[0] + model.phi * ta[src * 5 + 0];
+      const float dxs = eps[0] + model.phi * ta[src * 5 + 1];
+      const float d0 = (res ? 0.f : ta[i * 5 + 2]) - dldx * dxp;
+      const float d1 = (res ? 0.f : ta[i * 5 + 3]) - dldx * dxs;
+      const float d2 = (res ? 0.f : ta[i * 5 + 4]) - dldb;
+      tb[i * 5 + 0] = dxp;
+      tb[i * 5 + 1] = dxs;
+      tb[i * 5 + 2] = d0;
+      tb[i * 5 + 3] = d1;
+      tb[i * 5 + 4] = d2;
+      grad_add(gacc, l, d0, d1, d2);
+    } else {
+      l = (res ? 0.f : lwa[i]) - model.lik(x, ycur);
+    }
+#pragma unroll
+    for (int q = 0; q < D; ++q) xb[i * D + q] = x[q];
+    lwb[i] = l;
+    lse_add(acc, l);
+    if (F.out_offset >= 0) {
+      const int64_t o = F.out_offset + t * n + i;
+      if (a.out_value)
+#pragma unroll
+        for (int q = 0; q < D; ++q) a.out_value[o * D + q] = x[q];
+      if (a.out_lw) a.out_lw[o] = l;
+      if (a.out_anc) a.out_anc[o] = (int32_t)src;
+    }
+  }
+  lf_store_partials<GRAD>(a, c, acc, gacc, s);
+}
+
+// Final particles of filters whose last step wrote buffer 1 are copied to buffer 0 (as pf_kernel).
+template <int D>
+__global__ __launch_bounds__(LT) void lf_finish_kernel(LfArgs a) {
+  const int64_t c = blockIdx.x;
+  const LfDesc F = a.desc[a.chunk_filter[c]];
+  if (!((F.T - 1) & 1)) return;
+  const int64_t i = (c - F.chunk0) * LT + threadIdx.x;
+  if (i >= F.n) return;
+#pragma unroll
+  for (int q = 0; q < D; ++q) a.x0[(F.state_offset + i) * D + q] = a.x1[(F.state_offset + i) * D + q];
+  a.lw0[F.state_offset + i] = a.lw1[F.state_offset + i];
+}
+
+template <class Model, int D, int DY, bool GRAD = false>
+int launch_pf_large(const LfArgs& a, int64_t max_T, hipStream_t st) {
+  const dim3 chunks((unsigned)a.n_chunks), filters((unsigned)a.n_large), blk(LT);
+  hipLaunchKernelGGL((lf_init_kernel<Model, D, DY, GRAD>), chunks, blk, 0, st, a);
+  hipLaunchKernelGGL((lf_stats_kernel<GRAD>), filters, blk, 0, st, a, (int64_t)0);
+  for (int64_t t = 1; t < max_T; ++t) {
+    hipLaunchKernelGGL(lf_totals_kernel, chunks, blk, 0, st, a, t);
+    hipLaunchKernelGGL(lf_csum_kernel, chunks, blk, 0, st, a, t);
+    hipLaunchKernelGGL((lf_step_kernel<Model, D, DY, GRAD>), chunks, blk, 0, st, a, t);
+    hipLaunchKernelGGL((lf_stats_kernel<GRAD>), filters, blk, 0, st, a, t);
+  }
+  hipLaunchKernelGGL((lf_finish_kernel<D>), chunks, blk, 0, st, a);
+  MOCAT_LAUNCH_CHECK();
+  return MOCAT_OK;
+}
+
 template <class Model, int D, int DY, bool GRAD = false>
 int launch_pf(const PfArgs& a, hipStream_t st) {
   const int grid = a.n_filters < 4096 ? a.n_filters : 4096;
   hipLaunchKernelGGL((pf_kernel<Model, D, DY, GRAD>), dim3(grid), dim3(NT), 0, st, a);
   MOCAT_LAUNCH_CHECK();
   return MOCAT_OK;
+}
+
+
+struct LfLayout {
+  int n_large = 0;
+  int64_t n_chunks = 0, max_T = 0;
+  int64_t o_desc, o_cf, o_m, o_s1, o_s2, o_g, o_tot, o_st, bytes;
+};
+
+int64_t lf_align(int64_t v) { return (v + 255) / 256 * 256; }
+
+LfLayout lf_layout(const mocat_pf_filter* filters, int n_filters) {
+  LfLayout L;
+  for (int f = 0; f < n_filters; ++f) {
+    if (!(filters[f].flags & kPfLarge)) continue;
+    ++L.n_large;
+    L.n_chunks += (filters[f].n + LT - 1) / LT;
+    L.max_T = std::max<int64_t>(L.max_T, filters[f].T);
+  }
+  int64_t o = 0;
+  L.o_desc = o;
+  o = lf_align(o + (int64_t)L.n_large * (int64_t)sizeof(LfDesc));
+  L.o_cf = o;
+  o = lf_align(o + L.n_chunks * 4);
+  L.o_m = o;
+  o = lf_align(o + L.n_chunks * 4);
+  L.o_s1 = o;
+  o = lf_align(o + L.n_chunks * 8);
+  L.o_s2 = o;
+  o = lf_align(o + L.n_chunks * 8);
+  L.o_g = o;
+  o = lf_align(o + L.n_chunks * 24);
+  L.o_tot = o;
+  o = lf_align(o + L.n_chunks * 8);
+  L.o_st = o;
+  o = lf_align(o + (int64_t)L.n_large * (int64_t)sizeof(LfState));
+  L.bytes = o;
+  return L;
 }
 
 }  // namespace
@@ -676,6 +1162,68 @@ int mocat_pf_run(int model, int d, int dy, const mocat_pf_filter* filters, int n
   if (model != 0) return MOCAT_ERR_ARG;
 #define MOCAT_PF_CASE(DD, YY) \
   if (d == DD && dy == YY) return launch_pf<Lgssm<DD, YY>, DD, YY>(a, stream);
+  MOCAT_PF_CASE(1, 1) MOCAT_PF_CASE(2, 2) MOCAT_PF_CASE(4, 4) MOCAT_PF_CASE(4, 2) MOCAT_PF_CASE(5, 5)
+  MOCAT_PF_CASE(8, 8) MOCAT_PF_CASE(2, 1) MOCAT_PF_CASE(3, 3) MOCAT_PF_CASE(4, 1) MOCAT_PF_CASE(8, 4)
+#undef MOCAT_PF_CASE
+  return MOCAT_ERR_ARG;
+}
+
+int mocat_pf_large_workspace_bytes(const mocat_pf_filter* host_filters, int n_filters, int64_t* bytes) {
+  if (!host_filters || n_filters < 0 || !bytes) return MOCAT_ERR_ARG;
+  *bytes = lf_layout(host_filters, n_filters).bytes;
+  return MOCAT_OK;
+}
+
+int mocat_pf_run_large(int model, int d, int dy, const mocat_pf_filter* host_filters, int n_filters,
+                       const float* params, const float* y, float* x_ws0, float* x_ws1, float* lw_ws0, float* lw_ws1, uint64_t* csum_ws,
+                       int resampling, float ess_threshold, float* out_value, float* out_lw, int32_t* out_anc,
+                       float* out_ess, double* out_loglik_t, double* out_loglik, float* tan_ws0, float* tan_ws1,
+                       double* out_grad, void* workspace, int64_t workspace_bytes, hipStream_t stream) {
+  if (n_filters < 0 || !host_filters || !params || !y || !x_ws0 || !x_ws1 || !lw_ws0 || !lw_ws1 || !csum_ws ||
+      !out_loglik || resampling < 0 || resampling > 2)
+    return MOCAT_ERR_ARG;
+  if (out_grad && (model != 1 || !tan_ws0 || !tan_ws1)) return MOCAT_ERR_ARG;
+  const LfLayout L = lf_layout(host_filters, n_filters);
+  if (L.n_large == 0) return MOCAT_OK;
+  if (!workspace || workspace_bytes < L.bytes) return MOCAT_ERR_SIZE;
+  if (L.n_chunks > 0x7FFFFFFFll || L.n_large > 0x7FFFFFFF) return MOCAT_ERR_SIZE;
+  std::vector<LfDesc> desc;
+  std::vector<int32_t> chunk_filter;
+  desc.reserve(L.n_large);
+  chunk_filter.reserve(L.n_chunks);
+  int64_t chunk0 = 0;
+  for (int f = 0; f < n_filters; ++f) {
+    const mocat_pf_filter& F = host_filters[f];
+    if (!(F.flags & kPfLarge)) continue;
+    if (F.n <= 0 || F.T <= 0 || F.n > 0x7FFFFFFFll || F.T > 0xFFFFFFFFll) return MOCAT_ERR_ARG;
+    if (resampling == 2 && F.n * F.n >= 0xFFFFFFFFll) return MOCAT_ERR_SIZE;
+    const int64_t nchunk = (F.n + LT - 1) / LT;
+    desc.push_back(LfDesc{F.n, F.T, F.y_offset, F.state_offset, F.out_offset, F.t_offset, chunk0, {F.key[0], F.key[1]},
+                          F.param_offset, f});
+    for (int64_t k = 0; k < nchunk; ++k) chunk_filter.push_back((int32_t)(desc.size() - 1));
+    chunk0 += nchunk;
+  }
+  char* ws = static_cast<char*>(workspace);
+  if (hipMemcpyAsync(ws + L.o_desc, desc.data(), desc.size() * sizeof(LfDesc), hipMemcpyHostToDevice, stream) !=
+          hipSuccess ||
+      hipMemcpyAsync(ws + L.o_cf, chunk_filter.data(), chunk_filter.size() * sizeof(int32_t), hipMemcpyHostToDevice,
+                     stream) != hipSuccess ||
+      hipStreamSynchronize(stream) != hipSuccess)  // the host vectors are released on return
+    return MOCAT_ERR_HIP;
+  LfArgs a{reinterpret_cast<const LfDesc*>(ws + L.o_desc), reinterpret_cast<const int32_t*>(ws + L.o_cf), L.n_large,
+           L.n_chunks, params, y, x_ws0, x_ws1, lw_ws0, lw_ws1, csum_ws, tan_ws0, tan_ws1, resampling,
+           ess_threshold, out_value, out_lw, out_anc, out_ess, out_loglik_t, out_loglik, out_grad,
+           reinterpret_cast<float*>(ws + L.o_m), reinterpret_cast<double*>(ws + L.o_s1),
+           reinterpret_cast<double*>(ws + L.o_s2), reinterpret_cast<double*>(ws + L.o_g),
+           reinterpret_cast<uint64_t*>(ws + L.o_tot), reinterpret_cast<LfState*>(ws + L.o_st)};
+  if (model == 1) {
+    if (d != 1 || dy != 1) return MOCAT_ERR_ARG;
+    return out_grad ? launch_pf_large<Sv, 1, 1, true>(a, L.max_T, stream)
+                    : launch_pf_large<Sv, 1, 1>(a, L.max_T, stream);
+  }
+  if (model != 0) return MOCAT_ERR_ARG;
+#define MOCAT_PF_CASE(DD, YY) \
+  if (d == DD && dy == YY) return launch_pf_large<Lgssm<DD, YY>, DD, YY>(a, L.max_T, stream);
   MOCAT_PF_CASE(1, 1) MOCAT_PF_CASE(2, 2) MOCAT_PF_CASE(4, 4) MOCAT_PF_CASE(4, 2) MOCAT_PF_CASE(5, 5)
   MOCAT_PF_CASE(8, 8) MOCAT_PF_CASE(2, 1) MOCAT_PF_CASE(3, 3) MOCAT_PF_CASE(4, 1) MOCAT_PF_CASE(8, 4)
 #undef MOCAT_PF_CASE

@@ -25,12 +25,16 @@ _P = _lib.c_ptr
 _lib.register("mocat_pf_param_count", [_lib.c_i32, _lib.c_i32, _lib.c_i32, _P])
 _lib.register("mocat_pf_run", [_lib.c_i32, _lib.c_i32, _lib.c_i32, _P, _lib.c_i32, _P, _P, _P, _P, _P, _P, _P, _P,
                                _lib.c_i32, _lib.c_f32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P])
+_lib.register("mocat_pf_large_workspace_bytes", [_P, _lib.c_i32, _P])
+_lib.register("mocat_pf_run_large", [_lib.c_i32, _lib.c_i32, _lib.c_i32, _P, _lib.c_i32, _P, _P, _P, _P, _P, _P, _P,
+                                     _lib.c_i32, _lib.c_f32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _lib.c_i64, _P])
 _lib.register("mocat_resample", [_P, _lib.c_i64, _lib.c_i32, _lib.c_i32, _P, _P, _P, _P])
 _lib.register("mocat_resample_categorical", [_P, _lib.c_i64, _lib.c_i64, _P, _P, _P])
 
 FILTER_DTYPE = np.dtype([("n", "<i8"), ("T", "<i8"), ("y_offset", "<i8"), ("state_offset", "<i8"),
                          ("out_offset", "<i8"), ("t_offset", "<i8"), ("key", "<u4", (2,)),
-                         ("param_offset", "<i4"), ("reserved", "<i4")])
+                         ("param_offset", "<i4"), ("flags", "<i4")])
+PF_LARGE = 1  # flags bit: the filter runs on the chunked many-workgroup path (mocat_pf_run_large)
 assert FILTER_DTYPE.itemsize == 64
 
 
@@ -84,8 +88,17 @@ def _key_np(k):
     return np.asarray(k, np.uint32).reshape(2)
 
 
-def _launch(models, ys, keys, ns, ess_threshold, resampling, store, grad=False):
-    """Runs len(models) filters (same model class and dims) in one kernel launch."""
+def _large_flags(ns, Ts, large_n):
+    """Which filters take the chunked path: those with at least large_n particles (default 4096:
+    above it one workgroup per filter is slower than a few grid-wide launches per time step)."""
+    if large_n is None:
+        large_n = 4096
+    return [int(n) >= int(large_n) for n in ns]
+
+
+def _launch(models, ys, keys, ns, ess_threshold, resampling, store, grad=False, large_n=None):
+    """Runs len(models) filters (same model class and dims): filters below large_n particles in one
+    persistent launch (a workgroup per filter), the others through the chunked per-step path."""
     m0 = models[0]
     d, dy, kind = m0.dim, m0.dim_obs, m0.kernel_model
     for mdl in models:
@@ -94,13 +107,15 @@ def _launch(models, ys, keys, ns, ess_threshold, resampling, store, grad=False):
     F = len(models)
     Ts = [int(np.asarray(y).shape[0]) for y in ys]
     order = sorted(range(F), key=lambda i: -(ns[i] * Ts[i]))  # longest first
+    large = _large_flags(ns, Ts, large_n)
     desc = np.zeros(F, FILTER_DTYPE)
     params, yflat = [], []
     poff = yoff = soff = ooff = toff = 0
     for slot, i in enumerate(order):
         p = models[i].kernel_params()
         yi = np.asarray(ys[i], np.float32).reshape(Ts[i], dy)
-        desc[slot] = (ns[i], Ts[i], yoff, soff, ooff if store else -1, toff, _key_np(keys[i]), poff, 0)
+        desc[slot] = (ns[i], Ts[i], yoff, soff, ooff if store else -1, toff, _key_np(keys[i]), poff,
+                      PF_LARGE if large[i] else 0)
         params.append(p)
         yflat.append(yi.ravel())
         poff += p.size
@@ -130,11 +145,32 @@ def _launch(models, ys, keys, ns, ess_threshold, resampling, store, grad=False):
     tan0 = torch.empty(soff * 5, **f32) if grad else None
     tan1 = torch.empty(soff * 5, **f32) if grad else None
     out_grad = torch.empty((F, 3), dtype=torch.float64, device=dev) if grad else None
-    _lib.call("mocat_pf_run", kind, d, dy, _lib.ptr(desc_t), F, _lib.ptr(params_t), _lib.ptr(y_t), _lib.ptr(x0),
-              _lib.ptr(x1), _lib.ptr(lw0), _lib.ptr(lw1), _lib.ptr(cs), _lib.ptr(anc), RESAMPLING[resampling],
-              float(ess_threshold), _lib.ptr(out_value), _lib.ptr(out_lw), _lib.ptr(out_anc), _lib.ptr(out_ess),
-              _lib.ptr(out_llt), _lib.ptr(out_ll), _lib.ptr(tan0), _lib.ptr(tan1), _lib.ptr(out_grad),
-              _lib.stream_handle())
+    small = not all(large)
+    cur = torch.cuda.current_stream()
+    side = None
+    if small:
+        # the one-workgroup-per-filter kernel runs beside the chunked path's launches
+        side = torch.cuda.Stream(device=dev) if any(large) else cur
+        side.wait_stream(cur)
+        with torch.cuda.stream(side):
+            _lib.call("mocat_pf_run", kind, d, dy, _lib.ptr(desc_t), F, _lib.ptr(params_t), _lib.ptr(y_t),
+                      _lib.ptr(x0), _lib.ptr(x1), _lib.ptr(lw0), _lib.ptr(lw1), _lib.ptr(cs), _lib.ptr(anc),
+                      RESAMPLING[resampling], float(ess_threshold), _lib.ptr(out_value), _lib.ptr(out_lw),
+                      _lib.ptr(out_anc), _lib.ptr(out_ess), _lib.ptr(out_llt), _lib.ptr(out_ll), _lib.ptr(tan0),
+                      _lib.ptr(tan1), _lib.ptr(out_grad), _lib.stream_handle())
+    if any(large):
+        desc_host = np.ascontiguousarray(desc)
+        hp = desc_host.ctypes.data_as(_lib.ctypes.c_void_p)
+        nb = _lib.ctypes.c_int64(0)
+        _lib.call("mocat_pf_large_workspace_bytes", hp, F, _lib.ctypes.byref(nb))
+        ws = torch.empty(max(int(nb.value), 1), dtype=torch.uint8, device=dev)
+        _lib.call("mocat_pf_run_large", kind, d, dy, hp, F, _lib.ptr(params_t), _lib.ptr(y_t), _lib.ptr(x0),
+                  _lib.ptr(x1), _lib.ptr(lw0), _lib.ptr(lw1), _lib.ptr(cs), RESAMPLING[resampling],
+                  float(ess_threshold), _lib.ptr(out_value), _lib.ptr(out_lw), _lib.ptr(out_anc), _lib.ptr(out_ess),
+                  _lib.ptr(out_llt), _lib.ptr(out_ll), _lib.ptr(tan0), _lib.ptr(tan1), _lib.ptr(out_grad),
+                  _lib.ptr(ws), int(ws.numel()), _lib.stream_handle())
+    if side is not None and side is not cur:
+        cur.wait_stream(side)
     res = [None] * F
     for slot, i in enumerate(order):
         r = dict(desc=desc[slot])
@@ -184,16 +220,18 @@ def run_particle_filter_for_marginals(ssm_scenario, particle_filter, y, t, rando
 
 
 def run_particle_filters(ssm_scenarios, ys, random_keys, ns, ess_threshold: float = 0.5,
-                         resampling: str = "systematic", store: bool = False, grad: bool = False):
+                         resampling: str = "systematic", store: bool = False, grad: bool = False,
+                         large_n: int = None):
     """Independent bootstrap filters of uneven sizes in one launch. Returns a list of cdicts with
     log_likelihood (FP64), ess (T,), final_value, final_log_weight (and value/log_weight/ancestors when
     store=True). grad=True (stochastic volatility): log_likelihood_grad = d log Z / d(phi, sigma, beta)
-    of the fixed-resampling-index estimator."""
+    of the fixed-resampling-index estimator. Filters with at least large_n particles (default 4096)
+    run on the chunked path spread over the whole GPU; the results do not depend on the path."""
     ys = [y.detach().cpu().numpy() if isinstance(y, torch.Tensor) else np.asarray(y) for y in ys]
     if resampling == "categorical" and any(int(v) * int(v) >= 0xFFFFFFFF for v in ns):
         raise ValueError("categorical resampling draws an (n, n) Gumbel matrix: n * n must be < 2^32 - 1")
     res = _launch(list(ssm_scenarios), ys, list(random_keys), [int(v) for v in ns], ess_threshold, resampling, store,
-                  grad)
+                  grad, large_n)
     return [cdict(**{k: v for k, v in r.items() if k != "desc"}) for r in res]
 
 

@@ -155,3 +155,74 @@ def test_sv_loglik_gradient_fixed_index(m):
         np.testing.assert_allclose(g, og, rtol=2e-3, atol=2e-3 * (1 + np.abs(og).max()))
         # the gradient path leaves the log-likelihood untouched
         assert res[i].log_likelihood.item() == plain[i].log_likelihood.item()
+
+
+def _run_both(m, models, ys, keys, ns, thr, resampling, grad=False):
+    kw = dict(ess_threshold=thr, resampling=resampling, store=True, grad=grad)
+    big = m.ssm.run_particle_filters(models, ys, keys, ns, large_n=1, **kw)  # every filter chunked
+    one = m.ssm.run_particle_filters(models, ys, keys, ns, large_n=1 << 40, **kw)  # a workgroup per filter
+    return big, one
+
+
+@pytest.mark.parametrize("resampling,thr", [("systematic", 0.7), ("multinomial", 0.5), ("systematic", 1.0)])
+def test_chunked_path_matches_workgroup_path_sv(m, resampling, thr):
+    """Filters on the chunked many-workgroup path draw the same ancestors and particles as on the
+    one-workgroup path (same weights, key and quantisation); log-likelihoods agree to FP64 rounding of
+    the different summation order. Sizes straddle the 256-particle chunk."""
+    from mocat_amd import datasets
+    models, ys, _ = datasets.sv_batch_config(4, n_models=7, n_min=64, n_max=3000, t_min=3, t_max=70)
+    ns = [1, 255, 256, 257, 700, 2049, 3000]
+    keys = [k for k in m.random.split(m.random.PRNGKey(5), 7)]
+    big, one = _run_both(m, models, ys, keys, ns, thr, resampling)
+    for i in range(7):
+        np.testing.assert_array_equal(big[i].ancestors.cpu().numpy(), one[i].ancestors.cpu().numpy())
+        np.testing.assert_array_equal(big[i].value.cpu().numpy(), one[i].value.cpu().numpy())
+        np.testing.assert_array_equal(big[i].final_value.cpu().numpy(), one[i].final_value.cpu().numpy())
+        np.testing.assert_allclose(big[i].ess.cpu().numpy(), one[i].ess.cpu().numpy(), rtol=1e-6)
+        a, b = big[i].log_likelihood.item(), one[i].log_likelihood.item()
+        assert abs(a - b) <= 1e-10 * (1 + abs(b)), (i, a, b)
+
+
+def test_chunked_path_lgssm_same_path_as_fp64_oracle(m):
+    """A 5000-particle linear-Gaussian filter (20 chunks) on the chunked path vs the FP64 oracle
+    replaying its ancestors, and against the one-workgroup path."""
+    from mocat_amd import datasets
+    model, y, t = datasets.lgssm_config(3, 4, 25)
+    n = 5000
+    yn = y.cpu().numpy()
+    key = m.random.PRNGKey(8)
+    big, one = _run_both(m, [model], [yn], [key], [n], 0.5, "systematic")
+    r = big[0]
+    ess = r.ess.cpu().numpy()
+    flags = np.zeros(25, bool)
+    flags[1:] = ess[:-1] < 0.5 * n
+    o = opf.run_pf(_lgssm_np(model), yn, key.cpu().numpy(), n, 0.5, "systematic", ancestors=r.ancestors.cpu().numpy(),
+                   resample_flags=flags)
+    ll = r.log_likelihood.item()
+    assert abs(ll - o["log_likelihood"]) <= 1e-4 * abs(o["log_likelihood"]), (ll, o["log_likelihood"])
+    np.testing.assert_array_equal(r.ancestors.cpu().numpy(), one[0].ancestors.cpu().numpy())
+    assert abs(ll - one[0].log_likelihood.item()) <= 1e-10 * abs(ll)
+
+
+def test_chunked_path_categorical_and_gradient(m):
+    """mocat's Gumbel-max resampler and the SV forward-tangent gradient on the chunked path agree with
+    the one-workgroup path."""
+    from mocat_amd import datasets
+    models, ys, _ = datasets.sv_batch_config(9, n_models=3, n_min=64, n_max=600, t_min=5, t_max=30)
+    ns = [300, 513, 64]
+    keys = [k for k in m.random.split(m.random.PRNGKey(2), 3)]
+    big, one = _run_both(m, models, ys, keys, ns, 0.8, "categorical")
+    for i in range(3):
+        np.testing.assert_array_equal(big[i].ancestors.cpu().numpy(), one[i].ancestors.cpu().numpy())
+    big, one = _run_both(m, models, ys, keys, ns, 0.8, "systematic", grad=True)
+    for i in range(3):
+        np.testing.assert_allclose(big[i].log_likelihood_grad.cpu().numpy(), one[i].log_likelihood_grad.cpu().numpy(),
+                                   rtol=1e-9, atol=1e-9)
+        T = len(ys[i])
+        ess = big[i].ess.cpu().numpy()
+        flags = np.zeros(T, bool)
+        flags[1:] = ess[:-1] < 0.8 * ns[i]
+        og = opf.sv_loglik_grad(models[i].phi, models[i].sigma, models[i].beta, ys[i][:, 0], keys[i].cpu().numpy(),
+                                ns[i], big[i].ancestors.cpu().numpy(), flags)
+        np.testing.assert_allclose(big[i].log_likelihood_grad.cpu().numpy(), og, rtol=2e-3,
+                                   atol=2e-3 * (1 + np.abs(og).max()))
