@@ -184,7 +184,6 @@ struct Bf16Engine {
   using L = PackLayout<DP>;
   static constexpr int KB = L::KB, NB = L::NB, NW = NT / kWave, HW = NW / 2;
   static_assert(NW % 2 == 0, "Bf16Engine needs an even wave count");
-  static constexpr bool kPrefetch = DP <= 64;
   struct alignas(16) Smem {
     alignas(16) float gred[HW][CT * DP];
     alignas(16) bf16x8 thl[2][KB][64];  // theta^T operand, hi and lo parts, per lane
@@ -222,8 +221,8 @@ struct Bf16Engine {
       for (int r = 0; r < 16; ++r) gacc[nb][r] = 0.f;
     float upart = 0.f;
 
-    // software pipeline: the A operands (and y) of the next row block are loaded while the current
-    // block runs its MFMAs and elementwise work
+    // A operands are loaded at the top of each block: loading the next block's during the current one
+    // measured no faster, and inside the fused iteration kernel the extra registers spill
     bf16x8 ah[KB], al[KB];
     floatx4 yq[4];
     auto load_a = [&](int rb) {
@@ -235,9 +234,8 @@ struct Bf16Engine {
         al[kb] = A[(1 * KB + kb) * 64 + lane];
       }
     };
-    if (kPrefetch && w < nrb) load_a(w);
     for (int rb = w; rb < nrb; rb += NW) {
-      if (!kPrefetch) load_a(rb);
+      load_a(rb);
       const char* blk = base + (int64_t)rb * L::BLOCK_BYTES;
       const bf16x8* B = reinterpret_cast<const bf16x8*>(blk) + L::A_ELEMS / 8;
       floatx16 z;
@@ -261,7 +259,7 @@ struct Bf16Engine {
         yv[4 * q + 2] = yq[q].z;
         yv[4 * q + 3] = yq[q].w;
       }
-      // B operands of this block, then the next block's A operands
+      // B operands of this block
       bf16x8 xbh[2][NB], xbl[2][NB];
 #pragma unroll
       for (int kb2 = 0; kb2 < 2; ++kb2)
@@ -270,7 +268,6 @@ struct Bf16Engine {
           xbh[kb2][nb] = B[((0 * 2 + kb2) * NB + nb) * 64 + lane];
           xbl[kb2][nb] = B[((1 * 2 + kb2) * NB + nb) * 64 + lane];
         }
-      if (kPrefetch && rb + NW < nrb) load_a(rb + NW);
       bf16x8 rh[2], rl[2];
       if (want_pot) {
 #pragma unroll
@@ -456,20 +453,23 @@ struct LrArgs {
 };
 
 // Evaluates prior/likelihood potentials and gradients of the tile's theta (in LDS).
+// want_pot = false (intermediate leapfrog steps): the gradient only; the potentials are left as they were.
 template <class Engine, int DP, int NT>
 MOCAT_DEV void tile_eval(typename Engine::Smem& es, TileState<DP>& ts, const LrArgs& a, bool want_pot = true) {
   Engine::eval(es, a.X, a.y, a.packed, a.n, a.d, ts.theta, ts.lik, ts.glik, want_pot);
   const int t = threadIdx.x;
   for (int e = t; e < CT * DP; e += NT) ts.grad[e] = a.prior_prec * ts.theta[e] + a.temperature * ts.glik[e];
-  if (t < CT) {
-    float sq = 0.f;
-    for (int k = 0; k < a.d; ++k) sq += ts.theta[t * DP + k] * ts.theta[t * DP + k];
-    // prec == 0: improper flat prior, potential 0 (mocat's default uniform prior, core.py:183-189)
-    const float pr =
-        a.prior_prec > 0.f ? 0.5f * a.prior_prec * sq + 0.5f * (float)a.d * (kLog2Pi - logf(a.prior_prec)) : 0.f;
-    ts.prior[t] = pr;
-    ts.likv[t] = ts.lik[t];
-    ts.pot[t] = pr + a.temperature * ts.lik[t];
+  if (want_pot) {
+    chain_sqnorm<DP, NT>(ts.theta, a.d, ts.sq);
+    if (t < CT) {
+      // prec == 0: improper flat prior, potential 0 (mocat's default uniform prior, core.py:183-189)
+      const float pr = a.prior_prec > 0.f
+                           ? 0.5f * a.prior_prec * ts.sq[t] + 0.5f * (float)a.d * (kLog2Pi - logf(a.prior_prec))
+                           : 0.f;
+      ts.prior[t] = pr;
+      ts.likv[t] = ts.lik[t];
+      ts.pot[t] = pr + a.temperature * ts.lik[t];
+    }
   }
   __syncthreads();
 }
