@@ -164,11 +164,13 @@ int mocat_smc_temper(const float* log_weight, const float* lik, int64_t n, doubl
  * One SVGD.update with the Gaussian kernel and adagrad(step, momentum)
  * (mocat/src/transport/svgd.py:18-32,122-146; kernels.py:89-102): x (in/out), gsq and mom
  * (adagrad state, in/out) [n][d]; x_in and grad are the positions and potential gradients the Stein
- * direction is computed from; phi_out optional. d <= 128. workspace: mocat_svgd_workspace_bytes. */
+ * direction is computed from; phi_out optional. d <= 128. workspace: mocat_svgd_workspace_bytes.
+ * The kernel bandwidth is read from bandwidth_dev (a device float, e.g. the output of the median
+ * update, so the iteration needs no host round trip) when it is not NULL, else from bandwidth. */
 int mocat_svgd_workspace_bytes(int64_t n, int64_t d, int js_count, int64_t* bytes);
-int mocat_svgd_step(const float* x_in, const float* grad, int64_t n, int64_t d, float bandwidth, int js_count,
-                    void* workspace, float step, float momentum, float* x, float* gsq, float* mom, float* phi_out,
-                    hipStream_t stream);
+int mocat_svgd_step(const float* x_in, const float* grad, int64_t n, int64_t d, float bandwidth,
+                    const float* bandwidth_dev, int js_count, void* workspace, float step, float momentum, float* x,
+                    float* gsq, float* mom, float* phi_out, hipStream_t stream);
 /* Diagonal Gaussian mixture: potential[n] = -logsumexp_k(logc_k - 0.5 sum_j lam_kj (x_j - mu_kj)^2),
  * grad[n][d] (GaussianMixture, mocat/src/scenarios/toy_examples.py:54-116). K <= 8. */
 int mocat_gmm_potential_grad(const float* x, int64_t n, int64_t d, int K, const float* means, const float* lam,
@@ -183,6 +185,15 @@ int mocat_select2(const float* v, int64_t m, int64_t rank0, int64_t rank1, uint3
 /* partial[b] = sum of sqrt(d2) over a grid-stride share of the m entries (FP64), for
  * mean_bandwidth_update (mocat/src/kernels.py:227-229). */
 int mocat_sqrt_mean(const float* d2, int64_t m, double* partial, int n_partial, hipStream_t stream);
+/* The same three over the pairs only: the n^2 distances are the n diagonal zeros and each pair twice,
+ * so median and mean follow from the strict upper triangle (row < column). _upper computes the
+ * tiles on and above the diagonal (each pair once, so the matrix read is exactly symmetric);
+ * select2_upper takes ranks in [0, n(n-1)/2) over the strict upper triangle of the row-major d2;
+ * sqrt_sum_upper sums sqrt(d2) over it (FP64 partials). n >= 2. */
+int mocat_pairwise_sqdist_upper(const float* x, int64_t n, int64_t d, void* workspace, float* d2, hipStream_t stream);
+int mocat_select2_upper(const float* d2, int64_t n, int64_t rank0, int64_t rank1, uint32_t* workspace,
+                        hipStream_t stream);
+int mocat_sqrt_sum_upper(const float* d2, int64_t n, double* partial, int n_partial, hipStream_t stream);
 
 /* ---- Kernel Stein discrepancy (replaces mocat/src/metrics.py:88-130, ksd with
  * ensemble_batchsize=None, for kernels.Gaussian (kernel 0) and kernels.IMQ (kernel 1) with mocat's

@@ -651,8 +651,13 @@ __global__ __launch_bounds__(256) void categorical_kernel(const float* __restric
 //              propagate, weight, outputs and the chunk accumulators of step t+1.
 // The weights are shifted by float(lse) and quantised with the shift of the largest weight exactly as
 // in pf_kernel, so for the same log-weights and key the ancestors are the same.
-constexpr int LT = 256;
+constexpr int LT = 256;          // threads per workgroup
 constexpr int LW = LT / kWave;
+constexpr int PPT = 4;           // particles per thread
+constexpr int CH = LT * PPT;     // particles per chunk (one workgroup)
+constexpr int SG = 256;          // prefix-sum search granularity (chunk ends staged in LDS)
+constexpr int kSearchWin = 2048;   // prefix sums of a block's ancestor chunks staged in LDS (16 KB)
+constexpr int kChunkEnds = 1024;   // SG-block end sums staged in LDS (filters up to 262144 particles)
 
 struct LfDesc {
   int64_t n, T, y_offset, state_offset, out_offset, t_offset, chunk0;
@@ -805,13 +810,15 @@ __global__ __launch_bounds__(LT) void lf_init_kernel(LfArgs a) {
   const int tid = threadIdx.x;
   if (tid == 0) model.load(a.params + F.param_offset);
   __syncthreads();
-  const int64_t n = F.n, i = (c - F.chunk0) * LT + tid;
+  const int64_t n = F.n;
   const Key key{F.key[0], F.key[1]};
   const Key sub = split_at(key, 2, 1);
   constexpr uint32_t hD = (D + 1) / 2;
   LseAcc acc{-INFINITY, 0.0, 0.0};
   GradAcc gacc{-INFINITY, 0.0, 0.0, 0.0, 0.0};
-  if (i < n) {
+  for (int pk = 0; pk < PPT; ++pk) {
+    const int64_t i = (c - F.chunk0) * CH + pk * LT + tid;
+    if (i >= n) break;
     const Key ki = split_at(sub, (uint32_t)n, (uint32_t)i);
     float eps[D], x[D];
 #pragma unroll
@@ -859,7 +866,7 @@ __global__ __launch_bounds__(LT) void lf_stats_kernel(LfArgs a, int64_t t) {
   const int fi = blockIdx.x;
   const LfDesc F = a.desc[fi];
   if (t >= F.T) return;
-  const int64_t n = F.n, nchunk = (n + LT - 1) / LT;
+  const int64_t n = F.n, nchunk = (n + CH - 1) / CH;
   LseAcc acc{-INFINITY, 0.0, 0.0};
   double g[3] = {0.0, 0.0, 0.0};
   for (int64_t k = threadIdx.x; k < nchunk; k += LT) {
@@ -934,9 +941,13 @@ __global__ __launch_bounds__(LT) void lf_totals_kernel(LfArgs a, int64_t t) {
   if (t >= F.T) return;
   const LfState st = a.st[fi];
   if (!st.res || a.resampling == 2) return;
-  const int64_t i = (c - F.chunk0) * LT + threadIdx.x;
   const float* lw = lf_lw_prev(a, t) + F.state_offset;
-  const uint64_t q = i < F.n ? quantise(__expf(lw[i] - st.m), st.shift) : 0ull;
+  uint64_t q = 0;
+#pragma unroll
+  for (int pk = 0; pk < PPT; ++pk) {
+    const int64_t i = (c - F.chunk0) * CH + pk * LT + threadIdx.x;
+    if (i < F.n) q += quantise(__expf(lw[i] - st.m), st.shift);
+  }
   const uint64_t tot = lf_block_sum(q, s);
   if (threadIdx.x == 0) a.ctot[c] = tot;
 }
@@ -953,11 +964,21 @@ __global__ __launch_bounds__(LT) void lf_csum_kernel(LfArgs a, int64_t t) {
   uint64_t off = 0;
   for (int64_t k = threadIdx.x; k < local; k += LT) off += a.ctot[F.chunk0 + k];
   off = lf_block_sum(off, s);
-  const int64_t i = local * LT + threadIdx.x;
+  // thread tid owns the PPT consecutive particles from local * CH + tid * PPT
+  const int64_t i0 = local * CH + (int64_t)threadIdx.x * PPT;
   const float* lw = lf_lw_prev(a, t) + F.state_offset;
-  const uint64_t q = i < F.n ? quantise(__expf(lw[i] - st.m), st.shift) : 0ull;
-  const uint64_t incl = lf_block_inclusive_scan(q, s);
-  if (i < F.n) a.csum[F.state_offset + i] = off + incl;
+  uint64_t q[PPT], own = 0;
+#pragma unroll
+  for (int pk = 0; pk < PPT; ++pk) {
+    q[pk] = i0 + pk < F.n ? quantise(__expf(lw[i0 + pk] - st.m), st.shift) : 0ull;
+    own += q[pk];
+  }
+  uint64_t run = off + lf_block_inclusive_scan(own, s) - own;
+#pragma unroll
+  for (int pk = 0; pk < PPT; ++pk) {
+    run += q[pk];
+    if (i0 + pk < F.n) a.csum[F.state_offset + i0 + pk] = run;
+  }
 }
 
 template <class Model, int D, int DY, bool GRAD>
@@ -969,14 +990,22 @@ __global__ __launch_bounds__(LT) void lf_step_kernel(LfArgs a, int64_t t) {
   const LfDesc F = a.desc[fi];
   if (t >= F.T) return;
   const int tid = threadIdx.x;
-  if (tid == 0) model.load(a.params + F.param_offset);
+  // the step key (and the systematic uniform) are the same for the whole chunk: derived once
+  __shared__ uint32_t sk[2];
+  __shared__ float su;
+  if (tid == 0) {
+    model.load(a.params + F.param_offset);
+    const Key key{F.key[0], F.key[1]};
+    const Key ikk = split_at(split_at(key, 2, 0), (uint32_t)(F.T - 1), (uint32_t)(t - 1));
+    sk[0] = ikk.a;
+    sk[1] = ikk.b;
+    su = uniform_from_bits(bits_at(ikk, 1, 0), 0.f, 1.f);
+  }
   __syncthreads();
   const LfState st = a.st[fi];
-  const int64_t n = F.n, T = F.T, i = (c - F.chunk0) * LT + tid;
+  const int64_t n = F.n, base = (c - F.chunk0) * CH;
   const bool res = st.res != 0;
-  const Key key{F.key[0], F.key[1]};
-  const Key rkey = split_at(key, 2, 0);
-  const Key ik = split_at(rkey, (uint32_t)(T - 1), (uint32_t)(t - 1));
+  const Key ik{sk[0], sk[1]};
   const bool odd = (t - 1) & 1;
   const float* xa = (odd ? a.x1 : a.x0) + F.state_offset * D;
   float* xb = (odd ? a.x0 : a.x1) + F.state_offset * D;
@@ -988,22 +1017,84 @@ __global__ __launch_bounds__(LT) void lf_step_kernel(LfArgs a, int64_t t) {
   constexpr uint32_t hD = (D + 1) / 2;
   LseAcc acc{-INFINITY, 0.0, 0.0};
   GradAcc gacc{-INFINITY, 0.0, 0.0, 0.0, 0.0};
-  if (i < n) {
+
+  // Ancestors by a two-level search in the exact prefix sums, without long chains of dependent
+  // global loads: the last sum of every SG-particle block is staged in LDS and searched first, then
+  // the first index inside the block found. Systematic targets rise with the particle index, so this
+  // chunk's ancestors lie in a contiguous run of blocks, staged in LDS too when it is short.
+  // (res and the method are per filter: the branches are uniform over the workgroup.)
+  __shared__ uint64_t cend[kChunkEnds];
+  __shared__ uint64_t win[kSearchWin];
+  __shared__ int64_t wk[2];
+  const uint64_t* C = a.csum + F.state_offset;
+  const int64_t nsg = (n + SG - 1) / SG;
+  const bool two_level = res && a.resampling != 2 && nsg <= kChunkEnds;
+  bool staged = false;
+  int64_t lo = 0;
+  const double W = (res && a.resampling != 2) ? (double)C[n - 1] : 0.0;
+  const double sstep = W / (double)n;
+  // first SG block whose last sum exceeds t (nsg: none does)
+  auto first_block = [&](double tg) {
+    int64_t l = 0, h = nsg;
+    while (l < h) {
+      const int64_t mid = (l + h) >> 1;
+      if ((double)cend[mid] > tg)
+        h = mid;
+      else
+        l = mid + 1;
+    }
+    return l;
+  };
+  if (two_level) {
+    for (int64_t k = tid; k < nsg; k += LT) cend[k] = C[(k + 1) * SG < n ? (k + 1) * SG - 1 : n - 1];
+    __syncthreads();
+    if (a.resampling == 0) {
+      if (tid == 0) {
+        const int64_t j1 = (base + CH < n ? base + CH : n) - 1;
+        wk[0] = first_block(((double)base + (double)su) * sstep);
+        wk[1] = first_block(((double)j1 + (double)su) * sstep);
+      }
+      __syncthreads();
+      const int64_t k0 = wk[0] < nsg ? wk[0] : nsg - 1, k1 = wk[1] < nsg ? wk[1] : nsg - 1;
+      lo = k0 * SG;
+      const int64_t len = ((k1 + 1) * SG < n ? (k1 + 1) * SG : n) - lo;
+      if (len <= kSearchWin) {
+        for (int64_t k = tid; k < len; k += LT) win[k] = C[lo + k];
+        __syncthreads();
+        staged = true;
+      }
+    }
+  }
+
+  for (int pk = 0; pk < PPT; ++pk) {
+    const int64_t i = base + pk * LT + tid;
+    if (i >= n) break;
     int64_t src = i;
-    if (res) {
-      if (a.resampling == 2) {
-        src = categorical_row(lwa, n, ik, (uint32_t)(n * n), i);
-      } else {
-        const uint64_t* C = a.csum + F.state_offset;
-        const double W = (double)C[n - 1];
-        double target;
-        if (a.resampling == 0) {
-          const double u = (double)uniform_from_bits(bits_at(ik, 1, 0), 0.f, 1.f);
-          target = ((double)i + u) * (W / (double)n);
+    if (res && a.resampling == 2) {
+      src = categorical_row(lwa, n, ik, (uint32_t)(n * n), i);
+    } else if (res) {
+      const double target =
+          a.resampling == 0
+              ? ((double)i + (double)su) * sstep
+              : (double)uniform_from_bits(bits_at(ik, (uint32_t)n, (uint32_t)i), 0.f, 1.f) * W;
+      if (two_level) {
+        const int64_t kc = first_block(target);
+        if (kc >= nsg) {
+          src = n - 1;
         } else {
-          const double u = (double)uniform_from_bits(bits_at(ik, (uint32_t)n, (uint32_t)i), 0.f, 1.f);
-          target = u * W;
+          // first p in block kc with C[p] > target (C at the block's last index exceeds it)
+          int64_t l = kc * SG, h = ((kc + 1) * SG < n ? (kc + 1) * SG : n) - 1;
+          while (l < h) {
+            const int64_t mid = (l + h) >> 1;
+            const uint64_t cm = staged ? win[mid - lo] : C[mid];
+            if ((double)cm > target)
+              h = mid;
+            else
+              l = mid + 1;
+          }
+          src = l;
         }
+      } else {
         src = search_csum(C, n, target);
       }
     }
@@ -1059,11 +1150,13 @@ __global__ __launch_bounds__(LT) void lf_finish_kernel(LfArgs a) {
   const int64_t c = blockIdx.x;
   const LfDesc F = a.desc[a.chunk_filter[c]];
   if (!((F.T - 1) & 1)) return;
-  const int64_t i = (c - F.chunk0) * LT + threadIdx.x;
-  if (i >= F.n) return;
+  for (int pk = 0; pk < PPT; ++pk) {
+    const int64_t i = (c - F.chunk0) * CH + pk * LT + threadIdx.x;
+    if (i >= F.n) return;
 #pragma unroll
-  for (int q = 0; q < D; ++q) a.x0[(F.state_offset + i) * D + q] = a.x1[(F.state_offset + i) * D + q];
-  a.lw0[F.state_offset + i] = a.lw1[F.state_offset + i];
+    for (int q = 0; q < D; ++q) a.x0[(F.state_offset + i) * D + q] = a.x1[(F.state_offset + i) * D + q];
+    a.lw0[F.state_offset + i] = a.lw1[F.state_offset + i];
+  }
 }
 
 template <class Model, int D, int DY, bool GRAD = false>
@@ -1104,7 +1197,7 @@ LfLayout lf_layout(const mocat_pf_filter* filters, int n_filters) {
   for (int f = 0; f < n_filters; ++f) {
     if (!(filters[f].flags & kPfLarge)) continue;
     ++L.n_large;
-    L.n_chunks += (filters[f].n + LT - 1) / LT;
+    L.n_chunks += (filters[f].n + CH - 1) / CH;
     L.max_T = std::max<int64_t>(L.max_T, filters[f].T);
   }
   int64_t o = 0;
@@ -1197,7 +1290,7 @@ int mocat_pf_run_large(int model, int d, int dy, const mocat_pf_filter* host_fil
     if (!(F.flags & kPfLarge)) continue;
     if (F.n <= 0 || F.T <= 0 || F.n > 0x7FFFFFFFll || F.T > 0xFFFFFFFFll) return MOCAT_ERR_ARG;
     if (resampling == 2 && F.n * F.n >= 0xFFFFFFFFll) return MOCAT_ERR_SIZE;
-    const int64_t nchunk = (F.n + LT - 1) / LT;
+    const int64_t nchunk = (F.n + CH - 1) / CH;
     desc.push_back(LfDesc{F.n, F.T, F.y_offset, F.state_offset, F.out_offset, F.t_offset, chunk0, {F.key[0], F.key[1]},
                           F.param_offset, f});
     for (int64_t k = 0; k < nchunk; ++k) chunk_filter.push_back((int32_t)(desc.size() - 1));

@@ -189,6 +189,71 @@ MOCAT_DEV floatx16 gram_block(const bf16x8* __restrict__ A, const ITile<DP>& it,
   return z;
 }
 
+// ---------------------------------------------------------------- strict upper triangle of the distance matrix
+// The squared-distance matrix is symmetric with a zero diagonal, so its order statistics and its mean
+// follow from the strict upper triangle (row < column): the full n^2 entries are the n diagonal zeros
+// and every upper entry twice. Only tiles on or above the diagonal are computed and read, each pair
+// once (the matrix the selection sees is exactly symmetric).
+
+// Lanes of a wave that share an 8-bit digit (ballots of the digit's bits); the lowest lane of each
+// group adds the group's size to h[digit]: one LDS atomic per distinct digit of the wave, with no
+// serialisation on a popular bin. Wave-uniform control flow is required.
+MOCAT_DEV void wave_multisplit_count(bool act, uint32_t dig, uint32_t* h, int lane) {
+  const uint64_t actm = __ballot(act);
+  if (actm == 0) return;
+  uint64_t same = actm;
+#pragma unroll
+  for (int b = 0; b < 8; ++b) {
+    const bool bit = (dig >> b) & 1u;
+    const uint64_t bb = __ballot(bit);
+    same &= bit ? bb : ~bb;
+  }
+  if (act && __ffsll((unsigned long long)same) - 1 == lane) atomicAdd(&h[dig], (uint32_t)__popcll(same));
+}
+
+// One radix pass over the strict upper triangle of the row-major n x n matrix v; rows are taken
+// round-robin by the workgroups, columns row+1..n-1 by the threads (block-uniform trip counts).
+__global__ __launch_bounds__(256) void radix_hist_upper_kernel(const uint32_t* __restrict__ v, int n, int shift,
+                                                               const uint32_t* __restrict__ state,
+                                                               uint32_t* __restrict__ hist) {
+  __shared__ uint32_t lh[2][256];
+  const uint32_t mask = shift == 24 ? 0u : (0xFFFFFFFFu << (shift + 8));
+  const uint32_t p0 = state[0] & mask, p1 = state[1] & mask;
+  const bool shared_prefix = p0 == p1;
+  const int lane = threadIdx.x & (kWave - 1);
+  for (int b = threadIdx.x; b < 512; b += 256) (&lh[0][0])[b] = 0u;
+  __syncthreads();
+  for (int row = blockIdx.x; row < n - 1; row += gridDim.x) {
+    const uint32_t* vr = v + (int64_t)row * n;
+    for (int c0 = row + 1; c0 < n; c0 += 256) {
+      const int c = c0 + threadIdx.x;
+      const bool valid = c < n;
+      const uint32_t u = valid ? vr[c] : 0u;
+      const uint32_t dig = (u >> shift) & 255u;
+      wave_multisplit_count(valid && (u & mask) == p0, dig, lh[0], lane);
+      if (!shared_prefix) wave_multisplit_count(valid && (u & mask) == p1, dig, lh[1], lane);
+    }
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < 512; b += 256) {
+    const uint32_t c = shared_prefix ? lh[0][b & 255] : (&lh[0][0])[b];
+    if (c) atomicAdd(&hist[b], c);
+  }
+}
+
+// sum of sqrt over the strict upper triangle, FP64, one partial per workgroup
+__global__ __launch_bounds__(256) void sqrt_sum_upper_kernel(const float* __restrict__ d2, int n,
+                                                             double* __restrict__ partial) {
+  __shared__ double sc[4];
+  double s = 0.0;
+  for (int row = blockIdx.x; row < n - 1; row += gridDim.x) {
+    const float* r = d2 + (int64_t)row * n;
+    for (int c = row + 1 + threadIdx.x; c < n; c += 256) s += (double)sqrtf(r[c]);
+  }
+  s = block_sum(s, sc);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
 // ---------------------------------------------------------------- phi (partial over a j range)
 // One workgroup = 4 waves = 128 particles i, 32 per wave, whose distance-GEMM operands (bf16 hi/lo
 // of the centred coordinates) stay in registers. The packed 32-particle j blocks of the workgroup's
@@ -202,8 +267,8 @@ template <int DP>
 __global__ __launch_bounds__(kPhiThreads) void svgd_phi_kernel(const char* __restrict__ packed,
                                                                const float* __restrict__ x,
                                                                const float* __restrict__ mean, int n, int d,
-                                                               float inv2h2, int js_count,
-                                                               float* __restrict__ part_o,
+                                                               float h_host, const float* __restrict__ h_dev,
+                                                               int js_count, float* __restrict__ part_o,
                                                                float* __restrict__ part_s) {
   using L = SvgdLayout<DP>;
   constexpr int KB = L::KB, NB = L::NB;
@@ -215,6 +280,8 @@ __global__ __launch_bounds__(kPhiThreads) void svgd_phi_kernel(const char* __res
   const int nrb = (n + 31) / 32;
   const int js = blockIdx.y;
   const int rb0 = (int)((int64_t)nrb * js / js_count), rb1 = (int)((int64_t)nrb * (js + 1) / js_count);
+  const float hb = h_dev ? *h_dev : h_host;
+  const float inv2h2 = 0.5f / (hb * hb);
 
   // B operand of the distance GEMM: lane holds particle i0 + col, columns kb*16 + g*8 + s
   bf16x8 bih[KB], bil[KB];
@@ -330,10 +397,13 @@ __global__ __launch_bounds__(kPhiThreads) void svgd_phi_kernel(const char* __res
 
 // phi from the partial sums, then the adagrad step (x, G, m updated in place); phi_out optional.
 __global__ void svgd_update_kernel(const float* __restrict__ part_o, const float* __restrict__ part_s, int js_count,
-                                   int n, int d, int DP, const float* __restrict__ mean, float inv_n, float inv_h2,
-                                   float step, float momentum, float* __restrict__ x, float* __restrict__ gsq,
-                                   float* __restrict__ mom, float* __restrict__ phi_out) {
+                                   int n, int d, int DP, const float* __restrict__ mean, float inv_n, float h_host,
+                                   const float* __restrict__ h_dev, float step, float momentum,
+                                   float* __restrict__ x, float* __restrict__ gsq, float* __restrict__ mom,
+                                   float* __restrict__ phi_out) {
   const int64_t total = (int64_t)n * d;
+  const float hb = h_dev ? *h_dev : h_host;
+  const float inv_h2 = 1.f / (hb * hb);
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
     const int64_t i = e / d;
     const int k = (int)(e - i * d);
@@ -393,7 +463,7 @@ __global__ __launch_bounds__(256) void gmm_kernel(const float* __restrict__ x, i
 // ---------------------------------------------------------------- pairwise squared distances
 template <int DP, int NT>
 __global__ __launch_bounds__(NT) void sqdist_kernel(const char* __restrict__ packed, const float* __restrict__ x,
-                                                    const float* __restrict__ mean, int n, int d,
+                                                    const float* __restrict__ mean, int n, int d, int upper,
                                                     float* __restrict__ d2) {
   using L = SvgdLayout<DP>;
   constexpr int NW = NT / kWave;
@@ -404,7 +474,9 @@ __global__ __launch_bounds__(NT) void sqdist_kernel(const char* __restrict__ pac
   load_itile<DP, NT>(it, x, mean, n, d, i0);
   const float ni = it.ni[col];
   const int i = i0 + col;
-  for (int rb = blockIdx.y * NW + w; rb < nrb; rb += gridDim.y * NW) {
+  // upper != 0: only the j blocks on or above the diagonal block (rows j <= columns i, tile-wise)
+  const int rb_end = upper ? min(nrb, (int)blockIdx.x + 1) : nrb;
+  for (int rb = blockIdx.y * NW + w; rb < rb_end; rb += gridDim.y * NW) {
     const char* blk = packed + (int64_t)rb * L::BLOCK_BYTES;
     const floatx16 z = gram_block<DP>(reinterpret_cast<const bf16x8*>(blk), it, lane);
     const float* nj = reinterpret_cast<const float*>(blk + L::BLOCK_BF16 * 2) + g * 16;
@@ -418,20 +490,6 @@ __global__ __launch_bounds__(NT) void sqdist_kernel(const char* __restrict__ pac
 
 // ---------------------------------------------------------------- exact radix select (2 ranks)
 // state[0..1] prefix bits, state[2..3] remaining rank, hist[2][256]
-// Adds, for every distinct digit among the wave's active lanes, that digit's lane count to h (one LDS
-// atomic per distinct digit): the top-byte pass sees very few distinct digits, which would otherwise
-// serialise on one LDS address. Wave-uniform control flow is required (ballots).
-MOCAT_DEV void wave_count(bool act, uint32_t dig, uint32_t* h, int lane) {
-  uint64_t pending = __ballot(act);
-  while (pending) {
-    const int leader = __ffsll((unsigned long long)pending) - 1;
-    const uint32_t ld = (uint32_t)__shfl((int)dig, leader, kWave);
-    const uint64_t same = __ballot(((pending >> lane) & 1ull) && dig == ld);
-    if (lane == leader) atomicAdd(&h[ld], (uint32_t)__popcll(same));
-    pending &= ~same;
-  }
-}
-
 // One radix pass of the two-rank selection: histogram of the byte at `shift` over the values whose
 // higher bytes equal each rank's prefix so far. When both ranks share the prefix (always on the top
 // byte, and usually after it for adjacent ranks) one histogram is counted and written to both.
@@ -453,12 +511,8 @@ __global__ __launch_bounds__(256) void radix_hist_kernel(const uint32_t* __restr
     const uint32_t u = valid ? v[e] : 0u;
     const uint32_t dig = (u >> shift) & 255u;
     const bool a0 = valid && (u & mask) == p0;
-    if (shift == 24) {
-      wave_count(a0, dig, lh[0], lane);
-    } else {
-      if (a0) atomicAdd(&lh[0][dig], 1u);
-      if (!shared_prefix && valid && (u & mask) == p1) atomicAdd(&lh[1][dig], 1u);
-    }
+    wave_multisplit_count(a0, dig, lh[0], lane);
+    if (!shared_prefix) wave_multisplit_count(valid && (u & mask) == p1, dig, lh[1], lane);
   }
   __syncthreads();
   for (int b = threadIdx.x; b < 512; b += 256) {
@@ -520,29 +574,28 @@ __global__ __launch_bounds__(256) void sqrt_sum_kernel(const float* __restrict__
 constexpr int kDistThreads = 256;
 
 template <int DP>
-int svgd_step_dp(const float* x_in, const float* grad, int n, int d, float bandwidth, int js_count, void* packed,
-                 float* mean, float* part_o, float* part_s, float step, float momentum, float* x, float* gsq,
-                 float* mom, float* phi_out, hipStream_t st) {
+int svgd_step_dp(const float* x_in, const float* grad, int n, int d, float bandwidth, const float* bandwidth_dev,
+                 int js_count, void* packed, float* mean, float* part_o, float* part_s, float step, float momentum,
+                 float* x, float* gsq, float* mom, float* phi_out, hipStream_t st) {
   using L = SvgdLayout<DP>;
   hipLaunchKernelGGL(colmean_kernel, dim3((d + kWave - 1) / kWave), dim3(kMeanThreads), 0, st, x_in, n, d, mean);
   const int nrb = (n + 31) / 32;
   const int64_t tot = (int64_t)nrb * (L::BLOCK_BF16 + 32);
   const unsigned pb = (unsigned)std::min<int64_t>((tot + 255) / 256, 65535);
   hipLaunchKernelGGL(svgd_pack_kernel<DP>, dim3(pb), dim3(256), 0, st, x_in, grad, mean, n, d, (char*)packed);
-  const float h2 = bandwidth * bandwidth;
   hipLaunchKernelGGL((svgd_phi_kernel<DP>), dim3((unsigned)((n + kPhiI - 1) / kPhiI), (unsigned)js_count),
-                     dim3(kPhiThreads), 0, st, (const char*)packed, x_in, mean, n, d, 0.5f / h2, js_count, part_o,
-                     part_s);
+                     dim3(kPhiThreads), 0, st, (const char*)packed, x_in, mean, n, d, bandwidth, bandwidth_dev,
+                     js_count, part_o, part_s);
   const int64_t tot2 = (int64_t)n * d;
   const unsigned ub = (unsigned)std::min<int64_t>((tot2 + 255) / 256, 65535);
   hipLaunchKernelGGL(svgd_update_kernel, dim3(ub), dim3(256), 0, st, part_o, part_s, js_count, n, d, DP, mean,
-                     1.f / (float)n, 1.f / h2, step, momentum, x, gsq, mom, phi_out);
+                     1.f / (float)n, bandwidth, bandwidth_dev, step, momentum, x, gsq, mom, phi_out);
   MOCAT_LAUNCH_CHECK();
   return MOCAT_OK;
 }
 
 template <int DP>
-int sqdist_dp(const float* x, int n, int d, void* packed, float* mean, float* d2, hipStream_t st) {
+int sqdist_dp(const float* x, int n, int d, void* packed, float* mean, int upper, float* d2, hipStream_t st) {
   using L = SvgdLayout<DP>;
   hipLaunchKernelGGL(colmean_kernel, dim3((d + kWave - 1) / kWave), dim3(kMeanThreads), 0, st, x, n, d, mean);
   const int nrb = (n + 31) / 32;
@@ -552,7 +605,7 @@ int sqdist_dp(const float* x, int n, int d, void* packed, float* mean, float* d2
   hipLaunchKernelGGL(svgd_pack_kernel<DP>, dim3(pb), dim3(256), 0, st, x, x, mean, n, d, (char*)packed);
   const int gy = std::max(1, std::min(64, nrb / (kDistThreads / kWave)));
   hipLaunchKernelGGL((sqdist_kernel<DP, kDistThreads>), dim3((unsigned)nrb, (unsigned)gy), dim3(kDistThreads), 0, st,
-                     (const char*)packed, x, mean, n, d, d2);
+                     (const char*)packed, x, mean, n, d, upper, d2);
   MOCAT_LAUNCH_CHECK();
   return MOCAT_OK;
 }
@@ -575,11 +628,11 @@ int mocat_svgd_workspace_bytes(int64_t n, int64_t d, int js_count, int64_t* byte
   return MOCAT_OK;
 }
 
-int mocat_svgd_step(const float* x_in, const float* grad, int64_t n, int64_t d, float bandwidth, int js_count,
-                    void* workspace, float step, float momentum, float* x, float* gsq, float* mom, float* phi_out,
-                    hipStream_t stream) {
+int mocat_svgd_step(const float* x_in, const float* grad, int64_t n, int64_t d, float bandwidth,
+                    const float* bandwidth_dev, int js_count, void* workspace, float step, float momentum, float* x,
+                    float* gsq, float* mom, float* phi_out, hipStream_t stream) {
   if (!x_in || !grad || !workspace || !x || !gsq || !mom || n <= 0 || n > (1 << 26) || d <= 0 || d > 128 ||
-      js_count <= 0 || js_count > 64 || !(bandwidth > 0.f))
+      js_count <= 0 || js_count > 64 || (!bandwidth_dev && !(bandwidth > 0.f)))
     return MOCAT_ERR_ARG;
   const int DP = d <= 64 ? 64 : 128;
   char* ws = (char*)workspace;
@@ -588,10 +641,10 @@ int mocat_svgd_step(const float* x_in, const float* grad, int64_t n, int64_t d, 
   float* part_o = (float*)(ws + pk + 512);
   float* part_s = part_o + (int64_t)js_count * n * 2 * DP;
   if (DP == 64)
-    return svgd_step_dp<64>(x_in, grad, (int)n, (int)d, bandwidth, js_count, ws, mean, part_o, part_s, step, momentum,
-                            x, gsq, mom, phi_out, stream);
-  return svgd_step_dp<128>(x_in, grad, (int)n, (int)d, bandwidth, js_count, ws, mean, part_o, part_s, step, momentum,
-                           x, gsq, mom, phi_out, stream);
+    return svgd_step_dp<64>(x_in, grad, (int)n, (int)d, bandwidth, bandwidth_dev, js_count, ws, mean, part_o, part_s,
+                            step, momentum, x, gsq, mom, phi_out, stream);
+  return svgd_step_dp<128>(x_in, grad, (int)n, (int)d, bandwidth, bandwidth_dev, js_count, ws, mean, part_o, part_s,
+                           step, momentum, x, gsq, mom, phi_out, stream);
 }
 
 int mocat_gmm_potential_grad(const float* x, int64_t n, int64_t d, int K, const float* means, const float* lam,
@@ -609,8 +662,17 @@ int mocat_pairwise_sqdist(const float* x, int64_t n, int64_t d, void* workspace,
   char* ws = (char*)workspace;
   const int64_t pk = (packed_bytes((int)n, (int)d) + 255) / 256 * 256;
   float* mean = (float*)(ws + pk);
-  return d <= 64 ? sqdist_dp<64>(x, (int)n, (int)d, ws, mean, d2, stream)
-                 : sqdist_dp<128>(x, (int)n, (int)d, ws, mean, d2, stream);
+  return d <= 64 ? sqdist_dp<64>(x, (int)n, (int)d, ws, mean, 0, d2, stream)
+                 : sqdist_dp<128>(x, (int)n, (int)d, ws, mean, 0, d2, stream);
+}
+
+int mocat_pairwise_sqdist_upper(const float* x, int64_t n, int64_t d, void* workspace, float* d2, hipStream_t stream) {
+  if (!x || !workspace || !d2 || n <= 0 || n > (1 << 16) || d <= 0 || d > 128) return MOCAT_ERR_ARG;
+  char* ws = (char*)workspace;
+  const int64_t pk = (packed_bytes((int)n, (int)d) + 255) / 256 * 256;
+  float* mean = (float*)(ws + pk);
+  return d <= 64 ? sqdist_dp<64>(x, (int)n, (int)d, ws, mean, 1, d2, stream)
+                 : sqdist_dp<128>(x, (int)n, (int)d, ws, mean, 1, d2, stream);
 }
 
 // Exact order statistics of m non-negative floats (bit patterns ordered as unsigned ints):
@@ -626,6 +688,29 @@ int mocat_select2(const float* v, int64_t m, int64_t rank0, int64_t rank1, uint3
                        workspace, workspace + 4);
     hipLaunchKernelGGL(radix_choose_kernel, dim3(1), dim3(256), 0, stream, shift, workspace, workspace + 4);
   }
+  MOCAT_LAUNCH_CHECK();
+  return MOCAT_OK;
+}
+
+int mocat_select2_upper(const float* d2, int64_t n, int64_t rank0, int64_t rank1, uint32_t* workspace,
+                        hipStream_t stream) {
+  const int64_t m = n * (n - 1) / 2;
+  if (!d2 || !workspace || n < 2 || n > (1 << 16) || rank0 < 0 || rank1 < 0 || rank0 >= m || rank1 >= m)
+    return MOCAT_ERR_ARG;
+  hipLaunchKernelGGL(select_init_kernel, dim3(1), dim3(256), 0, stream, workspace, (uint32_t)rank0, (uint32_t)rank1);
+  const unsigned blocks = (unsigned)std::min<int64_t>(n - 1, 2048);
+  for (int shift = 24; shift >= 0; shift -= 8) {
+    hipLaunchKernelGGL(radix_hist_upper_kernel, dim3(blocks), dim3(256), 0, stream, (const uint32_t*)d2, (int)n, shift,
+                       workspace, workspace + 4);
+    hipLaunchKernelGGL(radix_choose_kernel, dim3(1), dim3(256), 0, stream, shift, workspace, workspace + 4);
+  }
+  MOCAT_LAUNCH_CHECK();
+  return MOCAT_OK;
+}
+
+int mocat_sqrt_sum_upper(const float* d2, int64_t n, double* partial, int n_partial, hipStream_t stream) {
+  if (!d2 || !partial || n < 2 || n > (1 << 16) || n_partial <= 0 || n_partial > 4096) return MOCAT_ERR_ARG;
+  hipLaunchKernelGGL(sqrt_sum_upper_kernel, dim3((unsigned)n_partial), dim3(256), 0, stream, d2, (int)n, partial);
   MOCAT_LAUNCH_CHECK();
   return MOCAT_OK;
 }

@@ -20,6 +20,9 @@ _lib.register("mocat_svgd_workspace_bytes", [_lib.c_i64, _lib.c_i64, _lib.c_i32,
 _lib.register("mocat_pairwise_sqdist", [_P, _lib.c_i64, _lib.c_i64, _P, _P, _P])
 _lib.register("mocat_select2", [_P, _lib.c_i64, _lib.c_i64, _lib.c_i64, _P, _P])
 _lib.register("mocat_sqrt_mean", [_P, _lib.c_i64, _P, _lib.c_i32, _P])
+_lib.register("mocat_pairwise_sqdist_upper", [_P, _lib.c_i64, _lib.c_i64, _P, _P, _P])
+_lib.register("mocat_select2_upper", [_P, _lib.c_i64, _lib.c_i64, _lib.c_i64, _P, _P])
+_lib.register("mocat_sqrt_sum_upper", [_P, _lib.c_i64, _P, _lib.c_i32, _P])
 
 
 class Kernel:
@@ -85,23 +88,46 @@ def select_pair(v: torch.Tensor, rank0: int, rank1: int) -> torch.Tensor:
     return ws[:2].view(torch.float32).clone()
 
 
+def _sqdist_upper(vals: torch.Tensor) -> torch.Tensor:
+    """Squared distances on and above the diagonal blocks of the n x n matrix (each pair computed once;
+    the strict upper triangle, row < column, is what the selections read)."""
+    x = vals.to(torch.float32).contiguous()
+    n, d = x.shape
+    ws = _workspace(n, d, 1, x.device)
+    d2 = torch.empty((n, n), dtype=torch.float32, device=x.device)
+    _lib.call("mocat_pairwise_sqdist_upper", _lib.ptr(x), n, d, _lib.ptr(ws), _lib.ptr(d2), _lib.stream_handle())
+    return d2
+
+
 def median_bandwidth_update(vals: torch.Tensor) -> torch.Tensor:
-    """median(l2_distance_matrix(vals)) / sqrt(2 log n) (mocat/src/kernels.py:220-224)."""
+    """median(l2_distance_matrix(vals)) / sqrt(2 log n) (mocat/src/kernels.py:220-224), a device scalar.
+
+    The n^2 distances are the n diagonal zeros and every pair twice, so the k-th smallest (k >= n) is
+    the ((k - n) // 2)-th smallest pair distance: the selection runs over the n (n - 1) / 2 pairs."""
     n = vals.shape[0]
-    d2 = pairwise_sqdist(vals)
     m = n * n
-    sel = select_pair(d2, (m - 1) // 2, m // 2)
+    ranks = ((m - 1) // 2, m // 2)
+    if n < 3:
+        sel = select_pair(pairwise_sqdist(vals), *ranks)
+    else:
+        d2 = _sqdist_upper(vals)
+        up = [max(k - n, 0) // 2 for k in ranks]
+        ws = torch.empty(4 + 512, dtype=torch.int32, device=d2.device)
+        _lib.call("mocat_select2_upper", _lib.ptr(d2), n, up[0], up[1], _lib.ptr(ws), _lib.stream_handle())
+        sel = ws[:2].view(torch.float32).clone()  # both ranks are >= n when n >= 3
     med = 0.5 * (torch.sqrt(sel[0].double()) + torch.sqrt(sel[1].double()))
     return (med / math.sqrt(2 * math.log(n))).to(torch.float32)
 
 
 def mean_bandwidth_update(vals: torch.Tensor) -> torch.Tensor:
-    """mean(l2_distance_matrix(vals)) / sqrt(2 log n) (mocat/src/kernels.py:227-229)."""
+    """mean(l2_distance_matrix(vals)) / sqrt(2 log n) (mocat/src/kernels.py:227-229), a device scalar:
+    twice the FP64 sum of the pair distances over n^2."""
     n = vals.shape[0]
-    d2 = pairwise_sqdist(vals)
     parts = 1024
+    if n < 2:
+        return torch.full((), float("nan"), dtype=torch.float32, device=vals.device)
+    d2 = _sqdist_upper(vals)
     partial = torch.empty(parts, dtype=torch.float64, device=d2.device)
-    _lib.call("mocat_sqrt_mean", _lib.ptr(d2), n * n, _lib.ptr(partial), parts, _lib.stream_handle())
-    from .reduce import logsumexp  # noqa: F401  (keeps the reduction module bound)
-    mean = partial.sum() / (n * n)
+    _lib.call("mocat_sqrt_sum_upper", _lib.ptr(d2), n, _lib.ptr(partial), parts, _lib.stream_handle())
+    mean = 2.0 * partial.sum() / (n * n)
     return (mean / math.sqrt(2 * math.log(n))).to(torch.float32)

@@ -249,7 +249,7 @@ class RMMetropolisedSMCSampler(MetropolisedSMCSampler):
         self.parameters.rm_stepsize = rm_stepsize
 
 
-_lib.register("mocat_svgd_step", [_P, _P, _lib.c_i64, _lib.c_i64, _lib.c_f32, _lib.c_i32, _P, _lib.c_f32,
+_lib.register("mocat_svgd_step", [_P, _P, _lib.c_i64, _lib.c_i64, _lib.c_f32, _P, _lib.c_i32, _P, _lib.c_f32,
                                   _lib.c_f32, _P, _P, _P, _P, _P])
 
 
@@ -293,11 +293,13 @@ class SVGD(TransportSampler):
         return max(1, min(32, blocks, -(-256 // tiles)))
 
     def _adapt(self, x):
+        """The kernel bandwidth for the next update: a device scalar when adapted (no host sync), else
+        the fixed kernel parameter."""
         from .kernels import mean_bandwidth_update, median_bandwidth_update
         if self.bandwidth_update == "median":
-            return float(median_bandwidth_update(x).item())
+            return median_bandwidth_update(x)
         if self.bandwidth_update == "mean":
-            return float(mean_bandwidth_update(x).item())
+            return mean_bandwidth_update(x)
         return float(self.parameters.kernel_params.bandwidth)
 
     def step(self, state: cdict, extra: cdict, scenario, workspace=None):
@@ -315,7 +317,12 @@ class SVGD(TransportSampler):
             workspace = _workspace(n, d, self._js(n), x.device)
         x_in = x.clone()
         phi = torch.empty_like(x)
-        _lib.call("mocat_svgd_step", _lib.ptr(x_in), _lib.ptr(g), n, d, float(extra.bandwidth), self._js(n),
+        bw = extra.bandwidth
+        if isinstance(bw, torch.Tensor):
+            bw_dev, bw_host = bw.to(device=x.device, dtype=torch.float32).reshape(1).contiguous(), 1.0
+        else:
+            bw_dev, bw_host = None, float(bw)
+        _lib.call("mocat_svgd_step", _lib.ptr(x_in), _lib.ptr(g), n, d, bw_host, _lib.ptr(bw_dev), self._js(n),
                   _lib.ptr(workspace), step, float(self.parameters.momentum), _lib.ptr(x), _lib.ptr(extra.g_sq),
                   _lib.ptr(extra.m), _lib.ptr(phi), _lib.stream_handle())
         T = float(scenario.temperature)
@@ -354,7 +361,8 @@ class SVGD(TransportSampler):
         chain = cdict()
         for k in stack:
             chain.__dict__[k] = torch.stack(stacks[k])
-        chain.bandwidth = torch.tensor(bws, dtype=torch.float32, device=x.device)
+        chain.bandwidth = torch.stack([b.reshape(()).to(torch.float32) if isinstance(b, torch.Tensor)
+                                       else torch.tensor(b, dtype=torch.float32, device=x.device) for b in bws])
         chain.final_state = state
         chain.final_extra = extra
         return chain, elapsed, summary

@@ -123,3 +123,52 @@ def test_svgd_run_gaussian_moments(m):
         x = chain.final_state.value.cpu().numpy()
         np.testing.assert_allclose(x.mean(0), [0, 0], atol=0.1)
         np.testing.assert_allclose(np.var(x, 0), [1.0, 2.0], rtol=0.2)
+
+
+@pytest.mark.parametrize("n", [3, 4, 5, 33, 130, 1000])
+def test_median_over_pairs_exact(m, n):
+    """The pair-only selection returns exactly the median order statistics of the full n x n multiset
+    (n diagonal zeros and each pair twice) built from the same GPU distances."""
+    rs = np.random.default_rng(n)
+    x = rs.normal(0, 1.5, (n, 7)).astype(np.float32)
+    x[n // 2] = x[0]  # one zero-distance pair
+    d2 = m.kernels._sqdist_upper(torch.tensor(x, device="cuda")).cpu().numpy()
+    iu = np.triu_indices(n, 1)
+    full = np.sort(np.concatenate([np.zeros(n, np.float32), d2[iu], d2[iu]]))
+    mm = n * n
+    want = 0.5 * (np.sqrt(np.float64(full[(mm - 1) // 2])) + np.sqrt(np.float64(full[mm // 2])))
+    want = np.float32(want / np.sqrt(2 * np.log(n)))
+    got = m.kernels.median_bandwidth_update(torch.tensor(x, device="cuda")).item()
+    assert got == want
+    mean = np.float32(2 * np.sqrt(d2[iu].astype(np.float64)).sum() / (n * n) / np.sqrt(2 * np.log(n)))
+    assert m.kernels.mean_bandwidth_update(torch.tensor(x, device="cuda")).item() == pytest.approx(float(mean), rel=1e-6)
+
+
+def test_select_pairs_constant_and_ties(m):
+    """Every pair at one distance, and two clusters (ties across the whole top byte)."""
+    n = 300
+    x = np.zeros((n, 3), np.float32)
+    x[:, 0] = np.arange(n) % 2  # pair distances are 0 or 1
+    got = m.kernels.median_bandwidth_update(torch.tensor(x, device="cuda")).item()
+    full = np.sort(np.concatenate([np.zeros(n), (np.subtract.outer(x[:, 0], x[:, 0]) ** 2)[np.triu_indices(n, 1)].repeat(2)]))
+    mm = n * n
+    want = np.float32(0.5 * (np.sqrt(full[(mm - 1) // 2]) + np.sqrt(full[mm // 2])) / np.sqrt(2 * np.log(n)))
+    assert got == want
+
+
+def test_svgd_step_device_bandwidth_matches_host(m):
+    """The kernel reads the bandwidth from device memory with the same arithmetic as from the host."""
+    rs = np.random.default_rng(1)
+    n, d = 700, 30
+    x = rs.normal(0, 2, (n, d)).astype(np.float32)
+    g = rs.normal(0, 1, (n, d)).astype(np.float32)
+    outs = []
+    for bw in (2.5, torch.tensor(2.5, device="cuda")):
+        smp = m.SVGD(stepsize=0.01, bandwidth_update=None)
+        st = m.cdict(value=torch.tensor(x, device="cuda"), grad_potential=torch.tensor(g, device="cuda"))
+        ex = m.cdict(random_key=m.random.PRNGKey(0), iter=0, bandwidth=bw, g_sq=torch.zeros((n, d), device="cuda"),
+                     m=torch.zeros((n, d), device="cuda"))
+        sc = m.DiagonalGaussian(np.zeros(d), np.ones(d), prior_variance=math.inf)
+        outs.append((smp.step(st, ex, sc).cpu().numpy(), st.value.cpu().numpy()))
+    np.testing.assert_array_equal(outs[0][0], outs[1][0])
+    np.testing.assert_array_equal(outs[0][1], outs[1][1])
