@@ -195,6 +195,21 @@ int mocat_select2_upper(const float* d2, int64_t n, int64_t rank0, int64_t rank1
                         hipStream_t stream);
 int mocat_sqrt_sum_upper(const float* d2, int64_t n, double* partial, int n_partial, hipStream_t stream);
 
+/* n_iter iterations of SVGD on a diagonal Gaussian mixture in one call (the loop of
+ * mocat/src/sample.py:110-148 over SVGD.update, mocat/src/transport/svgd.py:122-146, with the
+ * bandwidth adapted after every update as the reference's median/mean update override,
+ * mocat/src/tests/test_transport.py:76-89): per iteration key <- split(key, n+2)[n], the Stein step
+ * of mocat_svgd_step with the bandwidth read from *bandwidth, the mixture's potential and gradient
+ * at the new particles (times temperature), then *bandwidth <- mean (bandwidth_mode 1) or median
+ * (2) update, or unchanged (0). In/out: x, gsq, mom, potential, grad [n][d] / [n], key[2],
+ * *bandwidth (device float). bandwidth_trace (optional) [n_iter]: the bandwidth after each
+ * iteration. No host synchronisation. workspace: mocat_svgd_gmm_run_workspace_bytes. */
+int mocat_svgd_gmm_run_workspace_bytes(int64_t n, int64_t d, int js_count, int64_t* bytes);
+int mocat_svgd_gmm_run(float* x, int64_t n, int64_t d, int K, const float* means, const float* lam, const float* logc,
+                       float temperature, float step, float momentum, int bandwidth_mode, float* bandwidth,
+                       int n_iter, int js_count, void* workspace, int64_t workspace_bytes, float* gsq, float* mom,
+                       float* potential, float* grad, uint32_t* key, float* bandwidth_trace, hipStream_t stream);
+
 /* ---- Kernel Stein discrepancy (replaces mocat/src/metrics.py:88-130, ksd with
  * ensemble_batchsize=None, for kernels.Gaussian (kernel 0) and kernels.IMQ (kernel 1) with mocat's
  * k_0 of mocat/src/metrics.py:116-124 and the kernel methods of mocat/src/kernels.py:83-111,

@@ -15,6 +15,8 @@
 // Bandwidth: median (or mean) of all N^2 pairwise distances, diagonal zeros included, divided by
 // sqrt(2 log N) (kernels.py:220-229). Squared distances are written once (N x N f32) and the two
 // middle order statistics found exactly by an 8-bit radix select over their bit patterns.
+#include <cmath>
+
 #include "common.hpp"
 
 using namespace mocat;
@@ -571,6 +573,54 @@ __global__ __launch_bounds__(256) void sqrt_sum_kernel(const float* __restrict__
   if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
+// ---------------------------------------------------------------- fused SVGD run helpers
+// key <- split(key, num)[idx] (SVGD.update's key advance)
+__global__ void key_advance_kernel(uint32_t* key, uint32_t num, uint32_t idx) {
+  if (threadIdx.x == 0) {
+    const Key k = split_at(Key{key[0], key[1]}, num, idx);
+    key[0] = k.a;
+    key[1] = k.b;
+  }
+}
+
+// median bandwidth from the two selected squared distances (bit patterns): FP64 square roots, their
+// mean, divided by sqrt(2 log n) (computed on the host), rounded to f32
+__global__ void median_bandwidth_kernel(const uint32_t* bits, double norm, float* h, float* trace) {
+  if (threadIdx.x == 0) {
+    const double v0 = (double)__uint_as_float(bits[0]), v1 = (double)__uint_as_float(bits[1]);
+    const double med = 0.5 * (sqrt(v0) + sqrt(v1));
+    const float hv = (float)(med / norm);
+    *h = hv;
+    if (trace) *trace = hv;
+  }
+}
+
+// mean bandwidth from the FP64 partial sums of the pair distances: 2 sum / n^2 / sqrt(2 log n)
+__global__ void mean_bandwidth_kernel(const double* partial, int n_partial, double inv_nn, double norm, float* h,
+                                      float* trace) {
+  __shared__ double sc[4];
+  double s = 0.0;
+  for (int b = threadIdx.x; b < n_partial; b += blockDim.x) s += partial[b];
+  s = block_sum(s, sc);
+  if (threadIdx.x == 0) {
+    const float hv = (float)(2.0 * s * inv_nn / norm);
+    *h = hv;
+    if (trace) *trace = hv;
+  }
+}
+
+__global__ void copy_bandwidth_kernel(const float* h, float* trace) {
+  if (threadIdx.x == 0) *trace = *h;
+}
+
+// potential and gradient times the temperature (SVGD uses T * U for a tempered scenario)
+__global__ void scale_kernel(float* pot, float* grad, int64_t n, int64_t nd, float T) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < nd; e += (int64_t)gridDim.x * blockDim.x) {
+    grad[e] *= T;
+    if (e < n) pot[e] *= T;
+  }
+}
+
 constexpr int kDistThreads = 256;
 
 template <int DP>
@@ -718,6 +768,79 @@ int mocat_sqrt_sum_upper(const float* d2, int64_t n, double* partial, int n_part
 int mocat_sqrt_mean(const float* d2, int64_t m, double* partial, int n_partial, hipStream_t stream) {
   if (!d2 || !partial || m <= 0 || n_partial <= 0 || n_partial > 4096) return MOCAT_ERR_ARG;
   hipLaunchKernelGGL(sqrt_sum_kernel, dim3((unsigned)n_partial), dim3(256), 0, stream, d2, m, partial);
+  MOCAT_LAUNCH_CHECK();
+  return MOCAT_OK;
+}
+
+// Workspace of mocat_svgd_gmm_run: the step workspace, the n x n distance matrix, the selection
+// state, FP64 partial sums, 256-byte aligned.
+static int64_t svgd_run_layout(int64_t n, int64_t d, int js_count, int64_t* o_d2, int64_t* o_sel, int64_t* o_part) {
+  int64_t step_bytes = 0;
+  mocat_svgd_workspace_bytes(n, d, js_count, &step_bytes);
+  int64_t o = (step_bytes + 255) / 256 * 256;
+  *o_d2 = o;
+  o += (n * n * 4 + 255) / 256 * 256;
+  *o_sel = o;
+  o += 4 * (4 + 512) + 256 - 4 * (4 + 512) % 256;
+  *o_part = o;
+  o += 1024 * 8;
+  return o;
+}
+
+int mocat_svgd_gmm_run_workspace_bytes(int64_t n, int64_t d, int js_count, int64_t* bytes) {
+  if (n <= 1 || n > (1 << 16) || d <= 0 || d > 128 || js_count <= 0 || js_count > 64 || !bytes) return MOCAT_ERR_ARG;
+  int64_t a, b, c;
+  *bytes = svgd_run_layout(n, d, js_count, &a, &b, &c);
+  return MOCAT_OK;
+}
+
+int mocat_svgd_gmm_run(float* x, int64_t n, int64_t d, int K, const float* means, const float* lam, const float* logc,
+                       float temperature, float step, float momentum, int bandwidth_mode, float* bandwidth,
+                       int n_iter, int js_count, void* workspace, int64_t workspace_bytes, float* gsq, float* mom,
+                       float* potential, float* grad, uint32_t* key, float* bandwidth_trace, hipStream_t stream) {
+  if (!x || !means || !lam || !logc || !bandwidth || !workspace || !gsq || !mom || !potential || !grad || !key ||
+      n <= 1 || n > (1 << 16) || d <= 0 || d > 128 || K <= 0 || K > 8 || n_iter < 0 || js_count <= 0 ||
+      js_count > 64 || bandwidth_mode < 0 || bandwidth_mode > 2)
+    return MOCAT_ERR_ARG;
+  if (bandwidth_mode == 2 && n < 3) return MOCAT_ERR_ARG;
+  int64_t o_d2, o_sel, o_part;
+  if (workspace_bytes < svgd_run_layout(n, d, js_count, &o_d2, &o_sel, &o_part)) return MOCAT_ERR_SIZE;
+  char* ws = static_cast<char*>(workspace);
+  float* d2 = reinterpret_cast<float*>(ws + o_d2);
+  uint32_t* sel = reinterpret_cast<uint32_t*>(ws + o_sel);
+  double* part = reinterpret_cast<double*>(ws + o_part);
+  const double norm = std::sqrt(2.0 * std::log((double)n));
+  const int64_t m = n * n;
+  const int64_t r0 = (m - 1) / 2, r1 = m / 2;  // both >= n for n >= 3
+  for (int it = 0; it < n_iter; ++it) {
+    // SVGD.update: key advance, Stein direction and adagrad step in place, then the potential
+    hipLaunchKernelGGL(key_advance_kernel, dim3(1), dim3(64), 0, stream, key, (uint32_t)(n + 2), (uint32_t)n);
+    int rc = mocat_svgd_step(x, grad, n, d, 1.f, bandwidth, js_count, workspace, step, momentum, x, gsq, mom, nullptr,
+                             stream);
+    if (rc != MOCAT_OK) return rc;
+    rc = mocat_gmm_potential_grad(x, n, d, K, means, lam, logc, potential, grad, stream);
+    if (rc != MOCAT_OK) return rc;
+    if (temperature != 1.f)
+      hipLaunchKernelGGL(scale_kernel, dim3((unsigned)std::min<int64_t>((n * d + 255) / 256, 4096)), dim3(256), 0,
+                         stream, potential, grad, n, n * d, temperature);
+    float* tr = bandwidth_trace ? bandwidth_trace + it : nullptr;
+    if (bandwidth_mode == 0) {
+      if (tr) hipLaunchKernelGGL(copy_bandwidth_kernel, dim3(1), dim3(64), 0, stream, bandwidth, tr);
+      continue;
+    }
+    rc = mocat_pairwise_sqdist_upper(x, n, d, workspace, d2, stream);
+    if (rc != MOCAT_OK) return rc;
+    if (bandwidth_mode == 2) {
+      rc = mocat_select2_upper(d2, n, (r0 - n) / 2, (r1 - n) / 2, sel, stream);
+      if (rc != MOCAT_OK) return rc;
+      hipLaunchKernelGGL(median_bandwidth_kernel, dim3(1), dim3(64), 0, stream, sel, norm, bandwidth, tr);
+    } else {
+      rc = mocat_sqrt_sum_upper(d2, n, part, 1024, stream);
+      if (rc != MOCAT_OK) return rc;
+      hipLaunchKernelGGL(mean_bandwidth_kernel, dim3(1), dim3(256), 0, stream, part, 1024,
+                         1.0 / ((double)n * (double)n), norm, bandwidth, tr);
+    }
+  }
   MOCAT_LAUNCH_CHECK();
   return MOCAT_OK;
 }

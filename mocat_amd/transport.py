@@ -249,6 +249,10 @@ class RMMetropolisedSMCSampler(MetropolisedSMCSampler):
         self.parameters.rm_stepsize = rm_stepsize
 
 
+_lib.register("mocat_svgd_gmm_run_workspace_bytes", [_lib.c_i64, _lib.c_i64, _lib.c_i32, _P])
+_lib.register("mocat_svgd_gmm_run", [_P, _lib.c_i64, _lib.c_i64, _lib.c_i32, _P, _P, _P, _lib.c_f32, _lib.c_f32,
+                                     _lib.c_f32, _lib.c_i32, _P, _lib.c_i32, _lib.c_i32, _P, _lib.c_i64, _P, _P,
+                                     _P, _P, _P, _P, _P])
 _lib.register("mocat_svgd_step", [_P, _P, _lib.c_i64, _lib.c_i64, _lib.c_f32, _P, _lib.c_i32, _P, _lib.c_f32,
                                   _lib.c_f32, _P, _P, _P, _P, _P])
 
@@ -348,6 +352,10 @@ class SVGD(TransportSampler):
         extra.bandwidth = float(self.parameters.kernel_params.bandwidth)
         extra.bandwidth = self._adapt(x)
         from .kernels import _workspace
+        from .scenarios import GaussianMixture
+        if (isinstance(scenario, GaussianMixture) and not stack and not callable(self.parameters.stepsize)
+                and n >= 3):
+            return self._run_gmm_fused(scenario, state, extra, T, summary, t0)
         ws = _workspace(n, x.shape[1], self._js(n), x.device)
         stacks = {k: [getattr(state, k).clone()] for k in stack}
         bws = [extra.bandwidth]
@@ -363,6 +371,41 @@ class SVGD(TransportSampler):
             chain.__dict__[k] = torch.stack(stacks[k])
         chain.bandwidth = torch.stack([b.reshape(()).to(torch.float32) if isinstance(b, torch.Tensor)
                                        else torch.tensor(b, dtype=torch.float32, device=x.device) for b in bws])
+        chain.final_state = state
+        chain.final_extra = extra
+        return chain, elapsed, summary
+
+    def _run_gmm_fused(self, scenario, state, extra, T, summary, t0):
+        """All iterations in one library call (Gaussian-mixture targets): the same updates, in the same
+        order, as repeated step() calls, with no host round trip between iterations."""
+        x = state.value
+        n, d = x.shape
+        js = self._js(n)
+        mode = {"median": 2, "mean": 1, None: 0}[self.bandwidth_update]
+        bw = extra.bandwidth
+        bw = (bw.to(device=x.device, dtype=torch.float32).reshape(1).clone() if isinstance(bw, torch.Tensor)
+              else torch.full((1,), float(bw), dtype=torch.float32, device=x.device))
+        bw0 = bw.clone()
+        nb = _lib.ctypes.c_int64(0)
+        _lib.call("mocat_svgd_gmm_run_workspace_bytes", n, d, js, _lib.ctypes.byref(nb))
+        ws = torch.empty(int(nb.value), dtype=torch.uint8, device=x.device)
+        trace = torch.empty(self.max_iter, dtype=torch.float32, device=x.device)
+        pot = state.potential.to(torch.float32).contiguous()
+        grad = state.grad_potential.to(torch.float32).contiguous()
+        key = extra.random_key.clone().contiguous()
+        _lib.call("mocat_svgd_gmm_run", _lib.ptr(x), n, d, scenario.n_components, _lib.ptr(scenario.means),
+                  _lib.ptr(scenario.lam), _lib.ptr(scenario.logc), float(T), float(self.parameters.stepsize),
+                  float(self.parameters.momentum), mode, _lib.ptr(bw), int(self.max_iter), js, _lib.ptr(ws),
+                  int(ws.numel()), _lib.ptr(extra.g_sq), _lib.ptr(extra.m), _lib.ptr(pot), _lib.ptr(grad),
+                  _lib.ptr(key), _lib.ptr(trace), _lib.stream_handle())
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - t0
+        state.potential, state.grad_potential = pot, grad
+        extra.iter += self.max_iter
+        extra.random_key = key
+        extra.bandwidth = bw[0]
+        chain = cdict()
+        chain.bandwidth = torch.cat([bw0, trace])
         chain.final_state = state
         chain.final_extra = extra
         return chain, elapsed, summary

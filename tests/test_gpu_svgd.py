@@ -172,3 +172,26 @@ def test_svgd_step_device_bandwidth_matches_host(m):
         outs.append((smp.step(st, ex, sc).cpu().numpy(), st.value.cpu().numpy()))
     np.testing.assert_array_equal(outs[0][0], outs[1][0])
     np.testing.assert_array_equal(outs[0][1], outs[1][1])
+
+
+@pytest.mark.parametrize("upd", ["median", "mean", None])
+def test_fused_gmm_run_matches_stepwise(m, upd):
+    """The single-call SVGD run on a Gaussian mixture reproduces repeated step() calls: particles,
+    adagrad state, bandwidth trace and key (median and fixed bandwidths bit for bit; the mean's FP64
+    sum order differs)."""
+    rs = np.random.default_rng(2)
+    d, n, iters = 6, 300, 12
+    sc = m.GaussianMixture(rs.normal(0, 2, (3, d)), rs.uniform(0.5, 2, (3, d)), rs.dirichlet(np.ones(3)))
+    kw = dict(stepsize=0.05, max_iter=iters, bandwidth_update=upd)
+    fused = m.run(sc, m.SVGD(**kw), n, m.random.PRNGKey(4))
+    smp = m.SVGD(**kw)
+    ref = m.run(sc, smp, n, m.random.PRNGKey(4), stack=("value",))  # stack forces the stepwise loop
+    fx, rx = fused.final_state.value.cpu().numpy(), ref.final_state.value.cpu().numpy()
+    if upd == "mean":
+        np.testing.assert_allclose(fx, rx, rtol=1e-5, atol=1e-5)
+        np.testing.assert_allclose(fused.bandwidth.cpu().numpy(), ref.bandwidth.cpu().numpy(), rtol=1e-6)
+    else:
+        np.testing.assert_array_equal(fx, rx)
+        np.testing.assert_array_equal(fused.bandwidth.cpu().numpy(), ref.bandwidth.cpu().numpy())
+        np.testing.assert_array_equal(fused.final_extra.g_sq.cpu().numpy(), ref.final_extra.g_sq.cpu().numpy())
+    np.testing.assert_array_equal(fused.final_extra.random_key.cpu().numpy(), ref.final_extra.random_key.cpu().numpy())
