@@ -197,6 +197,12 @@ MOCAT_DEV floatx16 gram_block(const bf16x8* __restrict__ A, const ITile<DP>& it,
 // and every upper entry twice. Only tiles on or above the diagonal are computed and read, each pair
 // once (the matrix the selection sees is exactly symmetric).
 
+constexpr int kRowUnroll = 4;  // row elements loaded per thread before they are used
+// Workgroups of the selection passes: each ends with global atomics on a few shared counters or
+// histogram bins, which serialise across workgroups, so one workgroup per compute unit.
+constexpr int kSelectBlocks = 256;
+constexpr int kSelThreads = 1024;  // 16 waves per compute unit to cover the load latency
+
 // Lanes of a wave that share an 8-bit digit (ballots of the digit's bits); the lowest lane of each
 // group adds the group's size to h[digit]: one LDS atomic per distinct digit of the wave, with no
 // serialisation on a popular bin. Wave-uniform control flow is required.
@@ -215,7 +221,7 @@ MOCAT_DEV void wave_multisplit_count(bool act, uint32_t dig, uint32_t* h, int la
 
 // One radix pass over the strict upper triangle of the row-major n x n matrix v; rows are taken
 // round-robin by the workgroups, columns row+1..n-1 by the threads (block-uniform trip counts).
-__global__ __launch_bounds__(256) void radix_hist_upper_kernel(const uint32_t* __restrict__ v, int n, int shift,
+__global__ __launch_bounds__(kSelThreads) void radix_hist_upper_kernel(const uint32_t* __restrict__ v, int n, int shift,
                                                                const uint32_t* __restrict__ state,
                                                                uint32_t* __restrict__ hist) {
   __shared__ uint32_t lh[2][256];
@@ -223,21 +229,29 @@ __global__ __launch_bounds__(256) void radix_hist_upper_kernel(const uint32_t* _
   const uint32_t p0 = state[0] & mask, p1 = state[1] & mask;
   const bool shared_prefix = p0 == p1;
   const int lane = threadIdx.x & (kWave - 1);
-  for (int b = threadIdx.x; b < 512; b += 256) (&lh[0][0])[b] = 0u;
+  for (int b = threadIdx.x; b < 512; b += kSelThreads) (&lh[0][0])[b] = 0u;
   __syncthreads();
   for (int row = blockIdx.x; row < n - 1; row += gridDim.x) {
     const uint32_t* vr = v + (int64_t)row * n;
-    for (int c0 = row + 1; c0 < n; c0 += 256) {
-      const int c = c0 + threadIdx.x;
-      const bool valid = c < n;
-      const uint32_t u = valid ? vr[c] : 0u;
-      const uint32_t dig = (u >> shift) & 255u;
-      wave_multisplit_count(valid && (u & mask) == p0, dig, lh[0], lane);
-      if (!shared_prefix) wave_multisplit_count(valid && (u & mask) == p1, dig, lh[1], lane);
+    for (int c0 = row + 1; c0 < n; c0 += kRowUnroll * kSelThreads) {
+      uint32_t uu[kRowUnroll];
+#pragma unroll
+      for (int k = 0; k < kRowUnroll; ++k) {
+        const int c = c0 + k * kSelThreads + (int)threadIdx.x;
+        uu[k] = c < n ? vr[c] : 0u;
+      }
+#pragma unroll
+      for (int k = 0; k < kRowUnroll; ++k) {
+        const bool valid = c0 + k * kSelThreads + (int)threadIdx.x < n;
+        const uint32_t u = uu[k];
+        const uint32_t dig = (u >> shift) & 255u;
+        wave_multisplit_count(valid && (u & mask) == p0, dig, lh[0], lane);
+        if (!shared_prefix) wave_multisplit_count(valid && (u & mask) == p1, dig, lh[1], lane);
+      }
     }
   }
   __syncthreads();
-  for (int b = threadIdx.x; b < 512; b += 256) {
+  for (int b = threadIdx.x; b < 512; b += kSelThreads) {
     const uint32_t c = shared_prefix ? lh[0][b & 255] : (&lh[0][0])[b];
     if (c) atomicAdd(&hist[b], c);
   }
@@ -250,7 +264,16 @@ __global__ __launch_bounds__(256) void sqrt_sum_upper_kernel(const float* __rest
   double s = 0.0;
   for (int row = blockIdx.x; row < n - 1; row += gridDim.x) {
     const float* r = d2 + (int64_t)row * n;
-    for (int c = row + 1 + threadIdx.x; c < n; c += 256) s += (double)sqrtf(r[c]);
+    for (int c0 = row + 1; c0 < n; c0 += kRowUnroll * 256) {
+      float vv[kRowUnroll];
+#pragma unroll
+      for (int k = 0; k < kRowUnroll; ++k) {
+        const int c = c0 + k * 256 + (int)threadIdx.x;
+        vv[k] = c < n ? r[c] : 0.f;
+      }
+#pragma unroll
+      for (int k = 0; k < kRowUnroll; ++k) s += (double)sqrtf(vv[k]);
+    }
   }
   s = block_sum(s, sc);
   if (threadIdx.x == 0) partial[blockIdx.x] = s;
@@ -573,6 +596,180 @@ __global__ __launch_bounds__(256) void sqrt_sum_kernel(const float* __restrict__
   if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
+// ---------------------------------------------------------------- bracketed median (fused SVGD run)
+// Between SVGD iterations the median pair distance moves little, so each selection first counts the
+// pairs below a bracket [lo, hi) around the previous median and copies those inside it to a candidate
+// buffer. When the wanted ranks fall inside the bracket (and the candidates fit), the radix passes
+// run over the candidates only, with the ranks shifted by the count below; otherwise over all pairs.
+// Either way the result is the exact order statistic. The bracket half-width (relative) shrinks
+// after hits and grows after misses.
+struct Bracket {
+  uint32_t lo, hi;     // bit patterns of non-negative floats (ordered as unsigned ints)
+  uint32_t below;      // pairs < lo
+  uint32_t inside;     // pairs in [lo, hi) (the candidate count, possibly above the capacity)
+  uint32_t ok;         // the last selection ran on the candidates
+  float delta;         // relative half-width of the next bracket
+  uint32_t overflow;   // some workgroup had more candidates than its LDS buffer
+};
+
+constexpr uint32_t kBracketCap = 1u << 20;  // candidate capacity (4 MB)
+
+__global__ __launch_bounds__(kSelThreads) void bracket_count_kernel(const uint32_t* __restrict__ v, int n, Bracket* br,
+                                                            uint32_t* __restrict__ cand) {
+  // candidates gather in LDS first (one LDS atomic per wave and tile) and reach the global buffer with
+  // one global atomic per workgroup; a workgroup whose candidates overflow its LDS marks the bracket
+  // unusable (the selection then runs over all pairs)
+  constexpr int kLocal = 4096;
+  __shared__ uint32_t lbuf[kLocal];
+  __shared__ uint32_t lcount, lbelow, lbase;
+  const uint32_t lo = br->lo, hi = br->hi;
+  const int lane = threadIdx.x & (kWave - 1);
+  if (threadIdx.x == 0) {
+    lcount = 0u;
+    lbelow = 0u;
+  }
+  __syncthreads();
+  uint32_t below = 0;
+  for (int row = blockIdx.x; row < n - 1; row += gridDim.x) {
+    const uint32_t* vr = v + (int64_t)row * n;
+    for (int c0 = row + 1; c0 < n; c0 += kRowUnroll * kSelThreads) {
+      uint32_t uu[kRowUnroll];  // loads of the whole stretch issued before any use
+#pragma unroll
+      for (int k = 0; k < kRowUnroll; ++k) {
+        const int c = c0 + k * kSelThreads + (int)threadIdx.x;
+        uu[k] = c < n ? vr[c] : 0xFFFFFFFFu;
+      }
+#pragma unroll
+      for (int k = 0; k < kRowUnroll; ++k) {
+        const int c = c0 + k * kSelThreads + (int)threadIdx.x;
+        const uint32_t u = uu[k];
+        const bool in = c < n && u >= lo && u < hi;
+        below += (c < n && u < lo) ? 1u : 0u;
+        const uint64_t m = __ballot(in);
+        if (m) {
+          uint32_t base = 0;
+          if (lane == 0) base = atomicAdd(&lcount, (uint32_t)__popcll(m));
+          base = (uint32_t)__shfl((int)base, 0, kWave);
+          if (in) {
+            const uint32_t idx = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+            if (idx < (uint32_t)kLocal) lbuf[idx] = u;
+          }
+        }
+      }
+    }
+  }
+  below = wave_sum(below);
+  if (lane == 0) atomicAdd(&lbelow, below);
+  __syncthreads();
+  const uint32_t cnt = lcount;
+  if (threadIdx.x == 0) {
+    atomicAdd(&br->below, lbelow);
+    lbase = cnt ? atomicAdd(&br->inside, cnt) : 0u;
+    if (cnt > (uint32_t)kLocal) atomicOr(&br->overflow, 1u);
+  }
+  __syncthreads();
+  const uint32_t gbase = lbase, keep = cnt < (uint32_t)kLocal ? cnt : (uint32_t)kLocal;
+  for (uint32_t k = threadIdx.x; k < keep; k += kSelThreads)
+    if (gbase + k < kBracketCap) cand[gbase + k] = lbuf[k];
+}
+
+// Sets up the two-rank selection: on the candidates (ranks minus the count below) when both ranks lie
+// inside the bracket and the candidates fit, else on all pairs.
+__global__ void bracket_decide_kernel(Bracket* br, uint32_t* __restrict__ sel, uint32_t rank0, uint32_t rank1) {
+  const bool ok = !br->overflow && br->below <= rank0 && (uint64_t)rank1 < (uint64_t)br->below + br->inside &&
+                  br->inside <= kBracketCap;
+  for (int b = threadIdx.x; b < 4 + 512; b += blockDim.x) sel[b] = 0u;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    sel[2] = ok ? rank0 - br->below : rank0;
+    sel[3] = ok ? rank1 - br->below : rank1;
+    br->ok = ok ? 1u : 0u;
+  }
+}
+
+// One radix pass over the candidates or over all pairs (upper triangle), as decided.
+__global__ __launch_bounds__(kSelThreads) void radix_hist_bracket_kernel(const uint32_t* __restrict__ v, int n,
+                                                                 const uint32_t* __restrict__ cand,
+                                                                 const Bracket* br, int shift,
+                                                                 const uint32_t* __restrict__ state,
+                                                                 uint32_t* __restrict__ hist) {
+  __shared__ uint32_t lh[2][256];
+  const uint32_t mask = shift == 24 ? 0u : (0xFFFFFFFFu << (shift + 8));
+  const uint32_t p0 = state[0] & mask, p1 = state[1] & mask;
+  const bool shared_prefix = p0 == p1;
+  const int lane = threadIdx.x & (kWave - 1);
+  for (int b = threadIdx.x; b < 512; b += kSelThreads) (&lh[0][0])[b] = 0u;
+  __syncthreads();
+  if (br->ok) {
+    const int64_t m = br->inside;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t base = blockIdx.x * (int64_t)blockDim.x; base < m; base += stride) {
+      const int64_t e = base + threadIdx.x;
+      const bool valid = e < m;
+      const uint32_t u = valid ? cand[e] : 0u;
+      const uint32_t dig = (u >> shift) & 255u;
+      wave_multisplit_count(valid && (u & mask) == p0, dig, lh[0], lane);
+      if (!shared_prefix) wave_multisplit_count(valid && (u & mask) == p1, dig, lh[1], lane);
+    }
+  } else {
+    for (int row = blockIdx.x; row < n - 1; row += gridDim.x) {
+      const uint32_t* vr = v + (int64_t)row * n;
+      for (int c0 = row + 1; c0 < n; c0 += kRowUnroll * kSelThreads) {
+        uint32_t uu[kRowUnroll];
+#pragma unroll
+        for (int k = 0; k < kRowUnroll; ++k) {
+          const int c = c0 + k * kSelThreads + (int)threadIdx.x;
+          uu[k] = c < n ? vr[c] : 0u;
+        }
+#pragma unroll
+        for (int k = 0; k < kRowUnroll; ++k) {
+          const bool valid = c0 + k * kSelThreads + (int)threadIdx.x < n;
+          const uint32_t u = uu[k];
+          const uint32_t dig = (u >> shift) & 255u;
+          wave_multisplit_count(valid && (u & mask) == p0, dig, lh[0], lane);
+          if (!shared_prefix) wave_multisplit_count(valid && (u & mask) == p1, dig, lh[1], lane);
+        }
+      }
+    }
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < 512; b += kSelThreads) {
+    const uint32_t c = shared_prefix ? lh[0][b & 255] : (&lh[0][0])[b];
+    if (c) atomicAdd(&hist[b], c);
+  }
+}
+
+// Next bracket around the selected value; counters cleared for the next count pass.
+__global__ void bracket_update_kernel(Bracket* br, const uint32_t* __restrict__ sel) {
+  if (threadIdx.x != 0) return;
+  float delta = br->delta;
+  if (!br->ok)
+    delta = fminf(delta * 4.f, 1.f);
+  else if (br->inside < kBracketCap / 8)
+    delta = fmaxf(delta * 0.75f, 1e-5f);
+  const float center = __uint_as_float(sel[1]);
+  const float lo = center * (1.f - delta), hi = center * (1.f + delta);
+  br->lo = __float_as_uint(fmaxf(lo, 0.f));
+  br->hi = hi > center ? __float_as_uint(hi) : 0x7F800001u;  // above every finite value
+  br->delta = delta;
+  br->below = 0u;
+  br->inside = 0u;
+  br->overflow = 0u;
+}
+
+// First bracket: everything (the first selection runs over all pairs).
+__global__ void bracket_init_kernel(Bracket* br) {
+  if (threadIdx.x == 0) {
+    br->lo = 0u;
+    br->hi = 0u;  // empty: no candidates, the selection falls back to all pairs
+    br->below = 0u;
+    br->inside = 0u;
+    br->ok = 0u;
+    br->overflow = 0u;
+    br->delta = 2e-3f;
+  }
+}
+
 // ---------------------------------------------------------------- fused SVGD run helpers
 // key <- split(key, num)[idx] (SVGD.update's key advance)
 __global__ void key_advance_kernel(uint32_t* key, uint32_t num, uint32_t idx) {
@@ -732,7 +929,7 @@ int mocat_select2(const float* v, int64_t m, int64_t rank0, int64_t rank1, uint3
   if (!v || !workspace || m <= 0 || rank0 < 0 || rank1 < 0 || rank0 >= m || rank1 >= m || m > 0xFFFFFFFFll)
     return MOCAT_ERR_ARG;
   hipLaunchKernelGGL(select_init_kernel, dim3(1), dim3(256), 0, stream, workspace, (uint32_t)rank0, (uint32_t)rank1);
-  const int64_t blocks = std::min<int64_t>((m + 255) / 256, 2048);
+  const int64_t blocks = std::min<int64_t>((m + 255) / 256, kSelectBlocks);
   for (int shift = 24; shift >= 0; shift -= 8) {
     hipLaunchKernelGGL(radix_hist_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, (const uint32_t*)v, m, shift,
                        workspace, workspace + 4);
@@ -748,9 +945,9 @@ int mocat_select2_upper(const float* d2, int64_t n, int64_t rank0, int64_t rank1
   if (!d2 || !workspace || n < 2 || n > (1 << 16) || rank0 < 0 || rank1 < 0 || rank0 >= m || rank1 >= m)
     return MOCAT_ERR_ARG;
   hipLaunchKernelGGL(select_init_kernel, dim3(1), dim3(256), 0, stream, workspace, (uint32_t)rank0, (uint32_t)rank1);
-  const unsigned blocks = (unsigned)std::min<int64_t>(n - 1, 2048);
+  const unsigned blocks = (unsigned)std::min<int64_t>(n - 1, kSelectBlocks);
   for (int shift = 24; shift >= 0; shift -= 8) {
-    hipLaunchKernelGGL(radix_hist_upper_kernel, dim3(blocks), dim3(256), 0, stream, (const uint32_t*)d2, (int)n, shift,
+    hipLaunchKernelGGL(radix_hist_upper_kernel, dim3(blocks), dim3(kSelThreads), 0, stream, (const uint32_t*)d2, (int)n, shift,
                        workspace, workspace + 4);
     hipLaunchKernelGGL(radix_choose_kernel, dim3(1), dim3(256), 0, stream, shift, workspace, workspace + 4);
   }
@@ -784,6 +981,8 @@ static int64_t svgd_run_layout(int64_t n, int64_t d, int js_count, int64_t* o_d2
   o += 4 * (4 + 512) + 256 - 4 * (4 + 512) % 256;
   *o_part = o;
   o += 1024 * 8;
+  o = (o + 255) / 256 * 256;  // bracket state (256 B) then the candidate buffer
+  o += 256 + (int64_t)kBracketCap * 4;
   return o;
 }
 
@@ -809,6 +1008,10 @@ int mocat_svgd_gmm_run(float* x, int64_t n, int64_t d, int K, const float* means
   float* d2 = reinterpret_cast<float*>(ws + o_d2);
   uint32_t* sel = reinterpret_cast<uint32_t*>(ws + o_sel);
   double* part = reinterpret_cast<double*>(ws + o_part);
+  const int64_t o_br = (o_part + 1024 * 8 + 255) / 256 * 256;
+  Bracket* br = reinterpret_cast<Bracket*>(ws + o_br);
+  uint32_t* cand = reinterpret_cast<uint32_t*>(ws + o_br + 256);
+  if (bandwidth_mode == 2) hipLaunchKernelGGL(bracket_init_kernel, dim3(1), dim3(64), 0, stream, br);
   const double norm = std::sqrt(2.0 * std::log((double)n));
   const int64_t m = n * n;
   const int64_t r0 = (m - 1) / 2, r1 = m / 2;  // both >= n for n >= 3
@@ -831,9 +1034,18 @@ int mocat_svgd_gmm_run(float* x, int64_t n, int64_t d, int K, const float* means
     rc = mocat_pairwise_sqdist_upper(x, n, d, workspace, d2, stream);
     if (rc != MOCAT_OK) return rc;
     if (bandwidth_mode == 2) {
-      rc = mocat_select2_upper(d2, n, (r0 - n) / 2, (r1 - n) / 2, sel, stream);
-      if (rc != MOCAT_OK) return rc;
+      const unsigned rows = (unsigned)std::min<int64_t>(n - 1, kSelectBlocks);
+      hipLaunchKernelGGL(bracket_count_kernel, dim3(rows), dim3(kSelThreads), 0, stream, (const uint32_t*)d2, (int)n, br,
+                         cand);
+      hipLaunchKernelGGL(bracket_decide_kernel, dim3(1), dim3(256), 0, stream, br, sel, (uint32_t)((r0 - n) / 2),
+                         (uint32_t)((r1 - n) / 2));
+      for (int shift = 24; shift >= 0; shift -= 8) {
+        hipLaunchKernelGGL(radix_hist_bracket_kernel, dim3(rows), dim3(kSelThreads), 0, stream, (const uint32_t*)d2, (int)n,
+                           (const uint32_t*)cand, (const Bracket*)br, shift, sel, sel + 4);
+        hipLaunchKernelGGL(radix_choose_kernel, dim3(1), dim3(256), 0, stream, shift, sel, sel + 4);
+      }
       hipLaunchKernelGGL(median_bandwidth_kernel, dim3(1), dim3(64), 0, stream, sel, norm, bandwidth, tr);
+      hipLaunchKernelGGL(bracket_update_kernel, dim3(1), dim3(64), 0, stream, br, sel);
     } else {
       rc = mocat_sqrt_sum_upper(d2, n, part, 1024, stream);
       if (rc != MOCAT_OK) return rc;

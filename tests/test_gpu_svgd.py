@@ -195,3 +195,16 @@ def test_fused_gmm_run_matches_stepwise(m, upd):
         np.testing.assert_array_equal(fused.bandwidth.cpu().numpy(), ref.bandwidth.cpu().numpy())
         np.testing.assert_array_equal(fused.final_extra.g_sq.cpu().numpy(), ref.final_extra.g_sq.cpu().numpy())
     np.testing.assert_array_equal(fused.final_extra.random_key.cpu().numpy(), ref.final_extra.random_key.cpu().numpy())
+
+
+def test_fused_median_bracket_exact_over_many_iterations(m):
+    """Small steps keep the median inside the bracket of the previous one, so most selections run on
+    the candidate set: the bandwidth trace still equals the stepwise full selection bit for bit."""
+    rs = np.random.default_rng(7)
+    d, n, iters = 10, 1000, 40
+    sc = m.GaussianMixture(rs.normal(0, 1, (2, d)), rs.uniform(0.5, 2, (2, d)), np.array([0.3, 0.7]))
+    kw = dict(stepsize=0.005, max_iter=iters, bandwidth_update="median")
+    fused = m.run(sc, m.SVGD(**kw), n, m.random.PRNGKey(1))
+    ref = m.run(sc, m.SVGD(**kw), n, m.random.PRNGKey(1), stack=("value",))
+    np.testing.assert_array_equal(fused.bandwidth.cpu().numpy(), ref.bandwidth.cpu().numpy())
+    np.testing.assert_array_equal(fused.final_state.value.cpu().numpy(), ref.final_state.value.cpu().numpy())
