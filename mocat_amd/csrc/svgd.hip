@@ -820,16 +820,22 @@ __global__ void scale_kernel(float* pot, float* grad, int64_t n, int64_t nd, flo
 
 constexpr int kDistThreads = 256;
 
+// centring mean and the packed operands of x (A layout, norms) and [grad | x] (B layout)
 template <int DP>
-int svgd_step_dp(const float* x_in, const float* grad, int n, int d, float bandwidth, const float* bandwidth_dev,
-                 int js_count, void* packed, float* mean, float* part_o, float* part_s, float step, float momentum,
-                 float* x, float* gsq, float* mom, float* phi_out, hipStream_t st) {
+void svgd_prepare_dp(const float* x, const float* grad, int n, int d, void* packed, float* mean, hipStream_t st) {
   using L = SvgdLayout<DP>;
-  hipLaunchKernelGGL(colmean_kernel, dim3((d + kWave - 1) / kWave), dim3(kMeanThreads), 0, st, x_in, n, d, mean);
+  hipLaunchKernelGGL(colmean_kernel, dim3((d + kWave - 1) / kWave), dim3(kMeanThreads), 0, st, x, n, d, mean);
   const int nrb = (n + 31) / 32;
   const int64_t tot = (int64_t)nrb * (L::BLOCK_BF16 + 32);
   const unsigned pb = (unsigned)std::min<int64_t>((tot + 255) / 256, 65535);
-  hipLaunchKernelGGL(svgd_pack_kernel<DP>, dim3(pb), dim3(256), 0, st, x_in, grad, mean, n, d, (char*)packed);
+  hipLaunchKernelGGL(svgd_pack_kernel<DP>, dim3(pb), dim3(256), 0, st, x, grad, mean, n, d, (char*)packed);
+}
+
+// Stein direction from the prepared operands of x_in, then the adagrad step of x
+template <int DP>
+void svgd_phi_update_dp(const float* x_in, int n, int d, float bandwidth, const float* bandwidth_dev, int js_count,
+                        const void* packed, const float* mean, float* part_o, float* part_s, float step,
+                        float momentum, float* x, float* gsq, float* mom, float* phi_out, hipStream_t st) {
   hipLaunchKernelGGL((svgd_phi_kernel<DP>), dim3((unsigned)((n + kPhiI - 1) / kPhiI), (unsigned)js_count),
                      dim3(kPhiThreads), 0, st, (const char*)packed, x_in, mean, n, d, bandwidth, bandwidth_dev,
                      js_count, part_o, part_s);
@@ -837,22 +843,34 @@ int svgd_step_dp(const float* x_in, const float* grad, int n, int d, float bandw
   const unsigned ub = (unsigned)std::min<int64_t>((tot2 + 255) / 256, 65535);
   hipLaunchKernelGGL(svgd_update_kernel, dim3(ub), dim3(256), 0, st, part_o, part_s, js_count, n, d, DP, mean,
                      1.f / (float)n, bandwidth, bandwidth_dev, step, momentum, x, gsq, mom, phi_out);
+}
+
+template <int DP>
+int svgd_step_dp(const float* x_in, const float* grad, int n, int d, float bandwidth, const float* bandwidth_dev,
+                 int js_count, void* packed, float* mean, float* part_o, float* part_s, float step, float momentum,
+                 float* x, float* gsq, float* mom, float* phi_out, hipStream_t st) {
+  svgd_prepare_dp<DP>(x_in, grad, n, d, packed, mean, st);
+  svgd_phi_update_dp<DP>(x_in, n, d, bandwidth, bandwidth_dev, js_count, packed, mean, part_o, part_s, step,
+                         momentum, x, gsq, mom, phi_out, st);
   MOCAT_LAUNCH_CHECK();
   return MOCAT_OK;
 }
 
+// squared distances from prepared operands (upper != 0: the tiles on and above the diagonal)
 template <int DP>
-int sqdist_dp(const float* x, int n, int d, void* packed, float* mean, int upper, float* d2, hipStream_t st) {
-  using L = SvgdLayout<DP>;
-  hipLaunchKernelGGL(colmean_kernel, dim3((d + kWave - 1) / kWave), dim3(kMeanThreads), 0, st, x, n, d, mean);
+void sqdist_packed_dp(const float* x, int n, int d, const void* packed, const float* mean, int upper, float* d2,
+                      hipStream_t st) {
   const int nrb = (n + 31) / 32;
-  const int64_t tot = (int64_t)nrb * (L::BLOCK_BF16 + 32);
-  const unsigned pb = (unsigned)std::min<int64_t>((tot + 255) / 256, 65535);
-  // the packed B layout needs a gradient: pass x again (only the A layout and norms are read here)
-  hipLaunchKernelGGL(svgd_pack_kernel<DP>, dim3(pb), dim3(256), 0, st, x, x, mean, n, d, (char*)packed);
   const int gy = std::max(1, std::min(64, nrb / (kDistThreads / kWave)));
   hipLaunchKernelGGL((sqdist_kernel<DP, kDistThreads>), dim3((unsigned)nrb, (unsigned)gy), dim3(kDistThreads), 0, st,
                      (const char*)packed, x, mean, n, d, upper, d2);
+}
+
+template <int DP>
+int sqdist_dp(const float* x, int n, int d, void* packed, float* mean, int upper, float* d2, hipStream_t st) {
+  // the packed B layout needs a gradient: pass x again (only the A layout and norms are read here)
+  svgd_prepare_dp<DP>(x, x, n, d, packed, mean, st);
+  sqdist_packed_dp<DP>(x, n, d, packed, mean, upper, d2, st);
   MOCAT_LAUNCH_CHECK();
   return MOCAT_OK;
 }
@@ -1015,24 +1033,45 @@ int mocat_svgd_gmm_run(float* x, int64_t n, int64_t d, int K, const float* means
   const double norm = std::sqrt(2.0 * std::log((double)n));
   const int64_t m = n * n;
   const int64_t r0 = (m - 1) / 2, r1 = m / 2;  // both >= n for n >= 3
+  // the step workspace: packed operands | centring mean | partial sums (as mocat_svgd_step)
+  const int DP = d <= 64 ? 64 : 128;
+  const int64_t pk = (packed_bytes((int)n, (int)d) + 255) / 256 * 256;
+  float* mean = reinterpret_cast<float*>(ws + pk);
+  float* part_o = reinterpret_cast<float*>(ws + pk + 512);
+  float* part_s = part_o + (int64_t)js_count * n * 2 * DP;
+  // operands of the current particles and gradients; refreshed after every update, where they serve the
+  // distance matrix of the bandwidth update and the next Stein direction
+  auto prepare = [&]() {
+    if (DP == 64)
+      svgd_prepare_dp<64>(x, grad, (int)n, (int)d, ws, mean, stream);
+    else
+      svgd_prepare_dp<128>(x, grad, (int)n, (int)d, ws, mean, stream);
+  };
+  prepare();
   for (int it = 0; it < n_iter; ++it) {
     // SVGD.update: key advance, Stein direction and adagrad step in place, then the potential
     hipLaunchKernelGGL(key_advance_kernel, dim3(1), dim3(64), 0, stream, key, (uint32_t)(n + 2), (uint32_t)n);
-    int rc = mocat_svgd_step(x, grad, n, d, 1.f, bandwidth, js_count, workspace, step, momentum, x, gsq, mom, nullptr,
-                             stream);
-    if (rc != MOCAT_OK) return rc;
-    rc = mocat_gmm_potential_grad(x, n, d, K, means, lam, logc, potential, grad, stream);
+    if (DP == 64)
+      svgd_phi_update_dp<64>(x, (int)n, (int)d, 1.f, bandwidth, js_count, ws, mean, part_o, part_s, step, momentum, x,
+                             gsq, mom, nullptr, stream);
+    else
+      svgd_phi_update_dp<128>(x, (int)n, (int)d, 1.f, bandwidth, js_count, ws, mean, part_o, part_s, step, momentum,
+                              x, gsq, mom, nullptr, stream);
+    int rc = mocat_gmm_potential_grad(x, n, d, K, means, lam, logc, potential, grad, stream);
     if (rc != MOCAT_OK) return rc;
     if (temperature != 1.f)
       hipLaunchKernelGGL(scale_kernel, dim3((unsigned)std::min<int64_t>((n * d + 255) / 256, 4096)), dim3(256), 0,
                          stream, potential, grad, n, n * d, temperature);
+    prepare();
     float* tr = bandwidth_trace ? bandwidth_trace + it : nullptr;
     if (bandwidth_mode == 0) {
       if (tr) hipLaunchKernelGGL(copy_bandwidth_kernel, dim3(1), dim3(64), 0, stream, bandwidth, tr);
       continue;
     }
-    rc = mocat_pairwise_sqdist_upper(x, n, d, workspace, d2, stream);
-    if (rc != MOCAT_OK) return rc;
+    if (DP == 64)
+      sqdist_packed_dp<64>(x, (int)n, (int)d, ws, mean, 1, d2, stream);
+    else
+      sqdist_packed_dp<128>(x, (int)n, (int)d, ws, mean, 1, d2, stream);
     if (bandwidth_mode == 2) {
       const unsigned rows = (unsigned)std::min<int64_t>(n - 1, kSelectBlocks);
       hipLaunchKernelGGL(bracket_count_kernel, dim3(rows), dim3(kSelThreads), 0, stream, (const uint32_t*)d2, (int)n, br,
