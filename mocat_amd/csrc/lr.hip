@@ -235,6 +235,8 @@ struct Bf16Engine {
       }
     };
     for (int rb = w; rb < nrb; rb += NW) {
+      // DP = 128: re-read the theta operands from LDS each block instead of holding 64 more registers
+      if constexpr (DP > 64) asm volatile("" ::: "memory");
       load_a(rb);
       const char* blk = base + (int64_t)rb * L::BLOCK_BYTES;
       const bf16x8* B = reinterpret_cast<const bf16x8*>(blk) + L::A_ELEMS / 8;
