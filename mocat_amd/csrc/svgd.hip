@@ -486,30 +486,111 @@ __global__ __launch_bounds__(256) void gmm_kernel(const float* __restrict__ x, i
 }
 
 // ---------------------------------------------------------------- pairwise squared distances
-template <int DP, int NT>
-__global__ __launch_bounds__(NT) void sqdist_kernel(const char* __restrict__ packed, const float* __restrict__ x,
-                                                    const float* __restrict__ mean, int n, int d, int upper,
-                                                    float* __restrict__ d2) {
+// As the Stein kernel: a workgroup owns 128 particles i (32 per wave, operands in registers) and a
+// share of the j blocks, each staged once through LDS (A layout and norms only: 16.5 KB at d <= 128,
+// double buffered) for its four waves. d2[j][i] = |x_i|^2 + |x_j|^2 - 2 x_i.x_j (>= 0, exactly 0 on
+// the diagonal). upper != 0: only j blocks up to the last diagonal block of the workgroup.
+// One workgroup per task: a group of 128 particles i and a run of kDistJB j blocks. With upper != 0
+// group gi has 4 (gi + 1) j blocks (on and above its diagonal blocks), gi + 1 tasks: the triangle is
+// cut into equal tasks, numbered row by row.
+constexpr int kDistJB = kPhiI / 32;
+
+template <int DP>
+__global__ __launch_bounds__(kPhiThreads) void sqdist_kernel(const char* __restrict__ packed,
+                                                             const float* __restrict__ x,
+                                                             const float* __restrict__ mean, int n, int d,
+                                                             int upper, float* __restrict__ d2) {
   using L = SvgdLayout<DP>;
-  constexpr int NW = NT / kWave;
-  __shared__ ITile<DP> it;
+  constexpr int KB = L::KB;
+  constexpr int ABYTES = L::A_ELEMS * 2;  // A layout of a block; the norms follow the B layout
+  constexpr int CHUNKS = ABYTES / 16 + 8;  // + the 32 norms (128 B)
+  constexpr int PER = (CHUNKS + kPhiThreads - 1) / kPhiThreads;
+  __shared__ u32x4 stage[2][CHUNKS];
   const int t = threadIdx.x, lane = t & (kWave - 1), w = t / kWave, g = lane >> 5, col = lane & 31;
-  const int i0 = blockIdx.x * CT;
   const int nrb = (n + 31) / 32;
-  load_itile<DP, NT>(it, x, mean, n, d, i0);
-  const float ni = it.ni[col];
-  const int i = i0 + col;
-  // upper != 0: only the j blocks on or above the diagonal block (rows j <= columns i, tile-wise)
-  const int rb_end = upper ? min(nrb, (int)blockIdx.x + 1) : nrb;
-  for (int rb = blockIdx.y * NW + w; rb < rb_end; rb += gridDim.y * NW) {
+  const int tpg = (nrb + kDistJB - 1) / kDistJB;  // tasks per group, full matrix
+  int gi, task;
+  const int k = blockIdx.x;
+  if (upper) {
+    // k = gi (gi + 1) / 2 + task, task <= gi
+    gi = (int)((sqrt(8.0 * (double)k + 1.0) - 1.0) * 0.5);
+    while ((int64_t)(gi + 1) * (gi + 2) / 2 <= k) ++gi;
+    while ((int64_t)gi * (gi + 1) / 2 > k) --gi;
+    task = k - gi * (gi + 1) / 2;
+  } else {
+    gi = k / tpg;
+    task = k - gi * tpg;
+  }
+  const int i0 = gi * kPhiI + w * 32;
+  const int rb_all = upper ? min(nrb, (gi + 1) * (kPhiI / 32)) : nrb;
+  const int rb0 = min(rb_all, task * kDistJB), rb1 = min(rb_all, rb0 + kDistJB);
+
+  bf16x8 bih[KB], bil[KB];
+  float nih = 0.f;
+  {
+    const int i = i0 + col;
+#pragma unroll
+    for (int kb = 0; kb < KB; ++kb)
+#pragma unroll
+      for (int s2 = 0; s2 < 8; ++s2) {
+        const int k = kb * 16 + g * 8 + s2;
+        const float v = (i < n && k < d) ? x[(int64_t)i * d + k] - mean[k] : 0.f;
+        nih += v * v;
+        __bf16 h, l;
+        split_bf16(v, h, l);
+        bih[kb][s2] = h;
+        bil[kb][s2] = l;
+      }
+  }
+  const float ni = nih + __shfl_xor(nih, 32, kWave);
+
+  // chunk c < ABYTES / 16: A layout; the last 8 chunks: the norms
+  auto src_chunk = [&](int rb, int c) -> const u32x4* {
     const char* blk = packed + (int64_t)rb * L::BLOCK_BYTES;
-    const floatx16 z = gram_block<DP>(reinterpret_cast<const bf16x8*>(blk), it, lane);
-    const float* nj = reinterpret_cast<const float*>(blk + L::BLOCK_BF16 * 2) + g * 16;
+    return c < ABYTES / 16 ? reinterpret_cast<const u32x4*>(blk) + c
+                           : reinterpret_cast<const u32x4*>(blk + L::BLOCK_BF16 * 2) + (c - ABYTES / 16);
+  };
+  u32x4 regs[PER];
+  const int nsteps = rb1 - rb0;
+  if (nsteps > 0) {
+#pragma unroll
+    for (int p = 0; p < PER; ++p)
+      if (t + p * kPhiThreads < CHUNKS) stage[0][t + p * kPhiThreads] = *src_chunk(rb0, t + p * kPhiThreads);
+  }
+  __syncthreads();
+  for (int st = 0; st < nsteps; ++st) {
+    if (st + 1 < nsteps) {
+#pragma unroll
+      for (int p = 0; p < PER; ++p)
+        if (t + p * kPhiThreads < CHUNKS) regs[p] = *src_chunk(rb0 + st + 1, t + p * kPhiThreads);
+    }
+    const char* blk = reinterpret_cast<const char*>(stage[st & 1]);
+    const bf16x8* A = reinterpret_cast<const bf16x8*>(blk);
+    floatx16 z;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) z[r] = 0.f;
+#pragma unroll
+    for (int kb = 0; kb < KB; ++kb) {
+      const bf16x8 ah = A[(0 * KB + kb) * 64 + lane];
+      const bf16x8 al = A[(1 * KB + kb) * 64 + lane];
+      z = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bih[kb], z, 0, 0, 0);
+      z = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bil[kb], z, 0, 0, 0);
+      z = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bih[kb], z, 0, 0, 0);
+    }
+    const float* nj = reinterpret_cast<const float*>(blk + ABYTES) + g * 16;
+    const int jbase = (rb0 + st) * 32, i = i0 + col;
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
-      const int j = rb * 32 + acc_row(r, lane);
+      const int j = jbase + acc_row(r, lane);
       if (i < n && j < n) d2[(int64_t)j * n + i] = (i == j) ? 0.f : fmaxf(ni + nj[r] - 2.f * z[r], 0.f);
     }
+    if (st + 1 < nsteps) {
+      u32x4* dst = stage[(st + 1) & 1];
+#pragma unroll
+      for (int p = 0; p < PER; ++p)
+        if (t + p * kPhiThreads < CHUNKS) dst[t + p * kPhiThreads] = regs[p];
+    }
+    __syncthreads();
   }
 }
 
@@ -860,10 +941,11 @@ int svgd_step_dp(const float* x_in, const float* grad, int n, int d, float bandw
 template <int DP>
 void sqdist_packed_dp(const float* x, int n, int d, const void* packed, const float* mean, int upper, float* d2,
                       hipStream_t st) {
-  const int nrb = (n + 31) / 32;
-  const int gy = std::max(1, std::min(64, nrb / (kDistThreads / kWave)));
-  hipLaunchKernelGGL((sqdist_kernel<DP, kDistThreads>), dim3((unsigned)nrb, (unsigned)gy), dim3(kDistThreads), 0, st,
-                     (const char*)packed, x, mean, n, d, upper, d2);
+  const int groups = (n + kPhiI - 1) / kPhiI, nrb = (n + 31) / 32;
+  const int64_t tasks = upper ? (int64_t)groups * (groups + 1) / 2
+                              : (int64_t)groups * ((nrb + kDistJB - 1) / kDistJB);
+  hipLaunchKernelGGL((sqdist_kernel<DP>), dim3((unsigned)tasks), dim3(kPhiThreads), 0, st, (const char*)packed, x,
+                     mean, n, d, upper, d2);
 }
 
 template <int DP>
