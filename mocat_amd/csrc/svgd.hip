@@ -46,89 +46,116 @@ struct SvgdLayout {
   static constexpr int BLOCK_BYTES = BLOCK_BF16 * 2 + 32 * 4;
 };
 
-// Column means, FP64, in a fixed summation order. One workgroup per 64 columns: lane = column
-// (coalesced row segments), wave w sums rows w, w + 16, ... in 8 interleaved accumulators, and the
-// 16 wave sums are added in wave order.
-constexpr int kMeanThreads = 1024;
-__global__ __launch_bounds__(kMeanThreads) void colmean_kernel(const float* __restrict__ x, int n, int d,
-                                                               float* __restrict__ mean) {
-  constexpr int NW = kMeanThreads / kWave, U = 8;
-  __shared__ double red[NW][kWave];
+// Column means, FP64, in a fixed summation order, in two passes: workgroup (column group cg, row slice
+// rs) sums its slice's rows (lane = column, coalesced row segments; wave w takes rows w, w+4, ...;
+// the four wave sums in wave order) into partial[rs][column]; then one thread per column adds the
+// slices in order.
+constexpr int kMeanSlices = 64;
+__global__ __launch_bounds__(256) void colmean_part_kernel(const float* __restrict__ x, int n, int d, int slices,
+                                                           double* __restrict__ partial) {
+  __shared__ double red[4][kWave];
   const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
-  const int k = blockIdx.x * kWave + lane;
-  double acc[U];
-#pragma unroll
-  for (int u = 0; u < U; ++u) acc[u] = 0.0;
+  const int k = blockIdx.x * kWave + lane, rs = blockIdx.y;
+  const int per = (n + slices - 1) / slices, r0 = rs * per, r1 = min(n, r0 + per);
+  double acc0 = 0.0, acc1 = 0.0;
   if (k < d) {
-    int i = w;
-    for (; i + (U - 1) * NW < n; i += U * NW) {
-#pragma unroll
-      for (int u = 0; u < U; ++u) acc[u] += (double)x[(int64_t)(i + u * NW) * d + k];
+    int i = r0 + w;
+    for (; i + 4 < r1; i += 8) {
+      acc0 += (double)x[(int64_t)i * d + k];
+      acc1 += (double)x[(int64_t)(i + 4) * d + k];
     }
-    for (; i < n; i += NW) acc[0] += (double)x[(int64_t)i * d + k];
+    for (; i < r1; i += 4) acc0 += (double)x[(int64_t)i * d + k];
   }
-  double s = 0.0;
-#pragma unroll
-  for (int u = 0; u < U; ++u) s += acc[u];
-  red[w][lane] = s;
+  red[w][lane] = acc0 + acc1;
   __syncthreads();
-  if (w == 0 && k < d) {
-    double t = 0.0;
-    for (int q = 0; q < NW; ++q) t += red[q][lane];
-    mean[k] = (float)(t / (double)n);
-  }
+  if (w == 0 && k < d) partial[(int64_t)rs * d + k] = red[0][lane] + red[1][lane] + red[2][lane] + red[3][lane];
 }
 
-// Pack the centred particles (A layout), [grad | centred x] (B layout) and |x_j|^2 per 32-row block.
+__global__ __launch_bounds__(kWave) void colmean_final_kernel(const double* __restrict__ partial, int n, int d,
+                                                              int slices, float* __restrict__ mean) {
+  const int k = blockIdx.x * kWave + threadIdx.x;
+  if (k >= d) return;
+  // loads issued in groups of 16 ahead of the in-order additions (slices <= kMeanSlices)
+  double t = 0.0;
+  for (int q0 = 0; q0 < slices; q0 += 16) {
+    double v[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) v[u] = q0 + u < slices ? partial[(int64_t)(q0 + u) * d + k] : 0.0;
+#pragma unroll
+    for (int u = 0; u < 16; ++u) t += v[u];
+  }
+  mean[k] = (float)(t / (double)n);
+}
+
+// Pack the centred particles (A layout), [grad | centred x] (B layout) and |x_j - mean|^2 per
+// 32-row block, one workgroup per block: a thread writes 8 consecutive bf16 (16 bytes) of the hi and of
+// the lo part; the norms add the A threads' 8-column partial sums of a row in column order.
 template <int DP>
-__global__ void svgd_pack_kernel(const float* __restrict__ x, const float* __restrict__ g,
-                                 const float* __restrict__ mean, int n, int d, char* __restrict__ out) {
+__global__ __launch_bounds__(256) void svgd_pack_kernel(const float* __restrict__ x, const float* __restrict__ g,
+                                                        const float* __restrict__ mean, int n, int d,
+                                                        char* __restrict__ out) {
   using L = SvgdLayout<DP>;
-  const int nrb = (n + 31) / 32;
-  const int64_t per = L::BLOCK_BF16 + 32;
-  const int64_t total = (int64_t)nrb * per;
-  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
-    const int rb = (int)(e / per);
-    const int q = (int)(e - rb * per);
-    char* blk = out + (int64_t)rb * L::BLOCK_BYTES;
-    if (q < L::A_ELEMS) {
-      const int s = q & 7, lane = (q >> 3) & 63, pk = q >> 9;
-      const int part = pk / L::KB, kb = pk - part * L::KB;
-      const int row = rb * 32 + (lane & 31), col = kb * 16 + (lane >> 5) * 8 + s;
-      const float v = (row < n && col < d) ? x[(int64_t)row * d + col] - mean[col] : 0.f;
+  constexpr int KB = L::KB, NB = L::NB;
+  __shared__ float nsq[32][2 * KB];
+  const int rb = blockIdx.x, t = threadIdx.x;
+  char* blk = out + (int64_t)rb * L::BLOCK_BYTES;
+  bf16x8* A = reinterpret_cast<bf16x8*>(blk);
+  bf16x8* B = A + L::A_ELEMS / 8;
+  // A layout [2][KB][64][8]: lane l holds row l%32, columns kb*16 + (l/32)*8 + s
+  for (int task = t; task < KB * 64; task += 256) {
+    const int kb = task / 64, lane = task % 64;
+    const int rl = lane & 31, row = rb * 32 + rl, c0 = kb * 16 + (lane >> 5) * 8;
+    bf16x8 h8, l8;
+    float ss = 0.f;
+#pragma unroll
+    for (int s2 = 0; s2 < 8; ++s2) {
+      const int c = c0 + s2;
+      const float v = (row < n && c < d) ? x[(int64_t)row * d + c] - mean[c] : 0.f;
+      ss += v * v;
       __bf16 h, l;
       split_bf16(v, h, l);
-      reinterpret_cast<__bf16*>(blk)[q] = part ? l : h;
-    } else if (q < L::BLOCK_BF16) {
-      const int qq = q - L::A_ELEMS;
-      const int s = qq & 7, lane = (qq >> 3) & 63, rest = qq >> 9;
-      const int nb = rest % L::NB, pk2 = rest / L::NB, kb2 = pk2 & 1, part = pk2 >> 1;
-      const int row = rb * 32 + acc_row(kb2 * 8 + s, lane), col = nb * 32 + (lane & 31);
+      h8[s2] = h;
+      l8[s2] = l;
+    }
+    A[(0 * KB + kb) * 64 + lane] = h8;
+    A[(1 * KB + kb) * 64 + lane] = l8;
+    nsq[rl][kb * 2 + (lane >> 5)] = ss;
+  }
+  // B layout [2][2][NB][64][8]: lane l holds column nb*32 + l%32 (grad for < DP, centred x above), rows
+  // acc_row(kb2*8 + s, l)
+  for (int task = t; task < 2 * NB * 64; task += 256) {
+    const int kb2 = task / (NB * 64), rest = task % (NB * 64), nb = rest / 64, lane = rest % 64;
+    const int c = nb * 32 + (lane & 31);
+    bf16x8 h8, l8;
+#pragma unroll
+    for (int s2 = 0; s2 < 8; ++s2) {
+      const int row = rb * 32 + acc_row(kb2 * 8 + s2, lane);
       float v = 0.f;
       if (row < n) {
-        if (col < DP) {
-          if (col < d) v = g[(int64_t)row * d + col];
-        } else if (col - DP < d) {
-          v = x[(int64_t)row * d + (col - DP)] - mean[col - DP];
+        if (c < DP) {
+          if (c < d) v = g[(int64_t)row * d + c];
+        } else if (c - DP < d) {
+          v = x[(int64_t)row * d + (c - DP)] - mean[c - DP];
         }
       }
       __bf16 h, l;
       split_bf16(v, h, l);
-      reinterpret_cast<__bf16*>(blk)[q] = part ? l : h;
-    } else {
-      const int qq = q - L::BLOCK_BF16;
-      const int gg = qq >> 4, r = qq & 15;
-      const int row = rb * 32 + 8 * (r >> 2) + 4 * gg + (r & 3);
-      float nrm = -1.f;  // padding rows
-      if (row < n) {
-        nrm = 0.f;
-        for (int k = 0; k < d; ++k) {
-          const float v = x[(int64_t)row * d + k] - mean[k];
-          nrm += v * v;
-        }
-      }
-      reinterpret_cast<float*>(blk + L::BLOCK_BF16 * 2)[qq] = nrm;
+      h8[s2] = h;
+      l8[s2] = l;
     }
+    B[((0 * 2 + kb2) * NB + nb) * 64 + lane] = h8;
+    B[((1 * 2 + kb2) * NB + nb) * 64 + lane] = l8;
+  }
+  __syncthreads();
+  // norms in accumulator-row order: entry qq = g*16 + r is row 8*(r>>2) + 4*g + (r&3); padding rows -1
+  if (t < 32) {
+    const int gg = t >> 4, r = t & 15, rl = 8 * (r >> 2) + 4 * gg + (r & 3), row = rb * 32 + rl;
+    float nrm = -1.f;
+    if (row < n) {
+      nrm = 0.f;
+      for (int q = 0; q < 2 * KB; ++q) nrm += nsq[rl][q];
+    }
+    reinterpret_cast<float*>(blk + L::BLOCK_BF16 * 2)[t] = nrm;
   }
 }
 
@@ -420,6 +447,23 @@ __global__ __launch_bounds__(kPhiThreads) void svgd_phi_kernel(const char* __res
     }
 }
 
+// phi of one coordinate from its sums, then the adagrad step: returns the new coordinate, updates G
+// and m. Contraction off and the fused multiply-adds written out, so that every kernel using it rounds
+// alike whatever the surrounding code.
+struct AdagradOut {
+  float x, G, m, phi;
+};
+MOCAT_DEV AdagradOut svgd_adagrad(float xv, float xc, float kg, float kx, float s, float G, float mprev, float inv_n,
+                                  float inv_h2, float step, float momentum) {
+#pragma clang fp contract(off)
+  const float phi = inv_n * fmaf(fmaf(s, xc, -kx), inv_h2, -kg);
+  const float gr = -phi;
+  const float G2 = fmaf(gr, gr, G);
+  const float ginv = G2 > 0.f ? 1.f / sqrtf(G2) : 0.f;
+  const float m = fmaf(momentum, mprev, (1.f - momentum) * (gr * ginv));
+  return {fmaf(-step, m, xv), G2, m, phi};
+}
+
 // phi from the partial sums, then the adagrad step (x, G, m updated in place); phi_out optional.
 __global__ void svgd_update_kernel(const float* __restrict__ part_o, const float* __restrict__ part_s, int js_count,
                                    int n, int d, int DP, const float* __restrict__ mean, float inv_n, float h_host,
@@ -439,16 +483,12 @@ __global__ void svgd_update_kernel(const float* __restrict__ part_o, const float
       kx += o[DP + k];
       s += part_s[(int64_t)js * n + i];
     }
-    const float xc = x[e] - mean[k];
-    const float phi = inv_n * (-kg + (s * xc - kx) * inv_h2);
-    if (phi_out) phi_out[e] = phi;
-    const float gr = -phi;
-    const float G = gsq[e] + gr * gr;
-    gsq[e] = G;
-    const float ginv = G > 0.f ? 1.f / sqrtf(G) : 0.f;
-    const float m = (1.f - momentum) * (gr * ginv) + momentum * mom[e];
-    mom[e] = m;
-    x[e] = x[e] - step * m;
+    const float xv = x[e];
+    const AdagradOut u = svgd_adagrad(xv, xv - mean[k], kg, kx, s, gsq[e], mom[e], inv_n, inv_h2, step, momentum);
+    if (phi_out) phi_out[e] = u.phi;
+    gsq[e] = u.G;
+    mom[e] = u.m;
+    x[e] = u.x;
   }
 }
 
@@ -459,6 +499,9 @@ __global__ __launch_bounds__(256) void gmm_kernel(const float* __restrict__ x, i
                                                   const float* __restrict__ mu, const float* __restrict__ lam,
                                                   const float* __restrict__ logc, float* __restrict__ pot,
                                                   float* __restrict__ grad) {
+  // contraction off and the fused multiply-adds written out (as in svgd_update_gmm_kernel, which must
+  // round alike)
+#pragma clang fp contract(off)
   const int lane = threadIdx.x & (kWave - 1);
   const int64_t i = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
   if (i >= n) return;
@@ -469,10 +512,10 @@ __global__ __launch_bounds__(256) void gmm_kernel(const float* __restrict__ x, i
     float a = 0.f;
     for (int j = lane; j < d; j += kWave) {
       const float df = xi[j] - mu[k * d + j];
-      a += lam[k * d + j] * df * df;
+      a = fmaf(lam[k * d + j] * df, df, a);
     }
     a = wave_sum(a);
-    q[k] = logc[k] - 0.5f * a;
+    q[k] = fmaf(-0.5f, a, logc[k]);
     qmax = fmaxf(qmax, q[k]);
   }
   float ssum = 0.f;
@@ -480,7 +523,7 @@ __global__ __launch_bounds__(256) void gmm_kernel(const float* __restrict__ x, i
   if (lane == 0) pot[i] = -(qmax + __logf(ssum));
   for (int j = lane; j < d; j += kWave) {
     float gsum = 0.f;
-    for (int k = 0; k < K; ++k) gsum += __expf(q[k] - qmax) / ssum * lam[k * d + j] * (xi[j] - mu[k * d + j]);
+    for (int k = 0; k < K; ++k) gsum = fmaf(__expf(q[k] - qmax) / ssum * lam[k * d + j], xi[j] - mu[k * d + j], gsum);
     grad[i * d + j] = gsum;
   }
 }
@@ -691,12 +734,52 @@ struct Bracket {
   uint32_t ok;         // the last selection ran on the candidates
   float delta;         // relative half-width of the next bracket
   uint32_t overflow;   // some workgroup had more candidates than its LDS buffer
+  uint32_t ticket;     // count-pass workgroups done (the last one decides, then clears it)
 };
 
 constexpr uint32_t kBracketCap = 1u << 20;  // candidate capacity (4 MB)
 
+// Selection state of the fused run: st[k] (k = 0..4) = {prefix0, prefix1, rank0, rank1} before radix
+// pass k (st[4]: the selected bit patterns), then one 2 x 256-bin histogram per pass.
+constexpr int kSelStateWords = 5 * 4;
+constexpr int kSelWords = kSelStateWords + 4 * 512;
+
+// The digit choice of a finished pass, by threads 0..255 of the calling workgroup (bin = thread):
+// each rank's bin from prefix sums over the pass histogram, recorded in the rank's prefix and the rank
+// reduced (exactly one bin holds each rank). out (LDS) receives the next state; every thread of the
+// workgroup calls this (it holds barriers).
+MOCAT_DEV void radix_choose_block(int shift, const uint32_t* __restrict__ st_in, const uint32_t* __restrict__ hist,
+                                  uint32_t* out, uint32_t (*wtot)[4]) {
+  const int b = threadIdx.x, lane = b & (kWave - 1), w = b / kWave;
+  uint32_t c[2] = {0u, 0u}, inc[2] = {0u, 0u};
+  if (b < 4) out[b] = st_in[b];
+  if (b < 256) {
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      c[t] = hist[t * 256 + b];
+      inc[t] = wave_inclusive_scan(c[t]);
+      if (lane == kWave - 1) wtot[t][w] = inc[t];
+    }
+  }
+  __syncthreads();
+  if (b < 256) {
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      uint32_t off = 0;
+      for (int q = 0; q < w; ++q) off += wtot[t][q];
+      const uint32_t lo = off + inc[t] - c[t], hi = off + inc[t], k = st_in[2 + t];
+      if (k >= lo && k < hi) {
+        out[t] = (st_in[t] & ~(255u << shift)) | ((uint32_t)b << shift);
+        out[2 + t] = k - lo;
+      }
+    }
+  }
+  __syncthreads();
+}
+
 __global__ __launch_bounds__(kSelThreads) void bracket_count_kernel(const uint32_t* __restrict__ v, int n, Bracket* br,
-                                                            uint32_t* __restrict__ cand) {
+                                                            uint32_t* __restrict__ cand, uint32_t* __restrict__ st0,
+                                                            uint32_t rank0, uint32_t rank1) {
   // candidates gather in LDS first (one LDS atomic per wave and tile) and reach the global buffer with
   // one global atomic per workgroup; a workgroup whose candidates overflow its LDS marks the bracket
   // unusable (the selection then runs over all pairs)
@@ -752,31 +835,50 @@ __global__ __launch_bounds__(kSelThreads) void bracket_count_kernel(const uint32
   const uint32_t gbase = lbase, keep = cnt < (uint32_t)kLocal ? cnt : (uint32_t)kLocal;
   for (uint32_t k = threadIdx.x; k < keep; k += kSelThreads)
     if (gbase + k < kBracketCap) cand[gbase + k] = lbuf[k];
-}
-
-// Sets up the two-rank selection: on the candidates (ranks minus the count below) when both ranks lie
-// inside the bracket and the candidates fit, else on all pairs.
-__global__ void bracket_decide_kernel(Bracket* br, uint32_t* __restrict__ sel, uint32_t rank0, uint32_t rank1) {
-  const bool ok = !br->overflow && br->below <= rank0 && (uint64_t)rank1 < (uint64_t)br->below + br->inside &&
-                  br->inside <= kBracketCap;
-  for (int b = threadIdx.x; b < 4 + 512; b += blockDim.x) sel[b] = 0u;
-  __syncthreads();
+  // the last workgroup to finish sets up the selection: on the candidates (ranks minus the count
+  // below) when both ranks lie inside the bracket and the candidates fit, else on all pairs. The
+  // candidate stores and counter atomics of every workgroup precede its ticket (fence), and the
+  // totals are read with atomic loads after it.
   if (threadIdx.x == 0) {
-    sel[2] = ok ? rank0 - br->below : rank0;
-    sel[3] = ok ? rank1 - br->below : rank1;
-    br->ok = ok ? 1u : 0u;
+    __threadfence();
+    const uint32_t done = atomicAdd(&br->ticket, 1u);
+    if (done == gridDim.x - 1) {
+      __threadfence();
+      const uint32_t below = __hip_atomic_load(&br->below, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const uint32_t inside = __hip_atomic_load(&br->inside, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const uint32_t over = __hip_atomic_load(&br->overflow, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const bool ok = !over && below <= rank0 && (uint64_t)rank1 < (uint64_t)below + inside && inside <= kBracketCap;
+      st0[0] = 0u;
+      st0[1] = 0u;
+      st0[2] = ok ? rank0 - below : rank0;
+      st0[3] = ok ? rank1 - below : rank1;
+      br->ok = ok ? 1u : 0u;
+      br->ticket = 0u;
+    }
   }
 }
 
-// One radix pass over the candidates or over all pairs (upper triangle), as decided.
+// Radix pass `pass` (digit shift 24 - 8 pass) over the candidates or over all pairs (upper triangle),
+// as decided. Each workgroup first applies the digit choice of the previous pass to its state
+// (workgroup 0 records it as st[pass] for the next pass and the finishing kernel).
 __global__ __launch_bounds__(kSelThreads) void radix_hist_bracket_kernel(const uint32_t* __restrict__ v, int n,
                                                                  const uint32_t* __restrict__ cand,
-                                                                 const Bracket* br, int shift,
-                                                                 const uint32_t* __restrict__ state,
-                                                                 uint32_t* __restrict__ hist) {
+                                                                 const Bracket* br, int pass,
+                                                                 uint32_t* __restrict__ sel) {
   __shared__ uint32_t lh[2][256];
+  __shared__ uint32_t wtot[2][4];
+  __shared__ uint32_t cur[4];
+  const int shift = 24 - 8 * pass;
+  uint32_t* hist = sel + kSelStateWords + pass * 512;
+  if (pass == 0) {
+    if (threadIdx.x < 4) cur[threadIdx.x] = sel[threadIdx.x];
+    __syncthreads();
+  } else {
+    radix_choose_block(shift + 8, sel + 4 * (pass - 1), hist - 512, cur, wtot);
+    if (blockIdx.x == 0 && threadIdx.x < 4) sel[4 * pass + threadIdx.x] = cur[threadIdx.x];
+  }
   const uint32_t mask = shift == 24 ? 0u : (0xFFFFFFFFu << (shift + 8));
-  const uint32_t p0 = state[0] & mask, p1 = state[1] & mask;
+  const uint32_t p0 = cur[0] & mask, p1 = cur[1] & mask;
   const bool shared_prefix = p0 == p1;
   const int lane = threadIdx.x & (kWave - 1);
   for (int b = threadIdx.x; b < 512; b += kSelThreads) (&lh[0][0])[b] = 0u;
@@ -820,15 +922,28 @@ __global__ __launch_bounds__(kSelThreads) void radix_hist_bracket_kernel(const u
   }
 }
 
-// Next bracket around the selected value; counters cleared for the next count pass.
-__global__ void bracket_update_kernel(Bracket* br, const uint32_t* __restrict__ sel) {
+// End of a bracketed selection: the last digit choice (st[4] = the two order statistics), the median
+// bandwidth from them (FP64 square roots, their mean, divided by sqrt(2 log n) computed on the host,
+// rounded to f32), the next bracket around the upper one, and the counters and histograms cleared for
+// the next selection.
+__global__ __launch_bounds__(256) void bracket_finish_kernel(Bracket* br, uint32_t* __restrict__ sel, double norm,
+                                                             float* h, float* trace) {
+  __shared__ uint32_t wtot[2][4];
+  __shared__ uint32_t cur[4];
+  radix_choose_block(0, sel + 12, sel + kSelStateWords + 3 * 512, cur, wtot);
+  for (int b = threadIdx.x; b < 4 * 512; b += blockDim.x) sel[kSelStateWords + b] = 0u;
   if (threadIdx.x != 0) return;
+  for (int t = 0; t < 4; ++t) sel[16 + t] = cur[t];
+  const double v0 = (double)__uint_as_float(cur[0]), v1 = (double)__uint_as_float(cur[1]);
+  const float hv = (float)(0.5 * (sqrt(v0) + sqrt(v1)) / norm);
+  *h = hv;
+  if (trace) *trace = hv;
   float delta = br->delta;
   if (!br->ok)
     delta = fminf(delta * 4.f, 1.f);
   else if (br->inside < kBracketCap / 8)
     delta = fmaxf(delta * 0.75f, 1e-5f);
-  const float center = __uint_as_float(sel[1]);
+  const float center = __uint_as_float(cur[1]);
   const float lo = center * (1.f - delta), hi = center * (1.f + delta);
   br->lo = __float_as_uint(fmaxf(lo, 0.f));
   br->hi = hi > center ? __float_as_uint(hi) : 0x7F800001u;  // above every finite value
@@ -838,38 +953,92 @@ __global__ void bracket_update_kernel(Bracket* br, const uint32_t* __restrict__ 
   br->overflow = 0u;
 }
 
-// First bracket: everything (the first selection runs over all pairs).
-__global__ void bracket_init_kernel(Bracket* br) {
+// First bracket: empty (no candidates: the first selection runs over all pairs); histograms cleared.
+__global__ __launch_bounds__(256) void bracket_init_kernel(Bracket* br, uint32_t* __restrict__ sel) {
+  for (int b = threadIdx.x; b < kSelWords; b += blockDim.x) sel[b] = 0u;
   if (threadIdx.x == 0) {
     br->lo = 0u;
-    br->hi = 0u;  // empty: no candidates, the selection falls back to all pairs
+    br->hi = 0u;
     br->below = 0u;
     br->inside = 0u;
     br->ok = 0u;
     br->overflow = 0u;
+    br->ticket = 0u;
     br->delta = 2e-3f;
   }
 }
 
 // ---------------------------------------------------------------- fused SVGD run helpers
-// key <- split(key, num)[idx] (SVGD.update's key advance)
-__global__ void key_advance_kernel(uint32_t* key, uint32_t num, uint32_t idx) {
-  if (threadIdx.x == 0) {
-    const Key k = split_at(Key{key[0], key[1]}, num, idx);
+// The adagrad step of svgd_update_kernel and the mixture potential and gradient of gmm_kernel (times
+// the temperature T) in one pass, one wave per particle: the same operations in the same order, on
+// the updated row held in registers. Workgroup 0 also advances the key: key <- split(key, num)[idx]
+// (SVGD.update's key advance).
+__global__ __launch_bounds__(256) void svgd_update_gmm_kernel(
+    const float* __restrict__ part_o, const float* __restrict__ part_s, int js_count, int n, int d, int DP,
+    const float* __restrict__ mean, float inv_n, const float* __restrict__ h_dev, float step, float momentum,
+    float* __restrict__ x, float* __restrict__ gsq, float* __restrict__ mom, int K, const float* __restrict__ mu,
+    const float* __restrict__ lam, const float* __restrict__ logc, float T, float* __restrict__ pot,
+    float* __restrict__ grad, uint32_t* key, uint32_t knum, uint32_t kidx) {
+#pragma clang fp contract(off)
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    const Key k = split_at(Key{key[0], key[1]}, knum, kidx);
     key[0] = k.a;
     key[1] = k.b;
   }
-}
-
-// median bandwidth from the two selected squared distances (bit patterns): FP64 square roots, their
-// mean, divided by sqrt(2 log n) (computed on the host), rounded to f32
-__global__ void median_bandwidth_kernel(const uint32_t* bits, double norm, float* h, float* trace) {
-  if (threadIdx.x == 0) {
-    const double v0 = (double)__uint_as_float(bits[0]), v1 = (double)__uint_as_float(bits[1]);
-    const double med = 0.5 * (sqrt(v0) + sqrt(v1));
-    const float hv = (float)(med / norm);
-    *h = hv;
-    if (trace) *trace = hv;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int64_t i = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+  if (i >= n) return;
+  const float hb = *h_dev;
+  const float inv_h2 = 1.f / (hb * hb);
+  float xn[2] = {0.f, 0.f};  // d <= 128: columns lane and lane + 64
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    const int k = lane + q * kWave;
+    if (k >= d) continue;
+    const int64_t e = i * d + k;
+    float kg = 0.f, kx = 0.f, s = 0.f;
+    for (int js = 0; js < js_count; ++js) {
+      const float* o = part_o + ((int64_t)js * n + i) * 2 * DP;
+      kg += o[k];
+      kx += o[DP + k];
+      s += part_s[(int64_t)js * n + i];
+    }
+    const float xv = x[e];
+    const AdagradOut u = svgd_adagrad(xv, xv - mean[k], kg, kx, s, gsq[e], mom[e], inv_n, inv_h2, step, momentum);
+    gsq[e] = u.G;
+    mom[e] = u.m;
+    xn[q] = u.x;
+    x[e] = u.x;
+  }
+  float qk[8];
+  float qmax = -INFINITY;
+  for (int c = 0; c < K; ++c) {
+    float a = 0.f;
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const int j = lane + q * kWave;
+      if (j < d) {
+        const float df = xn[q] - mu[c * d + j];
+        a = fmaf(lam[c * d + j] * df, df, a);
+      }
+    }
+    a = wave_sum(a);
+    qk[c] = fmaf(-0.5f, a, logc[c]);
+    qmax = fmaxf(qmax, qk[c]);
+  }
+  float ssum = 0.f;
+  for (int c = 0; c < K; ++c) ssum += __expf(qk[c] - qmax);
+  if (lane == 0) {
+    const float u = -(qmax + __logf(ssum));
+    pot[i] = T != 1.f ? u * T : u;
+  }
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    const int j = lane + q * kWave;
+    if (j >= d) continue;
+    float gsum = 0.f;
+    for (int c = 0; c < K; ++c) gsum = fmaf(__expf(qk[c] - qmax) / ssum * lam[c * d + j], xn[q] - mu[c * d + j], gsum);
+    grad[i * d + j] = T != 1.f ? gsum * T : gsum;
   }
 }
 
@@ -891,25 +1060,19 @@ __global__ void copy_bandwidth_kernel(const float* h, float* trace) {
   if (threadIdx.x == 0) *trace = *h;
 }
 
-// potential and gradient times the temperature (SVGD uses T * U for a tempered scenario)
-__global__ void scale_kernel(float* pot, float* grad, int64_t n, int64_t nd, float T) {
-  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < nd; e += (int64_t)gridDim.x * blockDim.x) {
-    grad[e] *= T;
-    if (e < n) pot[e] *= T;
-  }
-}
-
-constexpr int kDistThreads = 256;
-
 // centring mean and the packed operands of x (A layout, norms) and [grad | x] (B layout)
 template <int DP>
 void svgd_prepare_dp(const float* x, const float* grad, int n, int d, void* packed, float* mean, hipStream_t st) {
-  using L = SvgdLayout<DP>;
-  hipLaunchKernelGGL(colmean_kernel, dim3((d + kWave - 1) / kWave), dim3(kMeanThreads), 0, st, x, n, d, mean);
   const int nrb = (n + 31) / 32;
-  const int64_t tot = (int64_t)nrb * (L::BLOCK_BF16 + 32);
-  const unsigned pb = (unsigned)std::min<int64_t>((tot + 255) / 256, 65535);
-  hipLaunchKernelGGL(svgd_pack_kernel<DP>, dim3(pb), dim3(256), 0, st, x, grad, mean, n, d, (char*)packed);
+  // column-mean partials live in the partial-sum region of the step workspace, which the Stein kernel
+  // writes only after the packing: slices * d doubles <= n * d * 8 bytes < its n * (2 * DP + 1) * 4
+  const int slices = std::max(1, std::min(kMeanSlices, (n + 63) / 64));
+  double* cpart = reinterpret_cast<double*>(reinterpret_cast<char*>(mean) + 512);
+  hipLaunchKernelGGL(colmean_part_kernel, dim3((d + kWave - 1) / kWave, slices), dim3(256), 0, st, x, n, d, slices,
+                     cpart);
+  hipLaunchKernelGGL(colmean_final_kernel, dim3((d + kWave - 1) / kWave), dim3(kWave), 0, st, (const double*)cpart, n,
+                     d, slices, mean);
+  hipLaunchKernelGGL(svgd_pack_kernel<DP>, dim3((unsigned)nrb), dim3(256), 0, st, x, grad, mean, n, d, (char*)packed);
 }
 
 // Stein direction from the prepared operands of x_in, then the adagrad step of x
@@ -1078,7 +1241,7 @@ static int64_t svgd_run_layout(int64_t n, int64_t d, int js_count, int64_t* o_d2
   *o_d2 = o;
   o += (n * n * 4 + 255) / 256 * 256;
   *o_sel = o;
-  o += 4 * (4 + 512) + 256 - 4 * (4 + 512) % 256;
+  o += ((int64_t)kSelWords * 4 + 255) / 256 * 256;
   *o_part = o;
   o += 1024 * 8;
   o = (o + 255) / 256 * 256;  // bracket state (256 B) then the candidate buffer
@@ -1111,7 +1274,7 @@ int mocat_svgd_gmm_run(float* x, int64_t n, int64_t d, int K, const float* means
   const int64_t o_br = (o_part + 1024 * 8 + 255) / 256 * 256;
   Bracket* br = reinterpret_cast<Bracket*>(ws + o_br);
   uint32_t* cand = reinterpret_cast<uint32_t*>(ws + o_br + 256);
-  if (bandwidth_mode == 2) hipLaunchKernelGGL(bracket_init_kernel, dim3(1), dim3(64), 0, stream, br);
+  if (bandwidth_mode == 2) hipLaunchKernelGGL(bracket_init_kernel, dim3(1), dim3(256), 0, stream, br, sel);
   const double norm = std::sqrt(2.0 * std::log((double)n));
   const int64_t m = n * n;
   const int64_t r0 = (m - 1) / 2, r1 = m / 2;  // both >= n for n >= 3
@@ -1130,20 +1293,19 @@ int mocat_svgd_gmm_run(float* x, int64_t n, int64_t d, int K, const float* means
       svgd_prepare_dp<128>(x, grad, (int)n, (int)d, ws, mean, stream);
   };
   prepare();
+  const unsigned phi_groups = (unsigned)((n + kPhiI - 1) / kPhiI), upd_blocks = (unsigned)((n + 3) / 4);
   for (int it = 0; it < n_iter; ++it) {
-    // SVGD.update: key advance, Stein direction and adagrad step in place, then the potential
-    hipLaunchKernelGGL(key_advance_kernel, dim3(1), dim3(64), 0, stream, key, (uint32_t)(n + 2), (uint32_t)n);
+    // SVGD.update: Stein direction, then the adagrad step in place with the key advance and the
+    // potential of the updated particles
     if (DP == 64)
-      svgd_phi_update_dp<64>(x, (int)n, (int)d, 1.f, bandwidth, js_count, ws, mean, part_o, part_s, step, momentum, x,
-                             gsq, mom, nullptr, stream);
+      hipLaunchKernelGGL((svgd_phi_kernel<64>), dim3(phi_groups, (unsigned)js_count), dim3(kPhiThreads), 0, stream,
+                         (const char*)ws, x, mean, (int)n, (int)d, 1.f, bandwidth, js_count, part_o, part_s);
     else
-      svgd_phi_update_dp<128>(x, (int)n, (int)d, 1.f, bandwidth, js_count, ws, mean, part_o, part_s, step, momentum,
-                              x, gsq, mom, nullptr, stream);
-    int rc = mocat_gmm_potential_grad(x, n, d, K, means, lam, logc, potential, grad, stream);
-    if (rc != MOCAT_OK) return rc;
-    if (temperature != 1.f)
-      hipLaunchKernelGGL(scale_kernel, dim3((unsigned)std::min<int64_t>((n * d + 255) / 256, 4096)), dim3(256), 0,
-                         stream, potential, grad, n, n * d, temperature);
+      hipLaunchKernelGGL((svgd_phi_kernel<128>), dim3(phi_groups, (unsigned)js_count), dim3(kPhiThreads), 0, stream,
+                         (const char*)ws, x, mean, (int)n, (int)d, 1.f, bandwidth, js_count, part_o, part_s);
+    hipLaunchKernelGGL(svgd_update_gmm_kernel, dim3(upd_blocks), dim3(256), 0, stream, part_o, part_s, js_count,
+                       (int)n, (int)d, DP, mean, 1.f / (float)n, bandwidth, step, momentum, x, gsq, mom, K, means, lam,
+                       logc, temperature, potential, grad, key, (uint32_t)(n + 2), (uint32_t)n);
     prepare();
     float* tr = bandwidth_trace ? bandwidth_trace + it : nullptr;
     if (bandwidth_mode == 0) {
@@ -1157,18 +1319,13 @@ int mocat_svgd_gmm_run(float* x, int64_t n, int64_t d, int K, const float* means
     if (bandwidth_mode == 2) {
       const unsigned rows = (unsigned)std::min<int64_t>(n - 1, kSelectBlocks);
       hipLaunchKernelGGL(bracket_count_kernel, dim3(rows), dim3(kSelThreads), 0, stream, (const uint32_t*)d2, (int)n, br,
-                         cand);
-      hipLaunchKernelGGL(bracket_decide_kernel, dim3(1), dim3(256), 0, stream, br, sel, (uint32_t)((r0 - n) / 2),
-                         (uint32_t)((r1 - n) / 2));
-      for (int shift = 24; shift >= 0; shift -= 8) {
-        hipLaunchKernelGGL(radix_hist_bracket_kernel, dim3(rows), dim3(kSelThreads), 0, stream, (const uint32_t*)d2, (int)n,
-                           (const uint32_t*)cand, (const Bracket*)br, shift, sel, sel + 4);
-        hipLaunchKernelGGL(radix_choose_kernel, dim3(1), dim3(256), 0, stream, shift, sel, sel + 4);
-      }
-      hipLaunchKernelGGL(median_bandwidth_kernel, dim3(1), dim3(64), 0, stream, sel, norm, bandwidth, tr);
-      hipLaunchKernelGGL(bracket_update_kernel, dim3(1), dim3(64), 0, stream, br, sel);
+                         cand, sel, (uint32_t)((r0 - n) / 2), (uint32_t)((r1 - n) / 2));
+      for (int pass = 0; pass < 4; ++pass)
+        hipLaunchKernelGGL(radix_hist_bracket_kernel, dim3(rows), dim3(kSelThreads), 0, stream, (const uint32_t*)d2,
+                           (int)n, (const uint32_t*)cand, (const Bracket*)br, pass, sel);
+      hipLaunchKernelGGL(bracket_finish_kernel, dim3(1), dim3(256), 0, stream, br, sel, norm, bandwidth, tr);
     } else {
-      rc = mocat_sqrt_sum_upper(d2, n, part, 1024, stream);
+      const int rc = mocat_sqrt_sum_upper(d2, n, part, 1024, stream);
       if (rc != MOCAT_OK) return rc;
       hipLaunchKernelGGL(mean_bandwidth_kernel, dim3(1), dim3(256), 0, stream, part, 1024,
                          1.0 / ((double)n * (double)n), norm, bandwidth, tr);
