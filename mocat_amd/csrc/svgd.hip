@@ -159,6 +159,36 @@ __global__ __launch_bounds__(256) void svgd_pack_kernel(const float* __restrict_
   }
 }
 
+// The distance GEMM's B operand for the 32 particles of packed block rb, read from its A layout (the
+// two operand layouts of the 32x32x16 product coincide: lane l holds particle l % 32, columns
+// kb * 16 + (l / 32) * 8 + s), and the norm of particle rb * 32 + lane % 32 (norms are stored in
+// accumulator-row order). Blocks past the end give zero operands.
+template <int DP>
+MOCAT_DEV float load_i_operands(const char* __restrict__ packed, int rb, int nrb, int lane, bf16x8* bih,
+                                bf16x8* bil) {
+  using L = SvgdLayout<DP>;
+  constexpr int KB = L::KB;
+  if (rb >= nrb) {
+#pragma unroll
+    for (int kb = 0; kb < KB; ++kb)
+#pragma unroll
+      for (int s2 = 0; s2 < 8; ++s2) {
+        bih[kb][s2] = (__bf16)0.f;
+        bil[kb][s2] = (__bf16)0.f;
+      }
+    return -1.f;
+  }
+  const char* blk = packed + (int64_t)rb * L::BLOCK_BYTES;
+  const bf16x8* A = reinterpret_cast<const bf16x8*>(blk);
+#pragma unroll
+  for (int kb = 0; kb < KB; ++kb) {
+    bih[kb] = A[kb * 64 + lane];
+    bil[kb] = A[(KB + kb) * 64 + lane];
+  }
+  const int col = lane & 31;
+  return reinterpret_cast<const float*>(blk + L::BLOCK_BF16 * 2)[((col >> 2) & 1) * 16 + 4 * (col >> 3) + (col & 3)];
+}
+
 // Tile of 32 particles i: centred coordinates in LDS, their squared norms, and the bf16 hi/lo
 // B operand of the distance GEMM (lane: particle l%32, columns kb*16 + (l/32)*8 + s).
 template <int DP>
@@ -316,9 +346,7 @@ typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 constexpr int kPhiWaves = 4, kPhiThreads = kPhiWaves * kWave, kPhiI = kPhiWaves * 32;
 
 template <int DP>
-__global__ __launch_bounds__(kPhiThreads) void svgd_phi_kernel(const char* __restrict__ packed,
-                                                               const float* __restrict__ x,
-                                                               const float* __restrict__ mean, int n, int d,
+__global__ __launch_bounds__(kPhiThreads) void svgd_phi_kernel(const char* __restrict__ packed, int n,
                                                                float h_host, const float* __restrict__ h_dev,
                                                                int js_count, float* __restrict__ part_o,
                                                                float* __restrict__ part_s) {
@@ -337,23 +365,7 @@ __global__ __launch_bounds__(kPhiThreads) void svgd_phi_kernel(const char* __res
 
   // B operand of the distance GEMM: lane holds particle i0 + col, columns kb*16 + g*8 + s
   bf16x8 bih[KB], bil[KB];
-  float nih = 0.f;
-  {
-    const int i = i0 + col;
-#pragma unroll
-    for (int kb = 0; kb < KB; ++kb)
-#pragma unroll
-      for (int s2 = 0; s2 < 8; ++s2) {
-        const int k = kb * 16 + g * 8 + s2;
-        const float v = (i < n && k < d) ? x[(int64_t)i * d + k] - mean[k] : 0.f;
-        nih += v * v;
-        __bf16 h, l;
-        split_bf16(v, h, l);
-        bih[kb][s2] = h;
-        bil[kb][s2] = l;
-      }
-  }
-  const float ni = nih + __shfl_xor(nih, 32, kWave);
+  const float ni = load_i_operands<DP>(packed, i0 / 32, nrb, lane, bih, bil);
 
   floatx16 gacc[NB];
 #pragma unroll
@@ -539,10 +551,8 @@ __global__ __launch_bounds__(256) void gmm_kernel(const float* __restrict__ x, i
 constexpr int kDistJB = kPhiI / 32;
 
 template <int DP>
-__global__ __launch_bounds__(kPhiThreads) void sqdist_kernel(const char* __restrict__ packed,
-                                                             const float* __restrict__ x,
-                                                             const float* __restrict__ mean, int n, int d,
-                                                             int upper, float* __restrict__ d2) {
+__global__ __launch_bounds__(kPhiThreads) void sqdist_kernel(const char* __restrict__ packed, int n, int upper,
+                                                             float* __restrict__ d2) {
   using L = SvgdLayout<DP>;
   constexpr int KB = L::KB;
   constexpr int ABYTES = L::A_ELEMS * 2;  // A layout of a block; the norms follow the B layout
@@ -569,23 +579,7 @@ __global__ __launch_bounds__(kPhiThreads) void sqdist_kernel(const char* __restr
   const int rb0 = min(rb_all, task * kDistJB), rb1 = min(rb_all, rb0 + kDistJB);
 
   bf16x8 bih[KB], bil[KB];
-  float nih = 0.f;
-  {
-    const int i = i0 + col;
-#pragma unroll
-    for (int kb = 0; kb < KB; ++kb)
-#pragma unroll
-      for (int s2 = 0; s2 < 8; ++s2) {
-        const int k = kb * 16 + g * 8 + s2;
-        const float v = (i < n && k < d) ? x[(int64_t)i * d + k] - mean[k] : 0.f;
-        nih += v * v;
-        __bf16 h, l;
-        split_bf16(v, h, l);
-        bih[kb][s2] = h;
-        bil[kb][s2] = l;
-      }
-  }
-  const float ni = nih + __shfl_xor(nih, 32, kWave);
+  const float ni = load_i_operands<DP>(packed, i0 / 32, nrb, lane, bih, bil);
 
   // chunk c < ABYTES / 16: A layout; the last 8 chunks: the norms
   auto src_chunk = [&](int rb, int c) -> const u32x4* {
@@ -734,7 +728,6 @@ struct Bracket {
   uint32_t ok;         // the last selection ran on the candidates
   float delta;         // relative half-width of the next bracket
   uint32_t overflow;   // some workgroup had more candidates than its LDS buffer
-  uint32_t ticket;     // count-pass workgroups done (the last one decides, then clears it)
 };
 
 constexpr uint32_t kBracketCap = 1u << 20;  // candidate capacity (4 MB)
@@ -778,8 +771,7 @@ MOCAT_DEV void radix_choose_block(int shift, const uint32_t* __restrict__ st_in,
 }
 
 __global__ __launch_bounds__(kSelThreads) void bracket_count_kernel(const uint32_t* __restrict__ v, int n, Bracket* br,
-                                                            uint32_t* __restrict__ cand, uint32_t* __restrict__ st0,
-                                                            uint32_t rank0, uint32_t rank1) {
+                                                            uint32_t* __restrict__ cand) {
   // candidates gather in LDS first (one LDS atomic per wave and tile) and reach the global buffer with
   // one global atomic per workgroup; a workgroup whose candidates overflow its LDS marks the bracket
   // unusable (the selection then runs over all pairs)
@@ -835,45 +827,38 @@ __global__ __launch_bounds__(kSelThreads) void bracket_count_kernel(const uint32
   const uint32_t gbase = lbase, keep = cnt < (uint32_t)kLocal ? cnt : (uint32_t)kLocal;
   for (uint32_t k = threadIdx.x; k < keep; k += kSelThreads)
     if (gbase + k < kBracketCap) cand[gbase + k] = lbuf[k];
-  // the last workgroup to finish sets up the selection: on the candidates (ranks minus the count
-  // below) when both ranks lie inside the bracket and the candidates fit, else on all pairs. The
-  // candidate stores and counter atomics of every workgroup precede its ticket (fence), and the
-  // totals are read with atomic loads after it.
-  if (threadIdx.x == 0) {
-    __threadfence();
-    const uint32_t done = atomicAdd(&br->ticket, 1u);
-    if (done == gridDim.x - 1) {
-      __threadfence();
-      const uint32_t below = __hip_atomic_load(&br->below, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      const uint32_t inside = __hip_atomic_load(&br->inside, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      const uint32_t over = __hip_atomic_load(&br->overflow, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      const bool ok = !over && below <= rank0 && (uint64_t)rank1 < (uint64_t)below + inside && inside <= kBracketCap;
-      st0[0] = 0u;
-      st0[1] = 0u;
-      st0[2] = ok ? rank0 - below : rank0;
-      st0[3] = ok ? rank1 - below : rank1;
-      br->ok = ok ? 1u : 0u;
-      br->ticket = 0u;
-    }
-  }
 }
 
-// Radix pass `pass` (digit shift 24 - 8 pass) over the candidates or over all pairs (upper triangle),
-// as decided. Each workgroup first applies the digit choice of the previous pass to its state
-// (workgroup 0 records it as st[pass] for the next pass and the finishing kernel).
+// Radix pass `pass` (digit shift 24 - 8 pass) over the candidates or over all pairs (upper triangle).
+// Pass 0 decides (every workgroup alike, from the finished count pass): on the candidates, with the
+// ranks reduced by the count below, when both ranks lie inside the bracket and the candidates fit, else
+// on all pairs; workgroup 0 records the decision and st[0]. Later passes first apply the digit choice
+// of the previous pass to their state (workgroup 0 records it as st[pass]).
 __global__ __launch_bounds__(kSelThreads) void radix_hist_bracket_kernel(const uint32_t* __restrict__ v, int n,
-                                                                 const uint32_t* __restrict__ cand,
-                                                                 const Bracket* br, int pass,
-                                                                 uint32_t* __restrict__ sel) {
+                                                                 const uint32_t* __restrict__ cand, Bracket* br,
+                                                                 int pass, uint32_t* __restrict__ sel,
+                                                                 uint32_t rank0, uint32_t rank1) {
   __shared__ uint32_t lh[2][256];
   __shared__ uint32_t wtot[2][4];
   __shared__ uint32_t cur[4];
   const int shift = 24 - 8 * pass;
   uint32_t* hist = sel + kSelStateWords + pass * 512;
+  bool use_cand;
   if (pass == 0) {
-    if (threadIdx.x < 4) cur[threadIdx.x] = sel[threadIdx.x];
+    const uint32_t below = br->below, inside = br->inside;
+    use_cand = !br->overflow && below <= rank0 && (uint64_t)rank1 < (uint64_t)below + inside &&
+               inside <= kBracketCap;
+    if (threadIdx.x == 0) {
+      cur[0] = 0u;
+      cur[1] = 0u;
+      cur[2] = use_cand ? rank0 - below : rank0;
+      cur[3] = use_cand ? rank1 - below : rank1;
+    }
     __syncthreads();
+    if (blockIdx.x == 0 && threadIdx.x < 4) sel[threadIdx.x] = cur[threadIdx.x];
+    if (blockIdx.x == 0 && threadIdx.x == 0) br->ok = use_cand ? 1u : 0u;
   } else {
+    use_cand = br->ok != 0u;
     radix_choose_block(shift + 8, sel + 4 * (pass - 1), hist - 512, cur, wtot);
     if (blockIdx.x == 0 && threadIdx.x < 4) sel[4 * pass + threadIdx.x] = cur[threadIdx.x];
   }
@@ -883,7 +868,7 @@ __global__ __launch_bounds__(kSelThreads) void radix_hist_bracket_kernel(const u
   const int lane = threadIdx.x & (kWave - 1);
   for (int b = threadIdx.x; b < 512; b += kSelThreads) (&lh[0][0])[b] = 0u;
   __syncthreads();
-  if (br->ok) {
+  if (use_cand) {
     const int64_t m = br->inside;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t base = blockIdx.x * (int64_t)blockDim.x; base < m; base += stride) {
@@ -963,7 +948,6 @@ __global__ __launch_bounds__(256) void bracket_init_kernel(Bracket* br, uint32_t
     br->inside = 0u;
     br->ok = 0u;
     br->overflow = 0u;
-    br->ticket = 0u;
     br->delta = 2e-3f;
   }
 }
@@ -990,21 +974,43 @@ __global__ __launch_bounds__(256) void svgd_update_gmm_kernel(
   if (i >= n) return;
   const float hb = *h_dev;
   const float inv_h2 = 1.f / (hb * hb);
+  // the j-split partial sums, added in split order; loads issued 8 splits ahead of the additions
+  constexpr int U = 8;
+  float kg[2] = {0.f, 0.f}, kx[2] = {0.f, 0.f}, s = 0.f;
+  for (int j0 = 0; j0 < js_count; j0 += U) {
+    float vg[U][2], vx[U][2], vs[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const bool in = j0 + u < js_count;
+      const float* o = part_o + ((int64_t)(j0 + u) * n + i) * 2 * DP;
+      vs[u] = in ? part_s[(int64_t)(j0 + u) * n + i] : 0.f;
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        const int k = lane + q * kWave;
+        vg[u][q] = in && k < d ? o[k] : 0.f;
+        vx[u][q] = in && k < d ? o[DP + k] : 0.f;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (j0 + u >= js_count) break;
+      s += vs[u];
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        kg[q] += vg[u][q];
+        kx[q] += vx[u][q];
+      }
+    }
+  }
   float xn[2] = {0.f, 0.f};  // d <= 128: columns lane and lane + 64
 #pragma unroll
   for (int q = 0; q < 2; ++q) {
     const int k = lane + q * kWave;
     if (k >= d) continue;
     const int64_t e = i * d + k;
-    float kg = 0.f, kx = 0.f, s = 0.f;
-    for (int js = 0; js < js_count; ++js) {
-      const float* o = part_o + ((int64_t)js * n + i) * 2 * DP;
-      kg += o[k];
-      kx += o[DP + k];
-      s += part_s[(int64_t)js * n + i];
-    }
     const float xv = x[e];
-    const AdagradOut u = svgd_adagrad(xv, xv - mean[k], kg, kx, s, gsq[e], mom[e], inv_n, inv_h2, step, momentum);
+    const AdagradOut u =
+        svgd_adagrad(xv, xv - mean[k], kg[q], kx[q], s, gsq[e], mom[e], inv_n, inv_h2, step, momentum);
     gsq[e] = u.G;
     mom[e] = u.m;
     xn[q] = u.x;
@@ -1081,8 +1087,8 @@ void svgd_phi_update_dp(const float* x_in, int n, int d, float bandwidth, const 
                         const void* packed, const float* mean, float* part_o, float* part_s, float step,
                         float momentum, float* x, float* gsq, float* mom, float* phi_out, hipStream_t st) {
   hipLaunchKernelGGL((svgd_phi_kernel<DP>), dim3((unsigned)((n + kPhiI - 1) / kPhiI), (unsigned)js_count),
-                     dim3(kPhiThreads), 0, st, (const char*)packed, x_in, mean, n, d, bandwidth, bandwidth_dev,
-                     js_count, part_o, part_s);
+                     dim3(kPhiThreads), 0, st, (const char*)packed, n, bandwidth, bandwidth_dev, js_count, part_o,
+                     part_s);
   const int64_t tot2 = (int64_t)n * d;
   const unsigned ub = (unsigned)std::min<int64_t>((tot2 + 255) / 256, 65535);
   hipLaunchKernelGGL(svgd_update_kernel, dim3(ub), dim3(256), 0, st, part_o, part_s, js_count, n, d, DP, mean,
@@ -1107,8 +1113,8 @@ void sqdist_packed_dp(const float* x, int n, int d, const void* packed, const fl
   const int groups = (n + kPhiI - 1) / kPhiI, nrb = (n + 31) / 32;
   const int64_t tasks = upper ? (int64_t)groups * (groups + 1) / 2
                               : (int64_t)groups * ((nrb + kDistJB - 1) / kDistJB);
-  hipLaunchKernelGGL((sqdist_kernel<DP>), dim3((unsigned)tasks), dim3(kPhiThreads), 0, st, (const char*)packed, x,
-                     mean, n, d, upper, d2);
+  hipLaunchKernelGGL((sqdist_kernel<DP>), dim3((unsigned)tasks), dim3(kPhiThreads), 0, st, (const char*)packed, n,
+                     upper, d2);
 }
 
 template <int DP>
@@ -1299,10 +1305,10 @@ int mocat_svgd_gmm_run(float* x, int64_t n, int64_t d, int K, const float* means
     // potential of the updated particles
     if (DP == 64)
       hipLaunchKernelGGL((svgd_phi_kernel<64>), dim3(phi_groups, (unsigned)js_count), dim3(kPhiThreads), 0, stream,
-                         (const char*)ws, x, mean, (int)n, (int)d, 1.f, bandwidth, js_count, part_o, part_s);
+                         (const char*)ws, (int)n, 1.f, bandwidth, js_count, part_o, part_s);
     else
       hipLaunchKernelGGL((svgd_phi_kernel<128>), dim3(phi_groups, (unsigned)js_count), dim3(kPhiThreads), 0, stream,
-                         (const char*)ws, x, mean, (int)n, (int)d, 1.f, bandwidth, js_count, part_o, part_s);
+                         (const char*)ws, (int)n, 1.f, bandwidth, js_count, part_o, part_s);
     hipLaunchKernelGGL(svgd_update_gmm_kernel, dim3(upd_blocks), dim3(256), 0, stream, part_o, part_s, js_count,
                        (int)n, (int)d, DP, mean, 1.f / (float)n, bandwidth, step, momentum, x, gsq, mom, K, means, lam,
                        logc, temperature, potential, grad, key, (uint32_t)(n + 2), (uint32_t)n);
@@ -1319,10 +1325,11 @@ int mocat_svgd_gmm_run(float* x, int64_t n, int64_t d, int K, const float* means
     if (bandwidth_mode == 2) {
       const unsigned rows = (unsigned)std::min<int64_t>(n - 1, kSelectBlocks);
       hipLaunchKernelGGL(bracket_count_kernel, dim3(rows), dim3(kSelThreads), 0, stream, (const uint32_t*)d2, (int)n, br,
-                         cand, sel, (uint32_t)((r0 - n) / 2), (uint32_t)((r1 - n) / 2));
+                         cand);
       for (int pass = 0; pass < 4; ++pass)
         hipLaunchKernelGGL(radix_hist_bracket_kernel, dim3(rows), dim3(kSelThreads), 0, stream, (const uint32_t*)d2,
-                           (int)n, (const uint32_t*)cand, (const Bracket*)br, pass, sel);
+                           (int)n, (const uint32_t*)cand, br, pass, sel, (uint32_t)((r0 - n) / 2),
+                           (uint32_t)((r1 - n) / 2));
       hipLaunchKernelGGL(bracket_finish_kernel, dim3(1), dim3(256), 0, stream, br, sel, norm, bandwidth, tr);
     } else {
       const int rc = mocat_sqrt_sum_upper(d2, n, part, 1024, stream);
