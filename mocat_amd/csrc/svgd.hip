@@ -540,6 +540,29 @@ __global__ __launch_bounds__(256) void gmm_kernel(const float* __restrict__ x, i
   }
 }
 
+// ---------------------------------------------------------------- bracketed median (fused SVGD run)
+// Between SVGD iterations the median pair distance moves little, so each selection first counts the
+// pairs below a bracket [lo, hi) around the previous median and copies those inside it to a candidate
+// buffer (in the fused run the distance kernel does this as it writes the distances). When the wanted
+// ranks fall inside the bracket (and the candidates fit), the radix passes
+// run over the candidates only, with the ranks shifted by the count below; otherwise over all pairs.
+// Either way the result is the exact order statistic. The bracket half-width (relative) shrinks
+// after hits and grows after misses.
+struct Bracket {
+  uint32_t lo, hi;     // bit patterns of non-negative floats (ordered as unsigned ints)
+  uint32_t below;      // pairs < lo
+  uint32_t inside;     // pairs in [lo, hi) (the candidate count, possibly above the capacity)
+  uint32_t ok;         // the last selection ran on the candidates
+  float delta;         // relative half-width of the next bracket
+  uint32_t overflow;   // some workgroup had more candidates than its LDS buffer
+  uint32_t done;       // the selection was finished by one workgroup in LDS (few candidates)
+  float prev;          // the previous selected value (0: none yet)
+};
+
+constexpr uint32_t kBracketCap = 1u << 20;  // candidate capacity (4 MB)
+constexpr uint32_t kSmallCap = 32768;       // candidates selected by one workgroup in LDS (128 KB)
+constexpr uint32_t kTargetCand = 4096;      // the bracket width aims at about this many candidates
+
 // ---------------------------------------------------------------- pairwise squared distances
 // As the Stein kernel: a workgroup owns 128 particles i (32 per wave, operands in registers) and a
 // share of the j blocks, each staged once through LDS (A layout and norms only: 16.5 KB at d <= 128,
@@ -550,16 +573,39 @@ __global__ __launch_bounds__(256) void gmm_kernel(const float* __restrict__ x, i
 // cut into equal tasks, numbered row by row.
 constexpr int kDistJB = kPhiI / 32;
 
-template <int DP>
+// |x_i - x_j|^2 from the norms and the Gram entry, clamped at 0; one explicit multiply-add (contraction
+// off), so that every use rounds alike
+MOCAT_DEV float pair_d2(float ni, float nj, float z) {
+#pragma clang fp contract(off)
+  return fmaxf(fmaf(-2.f, z, ni + nj), 0.f);
+}
+
+// COUNT: also the bracket count of the pairs i > j (the strict upper triangle of the selection):
+// pairs below br->lo counted, pairs in [lo, hi) gathered in LDS and appended to cand with one global
+// atomic per workgroup (a workgroup with more than its LDS holds marks the bracket unusable).
+template <int DP, bool COUNT>
 __global__ __launch_bounds__(kPhiThreads) void sqdist_kernel(const char* __restrict__ packed, int n, int upper,
-                                                             float* __restrict__ d2) {
+                                                             float* __restrict__ d2, Bracket* br,
+                                                             uint32_t* __restrict__ cand) {
   using L = SvgdLayout<DP>;
   constexpr int KB = L::KB;
   constexpr int ABYTES = L::A_ELEMS * 2;  // A layout of a block; the norms follow the B layout
   constexpr int CHUNKS = ABYTES / 16 + 8;  // + the 32 norms (128 B)
   constexpr int PER = (CHUNKS + kPhiThreads - 1) / kPhiThreads;
   __shared__ u32x4 stage[2][CHUNKS];
+  constexpr int kLocal = COUNT ? 2048 : 1;
+  __shared__ uint32_t lbuf[kLocal];
+  __shared__ uint32_t lcount, lbelow, lbase;
   const int t = threadIdx.x, lane = t & (kWave - 1), w = t / kWave, g = lane >> 5, col = lane & 31;
+  uint32_t lo = 0u, hi = 0u, below = 0u;
+  if constexpr (COUNT) {
+    lo = br->lo;
+    hi = br->hi;
+    if (t == 0) {
+      lcount = 0u;
+      lbelow = 0u;
+    }
+  }
   const int nrb = (n + 31) / 32;
   const int tpg = (nrb + kDistJB - 1) / kDistJB;  // tasks per group, full matrix
   int gi, task;
@@ -619,7 +665,24 @@ __global__ __launch_bounds__(kPhiThreads) void sqdist_kernel(const char* __restr
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
       const int j = jbase + acc_row(r, lane);
-      if (i < n && j < n) d2[(int64_t)j * n + i] = (i == j) ? 0.f : fmaxf(ni + nj[r] - 2.f * z[r], 0.f);
+      const float D = (i == j) ? 0.f : pair_d2(ni, nj[r], z[r]);
+      if (i < n && j < n) d2[(int64_t)j * n + i] = D;
+      if constexpr (COUNT) {
+        const uint32_t u = __float_as_uint(D);
+        const bool valid = i > j && i < n;
+        below += (valid && u < lo) ? 1u : 0u;
+        const bool in = valid && u >= lo && u < hi;
+        const uint64_t m = __ballot(in);
+        if (m) {  // one LDS atomic per wave for the pairs of this row of the tile
+          uint32_t base = 0;
+          if (lane == 0) base = atomicAdd(&lcount, (uint32_t)__popcll(m));
+          base = (uint32_t)__shfl((int)base, 0, kWave);
+          if (in) {
+            const uint32_t idx = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+            if (idx < (uint32_t)kLocal) lbuf[idx] = u;
+          }
+        }
+      }
     }
     if (st + 1 < nsteps) {
       u32x4* dst = stage[(st + 1) & 1];
@@ -628,6 +691,21 @@ __global__ __launch_bounds__(kPhiThreads) void sqdist_kernel(const char* __restr
         if (t + p * kPhiThreads < CHUNKS) dst[t + p * kPhiThreads] = regs[p];
     }
     __syncthreads();
+  }
+  if constexpr (COUNT) {
+    below = wave_sum(below);
+    if (lane == 0 && below) atomicAdd(&lbelow, below);
+    __syncthreads();
+    const uint32_t cnt = lcount;
+    if (t == 0) {
+      if (lbelow) atomicAdd(&br->below, lbelow);
+      lbase = cnt ? atomicAdd(&br->inside, cnt) : 0u;
+      if (cnt > (uint32_t)kLocal) atomicOr(&br->overflow, 1u);
+    }
+    __syncthreads();
+    const uint32_t gbase = lbase, keep = cnt < (uint32_t)kLocal ? cnt : (uint32_t)kLocal;
+    for (uint32_t k = t; k < keep; k += kPhiThreads)
+      if (gbase + k < kBracketCap) cand[gbase + k] = lbuf[k];
   }
 }
 
@@ -714,24 +792,6 @@ __global__ __launch_bounds__(256) void sqrt_sum_kernel(const float* __restrict__
   if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
-// ---------------------------------------------------------------- bracketed median (fused SVGD run)
-// Between SVGD iterations the median pair distance moves little, so each selection first counts the
-// pairs below a bracket [lo, hi) around the previous median and copies those inside it to a candidate
-// buffer. When the wanted ranks fall inside the bracket (and the candidates fit), the radix passes
-// run over the candidates only, with the ranks shifted by the count below; otherwise over all pairs.
-// Either way the result is the exact order statistic. The bracket half-width (relative) shrinks
-// after hits and grows after misses.
-struct Bracket {
-  uint32_t lo, hi;     // bit patterns of non-negative floats (ordered as unsigned ints)
-  uint32_t below;      // pairs < lo
-  uint32_t inside;     // pairs in [lo, hi) (the candidate count, possibly above the capacity)
-  uint32_t ok;         // the last selection ran on the candidates
-  float delta;         // relative half-width of the next bracket
-  uint32_t overflow;   // some workgroup had more candidates than its LDS buffer
-};
-
-constexpr uint32_t kBracketCap = 1u << 20;  // candidate capacity (4 MB)
-
 // Selection state of the fused run: st[k] (k = 0..4) = {prefix0, prefix1, rank0, rank1} before radix
 // pass k (st[4]: the selected bit patterns), then one 2 x 256-bin histogram per pass.
 constexpr int kSelStateWords = 5 * 4;
@@ -770,63 +830,49 @@ MOCAT_DEV void radix_choose_block(int shift, const uint32_t* __restrict__ st_in,
   __syncthreads();
 }
 
-__global__ __launch_bounds__(kSelThreads) void bracket_count_kernel(const uint32_t* __restrict__ v, int n, Bracket* br,
-                                                            uint32_t* __restrict__ cand) {
-  // candidates gather in LDS first (one LDS atomic per wave and tile) and reach the global buffer with
-  // one global atomic per workgroup; a workgroup whose candidates overflow its LDS marks the bracket
-  // unusable (the selection then runs over all pairs)
-  constexpr int kLocal = 4096;
-  __shared__ uint32_t lbuf[kLocal];
-  __shared__ uint32_t lcount, lbelow, lbase;
-  const uint32_t lo = br->lo, hi = br->hi;
+// Both order statistics of the m <= kSmallCap bracket candidates by one workgroup: the candidates
+// staged in LDS, then the four 8-bit digit passes with LDS histograms (wave multisplit counts) and the
+// same digit choice as the multi-workgroup passes. Digits inside the common prefix of lo and hi - 1
+// are shared by every candidate (they lie in [lo, hi)) and are taken without a scan. st (LDS, 2 x 4
+// words) starts as {0, 0, rank0, rank1} in st[0..3]; the result (the two bit patterns, then the
+// remaining ranks) ends in st[0..3]. Every thread of the workgroup calls this.
+MOCAT_DEV void select_small_block(const uint32_t* __restrict__ cand, uint32_t m, uint32_t lo, uint32_t hi,
+                                  uint32_t* sc, uint32_t* lh, uint32_t* st, uint32_t (*wtot)[4]) {
   const int lane = threadIdx.x & (kWave - 1);
-  if (threadIdx.x == 0) {
-    lcount = 0u;
-    lbelow = 0u;
-  }
-  __syncthreads();
-  uint32_t below = 0;
-  for (int row = blockIdx.x; row < n - 1; row += gridDim.x) {
-    const uint32_t* vr = v + (int64_t)row * n;
-    for (int c0 = row + 1; c0 < n; c0 += kRowUnroll * kSelThreads) {
-      uint32_t uu[kRowUnroll];  // loads of the whole stretch issued before any use
-#pragma unroll
-      for (int k = 0; k < kRowUnroll; ++k) {
-        const int c = c0 + k * kSelThreads + (int)threadIdx.x;
-        uu[k] = c < n ? vr[c] : 0xFFFFFFFFu;
-      }
-#pragma unroll
-      for (int k = 0; k < kRowUnroll; ++k) {
-        const int c = c0 + k * kSelThreads + (int)threadIdx.x;
-        const uint32_t u = uu[k];
-        const bool in = c < n && u >= lo && u < hi;
-        below += (c < n && u < lo) ? 1u : 0u;
-        const uint64_t m = __ballot(in);
-        if (m) {
-          uint32_t base = 0;
-          if (lane == 0) base = atomicAdd(&lcount, (uint32_t)__popcll(m));
-          base = (uint32_t)__shfl((int)base, 0, kWave);
-          if (in) {
-            const uint32_t idx = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-            if (idx < (uint32_t)kLocal) lbuf[idx] = u;
-          }
-        }
-      }
+  for (uint32_t k = threadIdx.x; k < m; k += blockDim.x) sc[k] = cand[k];
+  const uint32_t diff = lo ^ (hi - 1u);
+  const int common = diff ? __clz(diff) : 32;  // leading bits shared by every candidate
+  for (int pass = 0; pass < 4; ++pass) {
+    const int shift = 24 - 8 * pass;
+    const uint32_t mask = shift == 24 ? 0u : (0xFFFFFFFFu << (shift + 8));
+    uint32_t* in = st + (pass & 1) * 4;
+    uint32_t* out = st + ((pass + 1) & 1) * 4;
+    if (common >= 32 - shift) {
+      // the digit is lo's for every candidate: prefixes extended, ranks unchanged
+      if (threadIdx.x < 2) out[threadIdx.x] = in[threadIdx.x] | (lo & (255u << shift));
+      if (threadIdx.x >= 2 && threadIdx.x < 4) out[threadIdx.x] = in[threadIdx.x];
+      __syncthreads();
+      continue;
     }
+    for (int b = threadIdx.x; b < 512; b += blockDim.x) lh[b] = 0u;
+    __syncthreads();
+    const uint32_t p0 = in[0] & mask, p1 = in[1] & mask;
+    const bool shared_prefix = p0 == p1;
+    for (uint32_t k0 = 0; k0 < m; k0 += blockDim.x) {  // workgroup-uniform trip count
+      const uint32_t k = k0 + threadIdx.x;
+      const bool valid = k < m;
+      const uint32_t u = valid ? sc[k] : 0u, dig = (u >> shift) & 255u;
+      wave_multisplit_count(valid && (u & mask) == p0, dig, lh, lane);
+      if (!shared_prefix) wave_multisplit_count(valid && (u & mask) == p1, dig, lh + 256, lane);
+    }
+    __syncthreads();
+    if (shared_prefix) {
+      for (int b = threadIdx.x; b < 256; b += blockDim.x) lh[256 + b] = lh[b];
+      __syncthreads();
+    }
+    radix_choose_block(shift, in, lh, out, wtot);
   }
-  below = wave_sum(below);
-  if (lane == 0) atomicAdd(&lbelow, below);
-  __syncthreads();
-  const uint32_t cnt = lcount;
-  if (threadIdx.x == 0) {
-    atomicAdd(&br->below, lbelow);
-    lbase = cnt ? atomicAdd(&br->inside, cnt) : 0u;
-    if (cnt > (uint32_t)kLocal) atomicOr(&br->overflow, 1u);
-  }
-  __syncthreads();
-  const uint32_t gbase = lbase, keep = cnt < (uint32_t)kLocal ? cnt : (uint32_t)kLocal;
-  for (uint32_t k = threadIdx.x; k < keep; k += kSelThreads)
-    if (gbase + k < kBracketCap) cand[gbase + k] = lbuf[k];
+  // four passes: the result is back in st[0..3]
 }
 
 // Radix pass `pass` (digit shift 24 - 8 pass) over the candidates or over all pairs (upper triangle).
@@ -844,10 +890,12 @@ __global__ __launch_bounds__(kSelThreads) void radix_hist_bracket_kernel(const u
   const int shift = 24 - 8 * pass;
   uint32_t* hist = sel + kSelStateWords + pass * 512;
   bool use_cand;
+  if (pass > 0 && br->done) return;  // few candidates: the finishing kernel selects them
   if (pass == 0) {
     const uint32_t below = br->below, inside = br->inside;
     use_cand = !br->overflow && below <= rank0 && (uint64_t)rank1 < (uint64_t)below + inside &&
                inside <= kBracketCap;
+    const bool small = use_cand && inside <= kSmallCap;
     if (threadIdx.x == 0) {
       cur[0] = 0u;
       cur[1] = 0u;
@@ -856,7 +904,11 @@ __global__ __launch_bounds__(kSelThreads) void radix_hist_bracket_kernel(const u
     }
     __syncthreads();
     if (blockIdx.x == 0 && threadIdx.x < 4) sel[threadIdx.x] = cur[threadIdx.x];
-    if (blockIdx.x == 0 && threadIdx.x == 0) br->ok = use_cand ? 1u : 0u;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      br->ok = use_cand ? 1u : 0u;
+      br->done = small ? 1u : 0u;
+    }
+    if (small) return;
   } else {
     use_cand = br->ok != 0u;
     radix_choose_block(shift + 8, sel + 4 * (pass - 1), hist - 512, cur, wtot);
@@ -911,27 +963,50 @@ __global__ __launch_bounds__(kSelThreads) void radix_hist_bracket_kernel(const u
 // bandwidth from them (FP64 square roots, their mean, divided by sqrt(2 log n) computed on the host,
 // rounded to f32), the next bracket around the upper one, and the counters and histograms cleared for
 // the next selection.
-__global__ __launch_bounds__(256) void bracket_finish_kernel(Bracket* br, uint32_t* __restrict__ sel, double norm,
-                                                             float* h, float* trace) {
+__global__ __launch_bounds__(kSelThreads) void bracket_finish_kernel(Bracket* br, const uint32_t* __restrict__ cand,
+                                                                     uint32_t* __restrict__ sel, double norm, float* h,
+                                                                     float* trace) {
   __shared__ uint32_t wtot[2][4];
-  __shared__ uint32_t cur[4];
-  radix_choose_block(0, sel + 12, sel + kSelStateWords + 3 * 512, cur, wtot);
+  __shared__ uint32_t cur[8];
+  __shared__ uint32_t lh[512];
+  __shared__ uint32_t sc[kSmallCap];
+  if (br->done) {
+    // few candidates: all four digit passes here, in LDS, from st[0]
+    if (threadIdx.x < 4) cur[threadIdx.x] = sel[threadIdx.x];
+    __syncthreads();
+    select_small_block(cand, br->inside, br->lo, br->hi, sc, lh, cur, wtot);
+  } else {
+    radix_choose_block(0, sel + 12, sel + kSelStateWords + 3 * 512, cur, wtot);
+  }
   for (int b = threadIdx.x; b < 4 * 512; b += blockDim.x) sel[kSelStateWords + b] = 0u;
   if (threadIdx.x != 0) return;
   for (int t = 0; t < 4; ++t) sel[16 + t] = cur[t];
+  br->done = 0u;
   const double v0 = (double)__uint_as_float(cur[0]), v1 = (double)__uint_as_float(cur[1]);
   const float hv = (float)(0.5 * (sqrt(v0) + sqrt(v1)) / norm);
   *h = hv;
   if (trace) *trace = hv;
+  // half-width: narrower when the candidates overflowed or are many, wider after a miss or when very
+  // few (aiming at about kTargetCand); the bracket is centred on the linear extrapolation of the last
+  // two values (the median drifts smoothly from one SVGD iteration to the next: measured relative
+  // extrapolation errors of a few 1e-6, against drifts of about 1e-4 per iteration)
   float delta = br->delta;
-  if (!br->ok)
+  const uint32_t inside = br->inside;
+  if (br->overflow || inside > kBracketCap)
+    delta = fmaxf(delta * 0.5f, 1e-7f);
+  else if (!br->ok)
     delta = fminf(delta * 4.f, 1.f);
-  else if (br->inside < kBracketCap / 8)
-    delta = fmaxf(delta * 0.75f, 1e-5f);
-  const float center = __uint_as_float(cur[1]);
-  const float lo = center * (1.f - delta), hi = center * (1.f + delta);
+  else if (inside > 2 * kTargetCand)
+    delta = fmaxf(delta * 0.7f, 1e-7f);
+  else if (inside < kTargetCand / 4)
+    delta = fminf(delta * 1.25f, 1.f);
+  const float center = __uint_as_float(cur[1]), prev = br->prev;
+  float pred = prev > 0.f ? center + (center - prev) : center;
+  if (!(pred > 0.f)) pred = center;
+  br->prev = center;
+  const float lo = pred * (1.f - delta), hi = pred * (1.f + delta);
   br->lo = __float_as_uint(fmaxf(lo, 0.f));
-  br->hi = hi > center ? __float_as_uint(hi) : 0x7F800001u;  // above every finite value
+  br->hi = hi > pred ? __float_as_uint(hi) : 0x7F800001u;  // above every finite value
   br->delta = delta;
   br->below = 0u;
   br->inside = 0u;
@@ -948,6 +1023,8 @@ __global__ __launch_bounds__(256) void bracket_init_kernel(Bracket* br, uint32_t
     br->inside = 0u;
     br->ok = 0u;
     br->overflow = 0u;
+    br->done = 0u;
+    br->prev = 0.f;
     br->delta = 2e-3f;
   }
 }
@@ -1108,20 +1185,23 @@ int svgd_step_dp(const float* x_in, const float* grad, int n, int d, float bandw
 
 // squared distances from prepared operands (upper != 0: the tiles on and above the diagonal)
 template <int DP>
-void sqdist_packed_dp(const float* x, int n, int d, const void* packed, const float* mean, int upper, float* d2,
-                      hipStream_t st) {
+void sqdist_packed_dp(const void* packed, int n, int upper, float* d2, Bracket* br, uint32_t* cand, hipStream_t st) {
   const int groups = (n + kPhiI - 1) / kPhiI, nrb = (n + 31) / 32;
   const int64_t tasks = upper ? (int64_t)groups * (groups + 1) / 2
                               : (int64_t)groups * ((nrb + kDistJB - 1) / kDistJB);
-  hipLaunchKernelGGL((sqdist_kernel<DP>), dim3((unsigned)tasks), dim3(kPhiThreads), 0, st, (const char*)packed, n,
-                     upper, d2);
+  if (br)
+    hipLaunchKernelGGL((sqdist_kernel<DP, true>), dim3((unsigned)tasks), dim3(kPhiThreads), 0, st,
+                       (const char*)packed, n, upper, d2, br, cand);
+  else
+    hipLaunchKernelGGL((sqdist_kernel<DP, false>), dim3((unsigned)tasks), dim3(kPhiThreads), 0, st,
+                       (const char*)packed, n, upper, d2, (Bracket*)nullptr, (uint32_t*)nullptr);
 }
 
 template <int DP>
 int sqdist_dp(const float* x, int n, int d, void* packed, float* mean, int upper, float* d2, hipStream_t st) {
   // the packed B layout needs a gradient: pass x again (only the A layout and norms are read here)
   svgd_prepare_dp<DP>(x, x, n, d, packed, mean, st);
-  sqdist_packed_dp<DP>(x, n, d, packed, mean, upper, d2, st);
+  sqdist_packed_dp<DP>(packed, n, upper, d2, nullptr, nullptr, st);
   MOCAT_LAUNCH_CHECK();
   return MOCAT_OK;
 }
@@ -1318,19 +1398,20 @@ int mocat_svgd_gmm_run(float* x, int64_t n, int64_t d, int K, const float* means
       if (tr) hipLaunchKernelGGL(copy_bandwidth_kernel, dim3(1), dim3(64), 0, stream, bandwidth, tr);
       continue;
     }
+    // distances (and, for the median, the bracket count of the pairs as they are written)
+    Bracket* count_br = bandwidth_mode == 2 ? br : nullptr;
     if (DP == 64)
-      sqdist_packed_dp<64>(x, (int)n, (int)d, ws, mean, 1, d2, stream);
+      sqdist_packed_dp<64>(ws, (int)n, 1, d2, count_br, cand, stream);
     else
-      sqdist_packed_dp<128>(x, (int)n, (int)d, ws, mean, 1, d2, stream);
+      sqdist_packed_dp<128>(ws, (int)n, 1, d2, count_br, cand, stream);
     if (bandwidth_mode == 2) {
       const unsigned rows = (unsigned)std::min<int64_t>(n - 1, kSelectBlocks);
-      hipLaunchKernelGGL(bracket_count_kernel, dim3(rows), dim3(kSelThreads), 0, stream, (const uint32_t*)d2, (int)n, br,
-                         cand);
       for (int pass = 0; pass < 4; ++pass)
         hipLaunchKernelGGL(radix_hist_bracket_kernel, dim3(rows), dim3(kSelThreads), 0, stream, (const uint32_t*)d2,
                            (int)n, (const uint32_t*)cand, br, pass, sel, (uint32_t)((r0 - n) / 2),
                            (uint32_t)((r1 - n) / 2));
-      hipLaunchKernelGGL(bracket_finish_kernel, dim3(1), dim3(256), 0, stream, br, sel, norm, bandwidth, tr);
+      hipLaunchKernelGGL(bracket_finish_kernel, dim3(1), dim3(kSelThreads), 0, stream, br, (const uint32_t*)cand, sel,
+                         norm, bandwidth, tr);
     } else {
       const int rc = mocat_sqrt_sum_upper(d2, n, part, 1024, stream);
       if (rc != MOCAT_OK) return rc;

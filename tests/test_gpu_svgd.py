@@ -197,6 +197,20 @@ def test_fused_gmm_run_matches_stepwise(m, upd):
     np.testing.assert_array_equal(fused.final_extra.random_key.cpu().numpy(), ref.final_extra.random_key.cpu().numpy())
 
 
+def test_fused_median_all_selection_routes(m):
+    """n = 3000: the first selection runs over all pairs, the second on a wide bracket (many
+    candidates, multi-workgroup passes), later ones on few candidates (one workgroup in LDS); every
+    bandwidth equals the stepwise full selection bit for bit."""
+    rs = np.random.default_rng(11)
+    d, n, iters = 20, 3000, 8
+    sc = m.GaussianMixture(rs.normal(0, 1, (3, d)), rs.uniform(0.5, 2, (3, d)), rs.dirichlet(np.ones(3)))
+    kw = dict(stepsize=0.01, max_iter=iters, bandwidth_update="median")
+    fused = m.run(sc, m.SVGD(**kw), n, m.random.PRNGKey(3))
+    ref = m.run(sc, m.SVGD(**kw), n, m.random.PRNGKey(3), stack=("value",))
+    np.testing.assert_array_equal(fused.bandwidth.cpu().numpy(), ref.bandwidth.cpu().numpy())
+    np.testing.assert_array_equal(fused.final_state.value.cpu().numpy(), ref.final_state.value.cpu().numpy())
+
+
 def test_fused_median_bracket_exact_over_many_iterations(m):
     """Small steps keep the median inside the bracket of the previous one, so most selections run on
     the candidate set: the bandwidth trace still equals the stepwise full selection bit for bit."""
