@@ -428,14 +428,23 @@ __global__ __launch_bounds__(kPhiThreads) void svgd_phi_kernel(const char* __res
       kh[r >> 3][r & 7] = h;
       kl[r >> 3][r & 7] = l;
     }
+    // consecutive products go to different accumulators (a chain of dependent accumulations stalls on
+    // each result); each accumulator still receives its six products in the same order
 #pragma unroll
-    for (int nb = 0; nb < NB; ++nb) {
+    for (int kb2 = 0; kb2 < 2; ++kb2) {
 #pragma unroll
-      for (int kb2 = 0; kb2 < 2; ++kb2) {
+      for (int nb = 0; nb < NB; ++nb) {
         const bf16x8 xh = B[((0 * 2 + kb2) * NB + nb) * 64 + lane];
-        const bf16x8 xl = B[((1 * 2 + kb2) * NB + nb) * 64 + lane];
         gacc[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(kh[kb2], xh, gacc[nb], 0, 0, 0);
+      }
+#pragma unroll
+      for (int nb = 0; nb < NB; ++nb) {
+        const bf16x8 xl = B[((1 * 2 + kb2) * NB + nb) * 64 + lane];
         gacc[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(kh[kb2], xl, gacc[nb], 0, 0, 0);
+      }
+#pragma unroll
+      for (int nb = 0; nb < NB; ++nb) {
+        const bf16x8 xh = B[((0 * 2 + kb2) * NB + nb) * 64 + lane];
         gacc[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(kl[kb2], xh, gacc[nb], 0, 0, 0);
       }
     }
