@@ -374,23 +374,26 @@ __global__ __launch_bounds__(kPhiThreads) void svgd_phi_kernel(const char* __res
     for (int r = 0; r < 16; ++r) gacc[nb][r] = 0.f;
   float spart = 0.f;
 
-  // staging registers for the next block (static indices only, so they stay in VGPRs)
-  u32x4 regs[PER];
-  const int nsteps = rb1 - rb0;
-  if (nsteps > 0) {
-    const u32x4* src = reinterpret_cast<const u32x4*>(packed + (int64_t)rb0 * L::BLOCK_BYTES);
+  // j blocks staged global -> LDS directly (16 bytes per lane, no registers): lane l of wave w copies
+  // chunk p * kPhiThreads + w * 64 + l to the same LDS chunk; the copy into the other buffer overlaps
+  // the current block's products and is waited for before the step's closing barrier
+  auto stage_block = [&](int buf, int rb) {
+    const u32x4* src = reinterpret_cast<const u32x4*>(packed + (int64_t)rb * L::BLOCK_BYTES);
 #pragma unroll
-    for (int p = 0; p < PER; ++p)
-      if (t + p * kPhiThreads < CHUNKS) stage[0][t + p * kPhiThreads] = src[t + p * kPhiThreads];
-  }
+    for (int p = 0; p < PER; ++p) {
+      const int c = p * kPhiThreads + t;
+      if (c < CHUNKS)
+        __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)(src + c),
+                                         (__attribute__((address_space(3))) void*)(&stage[buf][c - lane]), 16, 0,
+                                         0);
+    }
+  };
+  const int nsteps = rb1 - rb0;
+  if (nsteps > 0) stage_block(0, rb0);
+  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
   __syncthreads();
   for (int st = 0; st < nsteps; ++st) {
-    if (st + 1 < nsteps) {
-      const u32x4* src = reinterpret_cast<const u32x4*>(packed + (int64_t)(rb0 + st + 1) * L::BLOCK_BYTES);
-#pragma unroll
-      for (int p = 0; p < PER; ++p)
-        if (t + p * kPhiThreads < CHUNKS) regs[p] = src[t + p * kPhiThreads];
-    }
+    if (st + 1 < nsteps) stage_block((st + 1) & 1, rb0 + st + 1);
     const char* blk = reinterpret_cast<const char*>(stage[st & 1]);
     const bf16x8* A = reinterpret_cast<const bf16x8*>(blk);
     const bf16x8* B = A + L::A_ELEMS / 8;
@@ -448,12 +451,7 @@ __global__ __launch_bounds__(kPhiThreads) void svgd_phi_kernel(const char* __res
         gacc[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(kl[kb2], xh, gacc[nb], 0, 0, 0);
       }
     }
-    if (st + 1 < nsteps) {
-      u32x4* dst = stage[(st + 1) & 1];
-#pragma unroll
-      for (int p = 0; p < PER; ++p)
-        if (t + p * kPhiThreads < CHUNKS) dst[t + p * kPhiThreads] = regs[p];
-    }
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
   }
   // accumulator rows = particles i (acc_row), columns = features; row sums of K from both lane halves
