@@ -340,11 +340,10 @@ MOCAT_DEV int64_t search_csum(const uint64_t* __restrict__ C, int64_t n, double 
   return lo < n ? lo : n - 1;
 }
 
-// Ancestor indices for weights w_i = weight(i) (all threads participate).
-// method 0 systematic, 1 multinomial (uniforms from key). csum: n uint64 of workspace.
+// Inclusive prefix sums of the quantised weights w_i = weight(i) into csum (n uint64), by one
+// workgroup (all threads participate); returns the total.
 template <class WeightFn>
-MOCAT_DEV void resample_block(WeightFn weight, int64_t n, Key key, int method, uint64_t* __restrict__ csum,
-                              int32_t* __restrict__ anc, BlockScratch& s) {
+MOCAT_DEV uint64_t quantised_csum_block(WeightFn weight, int64_t n, uint64_t* __restrict__ csum, BlockScratch& s) {
   const int t = threadIdx.x;
   // max weight
   float wmax = 0.f;
@@ -362,20 +361,31 @@ MOCAT_DEV void resample_block(WeightFn weight, int64_t n, Key key, int method, u
     run += quantise(weight(i), shift);
     csum[i] = run;
   }
+  return total;
+}
+
+// The one uniform of systematic resampling.
+MOCAT_DEV double systematic_uniform(Key key) { return (double)uniform_from_bits(bits_at(key, 1, 0), 0.f, 1.f); }
+
+// Ancestor of output j from the exact prefix sums (total W): systematic (method 0, u_sys =
+// systematic_uniform(key)) or multinomial (method 1, uniform j of n from key).
+MOCAT_DEV int32_t ancestor_at(const uint64_t* __restrict__ csum, int64_t n, double W, Key key, int method,
+                              double u_sys, int64_t j) {
+  if (method == 0) return (int32_t)search_csum(csum, n, ((double)j + u_sys) * (W / (double)n));
+  const double u = (double)uniform_from_bits(bits_at(key, (uint32_t)n, (uint32_t)j), 0.f, 1.f);
+  return (int32_t)search_csum(csum, n, u * W);
+}
+
+// Ancestor indices for weights w_i = weight(i) (all threads participate).
+// method 0 systematic, 1 multinomial (uniforms from key). csum: n uint64 of workspace.
+template <class WeightFn>
+MOCAT_DEV void resample_block(WeightFn weight, int64_t n, Key key, int method, uint64_t* __restrict__ csum,
+                              int32_t* __restrict__ anc, BlockScratch& s) {
+  const uint64_t total = quantised_csum_block(weight, n, csum, s);
   __threadfence_block();
   __syncthreads();
-  const double W = (double)total;
-  if (method == 0) {
-    const double u = (double)uniform_from_bits(bits_at(key, 1, 0), 0.f, 1.f);
-    const double step = W / (double)n;
-    for (int64_t j = t; j < n; j += NT) anc[j] = (int32_t)search_csum(csum, n, ((double)j + u) * step);
-  } else {
-    const uint32_t S = (uint32_t)n;
-    for (int64_t j = t; j < n; j += NT) {
-      const double u = (double)uniform_from_bits(bits_at(key, S, (uint32_t)j), 0.f, 1.f);
-      anc[j] = (int32_t)search_csum(csum, n, u * W);
-    }
-  }
+  const double W = (double)total, u_sys = method == 0 ? systematic_uniform(key) : 0.0;
+  for (int64_t j = threadIdx.x; j < n; j += NT) anc[j] = ancestor_at(csum, n, W, key, method, u_sys, j);
   __syncthreads();
 }
 
@@ -612,11 +622,12 @@ __global__ __launch_bounds__(NT) void pf_kernel(PfArgs a) {
 }
 
 // ---------------------------------------------------------------- standalone resampling
+// Two launches: the exact prefix sums by one workgroup, then one thread per output for the searches
+// (the same sums and searches as resample_block).
 // from_log: w holds log-weights, the weights used are exp(w - max w) (f32).
-__global__ __launch_bounds__(NT) void resample_kernel(const float* __restrict__ w, int64_t n, const uint32_t* key,
-                                                      int method, int from_log, uint64_t* csum, int32_t* idx) {
+__global__ __launch_bounds__(NT) void resample_csum_kernel(const float* __restrict__ w, int64_t n, int from_log,
+                                                           uint64_t* csum) {
   __shared__ BlockScratch s;
-  const Key k{key[0], key[1]};
   float m = 0.f;
   if (from_log) {
     float mm = -INFINITY;
@@ -628,7 +639,16 @@ __global__ __launch_bounds__(NT) void resample_kernel(const float* __restrict__ 
     const float v = from_log ? __expf(w[i] - m) : w[i];
     return v > 0.f ? v : 0.f;
   };
-  resample_block(wfn, n, k, method, csum, idx, s);
+  quantised_csum_block(wfn, n, csum, s);
+}
+
+__global__ __launch_bounds__(256) void resample_search_kernel(const uint64_t* __restrict__ csum, int64_t n,
+                                                              const uint32_t* key, int method, int32_t* idx) {
+  const Key k{key[0], key[1]};
+  const double W = (double)csum[n - 1];  // the inclusive sum at the last index is the total
+  const double u_sys = method == 0 ? systematic_uniform(k) : 0.0;
+  for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x)
+    idx[j] = ancestor_at(csum, n, W, k, method, u_sys, j);
 }
 
 __global__ __launch_bounds__(256) void categorical_kernel(const float* __restrict__ lw, int64_t n, int64_t n_out,
@@ -1327,8 +1347,10 @@ int mocat_resample(const float* weights, int64_t n, int method, int from_log, co
                    int32_t* idx, hipStream_t stream) {
   if (!weights || !key || !workspace || !idx || n <= 0 || n > 0x7FFFFFFFll || method < 0 || method > 1)
     return MOCAT_ERR_ARG;
-  hipLaunchKernelGGL(resample_kernel, dim3(1), dim3(NT), 0, stream, weights, n, key, method, from_log,
-                     (uint64_t*)workspace, idx);
+  hipLaunchKernelGGL(resample_csum_kernel, dim3(1), dim3(NT), 0, stream, weights, n, from_log, (uint64_t*)workspace);
+  const int64_t blocks = (n + 255) / 256;
+  hipLaunchKernelGGL(resample_search_kernel, dim3((unsigned)(blocks < 4096 ? blocks : 4096)), dim3(256), 0, stream,
+                     (const uint64_t*)workspace, n, key, method, idx);
   MOCAT_LAUNCH_CHECK();
   return MOCAT_OK;
 }
