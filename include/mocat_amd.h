@@ -39,6 +39,10 @@ int mocat_random_uniform(const uint32_t* keys, int64_t n_keys, int64_t size, flo
                          float* out, hipStream_t stream);
 /* normal(keys[i], (size,)) -> out[i][size] (f32). */
 int mocat_random_normal(const uint32_t* keys, int64_t n_keys, int64_t size, float* out, hipStream_t stream);
+/* randint(keys[i], (size,), minval, maxval) -> out[i][size] (int32), the draw behind
+ * random.choice(key, n, shape) with replacement in mocat/src/metrics.py:103 (KSD minibatches). */
+int mocat_random_randint(const uint32_t* keys, int64_t n_keys, int64_t size, int32_t minval, int32_t maxval,
+                         int32_t* out, hipStream_t stream);
 
 /* --------------------------------------------------------------------------------- reductions
  * logsumexp over rows of w[batch][n] (f32 in, FP64 accumulation and results): lse[b] =
@@ -219,6 +223,14 @@ int mocat_svgd_gmm_run(float* x, int64_t n, int64_t d, int K, const float* means
 int mocat_ksd_rows(const float* x, const float* grad_potential, const float* weights, int64_t n, int64_t d,
                    int kernel, float bandwidth, float c, float beta, int js_count, double* part,
                    hipStream_t stream);
+/* ---- KSD over random minibatches (replaces mocat/src/metrics.py:101-104,126-128, ksd with
+ * ensemble_batchsize = batch): inds[i][b] (int32, in [0, n)) are particle i's batch;
+ * row_k[i] = sum_b weights[inds[i][b]] * k_0(x_i, x_inds[i][b]) and row_w[i] = sum_b
+ * weights[inds[i][b]] (FP64, in b order; weights may be NULL). The caller forms
+ * sqrt(sum_i w_i row_k[i] / (sum(w) row_w[i])). */
+int mocat_ksd_batch_rows(const float* x, const float* grad_potential, const float* weights, int64_t n, int64_t d,
+                         int64_t batch, const int32_t* inds, int kernel, float bandwidth, float c, float beta,
+                         double* row_k, double* row_w, hipStream_t stream);
 
 #ifdef __cplusplus
 }

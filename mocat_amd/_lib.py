@@ -31,6 +31,7 @@ SIGNATURES = {
     "mocat_random_bits": [c_ptr, c_i64, c_i64, c_ptr, c_ptr],
     "mocat_random_uniform": [c_ptr, c_i64, c_i64, c_f32, c_f32, c_ptr, c_ptr],
     "mocat_random_normal": [c_ptr, c_i64, c_i64, c_ptr, c_ptr],
+    "mocat_random_randint": [c_ptr, c_i64, c_i64, c_i32, c_i32, c_ptr, c_ptr],
 }
 
 STATUS = {0: "ok", -1: "invalid argument", -2: "HIP launch/runtime error", -3: "size limit exceeded"}

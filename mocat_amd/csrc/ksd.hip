@@ -1,5 +1,5 @@
-// Kernel Stein discrepancy over a full ensemble (mocat/src/metrics.py:88-130) for mocat's Gaussian
-// and IMQ kernels (mocat/src/kernels.py:83-111, 164-215).
+// Kernel Stein discrepancy (mocat/src/metrics.py:88-130) for mocat's Gaussian and IMQ kernels
+// (mocat/src/kernels.py:83-111, 164-215), over the full ensemble or over random minibatches.
 //
 // k_0(x_i, x_j) depends on the pair only through four sums over the coordinates,
 //   r2 = |x_i - x_j|^2, a = (x_i - x_j).g_j, b = (x_i - x_j).g_i, c = g_i.g_j,
@@ -110,6 +110,44 @@ __global__ __launch_bounds__(kKsdThreads) void ksd_rows_kernel(const float* __re
   }
 }
 
+// Minibatch rows: one wave per particle i; lane b takes batch entries b, b + 64, ... and sums the
+// four coordinate sums of its pair in coordinate order (FP32, as the tile kernel), then k_0 in FP64;
+// the lanes' FP64 partial sums are combined by a fixed butterfly.
+__global__ __launch_bounds__(256) void ksd_batch_kernel(const float* __restrict__ x, const float* __restrict__ g,
+                                                        const float* __restrict__ w, int n, int64_t batch,
+                                                        const int32_t* __restrict__ inds, KsdParams p,
+                                                        double* __restrict__ row_k, double* __restrict__ row_w) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int64_t i = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+  if (i >= n) return;
+  const int d = p.d;
+  const float* xi = x + i * d;
+  const float* gi = g + i * d;
+  double sk = 0.0, sw = 0.0;
+  for (int64_t b = lane; b < batch; b += kWave) {
+    const int64_t j = inds[i * batch + b];
+    const float* xj = x + j * d;
+    const float* gj = g + j * d;
+    float r2 = 0.f, a = 0.f, bb = 0.f, c = 0.f;
+    for (int k = 0; k < d; ++k) {
+      const float df = xi[k] - xj[k];
+      r2 = fmaf(df, df, r2);
+      a = fmaf(df, gj[k], a);
+      bb = fmaf(df, gi[k], bb);
+      c = fmaf(gi[k], gj[k], c);
+    }
+    const double wj = w ? (double)w[j] : 1.0;
+    sk += wj * k0_pair(p, r2, a, bb, c);
+    sw += wj;
+  }
+  sk = wave_sum(sk);
+  sw = wave_sum(sw);
+  if (lane == 0) {
+    row_k[i] = sk;
+    row_w[i] = sw;
+  }
+}
+
 }  // namespace
 
 extern "C" {
@@ -123,6 +161,21 @@ int mocat_ksd_rows(const float* x, const float* grad_potential, const float* wei
   const KsdParams p{kernel, bandwidth, c, beta, (int)d};
   hipLaunchKernelGGL(ksd_rows_kernel, dim3((unsigned)((n + kTile - 1) / kTile), (unsigned)js_count),
                      dim3(kKsdThreads), 0, stream, x, grad_potential, weights, (int)n, p, js_count, part);
+  MOCAT_LAUNCH_CHECK();
+  return MOCAT_OK;
+}
+
+int mocat_ksd_batch_rows(const float* x, const float* grad_potential, const float* weights, int64_t n, int64_t d,
+                         int64_t batch, const int32_t* inds, int kernel, float bandwidth, float c, float beta,
+                         double* row_k, double* row_w, hipStream_t stream) {
+  if (!x || !grad_potential || !inds || !row_k || !row_w || n <= 0 || n > (1 << 24) || d <= 0 || d > (1 << 16) ||
+      batch <= 0 || batch > (1ll << 31) / n || (kernel != 0 && kernel != 1) || !(bandwidth > 0.f) ||
+      (kernel == 1 && !(c > 0.f)))
+    return MOCAT_ERR_ARG;
+  const KsdParams p{kernel, bandwidth, c, beta, (int)d};
+  const int per = 256 / kWave;
+  hipLaunchKernelGGL(ksd_batch_kernel, dim3((unsigned)((n + per - 1) / per)), dim3(256), 0, stream, x, grad_potential,
+                     weights, (int)n, batch, inds, p, row_k, row_w);
   MOCAT_LAUNCH_CHECK();
   return MOCAT_OK;
 }

@@ -35,6 +35,37 @@ __global__ __launch_bounds__(256) void rng_fill_kernel(const uint32_t* __restric
   }
 }
 
+// jax.random.randint(key, shape, minval, maxval) for int32 (the legacy algorithm): k1, k2 =
+// split(key); hi, lo = bits(k1), bits(k2) of the request's size; span = maxval - minval as uint32 (1
+// when maxval <= minval); m = (2^16 mod span)^2 mod span (the square wraps at 32 bits, as the uint32
+// product does); result = minval + ((hi mod span) * m + (lo mod span)) mod span, with the products
+// and sum wrapping at 32 bits.
+MOCAT_DEV int32_t randint_from_bits(uint32_t hb, uint32_t lb, int32_t minval, int32_t maxval) {
+  const uint32_t span = maxval <= minval ? 1u : (uint32_t)maxval - (uint32_t)minval;
+  uint32_t m = 65536u % span;
+  m = (m * m) % span;
+  const uint32_t off = ((hb % span) * m + lb % span) % span;
+  return (int32_t)((uint32_t)minval + off);
+}
+
+__global__ __launch_bounds__(256) void randint_kernel(const uint32_t* __restrict__ keys, int64_t n_keys, uint32_t S,
+                                                      int32_t minval, int32_t maxval, int32_t* __restrict__ out) {
+  const uint32_t h = (S + 1u) >> 1;
+  const int64_t total = n_keys * (int64_t)h;
+  for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < total;
+       g += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t kid = g / h;
+    const uint32_t i = (uint32_t)(g - kid * h);
+    const Key k{keys[2 * kid], keys[2 * kid + 1]};
+    uint32_t h0, h1, l0, l1;
+    bits_pair(split_at(k, 2, 0), S, i, h0, h1);
+    bits_pair(split_at(k, 2, 1), S, i, l0, l1);
+    int32_t* o = out + kid * (int64_t)S;
+    o[i] = randint_from_bits(h0, l0, minval, maxval);
+    if (i + h < S) o[i + h] = randint_from_bits(h1, l1, minval, maxval);
+  }
+}
+
 template <Out kind, typename T>
 int launch_fill(const uint32_t* keys, int64_t n_keys, int64_t size, float lo, float hi, T* out,
                 hipStream_t stream) {
@@ -76,6 +107,21 @@ int mocat_random_uniform(const uint32_t* keys, int64_t n_keys, int64_t size, flo
 
 int mocat_random_normal(const uint32_t* keys, int64_t n_keys, int64_t size, float* out, hipStream_t stream) {
   return launch_fill<Out::Normal, float>(keys, n_keys, size, 0.f, 0.f, out, stream);
+}
+
+int mocat_random_randint(const uint32_t* keys, int64_t n_keys, int64_t size, int32_t minval, int32_t maxval,
+                         int32_t* out, hipStream_t stream) {
+  if (n_keys < 0 || size < 0 || (n_keys > 0 && (!keys || !out))) return MOCAT_ERR_ARG;
+  if (size >= 0xFFFFFFFFll) return MOCAT_ERR_SIZE;
+  if (n_keys == 0 || size == 0) return MOCAT_OK;
+  const uint32_t S = (uint32_t)size;
+  const int64_t total = n_keys * (int64_t)((S + 1u) >> 1);
+  int64_t blocks = (total + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(randint_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, keys, n_keys, S, minval, maxval,
+                     out);
+  MOCAT_LAUNCH_CHECK();
+  return MOCAT_OK;
 }
 
 }  // extern "C"

@@ -21,6 +21,8 @@ from . import _lib as _l
 
 _l.register("mocat_ksd_rows", [_l.c_ptr, _l.c_ptr, _l.c_ptr, _l.c_i64, _l.c_i64, _l.c_i32, _l.c_f32, _l.c_f32,
                                _l.c_f32, _l.c_i32, _l.c_ptr, _l.c_ptr])
+_l.register("mocat_ksd_batch_rows", [_l.c_ptr, _l.c_ptr, _l.c_ptr, _l.c_i64, _l.c_i64, _l.c_i64, _l.c_ptr, _l.c_i32,
+                                     _l.c_f32, _l.c_f32, _l.c_f32, _l.c_ptr, _l.c_ptr, _l.c_ptr])
 
 
 def _next_pow_two(n: int) -> int:
@@ -84,15 +86,15 @@ def ksd(sample, kernel, grad_potential: torch.Tensor = None, log_weight: torch.T
 
     sample: (n, d) values or a cdict with .value (and .grad_potential when grad_potential is None).
     kernel: mocat_amd.kernels.Gaussian or IMQ; kernel_params override its parameters. As in mocat,
-    k_0 takes the potential gradient where a Stein kernel takes the score."""
+    k_0 takes the potential gradient where a Stein kernel takes the score. With ensemble_batchsize
+    B, particle i's sum runs over row i of random.choice(random_key, n, (n, B)) (with replacement)
+    and is normalised by the batch's weight sum, as mocat does."""
     from . import _lib
     vals = sample.value if hasattr(sample, "value") and not isinstance(sample, torch.Tensor) else sample
     if grad_potential is None and hasattr(sample, "grad_potential") and not isinstance(sample, torch.Tensor):
         grad_potential = sample.grad_potential
     elif grad_potential is None:
         raise TypeError("grad_potential not found")
-    if ensemble_batchsize is not None:
-        raise NotImplementedError("the KSD kernel sums over the full ensemble (ensemble_batchsize=None)")
     name = kernel.__class__.__name__
     if name not in _KSD_KERNELS:
         raise NotImplementedError(f"KSD is implemented for the Gaussian and IMQ kernels, not {name}")
@@ -108,6 +110,19 @@ def ksd(sample, kernel, grad_potential: torch.Tensor = None, log_weight: torch.T
         w = w64.to(torch.float32).contiguous()
     else:
         w64, w = torch.ones(n, dtype=torch.float64, device="cuda"), None
+    if ensemble_batchsize is not None:
+        # metrics.py:101-104: batch_i = random.choice(random_key, n, (n, ensemble_batchsize))[i]
+        if random_key is None:
+            raise TypeError("ensemble_batchsize needs random_key")
+        from . import random as mrandom
+        batch = int(ensemble_batchsize)
+        inds = mrandom.choice(random_key, n, (n, batch)).contiguous()
+        row_k = torch.empty(n, dtype=torch.float64, device="cuda")
+        row_w = torch.empty(n, dtype=torch.float64, device="cuda")
+        _lib.call("mocat_ksd_batch_rows", _lib.ptr(x), _lib.ptr(g), _lib.ptr(w), n, d, batch, _lib.ptr(inds),
+                  _KSD_KERNELS[name], float(params["bandwidth"]), float(params.get("c", 1.0)),
+                  float(params.get("beta", -0.5)), _lib.ptr(row_k), _lib.ptr(row_w), _lib.stream_handle())
+        return float(torch.sqrt((w64 * row_k / (w64.sum() * row_w)).sum()).item())
     tiles = -(-n // 64)
     js = max(1, min(tiles, 64))
     part = torch.empty((js, n), dtype=torch.float64, device="cuda")

@@ -98,3 +98,29 @@ def normal(key, shape: Shape = ()) -> torch.Tensor:
     out = torch.empty((*key.shape[:-1], *shape), dtype=torch.float32, device=key.device)
     _lib.call("mocat_random_normal", _lib.ptr(key), _n_keys(key), size, _lib.ptr(out), _lib.stream_handle())
     return out
+
+
+def randint(key, shape: Shape = (), minval: int = 0, maxval: int = 1) -> torch.Tensor:
+    """int32 draws in [minval, maxval) as jax.random.randint of the reference's era: two bit arrays
+    from split(key), combined modulo the span with uint32 arithmetic."""
+    key = as_key(key)
+    shape = _shape(shape)
+    size = math.prod(shape)
+    out = torch.empty((*key.shape[:-1], *shape), dtype=torch.int32, device=key.device)
+    _lib.call("mocat_random_randint", _lib.ptr(key), _n_keys(key), size, int(minval), int(maxval), _lib.ptr(out),
+              _lib.stream_handle())
+    return out
+
+
+def choice(key, a, shape: Shape = (), replace: bool = True, p=None) -> torch.Tensor:
+    """jax.random.choice with replacement and uniform probabilities (the form mocat calls,
+    mocat/src/metrics.py:103): randint(key, shape, 0, n), then a[ind] when a is an array."""
+    if not replace or p is not None:
+        raise NotImplementedError("choice is implemented with replace=True and p=None")
+    if isinstance(a, int):
+        return randint(key, shape, 0, a)
+    arr = torch.as_tensor(a)
+    if arr.dim() != 1:
+        raise ValueError("a must be an integer or one-dimensional")
+    ind = randint(key, shape, 0, arr.shape[0])
+    return arr.to(ind.device)[ind.long()]

@@ -69,8 +69,10 @@ def k0(x, y, gx, gy, kernel="gaussian", **params):
             + np.sum(gx * grad_y(xb, y, **params), -1) + call(xb, y, **params) * (gy @ gx))
 
 
-def ksd(x, g, kernel="gaussian", log_weight=None, **params):
-    """sqrt(sum_ij w_i w_j k_0(x_i, x_j)) / sum(w), full ensemble (ensemble_batchsize=None)."""
+def ksd(x, g, kernel="gaussian", log_weight=None, inds=None, **params):
+    """sqrt(sum_i sum_{j in batch_i} w_i w_j k_0(x_i, x_j) / (sum(w) sum_{j in batch_i} w_j))
+    (metrics.py:101-130): batch_i = all particles (ensemble_batchsize=None, inds None), or row i of
+    inds (n, batch), the indices random.choice(key, n, (n, batch)) draws."""
     x = np.asarray(x, np.float64)
     g = np.asarray(g, np.float64)
     n = len(x)
@@ -78,5 +80,9 @@ def ksd(x, g, kernel="gaussian", log_weight=None, **params):
     s = w.sum()
     total = 0.0
     for i in range(n):
-        total += w[i] * np.sum(w * k0(x[i], x, g[i], g, kernel, **params)) / (s * s)
+        if inds is None:
+            total += w[i] * np.sum(w * k0(x[i], x, g[i], g, kernel, **params)) / (s * s)
+        else:
+            b = np.asarray(inds[i], np.int64)
+            total += w[i] * np.sum(w[b] * k0(x[i], x[b], g[i], g[b], kernel, **params)) / (s * w[b].sum())
     return float(np.sqrt(total))

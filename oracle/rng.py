@@ -18,6 +18,13 @@ layout restated here:
   values agree with both to <= 5e-7 relative; the residual is the f32 `log` of the XLA build
   that printed them (its implied log1p input error is ~1e-6 absolute), which no test in the
   reference pins: exact f32 bits of normals beyond that tolerance are "parity unpinned".
+* randint (int32) / choice with replacement and no p (choice(key, n, shape) = randint(key, shape, 0,
+  n), used by mocat/src/metrics.py:103 for KSD minibatches): k1, k2 = split(key); hi, lo =
+  bits(k1), bits(k2); span = maxval - minval (uint32; 1 if maxval <= minval); m = (2^16 mod
+  span)^2 mod span; result = minval + ((hi mod span) m + lo mod span) mod span, uint32 arithmetic
+  wrapping at 32 bits (jax's legacy randint). No vector in the reference pins it: "parity
+  unpinned"; the tests check the special cases the formula fixes (span a power of two <= 2^16
+  gives lo mod span, span 1 gives minval) and the range.
 * categorical (Gumbel-max, used by mocat/src/ssm/filtering.py:196-199 and
   mocat/src/transport/smc.py:61-71): argmax_j(logits_j - log(-log(U_ij))), U ~ uniform(key,
   (n, N), tiny, 1). Pinned only by brute-force agreement on tiny inputs; exact indices depend on
@@ -98,6 +105,25 @@ def uniform_from_bits(b, minval=0.0, maxval=1.0):
 
 def uniform(key, shape=(), minval=0.0, maxval=1.0):
     return uniform_from_bits(bits(key, shape), minval, maxval)
+
+
+def randint(key, shape=(), minval=0, maxval=1):
+    """jax.random.randint (int32, legacy algorithm; see the module notes)."""
+    k1, k2 = split(key)
+    hi = bits(k1, shape).astype(np.uint64)
+    lo = bits(k2, shape).astype(np.uint64)
+    minval, maxval = int(minval), int(maxval)
+    span = 1 if maxval <= minval else (maxval - minval) & 0xFFFFFFFF
+    m = (65536 % span)
+    m = ((m * m) & 0xFFFFFFFF) % span
+    off = (((hi % span) * m) & 0xFFFFFFFF) + (lo % span)
+    off = (off & 0xFFFFFFFF) % span
+    return ((minval + off.astype(np.int64)) & 0xFFFFFFFF).astype(np.uint32).view(np.int32)
+
+
+def choice(key, n, shape=()):
+    """jax.random.choice(key, n, shape) with replacement and no p: randint(key, shape, 0, n)."""
+    return randint(key, shape, 0, n)
 
 
 NORMAL_LO = np.nextafter(F32(-1), F32(0))

@@ -43,3 +43,32 @@ def test_ksd_kernel_params_override_and_reference_pattern(m):
         b = m.ksd(torch.tensor(large), k, grad_potential=torch.tensor(large), bandwidth=h)
         assert b < a
         assert b == pytest.approx(ok.ksd(large, large, "gaussian", bandwidth=h), rel=2e-4)
+
+
+@pytest.mark.parametrize("n,d,batch", [(1, 3, 1), (70, 5, 16), (400, 30, 100), (300, 100, 65)])
+@pytest.mark.parametrize("kern", ["gaussian", "imq"])
+@pytest.mark.parametrize("weighted", [False, True])
+def test_ksd_minibatch_vs_oracle(m, n, d, batch, kern, weighted):
+    """ensemble_batchsize (metrics.py:101-104,126-128): particle i's batch is row i of
+    random.choice(key, n, (n, batch)); the same indices drive the oracle."""
+    from oracle import rng as orng
+    rs = np.random.default_rng(n + d + batch)
+    x = rs.normal(0, 1.5, (n, d)).astype(np.float32)
+    g = (x + rs.normal(0, 0.3, (n, d))).astype(np.float32)
+    lw = rs.normal(0, 1, n).astype(np.float32) if weighted else None
+    if kern == "gaussian":
+        kernel, prm = m.kernels.Gaussian(bandwidth=1.3 * np.sqrt(d)), dict(bandwidth=1.3 * np.sqrt(d))
+    else:
+        kernel, prm = m.kernels.IMQ(c=0.8, beta=-0.6, bandwidth=2.0), dict(c=0.8, beta=-0.6, bandwidth=2.0)
+    got = m.ksd(m.cdict(value=torch.tensor(x), grad_potential=torch.tensor(g)), kernel,
+                log_weight=None if lw is None else torch.tensor(lw), ensemble_batchsize=batch,
+                random_key=m.random.PRNGKey(5))
+    inds = orng.choice(orng.PRNGKey(5), n, (n, batch))
+    want = ok.ksd(x, g, kern, log_weight=lw, inds=inds, **prm)
+    assert got == pytest.approx(want, rel=2e-4), (got, want)
+
+
+def test_ksd_minibatch_needs_a_key(m):
+    x = torch.zeros(4, 2)
+    with pytest.raises(TypeError):
+        m.ksd(m.cdict(value=x, grad_potential=x), m.kernels.Gaussian(bandwidth=1.0), ensemble_batchsize=2)

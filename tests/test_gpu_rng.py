@@ -66,3 +66,23 @@ def test_errors_are_raised(mr):
     from mocat_amd._lib import MocatKernelError
     with pytest.raises(MocatKernelError):
         mr.bits(mr.PRNGKey(0), (2 ** 32,))
+
+
+@pytest.mark.parametrize("lo,hi", [(0, 4096), (5, 17), (0, 1), (3, 3), (-7, 1000003), (0, 2 ** 31 - 1), (0, 300)])
+@pytest.mark.parametrize("size", [1, 7, 4096, 30001])
+def test_randint_bit_exact(mr, lo, hi, size):
+    k = mr.split(mr.PRNGKey(11), 2)[1]
+    ko = orng.split(orng.PRNGKey(11), 2)[1]
+    np.testing.assert_array_equal(_np(mr.randint(k, (size,), lo, hi)), orng.randint(ko, (size,), lo, hi))
+
+
+def test_randint_batched_keys_and_choice(mr):
+    keys = mr.split(mr.PRNGKey(3), 5)
+    kn = orng.split(orng.PRNGKey(3), 5)
+    got = _np(mr.randint(keys, (6, 7), 2, 99))
+    assert got.shape == (5, 6, 7)
+    for i in range(5):
+        np.testing.assert_array_equal(got[i], orng.randint(kn[i], (6, 7), 2, 99))
+    np.testing.assert_array_equal(_np(mr.choice(keys[0], 4096, (64, 10))), orng.choice(kn[0], 4096, (64, 10)))
+    a = torch.arange(100, 150, device="cuda") * 3
+    np.testing.assert_array_equal(_np(mr.choice(keys[1], a, (20,))), _np(a)[orng.choice(kn[1], 50, (20,))])

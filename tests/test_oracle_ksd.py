@@ -93,3 +93,21 @@ def test_ksd_decreases_with_ensemble_size():
     k_small, k_large = ok.ksd(small, -small, "gaussian", bandwidth=1.0), ok.ksd(large, -large, "gaussian",
                                                                                   bandwidth=1.0)
     assert k_large < 0.1 and k_large < 0.5 * k_small
+
+
+def test_minibatch_with_every_particle_in_each_batch_is_the_full_ksd():
+    """metrics.py:101-128: batches that hold every particle once reduce to the full-ensemble sum
+    (the batch weight sum becomes sum(w)); a repeated index counts twice."""
+    rs = np.random.default_rng(5)
+    x, g = rs.normal(size=(12, 3)), rs.normal(size=(12, 3))
+    lw = rs.normal(size=12)
+    inds = np.tile(np.arange(12), (12, 1))
+    for kern, prm in (("gaussian", dict(bandwidth=1.3)), ("imq", dict(c=0.7, beta=-0.4, bandwidth=1.1))):
+        assert ok.ksd(x, g, kern, log_weight=lw, inds=inds, **prm) == pytest.approx(
+            ok.ksd(x, g, kern, log_weight=lw, **prm), rel=1e-12)
+    # a batch of one particle j: w_i k_0(x_i, x_j) / sum(w) summed over i
+    j = np.full((12, 1), 4)
+    w = np.exp(lw)
+    want = np.sqrt(sum(w[i] * ok.k0(x[i], x[[4]], g[i], g[[4]], "gaussian", bandwidth=1.3)[0] for i in range(12))
+                   / w.sum())
+    assert ok.ksd(x, g, "gaussian", log_weight=lw, inds=j, bandwidth=1.3) == pytest.approx(want, rel=1e-12)

@@ -86,3 +86,19 @@ def test_batched_helpers_match_scalar_path():
         np.testing.assert_array_equal(rng.bits_batch(keys, size), want)
     np.testing.assert_array_equal(rng.split_batch(keys, 3), np.stack([rng.split(k, 3) for k in keys]))
     np.testing.assert_array_equal(rng.normal_batch(keys, 7), np.stack([rng.normal(k, (7,)) for k in keys]))
+
+
+def test_randint_special_cases():
+    """randint's combination (see oracle/rng.py) reduces to the second key's bits modulo the span when
+    the span is a power of two up to 2^16 (2^32 is then a multiple of the span), and to minval when
+    the span is 1 or maxval <= minval; otherwise it stays in [minval, maxval)."""
+    k = rng.PRNGKey(7)
+    lo_bits = rng.bits(rng.split(k)[1], (1000,))
+    for span in (1, 2, 64, 4096, 65536):
+        np.testing.assert_array_equal(rng.randint(k, (1000,), 3, 3 + span), 3 + (lo_bits % span).astype(np.int64))
+    np.testing.assert_array_equal(rng.randint(k, (5,), 9, 2), np.full(5, 9))
+    for lo, hi in ((0, 3), (-7, 1000003), (5, 4096 + 5), (0, 2 ** 31 - 1)):
+        r = rng.randint(k, (20000,), lo, hi).astype(np.int64)
+        assert r.min() >= lo and r.max() < hi
+        assert len(np.unique(r)) > min(hi - lo, 20000) // 4
+    np.testing.assert_array_equal(rng.choice(k, 300, (4, 5)), rng.randint(k, (4, 5), 0, 300))
