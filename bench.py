@@ -230,9 +230,39 @@ def bench_svgd4(n=4096, d=100, iters=500):
     smp = m.SVGD(stepsize=1e-2, max_iter=iters, bandwidth_update="median")
     chain, dt = _timed(lambda: m.run(sc, smp, n, m.random.PRNGKey(0)))
     flop = 6.0 * n * n * d * iters
+    quality = svgd_quality(m, sc, smp, chain)
     print(json.dumps({"mode": "svgd4", "metric": "iterations/s", "value": iters / dt, "seconds": dt,
                       "stein_algorithmic_TFLOPs_incl_bandwidth_and_potential": flop / dt / 1e12,
-                      "final_bandwidth": chain.bandwidth[-1].item()}))
+                      "final_bandwidth": chain.bandwidth[-1].item(), **quality}))
+
+
+def svgd_quality(m, sc, smp, chain):
+    """Untimed checks on the final particles: the Stein direction of one more update (bf16x3 MFMA
+    path) against an FP64 evaluation of the same formula, and the KSD (Gaussian kernel at the final
+    bandwidth) after the first and the last iteration, both as mocat's ksd computes it (grad U where
+    a Stein kernel takes the score) and with the score -grad U (the textbook discrepancy)."""
+    import torch
+    st, ex = clone(chain.final_state), clone(chain.final_extra)
+    x = st.value.to(torch.float64)
+    g = st.grad_potential.to(torch.float64)
+    h = float(chain.bandwidth[-1].item())
+    phi = smp.step(st, ex, sc).to(torch.float64)
+    n = x.shape[0]
+    sq = (x * x).sum(1)
+    D = (sq[:, None] + sq[None, :] - 2.0 * x @ x.T).clamp_min(0.0)
+    K = torch.exp(-D / (2.0 * h * h))
+    ref = (-(K @ g) + (K.sum(1, keepdim=True) * x - K @ x) / (h * h)) / n
+    err = float(((phi - ref).norm() / ref.norm()).item())
+    kern = m.kernels.Gaussian(bandwidth=h)
+    x0 = m.run(sc, m.SVGD(stepsize=1e-2, max_iter=1, bandwidth_update="median"), n, m.random.PRNGKey(0))
+    first = x0.final_state  # one update from the same start: its potential gradient is at hand
+    last = chain.final_state
+    return {"stein_direction_rel_err_vs_fp64": err,
+            "mean_potential": {"after_first_iteration": float(first.potential.mean().item()),
+                               "final": float(last.potential.mean().item())},
+            "ksd_mocat_convention": {"after_first_iteration": m.ksd(first, kern), "final": m.ksd(last, kern)},
+            "ksd_score": {"after_first_iteration": m.ksd(first.value, kern, grad_potential=-first.grad_potential),
+                          "final": m.ksd(last.value, kern, grad_potential=-last.grad_potential)}}
 
 
 def main():
