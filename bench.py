@@ -173,9 +173,19 @@ def bench_pf1():
                                               "systematic")
     runb()
     res, dtb = _timed(runb)
+    # scaled variant for bandwidth evidence: one filter of 2^20 particles (chunked path, no trajectory)
+    big = 1 << 20
+    runs = lambda: m.ssm.run_particle_filters([model], [yn], [keys[0]], [big], 1.0, "systematic")
+    runs()
+    _, dts = _timed(runs)
+    steps = big * 100
     print(json.dumps({"mode": "pf1", "metric": "particle-steps/s", "single_filter": 4096 * 100 / dt,
                       "single_filter_ms": dt * 1e3, "batch256": 256 * 4096 * 100 / dtb, "batch256_ms": dtb * 1e3,
-                      "log_likelihood": out.log_likelihood.item(), "store_trajectory": True}))
+                      "log_likelihood": out.log_likelihood.item(), "store_trajectory": True,
+                      "scaled_n2e20": {"ms": dts * 1e3, "particle_steps_per_s": steps / dts,
+                                       "algorithmic_GBps": 48 * steps / dts / 1e9,
+                                       "hbm_frac": 48 * steps / dts / 8e12, "bytes_per_particle_step": 48,
+                                       "store_trajectory": False}}))
 
 
 def bench_pf5():
