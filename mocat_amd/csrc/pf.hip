@@ -677,7 +677,8 @@ constexpr int PPT = 4;           // particles per thread
 constexpr int CH = LT * PPT;     // particles per chunk (one workgroup)
 constexpr int SG = 256;          // prefix-sum search granularity (chunk ends staged in LDS)
 constexpr int kSearchWin = 2048;   // prefix sums of a block's ancestor chunks staged in LDS (16 KB)
-constexpr int kChunkEnds = 1024;   // SG-block end sums staged in LDS (filters up to 262144 particles)
+constexpr int kChunkEnds = 4096;   // SG-block end sums staged in LDS (filters up to 2^20 particles);
+                                   // dynamic LDS sized by the launch's largest filter (at most 32 KB)
 
 struct LfDesc {
   int64_t n, T, y_offset, state_offset, out_offset, t_offset, chunk0;
@@ -1002,7 +1003,7 @@ __global__ __launch_bounds__(LT) void lf_csum_kernel(LfArgs a, int64_t t) {
 }
 
 template <class Model, int D, int DY, bool GRAD>
-__global__ __launch_bounds__(LT) void lf_step_kernel(LfArgs a, int64_t t) {
+__global__ __launch_bounds__(LT) void lf_step_kernel(LfArgs a, int64_t t, int cend_slots) {
   __shared__ LfScratch s;
   __shared__ Model model;
   const int64_t c = blockIdx.x;
@@ -1043,12 +1044,12 @@ __global__ __launch_bounds__(LT) void lf_step_kernel(LfArgs a, int64_t t) {
   // the first index inside the block found. Systematic targets rise with the particle index, so this
   // chunk's ancestors lie in a contiguous run of blocks, staged in LDS too when it is short.
   // (res and the method are per filter: the branches are uniform over the workgroup.)
-  __shared__ uint64_t cend[kChunkEnds];
+  extern __shared__ uint64_t cend[];  // cend_slots entries (launch-time size)
   __shared__ uint64_t win[kSearchWin];
   __shared__ int64_t wk[2];
   const uint64_t* C = a.csum + F.state_offset;
   const int64_t nsg = (n + SG - 1) / SG;
-  const bool two_level = res && a.resampling != 2 && nsg <= kChunkEnds;
+  const bool two_level = res && a.resampling != 2 && nsg <= cend_slots;
   bool staged = false;
   int64_t lo = 0;
   const double W = (res && a.resampling != 2) ? (double)C[n - 1] : 0.0;
@@ -1180,14 +1181,15 @@ __global__ __launch_bounds__(LT) void lf_finish_kernel(LfArgs a) {
 }
 
 template <class Model, int D, int DY, bool GRAD = false>
-int launch_pf_large(const LfArgs& a, int64_t max_T, hipStream_t st) {
+int launch_pf_large(const LfArgs& a, int64_t max_T, int cend_slots, hipStream_t st) {
   const dim3 chunks((unsigned)a.n_chunks), filters((unsigned)a.n_large), blk(LT);
+  const size_t cend_bytes = (size_t)cend_slots * sizeof(uint64_t);
   hipLaunchKernelGGL((lf_init_kernel<Model, D, DY, GRAD>), chunks, blk, 0, st, a);
   hipLaunchKernelGGL((lf_stats_kernel<GRAD>), filters, blk, 0, st, a, (int64_t)0);
   for (int64_t t = 1; t < max_T; ++t) {
     hipLaunchKernelGGL(lf_totals_kernel, chunks, blk, 0, st, a, t);
     hipLaunchKernelGGL(lf_csum_kernel, chunks, blk, 0, st, a, t);
-    hipLaunchKernelGGL((lf_step_kernel<Model, D, DY, GRAD>), chunks, blk, 0, st, a, t);
+    hipLaunchKernelGGL((lf_step_kernel<Model, D, DY, GRAD>), chunks, blk, cend_bytes, st, a, t, cend_slots);
     hipLaunchKernelGGL((lf_stats_kernel<GRAD>), filters, blk, 0, st, a, t);
   }
   hipLaunchKernelGGL((lf_finish_kernel<D>), chunks, blk, 0, st, a);
@@ -1206,6 +1208,7 @@ int launch_pf(const PfArgs& a, hipStream_t st) {
 
 struct LfLayout {
   int n_large = 0;
+  int cend_slots = 1;  // SG-block end sums the step kernel stages: the largest filter's, up to kChunkEnds
   int64_t n_chunks = 0, max_T = 0;
   int64_t o_desc, o_cf, o_m, o_s1, o_s2, o_g, o_tot, o_st, bytes;
 };
@@ -1219,6 +1222,7 @@ LfLayout lf_layout(const mocat_pf_filter* filters, int n_filters) {
     ++L.n_large;
     L.n_chunks += (filters[f].n + CH - 1) / CH;
     L.max_T = std::max<int64_t>(L.max_T, filters[f].T);
+    L.cend_slots = (int)std::min<int64_t>(kChunkEnds, std::max<int64_t>(L.cend_slots, (filters[f].n + SG - 1) / SG));
   }
   int64_t o = 0;
   L.o_desc = o;
@@ -1331,12 +1335,12 @@ int mocat_pf_run_large(int model, int d, int dy, const mocat_pf_filter* host_fil
            reinterpret_cast<uint64_t*>(ws + L.o_tot), reinterpret_cast<LfState*>(ws + L.o_st)};
   if (model == 1) {
     if (d != 1 || dy != 1) return MOCAT_ERR_ARG;
-    return out_grad ? launch_pf_large<Sv, 1, 1, true>(a, L.max_T, stream)
-                    : launch_pf_large<Sv, 1, 1>(a, L.max_T, stream);
+    return out_grad ? launch_pf_large<Sv, 1, 1, true>(a, L.max_T, L.cend_slots, stream)
+                    : launch_pf_large<Sv, 1, 1>(a, L.max_T, L.cend_slots, stream);
   }
   if (model != 0) return MOCAT_ERR_ARG;
 #define MOCAT_PF_CASE(DD, YY) \
-  if (d == DD && dy == YY) return launch_pf_large<Lgssm<DD, YY>, DD, YY>(a, L.max_T, stream);
+  if (d == DD && dy == YY) return launch_pf_large<Lgssm<DD, YY>, DD, YY>(a, L.max_T, L.cend_slots, stream);
   MOCAT_PF_CASE(1, 1) MOCAT_PF_CASE(2, 2) MOCAT_PF_CASE(4, 4) MOCAT_PF_CASE(4, 2) MOCAT_PF_CASE(5, 5)
   MOCAT_PF_CASE(8, 8) MOCAT_PF_CASE(2, 1) MOCAT_PF_CASE(3, 3) MOCAT_PF_CASE(4, 1) MOCAT_PF_CASE(8, 4)
 #undef MOCAT_PF_CASE

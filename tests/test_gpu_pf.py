@@ -226,3 +226,20 @@ def test_chunked_path_categorical_and_gradient(m):
                                 ns[i], big[i].ancestors.cpu().numpy(), flags)
         np.testing.assert_allclose(big[i].log_likelihood_grad.cpu().numpy(), og, rtol=2e-3,
                                    atol=2e-3 * (1 + np.abs(og).max()))
+
+
+@pytest.mark.parametrize("resampling", ["systematic", "multinomial"])
+def test_chunked_path_staged_search_limits(m, resampling):
+    """Filters at the edge of the staged two-level ancestor search (2^20 particles: 4096 chunk-end
+    sums, the most staged) and just past it (2^20 + 1: the plain search in global memory), batched
+    with a small filter in one launch, against the one-workgroup path: same ancestors and particles."""
+    from mocat_amd import datasets
+    models, ys, _ = datasets.sv_batch_config(6, n_models=3, n_min=64, n_max=3000, t_min=6, t_max=6)
+    ns = [1 << 20, (1 << 20) + 1, 3000]
+    keys = [k for k in m.random.split(m.random.PRNGKey(9), 3)]
+    big, one = _run_both(m, models, ys, keys, ns, 1.0, resampling)
+    for i in range(3):
+        np.testing.assert_array_equal(big[i].ancestors.cpu().numpy(), one[i].ancestors.cpu().numpy())
+        np.testing.assert_array_equal(big[i].final_value.cpu().numpy(), one[i].final_value.cpu().numpy())
+        a, b = big[i].log_likelihood.item(), one[i].log_likelihood.item()
+        assert abs(a - b) <= 1e-10 * (1 + abs(b)), (i, a, b)
