@@ -725,6 +725,7 @@ struct LfArgs {
   double* part_g;    // [n_chunks][3] tangent sums (GRAD)
   uint64_t* ctot;    // [n_chunks] quantised weight totals
   LfState* st;       // [n_large]
+  uint64_t* cends;   // [n_chunks * CH / SG] prefix sums at the SG-block ends, per filter from chunk0 * CH / SG
 };
 
 struct LfScratch {
@@ -998,8 +999,67 @@ __global__ __launch_bounds__(LT) void lf_csum_kernel(LfArgs a, int64_t t) {
 #pragma unroll
   for (int pk = 0; pk < PPT; ++pk) {
     run += q[pk];
-    if (i0 + pk < F.n) a.csum[F.state_offset + i0 + pk] = run;
+    const int64_t i = i0 + pk;
+    if (i < F.n) {
+      a.csum[F.state_offset + i] = run;
+      // the SG-block end sums, contiguous per filter, for the step kernel's first search level
+      if ((i + 1) % SG == 0 || i == F.n - 1) a.cends[F.chunk0 * (CH / SG) + i / SG] = run;
+    }
   }
+}
+
+// dst[k] = src(k) for k < count, by the whole workgroup, U loads per thread issued before their stores.
+template <int U, class Src>
+MOCAT_DEV void stage_lds(uint64_t* dst, int64_t count, Src src) {
+  for (int64_t k0 = 0; k0 < count; k0 += (int64_t)U * LT) {
+    uint64_t v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t k = k0 + (int64_t)u * LT + threadIdx.x;
+      v[u] = k < count ? src(k) : 0ull;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t k = k0 + (int64_t)u * LT + threadIdx.x;
+      if (k < count) dst[k] = v[u];
+    }
+  }
+}
+
+// For P searches at once: out[p] = the first k in [b[p], b[p] + m[p]) with v(k) > tg[p] (v
+// non-decreasing), or b[p] + m[p] when there is none; inactive searches return b[p]. Each halving
+// step issues the reads of all unfinished searches before comparing.
+template <int P, class Get>
+MOCAT_DEV void upper_bound_lockstep(Get v, const int64_t* b, const int64_t* m, const double* tg, const bool* act,
+                                    int64_t* out) {
+  int64_t lo[P], len[P];
+#pragma unroll
+  for (int q = 0; q < P; ++q) {
+    lo[q] = b[q];
+    len[q] = act[q] ? m[q] : 0;
+  }
+  while (true) {
+    bool any = false;
+#pragma unroll
+    for (int q = 0; q < P; ++q) any |= len[q] > 0;
+    if (!any) break;
+    uint64_t val[P];
+#pragma unroll
+    for (int q = 0; q < P; ++q) val[q] = len[q] > 0 ? v(lo[q] + (len[q] >> 1)) : 0ull;
+#pragma unroll
+    for (int q = 0; q < P; ++q) {
+      if (len[q] <= 0) continue;
+      const int64_t h = len[q] >> 1;
+      if ((double)val[q] <= tg[q]) {
+        lo[q] += h + 1;
+        len[q] -= h + 1;
+      } else {
+        len[q] = h;
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < P; ++q) out[q] = lo[q];
 }
 
 template <class Model, int D, int DY, bool GRAD>
@@ -1011,22 +1071,9 @@ __global__ __launch_bounds__(LT) void lf_step_kernel(LfArgs a, int64_t t, int ce
   const LfDesc F = a.desc[fi];
   if (t >= F.T) return;
   const int tid = threadIdx.x;
-  // the step key (and the systematic uniform) are the same for the whole chunk: derived once
-  __shared__ uint32_t sk[2];
-  __shared__ float su;
-  if (tid == 0) {
-    model.load(a.params + F.param_offset);
-    const Key key{F.key[0], F.key[1]};
-    const Key ikk = split_at(split_at(key, 2, 0), (uint32_t)(F.T - 1), (uint32_t)(t - 1));
-    sk[0] = ikk.a;
-    sk[1] = ikk.b;
-    su = uniform_from_bits(bits_at(ikk, 1, 0), 0.f, 1.f);
-  }
-  __syncthreads();
   const LfState st = a.st[fi];
   const int64_t n = F.n, base = (c - F.chunk0) * CH;
   const bool res = st.res != 0;
-  const Key ik{sk[0], sk[1]};
   const bool odd = (t - 1) & 1;
   const float* xa = (odd ? a.x1 : a.x0) + F.state_offset * D;
   float* xb = (odd ? a.x0 : a.x1) + F.state_offset * D;
@@ -1042,7 +1089,8 @@ __global__ __launch_bounds__(LT) void lf_step_kernel(LfArgs a, int64_t t, int ce
   // Ancestors by a two-level search in the exact prefix sums, without long chains of dependent
   // global loads: the last sum of every SG-particle block is staged in LDS and searched first, then
   // the first index inside the block found. Systematic targets rise with the particle index, so this
-  // chunk's ancestors lie in a contiguous run of blocks, staged in LDS too when it is short.
+  // chunk's ancestors lie in a contiguous run of blocks, staged in LDS too when it is short. Each
+  // thread searches for its PPT particles in lockstep (independent LDS or global reads overlap).
   // (res and the method are per filter: the branches are uniform over the workgroup.)
   extern __shared__ uint64_t cend[];  // cend_slots entries (launch-time size)
   __shared__ uint64_t win[kSearchWin];
@@ -1054,71 +1102,116 @@ __global__ __launch_bounds__(LT) void lf_step_kernel(LfArgs a, int64_t t, int ce
   int64_t lo = 0;
   const double W = (res && a.resampling != 2) ? (double)C[n - 1] : 0.0;
   const double sstep = W / (double)n;
-  // first SG block whose last sum exceeds t (nsg: none does)
-  auto first_block = [&](double tg) {
-    int64_t l = 0, h = nsg;
-    while (l < h) {
-      const int64_t mid = (l + h) >> 1;
-      if ((double)cend[mid] > tg)
-        h = mid;
-      else
-        l = mid + 1;
+  // the chunk-end sums go first: their loads are in flight while thread 0 derives the step key
+  const uint64_t* CE = a.cends + F.chunk0 * (CH / SG);
+  if (two_level) stage_lds<8>(cend, nsg, [&](int64_t k) { return CE[k]; });
+  // the step key (and the systematic uniform) are the same for the whole chunk: derived once
+  __shared__ uint32_t sk[2];
+  __shared__ float su;
+  if (tid == 0) {
+    model.load(a.params + F.param_offset);
+    const Key key{F.key[0], F.key[1]};
+    const Key ikk = split_at(split_at(key, 2, 0), (uint32_t)(F.T - 1), (uint32_t)(t - 1));
+    sk[0] = ikk.a;
+    sk[1] = ikk.b;
+    su = uniform_from_bits(bits_at(ikk, 1, 0), 0.f, 1.f);
+  }
+  __syncthreads();
+  const Key ik{sk[0], sk[1]};
+  auto cend_at = [&](int64_t k) { return cend[k]; };
+  if (two_level && a.resampling == 0) {
+    if (tid == 0) {
+      const int64_t j1 = (base + CH < n ? base + CH : n) - 1;
+      const int64_t b0[2] = {0, 0}, m0[2] = {nsg, nsg};
+      const double tg[2] = {((double)base + (double)su) * sstep, ((double)j1 + (double)su) * sstep};
+      const bool on[2] = {true, true};
+      int64_t k[2];
+      upper_bound_lockstep<2>(cend_at, b0, m0, tg, on, k);
+      wk[0] = k[0];
+      wk[1] = k[1];
     }
-    return l;
-  };
-  if (two_level) {
-    for (int64_t k = tid; k < nsg; k += LT) cend[k] = C[(k + 1) * SG < n ? (k + 1) * SG - 1 : n - 1];
     __syncthreads();
-    if (a.resampling == 0) {
-      if (tid == 0) {
-        const int64_t j1 = (base + CH < n ? base + CH : n) - 1;
-        wk[0] = first_block(((double)base + (double)su) * sstep);
-        wk[1] = first_block(((double)j1 + (double)su) * sstep);
-      }
+    const int64_t k0 = wk[0] < nsg ? wk[0] : nsg - 1, k1 = wk[1] < nsg ? wk[1] : nsg - 1;
+    lo = k0 * SG;
+    const int64_t len = ((k1 + 1) * SG < n ? (k1 + 1) * SG : n) - lo;
+    if (len <= kSearchWin) {
+      stage_lds<8>(win, len, [&](int64_t k) { return C[lo + k]; });
       __syncthreads();
-      const int64_t k0 = wk[0] < nsg ? wk[0] : nsg - 1, k1 = wk[1] < nsg ? wk[1] : nsg - 1;
-      lo = k0 * SG;
-      const int64_t len = ((k1 + 1) * SG < n ? (k1 + 1) * SG : n) - lo;
-      if (len <= kSearchWin) {
-        for (int64_t k = tid; k < len; k += LT) win[k] = C[lo + k];
-        __syncthreads();
-        staged = true;
-      }
+      staged = true;
     }
   }
 
+  // ancestors of the thread's PPT particles
+  int64_t srcs[PPT];
+  bool live[PPT];
+#pragma unroll
+  for (int pk = 0; pk < PPT; ++pk) {
+    srcs[pk] = base + pk * LT + tid;
+    live[pk] = srcs[pk] < n;
+  }
+  if (res && a.resampling == 2) {
+#pragma unroll
+    for (int pk = 0; pk < PPT; ++pk)
+      if (live[pk]) srcs[pk] = categorical_row(lwa, n, ik, (uint32_t)(n * n), srcs[pk]);
+  } else if (res) {
+    double tg[PPT];
+#pragma unroll
+    for (int pk = 0; pk < PPT; ++pk) {
+      const int64_t i = srcs[pk];
+      tg[pk] = a.resampling == 0
+                   ? ((double)i + (double)su) * sstep
+                   : (double)uniform_from_bits(bits_at(ik, (uint32_t)n, (uint32_t)i), 0.f, 1.f) * W;
+    }
+    if (two_level) {
+      // first SG block whose last sum exceeds the target (nsg: none does), then the first index in it
+      int64_t kc[PPT], b[PPT], m[PPT], zero[PPT], all[PPT];
+#pragma unroll
+      for (int pk = 0; pk < PPT; ++pk) {
+        zero[pk] = 0;
+        all[pk] = nsg;
+      }
+      upper_bound_lockstep<PPT>(cend_at, zero, all, tg, live, kc);
+      bool in[PPT];
+#pragma unroll
+      for (int pk = 0; pk < PPT; ++pk) {
+        in[pk] = live[pk] && kc[pk] < nsg;
+        b[pk] = kc[pk] * SG;
+        // the block's last index has a sum above the target: search the indices before it
+        m[pk] = in[pk] ? ((kc[pk] + 1) * SG < n ? (kc[pk] + 1) * SG : n) - 1 - b[pk] : 0;
+      }
+      int64_t p[PPT];
+      if (staged)
+        upper_bound_lockstep<PPT>([&](int64_t k) { return win[k - lo]; }, b, m, tg, in, p);
+      else
+        upper_bound_lockstep<PPT>([&](int64_t k) { return C[k]; }, b, m, tg, in, p);
+#pragma unroll
+      for (int pk = 0; pk < PPT; ++pk)
+        if (live[pk]) srcs[pk] = in[pk] ? p[pk] : n - 1;
+    } else {
+      int64_t p[PPT], zero[PPT], all[PPT];
+#pragma unroll
+      for (int pk = 0; pk < PPT; ++pk) {
+        zero[pk] = 0;
+        all[pk] = n;
+      }
+      upper_bound_lockstep<PPT>([&](int64_t k) { return C[k]; }, zero, all, tg, live, p);
+#pragma unroll
+      for (int pk = 0; pk < PPT; ++pk)
+        if (live[pk]) srcs[pk] = p[pk] < n ? p[pk] : n - 1;
+    }
+  }
+  // ancestor states of all PPT particles loaded together
+  float xpre[PPT][D];
+#pragma unroll
+  for (int pk = 0; pk < PPT; ++pk)
+#pragma unroll
+    for (int q = 0; q < D; ++q) xpre[pk][q] = live[pk] ? xa[srcs[pk] * D + q] : 0.f;
+
+#pragma unroll
   for (int pk = 0; pk < PPT; ++pk) {
     const int64_t i = base + pk * LT + tid;
     if (i >= n) break;
-    int64_t src = i;
-    if (res && a.resampling == 2) {
-      src = categorical_row(lwa, n, ik, (uint32_t)(n * n), i);
-    } else if (res) {
-      const double target =
-          a.resampling == 0
-              ? ((double)i + (double)su) * sstep
-              : (double)uniform_from_bits(bits_at(ik, (uint32_t)n, (uint32_t)i), 0.f, 1.f) * W;
-      if (two_level) {
-        const int64_t kc = first_block(target);
-        if (kc >= nsg) {
-          src = n - 1;
-        } else {
-          // first p in block kc with C[p] > target (C at the block's last index exceeds it)
-          int64_t l = kc * SG, h = ((kc + 1) * SG < n ? (kc + 1) * SG : n) - 1;
-          while (l < h) {
-            const int64_t mid = (l + h) >> 1;
-            const uint64_t cm = staged ? win[mid - lo] : C[mid];
-            if ((double)cm > target)
-              h = mid;
-            else
-              l = mid + 1;
-          }
-          src = l;
-        }
-      } else {
-        src = search_csum(C, n, target);
-      }
-    }
+    const int64_t src = srcs[pk];
     const Key ki = split_at(ik, (uint32_t)n, (uint32_t)i);
     float eps[D], xp[D], x[D];
 #pragma unroll
@@ -1129,7 +1222,7 @@ __global__ __launch_bounds__(LT) void lf_step_kernel(LfArgs a, int64_t t, int ce
       if (q + hD < (uint32_t)D) eps[q + hD] = normal_from_bits(w1);
     }
 #pragma unroll
-    for (int q = 0; q < D; ++q) xp[q] = xa[src * D + q];
+    for (int q = 0; q < D; ++q) xp[q] = xpre[pk][q];
     model.transition(xp, eps, x);
     float l;
     if constexpr (GRAD) {
@@ -1210,7 +1303,7 @@ struct LfLayout {
   int n_large = 0;
   int cend_slots = 1;  // SG-block end sums the step kernel stages: the largest filter's, up to kChunkEnds
   int64_t n_chunks = 0, max_T = 0;
-  int64_t o_desc, o_cf, o_m, o_s1, o_s2, o_g, o_tot, o_st, bytes;
+  int64_t o_desc, o_cf, o_m, o_s1, o_s2, o_g, o_tot, o_st, o_cend, bytes;
 };
 
 int64_t lf_align(int64_t v) { return (v + 255) / 256 * 256; }
@@ -1241,6 +1334,8 @@ LfLayout lf_layout(const mocat_pf_filter* filters, int n_filters) {
   o = lf_align(o + L.n_chunks * 8);
   L.o_st = o;
   o = lf_align(o + (int64_t)L.n_large * (int64_t)sizeof(LfState));
+  L.o_cend = o;
+  o = lf_align(o + L.n_chunks * (CH / SG) * 8);
   L.bytes = o;
   return L;
 }
@@ -1332,7 +1427,8 @@ int mocat_pf_run_large(int model, int d, int dy, const mocat_pf_filter* host_fil
            ess_threshold, out_value, out_lw, out_anc, out_ess, out_loglik_t, out_loglik, out_grad,
            reinterpret_cast<float*>(ws + L.o_m), reinterpret_cast<double*>(ws + L.o_s1),
            reinterpret_cast<double*>(ws + L.o_s2), reinterpret_cast<double*>(ws + L.o_g),
-           reinterpret_cast<uint64_t*>(ws + L.o_tot), reinterpret_cast<LfState*>(ws + L.o_st)};
+           reinterpret_cast<uint64_t*>(ws + L.o_tot), reinterpret_cast<LfState*>(ws + L.o_st),
+           reinterpret_cast<uint64_t*>(ws + L.o_cend)};
   if (model == 1) {
     if (d != 1 || dy != 1) return MOCAT_ERR_ARG;
     return out_grad ? launch_pf_large<Sv, 1, 1, true>(a, L.max_T, L.cend_slots, stream)
