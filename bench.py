@@ -42,9 +42,9 @@ def parse():
     ap.add_argument("--cpu-iters", type=int, default=120)
     ap.add_argument("--plain-steps", type=int, default=30, help="iterations timed on the unfused FP32 path (0: skip)")
     ap.add_argument("--iters", type=int, default=500, help="iterations of the svgd4 workload")
-    ap.add_argument("--mode", default="hmc", choices=["hmc", "copy", "pf1", "pf5", "smc3", "svgd4"],
+    ap.add_argument("--mode", default="hmc", choices=["hmc", "copy", "pf1", "pf5", "smc3", "svgd4", "ffbsi"],
                     help="hmc: the headline workload. copy: a plain 256 MiB device copy (calibration run for HBM "
-                         "byte counters). pf1 / pf5 / smc3 / svgd4: the other workloads, one JSON line each")
+                         "byte counters). pf1 / pf5 / smc3 / svgd4 / ffbsi: the other workloads, one JSON line each")
     return ap.parse_args()
 
 
@@ -275,6 +275,23 @@ def svgd_quality(m, sc, smp, chain):
                           "final": m.ksd(last.value, kern, grad_potential=-last.grad_potential)}}
 
 
+def bench_ffbsi(n=4096, T=100):
+    """Backward simulation after the pf1 filter (LGSSM d = 4, T = 100): n filter particles and n
+    trajectories, full resampling (n x n transition potentials per step)."""
+    import mocat_amd as m
+    from mocat_amd import datasets
+    model, y, t = datasets.lgssm_config(0, 4, T)
+    marg = m.ssm.run_particle_filter_for_marginals(model, m.ssm.BootstrapFilter(), y, t, m.random.PRNGKey(0), n=n,
+                                                   ess_threshold=1.0, resampling="systematic")
+    run = lambda: m.ssm.backward_simulation(model, marg, m.random.PRNGKey(1), n)
+    run()
+    res, dt = _timed(run)
+    pairs = (T - 1) * n * n
+    print(json.dumps({"mode": "ffbsi", "metric": "pair evaluations/s", "value": pairs / dt, "ms": dt * 1e3,
+                      "particles": n, "trajectories": n, "T": T, "ms_per_backward_step": dt * 1e3 / (T - 1),
+                      "note": "per pair: one transition potential (d = 4) and half a threefry block"}))
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -293,7 +310,7 @@ def main():
         if a.mode == "svgd4":
             bench_svgd4(iters=a.iters)
         else:
-            {"pf1": bench_pf1, "pf5": bench_pf5, "smc3": bench_smc3}[a.mode]()
+            {"pf1": bench_pf1, "pf5": bench_pf5, "smc3": bench_smc3, "ffbsi": bench_ffbsi}[a.mode]()
         return
     import mocat_amd as m
     from mocat_amd import datasets

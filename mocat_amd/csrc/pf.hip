@@ -651,12 +651,37 @@ __global__ __launch_bounds__(256) void resample_search_kernel(const uint64_t* __
     idx[j] = ancestor_at(csum, n, W, k, method, u_sys, j);
 }
 
+// random.categorical(key, lw, shape=(n_out,)): one wave per output j, the lanes over the n Gumbel
+// draws of row j (elements j n + i of the (n_out, n) bits array); each lane keeps its first maximum,
+// and a butterfly takes the larger value, the lower index on ties (the first maximum overall).
 __global__ __launch_bounds__(256) void categorical_kernel(const float* __restrict__ lw, int64_t n, int64_t n_out,
                                                           const uint32_t* key, int32_t* idx) {
   const Key k{key[0], key[1]};
   const uint32_t S = (uint32_t)(n * n_out);
-  for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < n_out; j += (int64_t)gridDim.x * blockDim.x)
-    idx[j] = categorical_row(lw, n, k, S, j);
+  const int lane = threadIdx.x & (kWave - 1);
+  const int64_t waves = (int64_t)gridDim.x * (blockDim.x / kWave);
+  for (int64_t j = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; j < n_out; j += waves) {
+    float best = -INFINITY;
+    int64_t bi = -1;
+    for (int64_t i = lane; i < n; i += kWave) {
+      const float u = uniform_from_bits(bits_at(k, S, (uint32_t)(j * n + i)), 1.17549435e-38f, 1.f);
+      const float g = -logf(-logf(u)) + lw[i];
+      if (bi < 0 || g > best) {
+        best = g;
+        bi = i;
+      }
+    }
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) {
+      const float ob = __shfl_xor(best, o, kWave);
+      const int64_t oi = (int64_t)__shfl_xor((long long)bi, o, kWave);
+      if (oi >= 0 && (bi < 0 || ob > best || (ob == best && oi < bi))) {
+        best = ob;
+        bi = oi;
+      }
+    }
+    if (lane == 0) idx[j] = (int32_t)bi;
+  }
 }
 
 // ---------------------------------------------------------------- large filters: grid-wide launches per step
@@ -1459,7 +1484,7 @@ int mocat_resample_categorical(const float* log_weight, int64_t n, int64_t n_out
                                hipStream_t stream) {
   if (!log_weight || !key || !idx || n <= 0 || n_out <= 0) return MOCAT_ERR_ARG;
   if (n * n_out >= 0xFFFFFFFFll) return MOCAT_ERR_SIZE;
-  const int64_t blocks = (n_out + 255) / 256;
+  const int64_t blocks = (n_out + 3) / 4;  // one wave per output, four per workgroup
   hipLaunchKernelGGL(categorical_kernel, dim3((unsigned)(blocks < 65535 ? blocks : 65535)), dim3(256), 0, stream,
                      log_weight, n, n_out, key, idx);
   MOCAT_LAUNCH_CHECK();

@@ -2,3 +2,4 @@
 from .models import StateSpaceModel, TimeHomogenousLinearGaussian, StochasticVolatility
 from .filtering import (ParticleFilter, BootstrapFilter, run_particle_filter_for_marginals, run_particle_filters,
                         resample_indices, categorical, run_kalman_filter_for_marginals)
+from .backward import backward_simulation, forward_filtering_backward_simulation

@@ -94,6 +94,14 @@ class TimeHomogenousLinearGaussian(StateSpaceModel):
                                self.likelihood_matrix.ravel(), self.likelihood_precision_sqrt.ravel(),
                                [nlz]]).astype(np.float32)
 
+    def transition_params(self):
+        """Backward-simulation parameters: A, the transition precision square root inv(cholesky(Q))
+        and the normaliser (d log 2 pi - log det Q^-1) / 2 of mocat's gaussian_potential
+        (linear_gaussian.py:73-84, utils.py:49-105)."""
+        S = np.linalg.inv(self.transition_covariance_sqrt)
+        nlz = (self.dim * LOG2PI - np.log(1.0 / np.linalg.det(self.transition_covariance))) / 2
+        return np.concatenate([self.transition_matrix.ravel(), S.ravel(), [nlz]]).astype(np.float32)
+
     def _init_from_eps(self, e):
         return self.initial_covariance_sqrt @ e + self.initial_mean
 
@@ -128,6 +136,11 @@ class StochasticVolatility(StateSpaceModel):
     def kernel_params(self):
         return np.array([self.initial_mean, np.sqrt(self.initial_covariance), self.phi, self.sigma,
                          1.0 / self.beta], np.float32)
+
+    def transition_params(self):
+        """Backward-simulation parameters for unit steps: phi, 1/sigma and the normaliser
+        (log 2 pi - log sigma^-2) / 2 (stochastic_volatility.py:78-84, utils.py:49-105)."""
+        return np.array([self.phi, 1.0 / self.sigma, (LOG2PI + 2.0 * np.log(self.sigma)) / 2], np.float32)
 
     def _init_from_eps(self, e):
         return np.sqrt(self.initial_covariance) * e + self.initial_mean

@@ -1,0 +1,68 @@
+"""Forward-filtering backward-simulation, full resampling (mocat/src/ssm/backward.py:20-39, 210-243),
+restated in FP64 NumPy (test infrastructure only).
+
+  t_keys = split(key, T); x_{T-1} = particles_{T-1}[categorical(t_keys[T-1], lw_{T-1}, shape=(M,))]
+  for t = T-2 .. 0, trajectory m: key_m = split(t_keys[t], M)[m],
+      x_t[m] = particles_t[argmax_j(gumbel(key_m, (N,))[j] + lw_t[j] - U(particles_t[j] -> x_{t+1}[m]))]
+U is mocat's transition potential through gaussian_potential (utils.py:49-105):
+  linear Gaussian  0.5 |(x' - A x) @ inv(chol(Q))|^2 + (d log 2 pi + log det Q) / 2  (linear_gaussian.py:73-84)
+  stochastic vol.  0.5 (x' - phi x)^2 / sigma^2 + (log 2 pi + 2 log sigma) / 2, unit steps (:78-84)
+Pinned by the categorical frequencies of one backward step (the weights lw_j - U_j, a sign error
+flips them) and, for a linear-Gaussian model, by the RTS smoother means the trajectories converge
+to (tests/test_oracle_ffbsi.py); exact indices near f32 ties of the GPU's Gumbel-max are
+"parity unpinned", as for mocat's categorical.
+"""
+import numpy as np
+
+from . import rng
+
+LOG2PI = np.log(2 * np.pi)
+
+
+def lgssm_transition(A, Q):
+    A = np.asarray(A, np.float64)
+    Q = np.asarray(Q, np.float64)
+    S = np.linalg.inv(np.linalg.cholesky(Q))
+    nlz = (A.shape[0] * LOG2PI + np.log(np.linalg.det(Q))) / 2
+
+    def pot(xp, xn):  # xp (N, d), xn (d,) -> (N,)
+        diff = xn[None, :] - xp @ A.T
+        return 0.5 * np.sum((diff @ S) ** 2, -1) + nlz
+    return pot
+
+
+def sv_transition(phi, sigma):
+    nlz = (LOG2PI + 2 * np.log(sigma)) / 2
+
+    def pot(xp, xn):
+        return 0.5 * (xn[0] - phi * xp[:, 0]) ** 2 / sigma ** 2 + nlz
+    return pot
+
+
+def backward_simulation_full(pot, vals, log_weight, key, n_samps):
+    """Returns (trajectories (T, M, d), indices (T, M))."""
+    vals = np.asarray(vals, np.float64)
+    lw = np.asarray(log_weight, np.float64)
+    T, n, d = vals.shape
+    tk = rng.split(np.asarray(key, np.uint32), T)
+    idx = np.zeros((T, n_samps), np.int64)
+    idx[T - 1] = rng.categorical(tk[T - 1], lw[T - 1], n_samps)
+    x = vals[T - 1][idx[T - 1]]
+    out = np.zeros((T, n_samps, d))
+    out[T - 1] = x
+    for t in range(T - 2, -1, -1):
+        idx[t] = backward_step(pot, vals[t], lw[t], x, tk[t])
+        x = vals[t][idx[t]]
+        out[t] = x
+    return out, idx
+
+
+def backward_step(pot, vals_t, lw_t, x_next, key_t):
+    """Indices of one backward step for the trajectories x_next (M, d) with key_t = t_keys[t]."""
+    n = len(lw_t)
+    keys = rng.split(np.asarray(key_t, np.uint32), len(x_next))
+    out = np.zeros(len(x_next), np.int64)
+    for m in range(len(x_next)):
+        logits = np.asarray(lw_t, np.float64) - pot(np.asarray(vals_t, np.float64), np.asarray(x_next[m], np.float64))
+        out[m] = int(np.argmax(rng.gumbel(keys[m], (n,)) + logits))
+    return out

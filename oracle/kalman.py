@@ -57,3 +57,45 @@ def joint_loglik_bruteforce(m0, P0, A, Q, H, R, y):
     r = y.ravel() - mu
     _, logdet = np.linalg.slogdet(big)
     return -0.5 * (r @ np.linalg.solve(big, r) + logdet + len(r) * np.log(2 * np.pi))
+
+
+def smoother(m0, P0, A, Q, H, R, y):
+    """Rauch-Tung-Striebel smoothed means and covariances of x_t given y_0:T (FP64): the target
+    that backward simulation samples, in the limit of many particles."""
+    A, Q = (np.asarray(v, np.float64) for v in (A, Q))
+    _, mf, Pf = loglik(m0, P0, A, Q, H, R, y)
+    T = len(mf)
+    ms, Ps = mf.copy(), Pf.copy()
+    for t in range(T - 2, -1, -1):
+        Pp = A @ Pf[t] @ A.T + Q
+        G = Pf[t] @ A.T @ np.linalg.inv(Pp)
+        ms[t] = mf[t] + G @ (ms[t + 1] - A @ mf[t])
+        Ps[t] = Pf[t] + G @ (Ps[t + 1] - Pp) @ G.T
+    return ms, Ps
+
+
+def smoother_bruteforce(m0, P0, A, Q, H, R, y):
+    """E[x_t | y_0:T] and Cov from the joint Gaussian of (x_0:T, y_0:T) conditioned on y (small T)."""
+    A, Q, H, R = (np.asarray(v, np.float64) for v in (A, Q, H, R))
+    y = np.asarray(y, np.float64)
+    T, dy = y.shape
+    d = len(m0)
+    mx = [np.asarray(m0, np.float64)]
+    cx = [np.asarray(P0, np.float64)]
+    for t in range(1, T):
+        mx.append(A @ mx[-1])
+        cx.append(A @ cx[-1] @ A.T + Q)
+    Sxx = np.zeros((T * d, T * d))
+    for s in range(T):
+        for t in range(s, T):
+            c = np.linalg.matrix_power(A, t - s) @ cx[s]  # Cov(x_t, x_s)
+            Sxx[t * d:(t + 1) * d, s * d:(s + 1) * d] = c
+            Sxx[s * d:(s + 1) * d, t * d:(t + 1) * d] = c.T
+    Hb = np.kron(np.eye(T), H)
+    Syy = Hb @ Sxx @ Hb.T + np.kron(np.eye(T), R)
+    Sxy = Sxx @ Hb.T
+    mux = np.concatenate(mx)
+    K = Sxy @ np.linalg.inv(Syy)
+    mean = mux + K @ (y.ravel() - Hb @ mux)
+    cov = Sxx - K @ Sxy.T
+    return mean.reshape(T, d), np.stack([cov[t * d:(t + 1) * d, t * d:(t + 1) * d] for t in range(T)])
