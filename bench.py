@@ -286,10 +286,15 @@ def bench_ffbsi(n=4096, T=100):
     run = lambda: m.ssm.backward_simulation(model, marg, m.random.PRNGKey(1), n)
     run()
     res, dt = _timed(run)
+    runr = lambda: m.ssm.backward_simulation(model, marg, m.random.PRNGKey(1), n, maximum_rejections=32)
+    runr()
+    rr, dtr = _timed(runr)
     pairs = (T - 1) * n * n
     print(json.dumps({"mode": "ffbsi", "metric": "pair evaluations/s", "value": pairs / dt, "ms": dt * 1e3,
                       "particles": n, "trajectories": n, "T": T, "ms_per_backward_step": dt * 1e3 / (T - 1),
-                      "note": "per pair: one transition potential (d = 4) and half a threefry block"}))
+                      "note": "per pair: one transition potential (d = 4) and half a threefry block",
+                      "rejection_32": {"ms": dtr * 1e3, "ms_per_backward_step": dtr * 1e3 / (T - 1),
+                                       "density_evals_per_step": float(rr.num_transition_evals[1:].mean().item())}}))
 
 
 def main():

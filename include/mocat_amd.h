@@ -159,6 +159,18 @@ int mocat_resample_categorical(const float* log_weight, int64_t n, int64_t n_out
 int mocat_ffbsi_step(int model, int d, const float* params, const float* particles, const float* log_weight,
                      int64_t n, const float* x_next, int64_t m, const uint32_t* key, int32_t* idx, float* x_out,
                      hipStream_t stream);
+/* ---- Backward simulation by rejection (replaces mocat/src/ssm/backward.py:42-194, one step of the
+ * scan in backward_simulation_rejection, :197-207): initial categorical proposals, accept/reject
+ * rounds against a shared density bound (raised by the inflation factor when exceeded, which sends
+ * every trajectory back to proposing), at most max_rejections - 1 rounds, then full resampling for
+ * the trajectories still waiting. evals (one int64 on the device) receives the step's number of
+ * transition-density evaluations. Models and params as mocat_ffbsi_step; workspace:
+ * mocat_ffbsi_rejection_workspace_bytes(m). */
+int mocat_ffbsi_rejection_workspace_bytes(int64_t m, int64_t* bytes);
+int mocat_ffbsi_rejection_step(int model, int d, const float* params, const float* particles, const float* log_weight,
+                               int64_t n, const float* x_next, int64_t m, const uint32_t* key, int max_rejections,
+                               float init_bound, float bound_inflation, void* workspace, int64_t workspace_bytes,
+                               int32_t* idx, float* x_out, long long* evals, hipStream_t stream);
 
 /* -------------------------------------------------------------------------------- tempered SMC
  * dst[j][:] = src[idx[j]][:] for 4-byte elements, width per row: ensemble_state[resampled_indices]

@@ -1,9 +1,10 @@
 """Backward simulation for state-space models (mocat/src/ssm/backward.py): forward filtering then
 full backward resampling of whole trajectories on the GPU.
 
-Every backward step is one launch of the FFBSi kernel (one wave per trajectory, Gumbel-max over
-the N filter particles with the same key tree as mocat: t_keys = split(key, T), then
-split(t_keys[t], M) per trajectory), so the step costs N x M transition potentials.
+Full backward resampling is one launch of the FFBSi kernel per step (one wave per trajectory,
+Gumbel-max over the N filter particles with the same key tree as mocat: t_keys = split(key, T),
+then split(t_keys[t], M) per trajectory), N x M transition potentials per step. Rejection sampling
+(maximum_rejections > 0) runs mocat's accept/reject rounds on the GPU, two launches per round.
 """
 from __future__ import annotations
 
@@ -21,18 +22,20 @@ _P = _lib.c_ptr
 _lib.register("mocat_ffbsi_step", [_lib.c_i32, _lib.c_i32, _P, _P, _P, _lib.c_i64, _P, _lib.c_i64, _P, _P, _P, _P])
 _lib.register("mocat_resample_categorical", [_P, _lib.c_i64, _lib.c_i64, _P, _P, _P])
 _lib.register("mocat_gather_rows", [_P, _P, _lib.c_i64, _lib.c_i64, _P, _P])
+_lib.register("mocat_ffbsi_rejection_workspace_bytes", [_lib.c_i64, _P])
+_lib.register("mocat_ffbsi_rejection_step", [_lib.c_i32, _lib.c_i32, _P, _P, _P, _lib.c_i64, _P, _lib.c_i64, _P,
+                                             _lib.c_i32, _lib.c_f32, _lib.c_f32, _P, _lib.c_i64, _P, _P, _P, _P])
 
 
 def backward_simulation(ssm_scenario, marginal_particles: cdict, random_key, n_samps: int = None,
                         maximum_rejections: int = 0, transition_dens_bound_parameter: float = 0.,
                         bound_inflation: float = 1.01) -> cdict:
-    """mocat/src/ssm/backward.py:246-270 with maximum_rejections = 0 (backward_simulation_full,
-    :210-243). Returns a copy of marginal_particles without log_weight, with value (T, n_samps, d)
-    the backward trajectories, num_transition_evals (T,) and ancestors (T, n_samps): the filter
+    """mocat/src/ssm/backward.py:246-270: full backward resampling (maximum_rejections = 0,
+    backward_simulation_full :210-243) or rejection sampling against an adaptive density bound
+    (backward_simulation_rejection :197-207). Returns a copy of marginal_particles without
+    log_weight, with value (T, n_samps, d) the backward trajectories, num_transition_evals (T,)
+    (mocat's layout: a leading 0, then the steps from t = 0) and ancestors (T, n_samps): the filter
     particle each trajectory takes at each time."""
-    if maximum_rejections > 0:
-        raise NotImplementedError("rejection-sampled backward simulation (maximum_rejections > 0) is not built; "
-                                  "full backward simulation is exact for the same particle approximation")
     model = getattr(ssm_scenario, "kernel_model", None)
     if model is None or not hasattr(ssm_scenario, "transition_params"):
         raise NotImplementedError("backward simulation is implemented for the GPU state-space models")
@@ -56,14 +59,27 @@ def backward_simulation(ssm_scenario, marginal_particles: cdict, random_key, n_s
     _lib.call("mocat_resample_categorical", _lib.ptr(lw[T - 1]), n, m, _lib.ptr(keys[T - 1]), _lib.ptr(anc[T - 1]),
               st)
     _lib.call("mocat_gather_rows", _lib.ptr(vals[T - 1]), _lib.ptr(anc[T - 1]), m, d, _lib.ptr(out[T - 1]), st)
-    for ind in range(T - 2, -1, -1):
-        _lib.call("mocat_ffbsi_step", model, d, _lib.ptr(params), _lib.ptr(vals[ind]), _lib.ptr(lw[ind]), n,
-                  _lib.ptr(out[ind + 1]), m, _lib.ptr(keys[ind]), _lib.ptr(anc[ind]), _lib.ptr(out[ind]), st)
+    if maximum_rejections > 0:
+        nb = _lib.ctypes.c_int64(0)
+        _lib.call("mocat_ffbsi_rejection_workspace_bytes", m, _lib.ctypes.byref(nb))
+        ws = torch.empty(int(nb.value), dtype=torch.uint8, device="cuda")
+        evals = torch.zeros(T, dtype=torch.int64, device="cuda")
+        for ind in range(T - 2, -1, -1):
+            _lib.call("mocat_ffbsi_rejection_step", model, d, _lib.ptr(params), _lib.ptr(vals[ind]),
+                      _lib.ptr(lw[ind]), n, _lib.ptr(out[ind + 1]), m, _lib.ptr(keys[ind]), int(maximum_rejections),
+                      float(transition_dens_bound_parameter), float(bound_inflation), _lib.ptr(ws), int(ws.numel()),
+                      _lib.ptr(anc[ind]), _lib.ptr(out[ind]), _lib.ptr(evals[ind + 1]), st)
+        num_evals = evals.cpu().to(torch.float64)  # [0, steps t = 0 .. T-2] (backward.py:205)
+    else:
+        for ind in range(T - 2, -1, -1):
+            _lib.call("mocat_ffbsi_step", model, d, _lib.ptr(params), _lib.ptr(vals[ind]), _lib.ptr(lw[ind]), n,
+                      _lib.ptr(out[ind + 1]), m, _lib.ptr(keys[ind]), _lib.ptr(anc[ind]), _lib.ptr(out[ind]), st)
+        num_evals = torch.cat([torch.zeros(1, dtype=torch.float64),
+                               torch.full((T - 1,), float(n) * m, dtype=torch.float64)])
     res = cdict(**{k: v for k, v in marginal_particles.__dict__.items() if k != "log_weight"})
     res.value = out
     res.ancestors = anc
-    res.num_transition_evals = torch.cat([torch.zeros(1, dtype=torch.float64),
-                                          torch.full((T - 1,), float(n) * m, dtype=torch.float64)])
+    res.num_transition_evals = num_evals
     return res
 
 

@@ -66,3 +66,48 @@ def backward_step(pot, vals_t, lw_t, x_next, key_t):
         logits = np.asarray(lw_t, np.float64) - pot(np.asarray(vals_t, np.float64), np.asarray(x_next[m], np.float64))
         out[m] = int(np.argmax(rng.gumbel(keys[m], (n,)) + logits))
     return out
+
+
+def rejection_step(pot, vals_t, lw_t, x_next, key_t, maximum_rejections, init_bound, inflation):
+    """rejection_resampling (backward.py:130-194) for one backward step: returns (indices (M,),
+    transition-density evaluations). Densities, the bound and the acceptance tests in float32 as
+    mocat computes them; categorical draws by Gumbel-max in FP64."""
+    F = np.float32
+    vals_t = np.asarray(vals_t, np.float64)
+    lw_t = np.asarray(lw_t, np.float64)
+    M, n = len(x_next), len(lw_t)
+    k0, k1, k2 = rng.split(np.asarray(key_t, np.uint32), 3)
+    dens_of = lambda i, m: F(np.exp(-pot(vals_t[[i]], np.asarray(x_next[m], np.float64))[0]))
+    idx = rng.categorical(k0, lw_t, M).astype(np.int64)
+    dens = np.array([dens_of(idx[m], m) for m in range(M)], F)
+    mx = dens.max()
+    bound = F(mx * F(inflation)) if mx > F(init_bound) else F(init_bound)
+    wait = rng.uniform(k1, (M,)) > (dens / bound).astype(F)
+    keys = rng.split(k2, M)
+    evals = M
+    it = 1
+    while wait.sum() > 0 and it < maximum_rejections:
+        entered = wait.copy()
+        d_round = np.zeros(M, F)
+        new_wait = np.zeros(M, bool)
+        for m in range(M):
+            if not entered[m]:
+                continue
+            kr, kc, ku = rng.split(keys[m], 3)
+            idx[m] = int(np.argmax(rng.gumbel(kc, (n,)) + lw_t))
+            d_round[m] = dens_of(idx[m], m)
+            new_wait[m] = rng.uniform(ku, ()) > F(d_round[m] / bound)
+            keys[m] = kr
+        mx = d_round.max()
+        if mx > bound:
+            bound = F(mx * F(inflation))
+            new_wait[:] = True
+        wait = new_wait
+        evals += int(entered.sum())
+        it += 1
+    for m in range(M):
+        if wait[m]:
+            logits = lw_t - pot(vals_t, np.asarray(x_next[m], np.float64))
+            idx[m] = int(np.argmax(rng.gumbel(keys[m], (n,)) + logits))
+    evals += n * int(wait.sum())
+    return idx, evals

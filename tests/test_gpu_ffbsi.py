@@ -80,7 +80,61 @@ def test_lgssm_trajectories_converge_to_rts_smoother(m):
     assert res.time >= res.pf_time > 0 and res.bsi_time > 0
 
 
-def test_ffbsi_key_tree_and_rejection_refused(m):
+def _rejection_agreement(m, model, pot, n, n_samps, T, max_rej, bound, infl, seed):
+    sim = model.simulate(torch.arange(float(T)), m.random.PRNGKey(seed))
+    marg = m.ssm.run_particle_filter_for_marginals(model, m.ssm.BootstrapFilter(), sim.y, sim.t,
+                                                   m.random.PRNGKey(seed + 1), n=n, ess_threshold=0.5)
+    key = m.random.PRNGKey(seed + 2)
+    res = m.ssm.backward_simulation(model, marg, key, n_samps, maximum_rejections=max_rej,
+                                    transition_dens_bound_parameter=bound, bound_inflation=infl)
+    vals = marg.value.cpu().numpy().astype(np.float64)
+    lw = marg.log_weight.cpu().numpy().astype(np.float64)
+    out, anc = res.value.cpu().numpy(), res.ancestors.cpu().numpy()
+    tk = orng.split(key.cpu().numpy(), T)
+    agree, ev = [], []
+    for t in range(T - 2, -1, -1):
+        idx, evals = of.rejection_step(pot, vals[t], lw[t], out[t + 1], tk[t], max_rej, bound, infl)
+        agree.append(np.mean(anc[t] == idx))
+        ev.append((res.num_transition_evals[t + 1].item(), evals))
+    assert res.num_transition_evals[0].item() == 0
+    np.testing.assert_array_equal(out, np.take_along_axis(vals, anc[:, :, None].astype(np.int64), 1).astype(np.float32))
+    return np.array(agree), np.array(ev)
+
+
+@pytest.mark.parametrize("max_rej,bound", [(1, 0.0), (10, 0.0), (40, 5.0)])
+def test_rejection_backward_steps_vs_oracle(m, max_rej, bound):
+    """Rejection rounds (backward.py:42-194): the same proposals, acceptances, bound raises and
+    final full draws as the oracle on the same keys, every step; the evaluation counts follow."""
+    model, A, Q, H, R = _lgssm(m, 2, 21)
+    agree, ev = _rejection_agreement(m, model, of.lgssm_transition(A, Q), 300, 200, 6, max_rej, bound, 1.01, 30)
+    assert agree.min() >= 0.98, agree
+    assert np.all(np.abs(ev[:, 0] - ev[:, 1]) <= 0.02 * ev[:, 1] + 300), ev
+
+
+def test_rejection_sv_vs_oracle(m):
+    model = m.ssm.StochasticVolatility(transition_matrix=0.9, transition_covariance=0.4,
+                                       likelihood_covariance_diag=0.7)
+    agree, ev = _rejection_agreement(m, model, of.sv_transition(0.9, np.sqrt(0.4)), 400, 256, 6, 12, 0.0, 1.05, 40)
+    assert agree.min() >= 0.98, agree
+
+
+def test_rejection_trajectories_converge_to_rts_smoother(m):
+    d, T = 2, 15
+    model, A, Q, H, R = _lgssm(m, d, 7)
+    sim = model.simulate(torch.arange(float(T)), m.random.PRNGKey(8))
+    res = m.ssm.forward_filtering_backward_simulation(model, m.ssm.BootstrapFilter(), sim.y, sim.t, 2048,
+                                                      m.random.PRNGKey(10), ess_threshold=0.5, maximum_rejections=20)
+    ms, Ps = ok.smoother(np.zeros(d), np.eye(d), A, Q, H, R, sim.y.cpu().numpy())
+    x = res.value.cpu().numpy().astype(np.float64)
+    sd = np.sqrt(np.stack([np.diag(P) for P in Ps]))
+    assert (np.abs(x.mean(1) - ms) / sd).max() < 0.25
+    ratio = x.std(1) / sd
+    assert np.all((ratio > 0.75) & (ratio < 1.25)), ratio
+    # rejection needs far fewer density evaluations than the N x M of full resampling
+    assert res.num_transition_evals[1:].mean().item() < 0.2 * 2048 * 2048
+
+
+def test_ffbsi_key_tree(m):
     model, *_ = _lgssm(m, 2, 11)
     sim = model.simulate(torch.arange(6.0), m.random.PRNGKey(1))
     key = m.random.PRNGKey(12)
@@ -89,5 +143,3 @@ def test_ffbsi_key_tree_and_rejection_refused(m):
     marg = m.ssm.run_particle_filter_for_marginals(model, m.ssm.BootstrapFilter(), sim.y, sim.t, k[0], n=80)
     again = m.ssm.backward_simulation(model, marg, k[1], 50)
     np.testing.assert_array_equal(res.value.cpu().numpy(), again.value.cpu().numpy())
-    with pytest.raises(NotImplementedError):
-        m.ssm.backward_simulation(model, marg, k[1], 50, maximum_rejections=3)

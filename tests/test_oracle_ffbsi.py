@@ -60,3 +60,27 @@ def test_backward_trajectories_converge_to_the_rts_smoother():
     assert err.max() < 0.3, err
     ratio = traj[:, :, 0].std(1) / sd
     assert np.all((ratio > 0.7) & (ratio < 1.3)), ratio
+
+
+def test_rejection_step_draws_the_backward_kernel_weights():
+    """With the bound at the transition density's maximum (exp(-normaliser)) every accepted proposal
+    is an exact draw: one step's index frequencies over many keys equal w_j exp(-U_j) normalised;
+    with a single round the rest are full draws, the same distribution."""
+    A, Q = np.array([[0.8]]), np.array([[0.5]])
+    pot = of.lgssm_transition(A, Q)
+    vals = np.array([[-1.0], [0.0], [0.4], [1.5], [2.0]])
+    lw = np.array([0.2, -0.5, 0.0, 0.7, -1.0])
+    x1 = np.array([[0.3]])
+    logits = lw - pot(vals, x1[0])
+    p = np.exp(logits - logits.max())
+    p /= p.sum()
+    top = float(np.exp(-(np.log(2 * np.pi) + np.log(0.5)) / 2))
+    K = 2000
+    for rounds in (30, 1):
+        counts = np.zeros(5)
+        for s in range(K):
+            idx, evals = of.rejection_step(pot, vals, lw, x1, rng.PRNGKey(s), rounds, top, 1.01)
+            counts[idx[0]] += 1
+            assert evals >= 1
+        sd = np.sqrt(K * p * (1 - p))
+        assert np.all(np.abs(counts - K * p) < 4.5 * sd + 1), (rounds, counts, K * p)
