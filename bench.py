@@ -300,8 +300,8 @@ def bench_ffbsi(n=4096, T=100):
 def main():
     a = parse()
     if a.gpus != 1:
-        print("bench.py measures one GPU only: multi-GPU runs are not part of this build "
-              "(chains shard trivially across GPUs; see DESIGN.md)", file=sys.stderr)
+        print("bench.py measures one GPU only (independent chains and filters shard trivially across GPUs)",
+              file=sys.stderr)
         sys.exit(2)
     import numpy as np
     import torch

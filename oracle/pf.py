@@ -13,7 +13,7 @@ Models:
 The log-likelihood (absent in mocat) is log Z = sum_t [lse(lw_t) - lse(lw_res_{t-1})] (the SMC
 log-normaliser recursion of mocat/src/transport/smc.py:212-215), t = 0 term lse(lw_0) - log n.
 
-Resampling from given non-negative f32 weights (new in this build; mocat only has the Gumbel-max
+Resampling from given non-negative f32 weights (an addition to mocat, which only has the Gumbel-max
 categorical): integer weights q_i = floor(w_i * 2^s) with s = 52 - bitlen(n) - exponent(max w)
 (exact scaling), C = cumsum(q) exactly, target T_j: systematic (j + u) * (W / n), multinomial
 u_j * W (FP64, IEEE), ancestor = first i with C_i > T_j. Pinned by brute-force counting on small
