@@ -26,64 +26,58 @@ __global__ __launch_bounds__(256) void gather_kernel(const uint32_t* __restrict_
   }
 }
 
-struct Red {
-  double m, s1, s2;
-};
-
-__device__ Red red_combine(Red a, Red b) {
-  Red r;
-  r.m = fmax(a.m, b.m);
-  if (r.m == -INFINITY) return Red{-INFINITY, 0.0, 0.0};
-  const double fa = a.m == -INFINITY ? 0.0 : exp(a.m - r.m);
-  const double fb = b.m == -INFINITY ? 0.0 : exp(b.m - r.m);
-  r.s1 = a.s1 * fa + b.s1 * fb;
-  r.s2 = a.s2 * fa * fa + b.s2 * fb * fb;
-  return r;
-}
-
 struct Smem {
   double m[NW], s1[NW], s2[NW];
   double bc[8];
 };
 
-// (lse(w), lse(2w)) of w_i = lw_i - db * lik_i over the block, FP64.
+// (lse(w), lse(2w)) of w_i = lw_i - db * lik_i over the block, FP64, in two passes: the maximum m,
+// then sum exp(w_i - m) and its square (one exponential per element); partial sums are combined in a
+// fixed order (lanes by butterfly, waves in index order).
 __device__ void lse_pair(const float* __restrict__ lw, const float* __restrict__ lik, int64_t n, double db, Smem& s,
                          double& lse, double& lse2) {
   const int t = threadIdx.x, lane = t & (kWave - 1), w = t / kWave;
-  Red r{-INFINITY, 0.0, 0.0};
+  double mx = -INFINITY;
   int has_nan = 0;
   for (int64_t i = t; i < n; i += NT) {
     const double v = (double)lw[i] - db * (double)lik[i];
-    if (isnan(v)) {
-      has_nan = 1;
-      continue;
-    }
-    r = red_combine(r, Red{v, 1.0, 1.0});
+    if (isnan(v)) has_nan = 1;
+    else mx = fmax(mx, v);
   }
   const bool nan_seen = __syncthreads_or(has_nan) != 0;
-  for (int o = kWave / 2; o > 0; o >>= 1) {
-    Red b{__shfl_xor(r.m, o, kWave), __shfl_xor(r.s1, o, kWave), __shfl_xor(r.s2, o, kWave)};
-    r = red_combine(r, b);
+  mx = wave_max(mx);
+  if (lane == 0) s.m[w] = mx;
+  __syncthreads();
+  double m = s.m[0];
+  for (int i = 1; i < NW; ++i) m = fmax(m, s.m[i]);
+  if (nan_seen || m == -INFINITY) {
+    __syncthreads();  // every thread has read s.m before a later call rewrites it
+    lse = nan_seen ? NAN : -INFINITY;
+    lse2 = lse;
+    return;
   }
+  double s1 = 0.0, s2 = 0.0;
+  for (int64_t i = t; i < n; i += NT) {
+    const double e = exp((double)lw[i] - db * (double)lik[i] - m);
+    s1 += e;
+    s2 += e * e;
+  }
+  s1 = wave_sum(s1);
+  s2 = wave_sum(s2);
   __syncthreads();
   if (lane == 0) {
-    s.m[w] = r.m;
-    s.s1[w] = r.s1;
-    s.s2[w] = r.s2;
+    s.s1[w] = s1;
+    s.s2[w] = s2;
   }
   __syncthreads();
-  Red a{s.m[0], s.s1[0], s.s2[0]};
-  for (int i = 1; i < NW; ++i) a = red_combine(a, Red{s.m[i], s.s1[i], s.s2[i]});
-  if (nan_seen) {
-    lse = NAN;
-    lse2 = NAN;
-  } else if (a.m == -INFINITY) {
-    lse = -INFINITY;
-    lse2 = -INFINITY;
-  } else {
-    lse = a.m + log(a.s1);
-    lse2 = 2.0 * a.m + log(a.s2);
+  double a1 = 0.0, a2 = 0.0;
+  for (int i = 0; i < NW; ++i) {
+    a1 += s.s1[i];
+    a2 += s.s2[i];
   }
+  __syncthreads();  // every thread has read the partials before a later call rewrites them
+  lse = m + log(a1);
+  lse2 = 2.0 * m + log(a2);
 }
 
 struct TemperArgs {
