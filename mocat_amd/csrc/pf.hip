@@ -1232,52 +1232,78 @@ __global__ __launch_bounds__(LT) void lf_step_kernel(LfArgs a, int64_t t, int ce
 #pragma unroll
     for (int q = 0; q < D; ++q) xpre[pk][q] = live[pk] ? xa[srcs[pk] * D + q] : 0.f;
 
+  // the PPT particles in lockstep, phase by phase, so that their independent threefry chains, loads
+  // and arithmetic interleave; the accumulators still take them in particle order
+  float eps[PPT][D], xn[PPT][D], lwp[PPT];
 #pragma unroll
   for (int pk = 0; pk < PPT; ++pk) {
     const int64_t i = base + pk * LT + tid;
-    if (i >= n) break;
-    const int64_t src = srcs[pk];
+    lwp[pk] = (live[pk] && !res) ? lwa[i] : 0.f;
+    if (!live[pk]) continue;
     const Key ki = split_at(ik, (uint32_t)n, (uint32_t)i);
-    float eps[D], xp[D], x[D];
 #pragma unroll
     for (uint32_t q = 0; q < hD; ++q) {
       uint32_t w0, w1;
       bits_pair(ki, D, q, w0, w1);
-      eps[q] = normal_from_bits(w0);
-      if (q + hD < (uint32_t)D) eps[q + hD] = normal_from_bits(w1);
+      eps[pk][q] = normal_from_bits(w0);
+      if (q + hD < (uint32_t)D) eps[pk][q + hD] = normal_from_bits(w1);
     }
+  }
+  float tg[PPT][5];
+  if constexpr (GRAD) {
 #pragma unroll
-    for (int q = 0; q < D; ++q) xp[q] = xpre[pk][q];
-    model.transition(xp, eps, x);
-    float l;
+    for (int pk = 0; pk < PPT; ++pk) {
+      const int64_t i = base + pk * LT + tid;
+      tg[pk][0] = live[pk] ? ta[srcs[pk] * 5 + 0] : 0.f;
+      tg[pk][1] = live[pk] ? ta[srcs[pk] * 5 + 1] : 0.f;
+#pragma unroll
+      for (int q = 2; q < 5; ++q) tg[pk][q] = (live[pk] && !res) ? ta[i * 5 + q] : 0.f;
+    }
+  }
+#pragma unroll
+  for (int pk = 0; pk < PPT; ++pk)
+    if (live[pk]) model.transition(xpre[pk], eps[pk], xn[pk]);
+  float l[PPT], gd[PPT][3];
+#pragma unroll
+  for (int pk = 0; pk < PPT; ++pk) {
+    if (!live[pk]) continue;
     if constexpr (GRAD) {
       float dldx, dldb;
-      l = (res ? 0.f : lwa[i]) - model.lik_grad(x, ycur, dldx, dldb);
-      const float dxp = xp[0] + model.phi * ta[src * 5 + 0];
-      const float dxs = eps[0] + model.phi * ta[src * 5 + 1];
-      const float d0 = (res ? 0.f : ta[i * 5 + 2]) - dldx * dxp;
-      const float d1 = (res ? 0.f : ta[i * 5 + 3]) - dldx * dxs;
-      const float d2 = (res ? 0.f : ta[i * 5 + 4]) - dldb;
-      tb[i * 5 + 0] = dxp;
-      tb[i * 5 + 1] = dxs;
-      tb[i * 5 + 2] = d0;
-      tb[i * 5 + 3] = d1;
-      tb[i * 5 + 4] = d2;
-      grad_add(gacc, l, d0, d1, d2);
+      l[pk] = lwp[pk] - model.lik_grad(xn[pk], ycur, dldx, dldb);
+      const float dxp = xpre[pk][0] + model.phi * tg[pk][0];
+      const float dxs = eps[pk][0] + model.phi * tg[pk][1];
+      gd[pk][0] = tg[pk][2] - dldx * dxp;
+      gd[pk][1] = tg[pk][3] - dldx * dxs;
+      gd[pk][2] = tg[pk][4] - dldb;
+      tg[pk][0] = dxp;
+      tg[pk][1] = dxs;
     } else {
-      l = (res ? 0.f : lwa[i]) - model.lik(x, ycur);
+      l[pk] = lwp[pk] - model.lik(xn[pk], ycur);
+    }
+  }
+#pragma unroll
+  for (int pk = 0; pk < PPT; ++pk) {
+    if (!live[pk]) break;
+    const int64_t i = base + pk * LT + tid;
+    if constexpr (GRAD) {
+      tb[i * 5 + 0] = tg[pk][0];
+      tb[i * 5 + 1] = tg[pk][1];
+      tb[i * 5 + 2] = gd[pk][0];
+      tb[i * 5 + 3] = gd[pk][1];
+      tb[i * 5 + 4] = gd[pk][2];
+      grad_add(gacc, l[pk], gd[pk][0], gd[pk][1], gd[pk][2]);
     }
 #pragma unroll
-    for (int q = 0; q < D; ++q) xb[i * D + q] = x[q];
-    lwb[i] = l;
-    lse_add(acc, l);
+    for (int q = 0; q < D; ++q) xb[i * D + q] = xn[pk][q];
+    lwb[i] = l[pk];
+    lse_add(acc, l[pk]);
     if (F.out_offset >= 0) {
       const int64_t o = F.out_offset + t * n + i;
       if (a.out_value)
 #pragma unroll
-        for (int q = 0; q < D; ++q) a.out_value[o * D + q] = x[q];
-      if (a.out_lw) a.out_lw[o] = l;
-      if (a.out_anc) a.out_anc[o] = (int32_t)src;
+        for (int q = 0; q < D; ++q) a.out_value[o * D + q] = xn[pk][q];
+      if (a.out_lw) a.out_lw[o] = l[pk];
+      if (a.out_anc) a.out_anc[o] = (int32_t)srcs[pk];
     }
   }
   lf_store_partials<GRAD>(a, c, acc, gacc, s);
