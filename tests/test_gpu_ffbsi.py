@@ -143,3 +143,25 @@ def test_ffbsi_key_tree(m):
     marg = m.ssm.run_particle_filter_for_marginals(model, m.ssm.BootstrapFilter(), sim.y, sim.t, k[0], n=80)
     again = m.ssm.backward_simulation(model, marg, k[1], 50)
     np.testing.assert_array_equal(res.value.cpu().numpy(), again.value.cpu().numpy())
+
+
+@pytest.mark.parametrize("T,n,n_samps", [(1, 50, 70), (2, 1, 5), (3, 64, 200)])
+def test_short_series_and_more_trajectories_than_particles(m, T, n, n_samps):
+    """T = 1 is the final draw alone; one particle forces every index; more trajectories than
+    particles draw with replacement."""
+    model, A, Q, H, R = _lgssm(m, 2, T + n)
+    sim = model.simulate(torch.arange(float(T)), m.random.PRNGKey(2))
+    marg = m.ssm.run_particle_filter_for_marginals(model, m.ssm.BootstrapFilter(), sim.y, sim.t, m.random.PRNGKey(3),
+                                                   n=n)
+    key = m.random.PRNGKey(4)
+    for rej in (0, 5):
+        res = m.ssm.backward_simulation(model, marg, key, n_samps, maximum_rejections=rej)
+        assert res.value.shape == (T, n_samps, 2)
+        anc = res.ancestors.cpu().numpy()
+        assert anc.min() >= 0 and anc.max() < n
+        if n == 1:
+            assert (anc == 0).all()
+    lw = marg.log_weight.cpu().numpy().astype(np.float64)
+    tk = orng.split(key.cpu().numpy(), T)
+    full = m.ssm.backward_simulation(model, marg, key, n_samps)
+    assert np.mean(full.ancestors.cpu().numpy()[T - 1] == orng.categorical(tk[T - 1], lw[T - 1], n_samps)) >= 0.99

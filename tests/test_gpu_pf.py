@@ -243,3 +243,29 @@ def test_chunked_path_staged_search_limits(m, resampling):
         np.testing.assert_array_equal(big[i].final_value.cpu().numpy(), one[i].final_value.cpu().numpy())
         a, b = big[i].log_likelihood.item(), one[i].log_likelihood.item()
         assert abs(a - b) <= 1e-10 * (1 + abs(b)), (i, a, b)
+
+
+@pytest.mark.parametrize("large_n", [None, 1])
+def test_single_observation_and_two_steps(m, large_n):
+    """T = 1 (initial weighting only) and T = 2 on both paths against the oracle: log Z_0 =
+    lse(lw_0) - log n, and one resampling step."""
+    from mocat_amd import datasets
+    model, y, t = datasets.lgssm_config(2, 2, 2)
+    p = _lgssm_np(model)
+    yn = y.cpu().numpy()
+    for T in (1, 2):
+        r = m.ssm.run_particle_filters([model], [yn[:T]], [m.random.PRNGKey(4)], [4096], 1.0, "systematic",
+                                       store=True, large_n=large_n)[0]
+        o = opf.run_pf(p, yn[:T], orng.PRNGKey(4), 4096, 1.0, "systematic",
+                       ancestors=r.ancestors.cpu().numpy(), resample_flags=np.arange(T) > 0)
+        assert r.ess.shape == (T,)
+        assert abs(r.log_likelihood.item() - o["log_likelihood"]) <= 1e-4 * abs(o["log_likelihood"]) + 1e-6
+        np.testing.assert_allclose(r.final_value.cpu().numpy(), o["value"][-1], rtol=1e-4, atol=1e-4)
+
+
+def test_invalid_sizes_raise(m):
+    from mocat_amd import datasets
+    from mocat_amd._lib import MocatKernelError
+    model, y, t = datasets.lgssm_config(0, 4, 5)
+    with pytest.raises((MocatKernelError, ValueError)):
+        m.ssm.run_particle_filters([model], [y.cpu().numpy()], [m.random.PRNGKey(0)], [0], 1.0, "systematic")
