@@ -92,7 +92,11 @@ int mocat_hmc_lr_iterate(const float* X, const float* y, const void* packed, int
  *   [m0(d), L0(dxd), A(dxd), LQ(dxd), H(dyxd), SR(dyxdy), nlz] with SR = inv(chol(R)) and
  *   nlz = (dy log 2pi - log det R^-1)/2 as mocat's reset_covariance (mocat/src/utils.py:247-262);
  * model 1: 1-D StochasticVolatility (stochastic_volatility.py:61-108), params
- *   [m0, sqrt(P0), phi, sigma, 1/beta], unit time steps.
+ *   [m0, sqrt(P0), phi, sigma, 1/beta], unit time steps;
+ * model 2: TimeHomogenousLinearGaussian under OptimalNonLinearGaussianParticleFilter
+ *   (mocat/src/ssm/nonlinear_gaussian.py:134-281), params [m0(d), H(dyxd), K0(dxdy), C0(dxd),
+ *   A(dxd), K(dxdy), S(dxd), WC(dyxdy), nlzw] as its startup (:154-184); initial weights zero;
+ *   every particle's noise from normal(split(key, n)[0], (n, d)); n*d < 2^32.
  * Key tree per filter: (key, sub) = split(key); particles from split(sub, n); step t uses
  * split(key, T-1)[t-1] for resampling and split(., n) for the transitions (filtering.py:264-303).
  * resampling: 0 systematic, 1 multinomial, 2 categorical (mocat's Gumbel-max, n*n < 2^32 - 1);

@@ -53,6 +53,7 @@ constexpr double kLog2PiD = 1.83787706640934548356;
 //   x0 = L0 @ eps + m0 ; x' = A @ x + LQ @ eps ; lik = 0.5 |(y - H x) @ SR|^2 + nlz
 template <int D, int DY>
 struct Lgssm {
+  static constexpr bool kOptimal = false;
   static constexpr int NPARAM = D + 3 * D * D + DY * D + DY * DY + 1;
   float m0[D], L0[D * D], A[D * D], LQ[D * D], H[DY * D], SR[DY * DY], nlz;
   MOCAT_DEV void load(const float* p) {
@@ -124,6 +125,7 @@ struct Lgssm {
 //   lik = gaussian_potential(y, 0, sqrt_prec = inv_beta / exp(x/2))
 //       = 0.5 (y s)^2 + 0.5 log 2pi - log s,  s = inv_beta * exp(-x/2)
 struct Sv {
+  static constexpr bool kOptimal = false;
   static constexpr int NPARAM = 5;
   float m0, L0, phi, sigma, inv_beta;
   MOCAT_DEV void load(const float* p) {
@@ -150,6 +152,97 @@ struct Sv {
     return 0.5f * s2 + 0.5f * (float)kLog2PiD - __logf(s);
   }
 };
+
+// Optimal proposal for the linear-Gaussian model: OptimalNonLinearGaussianParticleFilter
+// (mocat/src/ssm/nonlinear_gaussian.py:134-281) with transition_function(x) = A x
+// (linear_gaussian.py:112-116). Matrices factored on the host as its startup (:154-184):
+// params: m0[D], H[DY*D], K0[D*DY], C0[D*D], A[D*D], K[D*DY], S[D*D], WC[DY*DY], nlzw
+//   x0 = m0 + K0 (y0 - H m0) + eps @ C0, lw0 = 0                                   (:198-224)
+//   mx = A x ; r = y - H mx ; x' = mx + K r + S eps ; lw += -(0.5 |r @ WC|^2 + nlzw)  (:265-281)
+// All particles' noise comes from one key per step: normal(split(key, n)[0], (n, D)).
+template <int D, int DY>
+struct LgssmOpt {
+  static constexpr bool kOptimal = true;
+  static constexpr int NPARAM = D + 3 * DY * D + 3 * D * D + DY * DY + 1;
+  float m0[D], H[DY * D], K0[D * DY], C0[D * D], A[D * D], K[D * DY], S[D * D], WC[DY * DY], nlzw;
+  MOCAT_DEV void load(const float* p) {
+    auto take = [&](float* dst, int cnt) {
+      for (int i = 0; i < cnt; ++i) dst[i] = p[i];
+      p += cnt;
+    };
+    take(m0, D);
+    take(H, DY * D);
+    take(K0, D * DY);
+    take(C0, D * D);
+    take(A, D * D);
+    take(K, D * DY);
+    take(S, D * D);
+    take(WC, DY * DY);
+    nlzw = p[0];
+  }
+  MOCAT_DEV void init_opt(const float* eps, const float* y0, float* x) const {
+    float r[DY];
+#pragma unroll
+    for (int j = 0; j < DY; ++j) {
+      float hm = 0.f;
+#pragma unroll
+      for (int k = 0; k < D; ++k) hm += H[j * D + k] * m0[k];
+      r[j] = y0[j] - hm;
+    }
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+      float g = 0.f, c = 0.f;
+#pragma unroll
+      for (int j = 0; j < DY; ++j) g += K0[i * DY + j] * r[j];
+#pragma unroll
+      for (int k = 0; k < D; ++k) c += eps[k] * C0[k * D + i];
+      x[i] = (m0[i] + g) + c;
+    }
+  }
+  // returns the weight potential; x receives the proposed state
+  MOCAT_DEV float step_opt(const float* xp, const float* eps, const float* y, float* x) const {
+    float mx[D], r[DY];
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+      float a = 0.f;
+#pragma unroll
+      for (int k = 0; k < D; ++k) a += A[i * D + k] * xp[k];
+      mx[i] = a;
+    }
+#pragma unroll
+    for (int j = 0; j < DY; ++j) {
+      float hx = 0.f;
+#pragma unroll
+      for (int k = 0; k < D; ++k) hx += H[j * D + k] * mx[k];
+      r[j] = y[j] - hx;
+    }
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+      float g = 0.f, b = 0.f;
+#pragma unroll
+      for (int j = 0; j < DY; ++j) g += K[i * DY + j] * r[j];
+#pragma unroll
+      for (int k = 0; k < D; ++k) b += S[i * D + k] * eps[k];
+      x[i] = (mx[i] + g) + b;
+    }
+    float s = 0.f;
+#pragma unroll
+    for (int k = 0; k < DY; ++k) {
+      float v = 0.f;
+#pragma unroll
+      for (int j = 0; j < DY; ++j) v += r[j] * WC[j * DY + k];
+      s += v * v;
+    }
+    return 0.5f * s + nlzw;
+  }
+};
+
+// Row `row` of normal(k0, (n, D)) (S = n * D elements in jax's halves layout).
+template <int D>
+MOCAT_DEV void shared_eps(Key k0, uint32_t S, uint32_t row, float* eps) {
+#pragma unroll
+  for (int q = 0; q < D; ++q) eps[q] = normal_from_bits(bits_at(k0, S, row * (uint32_t)D + (uint32_t)q));
+}
 
 // ---------------------------------------------------------------- block helpers
 struct LseAcc {
@@ -464,9 +557,16 @@ __global__ __launch_bounds__(NT) void pf_kernel(PfArgs a) {
 
     // ---- t = 0: initiate_particles
     LseAcc acc{-INFINITY, 0.0, 0.0};
+    const Key sub0 = Model::kOptimal ? split_at(sub, (uint32_t)n, 0u) : Key{0u, 0u};
     for (int64_t i = tid; i < n; i += NT) {
-      const Key ki = split_at(sub, (uint32_t)n, (uint32_t)i);
       float eps[D], x[D];
+      float l;
+      if constexpr (Model::kOptimal) {
+        shared_eps<D>(sub0, (uint32_t)(n * D), (uint32_t)i, eps);
+        model.init_opt(eps, yf, x);
+        l = 0.f;
+      } else {
+      const Key ki = split_at(sub, (uint32_t)n, (uint32_t)i);
       #pragma unroll
       for (uint32_t q = 0; q < hD; ++q) {
         uint32_t w0, w1;
@@ -475,7 +575,6 @@ __global__ __launch_bounds__(NT) void pf_kernel(PfArgs a) {
         if (q + hD < (uint32_t)D) eps[q + hD] = normal_from_bits(w1);
       }
       model.init(eps, x);
-      float l;
       if constexpr (GRAD) {
         float dldx, dldb;
         l = -model.lik_grad(x, yf, dldx, dldb);
@@ -488,6 +587,7 @@ __global__ __launch_bounds__(NT) void pf_kernel(PfArgs a) {
         grad_add(gacc, l, 0.f, 0.f, -dldb);
       } else {
         l = -model.lik(x, yf);
+      }
       }
       #pragma unroll
       for (int q = 0; q < D; ++q) xa[i * D + q] = x[q];
@@ -532,22 +632,29 @@ __global__ __launch_bounds__(NT) void pf_kernel(PfArgs a) {
       }
       acc = LseAcc{-INFINITY, 0.0, 0.0};
       gacc = GradAcc{-INFINITY, 0.0, 0.0, 0.0, 0.0};
+      const Key ik0 = Model::kOptimal ? split_at(ik, (uint32_t)n, 0u) : Key{0u, 0u};
       for (int64_t i = tid; i < n; i += NT) {
         const int64_t src = res ? (int64_t)anc[i] : i;
-        const Key ki = split_at(ik, (uint32_t)n, (uint32_t)i);
         float eps[D], xp[D], x[D];
-        #pragma unroll
-        for (uint32_t q = 0; q < hD; ++q) {
-          uint32_t w0, w1;
-          bits_pair(ki, D, q, w0, w1);
-          eps[q] = normal_from_bits(w0);
-          if (q + hD < (uint32_t)D) eps[q + hD] = normal_from_bits(w1);
+        if constexpr (Model::kOptimal) {
+          shared_eps<D>(ik0, (uint32_t)(n * D), (uint32_t)i, eps);
+        } else {
+          const Key ki = split_at(ik, (uint32_t)n, (uint32_t)i);
+          #pragma unroll
+          for (uint32_t q = 0; q < hD; ++q) {
+            uint32_t w0, w1;
+            bits_pair(ki, D, q, w0, w1);
+            eps[q] = normal_from_bits(w0);
+            if (q + hD < (uint32_t)D) eps[q + hD] = normal_from_bits(w1);
+          }
         }
         #pragma unroll
         for (int q = 0; q < D; ++q) xp[q] = xa[src * D + q];
-        model.transition(xp, eps, x);
         float l;
-        if constexpr (GRAD) {
+        if constexpr (Model::kOptimal) {
+          l = (res ? 0.f : lwa[i]) - model.step_opt(xp, eps, ycur, x);
+        } else if constexpr (GRAD) {
+          model.transition(xp, eps, x);
           float dldx, dldb;
           l = (res ? 0.f : lwa[i]) - model.lik_grad(x, ycur, dldx, dldb);
           const float dxp = xp[0] + model.phi * ta[src * 5 + 0];
@@ -562,6 +669,7 @@ __global__ __launch_bounds__(NT) void pf_kernel(PfArgs a) {
           tb[i * 5 + 4] = d2;
           grad_add(gacc, l, d0, d1, d2);
         } else {
+          model.transition(xp, eps, x);
           l = (res ? 0.f : lwa[i]) - model.lik(x, ycur);
         }
         #pragma unroll
@@ -863,21 +971,26 @@ __global__ __launch_bounds__(LT) void lf_init_kernel(LfArgs a) {
   constexpr uint32_t hD = (D + 1) / 2;
   LseAcc acc{-INFINITY, 0.0, 0.0};
   GradAcc gacc{-INFINITY, 0.0, 0.0, 0.0, 0.0};
+  const Key sub0 = Model::kOptimal ? split_at(sub, (uint32_t)n, 0u) : Key{0u, 0u};
   for (int pk = 0; pk < PPT; ++pk) {
     const int64_t i = (c - F.chunk0) * CH + pk * LT + tid;
     if (i >= n) break;
-    const Key ki = split_at(sub, (uint32_t)n, (uint32_t)i);
     float eps[D], x[D];
-#pragma unroll
-    for (uint32_t q = 0; q < hD; ++q) {
-      uint32_t w0, w1;
-      bits_pair(ki, D, q, w0, w1);
-      eps[q] = normal_from_bits(w0);
-      if (q + hD < (uint32_t)D) eps[q + hD] = normal_from_bits(w1);
-    }
-    model.init(eps, x);
     float l;
-    if constexpr (GRAD) {
+    if constexpr (Model::kOptimal) {
+      shared_eps<D>(sub0, (uint32_t)(n * D), (uint32_t)i, eps);
+      model.init_opt(eps, a.y + F.y_offset, x);
+      l = 0.f;
+    } else if constexpr (GRAD) {
+      const Key ki = split_at(sub, (uint32_t)n, (uint32_t)i);
+#pragma unroll
+      for (uint32_t q = 0; q < hD; ++q) {
+        uint32_t w0, w1;
+        bits_pair(ki, D, q, w0, w1);
+        eps[q] = normal_from_bits(w0);
+        if (q + hD < (uint32_t)D) eps[q + hD] = normal_from_bits(w1);
+      }
+      model.init(eps, x);
       float dldx, dldb;
       l = -model.lik_grad(x, a.y + F.y_offset, dldx, dldb);
       float* ta = a.tan0 + F.state_offset * 5;
@@ -888,6 +1001,15 @@ __global__ __launch_bounds__(LT) void lf_init_kernel(LfArgs a) {
       ta[i * 5 + 4] = -dldb;
       grad_add(gacc, l, 0.f, 0.f, -dldb);
     } else {
+      const Key ki = split_at(sub, (uint32_t)n, (uint32_t)i);
+#pragma unroll
+      for (uint32_t q = 0; q < hD; ++q) {
+        uint32_t w0, w1;
+        bits_pair(ki, D, q, w0, w1);
+        eps[q] = normal_from_bits(w0);
+        if (q + hD < (uint32_t)D) eps[q + hD] = normal_from_bits(w1);
+      }
+      model.init(eps, x);
       l = -model.lik(x, a.y + F.y_offset);
     }
     float* xa = a.x0 + F.state_offset * D;
@@ -1131,7 +1253,7 @@ __global__ __launch_bounds__(LT) void lf_step_kernel(LfArgs a, int64_t t, int ce
   const uint64_t* CE = a.cends + F.chunk0 * (CH / SG);
   if (two_level) stage_lds<8>(cend, nsg, [&](int64_t k) { return CE[k]; });
   // the step key (and the systematic uniform) are the same for the whole chunk: derived once
-  __shared__ uint32_t sk[2];
+  __shared__ uint32_t sk[4];
   __shared__ float su;
   if (tid == 0) {
     model.load(a.params + F.param_offset);
@@ -1140,6 +1262,11 @@ __global__ __launch_bounds__(LT) void lf_step_kernel(LfArgs a, int64_t t, int ce
     sk[0] = ikk.a;
     sk[1] = ikk.b;
     su = uniform_from_bits(bits_at(ikk, 1, 0), 0.f, 1.f);
+    if constexpr (Model::kOptimal) {
+      const Key k0 = split_at(ikk, (uint32_t)n, 0u);
+      sk[2] = k0.a;
+      sk[3] = k0.b;
+    }
   }
   __syncthreads();
   const Key ik{sk[0], sk[1]};
@@ -1240,6 +1367,10 @@ __global__ __launch_bounds__(LT) void lf_step_kernel(LfArgs a, int64_t t, int ce
     const int64_t i = base + pk * LT + tid;
     lwp[pk] = (live[pk] && !res) ? lwa[i] : 0.f;
     if (!live[pk]) continue;
+    if constexpr (Model::kOptimal) {
+      shared_eps<D>(Key{sk[2], sk[3]}, (uint32_t)(n * D), (uint32_t)i, eps[pk]);
+      continue;
+    }
     const Key ki = split_at(ik, (uint32_t)n, (uint32_t)i);
 #pragma unroll
     for (uint32_t q = 0; q < hD; ++q) {
@@ -1260,10 +1391,15 @@ __global__ __launch_bounds__(LT) void lf_step_kernel(LfArgs a, int64_t t, int ce
       for (int q = 2; q < 5; ++q) tg[pk][q] = (live[pk] && !res) ? ta[i * 5 + q] : 0.f;
     }
   }
+  float l[PPT], gd[PPT][3];
+  if constexpr (Model::kOptimal) {
+#pragma unroll
+    for (int pk = 0; pk < PPT; ++pk)
+      if (live[pk]) l[pk] = lwp[pk] - model.step_opt(xpre[pk], eps[pk], ycur, xn[pk]);
+  } else {
 #pragma unroll
   for (int pk = 0; pk < PPT; ++pk)
     if (live[pk]) model.transition(xpre[pk], eps[pk], xn[pk]);
-  float l[PPT], gd[PPT][3];
 #pragma unroll
   for (int pk = 0; pk < PPT; ++pk) {
     if (!live[pk]) continue;
@@ -1280,6 +1416,7 @@ __global__ __launch_bounds__(LT) void lf_step_kernel(LfArgs a, int64_t t, int ce
     } else {
       l[pk] = lwp[pk] - model.lik(xn[pk], ycur);
     }
+  }
   }
 #pragma unroll
   for (int pk = 0; pk < PPT; ++pk) {
@@ -1393,6 +1530,11 @@ LfLayout lf_layout(const mocat_pf_filter* filters, int n_filters) {
 
 }  // namespace
 
+// (d, dy) pairs compiled for the linear-Gaussian models
+#define MOCAT_PF_LG_SHAPES                                                                                  \
+  MOCAT_PF_CASE(1, 1) MOCAT_PF_CASE(2, 2) MOCAT_PF_CASE(4, 4) MOCAT_PF_CASE(4, 2) MOCAT_PF_CASE(5, 5) \
+  MOCAT_PF_CASE(8, 8) MOCAT_PF_CASE(2, 1) MOCAT_PF_CASE(3, 3) MOCAT_PF_CASE(4, 1) MOCAT_PF_CASE(8, 4)
+
 extern "C" {
 
 int mocat_pf_param_count(int model, int d, int dy, int64_t* count) {
@@ -1401,8 +1543,8 @@ int mocat_pf_param_count(int model, int d, int dy, int64_t* count) {
     *count = Sv::NPARAM;
     return MOCAT_OK;
   }
-  if (model != 0 || d < 1 || d > 8 || dy < 1 || dy > 8) return MOCAT_ERR_ARG;
-  *count = d + 3 * d * d + dy * d + dy * dy + 1;
+  if ((model != 0 && model != 2) || d < 1 || d > 8 || dy < 1 || dy > 8) return MOCAT_ERR_ARG;
+  *count = model == 0 ? d + 3 * d * d + dy * d + dy * dy + 1 : d + 3 * dy * d + 3 * d * d + dy * dy + 1;
   return MOCAT_OK;
 }
 
@@ -1422,11 +1564,19 @@ int mocat_pf_run(int model, int d, int dy, const mocat_pf_filter* filters, int n
     if (d != 1 || dy != 1) return MOCAT_ERR_ARG;
     return out_grad ? launch_pf<Sv, 1, 1, true>(a, stream) : launch_pf<Sv, 1, 1>(a, stream);
   }
+  if (model == 2) {
+    for (int f = 0; f < n_filters; ++f)
+      if (filters[f].n * (int64_t)d > 0xFFFFFFFFll) return MOCAT_ERR_SIZE;
+#define MOCAT_PF_CASE(DD, YY) \
+  if (d == DD && dy == YY) return launch_pf<LgssmOpt<DD, YY>, DD, YY>(a, stream);
+    MOCAT_PF_LG_SHAPES
+#undef MOCAT_PF_CASE
+    return MOCAT_ERR_ARG;
+  }
   if (model != 0) return MOCAT_ERR_ARG;
 #define MOCAT_PF_CASE(DD, YY) \
   if (d == DD && dy == YY) return launch_pf<Lgssm<DD, YY>, DD, YY>(a, stream);
-  MOCAT_PF_CASE(1, 1) MOCAT_PF_CASE(2, 2) MOCAT_PF_CASE(4, 4) MOCAT_PF_CASE(4, 2) MOCAT_PF_CASE(5, 5)
-  MOCAT_PF_CASE(8, 8) MOCAT_PF_CASE(2, 1) MOCAT_PF_CASE(3, 3) MOCAT_PF_CASE(4, 1) MOCAT_PF_CASE(8, 4)
+  MOCAT_PF_LG_SHAPES
 #undef MOCAT_PF_CASE
   return MOCAT_ERR_ARG;
 }
@@ -1460,6 +1610,7 @@ int mocat_pf_run_large(int model, int d, int dy, const mocat_pf_filter* host_fil
     if (!(F.flags & kPfLarge)) continue;
     if (F.n <= 0 || F.T <= 0 || F.n > 0x7FFFFFFFll || F.T > 0xFFFFFFFFll) return MOCAT_ERR_ARG;
     if (resampling == 2 && F.n * F.n >= 0xFFFFFFFFll) return MOCAT_ERR_SIZE;
+    if (model == 2 && F.n * (int64_t)d > 0xFFFFFFFFll) return MOCAT_ERR_SIZE;
     const int64_t nchunk = (F.n + CH - 1) / CH;
     desc.push_back(LfDesc{F.n, F.T, F.y_offset, F.state_offset, F.out_offset, F.t_offset, chunk0, {F.key[0], F.key[1]},
                           F.param_offset, f});
@@ -1485,11 +1636,17 @@ int mocat_pf_run_large(int model, int d, int dy, const mocat_pf_filter* host_fil
     return out_grad ? launch_pf_large<Sv, 1, 1, true>(a, L.max_T, L.cend_slots, stream)
                     : launch_pf_large<Sv, 1, 1>(a, L.max_T, L.cend_slots, stream);
   }
+  if (model == 2) {
+#define MOCAT_PF_CASE(DD, YY) \
+  if (d == DD && dy == YY) return launch_pf_large<LgssmOpt<DD, YY>, DD, YY>(a, L.max_T, L.cend_slots, stream);
+    MOCAT_PF_LG_SHAPES
+#undef MOCAT_PF_CASE
+    return MOCAT_ERR_ARG;
+  }
   if (model != 0) return MOCAT_ERR_ARG;
 #define MOCAT_PF_CASE(DD, YY) \
   if (d == DD && dy == YY) return launch_pf_large<Lgssm<DD, YY>, DD, YY>(a, L.max_T, L.cend_slots, stream);
-  MOCAT_PF_CASE(1, 1) MOCAT_PF_CASE(2, 2) MOCAT_PF_CASE(4, 4) MOCAT_PF_CASE(4, 2) MOCAT_PF_CASE(5, 5)
-  MOCAT_PF_CASE(8, 8) MOCAT_PF_CASE(2, 1) MOCAT_PF_CASE(3, 3) MOCAT_PF_CASE(4, 1) MOCAT_PF_CASE(8, 4)
+  MOCAT_PF_LG_SHAPES
 #undef MOCAT_PF_CASE
   return MOCAT_ERR_ARG;
 }

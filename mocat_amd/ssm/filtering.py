@@ -19,6 +19,8 @@ from .. import _lib
 from .. import random as mr
 from ..core import cdict
 
+LOG2PI = float(np.log(2 * np.pi))
+
 RESAMPLING = {"systematic": 0, "multinomial": 1, "categorical": 2}
 
 _P = _lib.c_ptr
@@ -52,6 +54,90 @@ class ParticleFilter:
 class BootstrapFilter(ParticleFilter):
     """Proposal = transition, weight = likelihood (mocat/src/ssm/filtering.py:142-170)."""
     name = "Bootstrap Filter"
+
+
+def _sym_chol(M):
+    # jnp.linalg.cholesky symmetrises its input, (M + M^T) / 2, before factoring
+    return np.linalg.cholesky(0.5 * (M + M.T))
+
+
+def _kalman_gain(cov_sqrt, H, lik_cov_sqrt, inv_mat=None):
+    """mocat/src/utils.py:476-484."""
+    if inv_mat is None:
+        hc = H @ cov_sqrt
+        inv_mat = np.linalg.inv(hc @ hc.T + lik_cov_sqrt @ lik_cov_sqrt.T)
+    return cov_sqrt @ cov_sqrt.T @ H.T @ inv_mat
+
+
+def _nlg_matrices(model):
+    """The linear-Gaussian model as mocat's NonLinearGaussian interface reads it: transition_function
+    x -> A x (linear_gaussian.py:112-116) and reset_covariance's square roots (utils.py:247-262)."""
+    if getattr(model, "kernel_model", None) != 0:
+        raise NotImplementedError("the optimal-proposal filter and the EnKF are compiled for the linear transition "
+                                  "x -> A x of TimeHomogenousLinearGaussian")
+    L0 = np.linalg.cholesky(model.initial_covariance)
+    return dict(m0=np.asarray(model.initial_mean, np.float64), L0=L0, S0=np.linalg.inv(L0),
+                A=np.asarray(model.transition_matrix, np.float64), Q=np.asarray(model.transition_covariance, np.float64),
+                LQ=model.transition_covariance_sqrt, H=np.asarray(model.likelihood_matrix, np.float64),
+                LR=model.likelihood_covariance_sqrt, SR=model.likelihood_precision_sqrt)
+
+
+def _initial_conditioning(M):
+    """initial_kalman_gain and initial_conditioned_covariance_sqrt (nonlinear_gaussian.py:161-170,
+    304-310), mocat's expressions as written."""
+    K0 = _kalman_gain(M["L0"], M["H"], M["LR"])
+    hs = M["H"].T @ M["SR"]
+    icp = M["S0"] @ M["S0"] + hs @ hs.T
+    L = _sym_chol(icp)
+    return K0, L, np.linalg.inv(L), icp
+
+
+class OptimalNonLinearGaussianParticleFilter(ParticleFilter):
+    """Proposal p(x_t | x_{t-1}, y_t), weight p(y_t | x_{t-1}) for Gaussian transition noise and a linear
+    likelihood (mocat/src/ssm/nonlinear_gaussian.py:134-281). Compiled for TimeHomogenousLinearGaussian
+    (transition_function = A x), the model mocat's examples run it on. Initial weights are zero, so the
+    log-likelihood returned estimates log p(y_1:T | y_0)."""
+    name = "Optimal Non-linear Gaussian Particle Filter"
+    kernel_model = 2
+
+    def startup(self, ssm_scenario):
+        """nonlinear_gaussian.py:154-184 in FP64."""
+        M = _nlg_matrices(ssm_scenario)
+        self.initial_kalman_gain, L, self.initial_conditioned_covariance_sqrt, icp = _initial_conditioning(M)
+        self.initial_conditioned_precision_sqrt = L
+        self.initial_conditioned_precision_det = np.linalg.det(icp)
+        hq = M["H"] @ M["LQ"]
+        W = np.linalg.inv(hq @ hq.T + M["LR"] @ M["LR"].T)
+        self.proposal_kalman_gain = _kalman_gain(M["LQ"], M["H"], M["LR"], inv_mat=W)
+        prop_cov = M["Q"] - self.proposal_kalman_gain @ M["H"] @ M["Q"]
+        self.proposal_covariance_sqrt = _sym_chol(prop_cov)
+        self.proposal_precision_sqrt = np.linalg.inv(self.proposal_covariance_sqrt)
+        self.proposal_precision_det = 1 / np.linalg.det(prop_cov)
+        self.weight_precision_sqrt = _sym_chol(W)
+        self.weight_precision_det = np.linalg.det(W)
+
+    def kernel_params(self, ssm_scenario):
+        """[m0, H, K0, C0, A, K, S, WC, nlzw] for the fused filter kernel (model 2)."""
+        self.startup(ssm_scenario)
+        M = _nlg_matrices(ssm_scenario)
+        dy = M["H"].shape[0]
+        nlzw = (dy * LOG2PI - np.log(self.weight_precision_det)) / 2
+        return np.concatenate([M["m0"].ravel(), M["H"].ravel(), self.initial_kalman_gain.ravel(),
+                               self.initial_conditioned_covariance_sqrt.ravel(), M["A"].ravel(),
+                               self.proposal_kalman_gain.ravel(), self.proposal_covariance_sqrt.ravel(),
+                               self.weight_precision_sqrt.ravel(), [nlzw]]).astype(np.float32)
+
+
+def _filter_kind(particle_filter):
+    if particle_filter is None:
+        return BootstrapFilter
+    cls = particle_filter if isinstance(particle_filter, type) else type(particle_filter)
+    if issubclass(cls, OptimalNonLinearGaussianParticleFilter):
+        return OptimalNonLinearGaussianParticleFilter
+    if issubclass(cls, BootstrapFilter):
+        return BootstrapFilter
+    raise NotImplementedError(f"{cls.__name__}: the fused filter kernels implement BootstrapFilter and "
+                              "OptimalNonLinearGaussianParticleFilter (EnsembleKalmanFilter: run_ensemble_kalman_filter)")
 
 
 def resample_indices(weights: torch.Tensor, random_key, method: str = "systematic",
@@ -96,7 +182,7 @@ def _large_flags(ns, Ts, large_n):
     return [int(n) >= int(large_n) for n in ns]
 
 
-def _launch(models, ys, keys, ns, ess_threshold, resampling, store, grad=False, large_n=None):
+def _launch(models, ys, keys, ns, ess_threshold, resampling, store, grad=False, large_n=None, particle_filter=None):
     """Runs len(models) filters (same model class and dims): filters below large_n particles in one
     persistent launch (a workgroup per filter), the others through the chunked per-step path."""
     m0 = models[0]
@@ -104,6 +190,13 @@ def _launch(models, ys, keys, ns, ess_threshold, resampling, store, grad=False, 
     for mdl in models:
         if (mdl.kernel_model, mdl.dim, mdl.dim_obs) != (kind, d, dy):
             raise ValueError("filters of one launch must share the model class and dimensions")
+    optimal = _filter_kind(particle_filter) is OptimalNonLinearGaussianParticleFilter
+    if optimal:
+        opt = particle_filter if isinstance(particle_filter, ParticleFilter) else particle_filter()
+        _nlg_matrices(m0)  # raises for models without a compiled optimal proposal
+        kind = OptimalNonLinearGaussianParticleFilter.kernel_model
+        if any(int(v) * d >= 2 ** 32 for v in ns):
+            raise ValueError("the optimal filter draws normal(key, (n, d)): n * d must be < 2^32")
     F = len(models)
     Ts = [int(np.asarray(y).shape[0]) for y in ys]
     order = sorted(range(F), key=lambda i: -(ns[i] * Ts[i]))  # longest first
@@ -112,7 +205,7 @@ def _launch(models, ys, keys, ns, ess_threshold, resampling, store, grad=False, 
     params, yflat = [], []
     poff = yoff = soff = ooff = toff = 0
     for slot, i in enumerate(order):
-        p = models[i].kernel_params()
+        p = opt.kernel_params(models[i]) if optimal else models[i].kernel_params()
         yi = np.asarray(ys[i], np.float32).reshape(Ts[i], dy)
         desc[slot] = (ns[i], Ts[i], yoff, soff, ooff if store else -1, toff, _key_np(keys[i]), poff,
                       PF_LARGE if large[i] else 0)
@@ -194,9 +287,7 @@ def _launch(models, ys, keys, ns, ess_threshold, resampling, store, grad=False, 
 def run_particle_filter_for_marginals(ssm_scenario, particle_filter, y, t, random_key, n: int = None,
                                       initial_sample: cdict = None, ess_threshold: float = 0.5,
                                       resampling: str = "categorical") -> cdict:
-    if not isinstance(particle_filter, BootstrapFilter) and not (
-            isinstance(particle_filter, type) and issubclass(particle_filter, BootstrapFilter)):
-        raise NotImplementedError("the fused filter kernel implements the bootstrap filter")
+    _filter_kind(particle_filter)
     if initial_sample is not None:
         raise NotImplementedError("resuming from initial_sample is not supported by the fused filter kernel")
     if resampling not in RESAMPLING:
@@ -211,7 +302,8 @@ def run_particle_filter_for_marginals(ssm_scenario, particle_filter, y, t, rando
         raise NotImplementedError("the stochastic-volatility kernel assumes unit time steps")
     if resampling == "categorical" and n * n >= 0xFFFFFFFF:
         raise ValueError("categorical resampling draws an (n, n) Gumbel matrix: n * n must be < 2^32 - 1")
-    r = _launch([ssm_scenario], [yn], [random_key], [int(n)], ess_threshold, resampling, True)[0]
+    r = _launch([ssm_scenario], [yn], [random_key], [int(n)], ess_threshold, resampling, True,
+                particle_filter=particle_filter)[0]
     dev = r["value"].device
     out = cdict(value=r["value"], log_weight=r["log_weight"], t=torch.as_tensor(tn, device=dev),
                 y=torch.as_tensor(yn, dtype=torch.float32, device=dev), ess=r["ess"],
@@ -221,17 +313,18 @@ def run_particle_filter_for_marginals(ssm_scenario, particle_filter, y, t, rando
 
 def run_particle_filters(ssm_scenarios, ys, random_keys, ns, ess_threshold: float = 0.5,
                          resampling: str = "systematic", store: bool = False, grad: bool = False,
-                         large_n: int = None):
+                         large_n: int = None, particle_filter=None):
     """Independent bootstrap filters of uneven sizes in one launch. Returns a list of cdicts with
     log_likelihood (FP64), ess (T,), final_value, final_log_weight (and value/log_weight/ancestors when
     store=True). grad=True (stochastic volatility): log_likelihood_grad = d log Z / d(phi, sigma, beta)
     of the fixed-resampling-index estimator. Filters with at least large_n particles (default 4096)
-    run on the chunked path spread over the whole GPU; the results do not depend on the path."""
+    run on the chunked path spread over the whole GPU; the results do not depend on the path.
+    particle_filter: BootstrapFilter (default) or OptimalNonLinearGaussianParticleFilter."""
     ys = [y.detach().cpu().numpy() if isinstance(y, torch.Tensor) else np.asarray(y) for y in ys]
     if resampling == "categorical" and any(int(v) * int(v) >= 0xFFFFFFFF for v in ns):
         raise ValueError("categorical resampling draws an (n, n) Gumbel matrix: n * n must be < 2^32 - 1")
     res = _launch(list(ssm_scenarios), ys, list(random_keys), [int(v) for v in ns], ess_threshold, resampling, store,
-                  grad, large_n)
+                  grad, large_n, particle_filter)
     return [cdict(**{k: v for k, v in r.items() if k != "desc"}) for r in res]
 
 

@@ -1,0 +1,121 @@
+"""GPU optimal-proposal particle filter and ensemble Kalman filter vs the oracle (same keys => same
+random path)."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import kalman as okal
+from oracle import nlg as onlg
+from oracle import rng as orng
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def m():
+    import mocat_amd
+    return mocat_amd
+
+
+def _p(model):
+    return onlg.startup(model.initial_mean, model.initial_covariance, model.transition_matrix,
+                        model.transition_covariance, model.likelihood_matrix, model.likelihood_covariance)
+
+
+def _full_model(m, d=4, dy=2, seed=0):
+    # non-diagonal P0, Q and R: mocat's initial conditioning expression (not the exact posterior) is followed
+    rs = np.random.default_rng(seed)
+    spd = lambda k, s: s * ((lambda B: B @ B.T / k)(rs.normal(0, 1, (k, k))) + 0.5 * np.eye(k))
+    A = 0.85 * np.eye(d) + rs.normal(0, 0.1, (d, d))
+    P0 = 0.5 * np.eye(d) + 0.05 * np.ones((d, d))
+    return m.ssm.TimeHomogenousLinearGaussian(rs.normal(0, 1, d), P0, A, spd(d, 0.2), rs.normal(0, 0.8, (dy, d)),
+                                              0.5 * np.eye(dy) + 0.1 * np.ones((dy, dy)))
+
+
+def _flags(ess, thr, n):
+    f = np.zeros(len(ess), bool)
+    f[1:] = ess[:-1] < thr * n
+    return f
+
+
+@pytest.mark.parametrize("resampling,thr,n", [("systematic", 0.5, 1024), ("multinomial", 0.99, 700),
+                                              ("systematic", 0.9, 5000)])
+def test_optimal_pf_same_path_as_fp64_oracle(m, resampling, thr, n):
+    """GPU f32 optimal filter vs the FP64 restatement on the same random path (the GPU's resampling
+    decisions and ancestors replayed); n = 5000 runs on the chunked path. Thresholds stay below 1: the
+    zero initial weights have ESS = n up to FP64 rounding, so at 1.0 the first decision is a rounding tie."""
+    from mocat_amd import datasets
+    model, y, t = datasets.lgssm_config(0, 4, 30)
+    out = m.ssm.run_particle_filter_for_marginals(model, m.ssm.OptimalNonLinearGaussianParticleFilter(), y, t,
+                                                  m.random.PRNGKey(1), n=n, ess_threshold=thr, resampling=resampling)
+    assert out.value.shape == (30, n, 4)
+    ess = out.ess.cpu().numpy()
+    o = onlg.run_optimal_pf(_p(model), y.cpu().numpy(), orng.PRNGKey(1), n, thr, resampling,
+                            ancestors=out.ancestors.cpu().numpy(), resample_flags=_flags(ess, thr, n))
+    np.testing.assert_allclose(out.value[0].cpu().numpy(), o["value"][0], rtol=1e-5, atol=1e-5)
+    np.testing.assert_array_equal(out.log_weight[0].cpu().numpy(), np.zeros(n, np.float32))
+    np.testing.assert_allclose(out.value[-1].cpu().numpy(), o["value"][-1], rtol=1e-3, atol=1e-3)
+    np.testing.assert_allclose(out.log_weight[-1].cpu().numpy(), o["log_weight"][-1], rtol=1e-4, atol=1e-3)
+    ll = out.log_likelihood.item()
+    assert abs(ll - o["log_likelihood"]) <= 1e-4 * abs(o["log_likelihood"]) + 1e-4, (ll, o["log_likelihood"])
+    np.testing.assert_allclose(ess, o["ess"], rtol=1e-3)
+
+
+def test_optimal_pf_chunked_matches_workgroup_path(m):
+    from mocat_amd import datasets
+    model, y, t = datasets.lgssm_config(2, 2, 40)
+    models, ys = [model] * 3, [y.cpu().numpy()] * 3
+    keys = [k for k in m.random.split(m.random.PRNGKey(4), 3)]
+    ns = [300, 4096, 9000]
+    kw = dict(ess_threshold=0.8, resampling="systematic", store=True,
+              particle_filter=m.ssm.OptimalNonLinearGaussianParticleFilter())
+    big = m.ssm.run_particle_filters(models, ys, keys, ns, large_n=1, **kw)
+    one = m.ssm.run_particle_filters(models, ys, keys, ns, large_n=1 << 40, **kw)
+    for i in range(3):
+        np.testing.assert_array_equal(big[i].ancestors.cpu().numpy(), one[i].ancestors.cpu().numpy())
+        np.testing.assert_array_equal(big[i].value.cpu().numpy(), one[i].value.cpu().numpy())
+        a, b = big[i].log_likelihood.item(), one[i].log_likelihood.item()
+        assert abs(a - b) <= 1e-10 * (1 + abs(b)), (i, a, b)
+
+
+def test_optimal_pf_full_covariances_same_path(m):
+    model = _full_model(m)
+    y = model.simulate(np.arange(12.0), m.random.PRNGKey(3)).y
+    n = 2000
+    out = m.ssm.run_particle_filter_for_marginals(model, m.ssm.OptimalNonLinearGaussianParticleFilter, y,
+                                                  np.arange(12.0), m.random.PRNGKey(5), n=n, ess_threshold=0.5,
+                                                  resampling="systematic")
+    ess = out.ess.cpu().numpy()
+    o = onlg.run_optimal_pf(_p(model), y.cpu().numpy(), orng.PRNGKey(5), n, 0.5, "systematic",
+                            ancestors=out.ancestors.cpu().numpy(), resample_flags=_flags(ess, 0.5, n))
+    np.testing.assert_allclose(out.value[0].cpu().numpy(), o["value"][0], rtol=1e-5, atol=1e-5)
+    np.testing.assert_allclose(out.value[-1].cpu().numpy(), o["value"][-1], rtol=1e-3, atol=1e-3)
+
+
+def test_optimal_pf_loglik_vs_kalman(m):
+    """The optimal filter's log-likelihood (zero initial weights) estimates log p(y_1:T | y_0): the
+    Kalman log-likelihood minus its t = 0 term. For a linear-Gaussian model its variance is small."""
+    from mocat_amd import datasets
+    from scipy.stats import multivariate_normal
+    model, y, t = datasets.lgssm_config(0, 4, 100)
+    R = 16
+    keys = m.random.split(m.random.PRNGKey(7), R)
+    res = m.ssm.run_particle_filters([model] * R, [y.cpu().numpy()] * R, [k for k in keys], [4096] * R,
+                                     ess_threshold=0.5, resampling="systematic",
+                                     particle_filter=m.ssm.OptimalNonLinearGaussianParticleFilter())
+    lls = np.array([r.log_likelihood.item() for r in res])
+    yn = y.cpu().numpy().astype(np.float64)
+    kll, _, _ = okal.loglik(model.initial_mean, model.initial_covariance, model.transition_matrix,
+                            model.transition_covariance, model.likelihood_matrix, model.likelihood_covariance, yn)
+    H = model.likelihood_matrix
+    ll0 = multivariate_normal(H @ model.initial_mean, H @ model.initial_covariance @ H.T +
+                              model.likelihood_covariance).logpdf(yn[0])
+    mean, se = lls.mean(), lls.std(ddof=1) / np.sqrt(R)
+    assert abs(mean - (kll - ll0)) < 4 * se + 0.5 * lls.var() + 1e-3, (mean, se, kll - ll0)
+
+
+def test_optimal_pf_rejects_other_models(m):
+    model = m.ssm.StochasticVolatility()
+    with pytest.raises(NotImplementedError):
+        m.ssm.run_particle_filter_for_marginals(model, m.ssm.OptimalNonLinearGaussianParticleFilter(),
+                                                np.zeros(5), np.arange(5.0), m.random.PRNGKey(0), n=16)
