@@ -176,6 +176,22 @@ int mocat_ffbsi_rejection_step(int model, int d, const float* params, const floa
                                float init_bound, float bound_inflation, void* workspace, int64_t workspace_bytes,
                                int32_t* idx, float* x_out, long long* evals, hipStream_t stream);
 
+/* ----------------------------------------------------------------------- ensemble Kalman filter
+ * run_particle_filter_for_marginals with EnsembleKalmanFilter (mocat/src/ssm/nonlinear_gaussian.py:
+ * 284-350, filtering.py:255-324) for TimeHomogenousLinearGaussian (transition_function = A x):
+ * params [m0(d), H(dyxd), K0(dxdy), C0(dxd), A(dxd), LQ(dxd), LR(dyxdy)] (K0, C0 as the filter's
+ * startup :297-310, LQ/LR = cholesky of Q/R); y [T][dy]; key: device u32[2]. With
+ * (rest, sub) = split(key): x_0 from normal(split(sub, n)[0], (n, d)); step t from
+ * (k1, k2) = split(split(split(rest, T-1)[t-1], n)[0]), normal(k1, (n, d)) the forecast noise and
+ * normal(k2, (n, dy)) the observation perturbation (filtering.py:267-294); gain from the forecast
+ * ensemble's covariance (ddof 1) in FP64. x_out [n][d] final ensemble (required); out_value [T][n][d] and
+ * out_gain [T-1][d][dy] optional. n >= 2, n * max(d, dy) < 2^32. Two launches per step. */
+int mocat_enkf_param_count(int d, int dy, int64_t* count);
+int mocat_enkf_workspace_bytes(int64_t n, int d, int64_t* bytes);
+int mocat_enkf_run(int d, int dy, const float* params, const float* y, int64_t T, int64_t n, const uint32_t* key,
+                   float* x_out, void* workspace, int64_t workspace_bytes, float* out_value, float* out_gain,
+                   hipStream_t stream);
+
 /* -------------------------------------------------------------------------------- tempered SMC
  * dst[j][:] = src[idx[j]][:] for 4-byte elements, width per row: ensemble_state[resampled_indices]
  * (mocat/src/transport/smc.py:68). */

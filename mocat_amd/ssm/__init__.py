@@ -1,6 +1,6 @@
-"""mocat.ssm surface (mocat/ssm.py): models, bootstrap and optimal-proposal filters, filtering drivers."""
+"""mocat.ssm surface (mocat/ssm.py): models, bootstrap, optimal-proposal and ensemble Kalman filters, filtering drivers."""
 from .models import StateSpaceModel, TimeHomogenousLinearGaussian, StochasticVolatility
-from .filtering import (ParticleFilter, BootstrapFilter, OptimalNonLinearGaussianParticleFilter,
+from .filtering import (ParticleFilter, BootstrapFilter, OptimalNonLinearGaussianParticleFilter, EnsembleKalmanFilter,
                         run_particle_filter_for_marginals, run_particle_filters, resample_indices, categorical,
                         run_kalman_filter_for_marginals)
 from .backward import backward_simulation, forward_filtering_backward_simulation

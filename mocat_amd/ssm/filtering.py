@@ -32,6 +32,10 @@ _lib.register("mocat_pf_run_large", [_lib.c_i32, _lib.c_i32, _lib.c_i32, _P, _li
                                      _lib.c_i32, _lib.c_f32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _lib.c_i64, _P])
 _lib.register("mocat_resample", [_P, _lib.c_i64, _lib.c_i32, _lib.c_i32, _P, _P, _P, _P])
 _lib.register("mocat_resample_categorical", [_P, _lib.c_i64, _lib.c_i64, _P, _P, _P])
+_lib.register("mocat_enkf_param_count", [_lib.c_i32, _lib.c_i32, _P])
+_lib.register("mocat_enkf_workspace_bytes", [_lib.c_i64, _lib.c_i32, _P])
+_lib.register("mocat_enkf_run", [_lib.c_i32, _lib.c_i32, _P, _P, _lib.c_i64, _lib.c_i64, _P, _P, _P, _lib.c_i64, _P, _P,
+                                 _P])
 
 FILTER_DTYPE = np.dtype([("n", "<i8"), ("T", "<i8"), ("y_offset", "<i8"), ("state_offset", "<i8"),
                          ("out_offset", "<i8"), ("t_offset", "<i8"), ("key", "<u4", (2,)),
@@ -128,16 +132,61 @@ class OptimalNonLinearGaussianParticleFilter(ParticleFilter):
                                self.weight_precision_sqrt.ravel(), [nlzw]]).astype(np.float32)
 
 
+class EnsembleKalmanFilter(ParticleFilter):
+    """Ensemble Kalman filter with perturbed observations (mocat/src/ssm/nonlinear_gaussian.py:284-350):
+    forecast mx = f(x) + Q noise, gain from the forecast ensemble's covariance (ddof 1), analysis
+    x = mx + K (y - H mx - R noise); log-weights stay zero. Compiled for TimeHomogenousLinearGaussian
+    (transition_function = A x). run_particle_filter_for_marginals returns its ensembles and adds
+    `gain` (T - 1, d, dy)."""
+    name = "Ensemble Kalman Filter"
+
+    def startup(self, ssm_scenario):
+        """nonlinear_gaussian.py:297-310 in FP64."""
+        M = _nlg_matrices(ssm_scenario)
+        self.initial_kalman_gain, _, self.initial_conditioned_covariance_sqrt, _ = _initial_conditioning(M)
+
+    def kernel_params(self, ssm_scenario):
+        """[m0, H, K0, C0, A, LQ, LR] for the EnKF kernels."""
+        self.startup(ssm_scenario)
+        M = _nlg_matrices(ssm_scenario)
+        return np.concatenate([M["m0"].ravel(), M["H"].ravel(), self.initial_kalman_gain.ravel(),
+                               self.initial_conditioned_covariance_sqrt.ravel(), M["A"].ravel(), M["LQ"].ravel(),
+                               M["LR"].ravel()]).astype(np.float32)
+
+
 def _filter_kind(particle_filter):
     if particle_filter is None:
         return BootstrapFilter
     cls = particle_filter if isinstance(particle_filter, type) else type(particle_filter)
-    if issubclass(cls, OptimalNonLinearGaussianParticleFilter):
-        return OptimalNonLinearGaussianParticleFilter
-    if issubclass(cls, BootstrapFilter):
-        return BootstrapFilter
-    raise NotImplementedError(f"{cls.__name__}: the fused filter kernels implement BootstrapFilter and "
-                              "OptimalNonLinearGaussianParticleFilter (EnsembleKalmanFilter: run_ensemble_kalman_filter)")
+    for kind in (OptimalNonLinearGaussianParticleFilter, EnsembleKalmanFilter, BootstrapFilter):
+        if issubclass(cls, kind):
+            return kind
+    raise NotImplementedError(f"{cls.__name__}: the compiled filters are BootstrapFilter, "
+                              "OptimalNonLinearGaussianParticleFilter and EnsembleKalmanFilter")
+
+
+def _run_enkf(model, enkf, yn, tn, random_key, n, store=True):
+    d, dy = int(model.dim), int(model.dim_obs)
+    if n < 2:
+        raise ValueError("the ensemble Kalman filter needs n >= 2 (its covariance divides by n - 1)")
+    if n * max(d, dy) >= 2 ** 32:
+        raise ValueError("the EnKF draws normal(key, (n, max(d, dy))): n * max(d, dy) must be < 2^32")
+    dev = mr.default_device()
+    T = int(yn.shape[0])
+    params = torch.from_numpy(enkf.kernel_params(model)).to(dev)
+    y_t = torch.from_numpy(np.ascontiguousarray(yn, np.float32).reshape(T, dy)).to(dev)
+    key = mr.as_key(random_key).reshape(2).contiguous().to(dev)
+    nb = ctypes.c_int64(0)
+    _lib.call("mocat_enkf_workspace_bytes", int(n), d, ctypes.byref(nb))
+    ws = torch.empty(int(nb.value), dtype=torch.uint8, device=dev)
+    x = torch.empty((n, d), dtype=torch.float32, device=dev)
+    value = torch.empty((T, n, d), dtype=torch.float32, device=dev) if store else None
+    gain = torch.empty((max(T - 1, 0), d, dy), dtype=torch.float32, device=dev)
+    _lib.call("mocat_enkf_run", d, dy, _lib.ptr(params), _lib.ptr(y_t), T, int(n), _lib.ptr(key), _lib.ptr(x),
+              _lib.ptr(ws), int(ws.numel()), _lib.ptr(value), _lib.ptr(gain) if T > 1 else None, _lib.stream_handle())
+    return cdict(value=value, log_weight=torch.zeros((T, n), dtype=torch.float32, device=dev),
+                 t=torch.as_tensor(tn, device=dev), y=y_t, ess=torch.full((T,), float(n), device=dev), gain=gain,
+                 final_value=x)
 
 
 def resample_indices(weights: torch.Tensor, random_key, method: str = "systematic",
@@ -287,7 +336,7 @@ def _launch(models, ys, keys, ns, ess_threshold, resampling, store, grad=False, 
 def run_particle_filter_for_marginals(ssm_scenario, particle_filter, y, t, random_key, n: int = None,
                                       initial_sample: cdict = None, ess_threshold: float = 0.5,
                                       resampling: str = "categorical") -> cdict:
-    _filter_kind(particle_filter)
+    kind = _filter_kind(particle_filter)
     if initial_sample is not None:
         raise NotImplementedError("resuming from initial_sample is not supported by the fused filter kernel")
     if resampling not in RESAMPLING:
@@ -298,6 +347,10 @@ def run_particle_filter_for_marginals(ssm_scenario, particle_filter, y, t, rando
     if yn.ndim == 1:
         yn = yn[:, None]
     tn = t.detach().cpu().numpy() if isinstance(t, torch.Tensor) else np.asarray(t)
+    if kind is EnsembleKalmanFilter:
+        enkf = particle_filter if isinstance(particle_filter, ParticleFilter) else particle_filter()
+        _nlg_matrices(ssm_scenario)
+        return _run_enkf(ssm_scenario, enkf, yn, tn, random_key, int(n))
     if getattr(ssm_scenario, "kernel_model", None) == 1 and len(tn) > 1 and not np.allclose(np.diff(tn), 1.0):
         raise NotImplementedError("the stochastic-volatility kernel assumes unit time steps")
     if resampling == "categorical" and n * n >= 0xFFFFFFFF:
@@ -320,6 +373,8 @@ def run_particle_filters(ssm_scenarios, ys, random_keys, ns, ess_threshold: floa
     of the fixed-resampling-index estimator. Filters with at least large_n particles (default 4096)
     run on the chunked path spread over the whole GPU; the results do not depend on the path.
     particle_filter: BootstrapFilter (default) or OptimalNonLinearGaussianParticleFilter."""
+    if _filter_kind(particle_filter) is EnsembleKalmanFilter:
+        raise NotImplementedError("EnsembleKalmanFilter runs one ensemble per call: run_particle_filter_for_marginals")
     ys = [y.detach().cpu().numpy() if isinstance(y, torch.Tensor) else np.asarray(y) for y in ys]
     if resampling == "categorical" and any(int(v) * int(v) >= 0xFFFFFFFF for v in ns):
         raise ValueError("categorical resampling draws an (n, n) Gumbel matrix: n * n must be < 2^32 - 1")
