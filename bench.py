@@ -42,9 +42,9 @@ def parse():
     ap.add_argument("--cpu-iters", type=int, default=120)
     ap.add_argument("--plain-steps", type=int, default=30, help="iterations timed on the unfused FP32 path (0: skip)")
     ap.add_argument("--iters", type=int, default=500, help="iterations of the svgd4 workload")
-    ap.add_argument("--mode", default="hmc", choices=["hmc", "copy", "pf1", "pf5", "smc3", "svgd4", "ffbsi"],
+    ap.add_argument("--mode", default="hmc", choices=["hmc", "copy", "pf1", "pf5", "smc3", "svgd4", "ffbsi", "nlg"],
                     help="hmc: the headline workload. copy: a plain 256 MiB device copy (calibration run for HBM "
-                         "byte counters). pf1 / pf5 / smc3 / svgd4 / ffbsi: the other workloads, one JSON line each")
+                         "byte counters). pf1 / pf5 / smc3 / svgd4 / ffbsi / nlg: the other workloads, one JSON line each")
     return ap.parse_args()
 
 
@@ -297,6 +297,54 @@ def bench_ffbsi(n=4096, T=100):
                                        "density_evals_per_step": float(rr.num_transition_evals[1:].mean().item())}}))
 
 
+def bench_nlg(T=100):
+    """Optimal-proposal particle filter and ensemble Kalman filter on the pf1 linear-Gaussian model
+    (d = 4, T = 100): the optimal filter with 4096 particles (trajectory stored) and 2^20 (chunked path),
+    the EnKF with 2^20 members (every ensemble stored). Unit: particle-steps/s. CPU baseline: the
+    NumPy oracle's EnKF on 131072 members x 30 steps."""
+    import numpy as np
+    import mocat_amd as m
+    from mocat_amd import datasets
+    from oracle import nlg as onlg, rng as orng
+    model, y, t = datasets.lgssm_config(0, 4, T)
+    yn = y.cpu().numpy()
+    opt = m.ssm.OptimalNonLinearGaussianParticleFilter()
+    run1 = lambda: m.ssm.run_particle_filter_for_marginals(model, opt, y, t, m.random.PRNGKey(0), n=4096,
+                                                           resampling="systematic")
+    run1()
+    o1, dt1 = _timed(run1)
+    big = 1 << 20
+    runb = lambda: m.ssm.run_particle_filters([model], [yn], [m.random.PRNGKey(1)], [big], 0.5, "systematic",
+                                              particle_filter=opt)
+    runb()
+    _, dtb = _timed(runb)
+    enkf = m.ssm.EnsembleKalmanFilter()
+    rune = lambda: m.ssm.run_particle_filter_for_marginals(model, enkf, y, t, m.random.PRNGKey(2), n=big)
+    rune()
+    oe, dte = _timed(rune)
+    ps = big * T
+    # CPU: the oracle's EnKF (FP64 NumPy) on a bounded sample
+    p = onlg.startup(model.initial_mean, model.initial_covariance, model.transition_matrix,
+                     model.transition_covariance, model.likelihood_matrix, model.likelihood_covariance)
+    nc, tc = 131072, 30
+    t0 = time.perf_counter()
+    onlg.run_enkf(p, yn[:tc], orng.PRNGKey(2), nc)
+    dtc = time.perf_counter() - t0
+    print(json.dumps({"mode": "nlg", "metric": "particle-steps/s",
+                      "optimal_pf_4096": {"particle_steps_per_s": 4096 * T / dt1, "ms": dt1 * 1e3,
+                                          "log_likelihood_y1T_given_y0": o1.log_likelihood.item(),
+                                          "store_trajectory": True},
+                      "optimal_pf_n2e20": {"particle_steps_per_s": ps / dtb, "ms": dtb * 1e3,
+                                           "ms_per_step": dtb * 1e3 / T, "store_trajectory": False},
+                      "enkf_n2e20": {"particle_steps_per_s": ps / dte, "ms": dte * 1e3,
+                                     "ms_per_step": dte * 1e3 / (T - 1), "store_trajectory": True,
+                                     "bytes_per_particle_step": 48,
+                                     "algorithmic_GBps": 48 * big * (T - 1) / dte / 1e9,
+                                     "hbm_frac": 48 * big * (T - 1) / dte / 8e12},
+                      "cpu_baseline": {"value": nc * tc / dtc, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+                                       "sample": f"oracle EnKF, {nc} members x {tc} steps ({dtc:.1f} s)"}}))
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -315,7 +363,8 @@ def main():
         if a.mode == "svgd4":
             bench_svgd4(iters=a.iters)
         else:
-            {"pf1": bench_pf1, "pf5": bench_pf5, "smc3": bench_smc3, "ffbsi": bench_ffbsi}[a.mode]()
+            {"pf1": bench_pf1, "pf5": bench_pf5, "smc3": bench_smc3, "ffbsi": bench_ffbsi,
+             "nlg": bench_nlg}[a.mode]()
         return
     import mocat_amd as m
     from mocat_amd import datasets

@@ -166,6 +166,15 @@ MOCAT_DEV T wave_min(T v) {
   return v;
 }
 
+// Copies a parameter struct (floats only, in parameter order) to shared memory with the whole
+// workgroup, so that its global loads overlap; the caller synchronises before use.
+template <class Params>
+MOCAT_DEV void load_shared_params(Params& m, const float* p) {
+  static_assert(sizeof(Params) == Params::NPARAM * sizeof(float), "parameter struct must be NPARAM floats");
+  float* dst = reinterpret_cast<float*>(&m);
+  for (int i = threadIdx.x; i < Params::NPARAM; i += blockDim.x) dst[i] = p[i];
+}
+
 // Block-wide reductions; `scratch` must hold blockDim.x / 64 elements. All threads get the result.
 template <typename T>
 MOCAT_DEV T block_sum(T v, T* scratch) {

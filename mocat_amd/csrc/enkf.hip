@@ -14,7 +14,9 @@
 // (one workgroup: Chan's merge of the chunk means and centred second moments in FP64, K by a Cholesky
 // solve) and enkf_update (step t's analysis, then step t+1's forecast and chunk moments). Chunk
 // moments are two-pass within the chunk (mean, then centred products of the register-held forecasts),
-// so the covariance carries no cancellation from raw second moments.
+// so the covariance carries no cancellation from raw second moments. The noise is the bulk of the
+// arithmetic (d + dy threefry blocks per member and step); for even n a thread draws the rows i and
+// i + n/2 together, which halves it.
 #include "common.hpp"
 
 using namespace mocat;
@@ -65,11 +67,65 @@ struct EnkfArgs {
   float* out_gain;   // [T-1][D][DY] or NULL
 };
 
-// Row `row` of normal(k, (n, K)) for S = n * K elements (jax's halves layout).
-template <int K>
-MOCAT_DEV void normal_row(Key k, uint32_t S, uint32_t row, float* z) {
+// Particles of a thread. threefry_2x32 over S = n * K counters returns elements m and m + S/2 from one
+// call; for even n the element m + S/2 of row i is the same column of row i + n/2, so a thread takes
+// the rows b and b + n/2 together (EPPT/2 such pairs) and one call yields both rows' draws. For odd n
+// each row is drawn on its own. A chunk holds ECH rows either way.
+MOCAT_DEV void slot_rows(int64_t n, int64_t (&idx)[EPPT], bool (&live)[EPPT]) {
+  const int64_t c = blockIdx.x;
+  if ((n & 1) == 0) {
+    const int64_t half = n >> 1;
 #pragma unroll
-  for (int q = 0; q < K; ++q) z[q] = normal_from_bits(bits_at(k, S, row * (uint32_t)K + (uint32_t)q));
+    for (int pk = 0; pk < EPPT; ++pk) {
+      const int64_t b = c * (ECH / 2) + (pk >> 1) * EB + threadIdx.x;
+      idx[pk] = b + (pk & 1) * half;
+      live[pk] = b < half;
+    }
+  } else {
+#pragma unroll
+    for (int pk = 0; pk < EPPT; ++pk) {
+      idx[pk] = c * ECH + pk * EB + threadIdx.x;
+      live[pk] = idx[pk] < n;
+    }
+  }
+}
+
+MOCAT_DEV int64_t chunk_rows(int64_t n) {
+  const int64_t c = blockIdx.x;
+  if ((n & 1) == 0) {
+    const int64_t r = (n >> 1) - c * (ECH / 2);
+    return 2 * (r < ECH / 2 ? r : ECH / 2);
+  }
+  const int64_t r = n - c * ECH;
+  return r < ECH ? r : ECH;
+}
+
+// Rows idx[] of normal(k, (n, K)) (jax's halves layout over S = n * K elements).
+template <int K>
+MOCAT_DEV void normal_rows(Key k, int64_t n, const int64_t (&idx)[EPPT], const bool (&live)[EPPT],
+                           float (&z)[EPPT][K]) {
+  const uint32_t S = (uint32_t)(n * K);
+  if ((n & 1) == 0) {
+#pragma unroll
+    for (int pk = 0; pk < EPPT; pk += 2) {
+      if (!live[pk]) continue;
+#pragma unroll
+      for (int q = 0; q < K; ++q) {
+        uint32_t lo, hi;
+        bits_pair(k, S, (uint32_t)idx[pk] * (uint32_t)K + (uint32_t)q, lo, hi);
+        z[pk][q] = normal_from_bits(lo);
+        z[pk + 1][q] = normal_from_bits(hi);
+      }
+    }
+  } else {
+#pragma unroll
+    for (int pk = 0; pk < EPPT; ++pk) {
+      if (!live[pk]) continue;
+#pragma unroll
+      for (int q = 0; q < K; ++q)
+        z[pk][q] = normal_from_bits(bits_at(k, S, (uint32_t)idx[pk] * (uint32_t)K + (uint32_t)q));
+    }
+  }
 }
 
 // Workgroup sum of NV doubles per thread, fixed order; every thread gets the totals.
@@ -106,23 +162,21 @@ MOCAT_DEV void forecast_moments(const EnkfArgs& a, const EnkfModel<D, DY>& M, co
                                 const bool (&live)[EPPT], const int64_t (&idx)[EPPT], Key k1) {
   constexpr int NM = D * (D + 1) / 2;
   __shared__ double shm[EW][NM > D ? NM : D];
-  const uint32_t S1 = (uint32_t)(a.n * D);
-  float mx[EPPT][D];
+  float mx[EPPT][D], z[EPPT][D];
   double s[D];
 #pragma unroll
   for (int q = 0; q < D; ++q) s[q] = 0.0;
+  normal_rows<D>(k1, a.n, idx, live, z);
 #pragma unroll
   for (int pk = 0; pk < EPPT; ++pk) {
     if (!live[pk]) continue;
-    float z[D];
-    normal_row<D>(k1, S1, (uint32_t)idx[pk], z);
 #pragma unroll
     for (int i = 0; i < D; ++i) {
       float u = 0.f, v = 0.f;
 #pragma unroll
       for (int k = 0; k < D; ++k) {
         u += M.A[i * D + k] * x[pk][k];
-        v += M.LQ[i * D + k] * z[k];
+        v += M.LQ[i * D + k] * z[pk][k];
       }
       mx[pk][i] = u + v;
       a.mx[idx[pk] * D + i] = mx[pk][i];
@@ -130,7 +184,7 @@ MOCAT_DEV void forecast_moments(const EnkfArgs& a, const EnkfModel<D, DY>& M, co
     }
   }
   const int64_t c = blockIdx.x;
-  const int64_t cnt = (a.n - c * ECH) < ECH ? (a.n - c * ECH) : ECH;
+  const int64_t cnt = chunk_rows(a.n);
   block_sum_vec<D>(s, reinterpret_cast<double (*)[D]>(&shm[0][0]));
   double mean[D];
 #pragma unroll
@@ -166,8 +220,8 @@ __global__ __launch_bounds__(EB) void enkf_init_kernel(EnkfArgs a) {
   __shared__ EnkfModel<D, DY> M;
   __shared__ uint32_t sk[4];
   const int tid = threadIdx.x;
+  load_shared_params(M, a.params);
   if (tid == 0) {
-    M.load(a.params);
     const Key root{a.key[0], a.key[1]};
     const Key k0 = split_at(split_at(root, 2, 1), (uint32_t)a.n, 0u);
     sk[0] = k0.a;
@@ -199,23 +253,19 @@ __global__ __launch_bounds__(EB) void enkf_init_kernel(EnkfArgs a) {
       mc[i] = M.m0[i] + g;
     }
   }
-  const Key k0{sk[0], sk[1]};
-  const uint32_t S0 = (uint32_t)(a.n * D);
-  float x[EPPT][D];
+  float x[EPPT][D], z[EPPT][D];
   bool live[EPPT];
   int64_t idx[EPPT];
+  slot_rows(a.n, idx, live);
+  normal_rows<D>(Key{sk[0], sk[1]}, a.n, idx, live, z);
 #pragma unroll
   for (int pk = 0; pk < EPPT; ++pk) {
-    idx[pk] = (int64_t)blockIdx.x * ECH + pk * EB + tid;
-    live[pk] = idx[pk] < a.n;
     if (!live[pk]) continue;
-    float z[D];
-    normal_row<D>(k0, S0, (uint32_t)idx[pk], z);
 #pragma unroll
     for (int i = 0; i < D; ++i) {
       float c = 0.f;
 #pragma unroll
-      for (int k = 0; k < D; ++k) c += z[k] * M.C0[k * D + i];
+      for (int k = 0; k < D; ++k) c += z[pk][k] * M.C0[k * D + i];
       x[pk][i] = mc[i] + c;
       if (a.out_value) a.out_value[idx[pk] * D + i] = x[pk][i];
       if (a.T == 1) a.x[idx[pk] * D + i] = x[pk][i];
@@ -229,7 +279,11 @@ template <int D, int DY>
 __global__ __launch_bounds__(EB) void enkf_gain_kernel(EnkfArgs a, int64_t t) {
   constexpr int NM = D * (D + 1) / 2, NS = n_moments<D>();
   __shared__ double shm[EW][NM > D ? NM : D];
+  __shared__ float H[DY * D], LR[DY * DY];
   const int tid = threadIdx.x;
+  // H and LR staged by the workgroup (the reductions below synchronise before thread 0 reads them)
+  for (int i = tid; i < DY * D; i += EB) H[i] = a.params[D + i];
+  for (int i = tid; i < DY * DY; i += EB) LR[i] = a.params[D + 2 * DY * D + 3 * D * D + i];
   double s[D];
 #pragma unroll
   for (int q = 0; q < D; ++q) s[q] = 0.0;
@@ -258,8 +312,6 @@ __global__ __launch_bounds__(EB) void enkf_gain_kernel(EnkfArgs a, int64_t t) {
   }
   block_sum_vec<NM>(m2, reinterpret_cast<double (*)[NM]>(&shm[0][0]));
   if (tid != 0) return;
-  const float* H = a.params + D;
-  const float* LR = a.params + D + 2 * DY * D + 3 * D * D;
   double P[D][D];
   {
     int k = 0;
@@ -315,8 +367,8 @@ __global__ __launch_bounds__(EB) void enkf_update_kernel(EnkfArgs a, int64_t t) 
   __shared__ uint32_t sk[4];
   const int tid = threadIdx.x;
   const bool last = t + 1 >= a.T;
+  load_shared_params(M, a.params);
   if (tid == 0) {
-    M.load(a.params);
     Key k1, k2;
     step_keys(a.key, a.T, a.n, t, k1, k2);
     sk[0] = k2.a;
@@ -329,28 +381,29 @@ __global__ __launch_bounds__(EB) void enkf_update_kernel(EnkfArgs a, int64_t t) 
   }
   if (tid < D * DY) K[tid] = a.gain[tid];
   __syncthreads();
-  const Key k2{sk[0], sk[1]};
-  const uint32_t S2 = (uint32_t)(a.n * DY);
   const float* yt = a.y + t * DY;
-  float x[EPPT][D];
+  float x[EPPT][D], z[EPPT][DY];
   bool live[EPPT];
   int64_t idx[EPPT];
+  slot_rows(a.n, idx, live);
+  float mxa[EPPT][D];
+#pragma unroll
+  for (int pk = 0; pk < EPPT; ++pk)
+#pragma unroll
+    for (int q = 0; q < D; ++q) mxa[pk][q] = live[pk] ? a.mx[idx[pk] * D + q] : 0.f;
+  normal_rows<DY>(Key{sk[0], sk[1]}, a.n, idx, live, z);
 #pragma unroll
   for (int pk = 0; pk < EPPT; ++pk) {
-    idx[pk] = (int64_t)blockIdx.x * ECH + pk * EB + tid;
-    live[pk] = idx[pk] < a.n;
     if (!live[pk]) continue;
-    float mx[D], z[DY], r[DY];
-#pragma unroll
-    for (int q = 0; q < D; ++q) mx[q] = a.mx[idx[pk] * D + q];
-    normal_row<DY>(k2, S2, (uint32_t)idx[pk], z);
+    const float* mx = mxa[pk];
+    float r[DY];
 #pragma unroll
     for (int j = 0; j < DY; ++j) {
       float hx = 0.f, e = 0.f;
 #pragma unroll
       for (int k = 0; k < D; ++k) hx += M.H[j * D + k] * mx[k];
 #pragma unroll
-      for (int k = 0; k < DY; ++k) e += M.LR[j * DY + k] * z[k];
+      for (int k = 0; k < DY; ++k) e += M.LR[j * DY + k] * z[pk][k];
       r[j] = yt[j] - (hx + e);
     }
 #pragma unroll

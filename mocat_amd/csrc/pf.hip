@@ -538,7 +538,7 @@ __global__ __launch_bounds__(NT) void pf_kernel(PfArgs a) {
     if (F.flags & kPfLarge) continue;
     const int64_t n = F.n, T = F.T;
     __syncthreads();
-    if (tid == 0) model.load(a.params + F.param_offset);
+    load_shared_params(model, a.params + F.param_offset);
     __syncthreads();
     float* xa = a.x0 + F.state_offset * D;
     float* xb = a.x1 + F.state_offset * D;
@@ -963,7 +963,7 @@ __global__ __launch_bounds__(LT) void lf_init_kernel(LfArgs a) {
   const int64_t c = blockIdx.x;
   const LfDesc F = a.desc[a.chunk_filter[c]];
   const int tid = threadIdx.x;
-  if (tid == 0) model.load(a.params + F.param_offset);
+  load_shared_params(model, a.params + F.param_offset);
   __syncthreads();
   const int64_t n = F.n;
   const Key key{F.key[0], F.key[1]};
@@ -1255,8 +1255,8 @@ __global__ __launch_bounds__(LT) void lf_step_kernel(LfArgs a, int64_t t, int ce
   // the step key (and the systematic uniform) are the same for the whole chunk: derived once
   __shared__ uint32_t sk[4];
   __shared__ float su;
+  load_shared_params(model, a.params + F.param_offset);
   if (tid == 0) {
-    model.load(a.params + F.param_offset);
     const Key key{F.key[0], F.key[1]};
     const Key ikk = split_at(split_at(key, 2, 0), (uint32_t)(F.T - 1), (uint32_t)(t - 1));
     sk[0] = ikk.a;

@@ -18,7 +18,7 @@ filter, except that both filters draw all particles' noise from the FIRST key of
 Initial log-weights are zero for both (:211-224, 312-324); the EnKF's stay zero (ESS = n, no
 resampling at any ess_threshold <= 1).
 
-Pinning (tests/test_oracle_nlg.py): the optimal filter's step satisfies, for diagonal P0 and R,
+Pinning (tests/test_oracle_nlg.py): the optimal filter's step satisfies, for any SPD Q and R,
   log q(x' | x, y) + w(x, y) = log f(x' | x) + log g(y | x')
 with q, f, g independent Gaussian log-densities (a wrong gain, proposal covariance, weight precision
 or normaliser breaks it); the startup matrices equal the conditioning formulas written in
