@@ -794,11 +794,12 @@ __global__ __launch_bounds__(256) void categorical_kernel(const float* __restric
 
 // ---------------------------------------------------------------- large filters: grid-wide launches per step
 // One workgroup per filter leaves a large filter running long after the others have finished. Filters
-// flagged kPfLarge instead advance one time step per sequence of four launches over 256-particle
+// flagged kPfLarge instead advance one time step per sequence of three launches over 1024-particle
 // chunks (all flagged filters batched in every launch):
-//   lf_stats   one workgroup per filter: combine the chunk accumulators of step t in chunk order,
-//              log-likelihood and ESS outputs, the resampling decision and weight shift for step t+1;
-//   lf_totals  per chunk: sum of the quantised weights (resampling steps only);
+//   lf_prep    per chunk: combine the chunk accumulators of step t-1 in chunk order (every chunk of
+//              the filter alike; its first chunk writes the log-likelihood, ESS and filter state),
+//              the resampling decision and weight shift for step t, and the chunk's sum of quantised
+//              weights (resampling steps only);
 //   lf_csum    per chunk: offset = sum of the earlier chunks' totals, plus an inclusive block scan;
 //   lf_step    per chunk: ancestor (binary search in the exact prefix sums, or Gumbel-max), then
 //              propagate, weight, outputs and the chunk accumulators of step t+1.
@@ -1028,41 +1029,38 @@ __global__ __launch_bounds__(LT) void lf_init_kernel(LfArgs a) {
   lf_store_partials<GRAD>(a, c, acc, gacc, s);
 }
 
-// Step t's statistics (t = 0 after lf_init, else after lf_step(t)).
+MOCAT_DEV const float* lf_lw_prev(const LfArgs& a, int64_t t) { return ((t - 1) & 1) ? a.lw1 : a.lw0; }
+
+// One launch per step t >= 1 (and one after the last step):
+//  * the statistics of step t - 1 (LSE of its weights, ESS, log-likelihood, next resampling decision
+//    and weight shifts), reduced from the chunk partials in a fixed order by every chunk of the filter
+//    (identical results in every workgroup); the filter's first chunk also updates the filter state
+//    and writes the per-step outputs. The decision and shifts do not depend on the previous state, so
+//    no workgroup reads what another writes in this launch;
+//  * for step t, when it resamples, the chunk's total of quantised weights.
 template <bool GRAD>
-__global__ __launch_bounds__(LT) void lf_stats_kernel(LfArgs a, int64_t t) {
+__global__ __launch_bounds__(LT) void lf_prep_kernel(LfArgs a, int64_t t) {
   __shared__ LfScratch s;
-  const int fi = blockIdx.x;
+  const int64_t c = blockIdx.x;
+  const int fi = a.chunk_filter[c];
   const LfDesc F = a.desc[fi];
-  if (t >= F.T) return;
+  if (t - 1 >= F.T) return;
   const int64_t n = F.n, nchunk = (n + CH - 1) / CH;
   LseAcc acc{-INFINITY, 0.0, 0.0};
   double g[3] = {0.0, 0.0, 0.0};
   for (int64_t k = threadIdx.x; k < nchunk; k += LT) {
-    const int64_t c = F.chunk0 + k;
-    const LseAcc p{a.part_m[c], a.part_s1[c], a.part_s2[c]};
+    const int64_t cc = F.chunk0 + k;
+    const LseAcc p{a.part_m[cc], a.part_s1[cc], a.part_s2[cc]};
     if constexpr (GRAD) {
       // the tangent sums follow the LSE rescaling of their chunk
       const float mnew = fmaxf(acc.m, p.m);
       const double fo = acc.m == -INFINITY ? 0.0 : exp((double)acc.m - (double)mnew);
       const double fp = p.m == -INFINITY ? 0.0 : exp((double)p.m - (double)mnew);
-      for (int q = 0; q < 3; ++q) g[q] = g[q] * fo + a.part_g[c * 3 + q] * fp;
+      for (int q = 0; q < 3; ++q) g[q] = g[q] * fo + a.part_g[cc * 3 + q] * fp;
     }
     acc = lse_combine(acc, p);
   }
   const LseAcc r = lf_block_lse(acc, s);
-  double gm[3] = {0.0, 0.0, 0.0};
-  if constexpr (GRAD) {
-    // bring every thread's tangent sums to the block max, then add in thread order
-    const double f = acc.m == -INFINITY ? 0.0 : exp((double)acc.m - (double)r.m);
-    __shared__ double gs[LT][3];
-    for (int q = 0; q < 3; ++q) gs[threadIdx.x][q] = g[q] * f;
-    __syncthreads();
-    for (int k = 0; k < LT; ++k)
-      for (int q = 0; q < 3; ++q) gm[q] += gs[k][q];
-    for (int q = 0; q < 3; ++q) gm[q] /= r.s1;
-  }
-  if (threadIdx.x != 0) return;
   double lse, lse2;
   if (isnan(r.m)) {
     lse = NAN;
@@ -1074,48 +1072,56 @@ __global__ __launch_bounds__(LT) void lf_stats_kernel(LfArgs a, int64_t t) {
     lse = (double)r.m + log(r.s1);
     lse2 = 2.0 * (double)r.m + log(r.s2);
   }
-  LfState st = a.st[fi];
   const double logn = log((double)n);
-  st.loglik = t == 0 ? lse - logn : st.loglik + lse - st.lse_res;
   const double ess = exp(2.0 * lse - lse2);
-  if (a.out_ess) a.out_ess[F.t_offset + t] = (float)ess;
-  if (a.out_loglik_t) a.out_loglik_t[F.t_offset + t] = st.loglik;
-  if constexpr (GRAD) {
-    for (int q = 0; q < 3; ++q) {
-      st.gtot[q] = t == 0 ? gm[q] : st.gtot[q] + gm[q] - (st.res ? 0.0 : st.gprev[q]);
-      st.gprev[q] = gm[q];
+  // decision and weight shift for step t (pf_kernel: m = float(lse), shift from the largest weight
+  // exp(lw - m), i.e. exp(max lw - m): the hardware exponential is monotone)
+  const int res = ess < (double)a.ess_threshold * (double)n ? 1 : 0;
+  const float m = (float)lse;
+  const int shift = quant_shift(__expf(r.m - m), n);
+  if (c == F.chunk0) {
+    double gm[3] = {0.0, 0.0, 0.0};
+    if constexpr (GRAD) {
+      // bring every thread's tangent sums to the block max, then add in thread order
+      const double f = acc.m == -INFINITY ? 0.0 : exp((double)acc.m - (double)r.m);
+      __shared__ double gs[LT][3];
+      for (int q = 0; q < 3; ++q) gs[threadIdx.x][q] = g[q] * f;
+      __syncthreads();
+      for (int k = 0; k < LT; ++k)
+        for (int q = 0; q < 3; ++q) gm[q] += gs[k][q];
+      for (int q = 0; q < 3; ++q) gm[q] /= r.s1;
+    }
+    if (threadIdx.x == 0) {
+      const int64_t tp = t - 1;
+      LfState st = a.st[fi];
+      st.loglik = tp == 0 ? lse - logn : st.loglik + lse - st.lse_res;
+      if (a.out_ess) a.out_ess[F.t_offset + tp] = (float)ess;
+      if (a.out_loglik_t) a.out_loglik_t[F.t_offset + tp] = st.loglik;
+      if constexpr (GRAD) {
+        for (int q = 0; q < 3; ++q) {
+          st.gtot[q] = tp == 0 ? gm[q] : st.gtot[q] + gm[q] - (st.res ? 0.0 : st.gprev[q]);
+          st.gprev[q] = gm[q];
+        }
+      }
+      st.res = res;
+      st.lse_res = res ? logn : lse;
+      st.m = m;
+      st.shift = shift;
+      a.st[fi] = st;
+      if (tp == F.T - 1) {
+        a.out_loglik[F.f] = st.loglik;
+        if constexpr (GRAD)
+          for (int q = 0; q < 3; ++q) a.out_grad[(int64_t)F.f * 3 + q] = st.gtot[q];
+      }
     }
   }
-  // decision and weight shift for step t + 1 (pf_kernel: m = float(lse), shift from the largest
-  // weight exp(lw - m), i.e. exp(max lw - m): the hardware exponential is monotone)
-  st.res = ess < (double)a.ess_threshold * (double)n ? 1 : 0;
-  st.lse_res = st.res ? logn : lse;
-  st.m = (float)lse;
-  st.shift = quant_shift(__expf(r.m - st.m), n);
-  a.st[fi] = st;
-  if (t == F.T - 1) {
-    a.out_loglik[F.f] = st.loglik;
-    if constexpr (GRAD)
-      for (int q = 0; q < 3; ++q) a.out_grad[(int64_t)F.f * 3 + q] = st.gtot[q];
-  }
-}
-
-MOCAT_DEV const float* lf_lw_prev(const LfArgs& a, int64_t t) { return ((t - 1) & 1) ? a.lw1 : a.lw0; }
-
-__global__ __launch_bounds__(LT) void lf_totals_kernel(LfArgs a, int64_t t) {
-  __shared__ LfScratch s;
-  const int64_t c = blockIdx.x;
-  const int fi = a.chunk_filter[c];
-  const LfDesc F = a.desc[fi];
-  if (t >= F.T) return;
-  const LfState st = a.st[fi];
-  if (!st.res || a.resampling == 2) return;
+  if (t >= F.T || !res || a.resampling == 2) return;
   const float* lw = lf_lw_prev(a, t) + F.state_offset;
   uint64_t q = 0;
 #pragma unroll
   for (int pk = 0; pk < PPT; ++pk) {
     const int64_t i = (c - F.chunk0) * CH + pk * LT + threadIdx.x;
-    if (i < F.n) q += quantise(__expf(lw[i] - st.m), st.shift);
+    if (i < n) q += quantise(__expf(lw[i] - m), shift);
   }
   const uint64_t tot = lf_block_sum(q, s);
   if (threadIdx.x == 0) a.ctot[c] = tot;
@@ -1463,16 +1469,15 @@ __global__ __launch_bounds__(LT) void lf_finish_kernel(LfArgs a) {
 
 template <class Model, int D, int DY, bool GRAD = false>
 int launch_pf_large(const LfArgs& a, int64_t max_T, int cend_slots, hipStream_t st) {
-  const dim3 chunks((unsigned)a.n_chunks), filters((unsigned)a.n_large), blk(LT);
+  const dim3 chunks((unsigned)a.n_chunks), blk(LT);
   const size_t cend_bytes = (size_t)cend_slots * sizeof(uint64_t);
   hipLaunchKernelGGL((lf_init_kernel<Model, D, DY, GRAD>), chunks, blk, 0, st, a);
-  hipLaunchKernelGGL((lf_stats_kernel<GRAD>), filters, blk, 0, st, a, (int64_t)0);
   for (int64_t t = 1; t < max_T; ++t) {
-    hipLaunchKernelGGL(lf_totals_kernel, chunks, blk, 0, st, a, t);
+    hipLaunchKernelGGL((lf_prep_kernel<GRAD>), chunks, blk, 0, st, a, t);
     hipLaunchKernelGGL(lf_csum_kernel, chunks, blk, 0, st, a, t);
     hipLaunchKernelGGL((lf_step_kernel<Model, D, DY, GRAD>), chunks, blk, cend_bytes, st, a, t, cend_slots);
-    hipLaunchKernelGGL((lf_stats_kernel<GRAD>), filters, blk, 0, st, a, t);
   }
+  hipLaunchKernelGGL((lf_prep_kernel<GRAD>), chunks, blk, 0, st, a, max_T);  // the last step's statistics
   hipLaunchKernelGGL((lf_finish_kernel<D>), chunks, blk, 0, st, a);
   MOCAT_LAUNCH_CHECK();
   return MOCAT_OK;
