@@ -156,6 +156,23 @@ def _timed(fn):
     return out, time.perf_counter() - t0
 
 
+def _host_threads():
+    try:
+        from threadpoolctl import threadpool_info
+        return int(max([i.get("num_threads", 1) for i in threadpool_info()] or [1]))
+    except Exception:
+        return len(os.sched_getaffinity(0))
+
+
+def _cpu_sample(fn, units, unit, sample):
+    """Times fn() (the NumPy oracle on a bounded sample of the workload) on the host cores."""
+    t0 = time.perf_counter()
+    fn()
+    dt = time.perf_counter() - t0
+    return {"value": units / dt, "unit": unit, "cores": _host_threads(), "kind": "port",
+            "sample": f"{sample} ({dt:.1f} s)"}
+
+
 def bench_pf1():
     """Linear-Gaussian SSM d = 4, T = 100, N = 4096, systematic resampling every step; also 256
     independent filters in one launch. Unit: particle-steps/s."""
@@ -179,7 +196,14 @@ def bench_pf1():
     runs()
     _, dts = _timed(runs)
     steps = big * 100
-    print(json.dumps({"mode": "pf1", "metric": "particle-steps/s", "single_filter": 4096 * 100 / dt,
+    from oracle import pf as opf, rng as orng
+    p = opf.lgssm_params(model.initial_mean, model.initial_covariance, model.transition_matrix,
+                         model.transition_covariance, model.likelihood_matrix, model.likelihood_covariance)
+    cpu = _cpu_sample(lambda: opf.run_pf(p, yn, orng.PRNGKey(0), 4096, 1.0, "systematic"), 4096 * 100,
+                      "particle-steps/s", "oracle bootstrap filter on the whole workload: 4096 particles x 100 steps, "
+                      "systematic resampling every step")
+    print(json.dumps({"mode": "pf1", "metric": "particle-steps/s", "cpu_baseline": cpu,
+                      "single_filter": 4096 * 100 / dt,
                       "single_filter_ms": dt * 1e3, "batch256": 256 * 4096 * 100 / dtb, "batch256_ms": dtb * 1e3,
                       "log_likelihood": out.log_likelihood.item(), "store_trajectory": True,
                       "scaled_n2e20": {"ms": dts * 1e3, "particle_steps_per_s": steps / dts,
@@ -202,7 +226,27 @@ def bench_pf5():
     rung = lambda: m.ssm.run_particle_filters(models, ys, [k for k in keys], ns, 1.0, "systematic", grad=True)
     rung()
     _, dtg = _timed(rung)
-    print(json.dumps({"mode": "pf5", "metric": "particle-steps/s", "value": E / dt, "ms": dt * 1e3,
+    import numpy as np
+    from oracle import pf as opf, rng as orng
+    work = [n * len(y) for n, y in zip(ns, ys)]
+    order = list(np.argsort(work))
+    pick, tot = [], 0
+    for i in order[len(order) // 2:]:  # filters from the median size up, until ~6e6 particle-steps
+        pick.append(int(i))
+        tot += work[i]
+        if tot >= 6e6:
+            break
+
+    def cpu_run():
+        for i in pick:
+            md = models[i]
+            q = opf.sv_params(md.phi, md.sigma, md.beta, md.initial_mean, md.initial_covariance)
+            opf.run_pf(q, np.asarray(ys[i]), orng.PRNGKey(i), ns[i], 1.0, "systematic")
+
+    cpu = _cpu_sample(cpu_run, tot, "particle-steps/s",
+                      f"oracle SV filters {pick} of the batch ({tot} particle-steps, from the median filter size up)")
+    print(json.dumps({"mode": "pf5", "metric": "particle-steps/s", "cpu_baseline": cpu, "value": E / dt,
+                      "ms": dt * 1e3,
                       "with_gradient": E / dtg, "with_gradient_ms": dtg * 1e3,
                       "particle_steps": E, "largest_filter_particle_steps": big,
                       "bytes_per_particle_step": 24, "achieved_GBps_algorithmic": 24 * E / dt / 1e9}))
@@ -221,7 +265,23 @@ def bench_smc3(n_particles=65536):
     chain, dt = _timed(run)
     n_temp = int(chain.temperature.shape[0]) - 1
     evals = 6 * n_particles * n_temp
-    print(json.dumps({"mode": "smc3", "metric": "particle-moves/s", "value": 5 * n_particles * n_temp / dt,
+    import numpy as np
+    from oracle import mcmc as omcmc, rng as orng
+    tonp = lambda a: (a.cpu().numpy() if hasattr(a, "cpu") else np.asarray(a)).astype(np.float64)
+    Xn, yn = tonp(X), tonp(y)
+    pg = omcmc.lr_pot_grad_fn(Xn, yn, 0.1)
+    st, keys = omcmc.startup(orng.PRNGKey(0), 1024, 100, lambda k: np.sqrt(10.0) * orng.normal(k, (100,)), pg)
+
+    def cpu_run():
+        nonlocal st, keys
+        for it in range(1, 11):
+            st, keys, _, _ = omcmc.iterate(st, keys, it, pg, 1, np.inf, "rm", 0.651)
+
+    cpu = _cpu_sample(cpu_run, 1024 * 10, "particle-moves/s",
+                      "oracle RM-MALA moves of 1024 particles x 10 on the same X (20000 x 100); temperature search and "
+                      "resampling not included")
+    print(json.dumps({"mode": "smc3", "metric": "particle-moves/s", "cpu_baseline": cpu,
+                      "value": 5 * n_particles * n_temp / dt,
                       "seconds": dt, "temperatures": n_temp, "log_evidence": chain.log_norm_constant[-1].item(),
                       "gradient_evals_per_s": evals / dt, "algorithmic_TFLOPs": evals * 4 * 20000 * 100 / dt / 1e12}))
 
@@ -241,7 +301,22 @@ def bench_svgd4(n=4096, d=100, iters=500):
     chain, dt = _timed(lambda: m.run(sc, smp, n, m.random.PRNGKey(0)))
     flop = 6.0 * n * n * d * iters
     quality = svgd_quality(m, sc, smp, chain)
-    print(json.dumps({"mode": "svgd4", "metric": "iterations/s", "value": iters / dt, "seconds": dt,
+    from oracle import svgd as osvgd
+    nc = 1024
+    xc = rs.normal(0, 1, (nc, d))
+    gsq, mom = np.zeros((nc, d)), np.zeros((nc, d))
+
+    def cpu_run():
+        nonlocal xc, gsq, mom
+        for _ in range(8):
+            _, g = osvgd.gmm_potential_grad(xc, means, var, w)
+            h = osvgd.median_bandwidth(xc)
+            xc, gsq, mom = osvgd.adagrad(xc, gsq, mom, -osvgd.phi_gemm(xc, g, h), 1e-2)
+
+    cpu = _cpu_sample(cpu_run, 8, "iterations/s", f"oracle SVGD, {nc} particles x 8 iterations (median bandwidth, "
+                      "mixture gradient, Stein direction, adagrad); the workload's 4096 particles cost ~16x per iteration")
+    print(json.dumps({"mode": "svgd4", "metric": "iterations/s", "cpu_baseline": cpu, "value": iters / dt,
+                      "seconds": dt,
                       "stein_algorithmic_TFLOPs_incl_bandwidth_and_potential": flop / dt / 1e12,
                       "final_bandwidth": chain.bandwidth[-1].item(), **quality}))
 
@@ -290,7 +365,16 @@ def bench_ffbsi(n=4096, T=100):
     runr()
     rr, dtr = _timed(runr)
     pairs = (T - 1) * n * n
-    print(json.dumps({"mode": "ffbsi", "metric": "pair evaluations/s", "value": pairs / dt, "ms": dt * 1e3,
+    from oracle import ffbsi as offbsi, rng as orng
+    nc, tc = 1024, 10
+    vals = marg.value[:tc, :nc].double().cpu().numpy()
+    lws = marg.log_weight[:tc, :nc].double().cpu().numpy()
+    pot = offbsi.lgssm_transition(model.transition_matrix, model.transition_covariance)
+    cpu = _cpu_sample(lambda: offbsi.backward_simulation_full(pot, vals, lws, orng.PRNGKey(1), nc),
+                      (tc - 1) * nc * nc, "pair evaluations/s",
+                      f"oracle full backward simulation, {nc} particles x {nc} trajectories x {tc} steps")
+    print(json.dumps({"mode": "ffbsi", "metric": "pair evaluations/s", "cpu_baseline": cpu, "value": pairs / dt,
+                      "ms": dt * 1e3,
                       "particles": n, "trajectories": n, "T": T, "ms_per_backward_step": dt * 1e3 / (T - 1),
                       "note": "per pair: one transition potential (d = 4) and half a threefry block",
                       "rejection_32": {"ms": dtr * 1e3, "ms_per_backward_step": dtr * 1e3 / (T - 1),
