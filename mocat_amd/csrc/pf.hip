@@ -1082,12 +1082,15 @@ __global__ __launch_bounds__(LT) void lf_prep_kernel(LfArgs a, int64_t t) {
   if (c == F.chunk0) {
     double gm[3] = {0.0, 0.0, 0.0};
     if constexpr (GRAD) {
-      // bring every thread's tangent sums to the block max, then add in thread order
+      // bring every thread's tangent sums to the block max, then add: wave butterflies, waves in order
       const double f = acc.m == -INFINITY ? 0.0 : exp((double)acc.m - (double)r.m);
-      __shared__ double gs[LT][3];
-      for (int q = 0; q < 3; ++q) gs[threadIdx.x][q] = g[q] * f;
+      __shared__ double gs[LW][3];
+      double v[3];
+      for (int q = 0; q < 3; ++q) v[q] = wave_sum(g[q] * f);
+      if ((threadIdx.x & (kWave - 1)) == 0)
+        for (int q = 0; q < 3; ++q) gs[threadIdx.x / kWave][q] = v[q];
       __syncthreads();
-      for (int k = 0; k < LT; ++k)
+      for (int k = 0; k < LW; ++k)
         for (int q = 0; q < 3; ++q) gm[q] += gs[k][q];
       for (int q = 0; q < 3; ++q) gm[q] /= r.s1;
     }
@@ -1182,8 +1185,12 @@ MOCAT_DEV void stage_lds(uint64_t* dst, int64_t count, Src src) {
 // For P searches at once: out[p] = the first k in [b[p], b[p] + m[p]) with v(k) > tg[p] (v
 // non-decreasing), or b[p] + m[p] when there is none; inactive searches return b[p]. Each halving
 // step issues the reads of all unfinished searches before comparing.
+// The prefix sums are integers below 2^52 (exact in FP64), so for a target T >= 0 the test
+// (double)C <= T is C <= floor(T): targets are converted once and every level compares integers.
+MOCAT_DEV uint64_t target_floor(double t) { return t > 0.0 ? (uint64_t)floor(t) : 0ull; }
+
 template <int P, class Get>
-MOCAT_DEV void upper_bound_lockstep(Get v, const int64_t* b, const int64_t* m, const double* tg, const bool* act,
+MOCAT_DEV void upper_bound_lockstep(Get v, const int64_t* b, const int64_t* m, const uint64_t* tg, const bool* act,
                                     int64_t* out) {
   int64_t lo[P], len[P];
 #pragma unroll
@@ -1203,7 +1210,7 @@ MOCAT_DEV void upper_bound_lockstep(Get v, const int64_t* b, const int64_t* m, c
     for (int q = 0; q < P; ++q) {
       if (len[q] <= 0) continue;
       const int64_t h = len[q] >> 1;
-      if ((double)val[q] <= tg[q]) {
+      if (val[q] <= tg[q]) {
         lo[q] += h + 1;
         len[q] -= h + 1;
       } else {
@@ -1281,7 +1288,8 @@ __global__ __launch_bounds__(LT) void lf_step_kernel(LfArgs a, int64_t t, int ce
     if (tid == 0) {
       const int64_t j1 = (base + CH < n ? base + CH : n) - 1;
       const int64_t b0[2] = {0, 0}, m0[2] = {nsg, nsg};
-      const double tg[2] = {((double)base + (double)su) * sstep, ((double)j1 + (double)su) * sstep};
+      const uint64_t tg[2] = {target_floor(((double)base + (double)su) * sstep),
+                              target_floor(((double)j1 + (double)su) * sstep)};
       const bool on[2] = {true, true};
       int64_t k[2];
       upper_bound_lockstep<2>(cend_at, b0, m0, tg, on, k);
@@ -1312,13 +1320,13 @@ __global__ __launch_bounds__(LT) void lf_step_kernel(LfArgs a, int64_t t, int ce
     for (int pk = 0; pk < PPT; ++pk)
       if (live[pk]) srcs[pk] = categorical_row(lwa, n, ik, (uint32_t)(n * n), srcs[pk]);
   } else if (res) {
-    double tg[PPT];
+    uint64_t tg[PPT];
 #pragma unroll
     for (int pk = 0; pk < PPT; ++pk) {
       const int64_t i = srcs[pk];
-      tg[pk] = a.resampling == 0
-                   ? ((double)i + (double)su) * sstep
-                   : (double)uniform_from_bits(bits_at(ik, (uint32_t)n, (uint32_t)i), 0.f, 1.f) * W;
+      tg[pk] = target_floor(a.resampling == 0
+                                ? ((double)i + (double)su) * sstep
+                                : (double)uniform_from_bits(bits_at(ik, (uint32_t)n, (uint32_t)i), 0.f, 1.f) * W);
     }
     if (two_level) {
       // first SG block whose last sum exceeds the target (nsg: none does), then the first index in it
