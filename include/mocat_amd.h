@@ -114,10 +114,16 @@ typedef struct {
   int64_t t_offset;      /* steps into out_ess / out_loglik_t */
   uint32_t key[2];       /* random_key of the filter */
   int32_t param_offset;  /* floats into params */
-  int32_t flags;         /* bit 0 (MOCAT_PF_LARGE): run by mocat_pf_run_large, skipped by mocat_pf_run */
+  int32_t flags;         /* bit 0 (MOCAT_PF_LARGE): run by mocat_pf_run_large, skipped by mocat_pf_run;
+                            bit 1 (MOCAT_PF_RESUME): resume from a given sample, see below */
 } mocat_pf_filter;
 
 #define MOCAT_PF_LARGE 1
+/* run_particle_filter_for_marginals with initial_sample (mocat/src/ssm/filtering.py:264-278): the
+ * filter's particles and log-weights at t = 0 are read from x_ws0 / lw_ws0 at state_offset (the caller
+ * writes them) instead of drawn, y_0 is not read, step t uses split(key, T-1)[t-1], and the
+ * log-likelihood starts at 0 (it estimates log p(y_1:T-1 | the given sample)). Not with out_grad. */
+#define MOCAT_PF_RESUME 2
 
 int mocat_pf_param_count(int model, int d, int dy, int64_t* count);
 int mocat_pf_run(int model, int d, int dy, const mocat_pf_filter* filters, int n_filters, const float* params,
@@ -127,10 +133,10 @@ int mocat_pf_run(int model, int d, int dy, const mocat_pf_filter* filters, int n
                  float* tan_ws1, double* out_grad, hipStream_t stream);
 
 /* The same filters (same run_particle_filter_for_marginals semantics, outputs and workspaces as
- * mocat_pf_run) for those flagged MOCAT_PF_LARGE, spread over the whole GPU: per time step four
- * grid-wide launches over 256-particle chunks, all flagged filters batched in each (statistics and
- * resampling decision; quantised weight totals; exact prefix sums; ancestor search, propagation and
- * weighting). For the same log-weights and key the ancestors equal mocat_pf_run's. host_filters is a
+ * mocat_pf_run) for those flagged MOCAT_PF_LARGE, spread over the whole GPU: per time step three
+ * grid-wide launches over 1024-particle chunks, all flagged filters batched in each (the previous
+ * step's statistics, resampling decision and quantised weight totals; exact prefix sums; ancestor
+ * search, propagation and weighting). For the same log-weights and key the ancestors equal mocat_pf_run's. host_filters is a
  * host array (the launch sequence depends on n and T); workspace holds
  * mocat_pf_large_workspace_bytes bytes. Calls on one stream may follow mocat_pf_run with the same
  * arguments: the two paths touch disjoint filters. */

@@ -45,6 +45,11 @@ namespace {
 constexpr int NT = 512;
 constexpr int NW = NT / kWave;
 constexpr int32_t kPfLarge = 1;  // mocat_pf_filter.flags: run by mocat_pf_run_large, skipped by mocat_pf_run
+// mocat_pf_filter.flags: resume from a given sample (run_particle_filter_for_marginals with initial_sample,
+// mocat/src/ssm/filtering.py:264-278): the particles and log-weights of t = 0 are read from the state
+// workspaces x_ws0 / lw_ws0 instead of drawn, the step keys are split(key, T - 1) (no (key, sub) split)
+// and the log-likelihood starts at 0 (y_0 is not read).
+constexpr int32_t kPfResume = 2;
 constexpr double kLog2PiD = 1.83787706640934548356;
 
 // ---------------------------------------------------------------- models
@@ -548,7 +553,8 @@ __global__ __launch_bounds__(NT) void pf_kernel(PfArgs a) {
     int32_t* anc = a.anc + F.state_offset;
     const float* yf = a.y + F.y_offset;
     const Key key{F.key[0], F.key[1]};
-    const Key rkey = split_at(key, 2, 0), sub = split_at(key, 2, 1);
+    const bool resume = (F.flags & kPfResume) != 0;
+    const Key rkey = resume ? key : split_at(key, 2, 0), sub = split_at(key, 2, 1);
     constexpr uint32_t hD = (D + 1) / 2;
     float* ta = GRAD ? a.tan0 + F.state_offset * 5 : nullptr;
     float* tb = GRAD ? a.tan1 + F.state_offset * 5 : nullptr;
@@ -561,7 +567,11 @@ __global__ __launch_bounds__(NT) void pf_kernel(PfArgs a) {
     for (int64_t i = tid; i < n; i += NT) {
       float eps[D], x[D];
       float l;
-      if constexpr (Model::kOptimal) {
+      if (resume) {
+        #pragma unroll
+        for (int q = 0; q < D; ++q) x[q] = xa[i * D + q];
+        l = lwa[i];
+      } else if constexpr (Model::kOptimal) {
         shared_eps<D>(sub0, (uint32_t)(n * D), (uint32_t)i, eps);
         model.init_opt(eps, yf, x);
         l = 0.f;
@@ -606,7 +616,7 @@ __global__ __launch_bounds__(NT) void pf_kernel(PfArgs a) {
       for (int k = 0; k < 3; ++k) gtot[k] = gprev[k];
     }
     const double logn = log((double)n);
-    double loglik = lse - logn;
+    double loglik = resume ? 0.0 : lse - logn;
     double ess = exp(2.0 * lse - lse2);
     if (tid == 0) {
       if (a.out_ess) a.out_ess[F.t_offset] = (float)ess;
@@ -817,7 +827,7 @@ constexpr int kChunkEnds = 4096;   // SG-block end sums staged in LDS (filters u
 struct LfDesc {
   int64_t n, T, y_offset, state_offset, out_offset, t_offset, chunk0;
   uint32_t key[2];
-  int32_t param_offset, f;
+  int32_t param_offset, f, flags;
 };
 
 struct LfState {
@@ -978,7 +988,11 @@ __global__ __launch_bounds__(LT) void lf_init_kernel(LfArgs a) {
     if (i >= n) break;
     float eps[D], x[D];
     float l;
-    if constexpr (Model::kOptimal) {
+    if (F.flags & kPfResume) {
+#pragma unroll
+      for (int q = 0; q < D; ++q) x[q] = a.x0[(F.state_offset + i) * D + q];
+      l = a.lw0[F.state_offset + i];
+    } else if constexpr (Model::kOptimal) {
       shared_eps<D>(sub0, (uint32_t)(n * D), (uint32_t)i, eps);
       model.init_opt(eps, a.y + F.y_offset, x);
       l = 0.f;
@@ -1097,7 +1111,7 @@ __global__ __launch_bounds__(LT) void lf_prep_kernel(LfArgs a, int64_t t) {
     if (threadIdx.x == 0) {
       const int64_t tp = t - 1;
       LfState st = a.st[fi];
-      st.loglik = tp == 0 ? lse - logn : st.loglik + lse - st.lse_res;
+      st.loglik = tp == 0 ? ((F.flags & kPfResume) ? 0.0 : lse - logn) : st.loglik + lse - st.lse_res;
       if (a.out_ess) a.out_ess[F.t_offset + tp] = (float)ess;
       if (a.out_loglik_t) a.out_loglik_t[F.t_offset + tp] = st.loglik;
       if constexpr (GRAD) {
@@ -1271,7 +1285,8 @@ __global__ __launch_bounds__(LT) void lf_step_kernel(LfArgs a, int64_t t, int ce
   load_shared_params(model, a.params + F.param_offset);
   if (tid == 0) {
     const Key key{F.key[0], F.key[1]};
-    const Key ikk = split_at(split_at(key, 2, 0), (uint32_t)(F.T - 1), (uint32_t)(t - 1));
+    const Key rk = (F.flags & kPfResume) ? key : split_at(key, 2, 0);
+    const Key ikk = split_at(rk, (uint32_t)(F.T - 1), (uint32_t)(t - 1));
     sk[0] = ikk.a;
     sk[1] = ikk.b;
     su = uniform_from_bits(bits_at(ikk, 1, 0), 0.f, 1.f);
@@ -1571,6 +1586,9 @@ int mocat_pf_run(int model, int d, int dy, const mocat_pf_filter* filters, int n
     return MOCAT_ERR_ARG;
   if (out_grad && (model != 1 || !tan_ws0 || !tan_ws1)) return MOCAT_ERR_ARG;
   if (n_filters == 0) return MOCAT_OK;
+  if (out_grad)
+    for (int f = 0; f < n_filters; ++f)
+      if (filters[f].flags & kPfResume) return MOCAT_ERR_ARG;
   PfArgs a{filters, n_filters, params, y, x_ws0, x_ws1, lw_ws0, lw_ws1, csum_ws, anc_ws, resampling,
            ess_threshold, out_value, out_lw, out_anc, out_ess, out_loglik_t, out_loglik, tan_ws0, tan_ws1, out_grad};
   if (model == 1) {
@@ -1624,9 +1642,10 @@ int mocat_pf_run_large(int model, int d, int dy, const mocat_pf_filter* host_fil
     if (F.n <= 0 || F.T <= 0 || F.n > 0x7FFFFFFFll || F.T > 0xFFFFFFFFll) return MOCAT_ERR_ARG;
     if (resampling == 2 && F.n * F.n >= 0xFFFFFFFFll) return MOCAT_ERR_SIZE;
     if (model == 2 && F.n * (int64_t)d > 0xFFFFFFFFll) return MOCAT_ERR_SIZE;
+    if (out_grad && (F.flags & kPfResume)) return MOCAT_ERR_ARG;
     const int64_t nchunk = (F.n + CH - 1) / CH;
     desc.push_back(LfDesc{F.n, F.T, F.y_offset, F.state_offset, F.out_offset, F.t_offset, chunk0, {F.key[0], F.key[1]},
-                          F.param_offset, f});
+                          F.param_offset, f, F.flags});
     for (int64_t k = 0; k < nchunk; ++k) chunk_filter.push_back((int32_t)(desc.size() - 1));
     chunk0 += nchunk;
   }

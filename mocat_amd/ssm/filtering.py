@@ -41,6 +41,7 @@ FILTER_DTYPE = np.dtype([("n", "<i8"), ("T", "<i8"), ("y_offset", "<i8"), ("stat
                          ("out_offset", "<i8"), ("t_offset", "<i8"), ("key", "<u4", (2,)),
                          ("param_offset", "<i4"), ("flags", "<i4")])
 PF_LARGE = 1  # flags bit: the filter runs on the chunked many-workgroup path (mocat_pf_run_large)
+PF_RESUME = 2  # flags bit: start from given particles and log-weights (initial_sample)
 assert FILTER_DTYPE.itemsize == 64
 
 
@@ -231,7 +232,8 @@ def _large_flags(ns, Ts, large_n):
     return [int(n) >= int(large_n) for n in ns]
 
 
-def _launch(models, ys, keys, ns, ess_threshold, resampling, store, grad=False, large_n=None, particle_filter=None):
+def _launch(models, ys, keys, ns, ess_threshold, resampling, store, grad=False, large_n=None, particle_filter=None,
+            initial=None):
     """Runs len(models) filters (same model class and dims): filters below large_n particles in one
     persistent launch (a workgroup per filter), the others through the chunked per-step path."""
     m0 = models[0]
@@ -256,8 +258,9 @@ def _launch(models, ys, keys, ns, ess_threshold, resampling, store, grad=False, 
     for slot, i in enumerate(order):
         p = opt.kernel_params(models[i]) if optimal else models[i].kernel_params()
         yi = np.asarray(ys[i], np.float32).reshape(Ts[i], dy)
+        resume = initial is not None and initial[i] is not None
         desc[slot] = (ns[i], Ts[i], yoff, soff, ooff if store else -1, toff, _key_np(keys[i]), poff,
-                      PF_LARGE if large[i] else 0)
+                      (PF_LARGE if large[i] else 0) | (PF_RESUME if resume else 0))
         params.append(p)
         yflat.append(yi.ravel())
         poff += p.size
@@ -284,6 +287,17 @@ def _launch(models, ys, keys, ns, ess_threshold, resampling, store, grad=False, 
     out_ll = torch.empty(F, dtype=torch.float64, device=dev)
     if grad and kind != 1:
         raise NotImplementedError("log-likelihood gradients are implemented for the stochastic-volatility model")
+    if grad and initial is not None and any(v is not None for v in initial):
+        raise NotImplementedError("log-likelihood gradients start from the initial distribution, not from a sample")
+    if initial is not None:
+        # resumed filters read their t = 0 particles and log-weights from the state workspaces
+        for slot, i in enumerate(order):
+            if initial[i] is None:
+                continue
+            xi, li = initial[i]
+            s0 = int(desc[slot]["state_offset"])
+            x0[s0 * d:(s0 + ns[i]) * d] = torch.as_tensor(xi, device=dev).reshape(-1).to(torch.float32)
+            lw0[s0:s0 + ns[i]] = torch.as_tensor(li, device=dev).reshape(-1).to(torch.float32)
     tan0 = torch.empty(soff * 5, **f32) if grad else None
     tan1 = torch.empty(soff * 5, **f32) if grad else None
     out_grad = torch.empty((F, 3), dtype=torch.float64, device=dev) if grad else None
@@ -337,10 +351,12 @@ def run_particle_filter_for_marginals(ssm_scenario, particle_filter, y, t, rando
                                       initial_sample: cdict = None, ess_threshold: float = 0.5,
                                       resampling: str = "categorical") -> cdict:
     kind = _filter_kind(particle_filter)
-    if initial_sample is not None:
-        raise NotImplementedError("resuming from initial_sample is not supported by the fused filter kernel")
+    if initial_sample is not None and kind is EnsembleKalmanFilter:
+        raise NotImplementedError("the EnKF kernels start from the initial distribution (no initial_sample)")
     if resampling not in RESAMPLING:
         raise ValueError(f"resampling must be one of {list(RESAMPLING)}")
+    if initial_sample is not None:
+        n = int(initial_sample.value.shape[-2])
     if n is None:
         raise ValueError("n (number of particles) is required")
     yn = y.detach().cpu().numpy() if isinstance(y, torch.Tensor) else np.asarray(y)
@@ -355,12 +371,48 @@ def run_particle_filter_for_marginals(ssm_scenario, particle_filter, y, t, rando
         raise NotImplementedError("the stochastic-volatility kernel assumes unit time steps")
     if resampling == "categorical" and n * n >= 0xFFFFFFFF:
         raise ValueError("categorical resampling draws an (n, n) Gumbel matrix: n * n must be < 2^32 - 1")
+    if initial_sample is not None:
+        return _resume(ssm_scenario, particle_filter, yn, tn, random_key, int(n), initial_sample, ess_threshold,
+                       resampling)
     r = _launch([ssm_scenario], [yn], [random_key], [int(n)], ess_threshold, resampling, True,
                 particle_filter=particle_filter)[0]
     dev = r["value"].device
     out = cdict(value=r["value"], log_weight=r["log_weight"], t=torch.as_tensor(tn, device=dev),
                 y=torch.as_tensor(yn, dtype=torch.float32, device=dev), ess=r["ess"],
                 log_likelihood=r["log_likelihood"], ancestors=r["ancestors"])
+    return out
+
+
+def _resume(ssm_scenario, particle_filter, yn, tn, random_key, n, initial_sample, ess_threshold, resampling):
+    """run_particle_filter_for_marginals from initial_sample (mocat/src/ssm/filtering.py:264-324): the
+    scan starts from initial_sample[-1] with step keys split(random_key, len(y)) and every observation
+    is a propagation step; the new steps are appended to the sample's arrays. The resampling decision of
+    the first new step uses the ESS of the sample's last log-weights computed in FP64 (mocat reads its
+    stored f32 ESS). log_likelihood estimates log p(y | the sample)."""
+    if len(yn) == 0:
+        return initial_sample.copy()
+    dy = yn.shape[1]
+    x_init = initial_sample.value[-1]
+    lw_init = initial_sample.log_weight[-1]
+    yfull = np.concatenate([np.zeros((1, dy), yn.dtype), yn])  # row 0 stands for the sample (not read)
+    r = _launch([ssm_scenario], [yfull], [random_key], [n], ess_threshold, resampling, True,
+                particle_filter=particle_filter, initial=[(x_init, lw_init)])[0]
+    dev = r["value"].device
+    out = initial_sample.copy()
+    cat = lambda a, b: torch.cat([torch.as_tensor(a, device=dev).to(b.dtype), b], 0)
+    out.value = cat(initial_sample.value, r["value"][1:])
+    out.log_weight = cat(initial_sample.log_weight, r["log_weight"][1:])
+    out.ess = cat(torch.atleast_1d(torch.as_tensor(initial_sample.ess, device=dev)), r["ess"][1:])
+    y_new = torch.as_tensor(yn, dtype=torch.float32, device=dev)
+    out.y = cat(initial_sample.y, y_new) if getattr(initial_sample, "y", None) is not None else y_new
+    t_new = torch.as_tensor(tn, device=dev)
+    out.t = cat(torch.atleast_1d(torch.as_tensor(initial_sample.t, device=dev)), t_new) \
+        if getattr(initial_sample, "t", None) is not None else t_new
+    if getattr(initial_sample, "ancestors", None) is not None:
+        out.ancestors = cat(initial_sample.ancestors, r["ancestors"][1:])
+    else:
+        out.ancestors = r["ancestors"][1:]
+    out.log_likelihood = r["log_likelihood"]
     return out
 
 

@@ -141,23 +141,31 @@ def resample_indices(w, key, method):
 
 # ----------------------------------------------------------------------------- filter
 def run_pf(p, y, key, n, ess_threshold=0.5, resampling="systematic", dtype=np.float64, ancestors=None,
-           resample_flags=None):
+           resample_flags=None, initial=None):
     """Returns dict(value (T, n, d), log_weight (T, n), ess (T,), log_likelihood, ancestors (T, n)).
 
     ancestors / resample_flags (optional, (T, n) and (T,)): replay given resampling decisions and
-    indices instead of drawing them (the fixed-resampling-index path of another run)."""
+    indices instead of drawing them (the fixed-resampling-index path of another run).
+    initial = (x (n, d), log_weight (n,)): resume from a sample as run_particle_filter_for_marginals
+    with initial_sample (mocat/src/ssm/filtering.py:264-278): row 0 of y is not read, the step keys
+    are split(key, T - 1) (no (key, sub) split) and the log-likelihood starts at 0."""
     y = np.asarray(y, np.float64).reshape(len(y), -1)
     T = y.shape[0]
     d = p["d"]
     key = np.asarray(key, np.uint32)
-    rkey, sub = rng.split(key, 2)
-    eps = rng.normal_batch(rng.split(sub, n), d).astype(dtype)
-    x = init_state(p, eps).reshape(n, d)
-    lw = -lik_potential(p, x, y[0])
+    if initial is not None:
+        rkey = key
+        x = np.asarray(initial[0], np.float64).reshape(n, d)
+        lw = np.asarray(initial[1], np.float64).reshape(n)
+    else:
+        rkey, sub = rng.split(key, 2)
+        eps = rng.normal_batch(rng.split(sub, n), d).astype(dtype)
+        x = init_state(p, eps).reshape(n, d)
+        lw = -lik_potential(p, x, y[0])
     values, lws, esss, ancs = [x], [lw], [], [np.arange(n, dtype=np.int32)]
     l0, l2 = lse(lw), lse(2 * lw)
     esss.append(np.exp(2 * l0 - l2))
-    loglik = l0 - np.log(n)
+    loglik = 0.0 if initial is not None else l0 - np.log(n)
     int_keys = rng.split(rkey, T - 1) if T > 1 else np.zeros((0, 2), np.uint32)
     for t in range(1, T):
         ik = int_keys[t - 1]

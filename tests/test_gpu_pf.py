@@ -269,3 +269,38 @@ def test_invalid_sizes_raise(m):
     model, y, t = datasets.lgssm_config(0, 4, 5)
     with pytest.raises((MocatKernelError, ValueError)):
         m.ssm.run_particle_filters([model], [y.cpu().numpy()], [m.random.PRNGKey(0)], [0], 1.0, "systematic")
+
+
+@pytest.mark.parametrize("n,optimal", [(700, False), (5000, False), (1500, True), (6000, True)])
+def test_resume_from_initial_sample_repeats_the_run(m, n, optimal):
+    """initial_sample: the step keys are split(key, len(y)) directly, so resuming from a run's t = 0
+    sample with that run's first split key repeats its steps 1..T-1 bit for bit, on the one-workgroup
+    path (n < 4096) and the chunked path; the resumed log-likelihood lacks only the t = 0 term."""
+    from mocat_amd import datasets
+    model, y, t = datasets.lgssm_config(4, 4, 15)
+    pf = m.ssm.OptimalNonLinearGaussianParticleFilter() if optimal else m.ssm.BootstrapFilter()
+    key = m.random.PRNGKey(9)
+    kw = dict(ess_threshold=0.7, resampling="systematic")
+    full = m.ssm.run_particle_filter_for_marginals(model, pf, y, t, key, n=n, **kw)
+    first = full[0:1]
+    res = m.ssm.run_particle_filter_for_marginals(model, pf, y[1:], t[1:], m.random.split(key, 2)[0],
+                                                  initial_sample=first, **kw)
+    assert res.value.shape == full.value.shape
+    np.testing.assert_array_equal(res.value.cpu().numpy(), full.value.cpu().numpy())
+    np.testing.assert_array_equal(res.log_weight.cpu().numpy(), full.log_weight.cpu().numpy())
+    np.testing.assert_array_equal(res.ancestors.cpu().numpy(), full.ancestors.cpu().numpy())
+    np.testing.assert_allclose(res.ess.cpu().numpy(), full.ess.cpu().numpy(), rtol=1e-6)
+    lw0 = full.log_weight[0].double().cpu().numpy()
+    l0 = opf.lse(lw0) - np.log(n)
+    a, b = res.log_likelihood.item(), full.log_likelihood.item() - l0
+    assert abs(a - b) <= 1e-9 * (1 + abs(b)), (a, b)
+    # and the oracle, replaying the resumed run's ancestors from the same sample, agrees
+    if not optimal:
+        ess = res.ess.cpu().numpy()
+        flags = np.zeros(15, bool)
+        flags[1:] = ess[:-1] < 0.7 * n
+        o = opf.run_pf(_lgssm_np(model), y.cpu().numpy(), m.random.split(key, 2)[0].cpu().numpy(), n, 0.7,
+                       "systematic", ancestors=res.ancestors.cpu().numpy(), resample_flags=flags,
+                       initial=(full.value[0].cpu().numpy(), lw0))
+        np.testing.assert_allclose(res.value[-1].cpu().numpy(), o["value"][-1], rtol=1e-3, atol=1e-3)
+        assert abs(a - o["log_likelihood"]) <= 1e-4 * abs(o["log_likelihood"]) + 1e-4

@@ -113,3 +113,19 @@ def test_sv_tangent_gradient_matches_finite_differences():
           (ll(phi, sigma, beta + h) - ll(phi, sigma, beta - h)) / (2 * h)]
     np.testing.assert_allclose(g, fd, rtol=1e-5, atol=1e-5)
     assert abs(ref["log_likelihood"] - ll(phi, sigma, beta)) < 1e-12
+
+
+def test_resume_from_the_first_state_repeats_the_run():
+    """run_particle_filter_for_marginals with initial_sample splits its key straight into the step
+    keys; resuming from a run's t = 0 particles with that run's first split key repeats steps 1..T-1
+    exactly, and the resumed log-likelihood is the full one minus the t = 0 term."""
+    p = _lgssm(5, 3)
+    _, y = pf.simulate(p, 9, rng.PRNGKey(4))
+    key = rng.PRNGKey(6)
+    full = pf.run_pf(p, y, key, 300, 0.6, "systematic")
+    rkey = rng.split(key, 2)[0]
+    res = pf.run_pf(p, y, rkey, 300, 0.6, "systematic", initial=(full["value"][0], full["log_weight"][0]))
+    np.testing.assert_array_equal(res["value"], full["value"])
+    np.testing.assert_array_equal(res["ancestors"], full["ancestors"])
+    l0 = pf.lse(full["log_weight"][0]) - np.log(300)
+    assert abs(res["log_likelihood"] - (full["log_likelihood"] - l0)) < 1e-9
