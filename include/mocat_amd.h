@@ -115,7 +115,7 @@ typedef struct {
   uint32_t key[2];       /* random_key of the filter */
   int32_t param_offset;  /* floats into params */
   int32_t flags;         /* bit 0 (MOCAT_PF_LARGE): run by mocat_pf_run_large, skipped by mocat_pf_run;
-                            bit 1 (MOCAT_PF_RESUME): resume from a given sample, see below */
+                            bits 1-3: MOCAT_PF_RESUME, MOCAT_PF_STEP_KEY, MOCAT_PF_INIT_KEY below */
 } mocat_pf_filter;
 
 #define MOCAT_PF_LARGE 1
@@ -124,6 +124,11 @@ typedef struct {
  * writes them) instead of drawn, y_0 is not read, step t uses split(key, T-1)[t-1], and the
  * log-likelihood starts at 0 (it estimates log p(y_1:T-1 | the given sample)). Not with out_grad. */
 #define MOCAT_PF_RESUME 2
+/* mocat's one-step functions (filtering.py:173-253): MOCAT_PF_STEP_KEY makes the filter key itself the
+ * key of step 1 (propagate_particle_filter, with T = 2 and MOCAT_PF_RESUME); MOCAT_PF_INIT_KEY draws
+ * the initial particles from split(key, n) without the (key, sub) split (initiate_particles, T = 1). */
+#define MOCAT_PF_STEP_KEY 4
+#define MOCAT_PF_INIT_KEY 8
 
 int mocat_pf_param_count(int model, int d, int dy, int64_t* count);
 int mocat_pf_run(int model, int d, int dy, const mocat_pf_filter* filters, int n_filters, const float* params,

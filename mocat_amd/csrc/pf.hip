@@ -50,6 +50,11 @@ constexpr int32_t kPfLarge = 1;  // mocat_pf_filter.flags: run by mocat_pf_run_l
 // workspaces x_ws0 / lw_ws0 instead of drawn, the step keys are split(key, T - 1) (no (key, sub) split)
 // and the log-likelihood starts at 0 (y_0 is not read).
 constexpr int32_t kPfResume = 2;
+// mocat_pf_filter.flags for mocat's one-step functions (filtering.py:173-253): step t's key is the
+// filter key itself (propagate_particle_filter: T = 2), and the initial particles come from
+// split(key, n) without the (key, sub) split (initiate_particles).
+constexpr int32_t kPfStepKey = 4;
+constexpr int32_t kPfInitKey = 8;
 constexpr double kLog2PiD = 1.83787706640934548356;
 
 // ---------------------------------------------------------------- models
@@ -554,7 +559,8 @@ __global__ __launch_bounds__(NT) void pf_kernel(PfArgs a) {
     const float* yf = a.y + F.y_offset;
     const Key key{F.key[0], F.key[1]};
     const bool resume = (F.flags & kPfResume) != 0;
-    const Key rkey = resume ? key : split_at(key, 2, 0), sub = split_at(key, 2, 1);
+    const Key rkey = resume ? key : split_at(key, 2, 0);
+    const Key sub = (F.flags & kPfInitKey) ? key : split_at(key, 2, 1);
     constexpr uint32_t hD = (D + 1) / 2;
     float* ta = GRAD ? a.tan0 + F.state_offset * 5 : nullptr;
     float* tb = GRAD ? a.tan1 + F.state_offset * 5 : nullptr;
@@ -625,7 +631,7 @@ __global__ __launch_bounds__(NT) void pf_kernel(PfArgs a) {
 
     // ---- t = 1 .. T-1
     for (int64_t t = 1; t < T; ++t) {
-      const Key ik = split_at(rkey, (uint32_t)(T - 1), (uint32_t)(t - 1));
+      const Key ik = (F.flags & kPfStepKey) ? key : split_at(rkey, (uint32_t)(T - 1), (uint32_t)(t - 1));
       const bool res = ess < (double)a.ess_threshold * (double)n;
       const float* ycur = yf + t * DY;
       const double lse_res = res ? logn : lse;
@@ -978,7 +984,7 @@ __global__ __launch_bounds__(LT) void lf_init_kernel(LfArgs a) {
   __syncthreads();
   const int64_t n = F.n;
   const Key key{F.key[0], F.key[1]};
-  const Key sub = split_at(key, 2, 1);
+  const Key sub = (F.flags & kPfInitKey) ? key : split_at(key, 2, 1);
   constexpr uint32_t hD = (D + 1) / 2;
   LseAcc acc{-INFINITY, 0.0, 0.0};
   GradAcc gacc{-INFINITY, 0.0, 0.0, 0.0, 0.0};
@@ -1286,7 +1292,7 @@ __global__ __launch_bounds__(LT) void lf_step_kernel(LfArgs a, int64_t t, int ce
   if (tid == 0) {
     const Key key{F.key[0], F.key[1]};
     const Key rk = (F.flags & kPfResume) ? key : split_at(key, 2, 0);
-    const Key ikk = split_at(rk, (uint32_t)(F.T - 1), (uint32_t)(t - 1));
+    const Key ikk = (F.flags & kPfStepKey) ? key : split_at(rk, (uint32_t)(F.T - 1), (uint32_t)(t - 1));
     sk[0] = ikk.a;
     sk[1] = ikk.b;
     su = uniform_from_bits(bits_at(ikk, 1, 0), 0.f, 1.f);

@@ -2,5 +2,6 @@
 from .models import StateSpaceModel, TimeHomogenousLinearGaussian, StochasticVolatility
 from .filtering import (ParticleFilter, BootstrapFilter, OptimalNonLinearGaussianParticleFilter, EnsembleKalmanFilter,
                         run_particle_filter_for_marginals, run_particle_filters, resample_indices, categorical,
-                        run_kalman_filter_for_marginals)
+                        run_kalman_filter_for_marginals, initiate_particles, resample_particles,
+                        propagate_particle_filter)
 from .backward import backward_simulation, forward_filtering_backward_simulation

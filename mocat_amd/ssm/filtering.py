@@ -42,6 +42,8 @@ FILTER_DTYPE = np.dtype([("n", "<i8"), ("T", "<i8"), ("y_offset", "<i8"), ("stat
                          ("param_offset", "<i4"), ("flags", "<i4")])
 PF_LARGE = 1  # flags bit: the filter runs on the chunked many-workgroup path (mocat_pf_run_large)
 PF_RESUME = 2  # flags bit: start from given particles and log-weights (initial_sample)
+PF_STEP_KEY = 4  # flags bit: the key of step 1 is the filter key itself (propagate_particle_filter)
+PF_INIT_KEY = 8  # flags bit: initial particles from split(key, n) (initiate_particles)
 assert FILTER_DTYPE.itemsize == 64
 
 
@@ -233,7 +235,7 @@ def _large_flags(ns, Ts, large_n):
 
 
 def _launch(models, ys, keys, ns, ess_threshold, resampling, store, grad=False, large_n=None, particle_filter=None,
-            initial=None):
+            initial=None, extra_flags=0):
     """Runs len(models) filters (same model class and dims): filters below large_n particles in one
     persistent launch (a workgroup per filter), the others through the chunked per-step path."""
     m0 = models[0]
@@ -260,7 +262,7 @@ def _launch(models, ys, keys, ns, ess_threshold, resampling, store, grad=False, 
         yi = np.asarray(ys[i], np.float32).reshape(Ts[i], dy)
         resume = initial is not None and initial[i] is not None
         desc[slot] = (ns[i], Ts[i], yoff, soff, ooff if store else -1, toff, _key_np(keys[i]), poff,
-                      (PF_LARGE if large[i] else 0) | (PF_RESUME if resume else 0))
+                      (PF_LARGE if large[i] else 0) | (PF_RESUME if resume else 0) | extra_flags)
         params.append(p)
         yflat.append(yi.ravel())
         poff += p.size
@@ -380,6 +382,82 @@ def run_particle_filter_for_marginals(ssm_scenario, particle_filter, y, t, rando
     out = cdict(value=r["value"], log_weight=r["log_weight"], t=torch.as_tensor(tn, device=dev),
                 y=torch.as_tensor(yn, dtype=torch.float32, device=dev), ess=r["ess"],
                 log_likelihood=r["log_likelihood"], ancestors=r["ancestors"])
+    return out
+
+
+def _np2(y, dy=None):
+    yn = y.detach().cpu().numpy() if isinstance(y, torch.Tensor) else np.asarray(y)
+    return yn.reshape(-1, dy) if dy is not None else (yn[:, None] if yn.ndim == 1 else yn)
+
+
+def initiate_particles(ssm_scenario, particle_filter, n: int, random_key, y=None, t: float = None) -> cdict:
+    """mocat/src/ssm/filtering.py:173-193: particles from split(random_key, n) (the optimal filter:
+    normal(split(random_key, n)[0], (n, d))), log-weights -likelihood(y) (zero without y, and always
+    zero for the optimal filter). Returns cdict(value (1, n, d), log_weight (1, n), t (1,), y, ess (1,))."""
+    kind = _filter_kind(particle_filter)
+    if kind is EnsembleKalmanFilter:
+        raise NotImplementedError("EnsembleKalmanFilter: use run_particle_filter_for_marginals")
+    if kind is OptimalNonLinearGaussianParticleFilter and y is None:
+        raise ValueError("the optimal filter's initial sample conditions on y")
+    dy = int(ssm_scenario.dim_obs)
+    yv = np.zeros((1, dy), np.float32) if y is None else _np2(y, dy)[:1]
+    r = _launch([ssm_scenario], [yv], [random_key], [int(n)], 0.0, "systematic", True, particle_filter=particle_filter,
+                extra_flags=PF_INIT_KEY)[0]
+    dev = r["value"].device
+    lw = r["log_weight"] if y is not None else torch.zeros((1, int(n)), dtype=torch.float32, device=dev)
+    ess = r["ess"] if y is not None else torch.full((1,), float(n), device=dev)
+    return cdict(value=r["value"], log_weight=lw,
+                 t=torch.as_tensor([0.0 if t is None else float(t)], device=dev),
+                 y=torch.as_tensor(yv, device=dev) if y is not None else None, ess=ess)
+
+
+def resample_particles(particles: cdict, random_key, resample_full: bool = True) -> cdict:
+    """mocat/src/ssm/filtering.py:200-215: ancestors random.categorical(random_key, log_weight[-1], (n,))
+    (Gumbel-max, the categorical kernel); resample_full gathers every stored time point, else only the
+    latest; the latest log-weights become zero."""
+    out = particles.copy()
+    n = int(particles.value.shape[-2])
+    lw = particles.log_weight
+    idx = categorical(random_key, lw if lw.dim() == 1 else lw[-1]).long()
+    if lw.dim() == 1:
+        out.value = particles.value[..., idx, :]
+        out.log_weight = torch.zeros(n, dtype=lw.dtype, device=lw.device)
+        return out
+    if resample_full:
+        out.value = particles.value[:, idx, :]
+    else:
+        out.value = particles.value.clone()
+        out.value[-1] = particles.value[-1][idx]
+    out.log_weight = lw.clone()
+    out.log_weight[-1] = 0.0
+    return out
+
+
+def propagate_particle_filter(ssm_scenario, particle_filter, particles: cdict, y_new, t_new: float, random_key,
+                              ess_threshold: float = 0.5, resample_full: bool = True) -> cdict:
+    """mocat/src/ssm/filtering.py:218-253: resample (Gumbel-max with random_key) when the stored ESS
+    particles.ess[-1] < ess_threshold * n, then one propagation step with the proposal keys
+    split(random_key, n) (one resumed filter step that does not resample), appended to particles."""
+    kind = _filter_kind(particle_filter)
+    if kind is EnsembleKalmanFilter:
+        raise NotImplementedError("EnsembleKalmanFilter: use run_particle_filter_for_marginals")
+    n = int(particles.value.shape[1])
+    if float(torch.as_tensor(particles.ess).reshape(-1)[-1]) < ess_threshold * n:
+        particles = resample_particles(particles, random_key, resample_full)
+    dy = int(ssm_scenario.dim_obs)
+    yv = np.concatenate([np.zeros((1, dy), np.float32), _np2(y_new, dy)[:1].astype(np.float32)])
+    r = _launch([ssm_scenario], [yv], [random_key], [n], 0.0, "systematic", True, particle_filter=particle_filter,
+                initial=[(particles.value[-1], particles.log_weight[-1])], extra_flags=PF_STEP_KEY)[0]
+    dev = r["value"].device
+    out = particles.copy()
+    out.value = torch.cat([particles.value, r["value"][1:]], 0)
+    out.log_weight = torch.cat([particles.log_weight, r["log_weight"][1:]], 0)
+    y_row = torch.as_tensor(yv[1:], device=dev)
+    out.y = torch.cat([torch.as_tensor(particles.y, device=dev).reshape(-1, dy).to(y_row.dtype), y_row], 0) \
+        if getattr(particles, "y", None) is not None else y_row
+    out.t = torch.cat([torch.atleast_1d(torch.as_tensor(particles.t, device=dev)).to(torch.float32),
+                       torch.as_tensor([float(t_new)], device=dev)])
+    out.ess = torch.cat([torch.atleast_1d(torch.as_tensor(particles.ess, device=dev)).to(torch.float32), r["ess"][1:]])
     return out
 
 
