@@ -16,6 +16,7 @@ from .transport import (TransportSampler, SMCSampler, TemperedSMCSampler, Metrop
 from .scenarios import GaussianMixture
 from . import kernels
 from . import reduce, metrics, ssm
-from .metrics import ksd
+from .metrics import (ksd, autocorrelation, integrated_autocorrelation_time, ess_autocorrelation, log_ess_log_weight,
+                      ess_log_weight, squared_jumping_distance)
 
 __version__ = "0.1.0"

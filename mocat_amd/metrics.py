@@ -80,6 +80,14 @@ def ess_log_weight(log_weight) -> torch.Tensor:
 _KSD_KERNELS = {"Gaussian": 0, "IMQ": 1}
 
 
+def squared_jumping_distance(sample) -> torch.Tensor:
+    """Mean squared increment of a chain along its first axis, per coordinate
+    (mocat/src/metrics.py:81-85)."""
+    vals = sample.value if hasattr(sample, "value") else sample
+    vals = torch.as_tensor(vals)
+    return ((vals[1:] - vals[:-1]) ** 2).mean(0)
+
+
 def ksd(sample, kernel, grad_potential: torch.Tensor = None, log_weight: torch.Tensor = None,
         ensemble_batchsize: int = None, random_key=None, **kernel_params) -> float:
     """Kernel Stein discrepancy of a weighted ensemble (mocat/src/metrics.py:88-130).

@@ -83,3 +83,11 @@ def test_sampler_parameters_and_deepcopy():
     s2.name = "other"
     s2.parameters.other = 10.
     assert s.name == "test" and torch.equal(s.parameters.other, torch.zeros(2))
+
+
+def test_squared_jumping_distance():
+    # mocat/src/metrics.py:81-85: mean over the chain of squared increments, per coordinate
+    from mocat_amd import squared_jumping_distance
+    x = torch.tensor([[0., 1.], [1., 3.], [3., 3.]])
+    assert torch.equal(squared_jumping_distance(x), torch.tensor([2.5, 2.0]))
+    assert torch.equal(squared_jumping_distance(cdict(value=x)), torch.tensor([2.5, 2.0]))
