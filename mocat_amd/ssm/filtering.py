@@ -513,15 +513,31 @@ def run_particle_filters(ssm_scenarios, ys, random_keys, ns, ess_threshold: floa
     return [cdict(**{k: v for k, v in r.items() if k != "desc"}) for r in res]
 
 
-def run_kalman_filter_for_marginals(lgssm, y, t=None):
-    """Kalman filter means/covariances (mocat/src/ssm/linear_gaussian/kalman.py:16-57) plus the
-    log-likelihood, with the initial covariance (not its square root) at t = 0. FP64 torch on the GPU."""
+class KalmanOutput(tuple):
+    """(means, covariances) as mocat returns them, with the log-likelihood (and the same arrays) as
+    attributes."""
+
+    def __new__(cls, means, covs, log_likelihood):
+        obj = super().__new__(cls, (means, covs))
+        obj.value, obj.cov, obj.log_likelihood = means, covs, log_likelihood
+        return obj
+
+
+def _kalman_mats(lgssm):
     dev = mr.default_device()
     f = lambda a: torch.as_tensor(np.asarray(a), dtype=torch.float64, device=dev)
-    A, Q, H, R = f(lgssm.transition_matrix), f(lgssm.transition_covariance), f(lgssm.likelihood_matrix), \
-        f(lgssm.likelihood_covariance)
-    m, P = f(lgssm.initial_mean), f(lgssm.initial_covariance)
-    yy = torch.as_tensor(y, dtype=torch.float64, device=dev).reshape(len(y), -1)
+    return (f(lgssm.transition_matrix), f(lgssm.transition_covariance), f(lgssm.likelihood_matrix),
+            f(lgssm.likelihood_covariance), f(lgssm.initial_mean), f(lgssm.initial_covariance), dev)
+
+
+def run_kalman_filter_for_marginals(lgssm, y, t=None):
+    """Kalman filter means/covariances (mocat/src/ssm/linear_gaussian/kalman.py:16-57), returned as
+    mocat's (means (T, d), covs (T, d, d)) tuple; `.log_likelihood` holds log p(y_0:T) (not in mocat).
+    The t = 0 prior covariance is the initial covariance (mocat uses its square root, kalman.py:20,
+    the same for the identity). FP64 torch on the GPU."""
+    A, Q, H, R, m, P, dev = _kalman_mats(lgssm)
+    yy = torch.as_tensor(np.asarray(y.detach().cpu() if isinstance(y, torch.Tensor) else y), dtype=torch.float64,
+                         device=dev).reshape(len(y), -1)
     ll = torch.zeros((), dtype=torch.float64, device=dev)
     means, covs = [], []
     for k in range(yy.shape[0]):
@@ -538,4 +554,36 @@ def run_kalman_filter_for_marginals(lgssm, y, t=None):
         P = P - K @ H @ P
         means.append(m)
         covs.append(P)
-    return cdict(value=torch.stack(means), cov=torch.stack(covs), log_likelihood=ll)
+    return KalmanOutput(torch.stack(means), torch.stack(covs), ll)
+
+
+def run_kalman_smoother_for_marginals_and_lag_ones(lgssm, y, t=None, filter_output=None):
+    """Rauch-Tung-Striebel backward pass (mocat/src/ssm/linear_gaussian/kalman.py:60-98) in FP64: back
+    gain J = P_t A^T (A P_t A^T + Q)^-1, m_t = f_t + J (m_t+1 - A f_t), P_t = F_t + J (P_t+1 - A F_t A^T - Q) J^T,
+    and mocat's lag-one term written as it is there, f_t . m_t+1 + J (P_t+1 + (m_t+1 - f_t+1) . m_t+1)
+    (dot products of the mean vectors, as jnp's @ on 1-D arrays evaluates them). Returns
+    (means (T, d), covs (T, d, d), lag_ones (T-1, d, d))."""
+    A, Q, _, _, _, _, dev = _kalman_mats(lgssm)
+    f_mus, f_covs = filter_output if filter_output is not None else run_kalman_filter_for_marginals(lgssm, y, t)
+    f_mus = torch.as_tensor(f_mus, dtype=torch.float64, device=dev)
+    f_covs = torch.as_tensor(f_covs, dtype=torch.float64, device=dev)
+    T = f_mus.shape[0]
+    mu, cov = f_mus[-1], f_covs[-1]
+    mus, covs, lags = [mu], [cov], []
+    for i in range(T - 2, -1, -1):
+        fm, fc = f_mus[i], f_covs[i]
+        J = fc @ A.T @ torch.linalg.inv(A @ fc @ A.T + Q)
+        lag = fm @ mu + J @ (cov + (mu - f_mus[i + 1]) @ mu)
+        mu, cov = fm + J @ (mu - A @ fm), fc + J @ (cov - A @ fc @ A.T - Q) @ J.T
+        mus.append(mu)
+        covs.append(cov)
+        lags.append(lag)
+    d = f_mus.shape[1]
+    lag_t = torch.stack(lags[::-1]) if lags else torch.zeros((0, d, d), dtype=torch.float64, device=dev)
+    return torch.stack(mus[::-1]), torch.stack(covs[::-1]), lag_t
+
+
+def run_kalman_smoother_for_marginals(lgssm, y, t=None, filter_output=None):
+    """mocat/src/ssm/linear_gaussian/kalman.py:101-107: (means, covs) of the smoothing marginals."""
+    mus, covs, _ = run_kalman_smoother_for_marginals_and_lag_ones(lgssm, y, t, filter_output)
+    return mus, covs
