@@ -876,6 +876,7 @@ struct LfArgs {
   uint64_t* ctot;    // [n_chunks] quantised weight totals
   LfState* st;       // [n_large]
   uint64_t* cends;   // [n_chunks * CH / SG] prefix sums at the SG-block ends, per filter from chunk0 * CH / SG
+  int64_t* bounds;   // [n_chunks] systematic: ancestor of each chunk's first target
 };
 
 struct LfScratch {
@@ -1049,6 +1050,10 @@ __global__ __launch_bounds__(LT) void lf_init_kernel(LfArgs a) {
   lf_store_partials<GRAD>(a, c, acc, gacc, s);
 }
 
+// The prefix sums are integers below 2^52 (exact in FP64), so for a target T >= 0 the test
+// (double)C <= T is C <= floor(T): targets are converted once and every level compares integers.
+MOCAT_DEV uint64_t target_floor(double t) { return t > 0.0 ? (uint64_t)floor(t) : 0ull; }
+
 MOCAT_DEV const float* lf_lw_prev(const LfArgs& a, int64_t t) { return ((t - 1) & 1) ? a.lw1 : a.lw0; }
 
 // One launch per step t >= 1 (and one after the last step):
@@ -1159,9 +1164,23 @@ __global__ __launch_bounds__(LT) void lf_csum_kernel(LfArgs a, int64_t t) {
   const LfState st = a.st[fi];
   if (!st.res || a.resampling == 2) return;
   const int64_t local = c - F.chunk0;
-  uint64_t off = 0;
-  for (int64_t k = threadIdx.x; k < local; k += LT) off += a.ctot[F.chunk0 + k];
+  const int64_t nchunk = (F.n + CH - 1) / CH;
+  uint64_t off = 0, wsum = 0;
+  for (int64_t k = threadIdx.x; k < nchunk; k += LT) {
+    const uint64_t v = a.ctot[F.chunk0 + k];
+    wsum += v;
+    if (k < local) off += v;
+  }
   off = lf_block_sum(off, s);
+  wsum = lf_block_sum(wsum, s);
+  // systematic: the step key's uniform, as the step kernel derives it (thread 0, during the scan)
+  __shared__ float su;
+  if (a.resampling == 0 && threadIdx.x == 0) {
+    const Key key{F.key[0], F.key[1]};
+    const Key rk = (F.flags & kPfResume) ? key : split_at(key, 2, 0);
+    const Key ikk = (F.flags & kPfStepKey) ? key : split_at(rk, (uint32_t)(F.T - 1), (uint32_t)(t - 1));
+    su = uniform_from_bits(bits_at(ikk, 1, 0), 0.f, 1.f);
+  }
   // thread tid owns the PPT consecutive particles from local * CH + tid * PPT
   const int64_t i0 = local * CH + (int64_t)threadIdx.x * PPT;
   const float* lw = lf_lw_prev(a, t) + F.state_offset;
@@ -1172,15 +1191,42 @@ __global__ __launch_bounds__(LT) void lf_csum_kernel(LfArgs a, int64_t t) {
     own += q[pk];
   }
   uint64_t run = off + lf_block_inclusive_scan(own, s) - own;
+  __shared__ uint64_t cs[CH];  // this chunk's prefix sums (systematic boundary searches)
 #pragma unroll
   for (int pk = 0; pk < PPT; ++pk) {
     run += q[pk];
     const int64_t i = i0 + pk;
+    cs[threadIdx.x * PPT + pk] = run;
     if (i < F.n) {
       a.csum[F.state_offset + i] = run;
       // the SG-block end sums, contiguous per filter, for the step kernel's first search level
       if ((i + 1) % SG == 0 || i == F.n - 1) a.cends[F.chunk0 * (CH / SG) + i / SG] = run;
     }
+  }
+  if (a.resampling != 0) return;
+  // Systematic targets floor((j + u) W / n) rise with j, so chunk b's ancestors lie between the
+  // ancestors of its first target and of the next chunk's first target. The chunk whose sums hold a
+  // boundary target (earlier sums <= target < its last sum; the last chunk also takes targets at or
+  // beyond the total, clamped to n - 1 as in the search) finds its ancestor in LDS and records it.
+  __syncthreads();
+  const int64_t count = (F.n - local * CH) < CH ? (F.n - local * CH) : CH;
+  const uint64_t chunk_hi = cs[count - 1];
+  const bool last = local == nchunk - 1;
+  const double sstep = (double)wsum / (double)F.n;
+  for (int64_t b = threadIdx.x; b < nchunk; b += LT) {
+    const uint64_t tg = target_floor(((double)(b * CH) + (double)su) * sstep);
+    if (tg < off || (!last && tg >= chunk_hi)) continue;
+    int64_t lo = 0, len = count;
+    while (len > 0) {
+      const int64_t h = len >> 1;
+      if (cs[lo + h] <= tg) {
+        lo += h + 1;
+        len -= h + 1;
+      } else {
+        len = h;
+      }
+    }
+    a.bounds[F.chunk0 + b] = local * CH + (lo < count ? lo : count - 1);
   }
 }
 
@@ -1205,10 +1251,6 @@ MOCAT_DEV void stage_lds(uint64_t* dst, int64_t count, Src src) {
 // For P searches at once: out[p] = the first k in [b[p], b[p] + m[p]) with v(k) > tg[p] (v
 // non-decreasing), or b[p] + m[p] when there is none; inactive searches return b[p]. Each halving
 // step issues the reads of all unfinished searches before comparing.
-// The prefix sums are integers below 2^52 (exact in FP64), so for a target T >= 0 the test
-// (double)C <= T is C <= floor(T): targets are converted once and every level compares integers.
-MOCAT_DEV uint64_t target_floor(double t) { return t > 0.0 ? (uint64_t)floor(t) : 0ull; }
-
 template <int P, class Get>
 MOCAT_DEV void upper_bound_lockstep(Get v, const int64_t* b, const int64_t* m, const uint64_t* tg, const bool* act,
                                     int64_t* out) {
@@ -1274,12 +1316,20 @@ __global__ __launch_bounds__(LT) void lf_step_kernel(LfArgs a, int64_t t, int ce
   // (res and the method are per filter: the branches are uniform over the workgroup.)
   extern __shared__ uint64_t cend[];  // cend_slots entries (launch-time size)
   __shared__ uint64_t win[kSearchWin];
-  __shared__ int64_t wk[2];
   const uint64_t* C = a.csum + F.state_offset;
   const int64_t nsg = (n + SG - 1) / SG;
-  const bool two_level = res && a.resampling != 2 && nsg <= cend_slots;
+  // systematic: the chunk's ancestor window comes from the prefix-sum pass (bounds); multinomial
+  // targets are unordered and take the two-level search over the staged block-end sums
+  const bool window = res && a.resampling == 0;
+  const bool two_level = res && a.resampling == 1 && nsg <= cend_slots;
   bool staged = false;
-  int64_t lo = 0;
+  int64_t lo = 0, whi = n - 1;
+  if (window) {
+    const int64_t cl = c - F.chunk0, nchunk = (n + CH - 1) / CH;
+    const int64_t b0 = a.bounds[c], b1 = cl + 1 < nchunk ? a.bounds[c + 1] : n - 1;
+    lo = b0 < 0 ? 0 : (b0 > n - 1 ? n - 1 : b0);
+    whi = b1 < lo ? lo : (b1 > n - 1 ? n - 1 : b1);
+  }
   const double W = (res && a.resampling != 2) ? (double)C[n - 1] : 0.0;
   const double sstep = W / (double)n;
   // the chunk-end sums go first: their loads are in flight while thread 0 derives the step key
@@ -1305,27 +1355,10 @@ __global__ __launch_bounds__(LT) void lf_step_kernel(LfArgs a, int64_t t, int ce
   __syncthreads();
   const Key ik{sk[0], sk[1]};
   auto cend_at = [&](int64_t k) { return cend[k]; };
-  if (two_level && a.resampling == 0) {
-    if (tid == 0) {
-      const int64_t j1 = (base + CH < n ? base + CH : n) - 1;
-      const int64_t b0[2] = {0, 0}, m0[2] = {nsg, nsg};
-      const uint64_t tg[2] = {target_floor(((double)base + (double)su) * sstep),
-                              target_floor(((double)j1 + (double)su) * sstep)};
-      const bool on[2] = {true, true};
-      int64_t k[2];
-      upper_bound_lockstep<2>(cend_at, b0, m0, tg, on, k);
-      wk[0] = k[0];
-      wk[1] = k[1];
-    }
+  if (window && whi - lo + 1 <= kSearchWin) {
+    stage_lds<8>(win, whi - lo + 1, [&](int64_t k) { return C[lo + k]; });
     __syncthreads();
-    const int64_t k0 = wk[0] < nsg ? wk[0] : nsg - 1, k1 = wk[1] < nsg ? wk[1] : nsg - 1;
-    lo = k0 * SG;
-    const int64_t len = ((k1 + 1) * SG < n ? (k1 + 1) * SG : n) - lo;
-    if (len <= kSearchWin) {
-      stage_lds<8>(win, len, [&](int64_t k) { return C[lo + k]; });
-      __syncthreads();
-      staged = true;
-    }
+    staged = true;
   }
 
   // ancestors of the thread's PPT particles
@@ -1349,7 +1382,22 @@ __global__ __launch_bounds__(LT) void lf_step_kernel(LfArgs a, int64_t t, int ce
                                 ? ((double)i + (double)su) * sstep
                                 : (double)uniform_from_bits(bits_at(ik, (uint32_t)n, (uint32_t)i), 0.f, 1.f) * W);
     }
-    if (two_level) {
+    if (window) {
+      // the first index of [lo, whi] whose sum exceeds the target (whi + 1: none does, clamped)
+      int64_t b[PPT], m[PPT], p[PPT];
+#pragma unroll
+      for (int pk = 0; pk < PPT; ++pk) {
+        b[pk] = lo;
+        m[pk] = whi - lo + 1;
+      }
+      if (staged)
+        upper_bound_lockstep<PPT>([&](int64_t k) { return win[k - lo]; }, b, m, tg, live, p);
+      else
+        upper_bound_lockstep<PPT>([&](int64_t k) { return C[k]; }, b, m, tg, live, p);
+#pragma unroll
+      for (int pk = 0; pk < PPT; ++pk)
+        if (live[pk]) srcs[pk] = p[pk] < n ? p[pk] : n - 1;
+    } else if (two_level) {
       // first SG block whose last sum exceeds the target (nsg: none does), then the first index in it
       int64_t kc[PPT], b[PPT], m[PPT], zero[PPT], all[PPT];
 #pragma unroll
@@ -1499,7 +1547,8 @@ __global__ __launch_bounds__(LT) void lf_finish_kernel(LfArgs a) {
 template <class Model, int D, int DY, bool GRAD = false>
 int launch_pf_large(const LfArgs& a, int64_t max_T, int cend_slots, hipStream_t st) {
   const dim3 chunks((unsigned)a.n_chunks), blk(LT);
-  const size_t cend_bytes = (size_t)cend_slots * sizeof(uint64_t);
+  // the block-end sums are staged only by the multinomial search
+  const size_t cend_bytes = a.resampling == 1 ? (size_t)cend_slots * sizeof(uint64_t) : 0;
   hipLaunchKernelGGL((lf_init_kernel<Model, D, DY, GRAD>), chunks, blk, 0, st, a);
   for (int64_t t = 1; t < max_T; ++t) {
     hipLaunchKernelGGL((lf_prep_kernel<GRAD>), chunks, blk, 0, st, a, t);
@@ -1525,7 +1574,7 @@ struct LfLayout {
   int n_large = 0;
   int cend_slots = 1;  // SG-block end sums the step kernel stages: the largest filter's, up to kChunkEnds
   int64_t n_chunks = 0, max_T = 0;
-  int64_t o_desc, o_cf, o_m, o_s1, o_s2, o_g, o_tot, o_st, o_cend, bytes;
+  int64_t o_desc, o_cf, o_m, o_s1, o_s2, o_g, o_tot, o_st, o_cend, o_bnd, bytes;
 };
 
 int64_t lf_align(int64_t v) { return (v + 255) / 256 * 256; }
@@ -1558,6 +1607,8 @@ LfLayout lf_layout(const mocat_pf_filter* filters, int n_filters) {
   o = lf_align(o + (int64_t)L.n_large * (int64_t)sizeof(LfState));
   L.o_cend = o;
   o = lf_align(o + L.n_chunks * (CH / SG) * 8);
+  L.o_bnd = o;
+  o = lf_align(o + L.n_chunks * 8);
   L.bytes = o;
   return L;
 }
@@ -1668,7 +1719,7 @@ int mocat_pf_run_large(int model, int d, int dy, const mocat_pf_filter* host_fil
            reinterpret_cast<float*>(ws + L.o_m), reinterpret_cast<double*>(ws + L.o_s1),
            reinterpret_cast<double*>(ws + L.o_s2), reinterpret_cast<double*>(ws + L.o_g),
            reinterpret_cast<uint64_t*>(ws + L.o_tot), reinterpret_cast<LfState*>(ws + L.o_st),
-           reinterpret_cast<uint64_t*>(ws + L.o_cend)};
+           reinterpret_cast<uint64_t*>(ws + L.o_cend), reinterpret_cast<int64_t*>(ws + L.o_bnd)};
   if (model == 1) {
     if (d != 1 || dy != 1) return MOCAT_ERR_ARG;
     return out_grad ? launch_pf_large<Sv, 1, 1, true>(a, L.max_T, L.cend_slots, stream)
