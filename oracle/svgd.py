@@ -4,8 +4,9 @@ phi_pairwise: mocat's kernelised_grad_matrix (mocat/src/transport/svgd.py:18-32)
   phi_i = mean_j [ -k(x_j, x_i) gradU_j + (x_i - x_j) k(x_j, x_i) / h^2 ],
   with the Gaussian kernel of mocat/src/kernels.py:89-102.
 adagrad: jax.example_libraries.optimizers.adagrad(step, momentum=0.9) restated (third-party, not in
-  the reference tree; called at mocat/src/transport/svgd.py:56,104-106,138-139; "parity unpinned"
-  beyond its published update rule g_sq += g^2; m = (1 - mom) g / sqrt(g_sq) + mom m; x -= step m).
+  the reference tree; called at mocat/src/transport/svgd.py:56,104-106,138-139) from its published
+  update rule g_sq += g^2; m = (1 - mom) g / sqrt(g_sq) + mom m; x -= step m, pinned by closed forms
+  (first step, invariance to gradient scale, constant-gradient momentum sum; tests/test_oracle_svgd.py).
 gmm: mocat's GaussianMixture (mocat/src/scenarios/toy_examples.py:54-116) with diagonal covariances,
   in log-sum-exp form; det_convention "mocat" weights components by det(P), "textbook" by sqrt(det P).
 median / mean bandwidth: mocat/src/kernels.py:220-229 with numpy's median of the full distance matrix.

@@ -53,3 +53,29 @@ def test_gmm_grad_central_differences():
     fd = [(svgd.gmm_potential_grad(x + eps * e, mu, var, w)[0] - svgd.gmm_potential_grad(x - eps * e, mu, var, w)[0])
           / (2 * eps) for e in np.eye(3)]
     np.testing.assert_allclose(g[0], np.array(fd)[:, 0], rtol=1e-6, atol=1e-8)
+
+
+def test_adagrad_first_step_scale_invariance_and_constant_gradient():
+    """jax.example_libraries.optimizers.adagrad (used by mocat/src/transport/svgd.py:11,56): the first
+    step moves by step (1 - beta) sign(g) (zero for g = 0); scaling every gradient by c > 0 leaves the
+    path unchanged; for a constant gradient m_k = (1 - beta) sign(g) sum_i beta^(k-i) / sqrt(i)."""
+    from oracle import svgd as osvgd
+    beta, step = 0.9, 0.05
+    g = np.array([2.0, -0.5, 0.0])
+    x0 = np.array([1.0, 1.0, 1.0])
+    x1, gsq, m = osvgd.adagrad(x0, np.zeros(3), np.zeros(3), g, step, beta)
+    np.testing.assert_allclose(x1, x0 - step * (1 - beta) * np.sign(g), rtol=0, atol=1e-15)
+    rs = np.random.default_rng(0)
+    gs = rs.normal(0, 1, (6, 3))
+    xa = xb = x0
+    sa = sb = (np.zeros(3), np.zeros(3))
+    for k in range(6):
+        xa, *sa = osvgd.adagrad(xa, sa[0], sa[1], gs[k], step, beta)
+        xb, *sb = osvgd.adagrad(xb, sb[0], sb[1], 37.0 * gs[k], step, beta)
+    np.testing.assert_allclose(xa, xb, rtol=1e-12)
+    x, s = x0, (np.zeros(3), np.zeros(3))
+    gc = np.array([3.0, -1.0, 0.25])
+    for k in range(1, 8):
+        x, *s = osvgd.adagrad(x, s[0], s[1], gc, step, beta)
+        want = (1 - beta) * np.sign(gc) * sum(beta ** (k - i) / np.sqrt(i) for i in range(1, k + 1))
+        np.testing.assert_allclose(s[1], want, rtol=1e-12)
