@@ -22,9 +22,10 @@ layout restated here:
   n), used by mocat/src/metrics.py:103 for KSD minibatches): k1, k2 = split(key); hi, lo =
   bits(k1), bits(k2); span = maxval - minval (uint32; 1 if maxval <= minval); m = (2^16 mod
   span)^2 mod span; result = minval + ((hi mod span) m + lo mod span) mod span, uint32 arithmetic
-  wrapping at 32 bits (jax's legacy randint). No vector in the reference pins it: "parity
-  unpinned"; the tests check the special cases the formula fixes (span a power of two <= 2^16
-  gives lo mod span, span 1 gives minval) and the range.
+  wrapping at 32 bits (jax's legacy randint). No vector in the reference pins it. For spans up to
+  2^16 the combination cannot wrap and equals the 64-bit draw (hi 2^32 + lo) mod span, which the
+  tests check with Python integers; spans above 2^16 (where the product wraps) are "parity
+  unpinned" beyond the range check.
 * categorical (Gumbel-max, used by mocat/src/ssm/filtering.py:196-199 and
   mocat/src/transport/smc.py:61-71): argmax_j(logits_j - log(-log(U_ij))), U ~ uniform(key,
   (n, N), tiny, 1). Pinned only by brute-force agreement on tiny inputs; exact indices depend on

@@ -102,3 +102,18 @@ def test_randint_special_cases():
         assert r.min() >= lo and r.max() < hi
         assert len(np.unique(r)) > min(hi - lo, 20000) // 4
     np.testing.assert_array_equal(rng.choice(k, 300, (4, 5)), rng.randint(k, (4, 5), 0, 300))
+
+
+def test_randint_is_the_64_bit_draw_modulo_the_span():
+    """For spans up to 2^16 the legacy combination (hi mod s) * ((2^16 mod s)^2 mod s) + (lo mod s),
+    taken mod s, cannot wrap in uint32, and (2^16)^2 = 2^32, so it equals the 64-bit integer
+    (hi * 2^32 + lo) mod s, with hi and lo the two split keys' bits (computed here with Python
+    integers, none of the formula's steps)."""
+    k = rng.PRNGKey(11)
+    k1, k2 = rng.split(k)
+    hi = rng.bits(k1, (2000,)).astype(object)
+    lo = rng.bits(k2, (2000,)).astype(object)
+    for span in (3, 7, 10, 1000, 4097, 65535, 65536):
+        want = np.array([(int(h) * (1 << 32) + int(l)) % span for h, l in zip(hi, lo)], np.int64)
+        got = rng.randint(k, (2000,), -5, -5 + span).astype(np.int64) + 5
+        np.testing.assert_array_equal(got, want)
