@@ -373,3 +373,27 @@ def test_kalman_filter_and_smoother_match_the_oracle(m):
     np.testing.assert_allclose(sc.cpu().numpy(), oc, rtol=1e-9, atol=1e-12)
     sm2, sc2, lag = m.ssm.run_kalman_smoother_for_marginals_and_lag_ones(model, y, t, filter_output=(mus, covs))
     assert torch.equal(sm2, sm) and lag.shape == (11, 4, 4)
+
+
+def test_chunked_path_wide_ancestor_windows(m):
+    """Observation outliers concentrate the weight on a few particles far apart in index, so some
+    chunk's systematic targets switch between distant sources: its ancestor window exceeds the LDS
+    stage and the search runs over global prefix sums. Ancestors and particles must still equal the
+    one-workgroup path's."""
+    from mocat_amd import datasets
+    model, y, t = datasets.lgssm_config(5, 4, 10)
+    yo = y.cpu().numpy().copy()
+    yo[2] += 40.0
+    yo[5] -= 60.0
+    n = 9000
+    big, one = _run_both(m, [model], [yo], [m.random.PRNGKey(21)], [n], 1.0, "systematic")
+    anc = big[0].ancestors.cpu().numpy()
+    np.testing.assert_array_equal(anc, one[0].ancestors.cpu().numpy())
+    np.testing.assert_array_equal(big[0].value.cpu().numpy(), one[0].value.cpu().numpy())
+    # an outlier step resamples onto few sources spread over the index range (the -60 outlier may also
+    # leave a single source, the fully degenerate case)
+    spread = []
+    for k in (2, 5):
+        src = np.unique(anc[k + 1])
+        spread.append((len(src), int(src.max() - src.min())))
+    assert any(c < n // 10 and w > 2048 for c, w in spread), spread
