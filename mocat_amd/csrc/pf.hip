@@ -974,6 +974,70 @@ MOCAT_DEV void lf_store_partials(const LfArgs& a, int64_t c, LseAcc acc, GradAcc
   }
 }
 
+// Chunk accumulator in two passes, one exponential per value: the workgroup maximum of the values
+// (NaN excluded; a NaN anywhere makes the result's m NaN), then sum exp(v - M) and its square in FP64,
+// lanes by butterflies and waves in order. Every thread gets the result.
+template <int P>
+MOCAT_DEV LseAcc lf_block_lse2(const float (&v)[P], const bool (&live)[P], LfScratch& s) {
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  float m = -INFINITY;
+  int bad = 0;
+#pragma unroll
+  for (int pk = 0; pk < P; ++pk) {
+    if (!live[pk]) continue;
+    if (isnan(v[pk]))
+      bad = 1;
+    else
+      m = fmaxf(m, v[pk]);
+  }
+  m = wave_max(m);
+  bad = __any(bad) ? 1 : 0;
+  __syncthreads();
+  if (lane == 0) {
+    s.fm[w] = m;
+    s.u[w] = (uint64_t)bad;
+  }
+  __syncthreads();
+  float M = s.fm[0];
+  bool anybad = s.u[0] != 0;
+  for (int i = 1; i < LW; ++i) {
+    M = fmaxf(M, s.fm[i]);
+    anybad = anybad || s.u[i] != 0;
+  }
+  double s1 = 0.0, s2 = 0.0;
+  if (M != -INFINITY) {
+#pragma unroll
+    for (int pk = 0; pk < P; ++pk) {
+      if (!live[pk] || !(v[pk] > -INFINITY)) continue;  // -inf and NaN add nothing
+      const double e = exp((double)v[pk] - (double)M);
+      s1 += e;
+      s2 += e * e;
+    }
+  }
+  s1 = wave_sum(s1);
+  s2 = wave_sum(s2);
+  __syncthreads();
+  if (lane == 0) {
+    s.d1[w] = s1;
+    s.d2[w] = s2;
+  }
+  __syncthreads();
+  double r1 = s.d1[0], r2 = s.d2[0];
+  for (int i = 1; i < LW; ++i) {
+    r1 += s.d1[i];
+    r2 += s.d2[i];
+  }
+  return LseAcc{anybad ? NAN : M, r1, r2};
+}
+
+MOCAT_DEV void lf_store_lse(const LfArgs& a, int64_t c, LseAcc r) {
+  if (threadIdx.x == 0) {
+    a.part_m[c] = r.m;
+    a.part_s1[c] = r.s1;
+    a.part_s2[c] = r.s2;
+  }
+}
+
 template <class Model, int D, int DY, bool GRAD>
 __global__ __launch_bounds__(LT) void lf_init_kernel(LfArgs a) {
   __shared__ LfScratch s;
@@ -990,6 +1054,13 @@ __global__ __launch_bounds__(LT) void lf_init_kernel(LfArgs a) {
   LseAcc acc{-INFINITY, 0.0, 0.0};
   GradAcc gacc{-INFINITY, 0.0, 0.0, 0.0, 0.0};
   const Key sub0 = Model::kOptimal ? split_at(sub, (uint32_t)n, 0u) : Key{0u, 0u};
+  float lv[PPT];
+  bool lvalid[PPT];
+#pragma unroll
+  for (int pk = 0; pk < PPT; ++pk) {
+    lv[pk] = 0.f;
+    lvalid[pk] = false;
+  }
   for (int pk = 0; pk < PPT; ++pk) {
     const int64_t i = (c - F.chunk0) * CH + pk * LT + tid;
     if (i >= n) break;
@@ -1038,7 +1109,12 @@ __global__ __launch_bounds__(LT) void lf_init_kernel(LfArgs a) {
 #pragma unroll
     for (int q = 0; q < D; ++q) xa[i * D + q] = x[q];
     a.lw0[F.state_offset + i] = l;
-    lse_add(acc, l);
+    if constexpr (GRAD) {
+      lse_add(acc, l);
+    } else {
+      lv[pk] = l;
+      lvalid[pk] = true;
+    }
     if (F.out_offset >= 0) {
       if (a.out_value)
 #pragma unroll
@@ -1047,7 +1123,10 @@ __global__ __launch_bounds__(LT) void lf_init_kernel(LfArgs a) {
       if (a.out_anc) a.out_anc[F.out_offset + i] = (int32_t)i;
     }
   }
-  lf_store_partials<GRAD>(a, c, acc, gacc, s);
+  if constexpr (GRAD)
+    lf_store_partials<GRAD>(a, c, acc, gacc, s);
+  else
+    lf_store_lse(a, c, lf_block_lse2<PPT>(lv, lvalid, s));
 }
 
 // The prefix sums are integers below 2^52 (exact in FP64), so for a target T >= 0 the test
@@ -1073,19 +1152,71 @@ __global__ __launch_bounds__(LT) void lf_prep_kernel(LfArgs a, int64_t t) {
   const int64_t n = F.n, nchunk = (n + CH - 1) / CH;
   LseAcc acc{-INFINITY, 0.0, 0.0};
   double g[3] = {0.0, 0.0, 0.0};
-  for (int64_t k = threadIdx.x; k < nchunk; k += LT) {
-    const int64_t cc = F.chunk0 + k;
-    const LseAcc p{a.part_m[cc], a.part_s1[cc], a.part_s2[cc]};
-    if constexpr (GRAD) {
+  LseAcc r;
+  if constexpr (GRAD) {
+    for (int64_t k = threadIdx.x; k < nchunk; k += LT) {
+      const int64_t cc = F.chunk0 + k;
+      const LseAcc p{a.part_m[cc], a.part_s1[cc], a.part_s2[cc]};
       // the tangent sums follow the LSE rescaling of their chunk
       const float mnew = fmaxf(acc.m, p.m);
       const double fo = acc.m == -INFINITY ? 0.0 : exp((double)acc.m - (double)mnew);
       const double fp = p.m == -INFINITY ? 0.0 : exp((double)p.m - (double)mnew);
       for (int q = 0; q < 3; ++q) g[q] = g[q] * fo + a.part_g[cc * 3 + q] * fp;
+      acc = lse_combine(acc, p);
     }
-    acc = lse_combine(acc, p);
+    r = lf_block_lse(acc, s);
+  } else {
+    // two passes over the chunk partials: their maximum, then the rescaled sums (one exponential each)
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+    float m = -INFINITY;
+    int bad = 0;
+    for (int64_t k = threadIdx.x; k < nchunk; k += LT) {
+      const float pm = a.part_m[F.chunk0 + k];
+      if (isnan(pm))
+        bad = 1;
+      else
+        m = fmaxf(m, pm);
+    }
+    m = wave_max(m);
+    bad = __any(bad) ? 1 : 0;
+    __syncthreads();
+    if (lane == 0) {
+      s.fm[w] = m;
+      s.u[w] = (uint64_t)bad;
+    }
+    __syncthreads();
+    float M = s.fm[0];
+    bool anybad = s.u[0] != 0;
+    for (int i = 1; i < LW; ++i) {
+      M = fmaxf(M, s.fm[i]);
+      anybad = anybad || s.u[i] != 0;
+    }
+    double s1 = 0.0, s2 = 0.0;
+    if (M != -INFINITY) {
+      for (int64_t k = threadIdx.x; k < nchunk; k += LT) {
+        const int64_t cc = F.chunk0 + k;
+        const float pm = a.part_m[cc];
+        if (!(pm > -INFINITY)) continue;
+        const double f = exp((double)pm - (double)M);
+        s1 += a.part_s1[cc] * f;
+        s2 += a.part_s2[cc] * f * f;
+      }
+    }
+    s1 = wave_sum(s1);
+    s2 = wave_sum(s2);
+    __syncthreads();
+    if (lane == 0) {
+      s.d1[w] = s1;
+      s.d2[w] = s2;
+    }
+    __syncthreads();
+    double r1 = s.d1[0], r2 = s.d2[0];
+    for (int i = 1; i < LW; ++i) {
+      r1 += s.d1[i];
+      r2 += s.d2[i];
+    }
+    r = LseAcc{anybad ? NAN : M, r1, r2};
   }
-  const LseAcc r = lf_block_lse(acc, s);
   double lse, lse2;
   if (isnan(r.m)) {
     lse = NAN;
@@ -1516,7 +1647,7 @@ __global__ __launch_bounds__(LT) void lf_step_kernel(LfArgs a, int64_t t, int ce
 #pragma unroll
     for (int q = 0; q < D; ++q) xb[i * D + q] = xn[pk][q];
     lwb[i] = l[pk];
-    lse_add(acc, l[pk]);
+    if constexpr (GRAD) lse_add(acc, l[pk]);
     if (F.out_offset >= 0) {
       const int64_t o = F.out_offset + t * n + i;
       if (a.out_value)
@@ -1526,7 +1657,14 @@ __global__ __launch_bounds__(LT) void lf_step_kernel(LfArgs a, int64_t t, int ce
       if (a.out_anc) a.out_anc[o] = (int32_t)srcs[pk];
     }
   }
-  lf_store_partials<GRAD>(a, c, acc, gacc, s);
+  if constexpr (GRAD) {
+    lf_store_partials<GRAD>(a, c, acc, gacc, s);
+  } else {
+#pragma unroll
+    for (int pk = 0; pk < PPT; ++pk)
+      if (!live[pk]) l[pk] = 0.f;  // unused lanes: defined values, masked out
+    lf_store_lse(a, c, lf_block_lse2<PPT>(l, live, s));
+  }
 }
 
 // Final particles of filters whose last step wrote buffer 1 are copied to buffer 0 (as pf_kernel).
