@@ -886,52 +886,6 @@ struct LfScratch {
 };
 
 // chunk-level reductions in a fixed order (wave butterflies, then waves in order)
-MOCAT_DEV LseAcc lf_block_lse(LseAcc a, LfScratch& s) {
-  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
-  for (int o = kWave / 2; o > 0; o >>= 1) {
-    LseAcc b;
-    b.m = __shfl_xor(a.m, o, kWave);
-    b.s1 = __shfl_xor(a.s1, o, kWave);
-    b.s2 = __shfl_xor(a.s2, o, kWave);
-    a = lse_combine(a, b);
-  }
-  __syncthreads();
-  if (lane == 0) {
-    s.fm[w] = a.m;
-    s.d1[w] = a.s1;
-    s.d2[w] = a.s2;
-  }
-  __syncthreads();
-  LseAcc r{s.fm[0], s.d1[0], s.d2[0]};
-  for (int i = 1; i < LW; ++i) r = lse_combine(r, LseAcc{s.fm[i], s.d1[i], s.d2[i]});
-  return r;
-}
-
-MOCAT_DEV GradAcc lf_block_grad(GradAcc a, LfScratch& s) {
-  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
-  for (int o = kWave / 2; o > 0; o >>= 1) {
-    GradAcc b;
-    b.m = __shfl_xor(a.m, o, kWave);
-    b.s1 = __shfl_xor(a.s1, o, kWave);
-    b.g0 = __shfl_xor(a.g0, o, kWave);
-    b.g1 = __shfl_xor(a.g1, o, kWave);
-    b.g2 = __shfl_xor(a.g2, o, kWave);
-    a = grad_combine(a, b);
-  }
-  __syncthreads();
-  if (lane == 0) {
-    s.fm[w] = a.m;
-    s.d1[w] = a.s1;
-    s.g[w][0] = a.g0;
-    s.g[w][1] = a.g1;
-    s.g[w][2] = a.g2;
-  }
-  __syncthreads();
-  GradAcc r{s.fm[0], s.d1[0], s.g[0][0], s.g[0][1], s.g[0][2]};
-  for (int i = 1; i < LW; ++i) r = grad_combine(r, GradAcc{s.fm[i], s.d1[i], s.g[i][0], s.g[i][1], s.g[i][2]});
-  return r;
-}
-
 MOCAT_DEV uint64_t lf_block_sum(uint64_t v, LfScratch& s) {
   const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
   v = wave_sum(v);
@@ -952,26 +906,6 @@ MOCAT_DEV uint64_t lf_block_inclusive_scan(uint64_t v, LfScratch& s) {
   uint64_t off = 0;
   for (int i = 0; i < w; ++i) off += s.u[i];
   return off + incl;
-}
-
-// Stores a chunk's accumulators (LSE and, with GRAD, the weighted tangent sums relative to its max).
-template <bool GRAD>
-MOCAT_DEV void lf_store_partials(const LfArgs& a, int64_t c, LseAcc acc, GradAcc gacc, LfScratch& s) {
-  const LseAcc r = lf_block_lse(acc, s);
-  GradAcc gr{};
-  if constexpr (GRAD) gr = lf_block_grad(gacc, s);
-  if (threadIdx.x == 0) {
-    a.part_m[c] = r.m;
-    a.part_s1[c] = r.s1;
-    a.part_s2[c] = r.s2;
-    if constexpr (GRAD) {
-      // rescale the tangent sums to the LSE accumulator's max (both track the same weights)
-      const double f = (gr.m == -INFINITY || r.m == -INFINITY) ? 0.0 : exp((double)gr.m - (double)r.m);
-      a.part_g[c * 3 + 0] = gr.g0 * f;
-      a.part_g[c * 3 + 1] = gr.g1 * f;
-      a.part_g[c * 3 + 2] = gr.g2 * f;
-    }
-  }
 }
 
 // Chunk accumulator in two passes, one exponential per value: the workgroup maximum of the values
@@ -1030,6 +964,83 @@ MOCAT_DEV LseAcc lf_block_lse2(const float (&v)[P], const bool (&live)[P], LfScr
   return LseAcc{anybad ? NAN : M, r1, r2};
 }
 
+// As lf_block_lse2, also summing exp(v - M) * t[q] for three tangents (returned in g, relative to M).
+template <int P>
+MOCAT_DEV LseAcc lf_block_lse2g(const float (&v)[P], const float (&t)[P][3], const bool (&live)[P], LfScratch& s,
+                                double (&g)[3]) {
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  float m = -INFINITY;
+  int bad = 0;
+#pragma unroll
+  for (int pk = 0; pk < P; ++pk) {
+    if (!live[pk]) continue;
+    if (isnan(v[pk]))
+      bad = 1;
+    else
+      m = fmaxf(m, v[pk]);
+  }
+  m = wave_max(m);
+  bad = __any(bad) ? 1 : 0;
+  __syncthreads();
+  if (lane == 0) {
+    s.fm[w] = m;
+    s.u[w] = (uint64_t)bad;
+  }
+  __syncthreads();
+  float M = s.fm[0];
+  bool anybad = s.u[0] != 0;
+  for (int i = 1; i < LW; ++i) {
+    M = fmaxf(M, s.fm[i]);
+    anybad = anybad || s.u[i] != 0;
+  }
+  double s1 = 0.0, s2 = 0.0, g0 = 0.0, g1 = 0.0, g2 = 0.0;
+  if (M != -INFINITY) {
+#pragma unroll
+    for (int pk = 0; pk < P; ++pk) {
+      if (!live[pk] || !(v[pk] > -INFINITY)) continue;
+      const double e = exp((double)v[pk] - (double)M);
+      s1 += e;
+      s2 += e * e;
+      g0 += e * t[pk][0];
+      g1 += e * t[pk][1];
+      g2 += e * t[pk][2];
+    }
+  }
+  s1 = wave_sum(s1);
+  s2 = wave_sum(s2);
+  g0 = wave_sum(g0);
+  g1 = wave_sum(g1);
+  g2 = wave_sum(g2);
+  __syncthreads();
+  if (lane == 0) {
+    s.d1[w] = s1;
+    s.d2[w] = s2;
+    s.g[w][0] = g0;
+    s.g[w][1] = g1;
+    s.g[w][2] = g2;
+  }
+  __syncthreads();
+  double r1 = s.d1[0], r2 = s.d2[0];
+  g[0] = s.g[0][0];
+  g[1] = s.g[0][1];
+  g[2] = s.g[0][2];
+  for (int i = 1; i < LW; ++i) {
+    r1 += s.d1[i];
+    r2 += s.d2[i];
+    for (int q = 0; q < 3; ++q) g[q] += s.g[i][q];
+  }
+  return LseAcc{anybad ? NAN : M, r1, r2};
+}
+
+MOCAT_DEV void lf_store_lse_g(const LfArgs& a, int64_t c, LseAcc r, const double (&g)[3]) {
+  if (threadIdx.x == 0) {
+    a.part_m[c] = r.m;
+    a.part_s1[c] = r.s1;
+    a.part_s2[c] = r.s2;
+    for (int q = 0; q < 3; ++q) a.part_g[c * 3 + q] = g[q];
+  }
+}
+
 MOCAT_DEV void lf_store_lse(const LfArgs& a, int64_t c, LseAcc r) {
   if (threadIdx.x == 0) {
     a.part_m[c] = r.m;
@@ -1051,14 +1062,13 @@ __global__ __launch_bounds__(LT) void lf_init_kernel(LfArgs a) {
   const Key key{F.key[0], F.key[1]};
   const Key sub = (F.flags & kPfInitKey) ? key : split_at(key, 2, 1);
   constexpr uint32_t hD = (D + 1) / 2;
-  LseAcc acc{-INFINITY, 0.0, 0.0};
-  GradAcc gacc{-INFINITY, 0.0, 0.0, 0.0, 0.0};
   const Key sub0 = Model::kOptimal ? split_at(sub, (uint32_t)n, 0u) : Key{0u, 0u};
-  float lv[PPT];
+  float lv[PPT], lt[PPT][3];
   bool lvalid[PPT];
 #pragma unroll
   for (int pk = 0; pk < PPT; ++pk) {
     lv[pk] = 0.f;
+    lt[pk][0] = lt[pk][1] = lt[pk][2] = 0.f;
     lvalid[pk] = false;
   }
   for (int pk = 0; pk < PPT; ++pk) {
@@ -1092,7 +1102,7 @@ __global__ __launch_bounds__(LT) void lf_init_kernel(LfArgs a) {
       ta[i * 5 + 2] = 0.f;
       ta[i * 5 + 3] = 0.f;
       ta[i * 5 + 4] = -dldb;
-      grad_add(gacc, l, 0.f, 0.f, -dldb);
+      lt[pk][2] = -dldb;
     } else {
       const Key ki = split_at(sub, (uint32_t)n, (uint32_t)i);
 #pragma unroll
@@ -1109,12 +1119,8 @@ __global__ __launch_bounds__(LT) void lf_init_kernel(LfArgs a) {
 #pragma unroll
     for (int q = 0; q < D; ++q) xa[i * D + q] = x[q];
     a.lw0[F.state_offset + i] = l;
-    if constexpr (GRAD) {
-      lse_add(acc, l);
-    } else {
-      lv[pk] = l;
-      lvalid[pk] = true;
-    }
+    lv[pk] = l;
+    lvalid[pk] = true;
     if (F.out_offset >= 0) {
       if (a.out_value)
 #pragma unroll
@@ -1123,10 +1129,13 @@ __global__ __launch_bounds__(LT) void lf_init_kernel(LfArgs a) {
       if (a.out_anc) a.out_anc[F.out_offset + i] = (int32_t)i;
     }
   }
-  if constexpr (GRAD)
-    lf_store_partials<GRAD>(a, c, acc, gacc, s);
-  else
+  if constexpr (GRAD) {
+    double g[3];
+    const LseAcc r = lf_block_lse2g<PPT>(lv, lt, lvalid, s, g);
+    lf_store_lse_g(a, c, r, g);
+  } else {
     lf_store_lse(a, c, lf_block_lse2<PPT>(lv, lvalid, s));
+  }
 }
 
 // The prefix sums are integers below 2^52 (exact in FP64), so for a target T >= 0 the test
@@ -1150,23 +1159,11 @@ __global__ __launch_bounds__(LT) void lf_prep_kernel(LfArgs a, int64_t t) {
   const LfDesc F = a.desc[fi];
   if (t - 1 >= F.T) return;
   const int64_t n = F.n, nchunk = (n + CH - 1) / CH;
-  LseAcc acc{-INFINITY, 0.0, 0.0};
-  double g[3] = {0.0, 0.0, 0.0};
+  double gsum[3] = {0.0, 0.0, 0.0};  // tangent sums relative to the filter's maximum (GRAD)
   LseAcc r;
-  if constexpr (GRAD) {
-    for (int64_t k = threadIdx.x; k < nchunk; k += LT) {
-      const int64_t cc = F.chunk0 + k;
-      const LseAcc p{a.part_m[cc], a.part_s1[cc], a.part_s2[cc]};
-      // the tangent sums follow the LSE rescaling of their chunk
-      const float mnew = fmaxf(acc.m, p.m);
-      const double fo = acc.m == -INFINITY ? 0.0 : exp((double)acc.m - (double)mnew);
-      const double fp = p.m == -INFINITY ? 0.0 : exp((double)p.m - (double)mnew);
-      for (int q = 0; q < 3; ++q) g[q] = g[q] * fo + a.part_g[cc * 3 + q] * fp;
-      acc = lse_combine(acc, p);
-    }
-    r = lf_block_lse(acc, s);
-  } else {
-    // two passes over the chunk partials: their maximum, then the rescaled sums (one exponential each)
+  {
+    // two passes over the chunk partials: their maximum, then the rescaled sums (one exponential each;
+    // the tangent sums of a chunk are relative to its maximum, like its weight sums)
     const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
     float m = -INFINITY;
     int bad = 0;
@@ -1191,7 +1188,7 @@ __global__ __launch_bounds__(LT) void lf_prep_kernel(LfArgs a, int64_t t) {
       M = fmaxf(M, s.fm[i]);
       anybad = anybad || s.u[i] != 0;
     }
-    double s1 = 0.0, s2 = 0.0;
+    double s1 = 0.0, s2 = 0.0, g[3] = {0.0, 0.0, 0.0};
     if (M != -INFINITY) {
       for (int64_t k = threadIdx.x; k < nchunk; k += LT) {
         const int64_t cc = F.chunk0 + k;
@@ -1200,20 +1197,30 @@ __global__ __launch_bounds__(LT) void lf_prep_kernel(LfArgs a, int64_t t) {
         const double f = exp((double)pm - (double)M);
         s1 += a.part_s1[cc] * f;
         s2 += a.part_s2[cc] * f * f;
+        if constexpr (GRAD)
+          for (int q = 0; q < 3; ++q) g[q] += a.part_g[cc * 3 + q] * f;
       }
     }
     s1 = wave_sum(s1);
     s2 = wave_sum(s2);
+    if constexpr (GRAD)
+      for (int q = 0; q < 3; ++q) g[q] = wave_sum(g[q]);
     __syncthreads();
     if (lane == 0) {
       s.d1[w] = s1;
       s.d2[w] = s2;
+      if constexpr (GRAD)
+        for (int q = 0; q < 3; ++q) s.g[w][q] = g[q];
     }
     __syncthreads();
     double r1 = s.d1[0], r2 = s.d2[0];
+    if constexpr (GRAD)
+      for (int q = 0; q < 3; ++q) gsum[q] = s.g[0][q];
     for (int i = 1; i < LW; ++i) {
       r1 += s.d1[i];
       r2 += s.d2[i];
+      if constexpr (GRAD)
+        for (int q = 0; q < 3; ++q) gsum[q] += s.g[i][q];
     }
     r = LseAcc{anybad ? NAN : M, r1, r2};
   }
@@ -1237,19 +1244,8 @@ __global__ __launch_bounds__(LT) void lf_prep_kernel(LfArgs a, int64_t t) {
   const int shift = quant_shift(__expf(r.m - m), n);
   if (c == F.chunk0) {
     double gm[3] = {0.0, 0.0, 0.0};
-    if constexpr (GRAD) {
-      // bring every thread's tangent sums to the block max, then add: wave butterflies, waves in order
-      const double f = acc.m == -INFINITY ? 0.0 : exp((double)acc.m - (double)r.m);
-      __shared__ double gs[LW][3];
-      double v[3];
-      for (int q = 0; q < 3; ++q) v[q] = wave_sum(g[q] * f);
-      if ((threadIdx.x & (kWave - 1)) == 0)
-        for (int q = 0; q < 3; ++q) gs[threadIdx.x / kWave][q] = v[q];
-      __syncthreads();
-      for (int k = 0; k < LW; ++k)
-        for (int q = 0; q < 3; ++q) gm[q] += gs[k][q];
-      for (int q = 0; q < 3; ++q) gm[q] /= r.s1;
-    }
+    if constexpr (GRAD)
+      for (int q = 0; q < 3; ++q) gm[q] = gsum[q] / r.s1;
     if (threadIdx.x == 0) {
       const int64_t tp = t - 1;
       LfState st = a.st[fi];
@@ -1436,8 +1432,6 @@ __global__ __launch_bounds__(LT) void lf_step_kernel(LfArgs a, int64_t t, int ce
   float* tb = GRAD ? (odd ? a.tan0 : a.tan1) + F.state_offset * 5 : nullptr;
   const float* ycur = a.y + F.y_offset + t * DY;
   constexpr uint32_t hD = (D + 1) / 2;
-  LseAcc acc{-INFINITY, 0.0, 0.0};
-  GradAcc gacc{-INFINITY, 0.0, 0.0, 0.0, 0.0};
 
   // Ancestors by a two-level search in the exact prefix sums, without long chains of dependent
   // global loads: the last sum of every SG-particle block is staged in LDS and searched first, then
@@ -1642,12 +1636,10 @@ __global__ __launch_bounds__(LT) void lf_step_kernel(LfArgs a, int64_t t, int ce
       tb[i * 5 + 2] = gd[pk][0];
       tb[i * 5 + 3] = gd[pk][1];
       tb[i * 5 + 4] = gd[pk][2];
-      grad_add(gacc, l[pk], gd[pk][0], gd[pk][1], gd[pk][2]);
     }
 #pragma unroll
     for (int q = 0; q < D; ++q) xb[i * D + q] = xn[pk][q];
     lwb[i] = l[pk];
-    if constexpr (GRAD) lse_add(acc, l[pk]);
     if (F.out_offset >= 0) {
       const int64_t o = F.out_offset + t * n + i;
       if (a.out_value)
@@ -1657,12 +1649,18 @@ __global__ __launch_bounds__(LT) void lf_step_kernel(LfArgs a, int64_t t, int ce
       if (a.out_anc) a.out_anc[o] = (int32_t)srcs[pk];
     }
   }
+#pragma unroll
+  for (int pk = 0; pk < PPT; ++pk)
+    if (!live[pk]) l[pk] = 0.f;  // unused lanes: defined values, masked out
   if constexpr (GRAD) {
-    lf_store_partials<GRAD>(a, c, acc, gacc, s);
-  } else {
+    float tv[PPT][3];
 #pragma unroll
     for (int pk = 0; pk < PPT; ++pk)
-      if (!live[pk]) l[pk] = 0.f;  // unused lanes: defined values, masked out
+      for (int q = 0; q < 3; ++q) tv[pk][q] = live[pk] ? gd[pk][q] : 0.f;
+    double g[3];
+    const LseAcc r = lf_block_lse2g<PPT>(l, tv, live, s, g);
+    lf_store_lse_g(a, c, r, g);
+  } else {
     lf_store_lse(a, c, lf_block_lse2<PPT>(l, live, s));
   }
 }
