@@ -846,6 +846,15 @@ struct LfState {
   int pad;
 };
 
+// Per filter and step t >= 1: the step key (split(key, T-1)[t-1], or the filter key with
+// kPfStepKey), the optimal proposal's first split key split(step key, n)[0], and the systematic
+// uniform; derived for all steps at once by the init launch, read by the later launches.
+struct StepKey {
+  uint32_t k[4];
+  float su;
+  uint32_t pad;
+};
+
 struct LfArgs {
   const LfDesc* desc;
   const int32_t* chunk_filter;  // chunk -> index into desc
@@ -877,6 +886,7 @@ struct LfArgs {
   LfState* st;       // [n_large]
   uint64_t* cends;   // [n_chunks * CH / SG] prefix sums at the SG-block ends, per filter from chunk0 * CH / SG
   int64_t* bounds;   // [n_chunks] systematic: ancestor of each chunk's first target
+  StepKey* skeys;    // [t_offset + t] per filter
 };
 
 struct LfScratch {
@@ -1063,6 +1073,24 @@ __global__ __launch_bounds__(LT) void lf_init_kernel(LfArgs a) {
   const Key sub = (F.flags & kPfInitKey) ? key : split_at(key, 2, 1);
   constexpr uint32_t hD = (D + 1) / 2;
   const Key sub0 = Model::kOptimal ? split_at(sub, (uint32_t)n, 0u) : Key{0u, 0u};
+  if (c == F.chunk0) {
+    const Key rk = (F.flags & kPfResume) ? key : split_at(key, 2, 0);
+    for (int64_t t = 1 + tid; t < F.T; t += LT) {
+      const Key ikk = (F.flags & kPfStepKey) ? key : split_at(rk, (uint32_t)(F.T - 1), (uint32_t)(t - 1));
+      StepKey sk;
+      sk.k[0] = ikk.a;
+      sk.k[1] = ikk.b;
+      sk.k[2] = sk.k[3] = 0u;
+      if constexpr (Model::kOptimal) {
+        const Key k0 = split_at(ikk, (uint32_t)n, 0u);
+        sk.k[2] = k0.a;
+        sk.k[3] = k0.b;
+      }
+      sk.su = uniform_from_bits(bits_at(ikk, 1, 0), 0.f, 1.f);
+      sk.pad = 0u;
+      a.skeys[F.t_offset + t] = sk;
+    }
+  }
   float lv[PPT], lt[PPT][3];
   bool lvalid[PPT];
 #pragma unroll
@@ -1300,14 +1328,7 @@ __global__ __launch_bounds__(LT) void lf_csum_kernel(LfArgs a, int64_t t) {
   }
   off = lf_block_sum(off, s);
   wsum = lf_block_sum(wsum, s);
-  // systematic: the step key's uniform, as the step kernel derives it (thread 0, during the scan)
-  __shared__ float su;
-  if (a.resampling == 0 && threadIdx.x == 0) {
-    const Key key{F.key[0], F.key[1]};
-    const Key rk = (F.flags & kPfResume) ? key : split_at(key, 2, 0);
-    const Key ikk = (F.flags & kPfStepKey) ? key : split_at(rk, (uint32_t)(F.T - 1), (uint32_t)(t - 1));
-    su = uniform_from_bits(bits_at(ikk, 1, 0), 0.f, 1.f);
-  }
+  const float su = a.skeys[F.t_offset + t].su;  // the step's systematic uniform (init launch)
   // thread tid owns the PPT consecutive particles from local * CH + tid * PPT
   const int64_t i0 = local * CH + (int64_t)threadIdx.x * PPT;
   const float* lw = lf_lw_prev(a, t) + F.state_offset;
@@ -1460,23 +1481,11 @@ __global__ __launch_bounds__(LT) void lf_step_kernel(LfArgs a, int64_t t, int ce
   // the chunk-end sums go first: their loads are in flight while thread 0 derives the step key
   const uint64_t* CE = a.cends + F.chunk0 * (CH / SG);
   if (two_level) stage_lds<8>(cend, nsg, [&](int64_t k) { return CE[k]; });
-  // the step key (and the systematic uniform) are the same for the whole chunk: derived once
-  __shared__ uint32_t sk[4];
-  __shared__ float su;
+  // the step key, the optimal proposal's first split key and the systematic uniform (init launch)
+  const StepKey skv = a.skeys[F.t_offset + t];
+  const uint32_t sk[4] = {skv.k[0], skv.k[1], skv.k[2], skv.k[3]};
+  const float su = skv.su;
   load_shared_params(model, a.params + F.param_offset);
-  if (tid == 0) {
-    const Key key{F.key[0], F.key[1]};
-    const Key rk = (F.flags & kPfResume) ? key : split_at(key, 2, 0);
-    const Key ikk = (F.flags & kPfStepKey) ? key : split_at(rk, (uint32_t)(F.T - 1), (uint32_t)(t - 1));
-    sk[0] = ikk.a;
-    sk[1] = ikk.b;
-    su = uniform_from_bits(bits_at(ikk, 1, 0), 0.f, 1.f);
-    if constexpr (Model::kOptimal) {
-      const Key k0 = split_at(ikk, (uint32_t)n, 0u);
-      sk[2] = k0.a;
-      sk[3] = k0.b;
-    }
-  }
   __syncthreads();
   const Key ik{sk[0], sk[1]};
   auto cend_at = [&](int64_t k) { return cend[k]; };
@@ -1710,7 +1719,8 @@ struct LfLayout {
   int n_large = 0;
   int cend_slots = 1;  // SG-block end sums the step kernel stages: the largest filter's, up to kChunkEnds
   int64_t n_chunks = 0, max_T = 0;
-  int64_t o_desc, o_cf, o_m, o_s1, o_s2, o_g, o_tot, o_st, o_cend, o_bnd, bytes;
+  int64_t n_tk = 0;  // step-key slots: the largest t_offset + T of the flagged filters
+  int64_t o_desc, o_cf, o_m, o_s1, o_s2, o_g, o_tot, o_st, o_cend, o_bnd, o_keys, bytes;
 };
 
 int64_t lf_align(int64_t v) { return (v + 255) / 256 * 256; }
@@ -1722,6 +1732,7 @@ LfLayout lf_layout(const mocat_pf_filter* filters, int n_filters) {
     ++L.n_large;
     L.n_chunks += (filters[f].n + CH - 1) / CH;
     L.max_T = std::max<int64_t>(L.max_T, filters[f].T);
+    L.n_tk = std::max<int64_t>(L.n_tk, filters[f].t_offset + filters[f].T);
     L.cend_slots = (int)std::min<int64_t>(kChunkEnds, std::max<int64_t>(L.cend_slots, (filters[f].n + SG - 1) / SG));
   }
   int64_t o = 0;
@@ -1745,6 +1756,8 @@ LfLayout lf_layout(const mocat_pf_filter* filters, int n_filters) {
   o = lf_align(o + L.n_chunks * (CH / SG) * 8);
   L.o_bnd = o;
   o = lf_align(o + L.n_chunks * 8);
+  L.o_keys = o;
+  o = lf_align(o + L.n_tk * (int64_t)sizeof(StepKey));
   L.bytes = o;
   return L;
 }
@@ -1855,7 +1868,8 @@ int mocat_pf_run_large(int model, int d, int dy, const mocat_pf_filter* host_fil
            reinterpret_cast<float*>(ws + L.o_m), reinterpret_cast<double*>(ws + L.o_s1),
            reinterpret_cast<double*>(ws + L.o_s2), reinterpret_cast<double*>(ws + L.o_g),
            reinterpret_cast<uint64_t*>(ws + L.o_tot), reinterpret_cast<LfState*>(ws + L.o_st),
-           reinterpret_cast<uint64_t*>(ws + L.o_cend), reinterpret_cast<int64_t*>(ws + L.o_bnd)};
+           reinterpret_cast<uint64_t*>(ws + L.o_cend), reinterpret_cast<int64_t*>(ws + L.o_bnd),
+           reinterpret_cast<StepKey*>(ws + L.o_keys)};
   if (model == 1) {
     if (d != 1 || dy != 1) return MOCAT_ERR_ARG;
     return out_grad ? launch_pf_large<Sv, 1, 1, true>(a, L.max_T, L.cend_slots, stream)
