@@ -260,56 +260,6 @@ struct LseAcc {
   double s1, s2;
 };
 
-MOCAT_DEV LseAcc lse_combine(LseAcc a, LseAcc b) {
-  LseAcc r;
-  r.m = fmaxf(a.m, b.m);
-  if (r.m == -INFINITY) {
-    r.s1 = 0.0;
-    r.s2 = 0.0;
-    return r;
-  }
-  const double fa = a.m == -INFINITY ? 0.0 : exp((double)a.m - (double)r.m);
-  const double fb = b.m == -INFINITY ? 0.0 : exp((double)b.m - (double)r.m);
-  r.s1 = a.s1 * fa + b.s1 * fb;
-  r.s2 = a.s2 * fa * fa + b.s2 * fb * fb;
-  return r;
-}
-
-MOCAT_DEV void lse_add(LseAcc& a, float v) {
-  if (v == -INFINITY || isnan(v)) {
-    if (isnan(v)) a.m = NAN;
-    return;
-  }
-  if (v > a.m) {
-    const double f = a.m == -INFINITY ? 0.0 : exp((double)a.m - (double)v);
-    a.s1 = a.s1 * f + 1.0;
-    a.s2 = a.s2 * f * f + 1.0;
-    a.m = v;
-  } else {
-    const double e = exp((double)v - (double)a.m);
-    a.s1 += e;
-    a.s2 += e * e;
-  }
-}
-
-struct GradAcc {
-  float m;
-  double s1, g0, g1, g2;
-};
-
-MOCAT_DEV GradAcc grad_combine(GradAcc a, GradAcc b) {
-  GradAcc r;
-  r.m = fmaxf(a.m, b.m);
-  if (r.m == -INFINITY) return GradAcc{-INFINITY, 0.0, 0.0, 0.0, 0.0};
-  const double fa = a.m == -INFINITY ? 0.0 : exp((double)a.m - (double)r.m);
-  const double fb = b.m == -INFINITY ? 0.0 : exp((double)b.m - (double)r.m);
-  r.s1 = a.s1 * fa + b.s1 * fb;
-  r.g0 = a.g0 * fa + b.g0 * fb;
-  r.g1 = a.g1 * fa + b.g1 * fb;
-  r.g2 = a.g2 * fa + b.g2 * fb;
-  return r;
-}
-
 struct BlockScratch {
   float fm[NW];
   double d1[NW], d2[NW];
@@ -321,79 +271,91 @@ struct BlockScratch {
 };
 
 // Reduce per-thread LseAcc across the block; returns (lse, lse2) to all threads.
-MOCAT_DEV void block_lse(LseAcc a, BlockScratch& s, double& lse, double& lse2) {
+// Log-sum-exp of the log-weights (and, with GRAD, the weighted means of the tangents) over the
+// thread's particles i = tid, tid + NT, ... < n, whose log-weights (and tangents, 5 floats per
+// particle, the last three taken) the thread has just written: the workgroup maximum of the values (m_local: the thread's maximum,
+// NaN excluded; bad: a NaN was seen), then one FP64 exponential per particle and plain FP64 sums,
+// lanes by butterflies and waves in order.
+template <bool GRAD>
+MOCAT_DEV void block_lse2(const float* lw, const float* tan, int64_t n, float m_local, int bad, BlockScratch& s,
+                          double& lse, double& lse2, double* gout) {
   const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
-  for (int o = kWave / 2; o > 0; o >>= 1) {
-    LseAcc b;
-    b.m = __shfl_xor(a.m, o, kWave);
-    b.s1 = __shfl_xor(a.s1, o, kWave);
-    b.s2 = __shfl_xor(a.s2, o, kWave);
-    a = lse_combine(a, b);
+  float m = wave_max(m_local);
+  bad = __any(bad) ? 1 : 0;
+  __syncthreads();
+  if (lane == 0) {
+    s.fm[w] = m;
+    s.u64[w] = (uint64_t)bad;
+  }
+  __syncthreads();
+  float M = s.fm[0];
+  bool anybad = s.u64[0] != 0;
+  for (int k = 1; k < NW; ++k) {
+    M = fmaxf(M, s.fm[k]);
+    anybad = anybad || s.u64[k] != 0;
+  }
+  double s1 = 0.0, s2 = 0.0, g0 = 0.0, g1 = 0.0, g2 = 0.0;
+  if (M != -INFINITY && !anybad) {
+    for (int64_t i = threadIdx.x; i < n; i += NT) {
+      const float v = lw[i];
+      if (!(v > -INFINITY)) continue;
+      const double e = exp((double)v - (double)M);
+      s1 += e;
+      s2 += e * e;
+      if constexpr (GRAD) {
+        g0 += e * tan[i * 5 + 2];
+        g1 += e * tan[i * 5 + 3];
+        g2 += e * tan[i * 5 + 4];
+      }
+    }
+  }
+  s1 = wave_sum(s1);
+  s2 = wave_sum(s2);
+  if constexpr (GRAD) {
+    g0 = wave_sum(g0);
+    g1 = wave_sum(g1);
+    g2 = wave_sum(g2);
   }
   __syncthreads();
   if (lane == 0) {
-    s.fm[w] = a.m;
-    s.d1[w] = a.s1;
-    s.d2[w] = a.s2;
+    s.d1[w] = s1;
+    s.d2[w] = s2;
+    if constexpr (GRAD) {
+      s.g0[w] = g0;
+      s.g1[w] = g1;
+      s.g2[w] = g2;
+    }
   }
   __syncthreads();
-  LseAcc r{s.fm[0], s.d1[0], s.d2[0]};
-  for (int i = 1; i < NW; ++i) r = lse_combine(r, LseAcc{s.fm[i], s.d1[i], s.d2[i]});
-  if (isnan(r.m)) {
+  double r1 = s.d1[0], r2 = s.d2[0], q0 = 0.0, q1 = 0.0, q2 = 0.0;
+  if constexpr (GRAD) {
+    q0 = s.g0[0];
+    q1 = s.g1[0];
+    q2 = s.g2[0];
+  }
+  for (int k = 1; k < NW; ++k) {
+    r1 += s.d1[k];
+    r2 += s.d2[k];
+    if constexpr (GRAD) {
+      q0 += s.g0[k];
+      q1 += s.g1[k];
+      q2 += s.g2[k];
+    }
+  }
+  if (anybad) {
     lse = NAN;
     lse2 = NAN;
-  } else if (r.m == -INFINITY) {
+  } else if (M == -INFINITY) {
     lse = -INFINITY;
     lse2 = -INFINITY;
   } else {
-    lse = (double)r.m + log(r.s1);
-    lse2 = 2.0 * (double)r.m + log(r.s2);
+    lse = (double)M + log(r1);
+    lse2 = 2.0 * (double)M + log(r2);
   }
-}
-
-// Weighted means sum_i w_i t_i / sum_i w_i of the three tangents, w_i = exp(lw_i - max), over the block.
-MOCAT_DEV void block_grad(GradAcc a, BlockScratch& s, double out[3]) {
-  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
-  for (int o = kWave / 2; o > 0; o >>= 1) {
-    GradAcc b;
-    b.m = __shfl_xor(a.m, o, kWave);
-    b.s1 = __shfl_xor(a.s1, o, kWave);
-    b.g0 = __shfl_xor(a.g0, o, kWave);
-    b.g1 = __shfl_xor(a.g1, o, kWave);
-    b.g2 = __shfl_xor(a.g2, o, kWave);
-    a = grad_combine(a, b);
-  }
-  __syncthreads();
-  if (lane == 0) {
-    s.fm[w] = a.m;
-    s.d1[w] = a.s1;
-    s.g0[w] = a.g0;
-    s.g1[w] = a.g1;
-    s.g2[w] = a.g2;
-  }
-  __syncthreads();
-  GradAcc r{s.fm[0], s.d1[0], s.g0[0], s.g1[0], s.g2[0]};
-  for (int i = 1; i < NW; ++i) r = grad_combine(r, GradAcc{s.fm[i], s.d1[i], s.g0[i], s.g1[i], s.g2[i]});
-  out[0] = r.g0 / r.s1;
-  out[1] = r.g1 / r.s1;
-  out[2] = r.g2 / r.s1;
-}
-
-MOCAT_DEV void grad_add(GradAcc& a, float v, float t0, float t1, float t2) {
-  if (!(v > -INFINITY)) return;
-  if (v > a.m) {
-    const double f = a.m == -INFINITY ? 0.0 : exp((double)a.m - (double)v);
-    a.s1 = a.s1 * f + 1.0;
-    a.g0 = a.g0 * f + t0;
-    a.g1 = a.g1 * f + t1;
-    a.g2 = a.g2 * f + t2;
-    a.m = v;
-  } else {
-    const double e = exp((double)v - (double)a.m);
-    a.s1 += e;
-    a.g0 += e * t0;
-    a.g1 += e * t1;
-    a.g2 += e * t2;
+  if constexpr (GRAD) {
+    gout[0] = q0 / r1;
+    gout[1] = q1 / r1;
+    gout[2] = q2 / r1;
   }
 }
 
@@ -564,11 +526,11 @@ __global__ __launch_bounds__(NT) void pf_kernel(PfArgs a) {
     constexpr uint32_t hD = (D + 1) / 2;
     float* ta = GRAD ? a.tan0 + F.state_offset * 5 : nullptr;
     float* tb = GRAD ? a.tan1 + F.state_offset * 5 : nullptr;
-    GradAcc gacc{-INFINITY, 0.0, 0.0, 0.0, 0.0};
     double gtot[3] = {0.0, 0.0, 0.0}, gprev[3] = {0.0, 0.0, 0.0};
 
     // ---- t = 0: initiate_particles
-    LseAcc acc{-INFINITY, 0.0, 0.0};
+    float mloc = -INFINITY;
+    int bad0 = 0;
     const Key sub0 = Model::kOptimal ? split_at(sub, (uint32_t)n, 0u) : Key{0u, 0u};
     for (int64_t i = tid; i < n; i += NT) {
       float eps[D], x[D];
@@ -600,7 +562,6 @@ __global__ __launch_bounds__(NT) void pf_kernel(PfArgs a) {
         ta[i * 5 + 2] = 0.f;
         ta[i * 5 + 3] = 0.f;
         ta[i * 5 + 4] = -dldb;
-        grad_add(gacc, l, 0.f, 0.f, -dldb);
       } else {
         l = -model.lik(x, yf);
       }
@@ -608,7 +569,10 @@ __global__ __launch_bounds__(NT) void pf_kernel(PfArgs a) {
       #pragma unroll
       for (int q = 0; q < D; ++q) xa[i * D + q] = x[q];
       lwa[i] = l;
-      lse_add(acc, l);
+      if (isnan(l))
+        bad0 = 1;
+      else
+        mloc = fmaxf(mloc, l);
       if (a.out_value && F.out_offset >= 0)
         #pragma unroll
         for (int q = 0; q < D; ++q) a.out_value[(F.out_offset + i) * D + q] = x[q];
@@ -616,11 +580,9 @@ __global__ __launch_bounds__(NT) void pf_kernel(PfArgs a) {
       if (a.out_anc && F.out_offset >= 0) a.out_anc[F.out_offset + i] = (int32_t)i;
     }
     double lse, lse2;
-    block_lse(acc, s, lse, lse2);
-    if constexpr (GRAD) {
-      block_grad(gacc, s, gprev);
+    block_lse2<GRAD>(lwa, ta, n, mloc, bad0, s, lse, lse2, gprev);
+    if constexpr (GRAD)
       for (int k = 0; k < 3; ++k) gtot[k] = gprev[k];
-    }
     const double logn = log((double)n);
     double loglik = resume ? 0.0 : lse - logn;
     double ess = exp(2.0 * lse - lse2);
@@ -646,8 +608,8 @@ __global__ __launch_bounds__(NT) void pf_kernel(PfArgs a) {
           resample_block(wfn, n, ik, a.resampling, csum, anc, s);
         }
       }
-      acc = LseAcc{-INFINITY, 0.0, 0.0};
-      gacc = GradAcc{-INFINITY, 0.0, 0.0, 0.0, 0.0};
+      mloc = -INFINITY;
+      bad0 = 0;
       const Key ik0 = Model::kOptimal ? split_at(ik, (uint32_t)n, 0u) : Key{0u, 0u};
       for (int64_t i = tid; i < n; i += NT) {
         const int64_t src = res ? (int64_t)anc[i] : i;
@@ -683,7 +645,7 @@ __global__ __launch_bounds__(NT) void pf_kernel(PfArgs a) {
           tb[i * 5 + 2] = d0;
           tb[i * 5 + 3] = d1;
           tb[i * 5 + 4] = d2;
-          grad_add(gacc, l, d0, d1, d2);
+
         } else {
           model.transition(xp, eps, x);
           l = (res ? 0.f : lwa[i]) - model.lik(x, ycur);
@@ -691,7 +653,10 @@ __global__ __launch_bounds__(NT) void pf_kernel(PfArgs a) {
         #pragma unroll
         for (int q = 0; q < D; ++q) xb[i * D + q] = x[q];
         lwb[i] = l;
-        lse_add(acc, l);
+        if (isnan(l))
+          bad0 = 1;
+        else
+          mloc = fmaxf(mloc, l);
         if (F.out_offset >= 0) {
           const int64_t o = F.out_offset + t * n + i;
           if (a.out_value)
@@ -701,11 +666,10 @@ __global__ __launch_bounds__(NT) void pf_kernel(PfArgs a) {
           if (a.out_anc) a.out_anc[o] = (int32_t)src;
         }
       }
-      block_lse(acc, s, lse, lse2);
+      double gnow[3];
+      block_lse2<GRAD>(lwb, tb, n, mloc, bad0, s, lse, lse2, gnow);
       loglik += lse - lse_res;
       if constexpr (GRAD) {
-        double gnow[3];
-        block_grad(gacc, s, gnow);
         for (int k = 0; k < 3; ++k) {
           gtot[k] += gnow[k] - (res ? 0.0 : gprev[k]);
           gprev[k] = gnow[k];
