@@ -4,15 +4,18 @@
 Workload: X 2000 x 50 ~ N(0,1), theta* ~ N(0, I), y ~ Bernoulli(sigmoid(X theta*)), prior N(0, 10 I);
 8192 independent chains of Underdamped(leapfrog_steps=10, friction=inf) = HMC with Robbins-Monro
 stepsize adaptation to 0.651 acceptance; FP32 state, bf16x3 MFMA contractions.
-A step is one HMC iteration of all chains (one launch of the fused iteration kernel).
+A step is one HMC iteration of all chains; the K timed steps run as one launch of the fused kernel.
 
-value = ESS/s: per chain, mocat's ess_autocorrelation of the potential trace over the second half
-of the timed iterations, summed over chains, divided by the wall time of all timed iterations.
+value = ESS/s of the whole 2000-iteration chain, whatever --steps is: per chain, mocat's
+ess_autocorrelation of the potential trace over iterations 1001-2000, summed over chains, divided by
+the wall time of all 2000 iterations (one launch, started from the prior).
 
-Also reported: roofline of the fused kernel (algorithmic FLOPs 4*n*d*L per chain-iteration over its
-measured average launch time, against the 2.5 PFLOP/s dense bf16 MFMA peak), a CPU baseline (the
-oracle's NumPy port of the same chains on the host cores, bounded sample) and the unfused FP32
-torch path on the same GPU.
+Also reported: the error of the split-precision contraction against the FP64 engine at this shape,
+the roofline of the fused kernel (algorithmic FLOPs 4*n*d*L per chain-iteration over its average
+time per iteration from HIP events on the launch stream, against the 2.5 PFLOP/s dense bf16 MFMA
+peak), the same chains as unfused FP32 torch ops on the same GPU, and a CPU baseline (the oracle's
+NumPy port of the chain on the host cores: 64 chains of the whole 2000 iterations, ESS/s measured on
+those CPU chains).
 """
 import argparse
 import json
@@ -24,7 +27,7 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-N_ROWS, DIM, CHAINS, LEAPFROG = 2000, 50, 8192, 10
+N_ROWS, DIM, CHAINS, LEAPFROG, ITERATIONS = 2000, 50, 8192, 10, 2000
 PEAK_TFLOPS = {"bf16x3": 2500.0, "fp32": 157.3}
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hmc_lr_bf16x3_c8192_n2000_d50_traffic.json")
 
@@ -32,15 +35,15 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hmc_lr_bf16x3_c8192_n2000_d50_tra
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2000)
-    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--chains", type=int, default=CHAINS)
     ap.add_argument("--precision", default="bf16x3", choices=["bf16x3", "fp32"])
     ap.add_argument("--cpu-baseline", dest="cpu", action="store_true", default=True)
     ap.add_argument("--no-cpu-baseline", dest="cpu", action="store_false")
     ap.add_argument("--cpu-chains", type=int, default=64)
-    ap.add_argument("--cpu-iters", type=int, default=120)
-    ap.add_argument("--plain-steps", type=int, default=30, help="iterations timed on the unfused FP32 path (0: skip)")
+    ap.add_argument("--no-plain", dest="plain", action="store_false", default=True,
+                    help="skip the unfused FP32 torch chain on the same GPU")
     ap.add_argument("--iters", type=int, default=500, help="iterations of the svgd4 workload")
     ap.add_argument("--mode", default="hmc", choices=["hmc", "copy", "pf1", "pf5", "smc3", "svgd4", "ffbsi", "nlg"],
                     help="hmc: the headline workload. copy: a plain 256 MiB device copy (calibration run for HBM "
@@ -73,8 +76,10 @@ def clone(c):
     return out
 
 
-def plain_fp32_ms_per_iter(X, y, state, extra, steps, prec=0.1):
-    """The same HMC iteration as unfused FP32 torch ops (GEMM, elementwise, GEMM per leapfrog step)."""
+def plain_fp32_run(X, y, state, extra, iters, prec=0.1):
+    """The same HMC chain (same start, keys and RM adaptation) as unfused FP32 torch ops on the same
+    GPU: GEMM, elementwise, GEMM per leapfrog step, one op at a time. Returns (wall seconds of the
+    `iters` iterations, potential trace [iters][C])."""
     import torch
     import mocat_amd.random as mr
     Xt = torch.as_tensor(X, device="cuda")
@@ -85,6 +90,7 @@ def plain_fp32_ms_per_iter(X, y, state, extra, steps, prec=0.1):
     h = state.stepsize.clone()
     keys = extra.random_key.clone()
     C, d = th.shape
+    trace = torch.empty((iters, C), dtype=torch.float32, device="cuda")
 
     def pot_grad(t):
         z = t @ Xt.T
@@ -114,37 +120,37 @@ def plain_fp32_ms_per_iter(X, y, state, extra, steps, prec=0.1):
         g = torch.where(acc[:, None], gg, g)
         U = torch.where(acc, U1, U)
         h = torch.exp(torch.log(h) + it ** (-2 / 3) * (a - 0.651))
+        trace[it - 1] = U
 
-    for it in range(1, 4):
-        one(it)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for it in range(4, 4 + steps):
+    for it in range(1, iters + 1):
         one(it)
     torch.cuda.synchronize()
-    return (time.perf_counter() - t0) / steps * 1e3
+    return time.perf_counter() - t0, trace
 
 
-def cpu_baseline(X, y, chains, iters, ess_per_chain_iter):
-    """Oracle NumPy port of the same HMC chains on the host cores; bounded sample."""
+def cpu_baseline(X, y, chains, iters):
+    """The oracle's NumPy port of the same HMC chains on the host cores (bounded sample: `chains`
+    chains of the full `iters`-iteration chain). ESS/s is measured on these CPU chains exactly as the
+    headline: ESS of the potential trace over the second half, over the wall time of all iterations."""
     import numpy as np
+    import torch
     from oracle import mcmc as omcmc, rng as orng
-    try:
-        from threadpoolctl import threadpool_info
-        threads = max([i.get("num_threads", 1) for i in threadpool_info()] or [1])
-    except Exception:
-        threads = len(os.sched_getaffinity(0))
+    import mocat_amd as m
     pg = omcmc.lr_pot_grad_fn(X, y, 0.1)
     st, keys = omcmc.startup(orng.PRNGKey(0), chains, DIM, lambda k: np.sqrt(10.0) * orng.normal(k, (DIM,)), pg)
+    trace = np.empty((iters, chains), np.float32)
     t0 = time.perf_counter()
     for it in range(1, iters + 1):
         st, keys, a, acc = omcmc.iterate(st, keys, it, pg, LEAPFROG, np.inf, "rm", 0.651)
+        trace[it - 1] = st["potential"]
     dt = time.perf_counter() - t0
-    rate = chains * iters / dt  # chain-iterations / s
-    return {"value": rate * ess_per_chain_iter, "unit": "ESS/s", "cores": int(threads), "kind": "port",
-            "sample": f"{chains} chains x {iters} iterations of the NumPy oracle ({dt:.1f} s); ESS/s = "
-                      f"chain-iterations/s ({rate:.1f}) x ESS per chain-iteration of the GPU run (same algorithm)",
-            "chain_iterations_per_s": rate}
+    ess = float(m.metrics.ess_autocorrelation(torch.tensor(trace[iters // 2:], device="cuda")).sum().item())
+    return {"value": ess / dt, "unit": "ESS/s", "cores": _host_threads(), "kind": "port",
+            "sample": f"{chains} chains x {iters} iterations of the NumPy oracle ({dt:.1f} s); ESS of the potential "
+                      f"over iterations {iters // 2 + 1}-{iters} of these CPU chains / wall time of all iterations",
+            "ess_total": ess, "chain_iterations_per_s": chains * iters / dt}
 
 
 def _timed(fn):
@@ -457,43 +463,63 @@ def main():
     sc = m.LogisticRegression(X, y, prior_variance=10.0, precision=a.precision)
     sampler = m.Underdamped(leapfrog_steps=LEAPFROG)
     extra = m.cdict(random_key=m.random.PRNGKey(0))
-    state, extra = sampler.startup(sc, a.steps + 1, None, extra, n_chains=a.chains, correction=m.RMMetropolis())
+    state, extra = sampler.startup(sc, ITERATIONS, None, extra, n_chains=a.chains, correction=m.RMMetropolis())
+    evs = lambda: (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
 
+    # untimed warm-up iterations (first launches) on a copy of the start state
     if a.warmup > 0:
         ws, we = clone(state), clone(extra)
-        for _ in range(a.warmup):
-            sampler.iterate(sc, ws, we, 1)
+        sampler.iterate(sc, ws, we, a.warmup)
         torch.cuda.synchronize()
         del ws, we
 
+    # the timed steps: K iterations of all chains from the start state, one launch of the fused kernel
     K = a.steps
-    trace_pot = torch.empty((K, a.chains), dtype=torch.float32, device="cuda")
-    trace_alpha = torch.empty((K, a.chains), dtype=torch.float32, device="cuda")
-    ev0 = torch.cuda.Event(enable_timing=True)
-    ev1 = torch.cuda.Event(enable_timing=True)
+    ks, ke = clone(state), clone(extra)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sampler.iterate(sc, ks, ke, K)
+    torch.cuda.synchronize()
+    wall_steps = time.perf_counter() - t0
+    del ks, ke
+
+    # the headline: config 2's whole 2000-iteration chain from the start state, one launch, potential
+    # trace on the device; ESS over iterations 1001-2000 / wall time of all 2000 iterations
+    hs, he = clone(state), clone(extra)
+    trace_pot = torch.empty((ITERATIONS, a.chains), dtype=torch.float32, device="cuda")
+    trace_alpha = torch.empty((ITERATIONS, a.chains), dtype=torch.float32, device="cuda")
+    ev0, ev1 = evs()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     ev0.record()
-    for i in range(K):
-        sampler.iterate(sc, state, extra, 1, {"potential": trace_pot[i:i + 1], "alpha": trace_alpha[i:i + 1]})
+    sampler.iterate(sc, hs, he, ITERATIONS, {"potential": trace_pot, "alpha": trace_alpha})
     ev1.record()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    kernel_ms = ev0.elapsed_time(ev1) / K
+    kernel_ms = ev0.elapsed_time(ev1) / ITERATIONS  # HIP events on the launch stream
 
-    half = trace_pot[K // 2:]
-    ess = m.metrics.ess_autocorrelation(half)
-    ess_total = float(ess.sum().item())
+    half = trace_pot[ITERATIONS // 2:]
+    ess_total = float(m.metrics.ess_autocorrelation(half).sum().item())
     value = ess_total / wall
-    acc = float(trace_alpha[K // 2:].mean().item())
+    acc = float(trace_alpha[ITERATIONS // 2:].mean().item())
 
-    flop_per_launch = 4.0 * N_ROWS * DIM * LEAPFROG * a.chains
+    # stated error of the split-precision contraction at this shape: the final states of the chain
+    # evaluated by the bf16x3 engine and by the FP64 engine
+    ref = m.LogisticRegression(X, y, prior_variance=10.0, precision="fp64")
+    _, lp, _, gl = sc.evaluate(hs.value)
+    _, lp64, _, gl64 = ref.evaluate(hs.value)
+    gscale = gl64.abs().amax(1).clamp_min(1e-30)
+    err = {"reference": "FP64 engine (same final states of the 2000-iteration chain)",
+           "lik_potential_rel_max": float(((lp - lp64).abs() / lp64.abs().clamp_min(1e-30)).max().item()),
+           "grad_lik_rel_max": float(((gl - gl64).abs().amax(1) / gscale).max().item())}
+
+    flop_per_launch = 4.0 * N_ROWS * DIM * LEAPFROG * a.chains  # per iteration of all chains
     achieved = flop_per_launch / (kernel_ms * 1e-3) / 1e12
     peak = PEAK_TFLOPS[a.precision]
     traffic = None
     if os.path.exists(TRAFFIC_FILE):
         with open(TRAFFIC_FILE) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+            traffic = json.load(f).get("hbm_bytes_per_iteration")
 
     out = {
         "metric": "ESS/s (batched HMC, Bayesian logistic regression)",
@@ -502,7 +528,7 @@ def main():
         "n_gpus": 1,
         "steps": K,
         "warmup": a.warmup,
-        "ms_per_step": wall / K * 1e3,
+        "ms_per_step": wall_steps / K * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -511,25 +537,31 @@ def main():
         "config": {"workload": "batched HMC on Bayesian logistic regression", "n": N_ROWS, "d": DIM,
                    "chains": a.chains, "leapfrog_steps": LEAPFROG, "friction": "inf",
                    "correction": "RMMetropolis(target=0.651)", "precision": a.precision,
-                   "ess_window": f"iterations {K // 2 + 1}-{K}"},
+                   "iterations": ITERATIONS, "ess_window": f"iterations {ITERATIONS // 2 + 1}-{ITERATIONS}"},
         "ess_total": ess_total,
         "acceptance": acc,
-        "chain_iterations_per_s": a.chains * K / wall,
+        "chain_wall_s": wall,
+        "chain_iterations_per_s": a.chains * ITERATIONS / wall,
+        "error_vs_fp64": err,
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak, "traffic": traffic,
                      "kernel": "hmc_lr_kernel<Bf16Engine<64,512>>" if a.precision == "bf16x3"
                      else "hmc_lr_kernel<Fp32Engine<64,512>>",
-                     "kernel_ms": kernel_ms, "flop_per_launch": flop_per_launch,
-                     "issued_mfma_flop_per_launch": (3 * 4.0 * 64 * (math.ceil(N_ROWS / 32) * 32) * LEAPFROG
-                                                     * a.chains) if a.precision == "bf16x3" else None},
+                     "kernel_ms_per_iteration": kernel_ms, "flop_per_iteration": flop_per_launch,
+                     "iterations_per_launch": ITERATIONS,
+                     "issued_mfma_flop_per_iteration": (3 * 4.0 * 64 * (math.ceil(N_ROWS / 32) * 32) * LEAPFROG
+                                                        * a.chains) if a.precision == "bf16x3" else None},
     }
-    if a.plain_steps > 0:
-        ms_plain = plain_fp32_ms_per_iter(X, y, state, extra, a.plain_steps)
-        out["plain_fp32_gpu"] = {"ms_per_step": ms_plain, "speedup": ms_plain / (wall / K * 1e3),
-                                 "note": "unfused FP32 torch GEMM+elementwise HMC iteration, same GPU"}
+    if a.plain:
+        pw, ptrace = plain_fp32_run(X, y, state, extra, ITERATIONS)
+        pess = float(m.metrics.ess_autocorrelation(ptrace[ITERATIONS // 2:]).sum().item())
+        out["plain_fp32_gpu"] = {"value": pess / pw, "unit": "ESS/s", "ms_per_iteration": pw / ITERATIONS * 1e3,
+                                 "speedup_ess_per_s": value / (pess / pw),
+                                 "speedup_per_iteration": (pw / ITERATIONS) / (wall / ITERATIONS),
+                                 "note": "the same chains as unfused FP32 torch GEMM + elementwise ops, same GPU, "
+                                         f"{ITERATIONS} iterations"}
     if a.cpu:
-        ess_per_ci = ess_total / (a.chains * K)  # same ESS-window-over-total-time convention as value
-        out["cpu_baseline"] = cpu_baseline(X, y, a.cpu_chains, a.cpu_iters, ess_per_ci)
+        out["cpu_baseline"] = cpu_baseline(X, y, a.cpu_chains, ITERATIONS)
     print(json.dumps(out))
 
 

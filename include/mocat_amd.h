@@ -22,7 +22,7 @@ extern "C" {
 #endif
 
 /* Returns the ABI revision of the library (not a status). */
-int mocat_abi_version(void);
+int mocat_abi_version(void); /* this header: 2 */
 
 /* ---------------------------------------------------------------------------------------- RNG
  * jax.random.split / _random_bits / uniform / normal as called throughout mocat, e.g.
@@ -55,7 +55,9 @@ int mocat_logsumexp(const float* w, int64_t batch, int64_t n, double* lse, doubl
  * README.md:35-49; gradients replace jax.grad in mocat/src/core.py:204-230):
  *   prior = 0.5*prec*|t|^2 + 0.5*d*(log 2pi - log prec), lik = sum softplus(x.t) - y*x.t,
  *   grad_lik = X^T (sigmoid(X t) - y). X is [n][d] f32 row-major, y [n] f32, d <= 128.
- * precision: 0 = FP32 vector FMAs (reads X, y), 1 = bf16x3 MFMA (reads the packed buffer). */
+ * precision: 0 = FP32 vector FMAs (reads X, y), 1 = bf16x3 MFMA (reads the packed buffer),
+ * 2 = diagonal Gaussian likelihood (X = mean[d], y = precisions[d]), 3 = FP64 vector FMAs with f32
+ * inputs and outputs (reads X, y; the reference for the error of 0 and 1). */
 
 /* Bytes needed for the packed (bf16 hi/lo, MFMA operand order) copy of X and y. */
 int mocat_lr_packed_bytes(int64_t n, int64_t d, int64_t* bytes);
@@ -131,7 +133,12 @@ typedef struct {
 #define MOCAT_PF_INIT_KEY 8
 
 int mocat_pf_param_count(int model, int d, int dy, int64_t* count);
-int mocat_pf_run(int model, int d, int dy, const mocat_pf_filter* filters, int n_filters, const float* params,
+/* host_filters and filters hold the same n_filters descriptors: host_filters in host memory (read by
+ * the argument checks, not retained after return), filters in device memory (read by the kernel).
+ * Every filter not flagged MOCAT_PF_LARGE needs n >= 1, T >= 1, non-negative offsets and, when
+ * out_ess or out_loglik_t is given, t_offset >= 0. */
+int mocat_pf_run(int model, int d, int dy, const mocat_pf_filter* host_filters, const mocat_pf_filter* filters,
+                 int n_filters, const float* params,
                  const float* y, float* x_ws0, float* x_ws1, float* lw_ws0, float* lw_ws1, uint64_t* csum_ws,
                  int32_t* anc_ws, int resampling, float ess_threshold, float* out_value, float* out_lw,
                  int32_t* out_anc, float* out_ess, double* out_loglik_t, double* out_loglik, float* tan_ws0,
@@ -143,7 +150,8 @@ int mocat_pf_run(int model, int d, int dy, const mocat_pf_filter* filters, int n
  * step's statistics, resampling decision and quantised weight totals; exact prefix sums; ancestor
  * search, propagation and weighting). For the same log-weights and key the ancestors equal mocat_pf_run's. host_filters is a
  * host array (the launch sequence depends on n and T); workspace holds
- * mocat_pf_large_workspace_bytes bytes. Calls on one stream may follow mocat_pf_run with the same
+ * mocat_pf_large_workspace_bytes bytes, including T step-key slots per flagged filter (independent of
+ * t_offset, which only places out_ess / out_loglik_t and must be >= 0 when either is given). Calls on one stream may follow mocat_pf_run with the same
  * arguments: the two paths touch disjoint filters. */
 int mocat_pf_large_workspace_bytes(const mocat_pf_filter* host_filters, int n_filters, int64_t* bytes);
 int mocat_pf_run_large(int model, int d, int dy, const mocat_pf_filter* host_filters, int n_filters,

@@ -7,8 +7,9 @@
 //   grad_likelihood             = X^T (sigmoid(X theta) - y)
 // Chains/particles are processed in tiles of 32: the contraction over the data rows is the hot loop.
 //
-// Two engines evaluate (U, grad U) for a tile:
+// Three engines evaluate (U, grad U) for a tile:
 //   Fp32Engine - data rows staged through LDS in chunks, FP32 FMAs on the vector ALU.
+//   Fp64Engine - the same in FP64 (the reference for the error of the other two).
 //   Bf16Engine - MFMA v_mfma_f32_32x32x16_bf16 with split precision: every operand is
 //                hi + lo bf16 and each product is hi*hi + hi*lo + lo*hi in FP32 accumulation.
 //                Z = X Theta^T is accumulated with rows of X on the M side and the 32 chains on
@@ -154,6 +155,104 @@ struct Fp32Engine {
       const float cs = wave_sum(cacc[m]);
       if (lane == 0) lik[c] = cs;
     }
+    __syncthreads();
+  }
+};
+
+// ------------------------------------------------------------------------------------------
+// FP64 engine: the same contraction with every product, sum and row function in double (the
+// reference run that error statements of the split-precision engine are made against). The tile
+// state stays f32: theta is widened exactly, the outputs are rounded once. LDS: X chunk
+// [RC][DP+1] and R chunk [CT][RC+1] in double.
+template <int DP, int NT>
+struct Fp64Engine {
+  static constexpr int RC = 32;
+  static constexpr int XS = DP + 1;
+  static constexpr int RS = RC + 1;
+  struct alignas(16) Smem {
+    double x[RC * XS];
+    double r[CT * RS];
+    double y[RC];
+  };
+
+  static MOCAT_DEV void eval(Smem& s, const float* __restrict__ X, const float* __restrict__ y, const void*,
+                             int n, int d, const float* theta, float* lik, float* glik, bool /*want_pot*/) {
+    const int t = threadIdx.x;
+    const int lane = t & (kWave - 1), w = t / kWave;
+    constexpr int NW = NT / kWave;
+    constexpr int G1 = NW * 2;          // phase 1: chain groups (row = lane % 32)
+    constexpr int P1 = CT / G1;         // chains per thread in phase 1
+    constexpr int P2 = CT / NW;         // chains per thread in phase 2 (column = lane (+64))
+    constexpr int KPT = DP / kWave;
+    static_assert(CT % G1 == 0 && CT % NW == 0, "tile shape");
+    const int row = lane & 31, grp = w * 2 + (lane >> 5);
+    double gacc[P2][KPT];
+    double cacc[P1];
+#pragma unroll
+    for (int m = 0; m < P1; ++m) cacc[m] = 0.0;
+#pragma unroll
+    for (int m = 0; m < P2; ++m)
+#pragma unroll
+      for (int q = 0; q < KPT; ++q) gacc[m][q] = 0.0;
+    for (int i0 = 0; i0 < n; i0 += RC) {
+      __syncthreads();
+      for (int e = t; e < RC * DP; e += NT) {
+        const int i = e / DP, k = e - i * DP;
+        const int gi = i0 + i;
+        s.x[i * XS + k] = (gi < n && k < d) ? (double)X[(int64_t)gi * d + k] : 0.0;
+      }
+      for (int i = t; i < RC; i += NT) s.y[i] = (i0 + i < n) ? (double)y[i0 + i] : -1.0;
+      __syncthreads();
+      {
+        double z[P1];
+#pragma unroll
+        for (int m = 0; m < P1; ++m) z[m] = 0.0;
+        const double* xr = &s.x[row * XS];
+        for (int k = 0; k < d; ++k) {
+          const double xv = xr[k];
+#pragma unroll
+          for (int m = 0; m < P1; ++m) z[m] = fma(xv, (double)theta[(grp + G1 * m) * DP + k], z[m]);
+        }
+        const double yv = s.y[row];
+#pragma unroll
+        for (int m = 0; m < P1; ++m) {
+          double r = 0.0, c = 0.0;
+          if (yv >= 0.0) {
+            const double e = exp(-fabs(z[m]));
+            const double sig = z[m] >= 0.0 ? 1.0 / (1.0 + e) : e / (1.0 + e);
+            r = sig - yv;
+            c = fmax(z[m], 0.0) + log1p(e) - yv * z[m];
+          }
+          cacc[m] += c;
+          s.r[(grp + G1 * m) * RS + row] = r;
+        }
+      }
+      __syncthreads();
+      for (int i = 0; i < RC; ++i) {
+        double xv[KPT];
+#pragma unroll
+        for (int q = 0; q < KPT; ++q) xv[q] = s.x[i * XS + lane + kWave * q];
+#pragma unroll
+        for (int m = 0; m < P2; ++m) {
+          const double rv = s.r[(w + NW * m) * RS + i];
+#pragma unroll
+          for (int q = 0; q < KPT; ++q) gacc[m][q] = fma(rv, xv[q], gacc[m][q]);
+        }
+      }
+    }
+    __syncthreads();
+    // potentials: the two half-waves hold different chains; sum each chain's rows over its 32 lanes
+#pragma unroll
+    for (int m = 0; m < P1; ++m) {
+      double v = cacc[m];
+#pragma unroll
+      for (int off = 16; off > 0; off >>= 1) v += __shfl_xor(v, off, 32);
+      if (row == 0) lik[grp + G1 * m] = (float)v;
+    }
+#pragma unroll
+    for (int m = 0; m < P2; ++m)
+#pragma unroll
+      for (int q = 0; q < KPT; ++q) glik[(w + NW * m) * DP + lane + kWave * q] = (float)gacc[m][q];
     __syncthreads();
   }
 };
@@ -730,7 +829,7 @@ int dispatch_hmc(const HmcArgs& h, hipStream_t st) {
 
 int check_lr(const float* X, const float* y, const void* packed, int64_t n, int64_t d, int precision) {
   if (n <= 0 || d <= 0 || d > 128 || n > (1ll << 30)) return MOCAT_ERR_ARG;
-  if (precision == 0 || precision == 2) return (X && y) ? MOCAT_OK : MOCAT_ERR_ARG;
+  if (precision == 0 || precision == 2 || precision == 3) return (X && y) ? MOCAT_OK : MOCAT_ERR_ARG;
   if (precision == 1) return packed ? MOCAT_OK : MOCAT_ERR_ARG;
   return MOCAT_ERR_ARG;
 }
@@ -775,6 +874,8 @@ int mocat_lr_potential_grad(const float* X, const float* y, const void* packed, 
   LrArgs a{X, y, packed, (int)n, (int)d, prior_prec, temperature};
   if (precision == 2)
     return dispatch_eval<GaussEngine>(a, theta, C, prior_pot, lik_pot, grad_prior, grad_lik, stream);
+  if (precision == 3)
+    return dispatch_eval<Fp64Engine>(a, theta, C, prior_pot, lik_pot, grad_prior, grad_lik, stream);
   return precision == 0 ? dispatch_eval<Fp32Engine>(a, theta, C, prior_pot, lik_pot, grad_prior, grad_lik, stream)
                         : dispatch_eval<Bf16Engine>(a, theta, C, prior_pot, lik_pot, grad_prior, grad_lik, stream);
 }
@@ -797,6 +898,7 @@ int mocat_hmc_lr_iterate(const float* X, const float* y, const void* packed, int
             leapfrog_steps, friction, correction, target, rm_scale, rm_neg_exponent, adapt_cut_off,
             trace_value, trace_potential, trace_alpha, trace_stepsize, trace_momenta};
   if (precision == 2) return dispatch_hmc<GaussEngine>(h, stream);
+  if (precision == 3) return dispatch_hmc<Fp64Engine>(h, stream);
   return precision == 0 ? dispatch_hmc<Fp32Engine>(h, stream) : dispatch_hmc<Bf16Engine>(h, stream);
 }
 

@@ -796,6 +796,7 @@ constexpr int kChunkEnds = 4096;   // SG-block end sums staged in LDS (filters u
 
 struct LfDesc {
   int64_t n, T, y_offset, state_offset, out_offset, t_offset, chunk0;
+  int64_t key_slot;  // this filter's T step-key slots in LfArgs::skeys (disjoint per filter)
   uint32_t key[2];
   int32_t param_offset, f, flags;
 };
@@ -850,7 +851,7 @@ struct LfArgs {
   LfState* st;       // [n_large]
   uint64_t* cends;   // [n_chunks * CH / SG] prefix sums at the SG-block ends, per filter from chunk0 * CH / SG
   int64_t* bounds;   // [n_chunks] systematic: ancestor of each chunk's first target
-  StepKey* skeys;    // [t_offset + t] per filter
+  StepKey* skeys;    // [key_slot + t] per filter
 };
 
 struct LfScratch {
@@ -1052,7 +1053,7 @@ __global__ __launch_bounds__(LT) void lf_init_kernel(LfArgs a) {
       }
       sk.su = uniform_from_bits(bits_at(ikk, 1, 0), 0.f, 1.f);
       sk.pad = 0u;
-      a.skeys[F.t_offset + t] = sk;
+      a.skeys[F.key_slot + t] = sk;
     }
   }
   float lv[PPT], lt[PPT][3];
@@ -1292,7 +1293,7 @@ __global__ __launch_bounds__(LT) void lf_csum_kernel(LfArgs a, int64_t t) {
   }
   off = lf_block_sum(off, s);
   wsum = lf_block_sum(wsum, s);
-  const float su = a.skeys[F.t_offset + t].su;  // the step's systematic uniform (init launch)
+  const float su = a.skeys[F.key_slot + t].su;  // the step's systematic uniform (init launch)
   // thread tid owns the PPT consecutive particles from local * CH + tid * PPT
   const int64_t i0 = local * CH + (int64_t)threadIdx.x * PPT;
   const float* lw = lf_lw_prev(a, t) + F.state_offset;
@@ -1446,7 +1447,7 @@ __global__ __launch_bounds__(LT) void lf_step_kernel(LfArgs a, int64_t t, int ce
   const uint64_t* CE = a.cends + F.chunk0 * (CH / SG);
   if (two_level) stage_lds<8>(cend, nsg, [&](int64_t k) { return CE[k]; });
   // the step key, the optimal proposal's first split key and the systematic uniform (init launch)
-  const StepKey skv = a.skeys[F.t_offset + t];
+  const StepKey skv = a.skeys[F.key_slot + t];
   const uint32_t sk[4] = {skv.k[0], skv.k[1], skv.k[2], skv.k[3]};
   const float su = skv.su;
   load_shared_params(model, a.params + F.param_offset);
@@ -1683,7 +1684,7 @@ struct LfLayout {
   int n_large = 0;
   int cend_slots = 1;  // SG-block end sums the step kernel stages: the largest filter's, up to kChunkEnds
   int64_t n_chunks = 0, max_T = 0;
-  int64_t n_tk = 0;  // step-key slots: the largest t_offset + T of the flagged filters
+  int64_t n_tk = 0;  // step-key slots: the sum of T over the flagged filters
   int64_t o_desc, o_cf, o_m, o_s1, o_s2, o_g, o_tot, o_st, o_cend, o_bnd, o_keys, bytes;
 };
 
@@ -1696,7 +1697,7 @@ LfLayout lf_layout(const mocat_pf_filter* filters, int n_filters) {
     ++L.n_large;
     L.n_chunks += (filters[f].n + CH - 1) / CH;
     L.max_T = std::max<int64_t>(L.max_T, filters[f].T);
-    L.n_tk = std::max<int64_t>(L.n_tk, filters[f].t_offset + filters[f].T);
+    L.n_tk += std::max<int64_t>(filters[f].T, 0);
     L.cend_slots = (int)std::min<int64_t>(kChunkEnds, std::max<int64_t>(L.cend_slots, (filters[f].n + SG - 1) / SG));
   }
   int64_t o = 0;
@@ -1746,19 +1747,29 @@ int mocat_pf_param_count(int model, int d, int dy, int64_t* count) {
   return MOCAT_OK;
 }
 
-int mocat_pf_run(int model, int d, int dy, const mocat_pf_filter* filters, int n_filters, const float* params,
+int mocat_pf_run(int model, int d, int dy, const mocat_pf_filter* host_filters, const mocat_pf_filter* filters,
+                 int n_filters, const float* params,
                  const float* y, float* x_ws0, float* x_ws1, float* lw_ws0, float* lw_ws1, uint64_t* csum_ws,
                  int32_t* anc_ws, int resampling, float ess_threshold, float* out_value, float* out_lw,
                  int32_t* out_anc, float* out_ess, double* out_loglik_t, double* out_loglik, float* tan_ws0,
                  float* tan_ws1, double* out_grad, hipStream_t stream) {
-  if (n_filters < 0 || !filters || !params || !y || !x_ws0 || !x_ws1 || !lw_ws0 || !lw_ws1 || !csum_ws || !anc_ws ||
-      !out_loglik || resampling < 0 || resampling > 2)
+  if (n_filters < 0 || !host_filters || !filters || !params || !y || !x_ws0 || !x_ws1 || !lw_ws0 || !lw_ws1 ||
+      !csum_ws || !anc_ws || !out_loglik || resampling < 0 || resampling > 2)
     return MOCAT_ERR_ARG;
   if (out_grad && (model != 1 || !tan_ws0 || !tan_ws1)) return MOCAT_ERR_ARG;
   if (n_filters == 0) return MOCAT_OK;
-  if (out_grad)
-    for (int f = 0; f < n_filters; ++f)
-      if (filters[f].flags & kPfResume) return MOCAT_ERR_ARG;
+  // argument checks read the host copy only; the kernel reads the device copy
+  for (int f = 0; f < n_filters; ++f) {
+    const mocat_pf_filter& F = host_filters[f];
+    if (F.flags & kPfLarge) continue;
+    if (F.n <= 0 || F.T <= 0 || F.n > 0x7FFFFFFFll || F.T > 0xFFFFFFFFll || F.state_offset < 0 ||
+        F.y_offset < 0 || F.param_offset < 0)
+      return MOCAT_ERR_ARG;
+    if ((out_ess || out_loglik_t) && F.t_offset < 0) return MOCAT_ERR_ARG;
+    if (out_grad && (F.flags & kPfResume)) return MOCAT_ERR_ARG;
+    if (resampling == 2 && F.n * F.n >= 0xFFFFFFFFll) return MOCAT_ERR_SIZE;
+    if (model == 2 && F.n * (int64_t)d > 0xFFFFFFFFll) return MOCAT_ERR_SIZE;
+  }
   PfArgs a{filters, n_filters, params, y, x_ws0, x_ws1, lw_ws0, lw_ws1, csum_ws, anc_ws, resampling,
            ess_threshold, out_value, out_lw, out_anc, out_ess, out_loglik_t, out_loglik, tan_ws0, tan_ws1, out_grad};
   if (model == 1) {
@@ -1766,8 +1777,6 @@ int mocat_pf_run(int model, int d, int dy, const mocat_pf_filter* filters, int n
     return out_grad ? launch_pf<Sv, 1, 1, true>(a, stream) : launch_pf<Sv, 1, 1>(a, stream);
   }
   if (model == 2) {
-    for (int f = 0; f < n_filters; ++f)
-      if (filters[f].n * (int64_t)d > 0xFFFFFFFFll) return MOCAT_ERR_SIZE;
 #define MOCAT_PF_CASE(DD, YY) \
   if (d == DD && dy == YY) return launch_pf<LgssmOpt<DD, YY>, DD, YY>(a, stream);
     MOCAT_PF_LG_SHAPES
@@ -1805,17 +1814,21 @@ int mocat_pf_run_large(int model, int d, int dy, const mocat_pf_filter* host_fil
   std::vector<int32_t> chunk_filter;
   desc.reserve(L.n_large);
   chunk_filter.reserve(L.n_chunks);
-  int64_t chunk0 = 0;
+  int64_t chunk0 = 0, key_slot = 0;
   for (int f = 0; f < n_filters; ++f) {
     const mocat_pf_filter& F = host_filters[f];
     if (!(F.flags & kPfLarge)) continue;
-    if (F.n <= 0 || F.T <= 0 || F.n > 0x7FFFFFFFll || F.T > 0xFFFFFFFFll) return MOCAT_ERR_ARG;
+    if (F.n <= 0 || F.T <= 0 || F.n > 0x7FFFFFFFll || F.T > 0xFFFFFFFFll || F.state_offset < 0 ||
+        F.y_offset < 0 || F.param_offset < 0)
+      return MOCAT_ERR_ARG;
+    if ((out_ess || out_loglik_t) && F.t_offset < 0) return MOCAT_ERR_ARG;
     if (resampling == 2 && F.n * F.n >= 0xFFFFFFFFll) return MOCAT_ERR_SIZE;
     if (model == 2 && F.n * (int64_t)d > 0xFFFFFFFFll) return MOCAT_ERR_SIZE;
     if (out_grad && (F.flags & kPfResume)) return MOCAT_ERR_ARG;
     const int64_t nchunk = (F.n + CH - 1) / CH;
-    desc.push_back(LfDesc{F.n, F.T, F.y_offset, F.state_offset, F.out_offset, F.t_offset, chunk0, {F.key[0], F.key[1]},
-                          F.param_offset, f, F.flags});
+    desc.push_back(LfDesc{F.n, F.T, F.y_offset, F.state_offset, F.out_offset, F.t_offset, chunk0, key_slot,
+                          {F.key[0], F.key[1]}, F.param_offset, f, F.flags});
+    key_slot += F.T;
     for (int64_t k = 0; k < nchunk; ++k) chunk_filter.push_back((int32_t)(desc.size() - 1));
     chunk0 += nchunk;
   }

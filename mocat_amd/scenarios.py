@@ -16,7 +16,7 @@ from . import _lib
 from . import random as mr
 from .core import Scenario
 
-PRECISIONS = {"fp32": 0, "bf16x3": 1}
+PRECISIONS = {"fp32": 0, "bf16x3": 1, "fp64": 3}
 ENGINE_GAUSSIAN = 2
 
 _lib.register("mocat_gmm_potential_grad", [_lib.c_ptr, _lib.c_i64, _lib.c_i64, _lib.c_i32, _lib.c_ptr, _lib.c_ptr,
@@ -41,7 +41,8 @@ def _dev_tensor(a, dtype=torch.float32):
 class LogisticRegression(Scenario):
     """Bayesian logistic regression: y_i ~ Bernoulli(sigmoid(x_i . theta)), theta ~ N(0, v I).
 
-    precision: "bf16x3" (MFMA, hi/lo split, FP32 accumulation) or "fp32" (vector FMAs).
+    precision: "bf16x3" (MFMA, hi/lo split, FP32 accumulation), "fp32" (vector FMAs) or "fp64" (vector
+    FMAs in double with f32 inputs and outputs: the reference the other two are measured against).
     Supports d <= 128.
     """
     name = "Logistic Regression"

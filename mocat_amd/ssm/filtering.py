@@ -25,7 +25,7 @@ RESAMPLING = {"systematic": 0, "multinomial": 1, "categorical": 2}
 
 _P = _lib.c_ptr
 _lib.register("mocat_pf_param_count", [_lib.c_i32, _lib.c_i32, _lib.c_i32, _P])
-_lib.register("mocat_pf_run", [_lib.c_i32, _lib.c_i32, _lib.c_i32, _P, _lib.c_i32, _P, _P, _P, _P, _P, _P, _P, _P,
+_lib.register("mocat_pf_run", [_lib.c_i32, _lib.c_i32, _lib.c_i32, _P, _P, _lib.c_i32, _P, _P, _P, _P, _P, _P, _P, _P,
                                _lib.c_i32, _lib.c_f32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P])
 _lib.register("mocat_pf_large_workspace_bytes", [_P, _lib.c_i32, _P])
 _lib.register("mocat_pf_run_large", [_lib.c_i32, _lib.c_i32, _lib.c_i32, _P, _lib.c_i32, _P, _P, _P, _P, _P, _P, _P,
@@ -272,7 +272,8 @@ def _launch(models, ys, keys, ns, ess_threshold, resampling, store, grad=False, 
         toff += Ts[i]
     dev = mr.default_device()
     f32 = dict(dtype=torch.float32, device=dev)
-    desc_t = torch.from_numpy(desc.view(np.uint8).copy()).to(dev)
+    desc_host = np.ascontiguousarray(desc)
+    desc_t = torch.from_numpy(desc_host.view(np.uint8).copy()).to(dev)
     params_t = torch.from_numpy(np.concatenate(params)).to(dev)
     y_t = torch.from_numpy(np.concatenate(yflat).astype(np.float32)).to(dev)
     x0 = torch.empty(soff * d, **f32)
@@ -311,13 +312,13 @@ def _launch(models, ys, keys, ns, ess_threshold, resampling, store, grad=False, 
         side = torch.cuda.Stream(device=dev) if any(large) else cur
         side.wait_stream(cur)
         with torch.cuda.stream(side):
-            _lib.call("mocat_pf_run", kind, d, dy, _lib.ptr(desc_t), F, _lib.ptr(params_t), _lib.ptr(y_t),
+            _lib.call("mocat_pf_run", kind, d, dy, desc_host.ctypes.data_as(_lib.ctypes.c_void_p), _lib.ptr(desc_t),
+                      F, _lib.ptr(params_t), _lib.ptr(y_t),
                       _lib.ptr(x0), _lib.ptr(x1), _lib.ptr(lw0), _lib.ptr(lw1), _lib.ptr(cs), _lib.ptr(anc),
                       RESAMPLING[resampling], float(ess_threshold), _lib.ptr(out_value), _lib.ptr(out_lw),
                       _lib.ptr(out_anc), _lib.ptr(out_ess), _lib.ptr(out_llt), _lib.ptr(out_ll), _lib.ptr(tan0),
                       _lib.ptr(tan1), _lib.ptr(out_grad), _lib.stream_handle())
     if any(large):
-        desc_host = np.ascontiguousarray(desc)
         hp = desc_host.ctypes.data_as(_lib.ctypes.c_void_p)
         nb = _lib.ctypes.c_int64(0)
         _lib.call("mocat_pf_large_workspace_bytes", hp, F, _lib.ctypes.byref(nb))
