@@ -22,6 +22,8 @@
 // (mocat/src/mcmc/standard_mcmc.py:107-153, mocat/src/utils.py:108-146,
 //  mocat/src/mcmc/metropolis.py:48-161), one chain per (tile, slot), key tree per chain as mocat's
 // single-chain run (mocat/src/mcmc/sampler.py:92-111).
+#include <type_traits>
+
 #include "common.hpp"
 
 using namespace mocat;
@@ -50,18 +52,36 @@ MOCAT_DEV void lr_row(float z, float y, float& r, float& c) {
   c = valid ? fmaxf(z, 0.f) + __logf(ope) - y * z : 0.f;
 }
 
-// MFMA-engine rows, one form for both signs of z: t = exp(-z), sigmoid = 1 / (1 + t) (t = inf gives
-// 0), softplus = max(z + log(1 + t), 0) with t clamped below overflow. Padding rows hold x = 0,
-// y = 0.5, so r = 0 there; their potential terms are masked by the caller.
-MOCAT_DEV float lr_row_grad(float z, float y) {
-  const float t = __expf(-z);
+// MFMA-engine rows on the hardware transcendentals only (v_exp_f32 = 2^x, v_log_f32 = log2,
+// v_rcp_f32): t = 2^(-z log2 e) = exp(-z), r = 1 / (1 + t) - y; softplus(z) = max(z + ln2 log2(1 + t), 0)
+// with the exponent clamped below overflow (1 + t >= 1, so no denormal inputs reach v_log_f32).
+// Padding rows hold x = 0 and y = 0.5, so r = 0 there.
+constexpr float kLog2E = 1.44269504088896340736f;
+constexpr float kLn2 = 0.69314718055994530942f;
+// Both take zs = -z log2(e), the engine's accumulator (X is packed premultiplied by -log2(e)).
+MOCAT_DEV float lr_row_grad_hw(float zs, float y) {
+  const float t = __builtin_amdgcn_exp2f(zs);
   return __builtin_amdgcn_rcpf(1.f + t) - y;
 }
-MOCAT_DEV float lr_row_full(float z, float y, float& c) {
-  const float t = __expf(fminf(-z, 87.f));
+MOCAT_DEV float lr_row_full_hw(float zs, float y, float& c) {
+  const float t = __builtin_amdgcn_exp2f(fminf(zs, 126.f));
   const float ope = 1.f + t;
-  c = fmaxf(z + __logf(ope), 0.f) - y * z;
+  const float z = zs * -kLn2;
+  c = fmaxf(z + kLn2 * __builtin_amdgcn_logf(ope), 0.f) - y * z;
   return __builtin_amdgcn_rcpf(ope) - y;
+}
+
+// Split two floats into packed bf16 pairs, v = hi + lo: hi keeps the upper 16 bits of v (truncation,
+// one v_perm_b32 packs both), lo = v - hi exactly in FP32, rounded to bf16 by v_cvt_pk_bf16_f32.
+// |v - hi - lo| <= 2^-16 |v|, the same order as a rounded split.
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float floatx2 __attribute__((ext_vector_type(2)));
+MOCAT_DEV void split_pair(float v0, float v1, uint32_t& hi, uint32_t& lo) {
+  const uint32_t u0 = __float_as_uint(v0) & 0xffff0000u, u1 = __float_as_uint(v1) & 0xffff0000u;
+  hi = __builtin_amdgcn_perm(u1, u0, 0x07060302u);
+  const floatx2 l = {v0 - __uint_as_float(u0), v1 - __uint_as_float(u1)};
+  const bf16x2 lb = __builtin_convertvector(l, bf16x2);
+  lo = __builtin_bit_cast(uint32_t, lb);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -265,6 +285,9 @@ struct Fp64Engine {
 //   y layout [2][16] f32                  g = l/32: y[acc_row(r, l)], 0.5 on padding rows (x = 0 there)
 template <int DP>
 struct PackLayout {
+  // The engine reads X premultiplied by -log2(e) in the GEMM-1 (A) layout, so that its accumulators
+  // hold z' = -z log2(e) and exp(-z) = 2^z' needs no multiply; the B layout is plain X.
+  static constexpr float kAScale = -kLog2E;
   static constexpr int KB = DP / 16;
   static constexpr int NB = DP / 32;
   static constexpr int A_ELEMS = 2 * KB * 512;
@@ -278,26 +301,48 @@ MOCAT_DEV void split_bf16(float v, __bf16& hi, __bf16& lo) {
   lo = (__bf16)(v - (float)hi);
 }
 
+// ------------------------------------------------------------------------------------------
+// BF16 split-precision MFMA engine (every product hi*hi + hi*lo + lo*hi, FP32 accumulation).
+// Z = X Theta^T is accumulated with rows of X on the M side and the 32 chains on N, so each lane's
+// accumulator column is one chain; R = sigmoid(Z) - y is formed in registers and reused directly as
+// the A operand (chains x rows) of G = R X: the K slots of that operand are the rows the lane already
+// holds, and X's second packed layout lists its rows in the same order. With NW waves, wave w takes
+// the 32-row blocks rb = w + NW j; two waves per SIMD (NW = 8) let one wave's row functions on the
+// vector ALU overlap the other's MFMAs.
 template <int DP, int NT>
 struct Bf16Engine {
   using L = PackLayout<DP>;
-  static constexpr int KB = L::KB, NB = L::NB, NW = NT / kWave, HW = NW / 2;
-  static_assert(NW % 2 == 0, "Bf16Engine needs an even wave count");
+  static constexpr int KB = L::KB, NB = L::NB, NW = NT / kWave;
+  static_assert(NW % 4 == 0, "whole SIMDs");
+  static_assert(L::kAScale == -kLog2E, "the rows expect the scaled A layout");
+  static constexpr bool kThetaRegs = DP <= 64;
+  static constexpr int GS = DP + 4;  // row stride of the partial gradients (spreads LDS banks)
   struct alignas(16) Smem {
-    alignas(16) float gred[HW][CT * DP];
+    alignas(16) float gred[NW][CT * GS];
     alignas(16) bf16x8 thl[2][KB][64];  // theta^T operand, hi and lo parts, per lane
     float ured[NW][2][CT];
   };
+  struct ABuf {
+    bf16x8 h[KB], l[KB];
+  };
+  struct BBuf {
+    floatx4 y[4];
+    bf16x8 h[2][NB], l[2][NB];
+  };
 
-  // want_pot = false: gradient only (the leapfrog needs the potential only after its last step)
-  static MOCAT_DEV void eval(Smem& s, const float*, const float*, const void* packed, int n, int d,
+  static MOCAT_DEV void eval(Smem& s, const float* X, const float* y, const void* packed, int n, int d,
                              const float* theta, float* lik, float* glik, bool want_pot) {
-    const int t = threadIdx.x;
-    const int lane = t & (kWave - 1), w = t / kWave, g = lane >> 5, col = lane & 31;
-    const int nrb = (n + 31) / 32;
-    const char* base = reinterpret_cast<const char*>(packed);
+    // the potential terms are needed after the last leapfrog step only: two compiled passes, so that
+    // each block's row functions and MFMAs form one straight-line region the scheduler can interleave
+    if (want_pot)
+      pass<true>(s, packed, n, theta, lik, glik);
+    else
+      pass<false>(s, packed, n, theta, lik, glik);
+  }
 
-    // theta^T as B operand: lane holds chain `col`, columns kb*16 + g*8 + s; split once into LDS
+  template <bool WANT_POT>
+  static MOCAT_DEV void pass(Smem& s, const void* packed, int n, const float* theta, float* lik, float* glik) {
+    const int t = threadIdx.x;
     for (int e = t; e < KB * 64; e += NT) {
       const int kb = e / 64, ln = e % 64;
       const float* src = theta + (ln & 31) * DP + kb * 16 + (ln >> 5) * 8;
@@ -313,118 +358,13 @@ struct Bf16Engine {
       s.thl[1][kb][ln] = l8;
     }
     __syncthreads();
-    floatx16 gacc[NB];
-#pragma unroll
-    for (int nb = 0; nb < NB; ++nb)
-#pragma unroll
-      for (int r = 0; r < 16; ++r) gacc[nb][r] = 0.f;
-    float upart = 0.f;
-
-    // A operands are loaded at the top of each block: loading the next block's during the current one
-    // measured no faster, and inside the fused iteration kernel the extra registers spill
-    bf16x8 ah[KB], al[KB];
-    floatx4 yq[4];
-    auto load_a = [&](int rb) {
-      const char* blk = base + (int64_t)rb * L::BLOCK_BYTES;
-      const bf16x8* A = reinterpret_cast<const bf16x8*>(blk);
-#pragma unroll
-      for (int kb = 0; kb < KB; ++kb) {
-        ah[kb] = A[(0 * KB + kb) * 64 + lane];
-        al[kb] = A[(1 * KB + kb) * 64 + lane];
-      }
-    };
-    for (int rb = w; rb < nrb; rb += NW) {
-      // DP = 128: re-read the theta operands from LDS each block instead of holding 64 more registers
-      if constexpr (DP > 64) asm volatile("" ::: "memory");
-      load_a(rb);
-      const char* blk = base + (int64_t)rb * L::BLOCK_BYTES;
-      const bf16x8* B = reinterpret_cast<const bf16x8*>(blk) + L::A_ELEMS / 8;
-      floatx16 z;
-#pragma unroll
-      for (int r = 0; r < 16; ++r) z[r] = 0.f;
-#pragma unroll
-      for (int kb = 0; kb < KB; ++kb) {
-        const bf16x8 th = s.thl[0][kb][lane], tl = s.thl[1][kb][lane];
-        z = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[kb], th, z, 0, 0, 0);
-        z = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[kb], tl, z, 0, 0, 0);
-        z = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[kb], th, z, 0, 0, 0);
-      }
-      const floatx4* yv4 = reinterpret_cast<const floatx4*>(blk + L::BLOCK_BF16 * 2 + g * 64);
-#pragma unroll
-      for (int q = 0; q < 4; ++q) yq[q] = yv4[q];
-      float yv[16];
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        yv[4 * q] = yq[q].x;
-        yv[4 * q + 1] = yq[q].y;
-        yv[4 * q + 2] = yq[q].z;
-        yv[4 * q + 3] = yq[q].w;
-      }
-      // B operands of this block
-      bf16x8 xbh[2][NB], xbl[2][NB];
-#pragma unroll
-      for (int kb2 = 0; kb2 < 2; ++kb2)
-#pragma unroll
-        for (int nb = 0; nb < NB; ++nb) {
-          xbh[kb2][nb] = B[((0 * 2 + kb2) * NB + nb) * 64 + lane];
-          xbl[kb2][nb] = B[((1 * 2 + kb2) * NB + nb) * 64 + lane];
-        }
-      bf16x8 rh[2], rl[2];
-      if (want_pot) {
-#pragma unroll
-        for (int r = 0; r < 16; ++r) {
-          float cc;
-          const float rr = lr_row_full(z[r], yv[r], cc);
-          upart += (rb * 32 + acc_row(r, lane) < n) ? cc : 0.f;
-          __bf16 h, l;
-          split_bf16(rr, h, l);
-          rh[r >> 3][r & 7] = h;
-          rl[r >> 3][r & 7] = l;
-        }
-      } else {
-#pragma unroll
-        for (int r = 0; r < 16; ++r) {
-          const float rr = lr_row_grad(z[r], yv[r]);
-          __bf16 h, l;
-          split_bf16(rr, h, l);
-          rh[r >> 3][r & 7] = h;
-          rl[r >> 3][r & 7] = l;
-        }
-      }
-#pragma unroll
-      for (int kb2 = 0; kb2 < 2; ++kb2) {
-#pragma unroll
-        for (int nb = 0; nb < NB; ++nb) {
-          gacc[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(rh[kb2], xbh[kb2][nb], gacc[nb], 0, 0, 0);
-          gacc[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(rh[kb2], xbl[kb2][nb], gacc[nb], 0, 0, 0);
-          gacc[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(rl[kb2], xbh[kb2][nb], gacc[nb], 0, 0, 0);
-        }
-      }
-    }
-    // deterministic cross-wave reduction: upper half adds into lower half, then ordered sum
-    s.ured[w][g][col] = upart;
-    __syncthreads();
-    if (w >= HW) {
-#pragma unroll
-      for (int nb = 0; nb < NB; ++nb)
-#pragma unroll
-        for (int r = 0; r < 16; ++r) s.gred[w - HW][acc_row(r, lane) * DP + nb * 32 + col] = gacc[nb][r];
-    }
-    __syncthreads();
-    if (w < HW) {
-#pragma unroll
-      for (int nb = 0; nb < NB; ++nb)
-#pragma unroll
-        for (int r = 0; r < 16; ++r) {
-          float* p = &s.gred[w][acc_row(r, lane) * DP + nb * 32 + col];
-          *p = gacc[nb][r] + *p;
-        }
-    }
+    partials<WANT_POT>(s, packed, n);
     __syncthreads();
     for (int e = t; e < CT * DP; e += NT) {
-      float acc = s.gred[0][e];
+      const int c = e / DP, k = e - c * DP;
+      float acc = s.gred[0][c * GS + k];
 #pragma unroll
-      for (int q = 1; q < HW; ++q) acc += s.gred[q][e];
+      for (int q = 1; q < NW; ++q) acc += s.gred[q][c * GS + k];
       glik[e] = acc;
     }
     if (t < CT) {
@@ -433,6 +373,142 @@ struct Bf16Engine {
       lik[t] = acc;
     }
     __syncthreads();
+  }
+
+  // The block loop of one gradient evaluation. Theta's operands are read from s.thl (written by the
+  // caller before a barrier); each wave leaves its partial gradient in s.gred[w] ([chain][GS]) and its
+  // potential partials in s.ured[w]. The caller synchronises before reading them.
+  template <bool WANT_POT>
+  static MOCAT_DEV void partials(Smem& s, const void* packed, int n) {
+    const int t = threadIdx.x;
+    const int lane = t & (kWave - 1), w = t / kWave, g = lane >> 5, col = lane & 31;
+    const int nrb = (n + 31) / 32;
+    const int nmine = nrb > w ? (nrb - w + NW - 1) / NW : 0;  // blocks rb = w + j * NW, j < nmine
+    const char* base = reinterpret_cast<const char*>(packed);
+    bf16x8 th[kThetaRegs ? KB : 1], tl[kThetaRegs ? KB : 1];
+    if constexpr (kThetaRegs) {
+#pragma unroll
+      for (int kb = 0; kb < KB; ++kb) {
+        th[kb] = s.thl[0][kb][lane];
+        tl[kb] = s.thl[1][kb][lane];
+      }
+    }
+    floatx16 gacc[NB];
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) gacc[nb][r] = 0.f;
+    float upart = 0.f;
+
+    auto blk = [&](int j) { return base + (int64_t)(w + j * NW) * L::BLOCK_BYTES; };
+    auto load_a = [&](ABuf& a, int j) {
+      const bf16x8* A = reinterpret_cast<const bf16x8*>(blk(j));
+#pragma unroll
+      for (int kb = 0; kb < KB; ++kb) {
+        a.h[kb] = A[(0 * KB + kb) * 64 + lane];
+        a.l[kb] = A[(1 * KB + kb) * 64 + lane];
+      }
+    };
+    auto load_b = [&](BBuf& b, int j) {
+      const char* bp = blk(j);
+      const floatx4* yv4 = reinterpret_cast<const floatx4*>(bp + L::BLOCK_BF16 * 2 + g * 64);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) b.y[q] = yv4[q];
+      const bf16x8* B = reinterpret_cast<const bf16x8*>(bp) + L::A_ELEMS / 8;
+#pragma unroll
+      for (int kb2 = 0; kb2 < 2; ++kb2)
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) {
+          b.h[kb2][nb] = B[((0 * 2 + kb2) * NB + nb) * 64 + lane];
+          b.l[kb2][nb] = B[((1 * 2 + kb2) * NB + nb) * 64 + lane];
+        }
+    };
+    auto gemm1 = [&](const ABuf& a, floatx16& z) {
+#pragma unroll
+      for (int r = 0; r < 16; ++r) z[r] = 0.f;
+#pragma unroll
+      for (int kb = 0; kb < KB; ++kb) {
+        bf16x8 h8, l8;
+        if constexpr (kThetaRegs) {
+          h8 = th[kb];
+          l8 = tl[kb];
+        } else {
+          h8 = s.thl[0][kb][lane];
+          l8 = s.thl[1][kb][lane];
+        }
+        z = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a.h[kb], h8, z, 0, 0, 0);
+        z = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a.h[kb], l8, z, 0, 0, 0);
+        z = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a.l[kb], h8, z, 0, 0, 0);
+      }
+    };
+    // one block: rows of Z_j (zc) with labels and B operands bc, then G += R_j X_j
+    auto step = [&](int j, const floatx16& zc, const BBuf& bc) {
+      float yv[16];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        yv[4 * q] = bc.y[q].x;
+        yv[4 * q + 1] = bc.y[q].y;
+        yv[4 * q + 2] = bc.y[q].z;
+        yv[4 * q + 3] = bc.y[q].w;
+      }
+      float rv[16];
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        if constexpr (WANT_POT) {
+          float cc;
+          rv[r] = lr_row_full_hw(zc[r], yv[r], cc);
+          upart += cc;  // padding rows are taken out after the loop
+        } else {
+          rv[r] = lr_row_grad_hw(zc[r], yv[r]);
+        }
+      }
+      typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+      u32x4 ph[2], pl[2];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        uint32_t hw, lw;
+        split_pair(rv[2 * q], rv[2 * q + 1], hw, lw);
+        ph[q >> 2][q & 3] = hw;
+        pl[q >> 2][q & 3] = lw;
+      }
+#pragma unroll
+      for (int kb2 = 0; kb2 < 2; ++kb2) {
+        const bf16x8 rh = __builtin_bit_cast(bf16x8, ph[kb2]), rl = __builtin_bit_cast(bf16x8, pl[kb2]);
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) {
+          gacc[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(rh, bc.h[kb2][nb], gacc[nb], 0, 0, 0);
+          gacc[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(rh, bc.l[kb2][nb], gacc[nb], 0, 0, 0);
+          gacc[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(rl, bc.h[kb2][nb], gacc[nb], 0, 0, 0);
+        }
+      }
+    };
+    for (int j = 0; j < nmine; ++j) {
+      ABuf a;
+      BBuf b;
+      floatx16 z;
+      load_a(a, j);
+      gemm1(a, z);
+      load_b(b, j);
+      step(j, z, b);
+    }
+    if constexpr (WANT_POT) {
+      // padding rows of the last block (x = 0, y = 0.5: z = 0) each added the same term
+      const int rbl = nrb - 1;
+      if (rbl % NW == w) {
+        float cpad;
+        lr_row_full_hw(0.f, 0.5f, cpad);
+        int npad = 0;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) npad += (rbl * 32 + acc_row(r, lane) >= n);
+        upart -= (float)npad * cpad;
+      }
+    }
+    // every wave leaves its partials; readers sum them over the waves in order (deterministic)
+    s.ured[w][g][col] = upart;
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) s.gred[w][acc_row(r, lane) * GS + nb * 32 + col] = gacc[nb][r];
   }
 };
 
@@ -453,7 +529,7 @@ __global__ void lr_pack_kernel(const float* __restrict__ X, const float* __restr
       const int s = q & 7, lane = (q >> 3) & 63, pk = q >> 9;  // pk = part*KB + kb
       const int part = pk / L::KB, kb = pk - part * L::KB;
       const int row = rb * 32 + (lane & 31), colm = kb * 16 + (lane >> 5) * 8 + s;
-      const float v = (row < n && colm < d) ? X[(int64_t)row * d + colm] : 0.f;
+      const float v = (row < n && colm < d) ? X[(int64_t)row * d + colm] * L::kAScale : 0.f;
       __bf16 h, l;
       split_bf16(v, h, l);
       reinterpret_cast<__bf16*>(blk)[q] = part ? l : h;
@@ -802,6 +878,224 @@ __global__ __launch_bounds__(NT) void hmc_lr_kernel(HmcArgs h) {
 
 constexpr int kThreads = 512;
 
+// The same iteration as hmc_lr_kernel for the bf16x3 engine, with the tile state in
+// registers: thread t owns chain c = t % 32 and the EPT = DP / 8 columns k0 = EPT * (t / 32) .. of
+// position, momentum and gradient. A leapfrog step is then: update the owned elements and write their
+// bf16 hi/lo split into the engine's theta operand (barrier), the engine's block loop (barrier), and
+// the owned gradient elements summed from the four waves' partials. The refreshed momenta, position
+// and gradient of the pre-proposal state stay in global memory until a proposal is accepted (as in
+// hmc_lr_kernel); per-chain scalars and keys live in LDS.
+template <int DP, int NT>
+__global__ __launch_bounds__(NT) void hmc_lr_bf16_kernel(HmcArgs h) {
+  using E = Bf16Engine<DP, NT>;
+  constexpr int NG = NT / CT;  // column groups
+  constexpr int EPT = DP / NG;
+  static_assert(NG * EPT == DP && EPT % 4 == 0, "thread ownership");
+  __shared__ typename E::Smem es;
+  __shared__ float sq0[NG][CT], sq1[NG][CT], sqt[NG][CT];
+  __shared__ float step[CT], pot[CT], prior[CT], likv[CT], alpha[CT];
+  __shared__ uint32_t sub[CT][2];
+  __shared__ int accept[CT];
+  const int t = threadIdx.x, c = t % CT, kg = t / CT, k0 = kg * EPT;
+  const int d = h.lr.d;
+  const int64_t c0 = (int64_t)blockIdx.x * CT;
+  const int nvalid = (int)((h.C - c0) < CT ? (h.C - c0) : CT);
+  const bool cv = c < nvalid;
+  const int64_t row = (c0 + c) * (int64_t)d;
+  const float prec = h.lr.prior_prec, temp = h.lr.temperature;
+
+  float th[EPT], p[EPT], g[EPT];
+#pragma unroll
+  for (int q = 0; q < EPT; ++q) {
+    const int k = k0 + q;
+    const bool v = cv && k < d;
+    th[q] = v ? h.theta[row + k] : 0.f;
+    p[q] = v ? h.mom[row + k] : 0.f;
+    g[q] = v ? h.grad[row + k] : 0.f;
+  }
+  Key key{0u, 0u};
+  if (t < CT) {
+    const bool v = t < nvalid;
+    pot[t] = v ? h.pot[c0 + t] : 0.f;
+    prior[t] = v ? h.prior[c0 + t] : 0.f;
+    likv[t] = v ? h.lik[c0 + t] : 0.f;
+    step[t] = v ? h.step[c0 + t] : 1.f;  // padding chains: finite dummy stepsize
+    alpha[t] = v ? h.alpha[c0 + t] : 0.f;
+    key = v ? Key{h.keys[2 * (c0 + t)], h.keys[2 * (c0 + t) + 1]} : Key{0u, 0u};
+  }
+  const uint32_t L1 = (uint32_t)h.leapfrog_steps + 1u;
+  for (int it = 0; it < h.n_iter; ++it) {
+    const int64_t iter = h.iter0 + it + 1;
+    // ---- always: p <- -p ; (key, sub) = split(key) ; OU refresh (standard_mcmc.py:107-123); then the
+    // proposal's keys = split(key, L+1), key = keys[0] (standard_mcmc.py:129-131)
+    float pot0 = 0.f, prior0 = 0.f, lik0 = 0.f;
+    if (t < CT) {
+      const Key nk = split_at(key, 2, 0), sk = split_at(key, 2, 1);
+      sub[t][0] = sk.a;
+      sub[t][1] = sk.b;
+      key = split_at(nk, L1, 0);
+      pot0 = pot[t];
+      prior0 = prior[t];
+      lik0 = likv[t];
+    }
+    __syncthreads();
+    const float hh = step[c];
+    {
+      const float decay = __expf(-h.friction * hh);
+      const float noise = sqrtf(1.f - __expf(-2.f * h.friction * hh));
+      const Key sk{sub[c][0], sub[c][1]};
+      float s2 = 0.f;
+#pragma unroll
+      for (int q = 0; q < EPT; ++q) {
+        const int k = k0 + q;
+        if (k < d) {
+          p[q] = -p[q] * decay + noise * normal_from_bits(bits_at(sk, (uint32_t)d, (uint32_t)k));
+          if (cv) h.mom[row + k] = p[q];  // the refreshed momenta, kept for a rejection
+        }
+        s2 += p[q] * p[q];
+      }
+      sq0[kg][c] = s2;
+    }
+    // ---- leapfrog (utils.py:117-146)
+    for (int s = 0; s < h.leapfrog_steps; ++s) {
+      const bool last = s == h.leapfrog_steps - 1;
+      float s2 = 0.f;
+#pragma unroll
+      for (int q = 0; q < EPT; ++q) {
+        const float ph = p[q] - (hh / 2.f) * g[q];
+        p[q] = ph;
+        th[q] = (k0 + q < d) ? th[q] + hh * ph : 0.f;
+        s2 += th[q] * th[q];
+      }
+      if (last) sqt[kg][c] = s2;
+      typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+#pragma unroll
+      for (int j = 0; j < EPT / 4; ++j) {
+        bf16x4 h4, l4;
+#pragma unroll
+        for (int s4 = 0; s4 < 4; ++s4) {
+          __bf16 hb, lb;
+          split_bf16(th[4 * j + s4], hb, lb);
+          h4[s4] = hb;
+          l4[s4] = lb;
+        }
+        // columns colb .. colb + 3 of chain c: operand (kb, lane c + 32 * half), elements colb % 8 ..
+        const int colb = k0 + 4 * j;
+        const int ln = c + 32 * ((colb / 8) & 1);
+        reinterpret_cast<bf16x4*>(&es.thl[0][colb / 16][ln])[(colb / 4) & 1] = h4;
+        reinterpret_cast<bf16x4*>(&es.thl[1][colb / 16][ln])[(colb / 4) & 1] = l4;
+      }
+      __syncthreads();
+      if (last)
+        E::template partials<true>(es, h.lr.packed, h.lr.n);
+      else
+        E::template partials<false>(es, h.lr.packed, h.lr.n);
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < EPT; q += 4) {
+        floatx4 acc = *reinterpret_cast<const floatx4*>(&es.gred[0][c * E::GS + k0 + q]);
+#pragma unroll
+        for (int w = 1; w < E::NW; ++w) acc += *reinterpret_cast<const floatx4*>(&es.gred[w][c * E::GS + k0 + q]);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          g[q + e] = prec * th[q + e] + temp * acc[e];
+          p[q + e] = p[q + e] - (hh / 2.f) * g[q + e];
+        }
+      }
+      if (last && t < CT) {
+        float lk = 0.f, st = 0.f;
+        for (int w = 0; w < E::NW; ++w) lk += es.ured[w][0][t] + es.ured[w][1][t];
+        for (int q = 0; q < NG; ++q) st += sqt[q][t];
+        // prec == 0: improper flat prior, potential 0 (mocat's default uniform prior, core.py:183-189)
+        const float pr = prec > 0.f ? 0.5f * prec * st + 0.5f * (float)d * (kLog2Pi - logf(prec)) : 0.f;
+        prior[t] = pr;
+        likv[t] = lk;
+        pot[t] = pr + temp * lk;
+      }
+    }
+    {
+      float s2 = 0.f;
+#pragma unroll
+      for (int q = 0; q < EPT; ++q) s2 += p[q] * p[q];
+      sq1[kg][c] = s2;
+    }
+    __syncthreads();
+    // ---- correction (metropolis.py:48-70, 118-161)
+    if (t < CT) {
+      float k0s = 0.f, k1s = 0.f;
+      for (int q = 0; q < NG; ++q) {
+        k0s += sq0[q][t];
+        k1s += sq1[q][t];
+      }
+      const float kin0 = 0.5f * k0s + 0.5f * (float)d * kLog2Pi;
+      const float kin1 = 0.5f * k1s + 0.5f * (float)d * kLog2Pi;  // |-p|^2 == |p|^2
+      float a = 1.f;
+      int acc = 1;
+      if (h.correction > 0) {
+        a = fminf(1.f, __expf(-pot[t] + pot0 - kin1 + kin0));
+        a = isnan(a) ? 0.f : a;
+        const Key nk = split_at(key, 2, 0), sk = split_at(key, 2, 1);
+        key = nk;
+        const float u = uniform_from_bits(bits_at(sk, 1, 0), 0.f, 1.f);
+        acc = u < a;
+      }
+      accept[t] = acc;
+      alpha[t] = a;
+      if (!acc) {
+        pot[t] = pot0;
+        prior[t] = prior0;
+        likv[t] = lik0;
+      }
+      if (h.correction == 2 && (double)iter <= h.adapt_cut_off) {
+        const float rm = h.rm_scale * powf((float)iter, -h.rm_neg_exponent);
+        step[t] = expf(logf(step[t]) + rm * (a - h.target));
+      }
+    }
+    __syncthreads();
+    // accepted: momenta <- -p' and the proposal is stored; rejected: the pre-proposal state is read back
+    const bool acc = accept[c] != 0;
+#pragma unroll
+    for (int q = 0; q < EPT; ++q) {
+      const int k = k0 + q;
+      const bool v = cv && k < d;
+      if (acc) {
+        p[q] = -p[q];
+        if (v) {
+          h.theta[row + k] = th[q];
+          h.grad[row + k] = g[q];
+        }
+      } else {
+        p[q] = v ? h.mom[row + k] : 0.f;
+        th[q] = v ? h.theta[row + k] : 0.f;
+        g[q] = v ? h.grad[row + k] : 0.f;
+      }
+      if (v) {
+        if (h.tr_value) h.tr_value[((int64_t)it * h.C + c0 + c) * d + k] = th[q];
+        if (h.tr_mom) h.tr_mom[((int64_t)it * h.C + c0 + c) * d + k] = p[q];
+      }
+    }
+    if (t < nvalid) {
+      const int64_t o = (int64_t)it * h.C + c0 + t;
+      if (h.tr_pot) h.tr_pot[o] = pot[t];
+      if (h.tr_alpha) h.tr_alpha[o] = alpha[t];
+      if (h.tr_step) h.tr_step[o] = step[t];
+    }
+  }
+  // write back the final per-chain state (position and gradient are already in place)
+#pragma unroll
+  for (int q = 0; q < EPT; ++q)
+    if (cv && k0 + q < d) h.mom[row + k0 + q] = p[q];
+  if (t < nvalid) {
+    h.pot[c0 + t] = pot[t];
+    h.prior[c0 + t] = prior[t];
+    h.lik[c0 + t] = likv[t];
+    h.step[c0 + t] = step[t];
+    h.alpha[c0 + t] = alpha[t];
+    h.keys[2 * (c0 + t)] = key.a;
+    h.keys[2 * (c0 + t) + 1] = key.b;
+  }
+}
+
 template <template <int, int> class E>
 int dispatch_eval(const LrArgs& a, const float* theta, int64_t C, float* pp, float* lp, float* gp, float* gl,
                   hipStream_t st) {
@@ -823,6 +1117,33 @@ int dispatch_hmc(const HmcArgs& h, hipStream_t st) {
     hipLaunchKernelGGL((hmc_lr_kernel<E<64, kThreads>, 64, kThreads>), dim3(blocks), dim3(kThreads), 0, st, h);
   else
     hipLaunchKernelGGL((hmc_lr_kernel<E<128, kThreads>, 128, kThreads>), dim3(blocks), dim3(kThreads), 0, st, h);
+  MOCAT_LAUNCH_CHECK();
+  return MOCAT_OK;
+}
+
+// bf16x3: two waves per SIMD at d <= 64; at d <= 128 one wave per SIMD (the operand registers of a
+// 128-column block, and the LDS of eight waves' partial gradients, do not fit two waves)
+constexpr int kBf16Threads64 = 512, kBf16Threads128 = 256;
+
+int dispatch_eval_bf16(const LrArgs& a, const float* theta, int64_t C, float* pp, float* lp, float* gp, float* gl,
+                       hipStream_t st) {
+  const unsigned blocks = (unsigned)((C + CT - 1) / CT);
+  if (a.d <= 64)
+    hipLaunchKernelGGL((lr_eval_kernel<Bf16Engine<64, kBf16Threads64>, 64, kBf16Threads64>), dim3(blocks),
+                       dim3(kBf16Threads64), 0, st, a, theta, C, pp, lp, gp, gl);
+  else
+    hipLaunchKernelGGL((lr_eval_kernel<Bf16Engine<128, kBf16Threads128>, 128, kBf16Threads128>), dim3(blocks),
+                       dim3(kBf16Threads128), 0, st, a, theta, C, pp, lp, gp, gl);
+  MOCAT_LAUNCH_CHECK();
+  return MOCAT_OK;
+}
+
+int dispatch_hmc_bf16(const HmcArgs& h, hipStream_t st) {
+  const unsigned blocks = (unsigned)((h.C + CT - 1) / CT);
+  if (h.lr.d <= 64)
+    hipLaunchKernelGGL((hmc_lr_bf16_kernel<64, kBf16Threads64>), dim3(blocks), dim3(kBf16Threads64), 0, st, h);
+  else
+    hipLaunchKernelGGL((hmc_lr_bf16_kernel<128, kBf16Threads128>), dim3(blocks), dim3(kBf16Threads128), 0, st, h);
   MOCAT_LAUNCH_CHECK();
   return MOCAT_OK;
 }
@@ -877,7 +1198,7 @@ int mocat_lr_potential_grad(const float* X, const float* y, const void* packed, 
   if (precision == 3)
     return dispatch_eval<Fp64Engine>(a, theta, C, prior_pot, lik_pot, grad_prior, grad_lik, stream);
   return precision == 0 ? dispatch_eval<Fp32Engine>(a, theta, C, prior_pot, lik_pot, grad_prior, grad_lik, stream)
-                        : dispatch_eval<Bf16Engine>(a, theta, C, prior_pot, lik_pot, grad_prior, grad_lik, stream);
+                        : dispatch_eval_bf16(a, theta, C, prior_pot, lik_pot, grad_prior, grad_lik, stream);
 }
 
 int mocat_hmc_lr_iterate(const float* X, const float* y, const void* packed, int64_t n, int64_t d, int precision,
@@ -899,7 +1220,7 @@ int mocat_hmc_lr_iterate(const float* X, const float* y, const void* packed, int
             trace_value, trace_potential, trace_alpha, trace_stepsize, trace_momenta};
   if (precision == 2) return dispatch_hmc<GaussEngine>(h, stream);
   if (precision == 3) return dispatch_hmc<Fp64Engine>(h, stream);
-  return precision == 0 ? dispatch_hmc<Fp32Engine>(h, stream) : dispatch_hmc<Bf16Engine>(h, stream);
+  return precision == 0 ? dispatch_hmc<Fp32Engine>(h, stream) : dispatch_hmc_bf16(h, stream);
 }
 
 }  // extern "C"

@@ -11,6 +11,7 @@ import math
 
 import numpy as np
 import torch
+from torch.autograd.function import once_differentiable
 
 from . import _lib
 from . import random as mr
@@ -26,6 +27,36 @@ _lib.register("mocat_lr_pack", [_lib.c_ptr, _lib.c_ptr, _lib.c_i64, _lib.c_i64, 
 _lib.register("mocat_lr_potential_grad",
               [_lib.c_ptr, _lib.c_ptr, _lib.c_ptr, _lib.c_i64, _lib.c_i64, _lib.c_i32, _lib.c_f32, _lib.c_f32,
                _lib.c_ptr, _lib.c_i64, _lib.c_ptr, _lib.c_ptr, _lib.c_ptr, _lib.c_ptr, _lib.c_ptr])
+
+
+class FusedPotential(torch.autograd.Function):
+    """A fused scenario's potential as a differentiable function of x (the torch counterpart of the
+    jax.custom_vjp that mocat's jax.grad would otherwise differentiate through,
+    mocat/src/core.py:204-230). The forward pass runs the scenario's kernel, which evaluates the
+    potential and its hand-written adjoint together; the backward pass scales that adjoint by the
+    incoming gradient. `part` selects "prior", "likelihood" or "tempered" (prior + temperature * lik).
+    Second derivatives are not provided (the kernels have no second-order adjoint)."""
+
+    @staticmethod
+    def forward(ctx, x, scenario, temperature, part):
+        pp, lp, gp, gl = scenario.evaluate(x.detach(), temperature)
+        if part == "prior":
+            val, g = pp, gp
+        elif part == "likelihood":
+            val, g = lp, gl
+        elif part == "tempered":
+            val, g = pp + temperature * lp, gp + temperature * gl
+        else:
+            raise ValueError(f"unknown potential part {part!r}")
+        ctx.save_for_backward(g)
+        ctx.x_device = x.device
+        return val
+
+    @staticmethod
+    @once_differentiable
+    def backward(ctx, grad_val):
+        (g,) = ctx.saved_tensors
+        return (grad_val.to(g.device).unsqueeze(-1) * g).to(ctx.x_device), None, None, None
 
 
 def _dev_tensor(a, dtype=torch.float32):
@@ -90,11 +121,15 @@ class LogisticRegression(Scenario):
                   _lib.ptr(pp), _lib.ptr(lp), _lib.ptr(gp), _lib.ptr(gl), _lib.stream_handle())
         return pp, lp, gp, gl
 
+    # differentiable potentials: torch.autograd reaches the kernel's adjoint through FusedPotential
     def prior_potential(self, x, random_key=None):
-        return self.evaluate(x)[0]
+        return FusedPotential.apply(x, self, 1.0, "prior")
 
     def likelihood_potential(self, x, random_key=None):
-        return self.evaluate(x)[1]
+        return FusedPotential.apply(x, self, 1.0, "likelihood")
+
+    def tempered_potential(self, x, temperature, random_key=None):
+        return FusedPotential.apply(x, self, float(temperature), "tempered")
 
     def prior_potential_and_grad(self, x, random_key=None):
         pp, _, gp, _ = self.evaluate(x)
@@ -207,7 +242,10 @@ class GaussianMixture(Scenario):
         return torch.zeros(x.shape[:-1], dtype=torch.float32, device=x.device)
 
     def likelihood_potential(self, x, random_key=None):
-        return self.evaluate(x)[1]
+        return FusedPotential.apply(x, self, 1.0, "likelihood")
+
+    def tempered_potential(self, x, temperature, random_key=None):
+        return FusedPotential.apply(x, self, float(temperature), "tempered")
 
     def prior_potential_and_grad(self, x, random_key=None):
         return self.prior_potential(x), torch.zeros_like(x)
