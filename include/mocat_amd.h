@@ -167,6 +167,14 @@ int mocat_pf_run_large(int model, int d, int dy, const mocat_pf_filter* host_fil
  * workspace holds n uint64. New resamplers beside mocat's _resample (filtering.py:196-199). */
 int mocat_resample(const float* weights, int64_t n, int method, int from_log, const uint32_t* key, void* workspace,
                    int32_t* idx, hipStream_t stream);
+/* mocat_resample without a workspace argument and without allocating: no counterpart in the
+ * reference; for callers that cannot allocate device memory (code under HIP graph capture, serving
+ * loops, harnesses that forbid hipMalloc). The prefix sums go to a library-owned device array of
+ * 2^20 uint64 (8 MiB per GPU, held until the process ends): n > 2^20 returns -3 (size limit). All
+ * calls on one GPU share that array, so calls that overlap on one GPU (different streams, or other
+ * threads) race; calls ordered on one stream do not. Same results as mocat_resample. */
+int mocat_resample_noalloc(const float* weights, int64_t n, int method, int from_log, const uint32_t* key,
+                           int32_t* idx, hipStream_t stream);
 /* random.categorical(key, log_weight, shape=(n_out,)) by Gumbel-max: mocat's _resample
  * (mocat/src/ssm/filtering.py:196-199) and SMCSampler.resample (mocat/src/transport/smc.py:61-71). */
 int mocat_resample_categorical(const float* log_weight, int64_t n, int64_t n_out, const uint32_t* key, int32_t* idx,

@@ -20,6 +20,7 @@
 // The reference's resampler, random.categorical = Gumbel-max over an (n, n) Gumbel matrix
 // (filtering.py:196-199), is also provided (O(n^2)).
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 #include "common.hpp"
@@ -1729,6 +1730,45 @@ LfLayout lf_layout(const mocat_pf_filter* filters, int n_filters) {
 
 }  // namespace
 
+namespace {
+
+// The two launches of resampling from weights: exact prefix sums into ws (n uint64, every entry
+// written before it is read), then the ancestor search.
+int launch_resample(const float* weights, int64_t n, int method, int from_log, const uint32_t* key, uint64_t* ws,
+                    int32_t* idx, hipStream_t stream) {
+  hipLaunchKernelGGL(resample_csum_kernel, dim3(1), dim3(NT), 0, stream, weights, n, from_log, ws);
+  const int64_t blocks = (n + 255) / 256;
+  hipLaunchKernelGGL(resample_search_kernel, dim3((unsigned)(blocks < 4096 ? blocks : 4096)), dim3(256), 0, stream,
+                     (const uint64_t*)ws, n, key, method, idx);
+  MOCAT_LAUNCH_CHECK();
+  return MOCAT_OK;
+}
+
+// Scratch memory of mocat_resample_noalloc: the prefix sums of up to kScratchMaxWeights weights (8 MiB),
+// one copy per GPU, held until the process ends. Host code never passes the array's name to a kernel
+// (in host code it is a stand-in, not a device address): each device's copy is addressed through
+// the pointer hipGetSymbolAddress returns with that device current, looked up once per device.
+constexpr int64_t kScratchMaxWeights = int64_t(1) << 20;
+static __device__ __align__(256) uint64_t scratch[kScratchMaxWeights];
+constexpr int kScratchMaxDevices = 64;
+std::mutex g_scratch_mutex;
+uint64_t* g_scratch_ptr[kScratchMaxDevices] = {};
+
+int scratch_on_current_device(uint64_t** out) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kScratchMaxDevices) return MOCAT_ERR_HIP;
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return MOCAT_ERR_HIP;
+    g_scratch_ptr[dev] = static_cast<uint64_t*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return MOCAT_OK;
+}
+
+}  // namespace
+
 // (d, dy) pairs compiled for the linear-Gaussian models
 #define MOCAT_PF_LG_SHAPES                                                                                  \
   MOCAT_PF_CASE(1, 1) MOCAT_PF_CASE(2, 2) MOCAT_PF_CASE(4, 4) MOCAT_PF_CASE(4, 2) MOCAT_PF_CASE(5, 5) \
@@ -1871,12 +1911,17 @@ int mocat_resample(const float* weights, int64_t n, int method, int from_log, co
                    int32_t* idx, hipStream_t stream) {
   if (!weights || !key || !workspace || !idx || n <= 0 || n > 0x7FFFFFFFll || method < 0 || method > 1)
     return MOCAT_ERR_ARG;
-  hipLaunchKernelGGL(resample_csum_kernel, dim3(1), dim3(NT), 0, stream, weights, n, from_log, (uint64_t*)workspace);
-  const int64_t blocks = (n + 255) / 256;
-  hipLaunchKernelGGL(resample_search_kernel, dim3((unsigned)(blocks < 4096 ? blocks : 4096)), dim3(256), 0, stream,
-                     (const uint64_t*)workspace, n, key, method, idx);
-  MOCAT_LAUNCH_CHECK();
-  return MOCAT_OK;
+  return launch_resample(weights, n, method, from_log, key, static_cast<uint64_t*>(workspace), idx, stream);
+}
+
+int mocat_resample_noalloc(const float* weights, int64_t n, int method, int from_log, const uint32_t* key,
+                           int32_t* idx, hipStream_t stream) {
+  if (!weights || !key || !idx || n <= 0 || method < 0 || method > 1) return MOCAT_ERR_ARG;
+  if (n > kScratchMaxWeights) return MOCAT_ERR_SIZE;
+  uint64_t* ws = nullptr;
+  const int rc = scratch_on_current_device(&ws);
+  if (rc != MOCAT_OK) return rc;
+  return launch_resample(weights, n, method, from_log, key, ws, idx, stream);
 }
 
 int mocat_resample_categorical(const float* log_weight, int64_t n, int64_t n_out, const uint32_t* key, int32_t* idx,

@@ -31,6 +31,7 @@ _lib.register("mocat_pf_large_workspace_bytes", [_P, _lib.c_i32, _P])
 _lib.register("mocat_pf_run_large", [_lib.c_i32, _lib.c_i32, _lib.c_i32, _P, _lib.c_i32, _P, _P, _P, _P, _P, _P, _P,
                                      _lib.c_i32, _lib.c_f32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _lib.c_i64, _P])
 _lib.register("mocat_resample", [_P, _lib.c_i64, _lib.c_i32, _lib.c_i32, _P, _P, _P, _P])
+_lib.register("mocat_resample_noalloc", [_P, _lib.c_i64, _lib.c_i32, _lib.c_i32, _P, _P, _P])
 _lib.register("mocat_resample_categorical", [_P, _lib.c_i64, _lib.c_i64, _P, _P, _P])
 _lib.register("mocat_enkf_param_count", [_lib.c_i32, _lib.c_i32, _P])
 _lib.register("mocat_enkf_workspace_bytes", [_lib.c_i64, _lib.c_i32, _P])
