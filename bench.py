@@ -224,12 +224,21 @@ def bench_pf5():
     from mocat_amd import datasets
     models, ys, ns = datasets.sv_batch_config(0)
     keys = m.random.split(m.random.PRNGKey(0), len(models))
-    run = lambda: m.ssm.run_particle_filters(models, ys, [k for k in keys], ns, 1.0, "systematic")
+    # the schedule autotuner times the candidate schedules (chunked-path threshold, graph launch) on
+    # this batch once, untimed, and records its choice; the timed runs use the recorded schedule
+    from mocat_amd.ssm import schedule as sched
+    t0 = time.perf_counter()
+    run = lambda: m.ssm.run_particle_filters(models, ys, [k for k in keys], ns, 1.0, "systematic", schedule="auto")
     run()
+    tune_s = time.perf_counter() - t0
+    choice = sched.lookup(sched.signature(models, ys, ns, "systematic", False, False, None))
     res, dt = _timed(run)
+    n_large = sum(1 for v in ns if v >= choice["large_n"])
+    launches = (3 * (max(len(y) for y, v in zip(ys, ns) if v >= choice["large_n"]) - 1) + 3) if n_large else 0
     E = sum(n * len(y) for n, y in zip(ns, ys))
     big = max(n * len(y) for n, y in zip(ns, ys))
-    rung = lambda: m.ssm.run_particle_filters(models, ys, [k for k in keys], ns, 1.0, "systematic", grad=True)
+    rung = lambda: m.ssm.run_particle_filters(models, ys, [k for k in keys], ns, 1.0, "systematic", grad=True,
+                                              schedule="auto")
     rung()
     _, dtg = _timed(rung)
     import numpy as np
@@ -251,8 +260,15 @@ def bench_pf5():
 
     cpu = _cpu_sample(cpu_run, tot, "particle-steps/s",
                       f"oracle SV filters {pick} of the batch ({tot} particle-steps, from the median filter size up)")
+    gchoice = sched.lookup(sched.signature(models, ys, ns, "systematic", True, False, None))
     print(json.dumps({"mode": "pf5", "metric": "particle-steps/s", "cpu_baseline": cpu, "value": E / dt,
                       "ms": dt * 1e3,
+                      "schedule": {"chosen": {k: choice[k] for k in ("large_n", "graph")},
+                                   "timings_ms": choice["timings_ms"], "tuning_seconds": tune_s,
+                                   "chosen_with_gradient": {k: gchoice[k] for k in ("large_n", "graph")},
+                                   "filters_on_chunked_path": n_large,
+                                   "chunked_path_kernel_launches": launches,
+                                   "chunked_path_submissions": 1 if choice["graph"] else launches},
                       "with_gradient": E / dtg, "with_gradient_ms": dtg * 1e3,
                       "particle_steps": E, "largest_filter_particle_steps": big,
                       "bytes_per_particle_step": 24, "achieved_GBps_algorithmic": 24 * E / dt / 1e9}))
@@ -286,9 +302,33 @@ def bench_smc3(n_particles=65536):
     cpu = _cpu_sample(cpu_run, 1024 * 10, "particle-moves/s",
                       "oracle RM-MALA moves of 1024 particles x 10 on the same X (20000 x 100); temperature search and "
                       "resampling not included")
+    # stated error: the same run (same keys) with every contraction in FP64, and the Monte Carlo spread
+    # of the bf16x3 estimator over two more seeds
+    import torch
+    lz = chain.log_norm_constant[-1].item()
+    sc64 = m.LogisticRegression(X, y, prior_variance=10.0, precision="fp64")
+    chain64, dt64 = _timed(lambda: m.run(sc64, smp, n_particles, m.random.PRNGKey(0), stack=()))
+    lz64 = chain64.log_norm_constant[-1].item()
+    seeds = [lz] + [m.run(sc, smp, n_particles, m.random.PRNGKey(s), stack=()).log_norm_constant[-1].item()
+                    for s in (1, 2)]
+    xs = chain.final_state.value if hasattr(chain, "final_state") else None
+    err = None
+    if xs is not None:
+        _, lp, _, gl = sc.evaluate(xs)
+        _, lp64, _, gl64 = sc64.evaluate(xs)
+        err = {"lik_potential_rel_max": float(((lp - lp64).abs() / lp64.abs().clamp_min(1e-30)).max().item()),
+               "grad_lik_rel_max": float(((gl - gl64).abs().amax(1) / gl64.abs().amax(1).clamp_min(1e-30))
+                                         .max().item()),
+               "at": "final particles of the bf16x3 run, FP64 engine as reference"}
     print(json.dumps({"mode": "smc3", "metric": "particle-moves/s", "cpu_baseline": cpu,
                       "value": 5 * n_particles * n_temp / dt,
-                      "seconds": dt, "temperatures": n_temp, "log_evidence": chain.log_norm_constant[-1].item(),
+                      "seconds": dt, "temperatures": n_temp, "log_evidence": lz,
+                      "log_evidence_fp64_run": lz64, "temperatures_fp64_run": int(chain64.temperature.shape[0]) - 1,
+                      "fp64_run_seconds": dt64,
+                      "log_evidence_err_vs_fp64": abs(lz - lz64),
+                      "log_evidence_rel_err_vs_fp64": abs(lz - lz64) / abs(lz64),
+                      "log_evidence_seeds_0_1_2": seeds, "log_evidence_mc_sd": float(np.std(seeds, ddof=1)),
+                      "contraction_error_vs_fp64": err,
                       "gradient_evals_per_s": evals / dt, "algorithmic_TFLOPs": evals * 4 * 20000 * 100 / dt / 1e12}))
 
 
@@ -545,7 +585,7 @@ def main():
         "error_vs_fp64": err,
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak, "traffic": traffic,
-                     "kernel": "hmc_lr_kernel<Bf16Engine<64,512>>" if a.precision == "bf16x3"
+                     "kernel": "hmc_lr_bf16_kernel<64,512>" if a.precision == "bf16x3"
                      else "hmc_lr_kernel<Fp32Engine<64,512>>",
                      "kernel_ms_per_iteration": kernel_ms, "flop_per_iteration": flop_per_launch,
                      "iterations_per_launch": ITERATIONS,

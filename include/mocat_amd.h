@@ -22,7 +22,7 @@ extern "C" {
 #endif
 
 /* Returns the ABI revision of the library (not a status). */
-int mocat_abi_version(void); /* this header: 2 */
+int mocat_abi_version(void); /* this header: 3 */
 
 /* ---------------------------------------------------------------------------------------- RNG
  * jax.random.split / _random_bits / uniform / normal as called throughout mocat, e.g.
@@ -151,15 +151,20 @@ int mocat_pf_run(int model, int d, int dy, const mocat_pf_filter* host_filters, 
  * search, propagation and weighting). For the same log-weights and key the ancestors equal mocat_pf_run's. host_filters is a
  * host array (the launch sequence depends on n and T); workspace holds
  * mocat_pf_large_workspace_bytes bytes, including T step-key slots per flagged filter (independent of
- * t_offset, which only places out_ess / out_loglik_t and must be >= 0 when either is given). Calls on one stream may follow mocat_pf_run with the same
- * arguments: the two paths touch disjoint filters. */
+ * t_offset, which only places out_ess / out_loglik_t and must be >= 0 when either is given).
+ * launch_mode 0 queues the per-step launches on the stream; 1 captures them into one HIP graph,
+ * launches it on the stream and returns after it has run (when the stream is already being captured
+ * by the caller, the launches join the caller's graph instead). Results do not depend on the mode.
+ * Calls on one stream may follow mocat_pf_run with the same arguments: the two paths touch disjoint
+ * filters. */
 int mocat_pf_large_workspace_bytes(const mocat_pf_filter* host_filters, int n_filters, int64_t* bytes);
 int mocat_pf_run_large(int model, int d, int dy, const mocat_pf_filter* host_filters, int n_filters,
                        const float* params,
                        const float* y, float* x_ws0, float* x_ws1, float* lw_ws0, float* lw_ws1, uint64_t* csum_ws,
                        int resampling, float ess_threshold, float* out_value, float* out_lw, int32_t* out_anc,
                        float* out_ess, double* out_loglik_t, double* out_loglik, float* tan_ws0, float* tan_ws1,
-                       double* out_grad, void* workspace, int64_t workspace_bytes, hipStream_t stream);
+                       double* out_grad, void* workspace, int64_t workspace_bytes, int launch_mode,
+                       hipStream_t stream);
 
 /* Ancestor indices idx[n] from non-negative weights (from_log = 0) or log-weights (from_log = 1,
  * weights exp(w - max w)): method 0 systematic (u = uniform(key)), 1 multinomial

@@ -1655,21 +1655,63 @@ __global__ __launch_bounds__(LT) void lf_finish_kernel(LfArgs a) {
   }
 }
 
+// Capture stream of the graph launch mode, one per device (capture needs a stream other than the
+// null stream; the captured graph is then launched on the caller's stream).
+hipStream_t capture_stream() {
+  static std::mutex mu;
+  static hipStream_t streams[64] = {};
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return nullptr;
+  std::lock_guard<std::mutex> lock(mu);
+  if (!streams[dev] && hipStreamCreateWithFlags(&streams[dev], hipStreamNonBlocking) != hipSuccess)
+    streams[dev] = nullptr;
+  return streams[dev];
+}
+
+// launch_mode 0: the per-step launches go to the stream one by one. launch_mode 1: they are captured
+// into one graph (kernel nodes only) that is instantiated and launched on the stream as a single
+// submission; the call returns after the graph has run (the instantiated graph is released). When the
+// caller's stream is itself being captured, the launches simply join the caller's graph.
 template <class Model, int D, int DY, bool GRAD = false>
-int launch_pf_large(const LfArgs& a, int64_t max_T, int cend_slots, hipStream_t st) {
+int launch_pf_large(const LfArgs& a, int64_t max_T, int cend_slots, int launch_mode, hipStream_t st) {
   const dim3 chunks((unsigned)a.n_chunks), blk(LT);
   // the block-end sums are staged only by the multinomial search
   const size_t cend_bytes = a.resampling == 1 ? (size_t)cend_slots * sizeof(uint64_t) : 0;
-  hipLaunchKernelGGL((lf_init_kernel<Model, D, DY, GRAD>), chunks, blk, 0, st, a);
-  for (int64_t t = 1; t < max_T; ++t) {
-    hipLaunchKernelGGL((lf_prep_kernel<GRAD>), chunks, blk, 0, st, a, t);
-    hipLaunchKernelGGL(lf_csum_kernel, chunks, blk, 0, st, a, t);
-    hipLaunchKernelGGL((lf_step_kernel<Model, D, DY, GRAD>), chunks, blk, cend_bytes, st, a, t, cend_slots);
+  auto enqueue = [&](hipStream_t q) {
+    hipLaunchKernelGGL((lf_init_kernel<Model, D, DY, GRAD>), chunks, blk, 0, q, a);
+    for (int64_t t = 1; t < max_T; ++t) {
+      hipLaunchKernelGGL((lf_prep_kernel<GRAD>), chunks, blk, 0, q, a, t);
+      hipLaunchKernelGGL(lf_csum_kernel, chunks, blk, 0, q, a, t);
+      hipLaunchKernelGGL((lf_step_kernel<Model, D, DY, GRAD>), chunks, blk, cend_bytes, q, a, t, cend_slots);
+    }
+    hipLaunchKernelGGL((lf_prep_kernel<GRAD>), chunks, blk, 0, q, a, max_T);  // the last step's statistics
+    hipLaunchKernelGGL((lf_finish_kernel<D>), chunks, blk, 0, q, a);
+  };
+  hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+  if (launch_mode == 1 && hipStreamIsCapturing(st, &cap) != hipSuccess) return MOCAT_ERR_HIP;
+  if (launch_mode == 0 || cap != hipStreamCaptureStatusNone) {
+    enqueue(st);
+    MOCAT_LAUNCH_CHECK();
+    return MOCAT_OK;
   }
-  hipLaunchKernelGGL((lf_prep_kernel<GRAD>), chunks, blk, 0, st, a, max_T);  // the last step's statistics
-  hipLaunchKernelGGL((lf_finish_kernel<D>), chunks, blk, 0, st, a);
-  MOCAT_LAUNCH_CHECK();
-  return MOCAT_OK;
+  hipStream_t cs = capture_stream();
+  if (!cs) return MOCAT_ERR_HIP;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) return MOCAT_ERR_HIP;
+  enqueue(cs);
+  const hipError_t le = hipGetLastError();
+  if (hipStreamEndCapture(cs, &graph) != hipSuccess || le != hipSuccess || !graph) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return MOCAT_ERR_HIP;
+  }
+  hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (e != hipSuccess) return MOCAT_ERR_HIP;
+  e = hipGraphLaunch(exec, st);
+  const hipError_t se = hipStreamSynchronize(st);
+  (void)hipGraphExecDestroy(exec);
+  return (e == hipSuccess && se == hipSuccess) ? MOCAT_OK : MOCAT_ERR_HIP;
 }
 
 template <class Model, int D, int DY, bool GRAD = false>
@@ -1841,9 +1883,10 @@ int mocat_pf_run_large(int model, int d, int dy, const mocat_pf_filter* host_fil
                        const float* params, const float* y, float* x_ws0, float* x_ws1, float* lw_ws0, float* lw_ws1, uint64_t* csum_ws,
                        int resampling, float ess_threshold, float* out_value, float* out_lw, int32_t* out_anc,
                        float* out_ess, double* out_loglik_t, double* out_loglik, float* tan_ws0, float* tan_ws1,
-                       double* out_grad, void* workspace, int64_t workspace_bytes, hipStream_t stream) {
+                       double* out_grad, void* workspace, int64_t workspace_bytes, int launch_mode,
+                       hipStream_t stream) {
   if (n_filters < 0 || !host_filters || !params || !y || !x_ws0 || !x_ws1 || !lw_ws0 || !lw_ws1 || !csum_ws ||
-      !out_loglik || resampling < 0 || resampling > 2)
+      !out_loglik || resampling < 0 || resampling > 2 || launch_mode < 0 || launch_mode > 1)
     return MOCAT_ERR_ARG;
   if (out_grad && (model != 1 || !tan_ws0 || !tan_ws1)) return MOCAT_ERR_ARG;
   const LfLayout L = lf_layout(host_filters, n_filters);
@@ -1889,19 +1932,19 @@ int mocat_pf_run_large(int model, int d, int dy, const mocat_pf_filter* host_fil
            reinterpret_cast<StepKey*>(ws + L.o_keys)};
   if (model == 1) {
     if (d != 1 || dy != 1) return MOCAT_ERR_ARG;
-    return out_grad ? launch_pf_large<Sv, 1, 1, true>(a, L.max_T, L.cend_slots, stream)
-                    : launch_pf_large<Sv, 1, 1>(a, L.max_T, L.cend_slots, stream);
+    return out_grad ? launch_pf_large<Sv, 1, 1, true>(a, L.max_T, L.cend_slots, launch_mode, stream)
+                    : launch_pf_large<Sv, 1, 1>(a, L.max_T, L.cend_slots, launch_mode, stream);
   }
   if (model == 2) {
 #define MOCAT_PF_CASE(DD, YY) \
-  if (d == DD && dy == YY) return launch_pf_large<LgssmOpt<DD, YY>, DD, YY>(a, L.max_T, L.cend_slots, stream);
+  if (d == DD && dy == YY) return launch_pf_large<LgssmOpt<DD, YY>, DD, YY>(a, L.max_T, L.cend_slots, launch_mode, stream);
     MOCAT_PF_LG_SHAPES
 #undef MOCAT_PF_CASE
     return MOCAT_ERR_ARG;
   }
   if (model != 0) return MOCAT_ERR_ARG;
 #define MOCAT_PF_CASE(DD, YY) \
-  if (d == DD && dy == YY) return launch_pf_large<Lgssm<DD, YY>, DD, YY>(a, L.max_T, L.cend_slots, stream);
+  if (d == DD && dy == YY) return launch_pf_large<Lgssm<DD, YY>, DD, YY>(a, L.max_T, L.cend_slots, launch_mode, stream);
   MOCAT_PF_LG_SHAPES
 #undef MOCAT_PF_CASE
   return MOCAT_ERR_ARG;

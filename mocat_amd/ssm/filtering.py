@@ -29,7 +29,8 @@ _lib.register("mocat_pf_run", [_lib.c_i32, _lib.c_i32, _lib.c_i32, _P, _P, _lib.
                                _lib.c_i32, _lib.c_f32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P])
 _lib.register("mocat_pf_large_workspace_bytes", [_P, _lib.c_i32, _P])
 _lib.register("mocat_pf_run_large", [_lib.c_i32, _lib.c_i32, _lib.c_i32, _P, _lib.c_i32, _P, _P, _P, _P, _P, _P, _P,
-                                     _lib.c_i32, _lib.c_f32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _lib.c_i64, _P])
+                                     _lib.c_i32, _lib.c_f32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _lib.c_i64,
+                                     _lib.c_i32, _P])
 _lib.register("mocat_resample", [_P, _lib.c_i64, _lib.c_i32, _lib.c_i32, _P, _P, _P, _P])
 _lib.register("mocat_resample_noalloc", [_P, _lib.c_i64, _lib.c_i32, _lib.c_i32, _P, _P, _P])
 _lib.register("mocat_resample_categorical", [_P, _lib.c_i64, _lib.c_i64, _P, _P, _P])
@@ -236,9 +237,10 @@ def _large_flags(ns, Ts, large_n):
 
 
 def _launch(models, ys, keys, ns, ess_threshold, resampling, store, grad=False, large_n=None, particle_filter=None,
-            initial=None, extra_flags=0):
+            initial=None, extra_flags=0, graph=False):
     """Runs len(models) filters (same model class and dims): filters below large_n particles in one
-    persistent launch (a workgroup per filter), the others through the chunked per-step path."""
+    persistent launch (a workgroup per filter), the others through the chunked per-step path (its
+    launches captured into one HIP graph when graph=True)."""
     m0 = models[0]
     d, dy, kind = m0.dim, m0.dim_obs, m0.kernel_model
     for mdl in models:
@@ -328,7 +330,7 @@ def _launch(models, ys, keys, ns, ess_threshold, resampling, store, grad=False, 
                   _lib.ptr(x1), _lib.ptr(lw0), _lib.ptr(lw1), _lib.ptr(cs), RESAMPLING[resampling],
                   float(ess_threshold), _lib.ptr(out_value), _lib.ptr(out_lw), _lib.ptr(out_anc), _lib.ptr(out_ess),
                   _lib.ptr(out_llt), _lib.ptr(out_ll), _lib.ptr(tan0), _lib.ptr(tan1), _lib.ptr(out_grad),
-                  _lib.ptr(ws), int(ws.numel()), _lib.stream_handle())
+                  _lib.ptr(ws), int(ws.numel()), 1 if graph else 0, _lib.stream_handle())
     if side is not None and side is not cur:
         cur.wait_stream(side)
     res = [None] * F
@@ -498,20 +500,36 @@ def _resume(ssm_scenario, particle_filter, yn, tn, random_key, n, initial_sample
 
 def run_particle_filters(ssm_scenarios, ys, random_keys, ns, ess_threshold: float = 0.5,
                          resampling: str = "systematic", store: bool = False, grad: bool = False,
-                         large_n: int = None, particle_filter=None):
+                         large_n: int = None, particle_filter=None, schedule=None):
     """Independent bootstrap filters of uneven sizes in one launch. Returns a list of cdicts with
     log_likelihood (FP64), ess (T,), final_value, final_log_weight (and value/log_weight/ancestors when
     store=True). grad=True (stochastic volatility): log_likelihood_grad = d log Z / d(phi, sigma, beta)
     of the fixed-resampling-index estimator. Filters with at least large_n particles (default 4096)
     run on the chunked path spread over the whole GPU; the results do not depend on the path.
-    particle_filter: BootstrapFilter (default) or OptimalNonLinearGaussianParticleFilter."""
+    particle_filter: BootstrapFilter (default) or OptimalNonLinearGaussianParticleFilter.
+    schedule: None (large_n as given, stream launches), a dict {"large_n": int, "graph": bool}, or
+    "auto": the schedule autotuner (mocat_amd.ssm.schedule) times the candidate schedules on this
+    batch the first time its shape is seen, records the fastest, and every later call of the same
+    shape uses it."""
     if _filter_kind(particle_filter) is EnsembleKalmanFilter:
         raise NotImplementedError("EnsembleKalmanFilter runs one ensemble per call: run_particle_filter_for_marginals")
     ys = [y.detach().cpu().numpy() if isinstance(y, torch.Tensor) else np.asarray(y) for y in ys]
     if resampling == "categorical" and any(int(v) * int(v) >= 0xFFFFFFFF for v in ns):
         raise ValueError("categorical resampling draws an (n, n) Gumbel matrix: n * n must be < 2^32 - 1")
-    res = _launch(list(ssm_scenarios), ys, list(random_keys), [int(v) for v in ns], ess_threshold, resampling, store,
-                  grad, large_n, particle_filter)
+    models, keys, ns = list(ssm_scenarios), list(random_keys), [int(v) for v in ns]
+    graph = False
+    if schedule is not None:
+        from . import schedule as sched
+        if schedule == "auto":
+            run = lambda c: _launch(models, ys, keys, ns, ess_threshold, resampling, store, grad, c["large_n"],
+                                    particle_filter, graph=c["graph"])
+            choice = sched.choose(sched.signature(models, ys, ns, resampling, grad, store, particle_filter), run)
+        elif isinstance(schedule, dict):
+            choice = schedule
+        else:
+            raise ValueError('schedule must be None, "auto" or a dict {"large_n": int, "graph": bool}')
+        large_n, graph = int(choice["large_n"]), bool(choice["graph"])
+    res = _launch(models, ys, keys, ns, ess_threshold, resampling, store, grad, large_n, particle_filter, graph=graph)
     return [cdict(**{k: v for k, v in r.items() if k != "desc"}) for r in res]
 
 
