@@ -166,6 +166,21 @@ int mocat_pf_run_large(int model, int d, int dy, const mocat_pf_filter* host_fil
                        double* out_grad, void* workspace, int64_t workspace_bytes, int launch_mode,
                        hipStream_t stream);
 
+/* The batched per-step hooks of mocat's particle filters for the compiled models (model, d, dy and
+ * params as mocat_pf_run), one call per step over n particles, with the fused filters' noise draws:
+ * phase 0 = ParticleFilter.initial_sample_and_weight_vectorised (mocat/src/ssm/filtering.py:65-78):
+ *   x[i] = initial sample from keys[i], log_weight[i] = -likelihood(x[i], y) (y may be NULL: zeros);
+ *   model 2 (optimal proposal) conditions on y (required) from row i of normal(keys[0], (n, d)) with
+ *   log_weight 0 (mocat/src/ssm/nonlinear_gaussian.py:198-224);
+ * phase 1 = propose_and_intermediate_weight_vectorised (filtering.py:124-139, BootstrapFilter
+ *   :142-170; optimal proposal nonlinear_gaussian.py:264-281): x[i] from x_prev[i] and keys[i] (model
+ *   2: keys[0]), log_weight[i] = the intermediate log-weight at y;
+ * phase 2 = StateSpaceModel.likelihood_potential (mocat/src/ssm/ssm.py:73-81) of x_prev[i] at y into
+ *   log_weight[i] (models 0 and 1; x and keys unused).
+ * x_prev, x [n][d], y [dy], keys [n][2] (model 2: keys[0] only), log_weight [n]. */
+int mocat_pf_hook(int model, int d, int dy, int phase, const float* params, const float* x_prev, const float* y,
+                  const uint32_t* keys, int64_t n, float* x, float* log_weight, hipStream_t stream);
+
 /* Ancestor indices idx[n] from non-negative weights (from_log = 0) or log-weights (from_log = 1,
  * weights exp(w - max w)): method 0 systematic (u = uniform(key)), 1 multinomial
  * (u = uniform(key, (n,))); exact fixed-point cumulative weights. key is a device uint32[2];

@@ -1655,6 +1655,85 @@ __global__ __launch_bounds__(LT) void lf_finish_kernel(LfArgs a) {
   }
 }
 
+// ---------------------------------------------------------------- the batched per-step hooks
+// mocat's ParticleFilter.initial_sample_and_weight_vectorised / propose_and_intermediate_weight_vectorised
+// (mocat/src/ssm/filtering.py:65-78, 124-139; BootstrapFilter :142-170; the optimal filter's overrides,
+// mocat/src/ssm/nonlinear_gaussian.py:198-224, 264-281) and StateSpaceModel.likelihood_potential
+// (mocat/src/ssm/ssm.py:73-81) for the compiled models, one thread per particle with the same model code
+// and noise draws as the fused filters: particle i's noise from keys[i] (bootstrap) or row i of
+// normal(keys[0], (n, D)) (optimal proposal).
+//   phase 0 (initial): x[i] = initial sample; lw[i] = -likelihood(x[i], y) (0 without y; 0 for the
+//                      optimal filter, which conditions its sample on y)
+//   phase 1 (propose): x[i] = proposal from xprev[i]; lw[i] = intermediate log-weight (bootstrap:
+//                      -likelihood(x[i], y); optimal: the predictive weight)
+//   phase 2 (likelihood): lw[i] = likelihood potential of xprev[i] at y (x unused)
+template <class Model, int D, int DY>
+__global__ __launch_bounds__(256) void pf_hook_kernel(int phase, const float* __restrict__ params,
+                                                      const float* __restrict__ xprev, const float* __restrict__ y,
+                                                      const uint32_t* __restrict__ keys, int64_t n, float* xout,
+                                                      float* lw) {
+  __shared__ Model model;
+  load_shared_params(model, params);
+  __syncthreads();
+  constexpr uint32_t hD = (D + 1) / 2;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    float eps[D], xp[D], x[D];
+    if (phase == 2) {
+#pragma unroll
+      for (int q = 0; q < D; ++q) xp[q] = xprev[i * D + q];
+      if constexpr (Model::kOptimal) {
+        lw[i] = 0.f;  // not used: the optimal filter's likelihood hook is the bootstrap model's
+      } else {
+        lw[i] = model.lik(xp, y);
+      }
+      continue;
+    }
+    if constexpr (Model::kOptimal) {
+      shared_eps<D>(Key{keys[0], keys[1]}, (uint32_t)(n * D), (uint32_t)i, eps);
+    } else {
+      const Key ki{keys[2 * i], keys[2 * i + 1]};
+#pragma unroll
+      for (uint32_t q = 0; q < hD; ++q) {
+        uint32_t w0, w1;
+        bits_pair(ki, D, q, w0, w1);
+        eps[q] = normal_from_bits(w0);
+        if (q + hD < (uint32_t)D) eps[q + hD] = normal_from_bits(w1);
+      }
+    }
+    float l = 0.f;
+    if (phase == 0) {
+      if constexpr (Model::kOptimal) {
+        model.init_opt(eps, y, x);
+      } else {
+        model.init(eps, x);
+        if (y) l = -model.lik(x, y);
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < D; ++q) xp[q] = xprev[i * D + q];
+      if constexpr (Model::kOptimal) {
+        l = -model.step_opt(xp, eps, y, x);
+      } else {
+        model.transition(xp, eps, x);
+        l = -model.lik(x, y);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < D; ++q) xout[i * D + q] = x[q];
+    lw[i] = l;
+  }
+}
+
+template <class Model, int D, int DY>
+int launch_pf_hook(int phase, const float* params, const float* xprev, const float* y, const uint32_t* keys,
+                   int64_t n, float* x, float* lw, hipStream_t st) {
+  const int64_t blocks = (n + 255) / 256;
+  hipLaunchKernelGGL((pf_hook_kernel<Model, D, DY>), dim3((unsigned)(blocks < 8192 ? blocks : 8192)), dim3(256), 0,
+                     st, phase, params, xprev, y, keys, n, x, lw);
+  MOCAT_LAUNCH_CHECK();
+  return MOCAT_OK;
+}
+
 // Capture stream of the graph launch mode, one per device (capture needs a stream other than the
 // null stream; the captured graph is then launched on the caller's stream).
 hipStream_t capture_stream() {
@@ -1945,6 +2024,34 @@ int mocat_pf_run_large(int model, int d, int dy, const mocat_pf_filter* host_fil
   if (model != 0) return MOCAT_ERR_ARG;
 #define MOCAT_PF_CASE(DD, YY) \
   if (d == DD && dy == YY) return launch_pf_large<Lgssm<DD, YY>, DD, YY>(a, L.max_T, L.cend_slots, launch_mode, stream);
+  MOCAT_PF_LG_SHAPES
+#undef MOCAT_PF_CASE
+  return MOCAT_ERR_ARG;
+}
+
+int mocat_pf_hook(int model, int d, int dy, int phase, const float* params, const float* x_prev, const float* y,
+                  const uint32_t* keys, int64_t n, float* x, float* log_weight, hipStream_t stream) {
+  if (!params || !log_weight || n < 0 || phase < 0 || phase > 2) return MOCAT_ERR_ARG;
+  if (n == 0) return MOCAT_OK;
+  if (phase != 2 && (!keys || !x)) return MOCAT_ERR_ARG;
+  if (phase != 0 && !x_prev) return MOCAT_ERR_ARG;
+  if (phase != 0 && !y) return MOCAT_ERR_ARG;
+  if (model == 2 && (phase == 2 || !y)) return MOCAT_ERR_ARG;
+  if (n > 0x7FFFFFFFll || (model == 2 && n * (int64_t)d > 0xFFFFFFFFll)) return MOCAT_ERR_SIZE;
+  if (model == 1) {
+    if (d != 1 || dy != 1) return MOCAT_ERR_ARG;
+    return launch_pf_hook<Sv, 1, 1>(phase, params, x_prev, y, keys, n, x, log_weight, stream);
+  }
+  if (model == 2) {
+#define MOCAT_PF_CASE(DD, YY) \
+  if (d == DD && dy == YY) return launch_pf_hook<LgssmOpt<DD, YY>, DD, YY>(phase, params, x_prev, y, keys, n, x, log_weight, stream);
+    MOCAT_PF_LG_SHAPES
+#undef MOCAT_PF_CASE
+    return MOCAT_ERR_ARG;
+  }
+  if (model != 0) return MOCAT_ERR_ARG;
+#define MOCAT_PF_CASE(DD, YY) \
+  if (d == DD && dy == YY) return launch_pf_hook<Lgssm<DD, YY>, DD, YY>(phase, params, x_prev, y, keys, n, x, log_weight, stream);
   MOCAT_PF_LG_SHAPES
 #undef MOCAT_PF_CASE
   return MOCAT_ERR_ARG;

@@ -31,6 +31,7 @@ _lib.register("mocat_pf_large_workspace_bytes", [_P, _lib.c_i32, _P])
 _lib.register("mocat_pf_run_large", [_lib.c_i32, _lib.c_i32, _lib.c_i32, _P, _lib.c_i32, _P, _P, _P, _P, _P, _P, _P,
                                      _lib.c_i32, _lib.c_f32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _lib.c_i64,
                                      _lib.c_i32, _P])
+_lib.register("mocat_pf_hook", [_lib.c_i32, _lib.c_i32, _lib.c_i32, _lib.c_i32, _P, _P, _P, _P, _lib.c_i64, _P, _P, _P])
 _lib.register("mocat_resample", [_P, _lib.c_i64, _lib.c_i32, _lib.c_i32, _P, _P, _P, _P])
 _lib.register("mocat_resample_noalloc", [_P, _lib.c_i64, _lib.c_i32, _lib.c_i32, _P, _P, _P])
 _lib.register("mocat_resample_categorical", [_P, _lib.c_i64, _lib.c_i64, _P, _P, _P])
@@ -49,7 +50,67 @@ PF_INIT_KEY = 8  # flags bit: initial particles from split(key, n) (initiate_par
 assert FILTER_DTYPE.itemsize == 64
 
 
+HOOK_ERROR = ("{what}: the particle-filter hooks run on compiled models only (TimeHomogenousLinearGaussian and "
+              "StochasticVolatility with unit time steps under BootstrapFilter; TimeHomogenousLinearGaussian under "
+              "OptimalNonLinearGaussianParticleFilter). A StateSpaceModel or ParticleFilter written in Python has "
+              "no kernel on the fused path.")
+
+
+def _hook_target(particle_filter, ssm_scenario):
+    """(kernel model id, parameter vector) of the batched hooks, or NotImplementedError (HOOK_ERROR)."""
+    kind = BootstrapFilter if particle_filter is None else None
+    if kind is None:
+        try:
+            kind = _filter_kind(particle_filter)
+        except NotImplementedError:
+            raise NotImplementedError(HOOK_ERROR.format(what=type(particle_filter).__name__)) from None
+    if kind is EnsembleKalmanFilter:
+        raise NotImplementedError("EnsembleKalmanFilter: its analysis step needs the whole ensemble's covariance; "
+                                  "run it with run_particle_filter_for_marginals")
+    pf = particle_filter if isinstance(particle_filter, ParticleFilter) else kind()
+    # a subclass that overrides a per-particle rule would not be honoured by the compiled hook
+    for name in ("proposal_sample", "intermediate_log_weight", "initial_sample", "initial_log_weight"):
+        if getattr(type(pf), name, None) is not getattr(kind, name, None):
+            raise NotImplementedError(HOOK_ERROR.format(what=f"{type(pf).__name__}.{name}"))
+    if kind is OptimalNonLinearGaussianParticleFilter:
+        try:
+            return 2, pf.kernel_params(ssm_scenario)
+        except NotImplementedError:
+            raise NotImplementedError(HOOK_ERROR.format(what=type(ssm_scenario).__name__)) from None
+    km = getattr(ssm_scenario, "kernel_model", None)
+    if km not in (0, 1) or not hasattr(ssm_scenario, "kernel_params"):
+        raise NotImplementedError(HOOK_ERROR.format(what=type(ssm_scenario).__name__))
+    return km, ssm_scenario.kernel_params()
+
+
+def _unit_step(ssm_scenario, t_previous, t_new):
+    if getattr(ssm_scenario, "kernel_model", None) == 1 and t_previous is not None and t_new is not None:
+        if abs(float(t_new) - float(t_previous) - 1.0) > 1e-6:
+            raise NotImplementedError("the stochastic-volatility kernels assume unit time steps")
+
+
+def _hook(model, params, ssm_scenario, phase, x_prev, y, keys, n):
+    d, dy = int(ssm_scenario.dim), int(ssm_scenario.dim_obs)
+    dev = mr.default_device()
+    p = torch.from_numpy(np.ascontiguousarray(params, np.float32)).to(dev)
+    yt = None if y is None else torch.as_tensor(y, dtype=torch.float32, device=dev).reshape(-1).contiguous()
+    if yt is not None and yt.numel() != dy:
+        raise ValueError(f"y must hold dim_obs = {dy} values")
+    xp = None if x_prev is None else torch.as_tensor(x_prev, dtype=torch.float32, device=dev).reshape(n, d).contiguous()
+    kt = None if keys is None else mr.as_key(keys).reshape(-1, 2).contiguous()
+    if kt is not None and model != 2 and kt.shape[0] != n:
+        raise ValueError(f"expected {n} keys, got {kt.shape[0]}")
+    x = torch.empty((n, d), dtype=torch.float32, device=dev) if phase != 2 else None
+    lw = torch.empty(n, dtype=torch.float32, device=dev)
+    _lib.call("mocat_pf_hook", model, d, dy, phase, _lib.ptr(p), _lib.ptr(xp), _lib.ptr(yt), _lib.ptr(kt), int(n),
+              _lib.ptr(x), _lib.ptr(lw), _lib.stream_handle())
+    return x, lw
+
+
 class ParticleFilter:
+    """mocat's ParticleFilter (mocat/src/ssm/filtering.py:20-139). The batched hooks below are the seams
+    mocat's scan calls; here they run the compiled models' kernels (one thread per particle, the same
+    noise draws as the fused filters), and raise HOOK_ERROR's NotImplementedError for anything else."""
     name = "Particle Filter"
 
     def __init__(self, name: str = None):
@@ -59,10 +120,54 @@ class ParticleFilter:
     def __repr__(self):
         return f"mocat_amd.ParticleFilter.{self.__class__.__name__}"
 
+    def startup(self, ssm_scenario):
+        pass
+
+    def initial_sample_vectorised(self, ssm_scenario, y, t, random_keys):
+        """filtering.py:56-63: one initial sample per key (random_keys (n, 2))."""
+        return self.initial_sample_and_weight_vectorised(ssm_scenario, y, t, random_keys)[0]
+
+    def initial_sample_and_weight_vectorised(self, ssm_scenario, y, t, random_keys):
+        """filtering.py:65-78: (x (n, d), log_weight (n,)); log-weights are -likelihood(x, y), zeros
+        without y."""
+        model, params = _hook_target(self, ssm_scenario)
+        n = int(mr.as_key(random_keys).reshape(-1, 2).shape[0])
+        return _hook(model, params, ssm_scenario, 0, None, y, random_keys, n)
+
+    def proposal_sample_vectorised(self, ssm_scenario, x_previous, t_previous, y_new, t_new, random_keys):
+        """filtering.py:113-122."""
+        return self.propose_and_intermediate_weight_vectorised(ssm_scenario, x_previous, t_previous, y_new, t_new,
+                                                               random_keys)[0]
+
+    def propose_and_intermediate_weight_vectorised(self, ssm_scenario, x_previous, t_previous, y_new, t_new,
+                                                   random_keys):
+        """filtering.py:124-139: (x_new (n, d), intermediate log-weights (n,)) from x_previous (n, d) and
+        one key per particle."""
+        model, params = _hook_target(self, ssm_scenario)
+        _unit_step(ssm_scenario, t_previous, t_new)
+        xp = torch.as_tensor(x_previous)
+        n = int(xp.shape[0]) if xp.dim() > 1 else 1
+        return _hook(model, params, ssm_scenario, 1, xp, y_new, random_keys, n)
+
 
 class BootstrapFilter(ParticleFilter):
     """Proposal = transition, weight = likelihood (mocat/src/ssm/filtering.py:142-170)."""
     name = "Bootstrap Filter"
+
+    def initial_sample(self, ssm_scenario, y, t, random_key):
+        """filtering.py:40-45 for one key."""
+        return ssm_scenario.initial_sample(t, random_key)
+
+    def initial_log_weight(self, ssm_scenario, x, y, t):
+        return -ssm_scenario.likelihood_potential(x, y, t)
+
+    def proposal_sample(self, ssm_scenario, x_previous, t_previous, y_new, t_new, random_key):
+        """filtering.py:154-161: the transition sample."""
+        return ssm_scenario.transition_sample(x_previous, t_previous, t_new, random_key)
+
+    def intermediate_log_weight(self, ssm_scenario, x_previous, t_previous, x_new, y_new, t_new):
+        """filtering.py:163-170: -likelihood_potential(x_new, y_new)."""
+        return -ssm_scenario.likelihood_potential(x_new, y_new, t_new)
 
 
 def _sym_chol(M):

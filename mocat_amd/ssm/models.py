@@ -33,6 +33,46 @@ class StateSpaceModel:
     def kernel_params(self) -> np.ndarray:
         raise NotImplementedError
 
+    # -- per-particle model functions (mocat/src/ssm/ssm.py:47-81) on the filters' hook kernel; batched
+    # over leading key / particle axes. A model written in Python has no kernel: HOOK_ERROR.
+    def _hook_params(self):
+        from .filtering import HOOK_ERROR
+        if self.kernel_model not in (0, 1):
+            raise NotImplementedError(HOOK_ERROR.format(what=type(self).__name__))
+        return self.kernel_model, self.kernel_params()
+
+    def initial_sample(self, t, random_key):
+        """ssm.py:47-51: x_0 from one key ((2,) -> (d,)) or many ((..., 2) -> (..., d))."""
+        from .filtering import _hook
+        key = mr.as_key(random_key)
+        lead = tuple(key.shape[:-1])
+        n = int(np.prod(lead)) if lead else 1
+        model, params = self._hook_params()
+        x, _ = _hook(model, params, self, 0, None, None, key, n)
+        return x.reshape(lead + (self.dim,))
+
+    def transition_sample(self, x_previous, t_previous, t_new, random_key):
+        """ssm.py:53-60: x_new given x_previous, one key per particle."""
+        from .filtering import _hook, _unit_step
+        key = mr.as_key(random_key)
+        lead = tuple(key.shape[:-1])
+        n = int(np.prod(lead)) if lead else 1
+        model, params = self._hook_params()
+        _unit_step(self, t_previous, t_new)
+        y0 = np.zeros(self.dim_obs, np.float32)  # the weight the hook also returns is not used here
+        x, _ = _hook(model, params, self, 1, x_previous, y0, key, n)
+        return x.reshape(lead + (self.dim,))
+
+    def likelihood_potential(self, x, y, t=None):
+        """ssm.py:73-81: -log p(y | x) per particle (x (..., d) -> (...))."""
+        from .filtering import _hook
+        xt = torch.as_tensor(x, dtype=torch.float32)
+        lead = tuple(xt.shape[:-1])
+        n = int(np.prod(lead)) if lead else 1
+        model, params = self._hook_params()
+        _, lw = _hook(model, params, self, 2, xt, y, None, n)
+        return lw.reshape(lead)
+
     def simulate(self, t_all, random_key):
         """StateSpaceModel.simulate (mocat/src/ssm/ssm.py:138-163): keys split(key, 2T); latent path
         from the first T keys, observations from the last T. Returns cdict(x, y, t)."""

@@ -59,30 +59,50 @@ def backward_simulation_full(pot, vals, log_weight, key, n_samps):
 
 def backward_step(pot, vals_t, lw_t, x_next, key_t):
     """Indices of one backward step for the trajectories x_next (M, d) with key_t = t_keys[t]."""
+    return backward_step_margin(pot, vals_t, lw_t, x_next, key_t)[0]
+
+
+def backward_step_margin(pot, vals_t, lw_t, x_next, key_t):
+    """backward_step plus each draw's margin between its best and second-best score (FP64)."""
     n = len(lw_t)
     keys = rng.split(np.asarray(key_t, np.uint32), len(x_next))
     out = np.zeros(len(x_next), np.int64)
+    margin = np.zeros(len(x_next))
     for m in range(len(x_next)):
         logits = np.asarray(lw_t, np.float64) - pot(np.asarray(vals_t, np.float64), np.asarray(x_next[m], np.float64))
-        out[m] = int(np.argmax(rng.gumbel(keys[m], (n,)) + logits))
-    return out
+        i, mg = rng.argmax_margin((rng.gumbel(keys[m], (n,)) + logits)[None, :])
+        out[m], margin[m] = int(i[0]), float(mg[0])
+    return out, margin
 
 
-def rejection_step(pot, vals_t, lw_t, x_next, key_t, maximum_rejections, init_bound, inflation):
+def rejection_step(pot, vals_t, lw_t, x_next, key_t, maximum_rejections, init_bound, inflation, tie_tol=None):
     """rejection_resampling (backward.py:130-194) for one backward step: returns (indices (M,),
     transition-density evaluations). Densities, the bound and the acceptance tests in float32 as
-    mocat computes them; categorical draws by Gumbel-max in FP64."""
+    mocat computes them; categorical draws by Gumbel-max in FP64. With tie_tol, also returns a flag per
+    trajectory: some draw of it had a score margin <= tie_tol, or some acceptance test a uniform within
+    tie_tol (relative) of its threshold, so single-precision arithmetic may decide it differently."""
+    res = _rejection_step(pot, vals_t, lw_t, x_next, key_t, maximum_rejections, init_bound, inflation,
+                          1e-300 if tie_tol is None else tie_tol)
+    return res[:2] if tie_tol is None else res
+
+
+def _rejection_step(pot, vals_t, lw_t, x_next, key_t, maximum_rejections, init_bound, inflation, tie_tol):
     F = np.float32
     vals_t = np.asarray(vals_t, np.float64)
     lw_t = np.asarray(lw_t, np.float64)
     M, n = len(x_next), len(lw_t)
     k0, k1, k2 = rng.split(np.asarray(key_t, np.uint32), 3)
     dens_of = lambda i, m: F(np.exp(-pot(vals_t[[i]], np.asarray(x_next[m], np.float64))[0]))
-    idx = rng.categorical(k0, lw_t, M).astype(np.int64)
+    idx, mg = rng.categorical_margin(k0, lw_t, M)
+    idx = idx.astype(np.int64)
+    tie = mg <= tie_tol
     dens = np.array([dens_of(idx[m], m) for m in range(M)], F)
     mx = dens.max()
     bound = F(mx * F(inflation)) if mx > F(init_bound) else F(init_bound)
-    wait = rng.uniform(k1, (M,)) > (dens / bound).astype(F)
+    u0 = rng.uniform(k1, (M,))
+    thr0 = (dens / bound).astype(F)
+    tie |= np.abs(u0.astype(np.float64) - thr0) <= tie_tol * np.maximum(thr0, 1e-30)
+    wait = u0 > thr0
     keys = rng.split(k2, M)
     evals = M
     it = 1
@@ -94,9 +114,14 @@ def rejection_step(pot, vals_t, lw_t, x_next, key_t, maximum_rejections, init_bo
             if not entered[m]:
                 continue
             kr, kc, ku = rng.split(keys[m], 3)
-            idx[m] = int(np.argmax(rng.gumbel(kc, (n,)) + lw_t))
+            i, mgi = rng.argmax_margin((rng.gumbel(kc, (n,)) + lw_t)[None, :])
+            idx[m] = int(i[0])
+            tie[m] |= mgi[0] <= tie_tol
             d_round[m] = dens_of(idx[m], m)
-            new_wait[m] = rng.uniform(ku, ()) > F(d_round[m] / bound)
+            u = rng.uniform(ku, ())
+            thr = F(d_round[m] / bound)
+            tie[m] |= abs(float(u) - float(thr)) <= tie_tol * max(float(thr), 1e-30)
+            new_wait[m] = u > thr
             keys[m] = kr
         mx = d_round.max()
         if mx > bound:
@@ -108,6 +133,8 @@ def rejection_step(pot, vals_t, lw_t, x_next, key_t, maximum_rejections, init_bo
     for m in range(M):
         if wait[m]:
             logits = lw_t - pot(vals_t, np.asarray(x_next[m], np.float64))
-            idx[m] = int(np.argmax(rng.gumbel(keys[m], (n,)) + logits))
+            i, mgi = rng.argmax_margin((rng.gumbel(keys[m], (n,)) + logits)[None, :])
+            idx[m] = int(i[0])
+            tie[m] |= mgi[0] <= tie_tol
     evals += n * int(wait.sum())
-    return idx, evals
+    return idx, evals, tie

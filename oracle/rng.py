@@ -171,9 +171,25 @@ def gumbel(key, shape):
 
 def categorical(key, logits, n):
     """random.categorical(key, logits, shape=(n,)) by brute force (Gumbel-max)."""
+    return categorical_margin(key, logits, n)[0]
+
+
+def argmax_margin(scores):
+    """Row-wise (argmax, top1 - top2) of an (m, n) FP64 score array: a row whose margin is within the
+    f32 rounding of the scores can have a different first maximum in single precision."""
+    scores = np.asarray(scores, np.float64)
+    idx = np.argmax(scores, axis=-1)
+    if scores.shape[-1] < 2:
+        return idx.astype(np.int32), np.full(scores.shape[:-1], np.inf)
+    top2 = np.partition(scores, scores.shape[-1] - 2, axis=-1)[..., -2:]
+    return idx.astype(np.int32), top2[..., 1] - top2[..., 0]
+
+
+def categorical_margin(key, logits, n):
+    """categorical plus each draw's margin between the best and second-best Gumbel score (FP64)."""
     logits = np.asarray(logits, np.float64)
     g = gumbel(key, (n, logits.shape[-1]))
-    return np.argmax(g + logits[None, :], axis=-1).astype(np.int32)
+    return argmax_margin(g + logits[None, :])
 
 
 def bits_batch(keys, size):

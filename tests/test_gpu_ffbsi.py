@@ -23,19 +23,30 @@ def _lgssm(m, d, seed):
     return model, A, Q, H, R
 
 
+# a draw whose best two scores (Gumbel + log-weight - transition potential) are within TIE of each other
+# may be decided differently by the GPU's single-precision scores: every other draw must match exactly
+TIE = 1e-4
+
+
 def _check_steps(m, res, marg, pot, key, n_samps):
-    """Every step's indices against the oracle given the GPU's trajectories at the next time."""
+    """Every step's indices against the oracle given the GPU's trajectories at the next time: exact
+    wherever the draw is not a near-tie; returns the fraction of near-ties."""
     vals = marg.value.cpu().numpy().astype(np.float64)
     lw = marg.log_weight.cpu().numpy().astype(np.float64)
     out = res.value.cpu().numpy()
     anc = res.ancestors.cpu().numpy()
     T = vals.shape[0]
     tk = orng.split(key.cpu().numpy(), T)
-    agree = [np.mean(anc[T - 1] == orng.categorical(tk[T - 1], lw[T - 1], n_samps))]
-    for t in range(T - 2, -1, -1):
-        agree.append(np.mean(anc[t] == of.backward_step(pot, vals[t], lw[t], out[t + 1], tk[t])))
+    want, margin = orng.categorical_margin(tk[T - 1], lw[T - 1], n_samps)
+    ties = []
+    for t in range(T - 1, -1, -1):
+        if t < T - 1:
+            want, margin = of.backward_step_margin(pot, vals[t], lw[t], out[t + 1], tk[t])
+        decided = margin > TIE
+        np.testing.assert_array_equal(anc[t][decided], want[decided])
+        ties.append((~decided).mean())
     np.testing.assert_array_equal(out, np.take_along_axis(vals, anc[:, :, None].astype(np.int64), 1).astype(np.float32))
-    return np.array(agree)
+    return np.array(ties)
 
 
 @pytest.mark.parametrize("d,n,n_samps", [(1, 1, 3), (2, 300, 200), (4, 257, 64)])
@@ -48,8 +59,8 @@ def test_lgssm_backward_steps_vs_oracle(m, d, n, n_samps):
     res = m.ssm.backward_simulation(model, marg, key, n_samps)
     assert res.value.shape == (8, n_samps, d) and not hasattr(res, "log_weight")
     np.testing.assert_array_equal(res.num_transition_evals.numpy(), np.r_[0, np.full(7, n * n_samps)])
-    agree = _check_steps(m, res, marg, of.lgssm_transition(A, Q), key, n_samps)
-    assert agree.min() >= 0.99, agree
+    ties = _check_steps(m, res, marg, of.lgssm_transition(A, Q), key, n_samps)
+    assert ties.max() <= 0.01, ties
 
 
 def test_sv_backward_steps_vs_oracle(m):
@@ -60,8 +71,8 @@ def test_sv_backward_steps_vs_oracle(m):
                                                    n=500, ess_threshold=1.0)
     key = m.random.PRNGKey(6)
     res = m.ssm.backward_simulation(model, marg, key, 300)
-    agree = _check_steps(m, res, marg, of.sv_transition(0.95, np.sqrt(0.3)), key, 300)
-    assert agree.min() >= 0.99, agree
+    ties = _check_steps(m, res, marg, of.sv_transition(0.95, np.sqrt(0.3)), key, 300)
+    assert ties.max() <= 0.01, ties
 
 
 def test_lgssm_trajectories_converge_to_rts_smoother(m):
@@ -91,14 +102,16 @@ def _rejection_agreement(m, model, pot, n, n_samps, T, max_rej, bound, infl, see
     lw = marg.log_weight.cpu().numpy().astype(np.float64)
     out, anc = res.value.cpu().numpy(), res.ancestors.cpu().numpy()
     tk = orng.split(key.cpu().numpy(), T)
-    agree, ev = [], []
+    ties, ev = [], []
     for t in range(T - 2, -1, -1):
-        idx, evals = of.rejection_step(pot, vals[t], lw[t], out[t + 1], tk[t], max_rej, bound, infl)
-        agree.append(np.mean(anc[t] == idx))
+        idx, evals, tie = of.rejection_step(pot, vals[t], lw[t], out[t + 1], tk[t], max_rej, bound, infl, TIE)
+        # exact for every trajectory none of whose draws or acceptance tests was a near-tie
+        np.testing.assert_array_equal(anc[t][~tie], idx[~tie])
+        ties.append(tie.mean())
         ev.append((res.num_transition_evals[t + 1].item(), evals))
     assert res.num_transition_evals[0].item() == 0
     np.testing.assert_array_equal(out, np.take_along_axis(vals, anc[:, :, None].astype(np.int64), 1).astype(np.float32))
-    return np.array(agree), np.array(ev)
+    return np.array(ties), np.array(ev)
 
 
 @pytest.mark.parametrize("max_rej,bound", [(1, 0.0), (10, 0.0), (40, 5.0)])
@@ -106,16 +119,16 @@ def test_rejection_backward_steps_vs_oracle(m, max_rej, bound):
     """Rejection rounds (backward.py:42-194): the same proposals, acceptances, bound raises and
     final full draws as the oracle on the same keys, every step; the evaluation counts follow."""
     model, A, Q, H, R = _lgssm(m, 2, 21)
-    agree, ev = _rejection_agreement(m, model, of.lgssm_transition(A, Q), 300, 200, 6, max_rej, bound, 1.01, 30)
-    assert agree.min() >= 0.98, agree
+    ties, ev = _rejection_agreement(m, model, of.lgssm_transition(A, Q), 300, 200, 6, max_rej, bound, 1.01, 30)
+    assert ties.max() <= 0.02, ties
     assert np.all(np.abs(ev[:, 0] - ev[:, 1]) <= 0.02 * ev[:, 1] + 300), ev
 
 
 def test_rejection_sv_vs_oracle(m):
     model = m.ssm.StochasticVolatility(transition_matrix=0.9, transition_covariance=0.4,
                                        likelihood_covariance_diag=0.7)
-    agree, ev = _rejection_agreement(m, model, of.sv_transition(0.9, np.sqrt(0.4)), 400, 256, 6, 12, 0.0, 1.05, 40)
-    assert agree.min() >= 0.98, agree
+    ties, ev = _rejection_agreement(m, model, of.sv_transition(0.9, np.sqrt(0.4)), 400, 256, 6, 12, 0.0, 1.05, 40)
+    assert ties.max() <= 0.02, ties
 
 
 def test_rejection_trajectories_converge_to_rts_smoother(m):

@@ -37,11 +37,20 @@ def test_resampling_edge_cases(m):
             np.testing.assert_array_equal(got, opf.resample_indices(w, orng.PRNGKey(9), method))
 
 
-def test_categorical_matches_gumbel_max(m):
-    lw = np.log(np.random.default_rng(0).uniform(0.01, 1, 64)).astype(np.float32)
-    got = m.ssm.categorical(m.random.PRNGKey(3), torch.tensor(lw, device="cuda")).cpu().numpy()
-    want = orng.categorical(orng.PRNGKey(3), lw, 64)
-    assert (got == want).mean() >= 0.95  # argmax ties within f32 log rounding may differ
+# Gumbel scores -log(-log u) + lw are formed in single precision on the GPU (a few ulp of |score| <= ~25)
+# and in FP64 by the oracle: a draw whose best two scores are closer than TIE can have another first
+# maximum in f32. Every other draw must agree exactly, and such near-ties must stay rare.
+TIE = 2e-5
+
+
+@pytest.mark.parametrize("n,n_out", [(64, 4096), (4096, 4096), (1000, 4099)])
+def test_categorical_matches_gumbel_max(m, n, n_out):
+    lw = np.log(np.random.default_rng(n).uniform(0.01, 1, n)).astype(np.float32)
+    got = m.ssm.categorical(m.random.PRNGKey(3), torch.tensor(lw, device="cuda"), n_out).cpu().numpy()
+    want, margin = orng.categorical_margin(orng.PRNGKey(3), lw, n_out)
+    decided = margin > TIE
+    np.testing.assert_array_equal(got[decided], want[decided])
+    assert (~decided).sum() <= max(2, n_out // 1000), (~decided).sum()
 
 
 def _lgssm_np(model):
@@ -433,3 +442,25 @@ def test_graph_launch_and_tuned_schedule_give_the_same_results(m, grad, tmp_path
                 if grad:
                     np.testing.assert_allclose(a.log_likelihood_grad.cpu().numpy(),
                                                b.log_likelihood_grad.cpu().numpy(), rtol=1e-8, atol=1e-8)
+
+
+@pytest.mark.parametrize("n,T", [(4096, 600), (5003, 520)])
+def test_chunked_sv_long_run_same_path_as_fp64_oracle(m, n, T):
+    # the chunked many-workgroup path (filters from 4096 particles) at config-5 lengths: the FP64 oracle
+    # replaying the GPU's ancestors gives the same log-likelihood to 1e-4 relative after T steps of FP64
+    # accumulation, and the same log-likelihood gradient (fixed-resampling-index estimator)
+    from mocat_amd import datasets
+    models, ys, _ = datasets.sv_batch_config(8, n_models=1, t_min=T, t_max=T)
+    key = m.random.PRNGKey(21)
+    r = m.ssm.run_particle_filters(models, ys, [key], [n], ess_threshold=1.0, resampling="systematic", store=True,
+                                   grad=True, large_n=4096)[0]
+    md = models[0]
+    p = opf.sv_params(md.phi, md.sigma, md.beta)
+    flags = np.ones(T, bool)
+    flags[0] = False
+    anc = r.ancestors.cpu().numpy()
+    o = opf.run_pf(p, ys[0], key.cpu().numpy(), n, 1.0, "systematic", ancestors=anc, resample_flags=flags)
+    ll = r.log_likelihood.item()
+    assert abs(ll - o["log_likelihood"]) <= 1e-4 * abs(o["log_likelihood"]), (ll, o["log_likelihood"])
+    g = opf.sv_loglik_grad(md.phi, md.sigma, md.beta, ys[0], key.cpu().numpy(), n, anc, flags)
+    np.testing.assert_allclose(r.log_likelihood_grad.cpu().numpy(), g, rtol=2e-3, atol=2e-3 * np.abs(g).max())

@@ -117,3 +117,14 @@ def test_randint_is_the_64_bit_draw_modulo_the_span():
         want = np.array([(int(h) * (1 << 32) + int(l)) % span for h, l in zip(hi, lo)], np.int64)
         got = rng.randint(k, (2000,), -5, -5 + span).astype(np.int64) + 5
         np.testing.assert_array_equal(got, want)
+
+
+def test_argmax_margin_brute_force():
+    rs = np.random.default_rng(3)
+    s = rs.standard_normal((50, 7))
+    s[3, 2] = s[3, 5] = 9.0  # an exact tie: margin 0, first maximum
+    idx, mg = rng.argmax_margin(s)
+    for r in range(50):
+        order = np.sort(s[r])
+        assert idx[r] == int(np.argmax(s[r])) and mg[r] == order[-1] - order[-2]
+    assert mg[3] == 0.0 and idx[3] == 2
