@@ -879,10 +879,10 @@ __global__ __launch_bounds__(NT) void hmc_lr_kernel(HmcArgs h) {
 constexpr int kThreads = 512;
 
 // The same iteration as hmc_lr_kernel for the bf16x3 engine, with the tile state in
-// registers: thread t owns chain c = t % 32 and the EPT = DP / 8 columns k0 = EPT * (t / 32) .. of
-// position, momentum and gradient. A leapfrog step is then: update the owned elements and write their
-// bf16 hi/lo split into the engine's theta operand (barrier), the engine's block loop (barrier), and
-// the owned gradient elements summed from the four waves' partials. The refreshed momenta, position
+// registers: thread t owns chain c = t % 32 and the EPT = DP / (NT / 32) columns k0 = EPT * (t / 32) ..
+// of position, momentum and gradient. A leapfrog step is then: update the owned elements and write
+// their bf16 hi/lo split into the engine's theta operand (barrier), the engine's block loop (barrier),
+// and the owned gradient elements summed from every wave's partials. The refreshed momenta, position
 // and gradient of the pre-proposal state stay in global memory until a proposal is accepted (as in
 // hmc_lr_kernel); per-chain scalars and keys live in LDS.
 template <int DP, int NT>

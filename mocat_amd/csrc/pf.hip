@@ -58,6 +58,17 @@ constexpr int32_t kPfStepKey = 4;
 constexpr int32_t kPfInitKey = 8;
 constexpr double kLog2PiD = 1.83787706640934548356;
 
+// A compiler-only memory fence (no instruction). Placed at the top of a per-particle loop, it stops
+// the compiler from hoisting the model's LDS-resident matrices into registers for the whole loop
+// (at D = 8 that took more than the 256 VGPRs a 512-thread workgroup leaves and spilled to scratch);
+// each iteration re-reads them from LDS instead. At D >= 8 the models also fence between the rows
+// of each matrix product, so that no more than a few rows' loads are in flight at once.
+MOCAT_DEV void reload_model_each_particle() { asm volatile("" ::: "memory"); }
+template <int D>
+MOCAT_DEV void row_fence() {
+  if constexpr (D >= 8) asm volatile("" ::: "memory");
+}
+
 // ---------------------------------------------------------------- models
 // Linear Gaussian (mocat/src/ssm/linear_gaussian/linear_gaussian.py:46-51, 86-94, 118-128):
 // params: m0[D], L0[D*D], A[D*D], LQ[D*D], H[DY*D], SR[DY*DY], nlz
@@ -95,6 +106,7 @@ struct Lgssm {
       #pragma unroll
       for (int k = 0; k < D; ++k) a += L0[i * D + k] * eps[k];
       x[i] = a + m0[i];
+      row_fence<D>();
     }
   }
   MOCAT_DEV void transition(const float* xp, const float* eps, float* x) const {
@@ -107,6 +119,7 @@ struct Lgssm {
         b += LQ[i * D + k] * eps[k];
       }
       x[i] = a + b;
+      row_fence<D>();
     }
   }
   MOCAT_DEV float lik(const float* x, const float* y) const {
@@ -117,6 +130,7 @@ struct Lgssm {
       #pragma unroll
       for (int k = 0; k < D; ++k) hx += H[j * D + k] * x[k];
       diff[j] = y[j] - hx;
+      row_fence<D>();
     }
     float s = 0.f;
     #pragma unroll
@@ -125,6 +139,7 @@ struct Lgssm {
       #pragma unroll
       for (int j = 0; j < DY; ++j) v += diff[j] * SR[j * DY + k];
       s += v * v;
+      row_fence<D>();
     }
     return 0.5f * s + nlz;
   }
@@ -534,6 +549,7 @@ __global__ __launch_bounds__(NT) void pf_kernel(PfArgs a) {
     int bad0 = 0;
     const Key sub0 = Model::kOptimal ? split_at(sub, (uint32_t)n, 0u) : Key{0u, 0u};
     for (int64_t i = tid; i < n; i += NT) {
+      reload_model_each_particle();
       float eps[D], x[D];
       float l;
       if (resume) {
@@ -613,6 +629,7 @@ __global__ __launch_bounds__(NT) void pf_kernel(PfArgs a) {
       bad0 = 0;
       const Key ik0 = Model::kOptimal ? split_at(ik, (uint32_t)n, 0u) : Key{0u, 0u};
       for (int64_t i = tid; i < n; i += NT) {
+        reload_model_each_particle();
         const int64_t src = res ? (int64_t)anc[i] : i;
         float eps[D], xp[D], x[D];
         if constexpr (Model::kOptimal) {
