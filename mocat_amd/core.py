@@ -2,8 +2,10 @@
 
 `cdict` (mocat/src/core.py:20-84) is an attribute dictionary over torch device tensors:
 indexing slices every non-scalar tensor attribute, `+` concatenates along axis 0 (chains stacked
-over iterations), `save`/`load_cdict` persist it (tensors are written through torch.save with
-weights_only-compatible content).
+over iterations), `save`/`load_cdict` persist it in mocat's `.cdict` format (a pickle of a
+`mocat.src.core.cdict` whose arrays are numpy arrays, so mocat's `load_cdict` reads these files and
+this module reads mocat's; loading goes through an unpickler that resolves only cdict and numpy
+array classes).
 
 `Scenario` (mocat/src/core.py:124-261) keeps mocat's contract: prior_potential,
 likelihood_potential, the tempered potential `prior + temperature * likelihood`, and the derived
@@ -14,9 +16,12 @@ scenarios.GaussianMixture) provide batched implementations of those callables di
 from __future__ import annotations
 
 import copy
+import io
+import pickle
 from pathlib import Path
 from typing import Union
 
+import numpy as np
 import torch
 
 
@@ -100,8 +105,98 @@ def _from_host(obj, device):
     return obj
 
 
+# ---------------------------------------------------------------- mocat's .cdict format
+# mocat writes pickle.dump(cdict) (mocat/src/core.py:91-104); its jax arrays pickle as numpy arrays and
+# its load_cdict turns them back into jax arrays (:107-121). Files here hold the same objects: a
+# `mocat.src.core.cdict` / `static_cdict` (module path and class name as mocat's) with numpy arrays for
+# tensors, nested cdicts as such, and plain Python values.
+_MOCAT_MODULE = "mocat.src.core"
+_ZIP_MAGIC = b"PK\x03\x04"  # torch.save's zip container (files from earlier versions of this module)
+
+
+class _Global:
+    """Stands for the global `module.name` in a pickle (written by name, never called here)."""
+
+    def __init__(self, module, name):
+        self.module, self.name = module, name
+
+    def __call__(self, *args, **kwargs):
+        raise TypeError("a stand-in for a global in a pickle, not a callable")
+
+
+class _MocatPickler(pickle._Pickler):
+    """Pickles cdicts as mocat's classes and tensors as numpy arrays. Arrays name numpy's reconstructor
+    by its numpy 1.x path (`numpy.core.multiarray`), which numpy 1.x and 2.x both resolve."""
+
+    def reducer_override(self, obj):
+        if isinstance(obj, cdict):
+            cls = _Global(_MOCAT_MODULE, "static_cdict" if isinstance(obj, static_cdict) else "cdict")
+            return cls, (), dict(obj.__dict__)
+        if isinstance(obj, torch.Tensor):
+            t = obj.detach().cpu()
+            if t.dtype == torch.bfloat16:
+                raise TypeError("a bfloat16 tensor has no numpy counterpart in mocat's .cdict format")
+            func, args, *rest = t.numpy().__reduce_ex__(4)
+            if getattr(func, "__name__", "") != "_reconstruct":
+                raise TypeError(f"unexpected numpy reduction {func!r}")
+            return (_Global("numpy.core.multiarray", "_reconstruct"), args, *rest)
+        return NotImplemented
+
+    def save(self, obj, save_persistent_id=True):
+        if isinstance(obj, _Global):  # STACK_GLOBAL of its module and name
+            self.save(obj.module)
+            self.save(obj.name)
+            self.write(pickle.STACK_GLOBAL)
+            return
+        super().save(obj, save_persistent_id)
+
+
+# classes a .cdict file may name besides mocat's containers: what numpy arrays and scalars pickle as
+_np_multiarray = getattr(np, "_core", None)
+_np_multiarray = _np_multiarray.multiarray if _np_multiarray is not None else np.core.multiarray
+_NUMPY_GLOBALS = {
+    ("numpy._core.multiarray", "_reconstruct"): _np_multiarray._reconstruct,
+    ("numpy.core.multiarray", "_reconstruct"): _np_multiarray._reconstruct,
+    ("numpy._core.multiarray", "scalar"): _np_multiarray.scalar,
+    ("numpy.core.multiarray", "scalar"): _np_multiarray.scalar,
+    ("numpy", "ndarray"): np.ndarray,
+    ("numpy", "dtype"): np.dtype,
+    ("_codecs", "encode"): __import__("codecs").encode,
+}
+
+
+class _CdictUnpickler(pickle.Unpickler):
+    """Resolves mocat's cdict classes to this module's and numpy's array classes; refuses every other
+    global, so loading a file runs no code named by it."""
+
+    def find_class(self, module, name):
+        if module == _MOCAT_MODULE and name in ("cdict", "static_cdict"):
+            return static_cdict if name == "static_cdict" else cdict
+        if (module, name) in _NUMPY_GLOBALS:
+            return _NUMPY_GLOBALS[(module, name)]
+        raise pickle.UnpicklingError(f"a .cdict file may hold cdicts and numpy arrays only, not {module}.{name}")
+
+
+def _to_torch(obj, device):
+    if isinstance(obj, np.ndarray):
+        t = torch.from_numpy(np.ascontiguousarray(obj))
+        return t.to(device) if device is not None else t
+    if isinstance(obj, np.generic):
+        return obj.item()
+    if isinstance(obj, cdict):
+        for k, v in list(obj.__dict__.items()):
+            obj.__dict__[k] = _to_torch(v, device)
+        return obj
+    if isinstance(obj, (list, tuple)):
+        return type(obj)(_to_torch(v, device) for v in obj)
+    if isinstance(obj, dict):
+        return {k: _to_torch(v, device) for k, v in obj.items()}
+    return obj
+
+
 def save_cdict(in_cdict: cdict, path: Union[str, Path], overwrite: bool = False):
-    """Persist a cdict (mocat/src/core.py:91-104) to a `.cdict` file (torch.save of host tensors)."""
+    """Persist a cdict in mocat's .cdict format (mocat/src/core.py:91-104): the same suffix, overwrite and
+    error rules, and a pickle that mocat's load_cdict reads (tensors are written as host numpy arrays)."""
     path = Path(path)
     if path.suffix != ".cdict":
         path = path.with_suffix(".cdict")
@@ -111,18 +206,26 @@ def save_cdict(in_cdict: cdict, path: Union[str, Path], overwrite: bool = False)
             path.unlink()
         else:
             raise RuntimeError(f"File {path} already exists.")
-    torch.save(_to_host(in_cdict), path)
+    buf = io.BytesIO()
+    _MocatPickler(buf, protocol=4).dump(in_cdict)
+    path.write_bytes(buf.getvalue())
 
 
 def load_cdict(path: Union[str, Path], device=None) -> cdict:
-    """Load a `.cdict` written by save_cdict (mocat/src/core.py:107-121); tensors go to `device`."""
+    """Load a .cdict file written by save_cdict or by mocat (mocat/src/core.py:107-121); numpy arrays
+    become torch tensors on `device` (host tensors if None)."""
     path = Path(path)
     if not path.is_file():
         raise ValueError(f"Not a file: {path}")
     if path.suffix != ".cdict":
         raise ValueError(f"Not a .cdict file: {path}")
-    data = torch.load(path, weights_only=True)
-    return _from_host(data, device)
+    raw = path.read_bytes()
+    if raw[:4] == _ZIP_MAGIC:
+        return _from_host(torch.load(io.BytesIO(raw), weights_only=True), device)
+    data = _CdictUnpickler(io.BytesIO(raw)).load()
+    if not isinstance(data, cdict):
+        raise ValueError(f"{path} does not hold a cdict")
+    return _to_torch(data, device)
 
 
 def impl_checkable(func):

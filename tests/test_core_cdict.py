@@ -1,5 +1,10 @@
 """cdict container, its .cdict persistence and the Sampler base (mocat/src/core.py:20-121,
 mocat/src/sample.py), after mocat's own tests (mocat/src/tests/test_core.py:18-105). CPU only."""
+import pickle
+import sys
+import types
+
+import numpy as np
 import pytest
 import torch
 
@@ -57,8 +62,83 @@ def test_save_load_round_trip(tmp_path):
     assert d.scalar == 2.5 and d.name == "chain" and d.extra.iter == 7
     assert torch.equal(d.extra.random_key, c.extra.random_key)
     assert d.summary.sampler == "HMC" and d.summary.parameters.stepsize == 0.1
-    # the file is readable by a loader that executes nothing from it
-    torch.load(path, weights_only=True)
+    # mocat's format: a pickle naming mocat's cdict classes and numpy's array globals only
+    from mocat_amd.core import _CdictUnpickler
+    names = set()
+
+    class Recording(_CdictUnpickler):
+        def find_class(self, module, name):
+            names.add((module, name))
+            return super().find_class(module, name)
+
+    with open(path, "rb") as f:
+        Recording(f).load()
+    assert names == {("mocat.src.core", "cdict"), ("mocat.src.core", "static_cdict"),
+                     ("numpy.core.multiarray", "_reconstruct"), ("numpy", "ndarray"), ("numpy", "dtype")}, names
+
+
+def _fake_mocat(monkeypatch):
+    """mocat.src.core's cdict as mocat defines it (mocat/src/core.py:20-24,87-88), importable under
+    mocat's module path for the length of a test (mocat itself needs jax, which is not installed)."""
+    mod = types.ModuleType("mocat.src.core")
+
+    class MocatCdict:
+        def __init__(self, **kwargs):
+            self.__dict__.update(kwargs)
+
+    class MocatStatic(MocatCdict):
+        pass
+
+    for cls, name in ((MocatCdict, "cdict"), (MocatStatic, "static_cdict")):
+        cls.__module__, cls.__qualname__, cls.__name__ = "mocat.src.core", name, name
+        setattr(mod, name, cls)
+    for name in ("mocat", "mocat.src"):
+        monkeypatch.setitem(sys.modules, name, types.ModuleType(name))
+    monkeypatch.setitem(sys.modules, "mocat.src.core", mod)
+    return mod
+
+
+def test_files_interoperate_with_mocat(tmp_path, monkeypatch):
+    mod = _fake_mocat(monkeypatch)
+    # written here, read by mocat's pickle.load (mocat/src/core.py:113-114): mocat's classes, numpy arrays
+    c = cdict(value=torch.arange(6.).reshape(3, 2), key=torch.tensor([1, 2], dtype=torch.uint32), time=1.5,
+              summary=static_cdict(name="HMC"))
+    c.save(tmp_path / "ours.cdict")
+    with open(tmp_path / "ours.cdict", "rb") as f:
+        theirs = pickle.load(f)
+    assert type(theirs) is mod.cdict and type(theirs.summary) is mod.static_cdict
+    assert isinstance(theirs.value, np.ndarray) and theirs.value.dtype == np.float32
+    np.testing.assert_array_equal(theirs.value, np.arange(6.).reshape(3, 2))
+    assert theirs.key.dtype == np.uint32 and theirs.time == 1.5 and theirs.summary.name == "HMC"
+    # written by mocat's pickle.dump (mocat/src/core.py:103-104; its jax arrays pickle as numpy arrays),
+    # read here
+    m = mod.cdict(value=np.ones((4, 2), np.float32), log_weight=np.zeros(4), iter=np.int64(3), name="run",
+                  extra=mod.cdict(random_key=np.array([0, 7], np.uint32)))
+    with open(tmp_path / "theirs.cdict", "wb") as f:
+        pickle.dump(m, f)
+    d = load_cdict(tmp_path / "theirs.cdict")
+    assert isinstance(d, cdict) and isinstance(d.extra, cdict)
+    assert torch.equal(d.value, torch.ones((4, 2))) and d.log_weight.dtype == torch.float64
+    assert d.iter == 3 and d.name == "run" and torch.equal(d.extra.random_key, torch.tensor([0, 7], dtype=torch.uint32))
+
+
+class _NotACdict:
+    pass
+
+
+def test_load_refuses_other_globals(tmp_path):
+    # a file naming any global besides cdicts and numpy arrays is refused before anything is built
+    (tmp_path / "x.cdict").write_bytes(pickle.dumps(cdict(payload=_NotACdict())))
+    with pytest.raises(pickle.UnpicklingError):
+        load_cdict(tmp_path / "x.cdict")
+
+
+def test_load_reads_the_earlier_torch_format(tmp_path):
+    from mocat_amd.core import _to_host
+    c = cdict(value=torch.randn(3, 2), extra=cdict(iter=2))
+    torch.save(_to_host(c), tmp_path / "old.cdict")
+    d = load_cdict(tmp_path / "old.cdict")
+    assert torch.equal(d.value, c.value) and d.extra.iter == 2
 
 
 def test_save_load_errors(tmp_path):
