@@ -87,6 +87,31 @@ int mocat_hmc_lr_iterate(const float* X, const float* y, const void* packed, int
                          float* trace_value, float* trace_potential, float* trace_alpha, float* trace_stepsize,
                          float* trace_momenta, hipStream_t stream);
 
+/* The same iteration in three pieces for scenarios without a fused kernel (the caller evaluates the
+ * potentials and gradients between them, e.g. a user Scenario's jax.grad-derived
+ * *_potential_and_grad, mocat/src/core.py:204-230), with the fused kernel's key tree and arithmetic.
+ * Per iteration: mocat_hmc_begin, then per leapfrog step kick(drift = 1) on the proposal, the
+ * caller's evaluation at the new position, kick(drift = 0); then mocat_hmc_correct.
+ * begin: Underdamped.always (mocat/src/mcmc/standard_mcmc.py:107-123) and the proposal's key split
+ * (:125-137): momenta [C][d] refreshed in place and copied to mom0, kin0 [C] their kinetic energy,
+ * keys [C][2] advanced to split(key', L + 1)[0], step_keys [C][L][2] (or NULL) = the leapfrog
+ * potentials' keys split(key', L + 1)[1..L]. */
+int mocat_hmc_begin(int64_t C, int d, int leapfrog_steps, float friction, const float* stepsize, float* momenta,
+                    float* mom0, float* kin0, uint32_t* keys, uint32_t* step_keys, hipStream_t stream);
+/* Half a leapfrog step (mocat/src/utils.py:117-146): momenta -= stepsize/2 * grad; with drift,
+ * x += stepsize * momenta. */
+int mocat_hmc_kick(int64_t C, int d, float* x, float* momenta, const float* grad, const float* stepsize, int drift,
+                   hipStream_t stream);
+/* Metropolis / RMMetropolis correction of the proposal (x1, grad1, pot1, prior1, lik1; momenta = p')
+ * against the state (x, grad, pot, prior, lik), mocat/src/mcmc/metropolis.py:48-161 with
+ * Underdamped.acceptance_probability (standard_mcmc.py:139-153). iter = extra.iter after this
+ * iteration; correction and the RM arguments as mocat_hmc_lr_iterate. */
+int mocat_hmc_correct(int64_t C, int d, int64_t iter, int correction, float target, float rm_scale,
+                      float rm_neg_exponent, double adapt_cut_off, float* x, float* grad, float* pot, float* prior,
+                      float* lik, const float* x1, const float* grad1, const float* pot1, const float* prior1,
+                      const float* lik1, float* momenta, const float* mom0, const float* kin0, float* stepsize,
+                      float* alpha, uint32_t* keys, hipStream_t stream);
+
 /* ------------------------------------------------------------------------- particle filters
  * run_particle_filter_for_marginals with a BootstrapFilter (mocat/src/ssm/filtering.py:255-324,
  * 142-170) for many independent filters of uneven n and T in one launch (one workgroup per filter).

@@ -20,6 +20,7 @@ import io
 import pickle
 from pathlib import Path
 from typing import Union
+from warnings import warn
 
 import numpy as np
 import torch
@@ -237,16 +238,49 @@ def is_implemented(func):
     return not hasattr(func, "not_implemented")
 
 
+def _zero_potential(x, random_key=None):
+    return torch.zeros(x.shape[:-1], dtype=x.dtype, device=x.device)
+
+
+def _grad_and_value(fn, x, random_key=None):
+    """(grad, value) of a per-sample potential fn(x (d,), key) at x (..., d): torch.func.grad_and_value,
+    vmapped over the leading dimensions (the torch counterpart of mocat's vmap(value_and_grad))."""
+    from torch.func import grad_and_value, vmap
+    if fn is _zero_potential:
+        return torch.zeros_like(x), _zero_potential(x)
+    if x.dim() <= 1:
+        return grad_and_value(fn)(x, random_key)
+    lead, d = x.shape[:-1], x.shape[-1]
+    xf = x.reshape(-1, d)
+    kf, kdim = None, None
+    if random_key is not None:
+        rk = torch.as_tensor(random_key)
+        if rk.dim() > 1:
+            kf, kdim = rk.reshape(-1, rk.shape[-1]), 0
+        else:
+            kf = rk
+    g, v = vmap(grad_and_value(fn), in_dims=(0, kdim))(xf, kf)
+    return g.reshape(*lead, d), v.reshape(lead)
+
+
 class Scenario:
     """Target distribution: potential = prior_potential + temperature * likelihood_potential.
 
-    Subclasses implement batched device versions of prior_potential / likelihood_potential and
-    their `*_and_grad` variants over (..., d) tensors. mocat derives the gradients with jax.grad
-    (mocat/src/core.py:204-230); here a scenario supplies them, typically through a fused kernel.
+    Two kinds of scenario share this interface. Scenarios with a fused kernel
+    (scenarios.LogisticRegression, DiagonalGaussian, GaussianMixture) implement batched device
+    versions of the potentials and their `*_and_grad` variants over (..., d) tensors. A user scenario
+    is written as in mocat (mocat/src/core.py:146-230): subclass Scenario and implement
+    `likelihood_potential(x, random_key=None)` (or `potential`) and optionally `prior_potential`, each
+    with torch operations on one sample x of shape (d,). As mocat's __init__ does, a missing prior
+    becomes uniform (with a warning) and a lone `potential` becomes the likelihood; the gradients are
+    derived automatically (torch.func.grad_and_value where mocat uses jax's value_and_grad) and every
+    derived function also accepts a batch (..., d), evaluated with torch.func.vmap. The samplers run
+    such scenarios with their device kernels and call the potentials between launches.
     """
     name: str = "Scenario"
     dim: int = None
     temperature: float = 1.0
+    _fused_kernel = False  # set by the scenarios with a fused kernel
 
     def __init__(self, name: str = None, **kwargs):
         if name is not None:
@@ -254,6 +288,58 @@ class Scenario:
         for key, value in kwargs.items():
             if hasattr(self, key):
                 self.__dict__[key] = value
+        if not self._fused_kernel:
+            self._derive_user_potentials()
+
+    def _derive_user_potentials(self):
+        """mocat's Scenario.__init__ / init_grad (mocat/src/core.py:161-230) for a user scenario."""
+        has_lik = is_implemented(self.likelihood_potential)
+        has_pot = type(self).potential is not Scenario.potential
+        if not (has_lik or has_pot):
+            return
+        d = self.__dict__
+        if not has_lik:
+            if not is_implemented(self.prior_potential):
+                d["prior_potential"] = _zero_potential
+            d["likelihood_potential"] = self.potential
+        if not is_implemented(self.prior_potential):
+            warn(f"{self.name} prior_potential not initiated, assuming uniform")
+            d["prior_potential"] = _zero_potential
+        prior, lik = self.prior_potential, self.likelihood_potential
+        d["tempered_potential"] = lambda x, temperature, random_key=None: \
+            prior(x, random_key) + temperature * lik(x, random_key) if x.dim() <= 1 else \
+            self.evaluate(x, temperature, random_key, grads=False)[0]
+        d["potential"] = lambda x, random_key=None: self.tempered_potential(x, self.temperature, random_key)
+
+        def pair(fn):
+            def value_and_grad(x, random_key=None):
+                g, v = _grad_and_value(fn, x, random_key)
+                return v, g
+            return value_and_grad
+
+        d["prior_potential_and_grad"] = pair(prior)
+        d["likelihood_potential_and_grad"] = pair(lik)
+
+    def evaluate(self, x, temperature: float = 1.0, random_key=None, grads: bool = True):
+        """(prior_potential, likelihood_potential, grad_prior, grad_likelihood) of a user scenario at
+        x (..., d) in float32 on x's device (the batched form the samplers consume; the prior key is
+        passed to both potentials, as mocat's samplers do). grads=False: tempered potential only."""
+        if self._fused_kernel:
+            raise NotImplementedError("fused scenarios define evaluate")
+        x = torch.as_tensor(x)
+        if not grads:
+            from torch.func import vmap
+            prior, lik = self.prior_potential, self.likelihood_potential
+            xf = x.reshape(-1, x.shape[-1])
+            kdim = 0 if (random_key is not None and torch.as_tensor(random_key).dim() > 1) else None
+            kf = None if random_key is None else torch.as_tensor(random_key).reshape(
+                (-1, 2) if kdim == 0 else (2,))
+            f = lambda xx, kk: prior(xx, kk) + temperature * lik(xx, kk)
+            return (vmap(f, in_dims=(0, kdim))(xf, kf).reshape(x.shape[:-1]),)
+        gp, pp = _grad_and_value(self.prior_potential, x, random_key)
+        gl, lp = _grad_and_value(self.likelihood_potential, x, random_key)
+        f32 = lambda t: t.to(torch.float32).contiguous()
+        return f32(pp), f32(lp), f32(gp), f32(gl)
 
     def __repr__(self):
         return f"mocat_amd.Scenario.{self.__class__.__name__}"

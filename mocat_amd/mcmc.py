@@ -33,7 +33,24 @@ _lib.register("mocat_hmc_lr_iterate",
                _lib.c_f32, _lib.c_f32, _lib.c_f64, _lib.c_ptr, _lib.c_ptr, _lib.c_ptr, _lib.c_ptr, _lib.c_ptr,
                _lib.c_ptr])
 
+_lib.register("mocat_hmc_begin", [_lib.c_i64, _lib.c_i32, _lib.c_i32, _lib.c_f32, _lib.c_ptr, _lib.c_ptr, _lib.c_ptr,
+                                   _lib.c_ptr, _lib.c_ptr, _lib.c_ptr, _lib.c_ptr])
+_lib.register("mocat_hmc_kick", [_lib.c_i64, _lib.c_i32, _lib.c_ptr, _lib.c_ptr, _lib.c_ptr, _lib.c_ptr, _lib.c_i32,
+                                  _lib.c_ptr])
+_lib.register("mocat_hmc_correct",
+              [_lib.c_i64, _lib.c_i32, _lib.c_i64, _lib.c_i32, _lib.c_f32, _lib.c_f32, _lib.c_f32, _lib.c_f64]
+              + [_lib.c_ptr] * 17)
+
 DEFAULT_STACK = ("value", "potential", "alpha", "stepsize", "momenta")
+
+
+def evaluate(scenario, x, temperature, random_key=None):
+    """(prior_potential, likelihood_potential, grad_prior, grad_likelihood) at x (C, d): the fused
+    kernel of a compiled scenario, or a user scenario's torch potentials (their keys as mocat passes
+    them: `random_key` [C][2] per chain, or None)."""
+    if getattr(scenario, "_fused_kernel", False):
+        return scenario.evaluate(x, temperature)
+    return scenario.evaluate(x, temperature, random_key)
 
 
 class Correction:
@@ -136,10 +153,12 @@ class Underdamped(MCMCSampler):
             raise ValueError("stepsize must be given unless the correction is RMMetropolis")
         initial_state.stepsize = torch.full((C,), float(params.stepsize), dtype=torch.float32, device=keys.device)
         initial_state.alpha = torch.ones(C, dtype=torch.float32, device=keys.device)
-        # (key, prior_key, lik_key) = split(key, 3); the fused scenarios take no potential keys
-        keys = mr.split(keys, 3)[:, 0].contiguous()
+        # (key, prior_key, lik_key) = split(key, 3); mocat passes prior_key to both potentials (the fused
+        # scenarios take no potential keys)
+        k3 = mr.split(keys, 3)
+        keys = k3[:, 0].contiguous()
         T = float(scenario.temperature)
-        pp, lp, gp, gl = scenario.evaluate(initial_state.value, T)
+        pp, lp, gp, gl = evaluate(scenario, initial_state.value, T, k3[:, 1].contiguous())
         initial_state.prior_potential = pp
         initial_state.likelihood_potential = lp
         initial_state.potential = pp + T * lp
@@ -160,9 +179,11 @@ class Underdamped(MCMCSampler):
         return c.kernel_mode, float(self.tuning.target or 0.0), 1.0, 2 / 3, math.inf
 
     def iterate(self, scenario, state: cdict, extra: cdict, n_iter: int, traces: dict = None):
-        """Advance the batched state in place by n_iter iterations (one kernel launch)."""
+        """Advance the batched state in place by n_iter iterations (one kernel launch for a scenario
+        with a fused kernel; per leapfrog step two kick launches around the scenario's potentials
+        otherwise)."""
         if not hasattr(scenario, "kernel_args"):
-            raise NotImplementedError(f"{scenario.__class__.__name__} has no fused HIP kernel")
+            return self._iterate_user(scenario, state, extra, n_iter, traces or {})
         traces = traces or {}
         X, y, packed, nrows, d, prec, pprec = scenario.kernel_args()
         C = state.value.shape[0]
@@ -176,6 +197,47 @@ class Underdamped(MCMCSampler):
                   _lib.ptr(traces.get("value")), _lib.ptr(traces.get("potential")), _lib.ptr(traces.get("alpha")),
                   _lib.ptr(traces.get("stepsize")), _lib.ptr(traces.get("momenta")), _lib.stream_handle())
         extra.iter += int(n_iter)
+        return state, extra
+
+    def _iterate_user(self, scenario, state: cdict, extra: cdict, n_iter: int, traces: dict):
+        """The fused kernel's iteration for a scenario without one: mocat_hmc_begin (always and the
+        proposal's keys), per leapfrog step kick(drift) -> the scenario's potentials and gradients at
+        the new position (with that step's key, as mocat's leapfrog, mocat/src/utils.py:117-146) ->
+        kick, then mocat_hmc_correct. Same key tree and update arithmetic as the fused kernel."""
+        C, d = state.value.shape
+        p = extra.parameters
+        L = int(p.leapfrog_steps)
+        T = float(scenario.temperature)
+        mode, target, rm_scale, rm_exp, cut = self._correction_args()
+        dev = state.value.device
+        mom0 = torch.empty_like(state.momenta)
+        kin0 = torch.empty(C, dtype=torch.float32, device=dev)
+        step_keys = torch.empty((C, max(L, 1), 2), dtype=torch.uint32, device=dev)
+        st = _lib.stream_handle()
+        for it in range(int(n_iter)):
+            _lib.call("mocat_hmc_begin", C, d, L, float(p.friction), _lib.ptr(state.stepsize), _lib.ptr(state.momenta),
+                      _lib.ptr(mom0), _lib.ptr(kin0), _lib.ptr(extra.random_key), _lib.ptr(step_keys), st)
+            x1 = state.value.clone()
+            g1 = state.grad_potential.clone()
+            pot1, prior1, lik1 = state.potential.clone(), state.prior_potential.clone(), \
+                state.likelihood_potential.clone()
+            for s_ in range(L):
+                _lib.call("mocat_hmc_kick", C, d, _lib.ptr(x1), _lib.ptr(state.momenta), _lib.ptr(g1),
+                          _lib.ptr(state.stepsize), 1, st)
+                prior1, lik1, gp, gl = evaluate(scenario, x1, T, step_keys[:, s_].contiguous())
+                pot1 = (prior1 + T * lik1).contiguous()
+                g1 = (gp + T * gl).contiguous()
+                _lib.call("mocat_hmc_kick", C, d, None, _lib.ptr(state.momenta), _lib.ptr(g1),
+                          _lib.ptr(state.stepsize), 0, st)
+            extra.iter += 1
+            _lib.call("mocat_hmc_correct", C, d, int(extra.iter), mode, target, rm_scale, rm_exp, cut,
+                      _lib.ptr(state.value), _lib.ptr(state.grad_potential), _lib.ptr(state.potential),
+                      _lib.ptr(state.prior_potential), _lib.ptr(state.likelihood_potential), _lib.ptr(x1), _lib.ptr(g1),
+                      _lib.ptr(pot1), _lib.ptr(prior1), _lib.ptr(lik1),
+                      _lib.ptr(state.momenta), _lib.ptr(mom0), _lib.ptr(kin0), _lib.ptr(state.stepsize),
+                      _lib.ptr(state.alpha), _lib.ptr(extra.random_key), st)
+            for k, v in traces.items():
+                v[it].copy_(getattr(state, k))
         return state, extra
 
     def update(self, scenario, state: cdict, extra: cdict):

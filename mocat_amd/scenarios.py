@@ -77,6 +77,7 @@ class LogisticRegression(Scenario):
     Supports d <= 128.
     """
     name = "Logistic Regression"
+    _fused_kernel = True
 
     def __init__(self, X, y, prior_variance: float = 10.0, precision: str = "bf16x3", name: str = None):
         super().__init__(name=name)
@@ -193,6 +194,7 @@ class GaussianMixture(Scenario):
     No prior: prior_potential = 0 (mocat warns and assumes a uniform prior, core.py:183-189).
     """
     name = "Gaussian Mixture"
+    _fused_kernel = True
 
     def __init__(self, means, covariances=None, weights=None, det_convention: str = "textbook", name: str = None):
         super().__init__(name=name)

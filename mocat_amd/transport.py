@@ -26,7 +26,7 @@ import torch
 from . import _lib
 from . import random as mr
 from .core import Scenario, cdict, is_implemented
-from .mcmc import Metropolis, check_correction
+from .mcmc import Metropolis, check_correction, evaluate
 from .sample import Sampler
 
 _P = _lib.c_ptr
@@ -131,10 +131,30 @@ class MetropolisedSMCSampler(TemperedSMCSampler):
             setattr(state, k, gather_rows(getattr(state, k), idx))
         return state, idx
 
+    def _user_moves(self, scenario, state, stepsize, keys, beta, steps, alpha_trace):
+        """forward_proposal's corrected Underdamped updates at temperature beta for a scenario without a
+        fused kernel: the MCMC sampler's iteration in pieces (its potentials between launches), the
+        same key tree and arithmetic as the fused moves (Metropolis, fixed stepsize, extra.iter from 0)."""
+        ms = self.mcmc_sampler
+        p = self.parameters
+        saved, saved_T = ms.correction, scenario.temperature
+        ms.correction = Metropolis()
+        scenario.temperature = beta
+        try:
+            st = cdict(value=state.value, momenta=state.momenta, grad_potential=state.grad_potential,
+                       potential=state.potential, prior_potential=state.prior_potential,
+                       likelihood_potential=state.likelihood_potential, stepsize=stepsize, alpha=state.alpha)
+            ex = cdict(random_key=keys, iter=0,
+                       parameters=cdict(leapfrog_steps=int(p.leapfrog_steps), friction=float(p.friction)))
+            ms._iterate_user(scenario, st, ex, steps, {"alpha": alpha_trace})
+        finally:
+            ms.correction, scenario.temperature = saved, saved_T
+        for k in ("potential", "prior_potential", "likelihood_potential"):
+            setattr(state, k, getattr(st, k))
+
     def run_fused(self, scenario, n, initial_state, initial_extra, n_chains=None,
                   stack=("value", "log_weight"), **kwargs):
-        if not hasattr(scenario, "kernel_args"):
-            raise NotImplementedError(f"{scenario.__class__.__name__} has no fused HIP kernel")
+        fused = hasattr(scenario, "kernel_args")
         ms = self.mcmc_sampler
         ms.correction = check_correction(ms.correction)
         if not isinstance(ms.correction, Metropolis):
@@ -152,7 +172,7 @@ class MetropolisedSMCSampler(TemperedSMCSampler):
         # ---- startup (TransportSampler / TemperedSMCSampler / MetropolisedSMCSampler.startup)
         state, key = self.startup_particles(scenario, n, initial_state, key)
         key = mr.split(key, 2 * n + 1)[-1].contiguous()
-        pp, lp, gp, gl = scenario.evaluate(state.value, 0.0)
+        pp, lp, gp, gl = evaluate(scenario, state.value, 0.0)
         state.prior_potential, state.likelihood_potential = pp, lp
         lw0 = torch.zeros(n, dtype=torch.float32, device=pp.device)
         ess = float(n)
@@ -161,8 +181,7 @@ class MetropolisedSMCSampler(TemperedSMCSampler):
         lnc = float(out[2]) - math.log(n)      # logsumexp(lw, b=1/n)
         ess = float(out[1])
         key = mr.split(key, 3)[0].contiguous()  # vmapped Underdamped.startup
-        scenario_T = beta
-        pp, lp, gp, gl = scenario.evaluate(state.value, scenario_T)
+        pp, lp, gp, gl = evaluate(scenario, state.value, beta)
         state.prior_potential, state.likelihood_potential = pp, lp
         state.potential = pp + beta * lp
         state.grad_potential = (gp + beta * gl).contiguous()
@@ -185,7 +204,8 @@ class MetropolisedSMCSampler(TemperedSMCSampler):
 
         record(1.0, 0)
         it = 0
-        X, y, packed, nrows, d, prec, pprec = scenario.kernel_args()
+        if fused:
+            X, y, packed, nrows, d, prec, pprec = scenario.kernel_args()
         steps = int(p.mcmc_steps)
         alpha_trace = torch.empty((steps, n), dtype=torch.float32, device=pp.device)
         while beta < self.max_temperature and it < self.max_iter:
@@ -198,17 +218,20 @@ class MetropolisedSMCSampler(TemperedSMCSampler):
                 ess = float(n)
             # forward_proposal: Underdamped.startup then mcmc_steps corrected updates at `beta`
             pkeys = mr.split(keys_all[:n].contiguous(), 3)[:, 0].contiguous()
-            pp, lp, gp, gl = scenario.evaluate(state.value, beta)
+            pp, lp, gp, gl = evaluate(scenario, state.value, beta)
             state.potential = pp + beta * lp
             state.grad_potential = (gp + beta * gl).contiguous()
             state.prior_potential, state.likelihood_potential = pp, lp
             stepsize = torch.full((n,), h, dtype=torch.float32, device=lw.device)
-            _lib.call("mocat_hmc_lr_iterate", X, y, packed, nrows, d, prec, pprec, float(beta), n,
-                      _lib.ptr(state.value), _lib.ptr(state.momenta), _lib.ptr(state.grad_potential),
-                      _lib.ptr(state.potential), _lib.ptr(state.prior_potential),
-                      _lib.ptr(state.likelihood_potential), _lib.ptr(stepsize), _lib.ptr(state.alpha),
-                      _lib.ptr(pkeys), 0, steps, int(p.leapfrog_steps), float(p.friction), 1, 0.0, 1.0, 2 / 3,
-                      math.inf, _P(0), _P(0), _lib.ptr(alpha_trace), _P(0), _P(0), _lib.stream_handle())
+            if not fused:
+                self._user_moves(scenario, state, stepsize, pkeys, beta, steps, alpha_trace)
+            else:
+                _lib.call("mocat_hmc_lr_iterate", X, y, packed, nrows, d, prec, pprec, float(beta), n,
+                          _lib.ptr(state.value), _lib.ptr(state.momenta), _lib.ptr(state.grad_potential),
+                          _lib.ptr(state.potential), _lib.ptr(state.prior_potential),
+                          _lib.ptr(state.likelihood_potential), _lib.ptr(stepsize), _lib.ptr(state.alpha),
+                          _lib.ptr(pkeys), 0, steps, int(p.leapfrog_steps), float(p.friction), 1, 0.0, 1.0, 2 / 3,
+                          math.inf, _P(0), _P(0), _lib.ptr(alpha_trace), _P(0), _P(0), _lib.stream_handle())
             # adapt: next temperature, reweight, ESS, log-normaliser, RM stepsize
             lw, out = self._temper(lw, state.likelihood_potential, beta, ess, it, math.log(h), alpha_trace)
             beta_new = float(out[0])
@@ -330,15 +353,13 @@ class SVGD(TransportSampler):
                   _lib.ptr(workspace), step, float(self.parameters.momentum), _lib.ptr(x), _lib.ptr(extra.g_sq),
                   _lib.ptr(extra.m), _lib.ptr(phi), _lib.stream_handle())
         T = float(scenario.temperature)
-        _, lp, _, gl = scenario.evaluate(x, T)
-        state.potential = T * lp
-        state.grad_potential = (T * gl).contiguous()
+        pp, lp, gp, gl = evaluate(scenario, x, T)
+        state.potential = pp + T * lp
+        state.grad_potential = (gp + T * gl).contiguous()
         extra.bandwidth = self._adapt(x)
         return phi
 
     def run_fused(self, scenario, n, initial_state, initial_extra, n_chains=None, stack=(), **kwargs):
-        if not hasattr(scenario, "evaluate"):
-            raise NotImplementedError(f"{scenario.__class__.__name__} has no fused HIP kernel")
         summary = self.summary(scenario, initial_state, initial_extra)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -346,8 +367,8 @@ class SVGD(TransportSampler):
         x = state.value.to(torch.float32).contiguous()
         key = mr.split(key, n + 1)[-1].contiguous()
         T = float(scenario.temperature)
-        _, lp, _, gl = scenario.evaluate(x, T)
-        state = cdict(value=x, potential=T * lp, grad_potential=(T * gl).contiguous())
+        pp, lp, gp, gl = evaluate(scenario, x, T)
+        state = cdict(value=x, potential=pp + T * lp, grad_potential=(gp + T * gl).contiguous())
         extra = cdict(random_key=key, iter=0, g_sq=torch.zeros_like(x), m=torch.zeros_like(x))
         extra.bandwidth = float(self.parameters.kernel_params.bandwidth)
         extra.bandwidth = self._adapt(x)
