@@ -34,6 +34,12 @@ COMMON_FLAGS = [
 ]
 
 
+# Per-source additions. lr.hip: the iterative ILP machine scheduler orders the bf16x3 engine's block loop
+# (loads, MFMAs, row functions) measurably better than the default (5% per HMC iteration at d = 50,
+# 4% at d = 100, identical results).
+SOURCE_FLAGS = {"lr.hip": ["-mllvm", "-amdgpu-sched-strategy=iterative-ilp"]}
+
+
 def hipcc() -> str:
     cand = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
     if not Path(cand).exists():
@@ -47,7 +53,7 @@ def sources():
 
 def compile_one(src: Path) -> Path:
     obj = BUILD / (src.stem + ".o")
-    cmd = [hipcc(), *COMMON_FLAGS, "-I", str(HERE), "-c", str(src), "-o", str(obj)]
+    cmd = [hipcc(), *COMMON_FLAGS, *SOURCE_FLAGS.get(src.name, []), "-I", str(HERE), "-c", str(src), "-o", str(obj)]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")

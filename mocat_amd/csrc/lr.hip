@@ -487,8 +487,13 @@ struct Bf16Engine {
       BBuf b;
       floatx16 z;
       load_a(a, j);
+      // At d <= 64 (two waves per SIMD) a wave issuing its first GEMM and the next loads runs at raised
+      // priority, so that the other wave's row functions fill the vector ALU meanwhile (measured 3%
+      // faster per HMC iteration at the bench shape; at d > 64, one wave per SIMD, it does not apply)
+      if constexpr (DP == 64) __builtin_amdgcn_s_setprio(1);
       gemm1(a, z);
       load_b(b, j);
+      if constexpr (DP == 64) __builtin_amdgcn_s_setprio(0);
       step(j, z, b);
     }
     if constexpr (WANT_POT) {
