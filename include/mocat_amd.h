@@ -294,6 +294,13 @@ int mocat_svgd_step(const float* x_in, const float* grad, int64_t n, int64_t d, 
  * grad[n][d] (GaussianMixture, mocat/src/scenarios/toy_examples.py:54-116). K <= 8. */
 int mocat_gmm_potential_grad(const float* x, int64_t n, int64_t d, int K, const float* means, const float* lam,
                              const float* logc, float* potential, float* grad, hipStream_t stream);
+/* Gaussian mixture with full covariances: potential[n] = -logsumexp_k(logc_k - 0.5 r_k^T S_k r_k),
+ * r_k = x - mu_k, grad[n][d] = sum_k softmax_k S_k r_k; S [K][d][d] symmetric (the precision P_k, or
+ * L_k^T L_k with L_k = cholesky(P_k) for mocat's component potential as written,
+ * mocat/src/scenarios/toy_examples.py:95-100). K <= 8, d <= 128. */
+int mocat_gmm_full_potential_grad(const float* x, int64_t n, int64_t d, int K, const float* means,
+                                  const float* S, const float* logc, float* potential, float* grad,
+                                  hipStream_t stream);
 /* d2[n][n] = |x_i - x_j|^2 (diagonal 0), the distance matrix of l2_distance_matrix
  * (mocat/src/utils.py:437-439) squared. workspace: mocat_svgd_workspace_bytes(n, d, 1). */
 int mocat_pairwise_sqdist(const float* x, int64_t n, int64_t d, void* workspace, float* d2, hipStream_t stream);

@@ -547,6 +547,89 @@ __global__ __launch_bounds__(256) void gmm_kernel(const float* __restrict__ x, i
   }
 }
 
+// Full covariances: U(x) = -logsumexp_k [ logc_k - 0.5 r_k^T S_k r_k ], grad = sum_k softmax_k S_k r_k with
+// r_k = x - mu_k and S_k symmetric ([K][d][d]: the precision, or mocat's L^T L form,
+// mocat/src/scenarios/toy_examples.py:95-100). A wave evaluates 4 particles; lane l holds columns l and
+// l + 64 (d <= 128); for each component the rows of S_k are read once per wave (coalesced, L2-resident)
+// and applied to the 4 particles, the row's coefficient r_i broadcast from its lane by readlane.
+constexpr int kGmmFullPer = 4;
+template <int K>
+__global__ __launch_bounds__(256) void gmm_full_kernel(const float* __restrict__ x, int n, int d,
+                                                       const float* __restrict__ mu, const float* __restrict__ S,
+                                                       const float* __restrict__ logc, float* __restrict__ pot,
+                                                       float* __restrict__ grad) {
+#pragma clang fp contract(off)
+  const int lane = threadIdx.x & (kWave - 1);
+  const int64_t p0 = ((int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave) * kGmmFullPer;
+  if (p0 >= n) return;
+  float xv[kGmmFullPer][2];
+#pragma unroll
+  for (int p = 0; p < kGmmFullPer; ++p)
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      const int col = lane + c * kWave;
+      xv[p][c] = (p0 + p < n && col < d) ? x[(p0 + p) * d + col] : 0.f;
+    }
+  float q[K][kGmmFullPer], sv[K][kGmmFullPer][2];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    float r[kGmmFullPer][2], s[kGmmFullPer][2];
+#pragma unroll
+    for (int p = 0; p < kGmmFullPer; ++p)
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        const int col = lane + c * kWave;
+        r[p][c] = col < d ? xv[p][c] - mu[k * d + col] : 0.f;
+        s[p][c] = 0.f;
+      }
+    const float* Sk = S + (int64_t)k * d * d;
+    for (int i = 0; i < d; ++i) {
+      float srow[2];
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        const int col = lane + c * kWave;
+        srow[c] = col < d ? Sk[(int64_t)i * d + col] : 0.f;
+      }
+      const int src = i & (kWave - 1);
+#pragma unroll
+      for (int p = 0; p < kGmmFullPer; ++p) {
+        const float rv = i < kWave ? r[p][0] : r[p][1];
+        const float ri = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(rv), src));
+#pragma unroll
+        for (int c = 0; c < 2; ++c) s[p][c] = fmaf(srow[c], ri, s[p][c]);
+      }
+    }
+#pragma unroll
+    for (int p = 0; p < kGmmFullPer; ++p) {
+      float a = fmaf(r[p][0], s[p][0], r[p][1] * s[p][1]);
+      a = wave_sum(a);
+      q[k][p] = fmaf(-0.5f, a, logc[k]);
+#pragma unroll
+      for (int c = 0; c < 2; ++c) sv[k][p][c] = s[p][c];
+    }
+  }
+#pragma unroll
+  for (int p = 0; p < kGmmFullPer; ++p) {
+    if (p0 + p >= n) break;
+    float qmax = -INFINITY;
+#pragma unroll
+    for (int k = 0; k < K; ++k) qmax = fmaxf(qmax, q[k][p]);
+    float ssum = 0.f;
+#pragma unroll
+    for (int k = 0; k < K; ++k) ssum += __expf(q[k][p] - qmax);
+    if (lane == 0) pot[p0 + p] = -(qmax + __logf(ssum));
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      const int col = lane + c * kWave;
+      if (col >= d) continue;
+      float gsum = 0.f;
+#pragma unroll
+      for (int k = 0; k < K; ++k) gsum = fmaf(__expf(q[k][p] - qmax) / ssum, sv[k][p][c], gsum);
+      grad[(p0 + p) * d + col] = gsum;
+    }
+  }
+}
+
 // ---------------------------------------------------------------- bracketed median (fused SVGD run)
 // Between SVGD iterations the median pair distance moves little, so each selection first counts the
 // pairs below a bracket [lo, hi) around the previous median and copies those inside it to a candidate
@@ -1256,6 +1339,27 @@ int mocat_gmm_potential_grad(const float* x, int64_t n, int64_t d, int K, const 
   const int per = 256 / kWave;
   hipLaunchKernelGGL(gmm_kernel, dim3((unsigned)((n + per - 1) / per)), dim3(256), 0, stream, x, (int)n, (int)d, K,
                      means, lam, logc, potential, grad);
+  MOCAT_LAUNCH_CHECK();
+  return MOCAT_OK;
+}
+
+int mocat_gmm_full_potential_grad(const float* x, int64_t n, int64_t d, int K, const float* means,
+                                  const float* S, const float* logc, float* potential, float* grad,
+                                  hipStream_t stream) {
+  if (!x || !means || !S || !logc || !potential || !grad || n <= 0 || d <= 0 || d > 128 || K <= 0 || K > 8)
+    return MOCAT_ERR_ARG;
+  const int64_t per_block = (256 / kWave) * kGmmFullPer;
+  const dim3 grid((unsigned)((n + per_block - 1) / per_block));
+  switch (K) {
+#define MOCAT_GMM_FULL(KK)                                                                                   \
+  case KK:                                                                                                 \
+    hipLaunchKernelGGL(gmm_full_kernel<KK>, grid, dim3(256), 0, stream, x, (int)n, (int)d, means, S, logc,   \
+                       potential, grad);                                                                    \
+    break;
+    MOCAT_GMM_FULL(1) MOCAT_GMM_FULL(2) MOCAT_GMM_FULL(3) MOCAT_GMM_FULL(4)
+    MOCAT_GMM_FULL(5) MOCAT_GMM_FULL(6) MOCAT_GMM_FULL(7) MOCAT_GMM_FULL(8)
+#undef MOCAT_GMM_FULL
+  }
   MOCAT_LAUNCH_CHECK();
   return MOCAT_OK;
 }

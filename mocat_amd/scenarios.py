@@ -22,6 +22,8 @@ ENGINE_GAUSSIAN = 2
 
 _lib.register("mocat_gmm_potential_grad", [_lib.c_ptr, _lib.c_i64, _lib.c_i64, _lib.c_i32, _lib.c_ptr, _lib.c_ptr,
                                            _lib.c_ptr, _lib.c_ptr, _lib.c_ptr, _lib.c_ptr])
+_lib.register("mocat_gmm_full_potential_grad", [_lib.c_ptr, _lib.c_i64, _lib.c_i64, _lib.c_i32, _lib.c_ptr,
+                                                _lib.c_ptr, _lib.c_ptr, _lib.c_ptr, _lib.c_ptr, _lib.c_ptr])
 _lib.register("mocat_lr_packed_bytes", [_lib.c_i64, _lib.c_i64, _lib.c_ptr])
 _lib.register("mocat_lr_pack", [_lib.c_ptr, _lib.c_ptr, _lib.c_i64, _lib.c_i64, _lib.c_ptr, _lib.c_ptr])
 _lib.register("mocat_lr_potential_grad",
@@ -186,11 +188,15 @@ class DiagonalGaussian(LogisticRegression):
 
 
 class GaussianMixture(Scenario):
-    """Gaussian mixture with diagonal covariances (mocat/src/scenarios/toy_examples.py:54-116).
+    """Gaussian mixture (mocat/src/scenarios/toy_examples.py:54-116).
 
-    potential(x) = -log sum_k w_k c_k exp(-0.5 sum_j (x_j - mu_kj)^2 / var_kj), evaluated in
-    log-sum-exp form. det_convention="textbook" uses c_k = sqrt(det P_k) (2 pi)^(-d/2) (a normalised
-    mixture); "mocat" uses det(P_k) (2 pi)^(-d/2), the reference's weighting (toy_examples.py:101-103).
+    potential(x) = -log sum_k w_k c_k exp(-0.5 (x - mu_k)^T S_k (x - mu_k)), evaluated in log-sum-exp
+    form. det_convention="textbook": S_k = P_k = inv(Sigma_k) and c_k = sqrt(det P_k) (2 pi)^(-d/2) (a
+    normalised mixture); "mocat": the reference's component potential as written (toy_examples.py:95-103),
+    c_k = det(P_k) (2 pi)^(-d/2) and the quadratic form |L_k (x - mu_k)|^2 of L_k = cholesky(P_k), i.e.
+    S_k = L_k^T L_k (equal to P_k for diagonal covariances). Diagonal covariances (given as [K][d]
+    variances or diagonal matrices) run on mocat_gmm_potential_grad and the one-call SVGD run; full
+    covariances on mocat_gmm_full_potential_grad.
     No prior: prior_potential = 0 (mocat warns and assumes a uniform prior, core.py:183-189).
     """
     name = "Gaussian Mixture"
@@ -204,29 +210,48 @@ class GaussianMixture(Scenario):
         K, d = mu.shape
         if K > 8:
             raise ValueError("at most 8 components")
+        full = None
         if covariances is None:
             var = np.ones((K, d))
         else:
             c = np.asarray(covariances.detach().cpu() if isinstance(covariances, torch.Tensor) else covariances,
                            np.float64)
+            if c.ndim == 2 and c.shape == (d, d):  # one covariance matrix for every component (as mocat)
+                c = np.repeat(c[None], K, 0)
             if c.ndim == 3:
-                if not np.allclose(c, np.einsum("kij,ij->kij", c, np.eye(d))):
-                    raise NotImplementedError("the fused mixture kernel takes diagonal covariances")
                 var = np.einsum("kii->ki", c)
-            elif c.ndim == 2 and c.shape == (d, d):
-                var = np.repeat(np.diag(c)[None], K, 0)
-            else:
+                if not np.allclose(c, np.einsum("ki,ij->kij", var, np.eye(d)), rtol=0.0, atol=0.0):
+                    full = c
+            else:  # [K][d] (or broadcastable) variances
                 var = np.broadcast_to(c, (K, d)).astype(np.float64)
         w = np.ones(K) / K if weights is None else np.asarray(weights, np.float64)
-        lam = 1.0 / var
-        factor = {"textbook": 0.5, "mocat": 1.0}[det_convention]
-        logc = np.log(w) + factor * np.log(lam).sum(1) - 0.5 * d * np.log(2 * np.pi)
         dev = mr.default_device()
         self.dim, self.n_components = int(d), int(K)
         self.means = torch.tensor(mu, dtype=torch.float32, device=dev).contiguous()
-        self.lam = torch.tensor(lam, dtype=torch.float32, device=dev).contiguous()
-        self.logc = torch.tensor(logc, dtype=torch.float32, device=dev).contiguous()
         self.weights = w
+        self.full = full is not None
+        if self.full:
+            # the component matrices in FP64 on the host (mocat's __setattr__, toy_examples.py:73-88):
+            # P = inv(Sigma), L = cholesky(P) (input symmetrised, as jnp.linalg.cholesky reads one triangle)
+            P = np.linalg.inv(full)
+            L = np.linalg.cholesky(0.5 * (P + np.transpose(P, (0, 2, 1))))
+            detP = np.linalg.det(P)
+            if det_convention == "mocat":
+                S = np.einsum("kji,kjl->kil", L, L)
+                logc = np.log(w * detP) - 0.5 * d * np.log(2 * np.pi)
+            else:
+                S = 0.5 * (P + np.transpose(P, (0, 2, 1)))
+                logc = np.log(w) + 0.5 * np.log(detP) - 0.5 * d * np.log(2 * np.pi)
+            self.S = torch.tensor(S, dtype=torch.float32, device=dev).contiguous()
+            self.lam = None
+            self.covariances = full
+        else:
+            lam = 1.0 / var
+            factor = {"textbook": 0.5, "mocat": 1.0}[det_convention]
+            logc = np.log(w) + factor * np.log(lam).sum(1) - 0.5 * d * np.log(2 * np.pi)
+            self.lam = torch.tensor(lam, dtype=torch.float32, device=dev).contiguous()
+            self.S = None
+        self.logc = torch.tensor(logc, dtype=torch.float32, device=dev).contiguous()
         self.variances = var
 
     def evaluate(self, x, temperature: float = 1.0):
@@ -235,8 +260,13 @@ class GaussianMixture(Scenario):
         n = int(math.prod(batch)) if len(batch) else 1
         pot = torch.empty(batch, dtype=torch.float32, device=x.device)
         grad = torch.empty_like(x)
-        _lib.call("mocat_gmm_potential_grad", _lib.ptr(x), n, self.dim, self.n_components, _lib.ptr(self.means),
-                  _lib.ptr(self.lam), _lib.ptr(self.logc), _lib.ptr(pot), _lib.ptr(grad), _lib.stream_handle())
+        if self.full:
+            _lib.call("mocat_gmm_full_potential_grad", _lib.ptr(x), n, self.dim, self.n_components,
+                      _lib.ptr(self.means), _lib.ptr(self.S), _lib.ptr(self.logc), _lib.ptr(pot), _lib.ptr(grad),
+                      _lib.stream_handle())
+        else:
+            _lib.call("mocat_gmm_potential_grad", _lib.ptr(x), n, self.dim, self.n_components, _lib.ptr(self.means),
+                      _lib.ptr(self.lam), _lib.ptr(self.logc), _lib.ptr(pot), _lib.ptr(grad), _lib.stream_handle())
         zeros = torch.zeros_like(pot)
         return zeros, pot, torch.zeros_like(grad), grad
 

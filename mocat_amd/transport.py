@@ -374,8 +374,8 @@ class SVGD(TransportSampler):
         extra.bandwidth = self._adapt(x)
         from .kernels import _workspace
         from .scenarios import GaussianMixture
-        if (isinstance(scenario, GaussianMixture) and not stack and not callable(self.parameters.stepsize)
-                and n >= 3):
+        if (isinstance(scenario, GaussianMixture) and not scenario.full and not stack
+                and not callable(self.parameters.stepsize) and n >= 3):
             return self._run_gmm_fused(scenario, state, extra, T, summary, t0)
         ws = _workspace(n, x.shape[1], self._js(n), x.device)
         stacks = {k: [getattr(state, k).clone()] for k in stack}

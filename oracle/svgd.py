@@ -9,6 +9,10 @@ adagrad: jax.example_libraries.optimizers.adagrad(step, momentum=0.9) restated (
   (first step, invariance to gradient scale, constant-gradient momentum sum; tests/test_oracle_svgd.py).
 gmm: mocat's GaussianMixture (mocat/src/scenarios/toy_examples.py:54-116) with diagonal covariances,
   in log-sum-exp form; det_convention "mocat" weights components by det(P), "textbook" by sqrt(det P).
+gmm_full: the same with full covariances. mocat's component potential (toy_examples.py:95-100) is
+  0.5 |(x - mu) @ L^T|^2 - log(w det(P) / (2 pi)^(d/2)) with L = cholesky(inv(Sigma)) lower, i.e. the
+  quadratic form of L^T L (which equals P only when the factors commute, e.g. diagonal Sigma);
+  "mocat" keeps that form and det(P), "textbook" uses P and sqrt(det P) (a normalised mixture).
 median / mean bandwidth: mocat/src/kernels.py:220-229 with numpy's median of the full distance matrix.
 Pinned by mocat's kernel and GaussianMixture known-answer tests (mocat/src/tests/test_kernels.py:16-28,
 mocat/src/tests/test_toy_scenarios.py:59-106) and a brute-force check of the GEMM form of phi.
@@ -68,6 +72,36 @@ def gmm_potential_grad(x, means, var, weights, det_convention="textbook"):
     r = np.exp(q - lse[:, None])
     grad = np.sum(r[:, :, None] * lam[None] * (x[:, None, :] - mu[None]), axis=1)
     return -lse, grad
+
+
+def gmm_full_matrices(covs, weights, det_convention="textbook"):
+    """(S_k, logc_k): component k's potential is 0.5 r^T S_k r - logc_k with r = x - mu_k."""
+    covs = np.asarray(covs, np.float64)
+    K, d, _ = covs.shape
+    P = np.linalg.inv(covs)
+    L = np.linalg.cholesky(0.5 * (P + np.transpose(P, (0, 2, 1))))
+    detP = np.linalg.det(P)
+    w = np.asarray(weights, np.float64)
+    if det_convention == "mocat":
+        S = np.einsum("kji,kjl->kil", L, L)          # L^T L
+        logc = np.log(w * detP) - 0.5 * d * np.log(2 * np.pi)
+    else:
+        S = P
+        logc = np.log(w) + 0.5 * np.log(detP) - 0.5 * d * np.log(2 * np.pi)
+    return S, logc
+
+
+def gmm_full_potential_grad(x, means, covs, weights, det_convention="textbook"):
+    x = np.asarray(x, np.float64)
+    mu = np.asarray(means, np.float64)
+    S, logc = gmm_full_matrices(covs, weights, det_convention)
+    r = x[:, None, :] - mu[None]                         # [n][K][d]
+    Sr = np.einsum("kij,nkj->nki", S, r)
+    q = logc[None, :] - 0.5 * np.sum(r * Sr, axis=2)
+    m = q.max(1, keepdims=True)
+    lse = m[:, 0] + np.log(np.exp(q - m).sum(1))
+    wk = np.exp(q - lse[:, None])
+    return -lse, np.sum(wk[:, :, None] * Sr, axis=1)
 
 
 def distance_matrix(x):

@@ -33,6 +33,63 @@ def test_gmm_potential_grad(m, conv):
     np.testing.assert_allclose(grad.cpu().numpy(), ograd, rtol=1e-4, atol=1e-3)
 
 
+def _spd(rs, K, d):
+    A = rs.normal(0, 1, (K, d, d))
+    return np.einsum("kij,klj->kil", A, A) / d + 0.5 * np.eye(d)[None]
+
+
+@pytest.mark.parametrize("conv", ["textbook", "mocat"])
+@pytest.mark.parametrize("n,d,K", [(1000, 7, 3), (513, 100, 4), (65, 128, 8), (3, 1, 2)])
+def test_gmm_full_covariance_vs_fp64(m, conv, n, d, K):
+    # mocat's GaussianMixture with full covariances (toy_examples.py:73-103), both det/quadratic conventions
+    rs = np.random.default_rng(d + K)
+    mu = rs.normal(0, 1.5, (K, d))
+    covs = _spd(rs, K, d)
+    w = rs.dirichlet(np.ones(K))
+    sc = m.GaussianMixture(mu, covs, w, det_convention=conv)
+    assert sc.full == (d > 1)  # 1 x 1 covariances are diagonal: the diagonal kernel
+    x = rs.normal(0, 1.5, (n, d)).astype(np.float32)
+    _, pot, _, grad = sc.evaluate(torch.tensor(x, device="cuda"))
+    opot, ograd = osv.gmm_full_potential_grad(x, mu, covs, w, conv)
+    scale = np.abs(opot) + d
+    assert np.max(np.abs(pot.cpu().numpy() - opot) / scale) < 1e-5
+    gscale = np.abs(ograd).max(1, keepdims=True) + 1e-3
+    assert np.max(np.abs(grad.cpu().numpy() - ograd) / gscale) < 5e-5
+
+
+def test_gmm_full_covariance_samplers_follow_user_scenario(m):
+    # the full-covariance mixture through HMC (the iteration in pieces) and SVGD (stepwise path) equals
+    # the same potential written as a user scenario in torch, on the same keys
+    rs = np.random.default_rng(5)
+    K, d = 3, 6
+    mu = rs.normal(0, 2, (K, d))
+    covs = _spd(rs, K, d)
+    w = rs.dirichlet(np.ones(K))
+    fused = m.GaussianMixture(mu, covs, w)
+    S, logc, mut = fused.S, fused.logc, fused.means
+
+    class UserMixture(m.Scenario):
+        name = "user mixture"
+        dim = d
+
+        def prior_potential(self, x, random_key=None):
+            return torch.sum(0.0 * x)
+
+        def likelihood_potential(self, x, random_key=None):
+            r = x - mut
+            return -torch.logsumexp(logc - 0.5 * torch.einsum("ki,kij,kj->k", r, S, r), 0)
+
+    user = UserMixture()
+    hmc = [m.run(sc, m.Underdamped(stepsize=0.2, leapfrog_steps=3), 20, m.random.PRNGKey(2), n_chains=64,
+                 correction=m.Metropolis()) for sc in (fused, user)]
+    same = (hmc[0].value - hmc[1].value).abs().amax(dim=(0, 2)) < 1e-4
+    assert same.float().mean().item() >= 0.9
+    x0 = torch.tensor(rs.normal(0, 1, (256, d)), dtype=torch.float32, device="cuda")
+    sv = [m.run(sc, m.SVGD(stepsize=0.1, max_iter=4, bandwidth_update="median"), 256, m.random.PRNGKey(1),
+                initial_state=m.cdict(value=x0.clone()), stack=("value",)).value for sc in (fused, user)]
+    torch.testing.assert_close(sv[0], sv[1], rtol=1e-4, atol=1e-4)
+
+
 def test_gmm_reference_kats_on_gpu(m):
     sc = m.GaussianMixture(np.arange(15.0).reshape(5, 3))
     x = torch.tensor([[0.0, 0, 0], [2.0, 2, 2]], device="cuda")
