@@ -22,7 +22,7 @@ extern "C" {
 #endif
 
 /* Returns the ABI revision of the library (not a status). */
-int mocat_abi_version(void); /* this header: 3 */
+int mocat_abi_version(void); /* this header: 4 */
 
 /* ---------------------------------------------------------------------------------------- RNG
  * jax.random.split / _random_bits / uniform / normal as called throughout mocat, e.g.
@@ -119,7 +119,7 @@ int mocat_hmc_correct(int64_t C, int d, int64_t iter, int correction, float targ
  *   [m0(d), L0(dxd), A(dxd), LQ(dxd), H(dyxd), SR(dyxdy), nlz] with SR = inv(chol(R)) and
  *   nlz = (dy log 2pi - log det R^-1)/2 as mocat's reset_covariance (mocat/src/utils.py:247-262);
  * model 1: 1-D StochasticVolatility (stochastic_volatility.py:61-108), params
- *   [m0, sqrt(P0), phi, sigma, 1/beta], unit time steps;
+ *   [m0, sqrt(P0), phi, sigma, 1/beta]; observation times enter through step_scale (below);
  * model 2: TimeHomogenousLinearGaussian under OptimalNonLinearGaussianParticleFilter
  *   (mocat/src/ssm/nonlinear_gaussian.py:134-281), params [m0(d), H(dyxd), K0(dxdy), C0(dxd),
  *   A(dxd), K(dxdy), S(dxd), WC(dyxdy), nlzw] as its startup (:154-184); initial weights zero;
@@ -161,10 +161,14 @@ int mocat_pf_param_count(int model, int d, int dy, int64_t* count);
 /* host_filters and filters hold the same n_filters descriptors: host_filters in host memory (read by
  * the argument checks, not retained after return), filters in device memory (read by the kernel).
  * Every filter not flagged MOCAT_PF_LARGE needs n >= 1, T >= 1, non-negative offsets and, when
- * out_ess or out_loglik_t is given, t_offset >= 0. */
+ * out_ess or out_loglik_t is given, t_offset >= 0.
+ * step_scale (NULL: unit time steps) holds, per row of y (index y_offset / dy + t), the noise scale of
+ * the step into row t for the stochastic-volatility model (model 1): sqrt(t_t - t_{t-1}), mocat's
+ * transition_sample (mocat/src/ssm/scenarios/stochastic_volatility.py:86-93); the linear-Gaussian
+ * models are time homogeneous and do not read it. */
 int mocat_pf_run(int model, int d, int dy, const mocat_pf_filter* host_filters, const mocat_pf_filter* filters,
                  int n_filters, const float* params,
-                 const float* y, float* x_ws0, float* x_ws1, float* lw_ws0, float* lw_ws1, uint64_t* csum_ws,
+                 const float* y, const float* step_scale, float* x_ws0, float* x_ws1, float* lw_ws0, float* lw_ws1, uint64_t* csum_ws,
                  int32_t* anc_ws, int resampling, float ess_threshold, float* out_value, float* out_lw,
                  int32_t* out_anc, float* out_ess, double* out_loglik_t, double* out_loglik, float* tan_ws0,
                  float* tan_ws1, double* out_grad, hipStream_t stream);
@@ -185,7 +189,7 @@ int mocat_pf_run(int model, int d, int dy, const mocat_pf_filter* host_filters, 
 int mocat_pf_large_workspace_bytes(const mocat_pf_filter* host_filters, int n_filters, int64_t* bytes);
 int mocat_pf_run_large(int model, int d, int dy, const mocat_pf_filter* host_filters, int n_filters,
                        const float* params,
-                       const float* y, float* x_ws0, float* x_ws1, float* lw_ws0, float* lw_ws1, uint64_t* csum_ws,
+                       const float* y, const float* step_scale, float* x_ws0, float* x_ws1, float* lw_ws0, float* lw_ws1, uint64_t* csum_ws,
                        int resampling, float ess_threshold, float* out_value, float* out_lw, int32_t* out_anc,
                        float* out_ess, double* out_loglik_t, double* out_loglik, float* tan_ws0, float* tan_ws1,
                        double* out_grad, void* workspace, int64_t workspace_bytes, int launch_mode,
@@ -202,9 +206,11 @@ int mocat_pf_run_large(int model, int d, int dy, const mocat_pf_filter* host_fil
  *   2: keys[0]), log_weight[i] = the intermediate log-weight at y;
  * phase 2 = StateSpaceModel.likelihood_potential (mocat/src/ssm/ssm.py:73-81) of x_prev[i] at y into
  *   log_weight[i] (models 0 and 1; x and keys unused).
- * x_prev, x [n][d], y [dy], keys [n][2] (model 2: keys[0] only), log_weight [n]. */
+ * x_prev, x [n][d], y [dy], keys [n][2] (model 2: keys[0] only), log_weight [n]; step_scale: the
+ * step's noise scale for model 1 in phase 1 (sqrt(t_new - t_previous); 1 for unit steps). */
 int mocat_pf_hook(int model, int d, int dy, int phase, const float* params, const float* x_prev, const float* y,
-                  const uint32_t* keys, int64_t n, float* x, float* log_weight, hipStream_t stream);
+                  const uint32_t* keys, float step_scale, int64_t n, float* x, float* log_weight,
+                  hipStream_t stream);
 
 /* Ancestor indices idx[n] from non-negative weights (from_log = 0) or log-weights (from_log = 1,
  * weights exp(w - max w)): method 0 systematic (u = uniform(key)), 1 multinomial

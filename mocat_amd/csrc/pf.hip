@@ -76,6 +76,7 @@ MOCAT_DEV void row_fence() {
 template <int D, int DY>
 struct Lgssm {
   static constexpr bool kOptimal = false;
+  static constexpr bool kTimeScaled = false;  // time homogeneous: the observation times are not read
   static constexpr int NPARAM = D + 3 * D * D + DY * D + DY * DY + 1;
   float m0[D], L0[D * D], A[D * D], LQ[D * D], H[DY * D], SR[DY * DY], nlz;
   MOCAT_DEV void load(const float* p) {
@@ -152,6 +153,8 @@ struct Lgssm {
 //       = 0.5 (y s)^2 + 0.5 log 2pi - log s,  s = inv_beta * exp(-x/2)
 struct Sv {
   static constexpr bool kOptimal = false;
+  // the transition noise scales with sqrt(t_new - t_previous) (stochastic_volatility.py:86-93)
+  static constexpr bool kTimeScaled = true;
   static constexpr int NPARAM = 5;
   float m0, L0, phi, sigma, inv_beta;
   MOCAT_DEV void load(const float* p) {
@@ -163,6 +166,11 @@ struct Sv {
   }
   MOCAT_DEV void init(const float* eps, float* x) const { x[0] = eps[0] * L0 + m0; }
   MOCAT_DEV void transition(const float* xp, const float* eps, float* x) const { x[0] = xp[0] * phi + eps[0] * sigma; }
+  // with the step's noise scale s = sqrt(t_new - t_previous): x' = phi x + (sigma eps) s, the product in
+  // mocat's order (normal @ cov_sqrt.T * sqrt(dt)); s = 1 gives transition() bit for bit
+  MOCAT_DEV void transition_s(const float* xp, const float* eps, float* x, float s) const {
+    x[0] = xp[0] * phi + (eps[0] * sigma) * s;
+  }
   // one expression for both paths, so that computing the gradient leaves the log-weights bit-identical
   MOCAT_DEV float lik(const float* x, const float* y) const {
     float dldx, dldb;
@@ -189,6 +197,7 @@ struct Sv {
 template <int D, int DY>
 struct LgssmOpt {
   static constexpr bool kOptimal = true;
+  static constexpr bool kTimeScaled = false;
   static constexpr int NPARAM = D + 3 * DY * D + 3 * D * D + DY * DY + 1;
   float m0[D], H[DY * D], K0[D * DY], C0[D * D], A[D * D], K[D * DY], S[D * D], WC[DY * DY], nlzw;
   MOCAT_DEV void load(const float* p) {
@@ -494,6 +503,7 @@ struct PfArgs {
   int n_filters;
   const float* params;
   const float* y;
+  const float* step_scale;  // [rows of y] noise scale of the step into row t (time-scaled models), or NULL
   float* x0;       // workspace [sum n][D]
   float* x1;       // workspace [sum n][D]
   float* lw0;      // workspace [sum n]
@@ -628,6 +638,9 @@ __global__ __launch_bounds__(NT) void pf_kernel(PfArgs a) {
       mloc = -INFINITY;
       bad0 = 0;
       const Key ik0 = Model::kOptimal ? split_at(ik, (uint32_t)n, 0u) : Key{0u, 0u};
+      float sc = 1.f;
+      if constexpr (Model::kTimeScaled)
+        if (a.step_scale) sc = a.step_scale[F.y_offset / DY + t];
       for (int64_t i = tid; i < n; i += NT) {
         reload_model_each_particle();
         const int64_t src = res ? (int64_t)anc[i] : i;
@@ -650,11 +663,11 @@ __global__ __launch_bounds__(NT) void pf_kernel(PfArgs a) {
         if constexpr (Model::kOptimal) {
           l = (res ? 0.f : lwa[i]) - model.step_opt(xp, eps, ycur, x);
         } else if constexpr (GRAD) {
-          model.transition(xp, eps, x);
+          model.transition_s(xp, eps, x, sc);
           float dldx, dldb;
           l = (res ? 0.f : lwa[i]) - model.lik_grad(x, ycur, dldx, dldb);
           const float dxp = xp[0] + model.phi * ta[src * 5 + 0];
-          const float dxs = eps[0] + model.phi * ta[src * 5 + 1];
+          const float dxs = eps[0] * sc + model.phi * ta[src * 5 + 1];
           const float d0 = (res ? 0.f : ta[i * 5 + 2]) - dldx * dxp;
           const float d1 = (res ? 0.f : ta[i * 5 + 3]) - dldx * dxs;
           const float d2 = (res ? 0.f : ta[i * 5 + 4]) - dldb;
@@ -665,7 +678,10 @@ __global__ __launch_bounds__(NT) void pf_kernel(PfArgs a) {
           tb[i * 5 + 4] = d2;
 
         } else {
-          model.transition(xp, eps, x);
+          if constexpr (Model::kTimeScaled)
+            model.transition_s(xp, eps, x, sc);
+          else
+            model.transition(xp, eps, x);
           l = (res ? 0.f : lwa[i]) - model.lik(x, ycur);
         }
         #pragma unroll
@@ -845,6 +861,7 @@ struct LfArgs {
   int64_t n_chunks;
   const float* params;
   const float* y;
+  const float* step_scale;
   float* x0;
   float* x1;
   float* lw0;
@@ -1597,9 +1614,17 @@ __global__ __launch_bounds__(LT) void lf_step_kernel(LfArgs a, int64_t t, int ce
     for (int pk = 0; pk < PPT; ++pk)
       if (live[pk]) l[pk] = lwp[pk] - model.step_opt(xpre[pk], eps[pk], ycur, xn[pk]);
   } else {
+  float sc = 1.f;
+  if constexpr (Model::kTimeScaled)
+    if (a.step_scale) sc = a.step_scale[F.y_offset / DY + t];
 #pragma unroll
   for (int pk = 0; pk < PPT; ++pk)
-    if (live[pk]) model.transition(xpre[pk], eps[pk], xn[pk]);
+    if (live[pk]) {
+      if constexpr (Model::kTimeScaled)
+        model.transition_s(xpre[pk], eps[pk], xn[pk], sc);
+      else
+        model.transition(xpre[pk], eps[pk], xn[pk]);
+    }
 #pragma unroll
   for (int pk = 0; pk < PPT; ++pk) {
     if (!live[pk]) continue;
@@ -1607,7 +1632,7 @@ __global__ __launch_bounds__(LT) void lf_step_kernel(LfArgs a, int64_t t, int ce
       float dldx, dldb;
       l[pk] = lwp[pk] - model.lik_grad(xn[pk], ycur, dldx, dldb);
       const float dxp = xpre[pk][0] + model.phi * tg[pk][0];
-      const float dxs = eps[pk][0] + model.phi * tg[pk][1];
+      const float dxs = eps[pk][0] * sc + model.phi * tg[pk][1];
       gd[pk][0] = tg[pk][2] - dldx * dxp;
       gd[pk][1] = tg[pk][3] - dldx * dxs;
       gd[pk][2] = tg[pk][4] - dldb;
@@ -1687,8 +1712,8 @@ __global__ __launch_bounds__(LT) void lf_finish_kernel(LfArgs a) {
 template <class Model, int D, int DY>
 __global__ __launch_bounds__(256) void pf_hook_kernel(int phase, const float* __restrict__ params,
                                                       const float* __restrict__ xprev, const float* __restrict__ y,
-                                                      const uint32_t* __restrict__ keys, int64_t n, float* xout,
-                                                      float* lw) {
+                                                      const uint32_t* __restrict__ keys, float step_scale, int64_t n,
+                                                      float* xout, float* lw) {
   __shared__ Model model;
   load_shared_params(model, params);
   __syncthreads();
@@ -1731,7 +1756,10 @@ __global__ __launch_bounds__(256) void pf_hook_kernel(int phase, const float* __
       if constexpr (Model::kOptimal) {
         l = -model.step_opt(xp, eps, y, x);
       } else {
-        model.transition(xp, eps, x);
+        if constexpr (Model::kTimeScaled)
+          model.transition_s(xp, eps, x, step_scale);
+        else
+          model.transition(xp, eps, x);
         l = -model.lik(x, y);
       }
     }
@@ -1743,10 +1771,10 @@ __global__ __launch_bounds__(256) void pf_hook_kernel(int phase, const float* __
 
 template <class Model, int D, int DY>
 int launch_pf_hook(int phase, const float* params, const float* xprev, const float* y, const uint32_t* keys,
-                   int64_t n, float* x, float* lw, hipStream_t st) {
+                   float step_scale, int64_t n, float* x, float* lw, hipStream_t st) {
   const int64_t blocks = (n + 255) / 256;
   hipLaunchKernelGGL((pf_hook_kernel<Model, D, DY>), dim3((unsigned)(blocks < 8192 ? blocks : 8192)), dim3(256), 0,
-                     st, phase, params, xprev, y, keys, n, x, lw);
+                     st, phase, params, xprev, y, keys, step_scale, n, x, lw);
   MOCAT_LAUNCH_CHECK();
   return MOCAT_OK;
 }
@@ -1927,7 +1955,7 @@ int mocat_pf_param_count(int model, int d, int dy, int64_t* count) {
 
 int mocat_pf_run(int model, int d, int dy, const mocat_pf_filter* host_filters, const mocat_pf_filter* filters,
                  int n_filters, const float* params,
-                 const float* y, float* x_ws0, float* x_ws1, float* lw_ws0, float* lw_ws1, uint64_t* csum_ws,
+                 const float* y, const float* step_scale, float* x_ws0, float* x_ws1, float* lw_ws0, float* lw_ws1, uint64_t* csum_ws,
                  int32_t* anc_ws, int resampling, float ess_threshold, float* out_value, float* out_lw,
                  int32_t* out_anc, float* out_ess, double* out_loglik_t, double* out_loglik, float* tan_ws0,
                  float* tan_ws1, double* out_grad, hipStream_t stream) {
@@ -1948,7 +1976,7 @@ int mocat_pf_run(int model, int d, int dy, const mocat_pf_filter* host_filters, 
     if (resampling == 2 && F.n * F.n >= 0xFFFFFFFFll) return MOCAT_ERR_SIZE;
     if (model == 2 && F.n * (int64_t)d > 0xFFFFFFFFll) return MOCAT_ERR_SIZE;
   }
-  PfArgs a{filters, n_filters, params, y, x_ws0, x_ws1, lw_ws0, lw_ws1, csum_ws, anc_ws, resampling,
+  PfArgs a{filters, n_filters, params, y, step_scale, x_ws0, x_ws1, lw_ws0, lw_ws1, csum_ws, anc_ws, resampling,
            ess_threshold, out_value, out_lw, out_anc, out_ess, out_loglik_t, out_loglik, tan_ws0, tan_ws1, out_grad};
   if (model == 1) {
     if (d != 1 || dy != 1) return MOCAT_ERR_ARG;
@@ -1976,7 +2004,7 @@ int mocat_pf_large_workspace_bytes(const mocat_pf_filter* host_filters, int n_fi
 }
 
 int mocat_pf_run_large(int model, int d, int dy, const mocat_pf_filter* host_filters, int n_filters,
-                       const float* params, const float* y, float* x_ws0, float* x_ws1, float* lw_ws0, float* lw_ws1, uint64_t* csum_ws,
+                       const float* params, const float* y, const float* step_scale, float* x_ws0, float* x_ws1, float* lw_ws0, float* lw_ws1, uint64_t* csum_ws,
                        int resampling, float ess_threshold, float* out_value, float* out_lw, int32_t* out_anc,
                        float* out_ess, double* out_loglik_t, double* out_loglik, float* tan_ws0, float* tan_ws1,
                        double* out_grad, void* workspace, int64_t workspace_bytes, int launch_mode,
@@ -2019,7 +2047,7 @@ int mocat_pf_run_large(int model, int d, int dy, const mocat_pf_filter* host_fil
       hipStreamSynchronize(stream) != hipSuccess)  // the host vectors are released on return
     return MOCAT_ERR_HIP;
   LfArgs a{reinterpret_cast<const LfDesc*>(ws + L.o_desc), reinterpret_cast<const int32_t*>(ws + L.o_cf), L.n_large,
-           L.n_chunks, params, y, x_ws0, x_ws1, lw_ws0, lw_ws1, csum_ws, tan_ws0, tan_ws1, resampling,
+           L.n_chunks, params, y, step_scale, x_ws0, x_ws1, lw_ws0, lw_ws1, csum_ws, tan_ws0, tan_ws1, resampling,
            ess_threshold, out_value, out_lw, out_anc, out_ess, out_loglik_t, out_loglik, out_grad,
            reinterpret_cast<float*>(ws + L.o_m), reinterpret_cast<double*>(ws + L.o_s1),
            reinterpret_cast<double*>(ws + L.o_s2), reinterpret_cast<double*>(ws + L.o_g),
@@ -2047,8 +2075,9 @@ int mocat_pf_run_large(int model, int d, int dy, const mocat_pf_filter* host_fil
 }
 
 int mocat_pf_hook(int model, int d, int dy, int phase, const float* params, const float* x_prev, const float* y,
-                  const uint32_t* keys, int64_t n, float* x, float* log_weight, hipStream_t stream) {
-  if (!params || !log_weight || n < 0 || phase < 0 || phase > 2) return MOCAT_ERR_ARG;
+                  const uint32_t* keys, float step_scale, int64_t n, float* x, float* log_weight,
+                  hipStream_t stream) {
+  if (!params || !log_weight || n < 0 || phase < 0 || phase > 2 || !(step_scale >= 0.f)) return MOCAT_ERR_ARG;
   if (n == 0) return MOCAT_OK;
   if (phase != 2 && (!keys || !x)) return MOCAT_ERR_ARG;
   if (phase != 0 && !x_prev) return MOCAT_ERR_ARG;
@@ -2057,18 +2086,18 @@ int mocat_pf_hook(int model, int d, int dy, int phase, const float* params, cons
   if (n > 0x7FFFFFFFll || (model == 2 && n * (int64_t)d > 0xFFFFFFFFll)) return MOCAT_ERR_SIZE;
   if (model == 1) {
     if (d != 1 || dy != 1) return MOCAT_ERR_ARG;
-    return launch_pf_hook<Sv, 1, 1>(phase, params, x_prev, y, keys, n, x, log_weight, stream);
+    return launch_pf_hook<Sv, 1, 1>(phase, params, x_prev, y, keys, step_scale, n, x, log_weight, stream);
   }
   if (model == 2) {
 #define MOCAT_PF_CASE(DD, YY) \
-  if (d == DD && dy == YY) return launch_pf_hook<LgssmOpt<DD, YY>, DD, YY>(phase, params, x_prev, y, keys, n, x, log_weight, stream);
+  if (d == DD && dy == YY) return launch_pf_hook<LgssmOpt<DD, YY>, DD, YY>(phase, params, x_prev, y, keys, step_scale, n, x, log_weight, stream);
     MOCAT_PF_LG_SHAPES
 #undef MOCAT_PF_CASE
     return MOCAT_ERR_ARG;
   }
   if (model != 0) return MOCAT_ERR_ARG;
 #define MOCAT_PF_CASE(DD, YY) \
-  if (d == DD && dy == YY) return launch_pf_hook<Lgssm<DD, YY>, DD, YY>(phase, params, x_prev, y, keys, n, x, log_weight, stream);
+  if (d == DD && dy == YY) return launch_pf_hook<Lgssm<DD, YY>, DD, YY>(phase, params, x_prev, y, keys, step_scale, n, x, log_weight, stream);
   MOCAT_PF_LG_SHAPES
 #undef MOCAT_PF_CASE
   return MOCAT_ERR_ARG;

@@ -26,12 +26,13 @@ RESAMPLING = {"systematic": 0, "multinomial": 1, "categorical": 2}
 _P = _lib.c_ptr
 _lib.register("mocat_pf_param_count", [_lib.c_i32, _lib.c_i32, _lib.c_i32, _P])
 _lib.register("mocat_pf_run", [_lib.c_i32, _lib.c_i32, _lib.c_i32, _P, _P, _lib.c_i32, _P, _P, _P, _P, _P, _P, _P, _P,
-                               _lib.c_i32, _lib.c_f32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P])
+                               _P, _lib.c_i32, _lib.c_f32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P])
 _lib.register("mocat_pf_large_workspace_bytes", [_P, _lib.c_i32, _P])
 _lib.register("mocat_pf_run_large", [_lib.c_i32, _lib.c_i32, _lib.c_i32, _P, _lib.c_i32, _P, _P, _P, _P, _P, _P, _P,
-                                     _lib.c_i32, _lib.c_f32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _lib.c_i64,
+                                     _P, _lib.c_i32, _lib.c_f32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _lib.c_i64,
                                      _lib.c_i32, _P])
-_lib.register("mocat_pf_hook", [_lib.c_i32, _lib.c_i32, _lib.c_i32, _lib.c_i32, _P, _P, _P, _P, _lib.c_i64, _P, _P, _P])
+_lib.register("mocat_pf_hook", [_lib.c_i32, _lib.c_i32, _lib.c_i32, _lib.c_i32, _P, _P, _P, _P, _lib.c_f32, _lib.c_i64,
+                                _P, _P, _P])
 _lib.register("mocat_resample", [_P, _lib.c_i64, _lib.c_i32, _lib.c_i32, _P, _P, _P, _P])
 _lib.register("mocat_resample_noalloc", [_P, _lib.c_i64, _lib.c_i32, _lib.c_i32, _P, _P, _P])
 _lib.register("mocat_resample_categorical", [_P, _lib.c_i64, _lib.c_i64, _P, _P, _P])
@@ -51,7 +52,7 @@ assert FILTER_DTYPE.itemsize == 64
 
 
 HOOK_ERROR = ("{what}: the particle-filter hooks run on compiled models only (TimeHomogenousLinearGaussian and "
-              "StochasticVolatility with unit time steps under BootstrapFilter; TimeHomogenousLinearGaussian under "
+              "StochasticVolatility under BootstrapFilter; TimeHomogenousLinearGaussian under "
               "OptimalNonLinearGaussianParticleFilter). A StateSpaceModel or ParticleFilter written in Python has "
               "no kernel on the fused path.")
 
@@ -83,13 +84,35 @@ def _hook_target(particle_filter, ssm_scenario):
     return km, ssm_scenario.kernel_params()
 
 
-def _unit_step(ssm_scenario, t_previous, t_new):
-    if getattr(ssm_scenario, "kernel_model", None) == 1 and t_previous is not None and t_new is not None:
-        if abs(float(t_new) - float(t_previous) - 1.0) > 1e-6:
-            raise NotImplementedError("the stochastic-volatility kernels assume unit time steps")
+def _step_scale(ssm_scenario, t_previous, t_new) -> float:
+    """The transition's noise scale for a step from t_previous to t_new: sqrt(t_new - t_previous) for
+    the stochastic-volatility model (mocat/src/ssm/scenarios/stochastic_volatility.py:86-93), 1 for the
+    time-homogeneous linear-Gaussian models and when either time is not given."""
+    if getattr(ssm_scenario, "kernel_model", None) != 1 or t_previous is None or t_new is None:
+        return 1.0
+    dt = float(t_new) - float(t_previous)
+    if not dt >= 0.0:
+        raise ValueError("observation times must not decrease")
+    return float(np.sqrt(np.float32(dt)))
 
 
-def _hook(model, params, ssm_scenario, phase, x_prev, y, keys, n):
+def _scale_rows(ssm_scenario, t, T):
+    """Per-row noise scales of one filter (row t: the step into observation t; row 0 unused), or None
+    for unit steps or a time-homogeneous model."""
+    if getattr(ssm_scenario, "kernel_model", None) != 1 or t is None:
+        return None
+    tn = np.asarray(t.detach().cpu().numpy() if isinstance(t, torch.Tensor) else t, np.float64).reshape(-1)
+    if len(tn) != T:
+        raise ValueError(f"expected {T} observation times, got {len(tn)}")
+    dt = np.diff(tn)
+    if np.any(dt < 0):
+        raise ValueError("observation times must not decrease")
+    rows = np.ones(T, np.float32)
+    rows[1:] = np.sqrt(dt.astype(np.float32))
+    return None if np.all(rows == 1.0) else rows
+
+
+def _hook(model, params, ssm_scenario, phase, x_prev, y, keys, n, step_scale=1.0):
     d, dy = int(ssm_scenario.dim), int(ssm_scenario.dim_obs)
     dev = mr.default_device()
     p = torch.from_numpy(np.ascontiguousarray(params, np.float32)).to(dev)
@@ -102,8 +125,8 @@ def _hook(model, params, ssm_scenario, phase, x_prev, y, keys, n):
         raise ValueError(f"expected {n} keys, got {kt.shape[0]}")
     x = torch.empty((n, d), dtype=torch.float32, device=dev) if phase != 2 else None
     lw = torch.empty(n, dtype=torch.float32, device=dev)
-    _lib.call("mocat_pf_hook", model, d, dy, phase, _lib.ptr(p), _lib.ptr(xp), _lib.ptr(yt), _lib.ptr(kt), int(n),
-              _lib.ptr(x), _lib.ptr(lw), _lib.stream_handle())
+    _lib.call("mocat_pf_hook", model, d, dy, phase, _lib.ptr(p), _lib.ptr(xp), _lib.ptr(yt), _lib.ptr(kt),
+              float(step_scale), int(n), _lib.ptr(x), _lib.ptr(lw), _lib.stream_handle())
     return x, lw
 
 
@@ -144,10 +167,10 @@ class ParticleFilter:
         """filtering.py:124-139: (x_new (n, d), intermediate log-weights (n,)) from x_previous (n, d) and
         one key per particle."""
         model, params = _hook_target(self, ssm_scenario)
-        _unit_step(ssm_scenario, t_previous, t_new)
+        sc = _step_scale(ssm_scenario, t_previous, t_new)
         xp = torch.as_tensor(x_previous)
         n = int(xp.shape[0]) if xp.dim() > 1 else 1
-        return _hook(model, params, ssm_scenario, 1, xp, y_new, random_keys, n)
+        return _hook(model, params, ssm_scenario, 1, xp, y_new, random_keys, n, sc)
 
 
 class BootstrapFilter(ParticleFilter):
@@ -342,7 +365,7 @@ def _large_flags(ns, Ts, large_n):
 
 
 def _launch(models, ys, keys, ns, ess_threshold, resampling, store, grad=False, large_n=None, particle_filter=None,
-            initial=None, extra_flags=0, graph=False):
+            initial=None, extra_flags=0, graph=False, ts=None):
     """Runs len(models) filters (same model class and dims): filters below large_n particles in one
     persistent launch (a workgroup per filter), the others through the chunked per-step path (its
     launches captured into one HIP graph when graph=True)."""
@@ -363,7 +386,7 @@ def _launch(models, ys, keys, ns, ess_threshold, resampling, store, grad=False, 
     order = sorted(range(F), key=lambda i: -(ns[i] * Ts[i]))  # longest first
     large = _large_flags(ns, Ts, large_n)
     desc = np.zeros(F, FILTER_DTYPE)
-    params, yflat = [], []
+    params, yflat, scales = [], [], []
     poff = yoff = soff = ooff = toff = 0
     for slot, i in enumerate(order):
         p = opt.kernel_params(models[i]) if optimal else models[i].kernel_params()
@@ -373,6 +396,7 @@ def _launch(models, ys, keys, ns, ess_threshold, resampling, store, grad=False, 
                       (PF_LARGE if large[i] else 0) | (PF_RESUME if resume else 0) | extra_flags)
         params.append(p)
         yflat.append(yi.ravel())
+        scales.append(_scale_rows(models[i], None if ts is None else ts[i], Ts[i]))
         poff += p.size
         yoff += yi.size
         soff += ns[i]
@@ -384,6 +408,11 @@ def _launch(models, ys, keys, ns, ess_threshold, resampling, store, grad=False, 
     desc_t = torch.from_numpy(desc_host.view(np.uint8).copy()).to(dev)
     params_t = torch.from_numpy(np.concatenate(params)).to(dev)
     y_t = torch.from_numpy(np.concatenate(yflat).astype(np.float32)).to(dev)
+    # per-row noise scales (row index = y_offset / dy + t), only when some filter has non-unit steps
+    scale_t = None
+    if any(sv is not None for sv in scales):
+        scale_t = torch.from_numpy(np.concatenate([np.ones(len(yf) // dy, np.float32) if sv is None else sv
+                                                   for sv, yf in zip(scales, yflat)])).to(dev)
     x0 = torch.empty(soff * d, **f32)
     x1 = torch.empty(soff * d, **f32)
     lw0 = torch.empty(soff, **f32)
@@ -421,7 +450,7 @@ def _launch(models, ys, keys, ns, ess_threshold, resampling, store, grad=False, 
         side.wait_stream(cur)
         with torch.cuda.stream(side):
             _lib.call("mocat_pf_run", kind, d, dy, desc_host.ctypes.data_as(_lib.ctypes.c_void_p), _lib.ptr(desc_t),
-                      F, _lib.ptr(params_t), _lib.ptr(y_t),
+                      F, _lib.ptr(params_t), _lib.ptr(y_t), _lib.ptr(scale_t),
                       _lib.ptr(x0), _lib.ptr(x1), _lib.ptr(lw0), _lib.ptr(lw1), _lib.ptr(cs), _lib.ptr(anc),
                       RESAMPLING[resampling], float(ess_threshold), _lib.ptr(out_value), _lib.ptr(out_lw),
                       _lib.ptr(out_anc), _lib.ptr(out_ess), _lib.ptr(out_llt), _lib.ptr(out_ll), _lib.ptr(tan0),
@@ -431,7 +460,8 @@ def _launch(models, ys, keys, ns, ess_threshold, resampling, store, grad=False, 
         nb = _lib.ctypes.c_int64(0)
         _lib.call("mocat_pf_large_workspace_bytes", hp, F, _lib.ctypes.byref(nb))
         ws = torch.empty(max(int(nb.value), 1), dtype=torch.uint8, device=dev)
-        _lib.call("mocat_pf_run_large", kind, d, dy, hp, F, _lib.ptr(params_t), _lib.ptr(y_t), _lib.ptr(x0),
+        _lib.call("mocat_pf_run_large", kind, d, dy, hp, F, _lib.ptr(params_t), _lib.ptr(y_t), _lib.ptr(scale_t),
+                  _lib.ptr(x0),
                   _lib.ptr(x1), _lib.ptr(lw0), _lib.ptr(lw1), _lib.ptr(cs), RESAMPLING[resampling],
                   float(ess_threshold), _lib.ptr(out_value), _lib.ptr(out_lw), _lib.ptr(out_anc), _lib.ptr(out_ess),
                   _lib.ptr(out_llt), _lib.ptr(out_ll), _lib.ptr(tan0), _lib.ptr(tan1), _lib.ptr(out_grad),
@@ -478,15 +508,13 @@ def run_particle_filter_for_marginals(ssm_scenario, particle_filter, y, t, rando
         enkf = particle_filter if isinstance(particle_filter, ParticleFilter) else particle_filter()
         _nlg_matrices(ssm_scenario)
         return _run_enkf(ssm_scenario, enkf, yn, tn, random_key, int(n))
-    if getattr(ssm_scenario, "kernel_model", None) == 1 and len(tn) > 1 and not np.allclose(np.diff(tn), 1.0):
-        raise NotImplementedError("the stochastic-volatility kernel assumes unit time steps")
     if resampling == "categorical" and n * n >= 0xFFFFFFFF:
         raise ValueError("categorical resampling draws an (n, n) Gumbel matrix: n * n must be < 2^32 - 1")
     if initial_sample is not None:
         return _resume(ssm_scenario, particle_filter, yn, tn, random_key, int(n), initial_sample, ess_threshold,
                        resampling)
     r = _launch([ssm_scenario], [yn], [random_key], [int(n)], ess_threshold, resampling, True,
-                particle_filter=particle_filter)[0]
+                particle_filter=particle_filter, ts=[tn])[0]
     dev = r["value"].device
     out = cdict(value=r["value"], log_weight=r["log_weight"], t=torch.as_tensor(tn, device=dev),
                 y=torch.as_tensor(yn, dtype=torch.float32, device=dev), ess=r["ess"],
@@ -555,8 +583,11 @@ def propagate_particle_filter(ssm_scenario, particle_filter, particles: cdict, y
         particles = resample_particles(particles, random_key, resample_full)
     dy = int(ssm_scenario.dim_obs)
     yv = np.concatenate([np.zeros((1, dy), np.float32), _np2(y_new, dy)[:1].astype(np.float32)])
+    t_last = getattr(particles, "t", None)
+    tv = None if t_last is None else [float(torch.as_tensor(t_last).reshape(-1)[-1]), float(t_new)]
     r = _launch([ssm_scenario], [yv], [random_key], [n], 0.0, "systematic", True, particle_filter=particle_filter,
-                initial=[(particles.value[-1], particles.log_weight[-1])], extra_flags=PF_STEP_KEY)[0]
+                initial=[(particles.value[-1], particles.log_weight[-1])], extra_flags=PF_STEP_KEY,
+                ts=None if tv is None else [tv])[0]
     dev = r["value"].device
     out = particles.copy()
     out.value = torch.cat([particles.value, r["value"][1:]], 0)
@@ -582,8 +613,11 @@ def _resume(ssm_scenario, particle_filter, yn, tn, random_key, n, initial_sample
     x_init = initial_sample.value[-1]
     lw_init = initial_sample.log_weight[-1]
     yfull = np.concatenate([np.zeros((1, dy), yn.dtype), yn])  # row 0 stands for the sample (not read)
+    t_prev = getattr(initial_sample, "t", None)
+    tfull = None if t_prev is None else np.concatenate([[float(torch.as_tensor(t_prev).reshape(-1)[-1])],
+                                                        np.asarray(tn, np.float64).reshape(-1)])
     r = _launch([ssm_scenario], [yfull], [random_key], [n], ess_threshold, resampling, True,
-                particle_filter=particle_filter, initial=[(x_init, lw_init)])[0]
+                particle_filter=particle_filter, initial=[(x_init, lw_init)], ts=None if tfull is None else [tfull])[0]
     dev = r["value"].device
     out = initial_sample.copy()
     cat = lambda a, b: torch.cat([torch.as_tensor(a, device=dev).to(b.dtype), b], 0)
@@ -605,7 +639,7 @@ def _resume(ssm_scenario, particle_filter, yn, tn, random_key, n, initial_sample
 
 def run_particle_filters(ssm_scenarios, ys, random_keys, ns, ess_threshold: float = 0.5,
                          resampling: str = "systematic", store: bool = False, grad: bool = False,
-                         large_n: int = None, particle_filter=None, schedule=None):
+                         large_n: int = None, particle_filter=None, schedule=None, ts=None):
     """Independent bootstrap filters of uneven sizes in one launch. Returns a list of cdicts with
     log_likelihood (FP64), ess (T,), final_value, final_log_weight (and value/log_weight/ancestors when
     store=True). grad=True (stochastic volatility): log_likelihood_grad = d log Z / d(phi, sigma, beta)
@@ -615,7 +649,9 @@ def run_particle_filters(ssm_scenarios, ys, random_keys, ns, ess_threshold: floa
     schedule: None (large_n as given, stream launches), a dict {"large_n": int, "graph": bool}, or
     "auto": the schedule autotuner (mocat_amd.ssm.schedule) times the candidate schedules on this
     batch the first time its shape is seen, records the fastest, and every later call of the same
-    shape uses it."""
+    shape uses it.
+    ts: optional list of observation-time arrays (one per filter, as ys); the stochastic-volatility
+    transition noise then scales with sqrt(t_t - t_{t-1}) (None: unit steps)."""
     if _filter_kind(particle_filter) is EnsembleKalmanFilter:
         raise NotImplementedError("EnsembleKalmanFilter runs one ensemble per call: run_particle_filter_for_marginals")
     ys = [y.detach().cpu().numpy() if isinstance(y, torch.Tensor) else np.asarray(y) for y in ys]
@@ -627,14 +663,15 @@ def run_particle_filters(ssm_scenarios, ys, random_keys, ns, ess_threshold: floa
         from . import schedule as sched
         if schedule == "auto":
             run = lambda c: _launch(models, ys, keys, ns, ess_threshold, resampling, store, grad, c["large_n"],
-                                    particle_filter, graph=c["graph"])
+                                    particle_filter, graph=c["graph"], ts=ts)
             choice = sched.choose(sched.signature(models, ys, ns, resampling, grad, store, particle_filter), run)
         elif isinstance(schedule, dict):
             choice = schedule
         else:
             raise ValueError('schedule must be None, "auto" or a dict {"large_n": int, "graph": bool}')
         large_n, graph = int(choice["large_n"]), bool(choice["graph"])
-    res = _launch(models, ys, keys, ns, ess_threshold, resampling, store, grad, large_n, particle_filter, graph=graph)
+    res = _launch(models, ys, keys, ns, ess_threshold, resampling, store, grad, large_n, particle_filter, graph=graph,
+                  ts=ts)
     return [cdict(**{k: v for k, v in r.items() if k != "desc"}) for r in res]
 
 

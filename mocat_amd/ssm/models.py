@@ -53,14 +53,13 @@ class StateSpaceModel:
 
     def transition_sample(self, x_previous, t_previous, t_new, random_key):
         """ssm.py:53-60: x_new given x_previous, one key per particle."""
-        from .filtering import _hook, _unit_step
+        from .filtering import _hook, _step_scale
         key = mr.as_key(random_key)
         lead = tuple(key.shape[:-1])
         n = int(np.prod(lead)) if lead else 1
         model, params = self._hook_params()
-        _unit_step(self, t_previous, t_new)
         y0 = np.zeros(self.dim_obs, np.float32)  # the weight the hook also returns is not used here
-        x, _ = _hook(model, params, self, 1, x_previous, y0, key, n)
+        x, _ = _hook(model, params, self, 1, x_previous, y0, key, n, _step_scale(self, t_previous, t_new))
         return x.reshape(lead + (self.dim,))
 
     def likelihood_potential(self, x, y, t=None):
@@ -155,7 +154,8 @@ class TimeHomogenousLinearGaussian(StateSpaceModel):
 class StochasticVolatility(StateSpaceModel):
     """1-D stochastic volatility: x_t = phi x_{t-1} + sigma eps, y_t = beta exp(x_t / 2) eta.
     Arguments as mocat's (mocat/src/ssm/scenarios/stochastic_volatility.py:24-57): transition_matrix
-    = phi, transition_covariance = sigma^2, likelihood_covariance_diag = beta^2; unit time steps."""
+    = phi, transition_covariance = sigma^2, likelihood_covariance_diag = beta^2. The transition noise
+    scales with sqrt(t_new - t_previous) (stochastic_volatility.py:86-93; unit steps when no times are given)."""
     name = "Stochastic Volatility"
     kernel_model = 1
 

@@ -67,9 +67,12 @@ def init_state(p, eps):
     return eps @ p["L0"].T + p["m0"]
 
 
-def transition(p, x, eps):
+def transition(p, x, eps, scale=1.0):
+    """One step; for stochastic volatility the noise scales with scale = sqrt(t_new - t_previous)
+    (mocat/src/ssm/scenarios/stochastic_volatility.py:86-93: normal @ cov_sqrt.T * sqrt(dt)); the
+    linear-Gaussian model is time homogeneous."""
     if p.get("sv"):
-        return x * p["phi"] + eps * p["sigma"]
+        return x * p["phi"] + (eps * p["sigma"]) * scale
     return x @ p["A"].T + eps @ p["LQ"].T
 
 
@@ -141,14 +144,16 @@ def resample_indices(w, key, method):
 
 # ----------------------------------------------------------------------------- filter
 def run_pf(p, y, key, n, ess_threshold=0.5, resampling="systematic", dtype=np.float64, ancestors=None,
-           resample_flags=None, initial=None):
+           resample_flags=None, initial=None, t=None):
     """Returns dict(value (T, n, d), log_weight (T, n), ess (T,), log_likelihood, ancestors (T, n)).
 
     ancestors / resample_flags (optional, (T, n) and (T,)): replay given resampling decisions and
     indices instead of drawing them (the fixed-resampling-index path of another run).
     initial = (x (n, d), log_weight (n,)): resume from a sample as run_particle_filter_for_marginals
     with initial_sample (mocat/src/ssm/filtering.py:264-278): row 0 of y is not read, the step keys
-    are split(key, T - 1) (no (key, sub) split) and the log-likelihood starts at 0."""
+    are split(key, T - 1) (no (key, sub) split) and the log-likelihood starts at 0.
+    t (optional, (T,)): observation times; the stochastic-volatility step into row t has noise scale
+    sqrt(t[t] - t[t-1]) (unit steps without t)."""
     y = np.asarray(y, np.float64).reshape(len(y), -1)
     T = y.shape[0]
     d = p["d"]
@@ -167,12 +172,12 @@ def run_pf(p, y, key, n, ess_threshold=0.5, resampling="systematic", dtype=np.fl
     esss.append(np.exp(2 * l0 - l2))
     loglik = 0.0 if initial is not None else l0 - np.log(n)
     int_keys = rng.split(rkey, T - 1) if T > 1 else np.zeros((0, 2), np.uint32)
-    for t in range(1, T):
-        ik = int_keys[t - 1]
-        res = esss[-1] < ess_threshold * n if resample_flags is None else bool(resample_flags[t])
+    for tt in range(1, T):
+        ik = int_keys[tt - 1]
+        res = esss[-1] < ess_threshold * n if resample_flags is None else bool(resample_flags[tt])
         if res:
             if ancestors is not None:
-                a = np.asarray(ancestors[t], np.int64)
+                a = np.asarray(ancestors[tt], np.int64)
             else:
                 w = np.exp(np.asarray(lw, np.float64) - l0).astype(np.float32)
                 a = resample_indices(w, ik, resampling)
@@ -183,8 +188,9 @@ def run_pf(p, y, key, n, ess_threshold=0.5, resampling="systematic", dtype=np.fl
             lw_res = lw
             lres = l0
         eps = rng.normal_batch(rng.split(ik, n), d).astype(dtype)
-        x = transition(p, x[a], eps).reshape(n, d)
-        lw = lw_res - lik_potential(p, x, y[t])
+        scale = 1.0 if t is None else float(np.sqrt(float(t[tt]) - float(t[tt - 1])))
+        x = transition(p, x[a], eps, scale).reshape(n, d)
+        lw = lw_res - lik_potential(p, x, y[tt])
         l0, l2 = lse(lw), lse(2 * lw)
         loglik += l0 - lres
         esss.append(np.exp(2 * l0 - l2))
@@ -219,10 +225,11 @@ def simulate(p, T, key):
     return xs, np.stack(ys)
 
 
-def sv_loglik_grad(phi, sigma, beta, y, key, n, ancestors, resample_flags, m0=0.0, var0=1.0):
+def sv_loglik_grad(phi, sigma, beta, y, key, n, ancestors, resample_flags, m0=0.0, var0=1.0, t=None):
     """d log Z / d(phi, sigma, beta) of the fixed-resampling-index estimator (ancestors and noise held
     fixed) for the 1-D stochastic-volatility filter, by forward tangents in FP64:
-      x_t = phi x_{a} + sigma eps,  dx/dphi = x_a + phi dx_a/dphi,  dx/dsigma = eps + phi dx_a/dsigma
+      x_t = phi x_{a} + sigma s eps,  dx/dphi = x_a + phi dx_a/dphi,  dx/dsigma = s eps + phi dx_a/dsigma
+      (s = sqrt(t_t - t_{t-1}) with observation times t, else 1)
       lik = 0.5 (y s)^2 + 0.5 log 2pi - log s, s = exp(-x/2) / beta:
           dlik/dx = 0.5 - 0.5 (y s)^2,  dlik/dbeta = (1 - (y s)^2) / beta
       d log Z = sum_t [avg_{w_t}(dlw_t) - avg_{w_res}(dlw_res)]
@@ -246,10 +253,11 @@ def sv_loglik_grad(phi, sigma, beta, y, key, n, ancestors, resample_flags, m0=0.
     w = np.exp(lw - lse(lw))
     grad = w @ dlw
     int_keys = rng.split(rkey, T - 1) if T > 1 else np.zeros((0, 2), np.uint32)
-    for t in range(1, T):
-        ik = int_keys[t - 1]
-        if resample_flags[t]:
-            a = np.asarray(ancestors[t], np.int64)
+    for tt in range(1, T):
+        ik = int_keys[tt - 1]
+        s = 1.0 if t is None else float(np.sqrt(float(t[tt]) - float(t[tt - 1])))
+        if resample_flags[tt]:
+            a = np.asarray(ancestors[tt], np.int64)
             lw_res, dlw_res = np.zeros(n), np.zeros((n, 3))
             res_term = np.zeros(3)
         else:
@@ -258,10 +266,10 @@ def sv_loglik_grad(phi, sigma, beta, y, key, n, ancestors, resample_flags, m0=0.
             res_term = np.exp(lw - lse(lw)) @ dlw
         e = rng.normal_batch(rng.split(ik, n), 1).astype(np.float64)[:, 0]
         xa, dpa, dsa = x[a], dxp[a], dxs[a]
-        x = phi * xa + sigma * e
+        x = phi * xa + (sigma * e) * s
         dxp = xa + phi * dpa
-        dxs = e + phi * dsa
-        lik, dldx, dldb = lik_parts(x, y[t])
+        dxs = e * s + phi * dsa
+        lik, dldx, dldb = lik_parts(x, y[tt])
         lw = lw_res - lik
         dlw = dlw_res - np.stack([dldx * dxp, dldx * dxs, dldb], 1)
         w = np.exp(lw - lse(lw))

@@ -7,6 +7,7 @@ import pytest
 import torch
 
 from oracle import pf as opf
+from oracle import rng as orng
 
 pytestmark = pytest.mark.gpu
 
@@ -78,8 +79,12 @@ def test_state_space_model_functions(m):
     want = opf.lik_potential(q, xs.cpu().numpy().astype(np.float64), np.array([0.3]))
     np.testing.assert_allclose(sv.likelihood_potential(xs, torch.tensor([0.3]), 0.0).cpu().numpy(), want,
                                rtol=1e-5, atol=1e-5)
-    with pytest.raises(NotImplementedError):
-        sv.transition_sample(xs, 0.0, 0.5, m.random.split(m.random.PRNGKey(0), 9))  # non-unit step
+    # a step of length 0.5: the noise scales with sqrt(0.5) (stochastic_volatility.py:86-93)
+    ks = m.random.split(m.random.PRNGKey(0), 9)
+    x_half = sv.transition_sample(xs, 0.0, 0.5, ks).cpu().numpy().astype(np.float64)
+    eps = np.stack([orng.normal(k, (1,)) for k in ks.cpu().numpy()]).astype(np.float64)
+    want = opf.transition(q, xs.cpu().numpy().astype(np.float64), eps, np.sqrt(0.5))
+    np.testing.assert_allclose(x_half, want, rtol=1e-5, atol=1e-6)
 
 
 def test_models_without_a_kernel_raise_the_documented_error(m):

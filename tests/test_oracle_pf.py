@@ -89,24 +89,27 @@ def test_pf_loglik_close_to_kalman_statistically():
 
 
 
-def test_sv_tangent_gradient_matches_finite_differences():
+@pytest.mark.parametrize("timed", [False, True])
+def test_sv_tangent_gradient_matches_finite_differences(timed):
     # fixed-resampling-index estimator: ancestors and noise held fixed; log Z is then smooth in the
-    # parameters and its gradient is checked by central differences of the same oracle filter
+    # parameters and its gradient is checked by central differences of the same oracle filter (also
+    # with irregular observation times, where the noise scales with sqrt(dt))
     phi, sigma, beta, n, T = 0.95, 0.25, 0.7, 200, 25
     p = pf.sv_params(phi, sigma, beta)
     _, y = pf.simulate(p, T, rng.PRNGKey(5))
     y = y[:, 0]
-    base = pf.run_pf(p, y, rng.PRNGKey(6), n, 1.0, "systematic")
+    tt = np.cumsum(np.random.default_rng(9).uniform(0.2, 3.0, T)) if timed else None
+    base = pf.run_pf(p, y, rng.PRNGKey(6), n, 1.0, "systematic", t=tt)
     anc = base["ancestors"]
     flags = np.ones(T, bool)
     flags[0] = False
     flags[5] = False  # one step without resampling exercises the carried log-weight tangents
-    ref = pf.run_pf(p, y, rng.PRNGKey(6), n, 1.0, "systematic", ancestors=anc, resample_flags=flags)
-    g = pf.sv_loglik_grad(phi, sigma, beta, y, rng.PRNGKey(6), n, anc, flags)
+    ref = pf.run_pf(p, y, rng.PRNGKey(6), n, 1.0, "systematic", ancestors=anc, resample_flags=flags, t=tt)
+    g = pf.sv_loglik_grad(phi, sigma, beta, y, rng.PRNGKey(6), n, anc, flags, t=tt)
 
     def ll(ph, sg, bt):
         return pf.run_pf(pf.sv_params(ph, sg, bt), y, rng.PRNGKey(6), n, 1.0, "systematic", ancestors=anc,
-                         resample_flags=flags)["log_likelihood"]
+                         resample_flags=flags, t=tt)["log_likelihood"]
     h = 1e-5
     fd = [(ll(phi + h, sigma, beta) - ll(phi - h, sigma, beta)) / (2 * h),
           (ll(phi, sigma + h, beta) - ll(phi, sigma - h, beta)) / (2 * h),
@@ -129,3 +132,23 @@ def test_resume_from_the_first_state_repeats_the_run():
     np.testing.assert_array_equal(res["ancestors"], full["ancestors"])
     l0 = pf.lse(full["log_weight"][0]) - np.log(300)
     assert abs(res["log_likelihood"] - (full["log_likelihood"] - l0)) < 1e-9
+
+
+def test_sv_time_steps_scale_the_transition_noise():
+    # unit observation times reproduce the unit-step run exactly
+    p = pf.sv_params(0.9, 0.4, 0.7)
+    y = np.random.default_rng(5).standard_normal((12, 1))
+    key = rng.PRNGKey(3)
+    a = pf.run_pf(p, y, key, 64, 1.0, "systematic")
+    b = pf.run_pf(p, y, key, 64, 1.0, "systematic", t=np.arange(12.0) + 3.0)
+    np.testing.assert_array_equal(a["value"], b["value"])
+    assert a["log_likelihood"] == b["log_likelihood"]
+    # a step of length dt adds variance sigma^2 dt (the transition noise scales with sqrt(dt); a scale of
+    # dt, or none, fails): x' = phi x + sigma sqrt(dt) eps from x ~ N(0, 1)
+    rs = np.random.default_rng(6)
+    x = rs.standard_normal((200000, 1))
+    eps = rs.standard_normal((200000, 1))
+    for dt in (0.25, 4.0):
+        xn = pf.transition(p, x, eps, np.sqrt(dt))
+        want = 0.9 ** 2 + 0.4 ** 2 * dt
+        assert abs(xn.var() - want) < 4 * want * np.sqrt(2.0 / 200000)
