@@ -118,8 +118,10 @@ int mocat_hmc_correct(int64_t C, int d, int64_t iter, int correction, float targ
  * model 0: TimeHomogenousLinearGaussian (linear_gaussian.py:46-51,86-94,118-128), params
  *   [m0(d), L0(dxd), A(dxd), LQ(dxd), H(dyxd), SR(dyxdy), nlz] with SR = inv(chol(R)) and
  *   nlz = (dy log 2pi - log det R^-1)/2 as mocat's reset_covariance (mocat/src/utils.py:247-262);
- * model 1: 1-D StochasticVolatility (stochastic_volatility.py:61-108), params
- *   [m0, sqrt(P0), phi, sigma, 1/beta]; observation times enter through step_scale (below);
+ * model 1: StochasticVolatility (stochastic_volatility.py:15-108), d = dy in 1..4; d = 1 params
+ *   [m0, sqrt(P0), phi, sigma, 1/beta], d > 1 params [m0(d), L0(dxd), A(dxd), LQ(dxd), R^-1/2(d)]
+ *   with L0, LQ the Cholesky factors of P0 and Q and R the likelihood covariance's diagonal;
+ *   observation times enter through step_scale (below);
  * model 2: TimeHomogenousLinearGaussian under OptimalNonLinearGaussianParticleFilter
  *   (mocat/src/ssm/nonlinear_gaussian.py:134-281), params [m0(d), H(dyxd), K0(dxdy), C0(dxd),
  *   A(dxd), K(dxdy), S(dxd), WC(dyxdy), nlzw] as its startup (:154-184); initial weights zero;
@@ -130,7 +132,7 @@ int mocat_hmc_correct(int64_t C, int d, int64_t iter, int correction, float targ
  * resample when ESS_{t-1} < ess_threshold * n. Outputs (optional unless noted): trajectories at
  * out_offset ([T][n][d], [T][n], ancestors [T][n]), ess / cumulative log-likelihood at t_offset [T],
  * out_loglik[n_filters] (required, FP64). Workspaces are sized by the sum of n over filters.
- * out_grad (model 1 only, else NULL): [n_filters][3] d log Z / d(phi, sigma, beta) of the
+ * out_grad (model 1 with d = 1 only, else NULL): [n_filters][3] d log Z / d(phi, sigma, beta) of the
  * fixed-resampling-index estimator by forward tangents (workspaces tan_ws0/1: [sum n][5] floats). */
 typedef struct {
   int64_t n;             /* particles */

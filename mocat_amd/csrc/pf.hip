@@ -187,6 +187,67 @@ struct Sv {
   }
 };
 
+// Multivariate stochastic volatility, dim D = dim_obs (mocat/src/ssm/scenarios/stochastic_volatility.py:15-108):
+// params: m0[D], L0[D*D], A[D*D], LQ[D*D], ib[D] (L0, LQ Cholesky factors of the initial and transition
+// covariances, ib_k = 1 / sqrt(likelihood_covariance_diag_k))
+//   x0 = L0 eps + m0 ; x' = A x + (LQ eps) s with s = sqrt(t_new - t_previous) ;
+//   lik = gaussian_potential(y, 0, sqrt_prec = ib / exp(x / 2)) = sum_k 0.5 (y_k s_k)^2 + 0.5 log 2pi - log s_k
+template <int D>
+struct SvD {
+  static constexpr bool kOptimal = false;
+  static constexpr bool kTimeScaled = true;
+  static constexpr int NPARAM = D + 3 * D * D + D;
+  float m0[D], L0[D * D], A[D * D], LQ[D * D], ib[D];
+  MOCAT_DEV void load(const float* p) {
+#pragma unroll
+    for (int i = 0; i < D; ++i) m0[i] = p[i];
+    p += D;
+#pragma unroll
+    for (int i = 0; i < D * D; ++i) L0[i] = p[i];
+    p += D * D;
+#pragma unroll
+    for (int i = 0; i < D * D; ++i) A[i] = p[i];
+    p += D * D;
+#pragma unroll
+    for (int i = 0; i < D * D; ++i) LQ[i] = p[i];
+    p += D * D;
+#pragma unroll
+    for (int i = 0; i < D; ++i) ib[i] = p[i];
+  }
+  MOCAT_DEV void init(const float* eps, float* x) const {
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+      float a = 0.f;
+#pragma unroll
+      for (int k = 0; k < D; ++k) a += L0[i * D + k] * eps[k];
+      x[i] = a + m0[i];
+    }
+  }
+  MOCAT_DEV void transition_s(const float* xp, const float* eps, float* x, float s) const {
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+      float a = 0.f, b = 0.f;
+#pragma unroll
+      for (int k = 0; k < D; ++k) {
+        a += A[i * D + k] * xp[k];
+        b += LQ[i * D + k] * eps[k];
+      }
+      x[i] = a + b * s;
+    }
+  }
+  MOCAT_DEV void transition(const float* xp, const float* eps, float* x) const { transition_s(xp, eps, x, 1.f); }
+  MOCAT_DEV float lik(const float* x, const float* y) const {
+    float acc = 0.f;
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      const float sk = ib[k] / __expf(0.5f * x[k]);
+      const float ys = y[k] * sk;
+      acc += 0.5f * (ys * ys) + 0.5f * (float)kLog2PiD - __logf(sk);
+    }
+    return acc;
+  }
+};
+
 // Optimal proposal for the linear-Gaussian model: OptimalNonLinearGaussianParticleFilter
 // (mocat/src/ssm/nonlinear_gaussian.py:134-281) with transition_function(x) = A x
 // (linear_gaussian.py:112-116). Matrices factored on the host as its startup (:154-184):
@@ -1945,7 +2006,8 @@ extern "C" {
 int mocat_pf_param_count(int model, int d, int dy, int64_t* count) {
   if (!count) return MOCAT_ERR_ARG;
   if (model == 1) {
-    *count = Sv::NPARAM;
+    if (d != dy || d < 1 || d > 4) return MOCAT_ERR_ARG;
+    *count = d == 1 ? Sv::NPARAM : 2 * d + 3 * d * d;
     return MOCAT_OK;
   }
   if ((model != 0 && model != 2) || d < 1 || d > 8 || dy < 1 || dy > 8) return MOCAT_ERR_ARG;
@@ -1979,8 +2041,13 @@ int mocat_pf_run(int model, int d, int dy, const mocat_pf_filter* host_filters, 
   PfArgs a{filters, n_filters, params, y, step_scale, x_ws0, x_ws1, lw_ws0, lw_ws1, csum_ws, anc_ws, resampling,
            ess_threshold, out_value, out_lw, out_anc, out_ess, out_loglik_t, out_loglik, tan_ws0, tan_ws1, out_grad};
   if (model == 1) {
-    if (d != 1 || dy != 1) return MOCAT_ERR_ARG;
-    return out_grad ? launch_pf<Sv, 1, 1, true>(a, stream) : launch_pf<Sv, 1, 1>(a, stream);
+    if (d != dy) return MOCAT_ERR_ARG;
+    if (d == 1) return out_grad ? launch_pf<Sv, 1, 1, true>(a, stream) : launch_pf<Sv, 1, 1>(a, stream);
+    if (out_grad) return MOCAT_ERR_ARG;  // the gradient is compiled for the 1-D model
+    if (d == 2) return launch_pf<SvD<2>, 2, 2>(a, stream);
+    if (d == 3) return launch_pf<SvD<3>, 3, 3>(a, stream);
+    if (d == 4) return launch_pf<SvD<4>, 4, 4>(a, stream);
+    return MOCAT_ERR_ARG;
   }
   if (model == 2) {
 #define MOCAT_PF_CASE(DD, YY) \
@@ -2055,7 +2122,12 @@ int mocat_pf_run_large(int model, int d, int dy, const mocat_pf_filter* host_fil
            reinterpret_cast<uint64_t*>(ws + L.o_cend), reinterpret_cast<int64_t*>(ws + L.o_bnd),
            reinterpret_cast<StepKey*>(ws + L.o_keys)};
   if (model == 1) {
-    if (d != 1 || dy != 1) return MOCAT_ERR_ARG;
+    if (d != dy) return MOCAT_ERR_ARG;
+    if (d != 1 && out_grad) return MOCAT_ERR_ARG;
+    if (d == 2) return launch_pf_large<SvD<2>, 2, 2>(a, L.max_T, L.cend_slots, launch_mode, stream);
+    if (d == 3) return launch_pf_large<SvD<3>, 3, 3>(a, L.max_T, L.cend_slots, launch_mode, stream);
+    if (d == 4) return launch_pf_large<SvD<4>, 4, 4>(a, L.max_T, L.cend_slots, launch_mode, stream);
+    if (d != 1) return MOCAT_ERR_ARG;
     return out_grad ? launch_pf_large<Sv, 1, 1, true>(a, L.max_T, L.cend_slots, launch_mode, stream)
                     : launch_pf_large<Sv, 1, 1>(a, L.max_T, L.cend_slots, launch_mode, stream);
   }
@@ -2085,8 +2157,12 @@ int mocat_pf_hook(int model, int d, int dy, int phase, const float* params, cons
   if (model == 2 && (phase == 2 || !y)) return MOCAT_ERR_ARG;
   if (n > 0x7FFFFFFFll || (model == 2 && n * (int64_t)d > 0xFFFFFFFFll)) return MOCAT_ERR_SIZE;
   if (model == 1) {
-    if (d != 1 || dy != 1) return MOCAT_ERR_ARG;
-    return launch_pf_hook<Sv, 1, 1>(phase, params, x_prev, y, keys, step_scale, n, x, log_weight, stream);
+    if (d != dy) return MOCAT_ERR_ARG;
+    if (d == 1) return launch_pf_hook<Sv, 1, 1>(phase, params, x_prev, y, keys, step_scale, n, x, log_weight, stream);
+    if (d == 2) return launch_pf_hook<SvD<2>, 2, 2>(phase, params, x_prev, y, keys, step_scale, n, x, log_weight, stream);
+    if (d == 3) return launch_pf_hook<SvD<3>, 3, 3>(phase, params, x_prev, y, keys, step_scale, n, x, log_weight, stream);
+    if (d == 4) return launch_pf_hook<SvD<4>, 4, 4>(phase, params, x_prev, y, keys, step_scale, n, x, log_weight, stream);
+    return MOCAT_ERR_ARG;
   }
   if (model == 2) {
 #define MOCAT_PF_CASE(DD, YY) \

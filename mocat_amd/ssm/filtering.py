@@ -425,8 +425,8 @@ def _launch(models, ys, keys, ns, ess_threshold, resampling, store, grad=False, 
     out_ess = torch.empty(toff, **f32)
     out_llt = torch.empty(toff, dtype=torch.float64, device=dev)
     out_ll = torch.empty(F, dtype=torch.float64, device=dev)
-    if grad and kind != 1:
-        raise NotImplementedError("log-likelihood gradients are implemented for the stochastic-volatility model")
+    if grad and (kind != 1 or d != 1):
+        raise NotImplementedError("log-likelihood gradients are implemented for the 1-D stochastic-volatility model")
     if grad and initial is not None and any(v is not None for v in initial):
         raise NotImplementedError("log-likelihood gradients start from the initial distribution, not from a sample")
     if initial is not None:

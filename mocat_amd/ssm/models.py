@@ -152,41 +152,70 @@ class TimeHomogenousLinearGaussian(StateSpaceModel):
 
 
 class StochasticVolatility(StateSpaceModel):
-    """1-D stochastic volatility: x_t = phi x_{t-1} + sigma eps, y_t = beta exp(x_t / 2) eta.
-    Arguments as mocat's (mocat/src/ssm/scenarios/stochastic_volatility.py:24-57): transition_matrix
-    = phi, transition_covariance = sigma^2, likelihood_covariance_diag = beta^2. The transition noise
-    scales with sqrt(t_new - t_previous) (stochastic_volatility.py:86-93; unit steps when no times are given)."""
+    """Stochastic volatility (mocat/src/ssm/scenarios/stochastic_volatility.py:15-108), dim = dim_obs <= 4:
+    x_t = A x_{t-1} + chol(Q) eps sqrt(t_t - t_{t-1}), y_t,k = sqrt(R_k) exp(x_t,k / 2) eta_k. Arguments as
+    mocat's: initial_mean, initial_covariance, transition_matrix (A), transition_covariance (Q),
+    likelihood_covariance_diag (R; a matrix's diagonal is taken). In 1-D they are the scalars phi = A,
+    sigma^2 = Q, beta^2 = R (attributes phi, sigma, beta; the log-likelihood gradient and backward
+    simulation are compiled for this case). The transition noise scales with sqrt(t_new - t_previous)
+    (unit steps when no times are given)."""
     name = "Stochastic Volatility"
     kernel_model = 1
 
     def __init__(self, dim: int = 1, initial_mean=None, initial_covariance=None, transition_matrix=None,
                  transition_covariance=None, likelihood_covariance_diag=None, name=None):
-        if dim != 1:
-            raise NotImplementedError("the fused kernel covers the 1-D stochastic volatility model")
+        d = int(dim)
+        if not 1 <= d <= 4:
+            raise NotImplementedError("the fused kernels cover the stochastic-volatility model for dim 1 to 4")
         if name is not None:
             self.name = name
-        self.dim = self.dim_obs = 1
-        f = lambda v, default: default if v is None else float(np.asarray(_np(v)).ravel()[0])
-        self.initial_mean = f(initial_mean, 0.0)
-        self.initial_covariance = f(initial_covariance, 1.0)
-        self.phi = f(transition_matrix, 1.0)
-        self.sigma = float(np.sqrt(f(transition_covariance, 1.0)))
-        self.beta = float(np.sqrt(f(likelihood_covariance_diag, 1.0)))
+        self.dim = self.dim_obs = d
+        if d == 1:
+            f = lambda v, default: default if v is None else float(np.asarray(_np(v)).ravel()[0])
+            self.initial_mean = f(initial_mean, 0.0)
+            self.initial_covariance = f(initial_covariance, 1.0)
+            self.phi = f(transition_matrix, 1.0)
+            self.sigma = float(np.sqrt(f(transition_covariance, 1.0)))
+            self.beta = float(np.sqrt(f(likelihood_covariance_diag, 1.0)))
+            return
+        mat = lambda v, default: default if v is None else _np(v).reshape(d, d)
+        self.initial_mean = np.zeros(d) if initial_mean is None else _np(initial_mean).reshape(d)
+        self.initial_covariance = mat(initial_covariance, np.eye(d))
+        self.transition_matrix = mat(transition_matrix, np.eye(d))
+        self.transition_covariance = mat(transition_covariance, np.eye(d))
+        r = np.ones(d) if likelihood_covariance_diag is None else _np(likelihood_covariance_diag)
+        self.likelihood_covariance_diag = np.diag(r).copy() if r.ndim == 2 else r.reshape(d)
+        # mocat's reset_covariance (utils.py:247-262): Cholesky factors in FP64
+        self.initial_covariance_sqrt = np.linalg.cholesky(self.initial_covariance)
+        self.transition_covariance_sqrt = np.linalg.cholesky(self.transition_covariance)
+        self.likelihood_precision_diag_sqrt = 1.0 / np.sqrt(self.likelihood_covariance_diag)
 
     def kernel_params(self):
-        return np.array([self.initial_mean, np.sqrt(self.initial_covariance), self.phi, self.sigma,
-                         1.0 / self.beta], np.float32)
+        if self.dim == 1:
+            return np.array([self.initial_mean, np.sqrt(self.initial_covariance), self.phi, self.sigma,
+                             1.0 / self.beta], np.float32)
+        return np.concatenate([self.initial_mean, self.initial_covariance_sqrt.ravel(), self.transition_matrix.ravel(),
+                               self.transition_covariance_sqrt.ravel(),
+                               self.likelihood_precision_diag_sqrt]).astype(np.float32)
 
     def transition_params(self):
         """Backward-simulation parameters for unit steps: phi, 1/sigma and the normaliser
         (log 2 pi - log sigma^-2) / 2 (stochastic_volatility.py:78-84, utils.py:49-105)."""
+        if self.dim != 1:
+            raise NotImplementedError("backward simulation is compiled for the 1-D stochastic-volatility model")
         return np.array([self.phi, 1.0 / self.sigma, (LOG2PI + 2.0 * np.log(self.sigma)) / 2], np.float32)
 
     def _init_from_eps(self, e):
-        return np.sqrt(self.initial_covariance) * e + self.initial_mean
+        if self.dim == 1:
+            return np.sqrt(self.initial_covariance) * e + self.initial_mean
+        return self.initial_covariance_sqrt @ e + self.initial_mean
 
     def _transition_from_eps(self, x, e, dt):
-        return x * self.phi + e * self.sigma * np.sqrt(dt)
+        if self.dim == 1:
+            return x * self.phi + e * self.sigma * np.sqrt(dt)
+        return self.transition_matrix @ x + (self.transition_covariance_sqrt @ e) * np.sqrt(dt)
 
     def _observe_from_eps(self, x, e):
-        return self.beta * np.exp(0.5 * x) * e
+        if self.dim == 1:
+            return self.beta * np.exp(0.5 * x) * e
+        return np.sqrt(self.likelihood_covariance_diag) * np.exp(0.5 * x) * e

@@ -9,7 +9,8 @@ Follows mocat's run_particle_filter_for_marginals (mocat/src/ssm/filtering.py:25
 Models:
   TimeHomogenousLinearGaussian  mocat/src/ssm/linear_gaussian/linear_gaussian.py:46-51,86-94,118-128,
                                 covariances factored as reset_covariance (mocat/src/utils.py:247-262)
-  StochasticVolatility (1-D)    mocat/src/ssm/scenarios/stochastic_volatility.py:61-108
+  StochasticVolatility          mocat/src/ssm/scenarios/stochastic_volatility.py:15-108 (1-D scalar form,
+                                and dim > 1 with full A, Q and a diagonal likelihood covariance)
 The log-likelihood (absent in mocat) is log Z = sum_t [lse(lw_t) - lse(lw_res_{t-1})] (the SMC
 log-normaliser recursion of mocat/src/transport/smc.py:212-215), t = 0 term lse(lw_0) - log n.
 
@@ -57,6 +58,21 @@ def sv_params(phi, sigma, beta, m0=0.0, var0=1.0):
                 inv_beta=1.0 / float(beta), d=1, dy=1, sv=True)
 
 
+def svd_params(m0, P0, A, Q, rdiag):
+    """Multivariate stochastic volatility (mocat/src/ssm/scenarios/stochastic_volatility.py:15-108):
+    x0 = L0 eps + m0, x' = A x + (LQ eps) s, lik = gaussian_potential(y, 0, sqrt_prec = r^-1/2 / exp(x/2))
+    with L0 = chol(P0), LQ = chol(Q) (reset_covariance, mocat/src/utils.py:247-262)."""
+    m0 = np.asarray(m0, np.float64).reshape(-1)
+    d = m0.size
+    return dict(m0=m0, L0=np.linalg.cholesky(np.asarray(P0, np.float64)), A=np.asarray(A, np.float64),
+                LQ=np.linalg.cholesky(np.asarray(Q, np.float64)),
+                ib=1.0 / np.sqrt(np.asarray(rdiag, np.float64).reshape(d)), d=d, dy=d, svd=True)
+
+
+def svd_flat(p):
+    return np.concatenate([p["m0"], p["L0"].ravel(), p["A"].ravel(), p["LQ"].ravel(), p["ib"]]).astype(np.float32)
+
+
 def sv_flat(p):
     return np.array([p["m0"][0], p["L0"][0, 0], p["phi"], p["sigma"], p["inv_beta"]], np.float32)
 
@@ -73,6 +89,8 @@ def transition(p, x, eps, scale=1.0):
     linear-Gaussian model is time homogeneous."""
     if p.get("sv"):
         return x * p["phi"] + (eps * p["sigma"]) * scale
+    if p.get("svd"):
+        return x @ p["A"].T + (eps @ p["LQ"].T) * scale
     return x @ p["A"].T + eps @ p["LQ"].T
 
 
@@ -80,6 +98,9 @@ def lik_potential(p, x, y):
     if p.get("sv"):
         s = p["inv_beta"] / np.exp(0.5 * x[:, 0])
         return 0.5 * (y[0] * s) ** 2 + 0.5 * LOG2PI - np.log(s)
+    if p.get("svd"):
+        s = p["ib"][None, :] / np.exp(0.5 * x)
+        return np.sum(0.5 * (y[None, :] * s) ** 2 + 0.5 * LOG2PI - np.log(s), axis=1)
     diff = y[None, :] - x @ p["H"].T
     v = diff @ p["SR"]
     return 0.5 * np.sum(v * v, axis=1) + p["nlz"]
@@ -219,6 +240,8 @@ def simulate(p, T, key):
         e = rng.normal(keys[T + t], (dy,)).astype(np.float64)
         if p.get("sv"):
             ys.append((1.0 / p["inv_beta"]) * np.exp(0.5 * xs[t]) * e)
+        elif p.get("svd"):
+            ys.append(np.exp(0.5 * xs[t]) * e / p["ib"])
         else:
             LR = np.linalg.inv(p["SR"])
             ys.append(p["H"] @ xs[t] + LR @ e)

@@ -200,6 +200,55 @@ def test_sv_irregular_times_vs_fp64_oracle(m, large_n):
         assert ll != unit[i].log_likelihood.item()  # the times change the run
 
 
+def _svd_case(d, seed):
+    rs = np.random.default_rng(seed)
+    A = 0.85 * np.eye(d) + 0.05 * rs.standard_normal((d, d))
+    B = rs.standard_normal((d, d))
+    Q = 0.1 * (B @ B.T / d + 0.5 * np.eye(d))
+    P0 = np.eye(d) + 0.2 * np.ones((d, d))
+    m0 = 0.1 * rs.standard_normal(d)
+    r = rs.uniform(0.4, 1.5, d)
+    return m0, P0, A, Q, r
+
+
+@pytest.mark.parametrize("large_n", [1 << 40, 1])
+@pytest.mark.parametrize("d", [2, 3, 4])
+def test_multivariate_sv_vs_fp64_oracle(m, d, large_n):
+    """mocat's StochasticVolatility with dim > 1 (full transition matrix and covariance, diagonal
+    likelihood covariance) and irregular times, both filter paths, vs the FP64 oracle on the GPU's
+    own ancestors; the model's hooks draw what the oracle's model functions give for the same keys."""
+    m0, P0, A, Q, r = _svd_case(d, d)
+    model = m.ssm.StochasticVolatility(d, m0, P0, A, Q, r)
+    p = opf.svd_params(m0, P0, A, Q, r)
+    T, n = 40, 1500
+    _, y = opf.simulate(p, T, orng.PRNGKey(d))
+    t = np.cumsum(np.random.default_rng(d).uniform(0.2, 2.0, T))
+    key = m.random.PRNGKey(30 + d)
+    res = m.ssm.run_particle_filters([model], [y.astype(np.float32)], [key], [n], ess_threshold=0.7,
+                                     resampling="systematic", store=True, large_n=large_n, ts=[t])[0]
+    ess = res.ess.cpu().numpy()
+    flags = np.zeros(T, bool)
+    flags[1:] = ess[:-1] < 0.7 * n
+    o = opf.run_pf(p, y, key.cpu().numpy(), n, 0.7, "systematic", ancestors=res.ancestors.cpu().numpy(),
+                   resample_flags=flags, t=t)
+    ll = res.log_likelihood.item()
+    assert abs(ll - o["log_likelihood"]) < 1e-4 * (1 + abs(ll)), (ll, o["log_likelihood"])
+    np.testing.assert_allclose(res.value.cpu().numpy(), o["value"], rtol=1e-4, atol=1e-4)
+    # hooks: initial_sample and transition_sample for one key per particle
+    ks = m.random.split(m.random.PRNGKey(2), 16)
+    eps = np.stack([orng.normal(k, (d,)) for k in ks.cpu().numpy()]).astype(np.float64)
+    x0 = model.initial_sample(0.0, ks)
+    np.testing.assert_allclose(x0.cpu().numpy(), opf.init_state(p, eps), rtol=1e-5, atol=1e-5)
+    x1 = model.transition_sample(x0, 0.0, 2.25, ks)
+    want = opf.transition(p, x0.cpu().numpy().astype(np.float64), eps, 1.5)
+    np.testing.assert_allclose(x1.cpu().numpy(), want, rtol=1e-5, atol=1e-5)
+    lik = model.likelihood_potential(x1, torch.tensor(y[0], dtype=torch.float32, device="cuda"), 0.0)
+    np.testing.assert_allclose(lik.cpu().numpy(), opf.lik_potential(p, x1.cpu().numpy().astype(np.float64), y[0]),
+                               rtol=1e-5, atol=1e-4)
+    with pytest.raises(NotImplementedError, match="1-D stochastic-volatility"):
+        m.ssm.run_particle_filters([model], [y.astype(np.float32)], [key], [n], grad=True, large_n=large_n)
+
+
 def test_sv_times_through_marginals_and_hooks(m):
     """run_particle_filter_for_marginals passes its t to the kernels; the transition hook with the same
     step scale draws the fused step's particles bit for bit."""

@@ -152,3 +152,30 @@ def test_sv_time_steps_scale_the_transition_noise():
         xn = pf.transition(p, x, eps, np.sqrt(dt))
         want = 0.9 ** 2 + 0.4 ** 2 * dt
         assert abs(xn.var() - want) < 4 * want * np.sqrt(2.0 / 200000)
+
+
+def test_multivariate_sv_reduces_to_1d_and_is_gaussian():
+    from scipy.stats import norm
+    # dim 1: the multivariate form is the scalar model
+    p1 = pf.sv_params(0.9, 0.3, 0.8, m0=0.1, var0=1.5)
+    pd = pf.svd_params([0.1], [[1.5]], [[0.9]], [[0.09]], [0.64])
+    y = np.random.default_rng(2).standard_normal((9, 1))
+    a = pf.run_pf(p1, y, rng.PRNGKey(1), 50, 0.7, "systematic", t=np.arange(9.0) * 0.5)
+    b = pf.run_pf(pd, y, rng.PRNGKey(1), 50, 0.7, "systematic", t=np.arange(9.0) * 0.5)
+    np.testing.assert_allclose(b["value"], a["value"], rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(b["log_likelihood"], a["log_likelihood"], rtol=1e-12)
+    # likelihood: independent normals with standard deviations sqrt(r_k) exp(x_k / 2)
+    p3 = pf.svd_params(np.zeros(3), np.eye(3), 0.9 * np.eye(3), np.eye(3), [0.5, 1.0, 2.0])
+    x = np.random.default_rng(3).standard_normal((4, 3))
+    yv = np.array([0.3, -1.0, 0.7])
+    want = -np.sum(norm(0, np.sqrt([0.5, 1.0, 2.0])[None, :] * np.exp(0.5 * x)).logpdf(yv[None, :]), axis=1)
+    np.testing.assert_allclose(pf.lik_potential(p3, x, yv), want, rtol=1e-12)
+    # transition covariance: A Sx A^T + Q dt (a dropped or misplaced factor fails)
+    rs = np.random.default_rng(4)
+    A = np.array([[0.9, 0.1], [-0.2, 0.8]])
+    Q = np.array([[0.5, 0.2], [0.2, 0.3]])
+    p2 = pf.svd_params(np.zeros(2), np.eye(2), A, Q, [1.0, 1.0])
+    x = rs.standard_normal((400000, 2))
+    e = rs.standard_normal((400000, 2))
+    c = np.cov(pf.transition(p2, x, e, np.sqrt(2.5)).T)
+    np.testing.assert_allclose(c, A @ A.T + 2.5 * Q, atol=0.02)
