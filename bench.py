@@ -224,7 +224,7 @@ def bench_pf5():
     from mocat_amd import datasets
     models, ys, ns = datasets.sv_batch_config(0)
     keys = m.random.split(m.random.PRNGKey(0), len(models))
-    # the schedule autotuner times the candidate schedules (chunked-path threshold, graph launch) on
+    # the schedule autotuner times the candidate schedules (chunked-path threshold x launch mode) on
     # this batch once, untimed, and records its choice; the timed runs use the recorded schedule
     from mocat_amd.ssm import schedule as sched
     t0 = time.perf_counter()
@@ -235,6 +235,8 @@ def bench_pf5():
     res, dt = _timed(run)
     n_large = sum(1 for v in ns if v >= choice["large_n"])
     launches = (3 * (max(len(y) for y, v in zip(ys, ns) if v >= choice["large_n"]) - 1) + 3) if n_large else 0
+    if n_large and choice["launch"] == "persistent":
+        launches = 3  # init, the persistent kernel, the abort check
     E = sum(n * len(y) for n, y in zip(ns, ys))
     big = max(n * len(y) for n, y in zip(ns, ys))
     rung = lambda: m.ssm.run_particle_filters(models, ys, [k for k in keys], ns, 1.0, "systematic", grad=True,
@@ -263,12 +265,12 @@ def bench_pf5():
     gchoice = sched.lookup(sched.signature(models, ys, ns, "systematic", True, False, None))
     print(json.dumps({"mode": "pf5", "metric": "particle-steps/s", "cpu_baseline": cpu, "value": E / dt,
                       "ms": dt * 1e3,
-                      "schedule": {"chosen": {k: choice[k] for k in ("large_n", "graph")},
+                      "schedule": {"chosen": {k: choice[k] for k in ("large_n", "launch")},
                                    "timings_ms": choice["timings_ms"], "tuning_seconds": tune_s,
-                                   "chosen_with_gradient": {k: gchoice[k] for k in ("large_n", "graph")},
+                                   "chosen_with_gradient": {k: gchoice[k] for k in ("large_n", "launch")},
                                    "filters_on_chunked_path": n_large,
                                    "chunked_path_kernel_launches": launches,
-                                   "chunked_path_submissions": 1 if choice["graph"] else launches},
+                                   "chunked_path_submissions": 1 if choice["launch"] == "graph" else launches},
                       "with_gradient": E / dtg, "with_gradient_ms": dtg * 1e3,
                       "particle_steps": E, "largest_filter_particle_steps": big,
                       "bytes_per_particle_step": 24, "achieved_GBps_algorithmic": 24 * E / dt / 1e9}))

@@ -185,7 +185,11 @@ int mocat_pf_run(int model, int d, int dy, const mocat_pf_filter* host_filters, 
  * t_offset, which only places out_ess / out_loglik_t and must be >= 0 when either is given).
  * launch_mode 0 queues the per-step launches on the stream; 1 captures them into one HIP graph,
  * launches it on the stream and returns after it has run (when the stream is already being captured
- * by the caller, the launches join the caller's graph instead). Results do not depend on the mode.
+ * by the caller, the launches join the caller's graph instead); 2 runs all steps in one persistent
+ * kernel (at most four workgroups per CU, all resident) with a grid barrier between the phases; a
+ * barrier that waits more than ~2^22 polls gives up, and every flagged filter's out_loglik (and
+ * out_grad) is then NaN. Results do not depend on the mode. The workspace also holds the step's split
+ * keys (8 bytes per particle, rounded up to whole chunks).
  * Calls on one stream may follow mocat_pf_run with the same arguments: the two paths touch disjoint
  * filters. */
 int mocat_pf_large_workspace_bytes(const mocat_pf_filter* host_filters, int n_filters, int64_t* bytes);

@@ -875,7 +875,8 @@ __global__ __launch_bounds__(256) void categorical_kernel(const float* __restric
 //              the filter alike; its first chunk writes the log-likelihood, ESS and filter state),
 //              the resampling decision and weight shift for step t, and the chunk's sum of quantised
 //              weights (resampling steps only);
-//   lf_csum    per chunk: offset = sum of the earlier chunks' totals, plus an inclusive block scan;
+//   lf_csum    per chunk: the chunk's threefry blocks of the step's split(key, n) (every step), then
+//              offset = sum of the earlier chunks' totals, plus an inclusive block scan;
 //   lf_step    per chunk: ancestor (binary search in the exact prefix sums, or Gumbel-max), then
 //              propagate, weight, outputs and the chunk accumulators of step t+1.
 // The weights are shifted by float(lse) and quantised with the shift of the largest weight exactly as
@@ -894,6 +895,7 @@ struct LfDesc {
   int64_t key_slot;  // this filter's T step-key slots in LfArgs::skeys (disjoint per filter)
   uint32_t key[2];
   int32_t param_offset, f, flags;
+  int32_t slot;  // index of the filter in desc and st
 };
 
 struct LfState {
@@ -917,7 +919,7 @@ struct StepKey {
 
 struct LfArgs {
   const LfDesc* desc;
-  const int32_t* chunk_filter;  // chunk -> index into desc
+  const LfDesc* cdesc;  // [n_chunks] the descriptor of each chunk's filter (one load per workgroup)
   int n_large;
   int64_t n_chunks;
   const float* params;
@@ -948,6 +950,9 @@ struct LfArgs {
   uint64_t* cends;   // [n_chunks * CH / SG] prefix sums at the SG-block ends, per filter from chunk0 * CH / SG
   int64_t* bounds;   // [n_chunks] systematic: ancestor of each chunk's first target
   StepKey* skeys;    // [key_slot + t] per filter
+  uint32_t* bar;     // persistent kernel: [0] barrier arrivals, [1] abort flag (zeroed by the init launch)
+  uint32_t* pkeys;   // [n_chunks * CH * 2] the step's split(step key, n), flat, per filter from chunk0 * CH * 2;
+                     // null for the optimal proposal (its noise is not drawn from per-particle keys)
 };
 
 struct LfScratch {
@@ -1121,12 +1126,10 @@ MOCAT_DEV void lf_store_lse(const LfArgs& a, int64_t c, LseAcc r) {
 }
 
 template <class Model, int D, int DY, bool GRAD>
-__global__ __launch_bounds__(LT) void lf_init_kernel(LfArgs a) {
-  __shared__ LfScratch s;
-  __shared__ Model model;
-  const int64_t c = blockIdx.x;
-  const LfDesc F = a.desc[a.chunk_filter[c]];
+MOCAT_DEV void lf_init_chunk(const LfArgs& a, int64_t c, LfScratch& s, Model& model) {
+  const LfDesc F = a.cdesc[c];
   const int tid = threadIdx.x;
+  if (a.bar && c == 0 && tid < 2) a.bar[tid] = 0u;  // the persistent kernel's barrier counter and abort flag
   load_shared_params(model, a.params + F.param_offset);
   __syncthreads();
   const int64_t n = F.n;
@@ -1227,6 +1230,13 @@ __global__ __launch_bounds__(LT) void lf_init_kernel(LfArgs a) {
   }
 }
 
+template <class Model, int D, int DY, bool GRAD>
+__global__ __launch_bounds__(LT) void lf_init_kernel(LfArgs a) {
+  __shared__ LfScratch s;
+  __shared__ Model model;
+  lf_init_chunk<Model, D, DY, GRAD>(a, blockIdx.x, s, model);
+}
+
 // The prefix sums are integers below 2^52 (exact in FP64), so for a target T >= 0 the test
 // (double)C <= T is C <= floor(T): targets are converted once and every level compares integers.
 MOCAT_DEV uint64_t target_floor(double t) { return t > 0.0 ? (uint64_t)floor(t) : 0ull; }
@@ -1241,27 +1251,52 @@ MOCAT_DEV const float* lf_lw_prev(const LfArgs& a, int64_t t) { return ((t - 1) 
 //    no workgroup reads what another writes in this launch;
 //  * for step t, when it resamples, the chunk's total of quantised weights.
 template <bool GRAD>
-__global__ __launch_bounds__(LT) void lf_prep_kernel(LfArgs a, int64_t t) {
-  __shared__ LfScratch s;
-  const int64_t c = blockIdx.x;
-  const int fi = a.chunk_filter[c];
-  const LfDesc F = a.desc[fi];
+MOCAT_DEV void lf_prep_chunk(const LfArgs& a, int64_t t, int64_t c, LfScratch& s) {
+  const LfDesc F = a.cdesc[c];
+  const int fi = F.slot;
   if (t - 1 >= F.T) return;
   const int64_t n = F.n, nchunk = (n + CH - 1) / CH;
+  // the chunk's weights for its quantised total are loaded first: they do not depend on the combine
+  const bool want_tot = t < F.T && a.resampling != 2;
+  const float* lwp = lf_lw_prev(a, t) + F.state_offset;
+  float lwv[PPT];
+#pragma unroll
+  for (int pk = 0; pk < PPT; ++pk) {
+    const int64_t i = (c - F.chunk0) * CH + pk * LT + threadIdx.x;
+    lwv[pk] = (want_tot && i < n) ? lwp[i] : 0.f;
+  }
   double gsum[3] = {0.0, 0.0, 0.0};  // tangent sums relative to the filter's maximum (GRAD)
   LseAcc r;
   {
     // two passes over the chunk partials: their maximum, then the rescaled sums (one exponential each;
-    // the tangent sums of a chunk are relative to its maximum, like its weight sums)
+    // the tangent sums of a chunk are relative to its maximum, like its weight sums). Filters of at
+    // most LT chunks (one partial per thread) load each partial once, in the first pass.
     const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
-    float m = -INFINITY;
+    const bool one = nchunk <= LT;
+    float m = -INFINITY, pm1 = -INFINITY;
+    double s1v = 0.0, s2v = 0.0, gv[3] = {0.0, 0.0, 0.0};
     int bad = 0;
-    for (int64_t k = threadIdx.x; k < nchunk; k += LT) {
-      const float pm = a.part_m[F.chunk0 + k];
-      if (isnan(pm))
-        bad = 1;
-      else
-        m = fmaxf(m, pm);
+    if (one) {
+      if (threadIdx.x < nchunk) {
+        const int64_t cc = F.chunk0 + threadIdx.x;
+        pm1 = a.part_m[cc];
+        s1v = a.part_s1[cc];
+        s2v = a.part_s2[cc];
+        if constexpr (GRAD)
+          for (int q = 0; q < 3; ++q) gv[q] = a.part_g[cc * 3 + q];
+        if (isnan(pm1))
+          bad = 1;
+        else
+          m = pm1;
+      }
+    } else {
+      for (int64_t k = threadIdx.x; k < nchunk; k += LT) {
+        const float pm = a.part_m[F.chunk0 + k];
+        if (isnan(pm))
+          bad = 1;
+        else
+          m = fmaxf(m, pm);
+      }
     }
     m = wave_max(m);
     bad = __any(bad) ? 1 : 0;
@@ -1278,7 +1313,15 @@ __global__ __launch_bounds__(LT) void lf_prep_kernel(LfArgs a, int64_t t) {
       anybad = anybad || s.u[i] != 0;
     }
     double s1 = 0.0, s2 = 0.0, g[3] = {0.0, 0.0, 0.0};
-    if (M != -INFINITY) {
+    if (M != -INFINITY && one) {
+      if (pm1 > -INFINITY) {
+        const double f = exp((double)pm1 - (double)M);
+        s1 += s1v * f;
+        s2 += s2v * f * f;
+        if constexpr (GRAD)
+          for (int q = 0; q < 3; ++q) g[q] += gv[q] * f;
+      }
+    } else if (M != -INFINITY) {
       for (int64_t k = threadIdx.x; k < nchunk; k += LT) {
         const int64_t cc = F.chunk0 + k;
         const float pm = a.part_m[cc];
@@ -1360,47 +1403,81 @@ __global__ __launch_bounds__(LT) void lf_prep_kernel(LfArgs a, int64_t t) {
     }
   }
   if (t >= F.T || !res || a.resampling == 2) return;
-  const float* lw = lf_lw_prev(a, t) + F.state_offset;
   uint64_t q = 0;
 #pragma unroll
   for (int pk = 0; pk < PPT; ++pk) {
     const int64_t i = (c - F.chunk0) * CH + pk * LT + threadIdx.x;
-    if (i < n) q += quantise(__expf(lw[i] - m), shift);
+    if (i < n) q += quantise(__expf(lwv[pk] - m), shift);
   }
   const uint64_t tot = lf_block_sum(q, s);
   if (threadIdx.x == 0) a.ctot[c] = tot;
 }
 
-__global__ __launch_bounds__(LT) void lf_csum_kernel(LfArgs a, int64_t t) {
+template <bool GRAD>
+__global__ __launch_bounds__(LT) void lf_prep_kernel(LfArgs a, int64_t t) {
   __shared__ LfScratch s;
-  const int64_t c = blockIdx.x;
-  const int fi = a.chunk_filter[c];
-  const LfDesc F = a.desc[fi];
+  lf_prep_chunk<GRAD>(a, t, blockIdx.x, s);
+}
+
+// cs: CH entries of LDS (this chunk's prefix sums, for the systematic boundary searches)
+MOCAT_DEV void lf_csum_chunk(const LfArgs& a, int64_t t, int64_t c, LfScratch& s, uint64_t* cs) {
+  const LfDesc F = a.cdesc[c];
+  const int fi = F.slot;
   if (t >= F.T) return;
-  const LfState st = a.st[fi];
-  if (!st.res || a.resampling == 2) return;
   const int64_t local = c - F.chunk0;
   const int64_t nchunk = (F.n + CH - 1) / CH;
+  // loads that do not depend on each other are issued first: the filter state, the chunk's weights and
+  // (one per thread, for filters of at most LT chunks) the chunk totals; the weights and totals are
+  // used on resampling steps only
+  const LfState st = a.st[fi];
+  const bool sums = a.resampling != 2;
+  const int64_t i0 = local * CH + (int64_t)threadIdx.x * PPT;  // thread tid owns PPT consecutive particles
+  const float* lw = lf_lw_prev(a, t) + F.state_offset;
+  float lwv[PPT];
+#pragma unroll
+  for (int pk = 0; pk < PPT; ++pk) lwv[pk] = (sums && i0 + pk < F.n) ? lw[i0 + pk] : 0.f;
+  const bool one = nchunk <= LT;
+  const uint64_t tot1 = (sums && one && threadIdx.x < nchunk) ? a.ctot[F.chunk0 + threadIdx.x] : 0ull;
+  if (a.pkeys) {
+    // split(step key, n) for the step kernel, one threefry block per key: block j of split's counter
+    // pairs (j, n + j) yields words j and n + j of the flat key array (key i = words 2i, 2i + 1), so
+    // the chunk's blocks j in [local * CH, (local + 1) * CH) fill words in both halves
+    const StepKey skv = a.skeys[F.key_slot + t];
+    const uint32_t un = (uint32_t)F.n;
+    uint32_t* PK = a.pkeys + F.chunk0 * (2 * CH);
+#pragma unroll
+    for (int pk = 0; pk < PPT; ++pk) {
+      const int64_t j = local * CH + pk * LT + threadIdx.x;
+      if (j < F.n) {
+        uint32_t x0 = (uint32_t)j, x1 = (uint32_t)j + un;
+        threefry2x32(skv.k[0], skv.k[1], x0, x1);
+        PK[j] = x0;
+        PK[j + F.n] = x1;
+      }
+    }
+  }
+  if (!st.res || !sums) return;
   uint64_t off = 0, wsum = 0;
-  for (int64_t k = threadIdx.x; k < nchunk; k += LT) {
-    const uint64_t v = a.ctot[F.chunk0 + k];
-    wsum += v;
-    if (k < local) off += v;
+  if (one) {
+    wsum = tot1;
+    off = threadIdx.x < local ? tot1 : 0ull;
+  } else {
+    for (int64_t k = threadIdx.x; k < nchunk; k += LT) {
+      const uint64_t v = a.ctot[F.chunk0 + k];
+      wsum += v;
+      if (k < local) off += v;
+    }
   }
   off = lf_block_sum(off, s);
   wsum = lf_block_sum(wsum, s);
   const float su = a.skeys[F.key_slot + t].su;  // the step's systematic uniform (init launch)
-  // thread tid owns the PPT consecutive particles from local * CH + tid * PPT
-  const int64_t i0 = local * CH + (int64_t)threadIdx.x * PPT;
-  const float* lw = lf_lw_prev(a, t) + F.state_offset;
   uint64_t q[PPT], own = 0;
 #pragma unroll
   for (int pk = 0; pk < PPT; ++pk) {
-    q[pk] = i0 + pk < F.n ? quantise(__expf(lw[i0 + pk] - st.m), st.shift) : 0ull;
+    q[pk] = i0 + pk < F.n ? quantise(__expf(lwv[pk] - st.m), st.shift) : 0ull;
     own += q[pk];
   }
   uint64_t run = off + lf_block_inclusive_scan(own, s) - own;
-  __shared__ uint64_t cs[CH];  // this chunk's prefix sums (systematic boundary searches)
 #pragma unroll
   for (int pk = 0; pk < PPT; ++pk) {
     run += q[pk];
@@ -1437,6 +1514,12 @@ __global__ __launch_bounds__(LT) void lf_csum_kernel(LfArgs a, int64_t t) {
     }
     a.bounds[F.chunk0 + b] = local * CH + (lo < count ? lo : count - 1);
   }
+}
+
+__global__ __launch_bounds__(LT) void lf_csum_kernel(LfArgs a, int64_t t) {
+  __shared__ LfScratch s;
+  __shared__ uint64_t cs[CH];
+  lf_csum_chunk(a, t, blockIdx.x, s, cs);
 }
 
 // dst[k] = src(k) for k < count, by the whole workgroup, U loads per thread issued before their stores.
@@ -1493,13 +1576,12 @@ MOCAT_DEV void upper_bound_lockstep(Get v, const int64_t* b, const int64_t* m, c
   for (int q = 0; q < P; ++q) out[q] = lo[q];
 }
 
+// win: kSearchWin entries of LDS; cend: cend_slots entries (staged by the multinomial search only)
 template <class Model, int D, int DY, bool GRAD>
-__global__ __launch_bounds__(LT) void lf_step_kernel(LfArgs a, int64_t t, int cend_slots) {
-  __shared__ LfScratch s;
-  __shared__ Model model;
-  const int64_t c = blockIdx.x;
-  const int fi = a.chunk_filter[c];
-  const LfDesc F = a.desc[fi];
+MOCAT_DEV void lf_step_chunk(const LfArgs& a, int64_t t, int cend_slots, int64_t c, LfScratch& s, Model& model,
+                             uint64_t* win, uint64_t* cend) {
+  const LfDesc F = a.cdesc[c];
+  const int fi = F.slot;
   if (t >= F.T) return;
   const int tid = threadIdx.x;
   const LfState st = a.st[fi];
@@ -1521,8 +1603,6 @@ __global__ __launch_bounds__(LT) void lf_step_kernel(LfArgs a, int64_t t, int ce
   // chunk's ancestors lie in a contiguous run of blocks, staged in LDS too when it is short. Each
   // thread searches for its PPT particles in lockstep (independent LDS or global reads overlap).
   // (res and the method are per filter: the branches are uniform over the workgroup.)
-  extern __shared__ uint64_t cend[];  // cend_slots entries (launch-time size)
-  __shared__ uint64_t win[kSearchWin];
   const uint64_t* C = a.csum + F.state_offset;
   const int64_t nsg = (n + SG - 1) / SG;
   // systematic: the chunk's ancestor window comes from the prefix-sum pass (bounds); multinomial
@@ -1547,6 +1627,14 @@ __global__ __launch_bounds__(LT) void lf_step_kernel(LfArgs a, int64_t t, int ce
   const uint32_t sk[4] = {skv.k[0], skv.k[1], skv.k[2], skv.k[3]};
   const float su = skv.su;
   load_shared_params(model, a.params + F.param_offset);
+  // the particles' split keys (lf_csum) are loaded before the ancestor search, which they do not need
+  const uint32_t* PK = a.pkeys + F.chunk0 * (2 * CH);
+  uint2 kwp[PPT];
+#pragma unroll
+  for (int pk = 0; pk < PPT; ++pk) {
+    const int64_t i = base + pk * LT + tid;
+    kwp[pk] = (!Model::kOptimal && i < n) ? *reinterpret_cast<const uint2*>(PK + 2 * i) : make_uint2(0u, 0u);
+  }
   __syncthreads();
   const Key ik{sk[0], sk[1]};
   auto cend_at = [&](int64_t k) { return cend[k]; };
@@ -1649,7 +1737,7 @@ __global__ __launch_bounds__(LT) void lf_step_kernel(LfArgs a, int64_t t, int ce
       shared_eps<D>(Key{sk[2], sk[3]}, (uint32_t)(n * D), (uint32_t)i, eps[pk]);
       continue;
     }
-    const Key ki = split_at(ik, (uint32_t)n, (uint32_t)i);
+    const Key ki{kwp[pk].x, kwp[pk].y};  // split(step key, n)[i] (lf_csum)
 #pragma unroll
     for (uint32_t q = 0; q < hD; ++q) {
       uint32_t w0, w1;
@@ -1743,11 +1831,19 @@ __global__ __launch_bounds__(LT) void lf_step_kernel(LfArgs a, int64_t t, int ce
   }
 }
 
+template <class Model, int D, int DY, bool GRAD>
+__global__ __launch_bounds__(LT) void lf_step_kernel(LfArgs a, int64_t t, int cend_slots) {
+  __shared__ LfScratch s;
+  __shared__ Model model;
+  __shared__ uint64_t win[kSearchWin];
+  extern __shared__ uint64_t cend[];  // cend_slots entries (launch-time size)
+  lf_step_chunk<Model, D, DY, GRAD>(a, t, cend_slots, blockIdx.x, s, model, win, cend);
+}
+
 // Final particles of filters whose last step wrote buffer 1 are copied to buffer 0 (as pf_kernel).
 template <int D>
-__global__ __launch_bounds__(LT) void lf_finish_kernel(LfArgs a) {
-  const int64_t c = blockIdx.x;
-  const LfDesc F = a.desc[a.chunk_filter[c]];
+MOCAT_DEV void lf_finish_chunk(const LfArgs& a, int64_t c) {
+  const LfDesc F = a.cdesc[c];
   if (!((F.T - 1) & 1)) return;
   for (int pk = 0; pk < PPT; ++pk) {
     const int64_t i = (c - F.chunk0) * CH + pk * LT + threadIdx.x;
@@ -1755,6 +1851,116 @@ __global__ __launch_bounds__(LT) void lf_finish_kernel(LfArgs a) {
 #pragma unroll
     for (int q = 0; q < D; ++q) a.x0[(F.state_offset + i) * D + q] = a.x1[(F.state_offset + i) * D + q];
     a.lw0[F.state_offset + i] = a.lw1[F.state_offset + i];
+  }
+}
+
+template <int D>
+__global__ __launch_bounds__(LT) void lf_finish_kernel(LfArgs a) {
+  lf_finish_chunk<D>(a, blockIdx.x);
+}
+
+// Grid-wide barrier of the persistent kernel. Every workgroup adds one to the arrival counter after a
+// release fence; barrier k is passed once the counter reaches k * gridDim.x, followed by an acquire
+// fence. The wait is bounded: a workgroup that polls kBarrierPolls times without seeing every arrival
+// sets the abort flag, and every workgroup leaves the kernel at its next barrier once the flag is set
+// (lf_abort_kernel then marks the results). All workgroups are resident at once (the grid is sized by
+// the occupancy limit), so a barrier is only ever waiting on running workgroups.
+constexpr uint32_t kBarrierPolls = 1u << 22;
+
+MOCAT_DEV bool lf_grid_sync(uint32_t* bar, uint32_t k) {
+  __shared__ int pass;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __hip_atomic_fetch_add(&bar[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint32_t target = k * gridDim.x;
+    int ok = 1;
+    for (uint32_t polls = 0;; ++polls) {
+      if (__hip_atomic_load(&bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) {
+        ok = 0;
+        break;
+      }
+      if (__hip_atomic_load(&bar[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= target) break;
+      if (polls >= kBarrierPolls) {
+        __hip_atomic_store(&bar[1], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        ok = 0;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    pass = ok;
+  }
+  __syncthreads();
+  return pass != 0;
+}
+
+// The phases as calls (not inlined): each allocates its registers on its own, and the kernel keeps
+// only the loop state live across them.
+template <bool GRAD>
+__device__ __noinline__ void lf_prep_call(const LfArgs& a, int64_t t, int64_t c, LfScratch& s) {
+  lf_prep_chunk<GRAD>(a, t, c, s);
+}
+__device__ __noinline__ void lf_csum_call(const LfArgs& a, int64_t t, int64_t c, LfScratch& s, uint64_t* cs) {
+  lf_csum_chunk(a, t, c, s, cs);
+}
+template <class Model, int D, int DY, bool GRAD>
+__device__ __noinline__ void lf_step_call(const LfArgs& a, int64_t t, int cend_slots, int64_t c, LfScratch& s,
+                                          Model& model, uint64_t* win, uint64_t* cend) {
+  lf_step_chunk<Model, D, DY, GRAD>(a, t, cend_slots, c, s, model, win, cend);
+}
+
+// launch_mode 2: steps 1 .. max_T - 1 (after lf_init_kernel) as one persistent kernel of gridDim.x
+// workgroups. Each step runs the three phases of the per-step launches (prep, prefix sums, step), a
+// grid barrier after each; in a phase workgroup b takes chunks b, b + G, b + 2G, ... of the chunks
+// still running. The filters are laid out in order of decreasing length, so the chunks running at
+// step t are the first act[t] (act[t] = chunks of filters with T > t). The chunk functions are those
+// of the per-step kernels, so the results are the same bit for bit.
+template <class Model, int D, int DY, bool GRAD>
+__global__ __launch_bounds__(LT) void lf_persist_kernel(LfArgs a, int64_t max_T, int cend_slots,
+                                                        const int32_t* __restrict__ act) {
+  __shared__ LfScratch s;
+  __shared__ Model model;
+  __shared__ uint64_t buf[kSearchWin];  // the prefix-sum phase's chunk sums, then the step's window
+  extern __shared__ uint64_t cend[];
+  static_assert(kSearchWin >= CH, "the LDS buffer holds a chunk of prefix sums");
+  const int64_t G = gridDim.x;
+  uint32_t k = 0;
+  for (int64_t t = 1; t < max_T; ++t) {
+    const int64_t ap = act[t - 1], as = act[t];
+    for (int64_t c = blockIdx.x; c < ap; c += G) {
+      __syncthreads();
+      lf_prep_call<GRAD>(a, t, c, s);
+    }
+    if (!lf_grid_sync(a.bar, ++k)) return;
+    for (int64_t c = blockIdx.x; c < as; c += G) {
+      __syncthreads();
+      lf_csum_call(a, t, c, s, buf);
+    }
+    if (!lf_grid_sync(a.bar, ++k)) return;
+    for (int64_t c = blockIdx.x; c < as; c += G) {
+      __syncthreads();
+      lf_step_call<Model, D, DY, GRAD>(a, t, cend_slots, c, s, model, buf, cend);
+    }
+    if (!lf_grid_sync(a.bar, ++k)) return;
+  }
+  const int64_t al = act[max_T - 1];
+  for (int64_t c = blockIdx.x; c < al; c += G) {
+    __syncthreads();
+    lf_prep_call<GRAD>(a, max_T, c, s);  // the last step's statistics
+  }
+  if (!lf_grid_sync(a.bar, ++k)) return;
+  for (int64_t c = blockIdx.x; c < a.n_chunks; c += G) lf_finish_chunk<D>(a, c);
+}
+
+// After the persistent kernel: if a barrier gave up, every filter's log-likelihood (and gradient) is NaN.
+__global__ __launch_bounds__(256) void lf_abort_kernel(LfArgs a) {
+  if (__hip_atomic_load(&a.bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u) return;
+  for (int i = threadIdx.x; i < a.n_large; i += 256) {
+    const int f = a.desc[i].f;
+    a.out_loglik[f] = NAN;
+    if (a.out_grad)
+      for (int q = 0; q < 3; ++q) a.out_grad[(int64_t)f * 3 + q] = NAN;
   }
 }
 
@@ -1853,23 +2059,45 @@ hipStream_t capture_stream() {
   return streams[dev];
 }
 
-// launch_mode 0: the per-step launches go to the stream one by one. launch_mode 1: they are captured
-// into one graph (kernel nodes only) that is instantiated and launched on the stream as a single
-// submission; the call returns after the graph has run (the instantiated graph is released). When the
-// caller's stream is itself being captured, the launches simply join the caller's graph.
+// launch_mode 0: the per-step launches go to the stream one by one, each over the chunks still
+// running (act: host copy of the running-chunk counts). launch_mode 1: they are captured into one
+// graph (kernel nodes only) that is instantiated and launched on the stream as a single submission;
+// the call returns after the graph has run (the instantiated graph is released). When the caller's
+// stream is itself being captured, the launches simply join the caller's graph. launch_mode 2: the
+// init launch, then lf_persist_kernel for all steps, then lf_abort_kernel.
+constexpr int kPersistBlocksPerCu = 4;
+
 template <class Model, int D, int DY, bool GRAD = false>
-int launch_pf_large(const LfArgs& a, int64_t max_T, int cend_slots, int launch_mode, hipStream_t st) {
+int launch_pf_large(const LfArgs& a, int64_t max_T, int cend_slots, const int32_t* act_dev,
+                    const std::vector<int32_t>& act, int launch_mode, hipStream_t st) {
   const dim3 chunks((unsigned)a.n_chunks), blk(LT);
   // the block-end sums are staged only by the multinomial search
   const size_t cend_bytes = a.resampling == 1 ? (size_t)cend_slots * sizeof(uint64_t) : 0;
+  auto grid = [&](int64_t t) { return dim3((unsigned)std::max<int32_t>(act[t], 1)); };
+  if (launch_mode == 2) {
+    auto kern = lf_persist_kernel<Model, D, DY, GRAD>;
+    int dev = 0, cus = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(kern), LT,
+                                                     cend_bytes) != hipSuccess)
+      return MOCAT_ERR_HIP;
+    if (per_cu < 1 || cus < 1) return MOCAT_ERR_SIZE;
+    const int64_t G = std::min<int64_t>(a.n_chunks, (int64_t)cus * std::min(per_cu, kPersistBlocksPerCu));
+    hipLaunchKernelGGL((lf_init_kernel<Model, D, DY, GRAD>), chunks, blk, 0, st, a);
+    hipLaunchKernelGGL(kern, dim3((unsigned)G), blk, cend_bytes, st, a, max_T, cend_slots, act_dev);
+    hipLaunchKernelGGL(lf_abort_kernel, dim3(1), dim3(256), 0, st, a);
+    MOCAT_LAUNCH_CHECK();
+    return MOCAT_OK;
+  }
   auto enqueue = [&](hipStream_t q) {
     hipLaunchKernelGGL((lf_init_kernel<Model, D, DY, GRAD>), chunks, blk, 0, q, a);
     for (int64_t t = 1; t < max_T; ++t) {
-      hipLaunchKernelGGL((lf_prep_kernel<GRAD>), chunks, blk, 0, q, a, t);
-      hipLaunchKernelGGL(lf_csum_kernel, chunks, blk, 0, q, a, t);
-      hipLaunchKernelGGL((lf_step_kernel<Model, D, DY, GRAD>), chunks, blk, cend_bytes, q, a, t, cend_slots);
+      hipLaunchKernelGGL((lf_prep_kernel<GRAD>), grid(t - 1), blk, 0, q, a, t);
+      hipLaunchKernelGGL(lf_csum_kernel, grid(t), blk, 0, q, a, t);
+      hipLaunchKernelGGL((lf_step_kernel<Model, D, DY, GRAD>), grid(t), blk, cend_bytes, q, a, t, cend_slots);
     }
-    hipLaunchKernelGGL((lf_prep_kernel<GRAD>), chunks, blk, 0, q, a, max_T);  // the last step's statistics
+    hipLaunchKernelGGL((lf_prep_kernel<GRAD>), grid(max_T - 1), blk, 0, q, a, max_T);  // the last step's statistics
     hipLaunchKernelGGL((lf_finish_kernel<D>), chunks, blk, 0, q, a);
   };
   hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
@@ -1913,7 +2141,7 @@ struct LfLayout {
   int cend_slots = 1;  // SG-block end sums the step kernel stages: the largest filter's, up to kChunkEnds
   int64_t n_chunks = 0, max_T = 0;
   int64_t n_tk = 0;  // step-key slots: the sum of T over the flagged filters
-  int64_t o_desc, o_cf, o_m, o_s1, o_s2, o_g, o_tot, o_st, o_cend, o_bnd, o_keys, bytes;
+  int64_t o_desc, o_cf, o_m, o_s1, o_s2, o_g, o_tot, o_st, o_cend, o_bnd, o_keys, o_act, o_bar, o_pk, bytes;
 };
 
 int64_t lf_align(int64_t v) { return (v + 255) / 256 * 256; }
@@ -1932,7 +2160,7 @@ LfLayout lf_layout(const mocat_pf_filter* filters, int n_filters) {
   L.o_desc = o;
   o = lf_align(o + (int64_t)L.n_large * (int64_t)sizeof(LfDesc));
   L.o_cf = o;
-  o = lf_align(o + L.n_chunks * 4);
+  o = lf_align(o + L.n_chunks * (int64_t)sizeof(LfDesc));
   L.o_m = o;
   o = lf_align(o + L.n_chunks * 4);
   L.o_s1 = o;
@@ -1951,6 +2179,12 @@ LfLayout lf_layout(const mocat_pf_filter* filters, int n_filters) {
   o = lf_align(o + L.n_chunks * 8);
   L.o_keys = o;
   o = lf_align(o + L.n_tk * (int64_t)sizeof(StepKey));
+  L.o_act = o;
+  o = lf_align(o + (L.max_T + 1) * 4);
+  L.o_bar = o;
+  o = lf_align(o + 8);
+  L.o_pk = o;
+  o = lf_align(o + L.n_chunks * CH * 8);
   L.bytes = o;
   return L;
 }
@@ -2077,7 +2311,7 @@ int mocat_pf_run_large(int model, int d, int dy, const mocat_pf_filter* host_fil
                        double* out_grad, void* workspace, int64_t workspace_bytes, int launch_mode,
                        hipStream_t stream) {
   if (n_filters < 0 || !host_filters || !params || !y || !x_ws0 || !x_ws1 || !lw_ws0 || !lw_ws1 || !csum_ws ||
-      !out_loglik || resampling < 0 || resampling > 2 || launch_mode < 0 || launch_mode > 1)
+      !out_loglik || resampling < 0 || resampling > 2 || launch_mode < 0 || launch_mode > 2)
     return MOCAT_ERR_ARG;
   if (out_grad && (model != 1 || !tan_ws0 || !tan_ws1)) return MOCAT_ERR_ARG;
   const LfLayout L = lf_layout(host_filters, n_filters);
@@ -2085,13 +2319,19 @@ int mocat_pf_run_large(int model, int d, int dy, const mocat_pf_filter* host_fil
   if (!workspace || workspace_bytes < L.bytes) return MOCAT_ERR_SIZE;
   if (L.n_chunks > 0x7FFFFFFFll || L.n_large > 0x7FFFFFFF) return MOCAT_ERR_SIZE;
   std::vector<LfDesc> desc;
-  std::vector<int32_t> chunk_filter;
+  std::vector<LfDesc> cdesc;
   desc.reserve(L.n_large);
-  chunk_filter.reserve(L.n_chunks);
+  cdesc.reserve(L.n_chunks);
+  // filters in order of decreasing length (ties in input order): the chunks still running at any step
+  // are then a prefix of the chunk list
+  std::vector<int> order;
+  for (int f = 0; f < n_filters; ++f)
+    if (host_filters[f].flags & kPfLarge) order.push_back(f);
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int x, int y) { return host_filters[x].T > host_filters[y].T; });
   int64_t chunk0 = 0, key_slot = 0;
-  for (int f = 0; f < n_filters; ++f) {
+  for (const int f : order) {
     const mocat_pf_filter& F = host_filters[f];
-    if (!(F.flags & kPfLarge)) continue;
     if (F.n <= 0 || F.T <= 0 || F.n > 0x7FFFFFFFll || F.T > 0xFFFFFFFFll || F.state_offset < 0 ||
         F.y_offset < 0 || F.param_offset < 0)
       return MOCAT_ERR_ARG;
@@ -2101,46 +2341,56 @@ int mocat_pf_run_large(int model, int d, int dy, const mocat_pf_filter* host_fil
     if (out_grad && (F.flags & kPfResume)) return MOCAT_ERR_ARG;
     const int64_t nchunk = (F.n + CH - 1) / CH;
     desc.push_back(LfDesc{F.n, F.T, F.y_offset, F.state_offset, F.out_offset, F.t_offset, chunk0, key_slot,
-                          {F.key[0], F.key[1]}, F.param_offset, f, F.flags});
+                          {F.key[0], F.key[1]}, F.param_offset, f, F.flags, (int32_t)desc.size()});
     key_slot += F.T;
-    for (int64_t k = 0; k < nchunk; ++k) chunk_filter.push_back((int32_t)(desc.size() - 1));
+    for (int64_t k = 0; k < nchunk; ++k) cdesc.push_back(desc.back());
     chunk0 += nchunk;
+  }
+  // act[t]: chunks of the filters with T > t, t = 0 .. max_T
+  std::vector<int32_t> act((size_t)L.max_T + 1, 0);
+  for (const LfDesc& F : desc) {
+    const int32_t nchunk = (int32_t)((F.n + CH - 1) / CH);
+    for (int64_t t = 0; t < F.T; ++t) act[(size_t)t] += nchunk;
   }
   char* ws = static_cast<char*>(workspace);
   if (hipMemcpyAsync(ws + L.o_desc, desc.data(), desc.size() * sizeof(LfDesc), hipMemcpyHostToDevice, stream) !=
           hipSuccess ||
-      hipMemcpyAsync(ws + L.o_cf, chunk_filter.data(), chunk_filter.size() * sizeof(int32_t), hipMemcpyHostToDevice,
+      hipMemcpyAsync(ws + L.o_cf, cdesc.data(), cdesc.size() * sizeof(LfDesc), hipMemcpyHostToDevice,
                      stream) != hipSuccess ||
+      hipMemcpyAsync(ws + L.o_act, act.data(), act.size() * sizeof(int32_t), hipMemcpyHostToDevice, stream) !=
+          hipSuccess ||
       hipStreamSynchronize(stream) != hipSuccess)  // the host vectors are released on return
     return MOCAT_ERR_HIP;
-  LfArgs a{reinterpret_cast<const LfDesc*>(ws + L.o_desc), reinterpret_cast<const int32_t*>(ws + L.o_cf), L.n_large,
+  const int32_t* act_dev = reinterpret_cast<const int32_t*>(ws + L.o_act);
+  LfArgs a{reinterpret_cast<const LfDesc*>(ws + L.o_desc), reinterpret_cast<const LfDesc*>(ws + L.o_cf), L.n_large,
            L.n_chunks, params, y, step_scale, x_ws0, x_ws1, lw_ws0, lw_ws1, csum_ws, tan_ws0, tan_ws1, resampling,
            ess_threshold, out_value, out_lw, out_anc, out_ess, out_loglik_t, out_loglik, out_grad,
            reinterpret_cast<float*>(ws + L.o_m), reinterpret_cast<double*>(ws + L.o_s1),
            reinterpret_cast<double*>(ws + L.o_s2), reinterpret_cast<double*>(ws + L.o_g),
            reinterpret_cast<uint64_t*>(ws + L.o_tot), reinterpret_cast<LfState*>(ws + L.o_st),
            reinterpret_cast<uint64_t*>(ws + L.o_cend), reinterpret_cast<int64_t*>(ws + L.o_bnd),
-           reinterpret_cast<StepKey*>(ws + L.o_keys)};
+           reinterpret_cast<StepKey*>(ws + L.o_keys), reinterpret_cast<uint32_t*>(ws + L.o_bar),
+           model == 2 ? nullptr : reinterpret_cast<uint32_t*>(ws + L.o_pk)};
   if (model == 1) {
     if (d != dy) return MOCAT_ERR_ARG;
     if (d != 1 && out_grad) return MOCAT_ERR_ARG;
-    if (d == 2) return launch_pf_large<SvD<2>, 2, 2>(a, L.max_T, L.cend_slots, launch_mode, stream);
-    if (d == 3) return launch_pf_large<SvD<3>, 3, 3>(a, L.max_T, L.cend_slots, launch_mode, stream);
-    if (d == 4) return launch_pf_large<SvD<4>, 4, 4>(a, L.max_T, L.cend_slots, launch_mode, stream);
+    if (d == 2) return launch_pf_large<SvD<2>, 2, 2>(a, L.max_T, L.cend_slots, act_dev, act, launch_mode, stream);
+    if (d == 3) return launch_pf_large<SvD<3>, 3, 3>(a, L.max_T, L.cend_slots, act_dev, act, launch_mode, stream);
+    if (d == 4) return launch_pf_large<SvD<4>, 4, 4>(a, L.max_T, L.cend_slots, act_dev, act, launch_mode, stream);
     if (d != 1) return MOCAT_ERR_ARG;
-    return out_grad ? launch_pf_large<Sv, 1, 1, true>(a, L.max_T, L.cend_slots, launch_mode, stream)
-                    : launch_pf_large<Sv, 1, 1>(a, L.max_T, L.cend_slots, launch_mode, stream);
+    return out_grad ? launch_pf_large<Sv, 1, 1, true>(a, L.max_T, L.cend_slots, act_dev, act, launch_mode, stream)
+                    : launch_pf_large<Sv, 1, 1>(a, L.max_T, L.cend_slots, act_dev, act, launch_mode, stream);
   }
   if (model == 2) {
 #define MOCAT_PF_CASE(DD, YY) \
-  if (d == DD && dy == YY) return launch_pf_large<LgssmOpt<DD, YY>, DD, YY>(a, L.max_T, L.cend_slots, launch_mode, stream);
+  if (d == DD && dy == YY) return launch_pf_large<LgssmOpt<DD, YY>, DD, YY>(a, L.max_T, L.cend_slots, act_dev, act, launch_mode, stream);
     MOCAT_PF_LG_SHAPES
 #undef MOCAT_PF_CASE
     return MOCAT_ERR_ARG;
   }
   if (model != 0) return MOCAT_ERR_ARG;
 #define MOCAT_PF_CASE(DD, YY) \
-  if (d == DD && dy == YY) return launch_pf_large<Lgssm<DD, YY>, DD, YY>(a, L.max_T, L.cend_slots, launch_mode, stream);
+  if (d == DD && dy == YY) return launch_pf_large<Lgssm<DD, YY>, DD, YY>(a, L.max_T, L.cend_slots, act_dev, act, launch_mode, stream);
   MOCAT_PF_LG_SHAPES
 #undef MOCAT_PF_CASE
   return MOCAT_ERR_ARG;

@@ -365,10 +365,11 @@ def _large_flags(ns, Ts, large_n):
 
 
 def _launch(models, ys, keys, ns, ess_threshold, resampling, store, grad=False, large_n=None, particle_filter=None,
-            initial=None, extra_flags=0, graph=False, ts=None):
+            initial=None, extra_flags=0, launch_mode=0, ts=None):
     """Runs len(models) filters (same model class and dims): filters below large_n particles in one
-    persistent launch (a workgroup per filter), the others through the chunked per-step path (its
-    launches captured into one HIP graph when graph=True)."""
+    persistent launch (a workgroup per filter), the others through the chunked per-step path
+    (launch_mode 0: three stream launches per step; 1: those launches captured into one HIP graph;
+    2: one persistent kernel for all steps)."""
     m0 = models[0]
     d, dy, kind = m0.dim, m0.dim_obs, m0.kernel_model
     for mdl in models:
@@ -465,7 +466,7 @@ def _launch(models, ys, keys, ns, ess_threshold, resampling, store, grad=False, 
                   _lib.ptr(x1), _lib.ptr(lw0), _lib.ptr(lw1), _lib.ptr(cs), RESAMPLING[resampling],
                   float(ess_threshold), _lib.ptr(out_value), _lib.ptr(out_lw), _lib.ptr(out_anc), _lib.ptr(out_ess),
                   _lib.ptr(out_llt), _lib.ptr(out_ll), _lib.ptr(tan0), _lib.ptr(tan1), _lib.ptr(out_grad),
-                  _lib.ptr(ws), int(ws.numel()), 1 if graph else 0, _lib.stream_handle())
+                  _lib.ptr(ws), int(ws.numel()), int(launch_mode), _lib.stream_handle())
     if side is not None and side is not cur:
         cur.wait_stream(side)
     res = [None] * F
@@ -646,8 +647,8 @@ def run_particle_filters(ssm_scenarios, ys, random_keys, ns, ess_threshold: floa
     of the fixed-resampling-index estimator. Filters with at least large_n particles (default 4096)
     run on the chunked path spread over the whole GPU; the results do not depend on the path.
     particle_filter: BootstrapFilter (default) or OptimalNonLinearGaussianParticleFilter.
-    schedule: None (large_n as given, stream launches), a dict {"large_n": int, "graph": bool}, or
-    "auto": the schedule autotuner (mocat_amd.ssm.schedule) times the candidate schedules on this
+    schedule: None (large_n as given, stream launches), a dict {"large_n": int, "launch": "stream" |
+    "graph" | "persistent"} (or {"large_n": int, "graph": bool}), or "auto": the schedule autotuner (mocat_amd.ssm.schedule) times the candidate schedules on this
     batch the first time its shape is seen, records the fastest, and every later call of the same
     shape uses it.
     ts: optional list of observation-time arrays (one per filter, as ys); the stochastic-volatility
@@ -658,20 +659,20 @@ def run_particle_filters(ssm_scenarios, ys, random_keys, ns, ess_threshold: floa
     if resampling == "categorical" and any(int(v) * int(v) >= 0xFFFFFFFF for v in ns):
         raise ValueError("categorical resampling draws an (n, n) Gumbel matrix: n * n must be < 2^32 - 1")
     models, keys, ns = list(ssm_scenarios), list(random_keys), [int(v) for v in ns]
-    graph = False
+    mode = 0
     if schedule is not None:
         from . import schedule as sched
         if schedule == "auto":
             run = lambda c: _launch(models, ys, keys, ns, ess_threshold, resampling, store, grad, c["large_n"],
-                                    particle_filter, graph=c["graph"], ts=ts)
+                                    particle_filter, launch_mode=sched.launch_mode(c), ts=ts)
             choice = sched.choose(sched.signature(models, ys, ns, resampling, grad, store, particle_filter), run)
         elif isinstance(schedule, dict):
             choice = schedule
         else:
-            raise ValueError('schedule must be None, "auto" or a dict {"large_n": int, "graph": bool}')
-        large_n, graph = int(choice["large_n"]), bool(choice["graph"])
-    res = _launch(models, ys, keys, ns, ess_threshold, resampling, store, grad, large_n, particle_filter, graph=graph,
-                  ts=ts)
+            raise ValueError('schedule must be None, "auto" or a dict {"large_n": int, "launch": str}')
+        large_n, mode = int(choice["large_n"]), sched.launch_mode(choice)
+    res = _launch(models, ys, keys, ns, ess_threshold, resampling, store, grad, large_n, particle_filter,
+                  launch_mode=mode, ts=ts)
     return [cdict(**{k: v for k, v in r.items() if k != "desc"}) for r in res]
 
 

@@ -2,11 +2,12 @@
 
 A batch of independent filters of uneven sizes is split between two launch structures: filters below
 a particle-count threshold run one workgroup each in a single persistent launch; the others run on the
-chunked path (1024-particle chunks spread over the GPU, three grid-wide launches per time step),
-either queued launch by launch or captured into one HIP graph. Which split and which launch mode is
-fastest depends on the batch (its size distribution and lengths), so the tuner times every candidate
-schedule on the batch itself, keeps the fastest, and records it under the batch's shape signature: in
-memory for the process and in a JSON file ($MOCAT_AMD_SCHEDULE_CACHE, else
+chunked path (1024-particle chunks spread over the GPU, three grid-wide phases per time step), either
+as three launches per step queued one by one ("stream"), those launches captured into one HIP graph
+("graph"), or one persistent kernel with grid barriers between the phases ("persistent"). Which split
+and which launch mode is fastest depends on the batch (its size distribution and lengths), so the
+tuner times every candidate schedule on the batch itself, keeps the fastest, and records it under the
+batch's shape signature: in memory for the process and in a JSON file ($MOCAT_AMD_SCHEDULE_CACHE, else
 ~/.cache/mocat_amd/pf_schedule.json) for later processes. Results do not depend on the schedule (the
 filters' key trees and arithmetic are the same on both paths), so tuning changes only the time.
 """
@@ -21,10 +22,22 @@ from pathlib import Path
 import numpy as np
 import torch
 
-CANDIDATE_LARGE_N = (1024, 2048, 4096, 8192, 16384)
-CANDIDATE_GRAPH = (False, True)
+CANDIDATE_LARGE_N = (1, 1024, 2048, 4096, 8192, 16384)  # 1: every filter on the chunked path
+# the persistent kernel is not a default candidate: its grid barriers (an agent-scope L2 writeback and
+# invalidate per workgroup) cost more than the kernel boundaries they replace on the batches measured
+CANDIDATE_LAUNCH = ("stream", "graph")
+LAUNCH_MODES = {"stream": 0, "graph": 1, "persistent": 2}
 
 _MEMO: dict = {}
+
+
+def launch_mode(choice: dict) -> int:
+    """The chunked path's launch mode of a schedule: its "launch" entry, else the older "graph" flag."""
+    if "launch" in choice:
+        if choice["launch"] not in LAUNCH_MODES:
+            raise ValueError(f"launch must be one of {sorted(LAUNCH_MODES)}")
+        return LAUNCH_MODES[choice["launch"]]
+    return 1 if choice.get("graph") else 0
 
 
 def cache_path() -> Path:
@@ -77,7 +90,7 @@ def tune(run, candidates=None, repeats: int = 2) -> dict:
     """Times run(schedule) for every candidate schedule (one untimed run each first, then the best of
     `repeats` timed runs, device-synchronised) and returns the fastest with all timings."""
     if candidates is None:
-        candidates = [{"large_n": n, "graph": g} for n in CANDIDATE_LARGE_N for g in CANDIDATE_GRAPH]
+        candidates = [{"large_n": n, "launch": g} for n in CANDIDATE_LARGE_N for g in CANDIDATE_LAUNCH]
     timings = []
     for c in candidates:
         run(c)
@@ -92,14 +105,15 @@ def tune(run, candidates=None, repeats: int = 2) -> dict:
     timings.sort(key=lambda tc: tc[0])
     choice = dict(timings[0][1])
     choice["ms"] = timings[0][0] * 1e3
-    choice["timings_ms"] = {f"large_n={c['large_n']},graph={int(c['graph'])}": t * 1e3 for t, c in timings}
+    choice["timings_ms"] = {f"large_n={c['large_n']},launch={c['launch']}": t * 1e3 for t, c in timings}
     return choice
 
 
 def choose(sig: str, run) -> dict:
-    """The recorded schedule for this signature, tuning (and recording) it on first use."""
+    """The recorded schedule for this signature, tuning (and recording) it on first use (records
+    without a launch entry predate the persistent mode and are tuned again)."""
     rec = lookup(sig)
-    if rec is None:
+    if rec is None or "launch" not in rec:
         rec = tune(run)
         record(sig, rec)
     return rec

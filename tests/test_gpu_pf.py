@@ -511,6 +511,61 @@ def test_chunked_path_wide_ancestor_windows(m):
     assert any(c < n // 10 and w > 2048 for c, w in spread), spread
 
 
+def _pf_outputs_equal(a, b):
+    for key in ("log_likelihood", "ess", "final_value", "final_log_weight", "value", "log_weight", "ancestors",
+                "log_likelihood_grad"):
+        if hasattr(a, key):
+            assert torch.equal(getattr(a, key), getattr(b, key)), key
+
+
+@pytest.mark.parametrize("case", ["sv", "sv_grad", "sv3_times", "lgssm_multinomial", "lgssm_categorical",
+                                  "optimal"])
+def test_persistent_chunked_kernel_repeats_the_stream_launches(m, case):
+    """launch "persistent" (one kernel for all steps, grid barriers between the phases) against the
+    three launches per step, every filter on the chunked path (large_n = 1), ragged n and T: every
+    output bit-identical (log-likelihoods, ESS, stored particles, weights and ancestors, gradients)."""
+    from mocat_amd import datasets
+    rs = np.random.default_rng(len(case))
+    ns = [1, 700, 1024, 1025, 3000, 9000, 20000]
+    Ts = [1, 2, 9, 31, 17, 40, 25]
+    grad, pf, resampling, ts = False, None, "systematic", None
+    if case.startswith("sv"):
+        d = 3 if case == "sv3_times" else 1
+        model = (m.ssm.StochasticVolatility(3, np.zeros(3), np.eye(3), 0.9 * np.eye(3), 0.05 * np.eye(3), np.ones(3))
+                 if d == 3 else m.ssm.StochasticVolatility(1, 0.0, 1.0, 0.95, 0.04, 0.6))
+        p = (opf.svd_params(np.zeros(3), np.eye(3), 0.9 * np.eye(3), 0.05 * np.eye(3), np.ones(3)) if d == 3
+             else opf.sv_params(0.95, 0.2, np.sqrt(0.6)))
+        ys = [opf.simulate(p, T, orng.PRNGKey(T))[1].astype(np.float32) for T in Ts]
+        grad = case == "sv_grad"
+        if d == 3:
+            ts = [np.cumsum(rs.uniform(0.2, 2.0, T)) for T in Ts]
+    else:
+        model, y, _ = datasets.lgssm_config(0, 4, 40)
+        ys = [y.cpu().numpy()[:T] for T in Ts]
+        resampling = {"lgssm_multinomial": "multinomial", "lgssm_categorical": "categorical"}.get(case, "systematic")
+        if case == "optimal":
+            pf = m.ssm.OptimalNonLinearGaussianParticleFilter
+    if resampling == "categorical":
+        ns = [v for v in ns if v * v < 0xFFFFFFFF]
+        ys = ys[:len(ns)]
+    keys = [k for k in m.random.split(m.random.PRNGKey(4), len(ns))]
+    models = [model] * len(ns)
+    thr = 0.7
+    run = lambda launch: m.ssm.run_particle_filters(models, ys, keys, ns, thr, resampling, store=True, grad=grad,
+                                                    particle_filter=pf, ts=ts,
+                                                    schedule={"large_n": 1, "launch": launch})
+    base, pers = run("stream"), run("persistent")
+    for a, b in zip(base, pers):
+        _pf_outputs_equal(a, b)
+        assert np.isfinite(b.log_likelihood.item())
+    # and the stream launches repeat the one-workgroup path's filters to FP64 summation order
+    small = m.ssm.run_particle_filters(models, ys, keys, ns, thr, resampling, store=True, grad=grad,
+                                       particle_filter=pf, ts=ts, large_n=1 << 40)
+    for a, b in zip(small, pers):
+        x, y = a.log_likelihood.item(), b.log_likelihood.item()
+        assert abs(x - y) <= 1e-9 * (1 + abs(y)), (x, y)
+
+
 @pytest.mark.parametrize("grad", [False, True])
 def test_graph_launch_and_tuned_schedule_give_the_same_results(m, grad, tmp_path, monkeypatch):
     # the chunked path's per-step launches captured into one HIP graph, and the autotuner's choice of
@@ -524,16 +579,18 @@ def test_graph_launch_and_tuned_schedule_give_the_same_results(m, grad, tmp_path
     assert any(n >= 4096 for n in ns) and any(n < 4096 for n in ns)
     monkeypatch.setenv("MOCAT_AMD_SCHEDULE_CACHE", str(tmp_path / "sched.json"))
     sched._MEMO.clear()
-    outs = [run(schedule={"large_n": 4096, "graph": True}), run(schedule={"large_n": 1024, "graph": True}),
-            run(schedule="auto")]
+    outs = [run(schedule={"large_n": 4096, "graph": True}), run(schedule={"large_n": 4096, "launch": "persistent"}),
+            run(schedule={"large_n": 1024, "graph": True}), run(schedule="auto")]
     rec = sched.lookup(sched.signature(models, ys, ns, "systematic", grad, False, None))
-    assert rec is not None and rec["large_n"] in sched.CANDIDATE_LARGE_N and len(rec["timings_ms"]) == 10
+    assert rec is not None and rec["large_n"] in sched.CANDIDATE_LARGE_N and rec["launch"] in sched.CANDIDATE_LAUNCH
+    assert len(rec["timings_ms"]) == len(sched.CANDIDATE_LARGE_N) * len(sched.CANDIDATE_LAUNCH)
     assert (tmp_path / "sched.json").exists()
-    # the same threshold with the graph: bit-identical; another threshold moves filters between the two
-    # paths, whose FP64 sums differ only in their order (as test_chunked_path_matches_workgroup_path_sv)
+    # the same threshold with the graph or the persistent kernel: bit-identical; another threshold moves
+    # filters between the two paths, whose FP64 sums differ only in their order (as
+    # test_chunked_path_matches_workgroup_path_sv)
     for k, out in enumerate(outs):
         for a, b in zip(base, out):
-            if k == 0:
+            if k <= 1:
                 assert a.log_likelihood.item() == b.log_likelihood.item()
                 assert torch.equal(a.ess, b.ess)
                 if grad:
