@@ -187,8 +187,8 @@ int mocat_pf_run(int model, int d, int dy, const mocat_pf_filter* host_filters, 
  * launches it on the stream and returns after it has run (when the stream is already being captured
  * by the caller, the launches join the caller's graph instead); 2 runs all steps in one persistent
  * kernel (at most four workgroups per CU, all resident) with a grid barrier between the phases; a
- * barrier that waits more than ~2^22 polls gives up, and every flagged filter's out_loglik (and
- * out_grad) is then NaN. Results do not depend on the mode. The workspace also holds the step's split
+ * barrier that waits more than 2^22 polls (or the count in the environment variable
+ * MOCAT_PF_BARRIER_POLLS) gives up, and every flagged filter's out_loglik (and out_grad) is then NaN. Results do not depend on the mode. The workspace also holds the step's split
  * keys (8 bytes per particle, rounded up to whole chunks).
  * Calls on one stream may follow mocat_pf_run with the same arguments: the two paths touch disjoint
  * filters. */

@@ -20,6 +20,7 @@
 // The reference's resampler, random.categorical = Gumbel-max over an (n, n) Gumbel matrix
 // (filtering.py:196-199), is also provided (O(n^2)).
 #include <algorithm>
+#include <cstdlib>
 #include <mutex>
 #include <vector>
 
@@ -1861,13 +1862,15 @@ __global__ __launch_bounds__(LT) void lf_finish_kernel(LfArgs a) {
 
 // Grid-wide barrier of the persistent kernel. Every workgroup adds one to the arrival counter after a
 // release fence; barrier k is passed once the counter reaches k * gridDim.x, followed by an acquire
-// fence. The wait is bounded: a workgroup that polls kBarrierPolls times without seeing every arrival
-// sets the abort flag, and every workgroup leaves the kernel at its next barrier once the flag is set
-// (lf_abort_kernel then marks the results). All workgroups are resident at once (the grid is sized by
-// the occupancy limit), so a barrier is only ever waiting on running workgroups.
+// fence. The wait is bounded: a workgroup that polls max_polls times (kBarrierPolls unless the
+// MOCAT_PF_BARRIER_POLLS environment variable sets another limit, which tests use to force the
+// abort) without seeing every arrival sets the abort flag, and every workgroup leaves the kernel at
+// its next barrier once the flag is set (lf_abort_kernel then marks the results). All workgroups are
+// resident at once (the grid is sized by the occupancy limit), so a barrier is only ever waiting on
+// running workgroups.
 constexpr uint32_t kBarrierPolls = 1u << 22;
 
-MOCAT_DEV bool lf_grid_sync(uint32_t* bar, uint32_t k) {
+MOCAT_DEV bool lf_grid_sync(uint32_t* bar, uint32_t k, uint32_t max_polls) {
   __shared__ int pass;
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -1880,12 +1883,12 @@ MOCAT_DEV bool lf_grid_sync(uint32_t* bar, uint32_t k) {
         ok = 0;
         break;
       }
-      if (__hip_atomic_load(&bar[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= target) break;
-      if (polls >= kBarrierPolls) {
+      if (polls >= max_polls) {  // max_polls = 0: every workgroup gives up at the first barrier
         __hip_atomic_store(&bar[1], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         ok = 0;
         break;
       }
+      if (__hip_atomic_load(&bar[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= target) break;
       __builtin_amdgcn_s_sleep(1);
     }
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
@@ -1918,7 +1921,7 @@ __device__ __noinline__ void lf_step_call(const LfArgs& a, int64_t t, int cend_s
 // of the per-step kernels, so the results are the same bit for bit.
 template <class Model, int D, int DY, bool GRAD>
 __global__ __launch_bounds__(LT) void lf_persist_kernel(LfArgs a, int64_t max_T, int cend_slots,
-                                                        const int32_t* __restrict__ act) {
+                                                        const int32_t* __restrict__ act, uint32_t max_polls) {
   __shared__ LfScratch s;
   __shared__ Model model;
   __shared__ uint64_t buf[kSearchWin];  // the prefix-sum phase's chunk sums, then the step's window
@@ -1932,24 +1935,24 @@ __global__ __launch_bounds__(LT) void lf_persist_kernel(LfArgs a, int64_t max_T,
       __syncthreads();
       lf_prep_call<GRAD>(a, t, c, s);
     }
-    if (!lf_grid_sync(a.bar, ++k)) return;
+    if (!lf_grid_sync(a.bar, ++k, max_polls)) return;
     for (int64_t c = blockIdx.x; c < as; c += G) {
       __syncthreads();
       lf_csum_call(a, t, c, s, buf);
     }
-    if (!lf_grid_sync(a.bar, ++k)) return;
+    if (!lf_grid_sync(a.bar, ++k, max_polls)) return;
     for (int64_t c = blockIdx.x; c < as; c += G) {
       __syncthreads();
       lf_step_call<Model, D, DY, GRAD>(a, t, cend_slots, c, s, model, buf, cend);
     }
-    if (!lf_grid_sync(a.bar, ++k)) return;
+    if (!lf_grid_sync(a.bar, ++k, max_polls)) return;
   }
   const int64_t al = act[max_T - 1];
   for (int64_t c = blockIdx.x; c < al; c += G) {
     __syncthreads();
     lf_prep_call<GRAD>(a, max_T, c, s);  // the last step's statistics
   }
-  if (!lf_grid_sync(a.bar, ++k)) return;
+  if (!lf_grid_sync(a.bar, ++k, max_polls)) return;
   for (int64_t c = blockIdx.x; c < a.n_chunks; c += G) lf_finish_chunk<D>(a, c);
 }
 
@@ -2085,7 +2088,9 @@ int launch_pf_large(const LfArgs& a, int64_t max_T, int cend_slots, const int32_
     if (per_cu < 1 || cus < 1) return MOCAT_ERR_SIZE;
     const int64_t G = std::min<int64_t>(a.n_chunks, (int64_t)cus * std::min(per_cu, kPersistBlocksPerCu));
     hipLaunchKernelGGL((lf_init_kernel<Model, D, DY, GRAD>), chunks, blk, 0, st, a);
-    hipLaunchKernelGGL(kern, dim3((unsigned)G), blk, cend_bytes, st, a, max_T, cend_slots, act_dev);
+    uint32_t max_polls = kBarrierPolls;
+    if (const char* e = std::getenv("MOCAT_PF_BARRIER_POLLS")) max_polls = (uint32_t)std::strtoul(e, nullptr, 10);
+    hipLaunchKernelGGL(kern, dim3((unsigned)G), blk, cend_bytes, st, a, max_T, cend_slots, act_dev, max_polls);
     hipLaunchKernelGGL(lf_abort_kernel, dim3(1), dim3(256), 0, st, a);
     MOCAT_LAUNCH_CHECK();
     return MOCAT_OK;

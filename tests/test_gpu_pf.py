@@ -566,6 +566,25 @@ def test_persistent_chunked_kernel_repeats_the_stream_launches(m, case):
         assert abs(x - y) <= 1e-9 * (1 + abs(y)), (x, y)
 
 
+def test_persistent_kernel_barrier_gives_up_cleanly(m, monkeypatch):
+    """The persistent kernel's grid barriers wait a bounded number of polls. With the limit forced to
+    zero from the host every workgroup gives up at the first barrier and leaves the kernel, and the call
+    returns NaN log-likelihoods (and gradients); the next call with the default limit is unaffected."""
+    from mocat_amd import datasets
+    models, ys, ns = datasets.sv_batch_config(3, n_models=4, n_min=3000, n_max=9000, t_min=10, t_max=20)
+    keys = [k for k in m.random.split(m.random.PRNGKey(6), len(models))]
+    run = lambda launch: m.ssm.run_particle_filters(models, ys, keys, ns, 1.0, "systematic", grad=True,
+                                                    schedule={"large_n": 1, "launch": launch})
+    base = run("stream")
+    monkeypatch.setenv("MOCAT_PF_BARRIER_POLLS", "0")
+    bad = run("persistent")
+    assert all(np.isnan(r.log_likelihood.item()) for r in bad)
+    assert all(torch.isnan(r.log_likelihood_grad).all() for r in bad)
+    monkeypatch.delenv("MOCAT_PF_BARRIER_POLLS")
+    for a, b in zip(base, run("persistent")):
+        _pf_outputs_equal(a, b)
+
+
 @pytest.mark.parametrize("grad", [False, True])
 def test_graph_launch_and_tuned_schedule_give_the_same_results(m, grad, tmp_path, monkeypatch):
     # the chunked path's per-step launches captured into one HIP graph, and the autotuner's choice of
