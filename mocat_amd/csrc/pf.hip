@@ -1578,6 +1578,38 @@ MOCAT_DEV void upper_bound_lockstep(Get v, const int64_t* b, const int64_t* m, c
 }
 
 // win: kSearchWin entries of LDS; cend: cend_slots entries (staged by the multinomial search only)
+// For P targets that do not decrease with p (a thread's consecutive systematic targets): out[p] = the
+// first k in [b, e) with v(k) > tg[p] (v non-decreasing), or e; inactive targets (the chunk's tail, all
+// after the active ones) return the previous result. The first target takes a binary search; each
+// next one steps forward from the previous result (its answer is no smaller) up to four times
+// (offspring counts average one) and finishes with a binary search over the rest when those do not
+// reach it.
+template <int P, class Get>
+MOCAT_DEV void upper_bound_rising(Get v, int64_t b, int64_t e, const uint64_t* tg, const bool* act, int64_t* out) {
+  int64_t lo = b;
+#pragma unroll
+  for (int q = 0; q < P; ++q) {
+    if (!act[q]) {
+      out[q] = lo;
+      continue;
+    }
+    if (q > 0)
+      for (int step = 0; step < 4 && lo < e && v(lo) <= tg[q]; ++step) ++lo;
+    int64_t len = e - lo;
+    if (q > 0 && (len <= 0 || v(lo) > tg[q])) len = 0;  // found by the steps
+    while (len > 0) {
+      const int64_t h = len >> 1;
+      if (v(lo + h) <= tg[q]) {
+        lo += h + 1;
+        len -= h + 1;
+      } else {
+        len = h;
+      }
+    }
+    out[q] = lo;
+  }
+}
+
 template <class Model, int D, int DY, bool GRAD>
 MOCAT_DEV void lf_step_chunk(const LfArgs& a, int64_t t, int cend_slots, int64_t c, LfScratch& s, Model& model,
                              uint64_t* win, uint64_t* cend) {
@@ -1633,7 +1665,7 @@ MOCAT_DEV void lf_step_chunk(const LfArgs& a, int64_t t, int cend_slots, int64_t
   uint2 kwp[PPT];
 #pragma unroll
   for (int pk = 0; pk < PPT; ++pk) {
-    const int64_t i = base + pk * LT + tid;
+    const int64_t i = base + tid * PPT + pk;
     kwp[pk] = (!Model::kOptimal && i < n) ? *reinterpret_cast<const uint2*>(PK + 2 * i) : make_uint2(0u, 0u);
   }
   __syncthreads();
@@ -1650,7 +1682,7 @@ MOCAT_DEV void lf_step_chunk(const LfArgs& a, int64_t t, int cend_slots, int64_t
   bool live[PPT];
 #pragma unroll
   for (int pk = 0; pk < PPT; ++pk) {
-    srcs[pk] = base + pk * LT + tid;
+    srcs[pk] = base + tid * PPT + pk;
     live[pk] = srcs[pk] < n;
   }
   if (res && a.resampling == 2) {
@@ -1667,17 +1699,13 @@ MOCAT_DEV void lf_step_chunk(const LfArgs& a, int64_t t, int cend_slots, int64_t
                                 : (double)uniform_from_bits(bits_at(ik, (uint32_t)n, (uint32_t)i), 0.f, 1.f) * W);
     }
     if (window) {
-      // the first index of [lo, whi] whose sum exceeds the target (whi + 1: none does, clamped)
-      int64_t b[PPT], m[PPT], p[PPT];
-#pragma unroll
-      for (int pk = 0; pk < PPT; ++pk) {
-        b[pk] = lo;
-        m[pk] = whi - lo + 1;
-      }
+      // the first index of [lo, whi] whose sum exceeds the target (whi + 1: none does, clamped); the
+      // thread's particles are consecutive, so their systematic targets and ancestors rise with pk
+      int64_t p[PPT];
       if (staged)
-        upper_bound_lockstep<PPT>([&](int64_t k) { return win[k - lo]; }, b, m, tg, live, p);
+        upper_bound_rising<PPT>([&](int64_t k) { return win[k - lo]; }, lo, whi + 1, tg, live, p);
       else
-        upper_bound_lockstep<PPT>([&](int64_t k) { return C[k]; }, b, m, tg, live, p);
+        upper_bound_rising<PPT>([&](int64_t k) { return C[k]; }, lo, whi + 1, tg, live, p);
 #pragma unroll
       for (int pk = 0; pk < PPT; ++pk)
         if (live[pk]) srcs[pk] = p[pk] < n ? p[pk] : n - 1;
@@ -1731,7 +1759,7 @@ MOCAT_DEV void lf_step_chunk(const LfArgs& a, int64_t t, int cend_slots, int64_t
   float eps[PPT][D], xn[PPT][D], lwp[PPT];
 #pragma unroll
   for (int pk = 0; pk < PPT; ++pk) {
-    const int64_t i = base + pk * LT + tid;
+    const int64_t i = base + tid * PPT + pk;
     lwp[pk] = (live[pk] && !res) ? lwa[i] : 0.f;
     if (!live[pk]) continue;
     if constexpr (Model::kOptimal) {
@@ -1751,7 +1779,7 @@ MOCAT_DEV void lf_step_chunk(const LfArgs& a, int64_t t, int cend_slots, int64_t
   if constexpr (GRAD) {
 #pragma unroll
     for (int pk = 0; pk < PPT; ++pk) {
-      const int64_t i = base + pk * LT + tid;
+      const int64_t i = base + tid * PPT + pk;
       tg[pk][0] = live[pk] ? ta[srcs[pk] * 5 + 0] : 0.f;
       tg[pk][1] = live[pk] ? ta[srcs[pk] * 5 + 1] : 0.f;
 #pragma unroll
@@ -1796,7 +1824,7 @@ MOCAT_DEV void lf_step_chunk(const LfArgs& a, int64_t t, int cend_slots, int64_t
 #pragma unroll
   for (int pk = 0; pk < PPT; ++pk) {
     if (!live[pk]) break;
-    const int64_t i = base + pk * LT + tid;
+    const int64_t i = base + tid * PPT + pk;
     if constexpr (GRAD) {
       tb[i * 5 + 0] = tg[pk][0];
       tb[i * 5 + 1] = tg[pk][1];
