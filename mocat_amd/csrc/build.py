@@ -39,7 +39,7 @@ COMMON_FLAGS = [
 # 4% at d = 100, identical results). pf.hip: without machine-level loop-invariant code motion, which
 # otherwise hoists dozens of constants (the FP64 exponential's and the normal quantile's coefficients)
 # and addresses out of the filters' step loops into registers held for the whole loop: the persistent
-# chunked kernel needs 117 instead of 172 VGPRs (four workgroups per CU instead of two), pf_kernel<Sv>
+# chunked kernel needs 124 instead of 183 VGPRs (four workgroups per CU instead of two), pf_kernel<Sv>
 # 143 instead of 185, and the d = 8 linear-Gaussian filter spills 10 registers instead of 52.
 SOURCE_FLAGS = {"lr.hip": ["-mllvm", "-amdgpu-sched-strategy=iterative-ilp"],
                 "pf.hip": ["-mllvm", "-disable-machine-licm"]}

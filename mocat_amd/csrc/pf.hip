@@ -1926,17 +1926,16 @@ MOCAT_DEV bool lf_grid_sync(uint32_t* bar, uint32_t k, uint32_t max_polls) {
   return pass != 0;
 }
 
-// The phases as calls (not inlined): each allocates its registers on its own, and the kernel keeps
-// only the loop state live across them.
+// The phases of the persistent kernel.
 template <bool GRAD>
-__device__ __noinline__ void lf_prep_call(const LfArgs& a, int64_t t, int64_t c, LfScratch& s) {
+MOCAT_DEV void lf_prep_call(const LfArgs& a, int64_t t, int64_t c, LfScratch& s) {
   lf_prep_chunk<GRAD>(a, t, c, s);
 }
-__device__ __noinline__ void lf_csum_call(const LfArgs& a, int64_t t, int64_t c, LfScratch& s, uint64_t* cs) {
+MOCAT_DEV void lf_csum_call(const LfArgs& a, int64_t t, int64_t c, LfScratch& s, uint64_t* cs) {
   lf_csum_chunk(a, t, c, s, cs);
 }
 template <class Model, int D, int DY, bool GRAD>
-__device__ __noinline__ void lf_step_call(const LfArgs& a, int64_t t, int cend_slots, int64_t c, LfScratch& s,
+MOCAT_DEV void lf_step_call(const LfArgs& a, int64_t t, int cend_slots, int64_t c, LfScratch& s,
                                           Model& model, uint64_t* win, uint64_t* cend) {
   lf_step_chunk<Model, D, DY, GRAD>(a, t, cend_slots, c, s, model, win, cend);
 }
