@@ -1620,6 +1620,11 @@ MOCAT_DEV void lf_step_chunk(const LfArgs& a, int64_t t, int cend_slots, int64_t
   const LfState st = a.st[fi];
   const int64_t n = F.n, base = (c - F.chunk0) * CH;
   const bool res = st.res != 0;
+  // particle of slot pk: four consecutive particles per thread (systematic targets rise within the
+  // thread: one search and forward steps), or, with the gradient, particles LT apart (the tangent
+  // records of 5 floats per particle are then stored by consecutive lanes)
+  constexpr bool kConsecutive = !GRAD;
+  auto pidx = [&](int pk) -> int64_t { return kConsecutive ? base + tid * PPT + pk : base + pk * LT + tid; };
   const bool odd = (t - 1) & 1;
   const float* xa = (odd ? a.x1 : a.x0) + F.state_offset * D;
   float* xb = (odd ? a.x0 : a.x1) + F.state_offset * D;
@@ -1665,7 +1670,7 @@ MOCAT_DEV void lf_step_chunk(const LfArgs& a, int64_t t, int cend_slots, int64_t
   uint2 kwp[PPT];
 #pragma unroll
   for (int pk = 0; pk < PPT; ++pk) {
-    const int64_t i = base + tid * PPT + pk;
+    const int64_t i = pidx(pk);
     kwp[pk] = (!Model::kOptimal && i < n) ? *reinterpret_cast<const uint2*>(PK + 2 * i) : make_uint2(0u, 0u);
   }
   __syncthreads();
@@ -1682,7 +1687,7 @@ MOCAT_DEV void lf_step_chunk(const LfArgs& a, int64_t t, int cend_slots, int64_t
   bool live[PPT];
 #pragma unroll
   for (int pk = 0; pk < PPT; ++pk) {
-    srcs[pk] = base + tid * PPT + pk;
+    srcs[pk] = pidx(pk);
     live[pk] = srcs[pk] < n;
   }
   if (res && a.resampling == 2) {
@@ -1699,13 +1704,25 @@ MOCAT_DEV void lf_step_chunk(const LfArgs& a, int64_t t, int cend_slots, int64_t
                                 : (double)uniform_from_bits(bits_at(ik, (uint32_t)n, (uint32_t)i), 0.f, 1.f) * W);
     }
     if (window) {
-      // the first index of [lo, whi] whose sum exceeds the target (whi + 1: none does, clamped); the
-      // thread's particles are consecutive, so their systematic targets and ancestors rise with pk
+      // the first index of [lo, whi] whose sum exceeds the target (whi + 1: none does, clamped)
       int64_t p[PPT];
-      if (staged)
-        upper_bound_rising<PPT>([&](int64_t k) { return win[k - lo]; }, lo, whi + 1, tg, live, p);
-      else
-        upper_bound_rising<PPT>([&](int64_t k) { return C[k]; }, lo, whi + 1, tg, live, p);
+      if constexpr (kConsecutive) {
+        if (staged)
+          upper_bound_rising<PPT>([&](int64_t k) { return win[k - lo]; }, lo, whi + 1, tg, live, p);
+        else
+          upper_bound_rising<PPT>([&](int64_t k) { return C[k]; }, lo, whi + 1, tg, live, p);
+      } else {
+        int64_t b[PPT], m[PPT];
+#pragma unroll
+        for (int pk = 0; pk < PPT; ++pk) {
+          b[pk] = lo;
+          m[pk] = whi - lo + 1;
+        }
+        if (staged)
+          upper_bound_lockstep<PPT>([&](int64_t k) { return win[k - lo]; }, b, m, tg, live, p);
+        else
+          upper_bound_lockstep<PPT>([&](int64_t k) { return C[k]; }, b, m, tg, live, p);
+      }
 #pragma unroll
       for (int pk = 0; pk < PPT; ++pk)
         if (live[pk]) srcs[pk] = p[pk] < n ? p[pk] : n - 1;
@@ -1759,7 +1776,7 @@ MOCAT_DEV void lf_step_chunk(const LfArgs& a, int64_t t, int cend_slots, int64_t
   float eps[PPT][D], xn[PPT][D], lwp[PPT];
 #pragma unroll
   for (int pk = 0; pk < PPT; ++pk) {
-    const int64_t i = base + tid * PPT + pk;
+    const int64_t i = pidx(pk);
     lwp[pk] = (live[pk] && !res) ? lwa[i] : 0.f;
     if (!live[pk]) continue;
     if constexpr (Model::kOptimal) {
@@ -1779,7 +1796,7 @@ MOCAT_DEV void lf_step_chunk(const LfArgs& a, int64_t t, int cend_slots, int64_t
   if constexpr (GRAD) {
 #pragma unroll
     for (int pk = 0; pk < PPT; ++pk) {
-      const int64_t i = base + tid * PPT + pk;
+      const int64_t i = pidx(pk);
       tg[pk][0] = live[pk] ? ta[srcs[pk] * 5 + 0] : 0.f;
       tg[pk][1] = live[pk] ? ta[srcs[pk] * 5 + 1] : 0.f;
 #pragma unroll
@@ -1824,7 +1841,7 @@ MOCAT_DEV void lf_step_chunk(const LfArgs& a, int64_t t, int cend_slots, int64_t
 #pragma unroll
   for (int pk = 0; pk < PPT; ++pk) {
     if (!live[pk]) break;
-    const int64_t i = base + tid * PPT + pk;
+    const int64_t i = pidx(pk);
     if constexpr (GRAD) {
       tb[i * 5 + 0] = tg[pk][0];
       tb[i * 5 + 1] = tg[pk][1];
