@@ -446,8 +446,9 @@ def _launch(models, ys, keys, ns, ess_threshold, resampling, store, grad=False, 
     cur = torch.cuda.current_stream()
     side = None
     if small:
-        # the one-workgroup-per-filter kernel runs beside the chunked path's launches
-        side = torch.cuda.Stream(device=dev) if any(large) else cur
+        # the one-workgroup-per-filter kernel runs beside the chunked path's launches, except before the
+        # persistent kernel (launch mode 2), whose grid barriers need all of its workgroups resident
+        side = torch.cuda.Stream(device=dev) if any(large) and launch_mode != 2 else cur
         side.wait_stream(cur)
         with torch.cuda.stream(side):
             _lib.call("mocat_pf_run", kind, d, dy, desc_host.ctypes.data_as(_lib.ctypes.c_void_p), _lib.ptr(desc_t),
