@@ -22,7 +22,7 @@ extern "C" {
 #endif
 
 /* Returns the ABI revision of the library (not a status). */
-int mocat_abi_version(void); /* this header: 4 */
+int mocat_abi_version(void); /* this header: 5 */
 
 /* ---------------------------------------------------------------------------------------- RNG
  * jax.random.split / _random_bits / uniform / normal as called throughout mocat, e.g.
@@ -160,6 +160,11 @@ typedef struct {
 #define MOCAT_PF_INIT_KEY 8
 
 int mocat_pf_param_count(int model, int d, int dy, int64_t* count);
+/* *compiled = 1 when mocat_pf_run / mocat_pf_run_large have kernels for (model, d, dy) (model 1:
+ * d = dy <= 4; models 0 and 2: the (d, dy) pairs (1,1) (2,1) (2,2) (3,3) (4,1) (4,2) (4,4) (5,5) (8,4)
+ * (8,8)), else 0. No counterpart in the reference (mocat traces any shape); its hosts use it to choose
+ * between the fused filters and the hook loop. */
+int mocat_pf_compiled_shape(int model, int d, int dy, int* compiled);
 /* host_filters and filters hold the same n_filters descriptors: host_filters in host memory (read by
  * the argument checks, not retained after return), filters in device memory (read by the kernel).
  * Every filter not flagged MOCAT_PF_LARGE needs n >= 1, T >= 1, non-negative offsets and, when
@@ -213,7 +218,10 @@ int mocat_pf_run_large(int model, int d, int dy, const mocat_pf_filter* host_fil
  * phase 2 = StateSpaceModel.likelihood_potential (mocat/src/ssm/ssm.py:73-81) of x_prev[i] at y into
  *   log_weight[i] (models 0 and 1; x and keys unused).
  * x_prev, x [n][d], y [dy], keys [n][2] (model 2: keys[0] only), log_weight [n]; step_scale: the
- * step's noise scale for model 1 in phase 1 (sqrt(t_new - t_previous); 1 for unit steps). */
+ * step's noise scale for model 1 in phase 1 (sqrt(t_new - t_previous); 1 for unit steps). Model 0 at
+ * a (d, dy) that mocat_pf_compiled_shape does not list runs a runtime-dimension kernel (d, dy <= 16,
+ * the same formulas and noise draws): the host runs such a filter's time loop over these hooks,
+ * mocat_gather_rows, mocat_resample / mocat_resample_categorical and mocat_logsumexp. */
 int mocat_pf_hook(int model, int d, int dy, int phase, const float* params, const float* x_prev, const float* y,
                   const uint32_t* keys, float step_scale, int64_t n, float* x, float* log_weight,
                   hipStream_t stream);

@@ -16,7 +16,7 @@ import torch
 
 _LIB = None
 LIB_PATH = Path(__file__).resolve().parent / "libmocat_amd.so"
-ABI_VERSION = 4
+ABI_VERSION = 5
 
 c_i64 = ctypes.c_int64
 c_i32 = ctypes.c_int32

@@ -28,6 +28,12 @@
 
 using namespace mocat;
 
+namespace mocat {
+// pf_generic.hip: the linear-Gaussian hooks at a runtime (d, d_y), both up to 16
+int launch_lg_generic_hook(int phase, int d, int dy, const float* params, const float* xprev, const float* y,
+                           const uint32_t* keys, int64_t n, float* x, float* lw, hipStream_t st);
+}  // namespace mocat
+
 extern "C" {
 typedef struct {
   int64_t n;
@@ -2293,8 +2299,23 @@ int mocat_pf_param_count(int model, int d, int dy, int64_t* count) {
     *count = d == 1 ? Sv::NPARAM : 2 * d + 3 * d * d;
     return MOCAT_OK;
   }
-  if ((model != 0 && model != 2) || d < 1 || d > 8 || dy < 1 || dy > 8) return MOCAT_ERR_ARG;
+  const int dmax = model == 0 ? 16 : 8;  // model 0: the compiled shapes, or the runtime-dimension hooks
+  if ((model != 0 && model != 2) || d < 1 || d > dmax || dy < 1 || dy > dmax) return MOCAT_ERR_ARG;
   *count = model == 0 ? d + 3 * d * d + dy * d + dy * dy + 1 : d + 3 * dy * d + 3 * d * d + dy * dy + 1;
+  return MOCAT_OK;
+}
+
+int mocat_pf_compiled_shape(int model, int d, int dy, int* compiled) {
+  if (!compiled) return MOCAT_ERR_ARG;
+  int c = 0;
+  if (model == 1) c = d == dy && d >= 1 && d <= 4;
+#define MOCAT_PF_CASE(DD, YY) \
+  if (d == DD && dy == YY) c = 1;
+  if (model == 0 || model == 2) {
+    MOCAT_PF_LG_SHAPES
+  }
+#undef MOCAT_PF_CASE
+  *compiled = c;
   return MOCAT_OK;
 }
 
@@ -2475,7 +2496,8 @@ int mocat_pf_hook(int model, int d, int dy, int phase, const float* params, cons
   if (d == DD && dy == YY) return launch_pf_hook<Lgssm<DD, YY>, DD, YY>(phase, params, x_prev, y, keys, step_scale, n, x, log_weight, stream);
   MOCAT_PF_LG_SHAPES
 #undef MOCAT_PF_CASE
-  return MOCAT_ERR_ARG;
+  // another (d, d_y): the runtime-dimension kernel (d, d_y up to 16)
+  return launch_lg_generic_hook(phase, d, dy, params, x_prev, y, keys, n, x, log_weight, stream);
 }
 
 int mocat_resample(const float* weights, int64_t n, int method, int from_log, const uint32_t* key, void* workspace,

@@ -88,7 +88,7 @@ int launch_fill(const uint32_t* keys, int64_t n_keys, int64_t size, float lo, fl
 
 extern "C" {
 
-int mocat_abi_version(void) { return 4; }
+int mocat_abi_version(void) { return 5; }
 
 int mocat_random_split(const uint32_t* keys, int64_t n_keys, int64_t num, uint32_t* out, hipStream_t stream) {
   if (num < 0) return MOCAT_ERR_ARG;

@@ -34,6 +34,8 @@ _lib.register("mocat_pf_run_large", [_lib.c_i32, _lib.c_i32, _lib.c_i32, _P, _li
 _lib.register("mocat_pf_hook", [_lib.c_i32, _lib.c_i32, _lib.c_i32, _lib.c_i32, _P, _P, _P, _P, _lib.c_f32, _lib.c_i64,
                                 _P, _P, _P])
 _lib.register("mocat_resample", [_P, _lib.c_i64, _lib.c_i32, _lib.c_i32, _P, _P, _P, _P])
+_lib.register("mocat_pf_compiled_shape", [_lib.c_i32, _lib.c_i32, _lib.c_i32, _P])
+_lib.register("mocat_gather_rows", [_P, _P, _lib.c_i64, _lib.c_i64, _P, _P])
 _lib.register("mocat_resample_noalloc", [_P, _lib.c_i64, _lib.c_i32, _lib.c_i32, _P, _P, _P])
 _lib.register("mocat_resample_categorical", [_P, _lib.c_i64, _lib.c_i64, _P, _P, _P])
 _lib.register("mocat_enkf_param_count", [_lib.c_i32, _lib.c_i32, _P])
@@ -364,6 +366,74 @@ def _large_flags(ns, Ts, large_n):
     return [int(n) >= int(large_n) for n in ns]
 
 
+def _compiled_shape(kind, d, dy) -> bool:
+    """Whether the fused filters (mocat_pf_run / mocat_pf_run_large) have kernels for (kind, d, dy)."""
+    c = _lib.ctypes.c_int(0)
+    _lib.call("mocat_pf_compiled_shape", int(kind), int(d), int(dy), _lib.ctypes.byref(c))
+    return bool(c.value)
+
+
+def _run_lg_hooks(model, y, key, n, ess_threshold, resampling, store, initial=None, flags=0):
+    """One bootstrap filter on the linear-Gaussian model at a (d, d_y) without fused kernels: the
+    scan of run_particle_filter_for_marginals (mocat/src/ssm/filtering.py:255-324) with the fused
+    filters' key tree, flags and log-likelihood recursion, as a host loop over the HIP kernels: the
+    runtime-dimension hooks (initial sample and weight, proposal and weight), mocat_gather_rows, the
+    resampling entries (systematic / multinomial from the log-weights, or mocat's Gumbel-max) and the
+    FP64 log-sum-exp. The resampling decision of each step reads the previous step's ESS on the host.
+    Returns the fields of _launch's per-filter result."""
+    from ..reduce import logsumexp_pair
+    dev = mr.default_device()
+    d, dy = int(model.dim), int(model.dim_obs)
+    y = np.asarray(y, np.float32).reshape(-1, dy)
+    T = y.shape[0]
+    params = model.kernel_params()
+    key = mr.as_key(key).reshape(2)
+    resume = initial is not None
+    sub = key if flags & PF_INIT_KEY else mr.split(key, 2)[1]
+    rk = key if resume else mr.split(key, 2)[0]
+    yt = torch.from_numpy(y).to(dev)
+    if resume:
+        x = torch.as_tensor(initial[0], device=dev).reshape(n, d).to(torch.float32).contiguous()
+        lw = torch.as_tensor(initial[1], device=dev).reshape(n).to(torch.float32).contiguous()
+    else:
+        x, lw = _hook(0, params, model, 0, None, yt[0], mr.split(sub, n), n)
+    logn = float(np.log(n))
+    lse, lse2 = (float(v) for v in logsumexp_pair(lw))
+    ll = 0.0 if resume else lse - logn
+    ess = [float(np.exp(2 * lse - lse2))]
+    llt = [ll]
+    ident = torch.arange(n, dtype=torch.int32, device=dev)
+    xs, lws, ancs = [x], [lw], [ident]
+    step_keys = None if flags & PF_STEP_KEY or T < 2 else mr.split(rk, T - 1)
+    for t in range(1, T):
+        ik = key if flags & PF_STEP_KEY else step_keys[t - 1]
+        res = ess[-1] < ess_threshold * n
+        if res:
+            idx = categorical(ik, lw) if resampling == "categorical" else \
+                resample_indices(lw, ik, resampling, log_weights=True)
+            xp = torch.empty((n, d), dtype=torch.float32, device=dev)
+            _lib.call("mocat_gather_rows", _lib.ptr(x), _lib.ptr(idx), n, d, _lib.ptr(xp), _lib.stream_handle())
+        else:
+            idx, xp = ident, x
+        x, inc = _hook(0, params, model, 1, xp, yt[t], mr.split(ik, n), n)
+        lw = inc if res else lw + inc
+        lse_prev = lse
+        lse, lse2 = (float(v) for v in logsumexp_pair(lw))
+        ll += lse - (logn if res else lse_prev)
+        ess.append(float(np.exp(2 * lse - lse2)))
+        llt.append(ll)
+        if store:
+            xs.append(x)
+            lws.append(lw)
+            ancs.append(idx)
+    r = dict(log_likelihood=torch.tensor(ll, dtype=torch.float64, device=dev),
+             ess=torch.tensor(ess, dtype=torch.float32, device=dev),
+             log_likelihood_t=torch.tensor(llt, dtype=torch.float64, device=dev), final_value=x, final_log_weight=lw)
+    if store:
+        r.update(value=torch.stack(xs), log_weight=torch.stack(lws), ancestors=torch.stack(ancs))
+    return r
+
+
 def _launch(models, ys, keys, ns, ess_threshold, resampling, store, grad=False, large_n=None, particle_filter=None,
             initial=None, extra_flags=0, launch_mode=0, ts=None):
     """Runs len(models) filters (same model class and dims): filters below large_n particles in one
@@ -382,6 +452,15 @@ def _launch(models, ys, keys, ns, ess_threshold, resampling, store, grad=False, 
         kind = OptimalNonLinearGaussianParticleFilter.kernel_model
         if any(int(v) * d >= 2 ** 32 for v in ns):
             raise ValueError("the optimal filter draws normal(key, (n, d)): n * d must be < 2^32")
+        if not _compiled_shape(kind, d, dy):
+            raise NotImplementedError("the optimal-proposal filter is compiled for (d, d_y) in (1,1) (2,1) (2,2) (3,3) "
+                                      "(4,1) (4,2) (4,4) (5,5) (8,4) (8,8)")
+    elif kind == 0 and not _compiled_shape(0, d, dy):
+        # another linear-Gaussian shape: the runtime-dimension hooks, one filter at a time
+        if grad:
+            raise NotImplementedError("log-likelihood gradients are implemented for the 1-D stochastic-volatility model")
+        return [_run_lg_hooks(models[i], ys[i], keys[i], int(ns[i]), ess_threshold, resampling, store,
+                              None if initial is None else initial[i], extra_flags) for i in range(len(models))]
     F = len(models)
     Ts = [int(np.asarray(y).shape[0]) for y in ys]
     order = sorted(range(F), key=lambda i: -(ns[i] * Ts[i]))  # longest first

@@ -40,4 +40,4 @@ def test_python_signatures_cover_header():
 @pytest.mark.skipif(not LIB.exists(), reason="library not built")
 def test_abi_version_without_gpu():
     lib = ctypes.CDLL(str(LIB))
-    assert lib.mocat_abi_version() == 4
+    assert lib.mocat_abi_version() == 5
