@@ -218,9 +218,9 @@ int mocat_pf_run_large(int model, int d, int dy, const mocat_pf_filter* host_fil
  * phase 2 = StateSpaceModel.likelihood_potential (mocat/src/ssm/ssm.py:73-81) of x_prev[i] at y into
  *   log_weight[i] (models 0 and 1; x and keys unused).
  * x_prev, x [n][d], y [dy], keys [n][2] (model 2: keys[0] only), log_weight [n]; step_scale: the
- * step's noise scale for model 1 in phase 1 (sqrt(t_new - t_previous); 1 for unit steps). Model 0 at
- * a (d, dy) that mocat_pf_compiled_shape does not list runs a runtime-dimension kernel (d, dy <= 16,
- * the same formulas and noise draws): the host runs such a filter's time loop over these hooks,
+ * step's noise scale for model 1 in phase 1 (sqrt(t_new - t_previous); 1 for unit steps). Models 0
+ * and 2 at a (d, dy) that mocat_pf_compiled_shape does not list run a runtime-dimension kernel (d,
+ * dy <= 16, the same formulas and noise draws): the host runs such a filter's time loop over these hooks,
  * mocat_gather_rows, mocat_resample / mocat_resample_categorical and mocat_logsumexp. */
 int mocat_pf_hook(int model, int d, int dy, int phase, const float* params, const float* x_prev, const float* y,
                   const uint32_t* keys, float step_scale, int64_t n, float* x, float* log_weight,

@@ -29,9 +29,9 @@
 using namespace mocat;
 
 namespace mocat {
-// pf_generic.hip: the linear-Gaussian hooks at a runtime (d, d_y), both up to 16
-int launch_lg_generic_hook(int phase, int d, int dy, const float* params, const float* xprev, const float* y,
-                           const uint32_t* keys, int64_t n, float* x, float* lw, hipStream_t st);
+// pf_generic.hip: the linear-Gaussian hooks (models 0 and 2) at a runtime (d, d_y), both up to 16
+int launch_lg_generic_hook(int model, int phase, int d, int dy, const float* params, const float* xprev,
+                           const float* y, const uint32_t* keys, int64_t n, float* x, float* lw, hipStream_t st);
 }  // namespace mocat
 
 extern "C" {
@@ -2299,8 +2299,8 @@ int mocat_pf_param_count(int model, int d, int dy, int64_t* count) {
     *count = d == 1 ? Sv::NPARAM : 2 * d + 3 * d * d;
     return MOCAT_OK;
   }
-  const int dmax = model == 0 ? 16 : 8;  // model 0: the compiled shapes, or the runtime-dimension hooks
-  if ((model != 0 && model != 2) || d < 1 || d > dmax || dy < 1 || dy > dmax) return MOCAT_ERR_ARG;
+  // models 0 and 2: the compiled shapes, or the runtime-dimension hooks up to 16
+  if ((model != 0 && model != 2) || d < 1 || d > 16 || dy < 1 || dy > 16) return MOCAT_ERR_ARG;
   *count = model == 0 ? d + 3 * d * d + dy * d + dy * dy + 1 : d + 3 * dy * d + 3 * d * d + dy * dy + 1;
   return MOCAT_OK;
 }
@@ -2489,7 +2489,7 @@ int mocat_pf_hook(int model, int d, int dy, int phase, const float* params, cons
   if (d == DD && dy == YY) return launch_pf_hook<LgssmOpt<DD, YY>, DD, YY>(phase, params, x_prev, y, keys, step_scale, n, x, log_weight, stream);
     MOCAT_PF_LG_SHAPES
 #undef MOCAT_PF_CASE
-    return MOCAT_ERR_ARG;
+    return launch_lg_generic_hook(2, phase, d, dy, params, x_prev, y, keys, n, x, log_weight, stream);
   }
   if (model != 0) return MOCAT_ERR_ARG;
 #define MOCAT_PF_CASE(DD, YY) \
@@ -2497,7 +2497,7 @@ int mocat_pf_hook(int model, int d, int dy, int phase, const float* params, cons
   MOCAT_PF_LG_SHAPES
 #undef MOCAT_PF_CASE
   // another (d, d_y): the runtime-dimension kernel (d, d_y up to 16)
-  return launch_lg_generic_hook(phase, d, dy, params, x_prev, y, keys, n, x, log_weight, stream);
+  return launch_lg_generic_hook(0, phase, d, dy, params, x_prev, y, keys, n, x, log_weight, stream);
 }
 
 int mocat_resample(const float* weights, int64_t n, int method, int from_log, const uint32_t* key, void* workspace,

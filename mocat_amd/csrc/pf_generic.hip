@@ -1,11 +1,14 @@
 // The linear-Gaussian model's per-particle functions at a runtime dimension, for the (d, d_y) outside the
-// shapes pf.hip compiles: the bootstrap filter's initial sample and weight, proposal and weight, and the
-// likelihood potential (mocat/src/ssm/linear_gaussian/linear_gaussian.py:46-51, 86-94, 118-128;
-// mocat/src/ssm/filtering.py:65-78, 142-170), d and d_y up to kLgMaxDim. One thread per particle, its
-// vectors in LDS (rows indexed at run time), the parameters in LDS, the formulas and summation order of
-// pf.hip's Lgssm<D, DY>, and the noise drawn from the particle's key as normal(key, (d,)) in jax's
-// layout. mocat_pf_hook sends model 0 here when its (d, d_y) is not compiled; the host runs such a
-// filter's time loop over these hooks and the library's resampling, gather and log-sum-exp kernels.
+// shapes pf.hip compiles, d and d_y up to kLgMaxDim:
+//   model 0, the bootstrap filter: initial sample and weight, proposal and weight, likelihood potential
+//     (mocat/src/ssm/linear_gaussian/linear_gaussian.py:46-51, 86-94, 118-128;
+//     mocat/src/ssm/filtering.py:65-78, 142-170), noise normal(keys[i], (d,)) per particle;
+//   model 2, the optimal proposal (mocat/src/ssm/nonlinear_gaussian.py:198-224, 264-281) from the
+//     host's startup matrices, noise row i of normal(keys[0], (n, d)).
+// One thread per particle, its vectors in LDS (rows indexed at run time), the parameters in LDS, and
+// the formulas and summation order of pf.hip's Lgssm<D, DY> and LgssmOpt<D, DY>. mocat_pf_hook sends
+// models 0 and 2 here when their (d, d_y) is not compiled; the host runs such a filter's time loop over
+// these hooks and the library's resampling, gather and log-sum-exp kernels.
 #include "common.hpp"
 
 namespace mocat {
@@ -14,8 +17,9 @@ constexpr int kLgMaxDim = 16;
 
 namespace {
 
-constexpr int kLgThreads = 128;  // per-thread vectors: 3 x 16 x 128 floats = 24 KB of LDS per workgroup
-constexpr int kLgMaxParams = kLgMaxDim + 5 * kLgMaxDim * kLgMaxDim + 1;
+constexpr int kLgThreads = 128;  // per-thread vectors: 4 x 16 x 128 floats = 32 KB of LDS per workgroup
+// model 0: d + 3 d^2 + dy d + dy^2 + 1; model 2: d + 3 dy d + 3 d^2 + dy^2 + 1
+constexpr int kLgMaxParams = kLgMaxDim + 7 * kLgMaxDim * kLgMaxDim + 1;
 
 struct LgView {
   const float *m0, *L0, *A, *LQ, *H, *SR;
@@ -40,9 +44,12 @@ MOCAT_DEV float lg_lik(const LgView& m, const float (*S)[kLgThreads], float (*T)
   return 0.5f * s + m.nlz;
 }
 
-// phase 0: x = L0 eps + m0, lw = -lik(x, y) (0 without y); phase 1: x = A x_prev + LQ eps,
-// lw = -lik(x, y); phase 2: lw = lik(x_prev, y). eps = normal(keys[i], (d,)).
-__global__ __launch_bounds__(kLgThreads) void lg_hook_kernel(int phase, int d, int dy,
+// Model 0: phase 0: x = L0 eps + m0, lw = -lik(x, y) (0 without y); phase 1: x = A x_prev + LQ eps,
+// lw = -lik(x, y); phase 2: lw = lik(x_prev, y); eps = normal(keys[i], (d,)).
+// Model 2 (params m0, H, K0, C0, A, K, S, WC, nlzw): phase 0: x = (m0 + K0 (y - H m0)) + eps @ C0,
+// lw = 0; phase 1: mx = A x_prev, r = y - H mx, x = (mx + K r) + S eps, lw = -(0.5 |r @ WC|^2 + nlzw);
+// eps = row i of normal(keys[0], (n, d)).
+__global__ __launch_bounds__(kLgThreads) void lg_hook_kernel(int model, int phase, int d, int dy,
                                                              const float* __restrict__ params,
                                                              const float* __restrict__ xprev,
                                                              const float* __restrict__ y,
@@ -50,9 +57,63 @@ __global__ __launch_bounds__(kLgThreads) void lg_hook_kernel(int phase, int d, i
                                                              float* __restrict__ xout, float* __restrict__ lw) {
   __shared__ float P[kLgMaxParams];
   __shared__ float E[kLgMaxDim][kLgThreads], XP[kLgMaxDim][kLgThreads], X[kLgMaxDim][kLgThreads];
-  const int np = d + 3 * d * d + dy * d + dy * dy + 1;
+  __shared__ float R[kLgMaxDim][kLgThreads];
+  const int np = model == 2 ? d + 3 * dy * d + 3 * d * d + dy * dy + 1 : d + 3 * d * d + dy * d + dy * dy + 1;
   for (int k = threadIdx.x; k < np; k += kLgThreads) P[k] = params[k];
   __syncthreads();
+  const int t = threadIdx.x;
+  if (model == 2) {
+    const float *m0 = P, *H = m0 + d, *K0 = H + dy * d, *C0 = K0 + d * dy, *A = C0 + d * d, *K = A + d * d,
+                *S = K + d * dy, *WC = S + d * d;
+    const float nlzw = WC[dy * dy];
+    const Key k0{keys[0], keys[1]};
+    const uint32_t total = (uint32_t)(n * d);
+    for (int64_t i = blockIdx.x * (int64_t)kLgThreads + t; i < n; i += (int64_t)gridDim.x * kLgThreads) {
+      for (int q = 0; q < d; ++q) E[q][t] = normal_from_bits(bits_at(k0, total, (uint32_t)(i * d + q)));
+      if (phase == 0) {
+        for (int j = 0; j < dy; ++j) {
+          float hm = 0.f;
+          for (int k = 0; k < d; ++k) hm += H[j * d + k] * m0[k];
+          R[j][t] = y[j] - hm;
+        }
+        for (int r = 0; r < d; ++r) {
+          float g = 0.f, c = 0.f;
+          for (int j = 0; j < dy; ++j) g += K0[r * dy + j] * R[j][t];
+          for (int k = 0; k < d; ++k) c += E[k][t] * C0[k * d + r];
+          X[r][t] = (m0[r] + g) + c;
+        }
+        for (int q = 0; q < d; ++q) xout[i * d + q] = X[q][t];
+        lw[i] = 0.f;
+        continue;
+      }
+      for (int q = 0; q < d; ++q) XP[q][t] = xprev[i * d + q];
+      for (int r = 0; r < d; ++r) {  // mx, in X (each row of X is written after its own row is read below)
+        float a = 0.f;
+        for (int k = 0; k < d; ++k) a += A[r * d + k] * XP[k][t];
+        X[r][t] = a;
+      }
+      for (int j = 0; j < dy; ++j) {
+        float hx = 0.f;
+        for (int k = 0; k < d; ++k) hx += H[j * d + k] * X[k][t];
+        R[j][t] = y[j] - hx;
+      }
+      for (int r = 0; r < d; ++r) {
+        float g = 0.f, b = 0.f;
+        for (int j = 0; j < dy; ++j) g += K[r * dy + j] * R[j][t];
+        for (int k = 0; k < d; ++k) b += S[r * d + k] * E[k][t];
+        X[r][t] = (X[r][t] + g) + b;
+      }
+      float s2 = 0.f;
+      for (int k = 0; k < dy; ++k) {
+        float v = 0.f;
+        for (int j = 0; j < dy; ++j) v += R[j][t] * WC[j * dy + k];
+        s2 += v * v;
+      }
+      for (int q = 0; q < d; ++q) xout[i * d + q] = X[q][t];
+      lw[i] = -(0.5f * s2 + nlzw);
+    }
+    return;
+  }
   LgView m;
   m.d = d;
   m.dy = dy;
@@ -63,7 +124,6 @@ __global__ __launch_bounds__(kLgThreads) void lg_hook_kernel(int phase, int d, i
   m.H = m.LQ + d * d;
   m.SR = m.H + dy * d;
   m.nlz = m.SR[dy * dy];
-  const int t = threadIdx.x;
   const uint32_t hd = ((uint32_t)d + 1u) >> 1;
   for (int64_t i = blockIdx.x * (int64_t)kLgThreads + t; i < n; i += (int64_t)gridDim.x * kLgThreads) {
     if (phase != 0)
@@ -105,13 +165,15 @@ __global__ __launch_bounds__(kLgThreads) void lg_hook_kernel(int phase, int d, i
 
 }  // namespace
 
-// Called by mocat_pf_hook (pf.hip) for model 0 at a (d, d_y) without a compiled kernel.
-int launch_lg_generic_hook(int phase, int d, int dy, const float* params, const float* xprev, const float* y,
-                           const uint32_t* keys, int64_t n, float* x, float* lw, hipStream_t st) {
-  if (d < 1 || dy < 1 || d > kLgMaxDim || dy > kLgMaxDim) return MOCAT_ERR_ARG;
+// Called by mocat_pf_hook (pf.hip) for models 0 and 2 at a (d, d_y) without a compiled kernel (its
+// argument checks done there: model 2 takes phases 0 and 1 with y, n * d < 2^32).
+int launch_lg_generic_hook(int model, int phase, int d, int dy, const float* params, const float* xprev,
+                           const float* y, const uint32_t* keys, int64_t n, float* x, float* lw, hipStream_t st) {
+  if ((model != 0 && model != 2) || d < 1 || dy < 1 || d > kLgMaxDim || dy > kLgMaxDim) return MOCAT_ERR_ARG;
+  if (model == 2 && (phase == 2 || !y)) return MOCAT_ERR_ARG;
   const int64_t blocks = (n + kLgThreads - 1) / kLgThreads;
-  hipLaunchKernelGGL(lg_hook_kernel, dim3((unsigned)(blocks < 8192 ? blocks : 8192)), dim3(kLgThreads), 0, st, phase,
-                     d, dy, params, xprev, y, keys, n, x, lw);
+  hipLaunchKernelGGL(lg_hook_kernel, dim3((unsigned)(blocks < 8192 ? blocks : 8192)), dim3(kLgThreads), 0, st, model,
+                     phase, d, dy, params, xprev, y, keys, n, x, lw);
   MOCAT_LAUNCH_CHECK();
   return MOCAT_OK;
 }

@@ -373,20 +373,23 @@ def _compiled_shape(kind, d, dy) -> bool:
     return bool(c.value)
 
 
-def _run_lg_hooks(model, y, key, n, ess_threshold, resampling, store, initial=None, flags=0):
-    """One bootstrap filter on the linear-Gaussian model at a (d, d_y) without fused kernels: the
-    scan of run_particle_filter_for_marginals (mocat/src/ssm/filtering.py:255-324) with the fused
-    filters' key tree, flags and log-likelihood recursion, as a host loop over the HIP kernels: the
-    runtime-dimension hooks (initial sample and weight, proposal and weight), mocat_gather_rows, the
-    resampling entries (systematic / multinomial from the log-weights, or mocat's Gumbel-max) and the
-    FP64 log-sum-exp. The resampling decision of each step reads the previous step's ESS on the host.
-    Returns the fields of _launch's per-filter result."""
+def _run_lg_hooks(model, y, key, n, ess_threshold, resampling, store, initial=None, flags=0, optimal=None):
+    """One bootstrap (or, with `optimal`, optimal-proposal) filter on the linear-Gaussian model at a
+    (d, d_y) without fused kernels: the scan of run_particle_filter_for_marginals
+    (mocat/src/ssm/filtering.py:255-324) with the fused filters' key tree, flags and log-likelihood
+    recursion, as a host loop over the HIP kernels: the runtime-dimension hooks (initial sample and
+    weight, proposal and weight; the optimal filter's noise from the first key of each split),
+    mocat_gather_rows, the resampling entries (systematic / multinomial from the log-weights, or
+    mocat's Gumbel-max) and the FP64 log-sum-exp. The resampling decision of each step reads the
+    previous step's ESS on the host. Returns the fields of _launch's per-filter result."""
     from ..reduce import logsumexp_pair
     dev = mr.default_device()
     d, dy = int(model.dim), int(model.dim_obs)
     y = np.asarray(y, np.float32).reshape(-1, dy)
     T = y.shape[0]
-    params = model.kernel_params()
+    kid = 0 if optimal is None else OptimalNonLinearGaussianParticleFilter.kernel_model
+    params = model.kernel_params() if optimal is None else optimal.kernel_params(model)
+    pkeys = (lambda k: mr.split(k, n)) if optimal is None else (lambda k: mr.split(k, n)[:1])
     key = mr.as_key(key).reshape(2)
     resume = initial is not None
     sub = key if flags & PF_INIT_KEY else mr.split(key, 2)[1]
@@ -396,7 +399,7 @@ def _run_lg_hooks(model, y, key, n, ess_threshold, resampling, store, initial=No
         x = torch.as_tensor(initial[0], device=dev).reshape(n, d).to(torch.float32).contiguous()
         lw = torch.as_tensor(initial[1], device=dev).reshape(n).to(torch.float32).contiguous()
     else:
-        x, lw = _hook(0, params, model, 0, None, yt[0], mr.split(sub, n), n)
+        x, lw = _hook(kid, params, model, 0, None, yt[0], pkeys(sub), n)
     logn = float(np.log(n))
     lse, lse2 = (float(v) for v in logsumexp_pair(lw))
     ll = 0.0 if resume else lse - logn
@@ -415,7 +418,7 @@ def _run_lg_hooks(model, y, key, n, ess_threshold, resampling, store, initial=No
             _lib.call("mocat_gather_rows", _lib.ptr(x), _lib.ptr(idx), n, d, _lib.ptr(xp), _lib.stream_handle())
         else:
             idx, xp = ident, x
-        x, inc = _hook(0, params, model, 1, xp, yt[t], mr.split(ik, n), n)
+        x, inc = _hook(kid, params, model, 1, xp, yt[t], pkeys(ik), n)
         lw = inc if res else lw + inc
         lse_prev = lse
         lse, lse2 = (float(v) for v in logsumexp_pair(lw))
@@ -452,15 +455,13 @@ def _launch(models, ys, keys, ns, ess_threshold, resampling, store, grad=False, 
         kind = OptimalNonLinearGaussianParticleFilter.kernel_model
         if any(int(v) * d >= 2 ** 32 for v in ns):
             raise ValueError("the optimal filter draws normal(key, (n, d)): n * d must be < 2^32")
-        if not _compiled_shape(kind, d, dy):
-            raise NotImplementedError("the optimal-proposal filter is compiled for (d, d_y) in (1,1) (2,1) (2,2) (3,3) "
-                                      "(4,1) (4,2) (4,4) (5,5) (8,4) (8,8)")
-    elif kind == 0 and not _compiled_shape(0, d, dy):
+    if kind in (0, 2) and not _compiled_shape(kind, d, dy):
         # another linear-Gaussian shape: the runtime-dimension hooks, one filter at a time
         if grad:
             raise NotImplementedError("log-likelihood gradients are implemented for the 1-D stochastic-volatility model")
         return [_run_lg_hooks(models[i], ys[i], keys[i], int(ns[i]), ess_threshold, resampling, store,
-                              None if initial is None else initial[i], extra_flags) for i in range(len(models))]
+                              None if initial is None else initial[i], extra_flags, opt if optimal else None)
+                for i in range(len(models))]
     F = len(models)
     Ts = [int(np.asarray(y).shape[0]) for y in ys]
     order = sorted(range(F), key=lambda i: -(ns[i] * Ts[i]))  # longest first

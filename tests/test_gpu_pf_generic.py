@@ -92,9 +92,34 @@ def test_generic_shape_hooks_and_batches(m):
                                                       k, n=n, ess_threshold=0.6, resampling="systematic")
         assert r.log_likelihood.item() == one.log_likelihood.item()
         assert torch.equal(r.value, one.value) and r.final_value.shape == (n, d)
-    with pytest.raises(NotImplementedError, match="optimal-proposal filter is compiled"):
-        m.ssm.run_particle_filter_for_marginals(model, m.ssm.OptimalNonLinearGaussianParticleFilter(), torch.tensor(y),
-                                                np.arange(12.0), keys[0], n=100)
+
+
+@pytest.mark.parametrize("d,dy,resampling", [(6, 3, "systematic"), (16, 9, "multinomial")])
+def test_generic_shape_optimal_filter_same_path_as_fp64_oracle(m, d, dy, resampling):
+    """OptimalNonLinearGaussianParticleFilter at a runtime-dimension shape vs the FP64 oracle on the
+    GPU's own resampling decisions and ancestors (zero initial weights; log p(y_1:T | y_0))."""
+    from oracle import nlg as onlg
+    model, p = _model(m, d, dy, 3 * d + dy)
+    q = onlg.startup(model.initial_mean, model.initial_covariance, model.transition_matrix,
+                     model.transition_covariance, model.likelihood_matrix, model.likelihood_covariance)
+    T, n, thr = 20, 1200, 0.7
+    _, y = opf.simulate(p, T, orng.PRNGKey(d))
+    key = m.random.PRNGKey(9)
+    out = m.ssm.run_particle_filter_for_marginals(model, m.ssm.OptimalNonLinearGaussianParticleFilter(),
+                                                  torch.tensor(y, dtype=torch.float32), np.arange(float(T)), key,
+                                                  n=n, ess_threshold=thr, resampling=resampling)
+    ess = out.ess.cpu().numpy()
+    flags = np.zeros(T, bool)
+    flags[1:] = ess[:-1] < thr * n
+    o = onlg.run_optimal_pf(q, y, key.cpu().numpy(), n, thr, resampling, ancestors=out.ancestors.cpu().numpy(),
+                            resample_flags=flags)
+    np.testing.assert_allclose(out.value[0].cpu().numpy(), o["value"][0], rtol=1e-5, atol=1e-5)
+    np.testing.assert_array_equal(out.log_weight[0].cpu().numpy(), np.zeros(n, np.float32))
+    np.testing.assert_allclose(out.value[-1].cpu().numpy(), o["value"][-1], rtol=1e-3, atol=1e-3)
+    np.testing.assert_allclose(out.log_weight[-1].cpu().numpy(), o["log_weight"][-1], rtol=1e-4, atol=1e-3)
+    ll = out.log_likelihood.item()
+    assert abs(ll - o["log_likelihood"]) <= 1e-4 * abs(o["log_likelihood"]) + 1e-4, (ll, o["log_likelihood"])
+    np.testing.assert_allclose(ess, o["ess"], rtol=1e-3)
 
 
 def test_generic_shape_one_step_functions_repeat_the_run(m):
