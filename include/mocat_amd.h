@@ -249,7 +249,8 @@ int mocat_resample_categorical(const float* log_weight, int64_t n, int64_t n_out
  * body of backward_simulation_full, :226-237). One backward step for m trajectories:
  * x_out[k] = particles[idx[k]], idx[k] = argmax_j(gumbel(split(key, m)[k], (n,))[j] + log_weight[j]
  * - U(particles[j] -> x_next[k])) (first maximum), U the transition potential of model 0 (linear
- * Gaussian, d in {1,2,3,4,5,8}; params A[d*d], inv(cholesky(Q))[d*d], normaliser) or model 1
+ * Gaussian: compiled for d in {1,2,3,4,5,8}, a runtime-dimension kernel for any other d <= 16; params
+ * A[d*d], inv(cholesky(Q))[d*d], normaliser) or model 1
  * (stochastic volatility, d = 1, unit steps; params phi, 1/sigma, normaliser). idx may be NULL.
  * particles [n][d], x_next and x_out [m][d]. */
 int mocat_ffbsi_step(int model, int d, const float* params, const float* particles, const float* log_weight,

@@ -135,6 +135,52 @@ int launch_ffbsi(const float* params, const float* xs, const float* lw, int64_t 
   return MOCAT_OK;
 }
 
+// The full backward step for the linear-Gaussian model at a d without a compiled kernel (d <= kRtMaxDim):
+// the same draw and potential (0.5 |(x' - A x) @ S|^2 + nlz, the products in LgTrans's order) with A and
+// S in LDS, each wave's x_{t+1} in LDS, and each lane's particle state and difference vector in its own
+// LDS rows (indexed at run time).
+constexpr int kRtMaxDim = 16;
+
+__global__ __launch_bounds__(256) void ffbsi_step_rt_kernel(int d, const float* __restrict__ params,
+                                                            const float* __restrict__ xs,
+                                                            const float* __restrict__ lw, int64_t n,
+                                                            const float* __restrict__ xnext, int64_t m_count,
+                                                            const uint32_t* __restrict__ key,
+                                                            int32_t* __restrict__ idx, float* __restrict__ xout) {
+  __shared__ float A[kRtMaxDim * kRtMaxDim], S[kRtMaxDim * kRtMaxDim];
+  __shared__ float XN[256 / kWave][kRtMaxDim];
+  __shared__ float XP[kRtMaxDim][256], DF[kRtMaxDim][256];
+  const int t = threadIdx.x, lane = t & (kWave - 1), w = t / kWave;
+  for (int k = t; k < d * d; k += 256) {
+    A[k] = params[k];
+    S[k] = params[d * d + k];
+  }
+  const float nlz = params[2 * d * d];
+  const int64_t m = (int64_t)blockIdx.x * (256 / kWave) + w;
+  if (lane < d && m < m_count) XN[w][lane] = xnext[m * d + lane];
+  __syncthreads();
+  if (m >= m_count) return;
+  const Key km = split_at(Key{key[0], key[1]}, (uint32_t)m_count, (uint32_t)m);
+  auto pot = [&](int64_t j) {
+    for (int k = 0; k < d; ++k) XP[k][t] = xs[j * d + k];
+    for (int i = 0; i < d; ++i) {
+      float a = 0.f;
+      for (int k = 0; k < d; ++k) a += A[i * d + k] * XP[k][t];
+      DF[i][t] = XN[w][i] - a;
+    }
+    float s = 0.f;
+    for (int k = 0; k < d; ++k) {
+      float v = 0.f;
+      for (int i = 0; i < d; ++i) v += DF[i][t] * S[i * d + k];
+      s += v * v;
+    }
+    return 0.5f * s + nlz;
+  };
+  const int64_t bi = gumbel_argmax_wave(km, (uint32_t)n, 0, n, lane, [&](int64_t j) { return lw[j] - pot(j); });
+  if (lane == 0 && idx) idx[m] = (int32_t)bi;
+  if (lane < d) xout[m * d + lane] = xs[bi * d + lane];
+}
+
 // ---------------------------------------------------------------- rejection backward simulation
 // mocat/src/ssm/backward.py:42-194 for one backward step (rejection_resampling): initial proposals
 // x0[m] = categorical(k0, lw, shape=(M,))[m] with (k0, k1, k2) = split(key_t, 3), densities
@@ -419,7 +465,14 @@ int mocat_ffbsi_step(int model, int d, const float* params, const float* particl
     case 4: return launch_ffbsi<LgTrans<4>, 4>(params, particles, log_weight, n, x_next, m, key, idx, x_out, stream);
     case 5: return launch_ffbsi<LgTrans<5>, 5>(params, particles, log_weight, n, x_next, m, key, idx, x_out, stream);
     case 8: return launch_ffbsi<LgTrans<8>, 8>(params, particles, log_weight, n, x_next, m, key, idx, x_out, stream);
-    default: return MOCAT_ERR_ARG;
+    default: {
+      if (d < 1 || d > kRtMaxDim) return MOCAT_ERR_ARG;
+      const int per = 256 / kWave;
+      hipLaunchKernelGGL(ffbsi_step_rt_kernel, dim3((unsigned)((m + per - 1) / per)), dim3(256), 0, stream, d, params,
+                         particles, log_weight, n, x_next, m, key, idx, x_out);
+      MOCAT_LAUNCH_CHECK();
+      return MOCAT_OK;
+    }
   }
 }
 

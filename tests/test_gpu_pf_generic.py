@@ -149,3 +149,34 @@ def test_generic_shape_one_step_functions_repeat_the_run(m):
     lw0 = full.log_weight[0].double()
     l0 = (torch.logsumexp(lw0, 0) - np.log(n)).item()
     assert abs(res.log_likelihood.item() - (full.log_likelihood.item() - l0)) < 1e-9 * (1 + abs(l0))
+
+
+@pytest.mark.parametrize("d,dy", [(6, 3), (11, 4)])
+def test_generic_shape_backward_simulation_vs_oracle(m, d, dy):
+    """Full backward resampling at a d without a compiled kernel: every step's indices equal the
+    oracle's on the same keys (given the GPU's trajectories at the next time), except draws whose best
+    two scores lie within 1e-4 of each other; the rejection sampler raises the documented error."""
+    from oracle import ffbsi as of
+    model, p = _model(m, d, dy, 5 * d)
+    T, n, n_samps = 6, 400, 96
+    _, y = opf.simulate(p, T, orng.PRNGKey(3))
+    marg = m.ssm.run_particle_filter_for_marginals(model, m.ssm.BootstrapFilter(), torch.tensor(y, dtype=torch.float32),
+                                                   np.arange(float(T)), m.random.PRNGKey(4), n=n, ess_threshold=0.5)
+    key = m.random.PRNGKey(5)
+    res = m.ssm.backward_simulation(model, marg, key, n_samps)
+    assert res.value.shape == (T, n_samps, d)
+    pot = of.lgssm_transition(model.transition_matrix, model.transition_covariance)
+    vals = marg.value.cpu().numpy().astype(np.float64)
+    lw = marg.log_weight.cpu().numpy().astype(np.float64)
+    out, anc = res.value.cpu().numpy(), res.ancestors.cpu().numpy()
+    tk = orng.split(key.cpu().numpy(), T)
+    ties = []
+    for t in range(T - 2, -1, -1):
+        want, margin = of.backward_step_margin(pot, vals[t], lw[t], out[t + 1], tk[t])
+        decided = margin > 1e-4
+        np.testing.assert_array_equal(anc[t][decided], want[decided])
+        ties.append((~decided).mean())
+    assert max(ties) <= 0.02, ties
+    np.testing.assert_array_equal(out, np.take_along_axis(vals, anc[:, :, None].astype(np.int64), 1).astype(np.float32))
+    with pytest.raises(NotImplementedError, match="rejection backward simulation is compiled"):
+        m.ssm.backward_simulation(model, marg, key, n_samps, maximum_rejections=4)
