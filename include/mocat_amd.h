@@ -284,6 +284,16 @@ int mocat_enkf_workspace_bytes(int64_t n, int d, int64_t* bytes);
 int mocat_enkf_run(int d, int dy, const float* params, const float* y, int64_t T, int64_t n, const uint32_t* key,
                    float* x_out, void* workspace, int64_t workspace_bytes, float* out_value, float* out_gain,
                    hipStream_t stream);
+/* The EnKF's member updates at any (d, dy) up to 16 (mocat_enkf_run runs the (d, dy) pairs that
+ * mocat_pf_compiled_shape lists for model 0), one launch per call, params as mocat_enkf_run and z the
+ * rows of normal(key, (n, d)) (phase 2: (n, dy)); key a device uint32[2]:
+ *   phase 0 (initial_sample, nonlinear_gaussian.py:312-324): x_out = m0 + K0 (y - H m0) + z @ C0;
+ *   phase 1 (the forecast of :326-350): x_out = A x_in + LQ z;
+ *   phase 2 (the analysis): x_out = x_in + K (y - (H x_in + LR z)) with the gain K [d][dy].
+ * A host runs the filter as phase 0, then per step phase 1, the gain from the forecast ensemble's
+ * covariance (FP64), and phase 2 (mocat_amd/ssm/filtering.py, _run_enkf_hooks). n * max(d, dy) < 2^32. */
+int mocat_enkf_hook(int phase, int d, int dy, const float* params, const float* x_in, const uint32_t* key, int64_t n,
+                    const float* y, const float* gain, float* x_out, hipStream_t stream);
 
 /* -------------------------------------------------------------------------------- tempered SMC
  * dst[j][:] = src[idx[j]][:] for 4-byte elements, width per row: ensemble_state[resampled_indices]

@@ -163,6 +163,73 @@ __global__ __launch_bounds__(kLgThreads) void lg_hook_kernel(int model, int phas
   }
 }
 
+// The ensemble Kalman filter's member updates at a runtime (d, d_y) (mocat/src/ssm/nonlinear_gaussian.py:
+// 297-350, EnsembleKalmanFilter on TimeHomogenousLinearGaussian's transition x -> A x), params as
+// mocat_enkf_run's [m0, H, K0, C0, A, LQ, LR]; z = rows of normal(key, (n, k)) (k = d, or d_y for the
+// observation perturbation), and the products in enkf.hip's order:
+//   phase 0: x_i = (m0 + K0 (y - H m0)) + z_i @ C0
+//   phase 1: mx_i = A x_i + LQ z_i (the forecast; xin = x, xout = mx)
+//   phase 2: x_i = mx_i + K (y - (H mx_i + LR z_i)) with the gain K [d][d_y] (xin = mx)
+__global__ __launch_bounds__(kLgThreads) void lg_enkf_kernel(int phase, int d, int dy, const float* __restrict__ params,
+                                                             const float* __restrict__ xin, const uint32_t* __restrict__ key,
+                                                             int64_t n, const float* __restrict__ y,
+                                                             const float* __restrict__ gain, float* __restrict__ xout) {
+  __shared__ float P[kLgMaxParams];
+  __shared__ float Kg[kLgMaxDim * kLgMaxDim];
+  __shared__ float Z[kLgMaxDim][kLgThreads], XI[kLgMaxDim][kLgThreads], R[kLgMaxDim][kLgThreads];
+  const int np = d + 2 * dy * d + 3 * d * d + dy * dy;
+  for (int k = threadIdx.x; k < np; k += kLgThreads) P[k] = params[k];
+  if (phase == 2)
+    for (int k = threadIdx.x; k < d * dy; k += kLgThreads) Kg[k] = gain[k];
+  __syncthreads();
+  const float *m0 = P, *H = m0 + d, *K0 = H + dy * d, *C0 = K0 + d * dy, *A = C0 + d * d, *LQ = A + d * d,
+              *LR = LQ + d * d;
+  const int t = threadIdx.x;
+  const Key k{key[0], key[1]};
+  const int kz = phase == 2 ? dy : d;  // noise columns
+  const uint32_t total = (uint32_t)(n * kz);
+  for (int64_t i = blockIdx.x * (int64_t)kLgThreads + t; i < n; i += (int64_t)gridDim.x * kLgThreads) {
+    for (int q = 0; q < kz; ++q) Z[q][t] = normal_from_bits(bits_at(k, total, (uint32_t)(i * kz + q)));
+    if (phase == 0) {
+      for (int j = 0; j < dy; ++j) {
+        float hm = 0.f;
+        for (int q = 0; q < d; ++q) hm += H[j * d + q] * m0[q];
+        R[j][t] = y[j] - hm;
+      }
+      for (int r = 0; r < d; ++r) {
+        float g = 0.f, c = 0.f;
+        for (int j = 0; j < dy; ++j) g += K0[r * dy + j] * R[j][t];
+        for (int q = 0; q < d; ++q) c += Z[q][t] * C0[q * d + r];
+        xout[i * d + r] = (m0[r] + g) + c;
+      }
+      continue;
+    }
+    for (int q = 0; q < d; ++q) XI[q][t] = xin[i * d + q];
+    if (phase == 1) {
+      for (int r = 0; r < d; ++r) {
+        float u = 0.f, v = 0.f;
+        for (int q = 0; q < d; ++q) {
+          u += A[r * d + q] * XI[q][t];
+          v += LQ[r * d + q] * Z[q][t];
+        }
+        xout[i * d + r] = u + v;
+      }
+      continue;
+    }
+    for (int j = 0; j < dy; ++j) {
+      float hx = 0.f, e = 0.f;
+      for (int q = 0; q < d; ++q) hx += H[j * d + q] * XI[q][t];
+      for (int q = 0; q < dy; ++q) e += LR[j * dy + q] * Z[q][t];
+      R[j][t] = y[j] - (hx + e);
+    }
+    for (int r = 0; r < d; ++r) {
+      float g = 0.f;
+      for (int j = 0; j < dy; ++j) g += Kg[r * dy + j] * R[j][t];
+      xout[i * d + r] = XI[r][t] + g;
+    }
+  }
+}
+
 }  // namespace
 
 // Called by mocat_pf_hook (pf.hip) for models 0 and 2 at a (d, d_y) without a compiled kernel (its
@@ -179,3 +246,21 @@ int launch_lg_generic_hook(int model, int phase, int d, int dy, const float* par
 }
 
 }  // namespace mocat
+
+extern "C" {
+
+int mocat_enkf_hook(int phase, int d, int dy, const float* params, const float* x_in, const uint32_t* key, int64_t n,
+                    const float* y, const float* gain, float* x_out, hipStream_t stream) {
+  using namespace mocat;
+  if (phase < 0 || phase > 2 || d < 1 || dy < 1 || d > kLgMaxDim || dy > kLgMaxDim || !params || !key || !x_out ||
+      n <= 0 || (phase != 1 && !y) || (phase != 0 && !x_in) || (phase == 2 && !gain))
+    return MOCAT_ERR_ARG;
+  if (n * (int64_t)(d > dy ? d : dy) >= ((int64_t)1 << 32)) return MOCAT_ERR_SIZE;
+  const int64_t blocks = (n + kLgThreads - 1) / kLgThreads;
+  hipLaunchKernelGGL(lg_enkf_kernel, dim3((unsigned)(blocks < 8192 ? blocks : 8192)), dim3(kLgThreads), 0, stream,
+                     phase, d, dy, params, x_in, key, n, y, gain, x_out);
+  MOCAT_LAUNCH_CHECK();
+  return MOCAT_OK;
+}
+
+}  // extern "C"

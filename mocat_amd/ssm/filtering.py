@@ -36,6 +36,7 @@ _lib.register("mocat_pf_hook", [_lib.c_i32, _lib.c_i32, _lib.c_i32, _lib.c_i32, 
 _lib.register("mocat_resample", [_P, _lib.c_i64, _lib.c_i32, _lib.c_i32, _P, _P, _P, _P])
 _lib.register("mocat_pf_compiled_shape", [_lib.c_i32, _lib.c_i32, _lib.c_i32, _P])
 _lib.register("mocat_gather_rows", [_P, _P, _lib.c_i64, _lib.c_i64, _P, _P])
+_lib.register("mocat_enkf_hook", [_lib.c_i32, _lib.c_i32, _lib.c_i32, _P, _P, _P, _lib.c_i64, _P, _P, _P, _P])
 _lib.register("mocat_resample_noalloc", [_P, _lib.c_i64, _lib.c_i32, _lib.c_i32, _P, _P, _P])
 _lib.register("mocat_resample_categorical", [_P, _lib.c_i64, _lib.c_i64, _P, _P, _P])
 _lib.register("mocat_enkf_param_count", [_lib.c_i32, _lib.c_i32, _P])
@@ -300,12 +301,58 @@ def _filter_kind(particle_filter):
                               "OptimalNonLinearGaussianParticleFilter and EnsembleKalmanFilter")
 
 
+def _run_enkf_hooks(model, enkf, yn, tn, random_key, n, store=True):
+    """The EnKF at a (d, d_y) without fused kernels (up to 16): the key tree of mocat_enkf_run, the
+    member updates by mocat_enkf_hook (initial ensemble; per step the forecast, then the analysis), and
+    each step's gain K = P H^T (H P H^T + LR LR^T)^-1 from the forecast ensemble's covariance P (two
+    passes in FP64: the mean, then the centred product as one GEMM), solved on the host in FP64."""
+    d, dy = int(model.dim), int(model.dim_obs)
+    dev = mr.default_device()
+    T = int(yn.shape[0])
+    params = torch.from_numpy(enkf.kernel_params(model)).to(dev)
+    M = _nlg_matrices(model)
+    H, LR = np.asarray(M["H"], np.float64), np.asarray(M["LR"], np.float64)
+    y_t = torch.from_numpy(np.ascontiguousarray(yn, np.float32).reshape(T, dy)).to(dev)
+    key = mr.as_key(random_key).reshape(2)
+    rk, sub = mr.split(key, 2)
+
+    def hook(phase, x_in, k, y, gain=None):
+        out = torch.empty((n, d), dtype=torch.float32, device=dev)
+        _lib.call("mocat_enkf_hook", phase, d, dy, _lib.ptr(params), _lib.ptr(x_in),
+                  _lib.ptr(mr.as_key(k).reshape(2).contiguous()), int(n), _lib.ptr(y), _lib.ptr(gain), _lib.ptr(out),
+                  _lib.stream_handle())
+        return out
+
+    x = hook(0, None, mr.split(sub, n)[0], y_t[0])
+    values, gains = [x], []
+    int_keys = mr.split(rk, T - 1) if T > 1 else None
+    for t in range(1, T):
+        k1, k2 = mr.split(mr.split(int_keys[t - 1], n)[0], 2)
+        mx = hook(1, x, k1, None)
+        c = mx.double()
+        c = c - c.mean(0, keepdim=True)
+        P = (c.T @ c / (n - 1)).cpu().numpy()
+        B = H @ P
+        K = np.linalg.solve(B @ H.T + LR @ LR.T, B).T
+        Kt = torch.from_numpy(np.ascontiguousarray(K, np.float32)).to(dev)
+        x = hook(2, mx, k2, y_t[t], Kt)
+        gains.append(Kt)
+        values.append(x)
+    gain = torch.stack(gains) if gains else torch.empty((0, d, dy), dtype=torch.float32, device=dev)
+    return cdict(value=torch.stack(values) if store else None,
+                 log_weight=torch.zeros((T, n), dtype=torch.float32, device=dev),
+                 t=torch.as_tensor(tn, device=dev), y=y_t, ess=torch.full((T,), float(n), device=dev), gain=gain,
+                 final_value=x)
+
+
 def _run_enkf(model, enkf, yn, tn, random_key, n, store=True):
     d, dy = int(model.dim), int(model.dim_obs)
     if n < 2:
         raise ValueError("the ensemble Kalman filter needs n >= 2 (its covariance divides by n - 1)")
     if n * max(d, dy) >= 2 ** 32:
         raise ValueError("the EnKF draws normal(key, (n, max(d, dy))): n * max(d, dy) must be < 2^32")
+    if not _compiled_shape(0, d, dy):
+        return _run_enkf_hooks(model, enkf, yn, tn, random_key, n, store)
     dev = mr.default_device()
     T = int(yn.shape[0])
     params = torch.from_numpy(enkf.kernel_params(model)).to(dev)

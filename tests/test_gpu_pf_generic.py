@@ -180,3 +180,25 @@ def test_generic_shape_backward_simulation_vs_oracle(m, d, dy):
     np.testing.assert_array_equal(out, np.take_along_axis(vals, anc[:, :, None].astype(np.int64), 1).astype(np.float32))
     with pytest.raises(NotImplementedError, match="rejection backward simulation is compiled"):
         m.ssm.backward_simulation(model, marg, key, n_samps, maximum_rejections=4)
+
+
+@pytest.mark.parametrize("d,dy,n", [(6, 3, 3000), (13, 5, 2)])
+def test_generic_shape_enkf_same_path_as_fp64_oracle(m, d, dy, n):
+    """EnsembleKalmanFilter at a runtime-dimension shape: the same keys and noise as the FP64 oracle;
+    ensembles and gains at every step (n = 2 is the smallest ensemble with a covariance)."""
+    from oracle import nlg as onlg
+    model, p = _model(m, d, dy, 7 * d + dy)
+    q = onlg.startup(model.initial_mean, model.initial_covariance, model.transition_matrix,
+                     model.transition_covariance, model.likelihood_matrix, model.likelihood_covariance)
+    T = 8
+    _, y = opf.simulate(p, T, orng.PRNGKey(d))
+    key = m.random.PRNGKey(17)
+    out = m.ssm.run_particle_filter_for_marginals(model, m.ssm.EnsembleKalmanFilter(), torch.tensor(y, dtype=torch.float32),
+                                                  np.arange(float(T)), key, n=n)
+    o = onlg.run_enkf(q, y, key.cpu().numpy(), n)
+    got = out.value.cpu().numpy()
+    assert got.shape == (T, n, d)
+    np.testing.assert_allclose(got[0], o["value"][0], rtol=1e-5, atol=1e-5)
+    np.testing.assert_allclose(got, o["value"], rtol=1e-4, atol=2e-4)
+    np.testing.assert_allclose(out.gain.cpu().numpy(), o["gain"], rtol=1e-4, atol=1e-5)
+    np.testing.assert_allclose(out.final_value.cpu().numpy(), o["value"][-1], rtol=1e-4, atol=2e-4)
