@@ -261,8 +261,8 @@ int mocat_ffbsi_step(int model, int d, const float* params, const float* particl
  * rounds against a shared density bound (raised by the inflation factor when exceeded, which sends
  * every trajectory back to proposing), at most max_rejections - 1 rounds, then full resampling for
  * the trajectories still waiting. evals (one int64 on the device) receives the step's number of
- * transition-density evaluations. Models and params as mocat_ffbsi_step, model 0 at the compiled d
- * only (another d returns -1); workspace: mocat_ffbsi_rejection_workspace_bytes(m). */
+ * transition-density evaluations. Models and params as mocat_ffbsi_step (model 0 at any d <= 16);
+ * workspace: mocat_ffbsi_rejection_workspace_bytes(m). */
 int mocat_ffbsi_rejection_workspace_bytes(int64_t m, int64_t* bytes);
 int mocat_ffbsi_rejection_step(int model, int d, const float* params, const float* particles, const float* log_weight,
                                int64_t n, const float* x_next, int64_t m, const uint32_t* key, int max_rejections,

@@ -20,7 +20,7 @@ namespace {
 template <int D>
 struct LgTrans {
   float A[D * D], S[D * D], nlz;
-  MOCAT_DEV void load(const float* p) {
+  MOCAT_DEV void load(const float* p, int /*d: D*/) {
 #pragma unroll
     for (int i = 0; i < D * D; ++i) A[i] = p[i];
 #pragma unroll
@@ -51,7 +51,7 @@ struct LgTrans {
 // stochastic volatility, unit steps (stochastic_volatility.py:78-84): params phi, 1/sigma, nlz
 struct SvTrans {
   float phi, isig, nlz;
-  MOCAT_DEV void load(const float* p) {
+  MOCAT_DEV void load(const float* p, int /*d: 1*/) {
     phi = p[0];
     isig = p[1];
     nlz = p[2];
@@ -59,6 +59,38 @@ struct SvTrans {
   MOCAT_DEV float pot(const float* xp, const float* xn) const {
     const float diff = xn[0] - xp[0] * phi;
     return 0.5f * (diff * diff) * (isig * isig) + nlz;
+  }
+};
+
+// linear Gaussian at a runtime d <= kRtMaxDim (the kernels' template D = 0): A and S read from params,
+// the difference vector in the calling lane's LDS row, the products in LgTrans's order
+constexpr int kRtMaxDim = 16;
+
+struct LgTransRt {
+  const float *A, *S;
+  float nlz;
+  int d;
+  MOCAT_DEV void load(const float* p, int dd) {
+    d = dd;
+    A = p;
+    S = p + dd * dd;
+    nlz = p[2 * dd * dd];
+  }
+  MOCAT_DEV float pot(const float* xp, const float* xn) const {
+    __shared__ float DF[kRtMaxDim][256];
+    const int t = threadIdx.x;
+    for (int i = 0; i < d; ++i) {
+      float a = 0.f;
+      for (int k = 0; k < d; ++k) a += A[i * d + k] * xp[k];
+      DF[i][t] = xn[i] - a;
+    }
+    float s = 0.f;
+    for (int k = 0; k < d; ++k) {
+      float v = 0.f;
+      for (int i = 0; i < d; ++i) v += DF[i][t] * S[i * d + k];
+      s += v * v;
+    }
+    return 0.5f * s + nlz;
   }
 };
 
@@ -114,7 +146,7 @@ __global__ __launch_bounds__(256) void ffbsi_step_kernel(const float* __restrict
   const int64_t m = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
   if (m >= m_count) return;
   Tr tr;
-  tr.load(params);
+  tr.load(params, D);
   const Key km = split_at(Key{key[0], key[1]}, (uint32_t)m_count, (uint32_t)m);
   float xn[D];
 #pragma unroll
@@ -139,7 +171,6 @@ int launch_ffbsi(const float* params, const float* xs, const float* lw, int64_t 
 // the same draw and potential (0.5 |(x' - A x) @ S|^2 + nlz, the products in LgTrans's order) with A and
 // S in LDS, each wave's x_{t+1} in LDS, and each lane's particle state and difference vector in its own
 // LDS rows (indexed at run time).
-constexpr int kRtMaxDim = 16;
 
 __global__ __launch_bounds__(256) void ffbsi_step_rt_kernel(int d, const float* __restrict__ params,
                                                             const float* __restrict__ xs,
@@ -208,23 +239,25 @@ struct RejBuffers {
   RejState* st;
 };
 
+// D = 0: the runtime dimension rt_d (LgTransRt); otherwise rt_d is unused
 template <class Tr, int D>
 __global__ __launch_bounds__(256) void rej_init_kernel(const float* __restrict__ params, const float* __restrict__ xs,
                                                        const float* __restrict__ lw, int64_t n,
                                                        const float* __restrict__ xnext, int64_t M,
                                                        const uint32_t* __restrict__ key, int32_t* __restrict__ idx,
-                                                       RejBuffers b) {
+                                                       RejBuffers b, int rt_d) {
+  const int64_t DD = D > 0 ? D : rt_d;
   const int lane = threadIdx.x & (kWave - 1);
   const int64_t m = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
   if (m >= M) return;
   Tr tr;
-  tr.load(params);
+  tr.load(params, (int)DD);
   const Key kt{key[0], key[1]};
   const int64_t bi = gumbel_argmax_wave(split_at(kt, 3, 0), (uint32_t)(M * n), (uint64_t)m * (uint64_t)n, n, lane,
                                         [&](int64_t j) { return lw[j]; });
   if (lane == 0) {
     idx[m] = (int32_t)bi;
-    b.dens[m] = expf(-tr.pot(xs + bi * D, xnext + m * D));
+    b.dens[m] = expf(-tr.pot(xs + bi * DD, xnext + m * DD));
   }
 }
 
@@ -311,7 +344,8 @@ template <class Tr, int D>
 __global__ __launch_bounds__(256) void rej_round_kernel(const float* __restrict__ params, const float* __restrict__ xs,
                                                         const float* __restrict__ lw, int64_t n,
                                                         const float* __restrict__ xnext, int64_t M,
-                                                        int32_t* __restrict__ idx, RejBuffers b) {
+                                                        int32_t* __restrict__ idx, RejBuffers b, int rt_d) {
+  const int64_t DD = D > 0 ? D : rt_d;
   const int lane = threadIdx.x & (kWave - 1);
   const int64_t m = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
   if (m >= M || !b.st->any) return;
@@ -324,12 +358,12 @@ __global__ __launch_bounds__(256) void rej_round_kernel(const float* __restrict_
     return;
   }
   Tr tr;
-  tr.load(params);
+  tr.load(params, (int)DD);
   const Key k{b.keys[2 * m], b.keys[2 * m + 1]};
   const Key kr = split_at(k, 3, 0), kc = split_at(k, 3, 1), ku = split_at(k, 3, 2);
   const int64_t bi = gumbel_argmax_wave(kc, (uint32_t)n, 0, n, lane, [&](int64_t j) { return lw[j]; });
   if (lane == 0) {
-    const float dd = expf(-tr.pot(xs + bi * D, xnext + m * D));
+    const float dd = expf(-tr.pot(xs + bi * DD, xnext + m * DD));
     const float u = uniform_from_bits(bits_at(ku, 1, 0), 0.f, 1.f);
     idx[m] = (int32_t)bi;
     b.dens[m] = dd;
@@ -346,7 +380,8 @@ __global__ __launch_bounds__(256) void rej_final_kernel(const float* __restrict_
                                                         const float* __restrict__ lw, int64_t n,
                                                         const float* __restrict__ xnext, int64_t M,
                                                         int32_t* __restrict__ idx, float* __restrict__ xout,
-                                                        RejBuffers b) {
+                                                        RejBuffers b, int rt_d) {
+  const int64_t DD = D > 0 ? D : rt_d;
   const int lane = threadIdx.x & (kWave - 1);
   const int64_t m = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
   if (m >= M) return;
@@ -354,35 +389,42 @@ __global__ __launch_bounds__(256) void rej_final_kernel(const float* __restrict_
   int64_t bi;
   if (wait) {
     Tr tr;
-    tr.load(params);
-    float xn[D];
-#pragma unroll
-    for (int q = 0; q < D; ++q) xn[q] = xnext[m * D + q];
+    tr.load(params, (int)DD);
     const Key k{b.keys[2 * m], b.keys[2 * m + 1]};
-    bi = gumbel_argmax_wave(k, (uint32_t)n, 0, n, lane, [&](int64_t j) { return lw[j] - tr.pot(xs + j * D, xn); });
+    if constexpr (D > 0) {
+      float xn[D];
+#pragma unroll
+      for (int q = 0; q < D; ++q) xn[q] = xnext[m * D + q];
+      bi = gumbel_argmax_wave(k, (uint32_t)n, 0, n, lane, [&](int64_t j) { return lw[j] - tr.pot(xs + j * D, xn); });
+    } else {
+      const float* xn = xnext + m * DD;
+      bi = gumbel_argmax_wave(k, (uint32_t)n, 0, n, lane, [&](int64_t j) { return lw[j] - tr.pot(xs + j * DD, xn); });
+    }
     if (lane == 0) idx[m] = (int32_t)bi;
   } else {
     bi = idx[m];
   }
   if (lane == 0) b.entered[m] = wait ? 1 : 0;
-  if (lane < D) xout[m * D + lane] = xs[bi * D + lane];
+  for (int64_t q = lane; q < DD; q += kWave) xout[m * DD + q] = xs[bi * DD + q];
 }
 
 template <class Tr, int D>
 int launch_rejection(const float* params, const float* xs, const float* lw, int64_t n, const float* xnext, int64_t M,
                      const uint32_t* key, int max_rejections, float init_bound, float inflation, RejBuffers b,
-                     int32_t* idx, float* xout, long long* evals_out, hipStream_t st) {
+                     int32_t* idx, float* xout, long long* evals_out, hipStream_t st, int rt_d = D) {
   const int per = 256 / kWave;
   const dim3 waves((unsigned)((M + per - 1) / per)), thr((unsigned)((M + 255) / 256));
-  hipLaunchKernelGGL((rej_init_kernel<Tr, D>), waves, dim3(256), 0, st, params, xs, lw, n, xnext, M, key, idx, b);
+  hipLaunchKernelGGL((rej_init_kernel<Tr, D>), waves, dim3(256), 0, st, params, xs, lw, n, xnext, M, key, idx, b,
+                     rt_d);
   hipLaunchKernelGGL(rej_reduce_kernel, dim3(1), dim3(1024), 0, st, 0, M, n, init_bound, inflation, b, evals_out);
   hipLaunchKernelGGL(rej_flags_kernel, thr, dim3(256), 0, st, M, key, b);
   hipLaunchKernelGGL(rej_any_kernel, dim3(1), dim3(1024), 0, st, M, b);
   for (int r = 1; r < max_rejections; ++r) {
-    hipLaunchKernelGGL((rej_round_kernel<Tr, D>), waves, dim3(256), 0, st, params, xs, lw, n, xnext, M, idx, b);
+    hipLaunchKernelGGL((rej_round_kernel<Tr, D>), waves, dim3(256), 0, st, params, xs, lw, n, xnext, M, idx, b, rt_d);
     hipLaunchKernelGGL(rej_reduce_kernel, dim3(1), dim3(1024), 0, st, 1, M, n, init_bound, inflation, b, evals_out);
   }
-  hipLaunchKernelGGL((rej_final_kernel<Tr, D>), waves, dim3(256), 0, st, params, xs, lw, n, xnext, M, idx, xout, b);
+  hipLaunchKernelGGL((rej_final_kernel<Tr, D>), waves, dim3(256), 0, st, params, xs, lw, n, xnext, M, idx, xout, b,
+                     rt_d);
   hipLaunchKernelGGL(rej_reduce_kernel, dim3(1), dim3(1024), 0, st, 2, M, n, init_bound, inflation, b, evals_out);
   MOCAT_LAUNCH_CHECK();
   return MOCAT_OK;
@@ -441,7 +483,10 @@ int mocat_ffbsi_rejection_step(int model, int d, const float* params, const floa
     case 4: MOCAT_REJ(LgTrans<4>, 4);
     case 5: MOCAT_REJ(LgTrans<5>, 5);
     case 8: MOCAT_REJ(LgTrans<8>, 8);
-    default: return MOCAT_ERR_ARG;
+    default:
+      if (d < 1 || d > kRtMaxDim) return MOCAT_ERR_ARG;
+      return launch_rejection<LgTransRt, 0>(params, particles, log_weight, n, x_next, m, key, max_rejections,
+                                            init_bound, bound_inflation, b, idx, x_out, evals, stream, d);
   }
 #undef MOCAT_REJ
 }

@@ -46,10 +46,8 @@ def backward_simulation(ssm_scenario, marginal_particles: cdict, random_key, n_s
     m = n if n_samps is None else int(n_samps)
     if n * m >= 0xFFFFFFFF:
         raise ValueError("the final-time draw uses an (n_samps, n) Gumbel matrix: n * n_samps must be < 2^32 - 1")
-    if maximum_rejections > 0 and model == 0 and d not in (1, 2, 3, 4, 5, 8):
-        raise NotImplementedError("rejection backward simulation is compiled for the linear-Gaussian model at d in "
-                                  "1, 2, 3, 4, 5, 8; full backward resampling (maximum_rejections = 0) runs at any d "
-                                  "up to 16")
+    if model == 0 and d > 16:
+        raise NotImplementedError("backward simulation runs the linear-Gaussian model up to d = 16")
     if model == 1 and hasattr(marginal_particles, "t") and T > 1:
         tt = torch.as_tensor(marginal_particles.t).double().cpu().numpy()
         if not np.allclose(np.diff(tt), 1.0):

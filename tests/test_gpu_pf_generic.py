@@ -153,9 +153,9 @@ def test_generic_shape_one_step_functions_repeat_the_run(m):
 
 @pytest.mark.parametrize("d,dy", [(6, 3), (11, 4)])
 def test_generic_shape_backward_simulation_vs_oracle(m, d, dy):
-    """Full backward resampling at a d without a compiled kernel: every step's indices equal the
-    oracle's on the same keys (given the GPU's trajectories at the next time), except draws whose best
-    two scores lie within 1e-4 of each other; the rejection sampler raises the documented error."""
+    """Backward simulation at a d without a compiled kernel: full resampling and rejection rounds, every
+    step's indices equal to the oracle's on the same keys (given the GPU's trajectories at the next
+    time), except draws whose best two scores (or acceptance tests) lie within 1e-4."""
     from oracle import ffbsi as of
     model, p = _model(m, d, dy, 5 * d)
     T, n, n_samps = 6, 400, 96
@@ -178,8 +178,22 @@ def test_generic_shape_backward_simulation_vs_oracle(m, d, dy):
         ties.append((~decided).mean())
     assert max(ties) <= 0.02, ties
     np.testing.assert_array_equal(out, np.take_along_axis(vals, anc[:, :, None].astype(np.int64), 1).astype(np.float32))
-    with pytest.raises(NotImplementedError, match="rejection backward simulation is compiled"):
-        m.ssm.backward_simulation(model, marg, key, n_samps, maximum_rejections=4)
+    # rejection rounds (backward.py:42-194) at this d: the same proposals, acceptances, bound raises and
+    # final full draws as the oracle on the same keys, except near-ties; the evaluation counts follow
+    for max_rej, bound in ((1, 0.0), (12, 0.0)):
+        res = m.ssm.backward_simulation(model, marg, key, n_samps, maximum_rejections=max_rej,
+                                        transition_dens_bound_parameter=bound, bound_inflation=1.01)
+        out, anc = res.value.cpu().numpy(), res.ancestors.cpu().numpy()
+        ties = []
+        for t in range(T - 2, -1, -1):
+            idx, evals, tie = of.rejection_step(pot, vals[t], lw[t], out[t + 1], tk[t], max_rej, bound, 1.01, 1e-4)
+            np.testing.assert_array_equal(anc[t][~tie], idx[~tie])
+            ties.append(tie.mean())
+            got = res.num_transition_evals[t + 1].item()
+            assert abs(got - evals) <= 0.02 * evals + n, (got, evals)
+        assert max(ties) <= 0.02, ties
+        np.testing.assert_array_equal(out, np.take_along_axis(vals, anc[:, :, None].astype(np.int64), 1)
+                                      .astype(np.float32))
 
 
 @pytest.mark.parametrize("d,dy,n", [(6, 3, 3000), (13, 5, 2)])
