@@ -48,8 +48,8 @@ struct SvgdLayout {
 
 // Column means, FP64, in a fixed summation order, in two passes: workgroup (column group cg, row slice
 // rs) sums its slice's rows (lane = column, coalesced row segments; wave w takes rows w, w+4, ...;
-// the four wave sums in wave order) into partial[rs][column]; then one thread per column adds the
-// slices in order.
+// the four wave sums in wave order) into partial[rs][column]; then every packing workgroup adds the
+// slices of each column in order (one thread per column), so all of them hold the same means.
 constexpr int kMeanSlices = 64;
 __global__ __launch_bounds__(256) void colmean_part_kernel(const float* __restrict__ x, int n, int d, int slices,
                                                            double* __restrict__ partial) {
@@ -71,46 +71,84 @@ __global__ __launch_bounds__(256) void colmean_part_kernel(const float* __restri
   if (w == 0 && k < d) partial[(int64_t)rs * d + k] = red[0][lane] + red[1][lane] + red[2][lane] + red[3][lane];
 }
 
-__global__ __launch_bounds__(kWave) void colmean_final_kernel(const double* __restrict__ partial, int n, int d,
-                                                              int slices, float* __restrict__ mean) {
-  const int k = blockIdx.x * kWave + threadIdx.x;
-  if (k >= d) return;
-  // loads issued in groups of 16 ahead of the in-order additions (slices <= kMeanSlices)
-  double t = 0.0;
-  for (int q0 = 0; q0 < slices; q0 += 16) {
-    double v[16];
-#pragma unroll
-    for (int u = 0; u < 16; ++u) v[u] = q0 + u < slices ? partial[(int64_t)(q0 + u) * d + k] : 0.0;
-#pragma unroll
-    for (int u = 0; u < 16; ++u) t += v[u];
-  }
-  mean[k] = (float)(t / (double)n);
-}
-
 // Pack the centred particles (A layout), [grad | centred x] (B layout) and |x_j - mean|^2 per
 // 32-row block, one workgroup per block: a thread writes 8 consecutive bf16 (16 bytes) of the hi and of
-// the lo part; the norms add the A threads' 8-column partial sums of a row in column order.
+// the lo part; the norms add the A threads' 8-column partial sums of a row in column order. The means
+// come from the column-mean partials (slices <= kMeanSlices); workgroup 0 also stores them in mean.
 template <int DP>
 __global__ __launch_bounds__(256) void svgd_pack_kernel(const float* __restrict__ x, const float* __restrict__ g,
-                                                        const float* __restrict__ mean, int n, int d,
+                                                        const double* __restrict__ cpart, int slices,
+                                                        float* __restrict__ mean, int n, int d,
                                                         char* __restrict__ out) {
   using L = SvgdLayout<DP>;
   constexpr int KB = L::KB, NB = L::NB;
+  static_assert(DP <= 256, "one thread per column for the means");
   __shared__ float nsq[32][2 * KB];
+  __shared__ float smean[DP];
   const int rb = blockIdx.x, t = threadIdx.x;
   char* blk = out + (int64_t)rb * L::BLOCK_BYTES;
   bf16x8* A = reinterpret_cast<bf16x8*>(blk);
   bf16x8* B = A + L::A_ELEMS / 8;
   // A layout [2][KB][64][8]: lane l holds row l%32, columns kb*16 + (l/32)*8 + s
-  for (int task = t; task < KB * 64; task += 256) {
-    const int kb = task / 64, lane = task % 64;
+  // a thread takes tasks t, t + 256, ... of each layout (whole rounds); all its loads of both layouts
+  // are issued before the first conversion
+  constexpr int PA = KB * 64 / 256, PB = 2 * NB * 64 / 256;
+  static_assert(KB * 64 % 256 == 0 && 2 * NB * 64 % 256 == 0, "whole task rounds per thread");
+  float va[PA][8], vb[PB][8];
+#pragma unroll
+  for (int p = 0; p < PA; ++p) {
+    const int task = p * 256 + t, kb = task / 64, lane = task % 64;
+    const int row = rb * 32 + (lane & 31), c0 = kb * 16 + (lane >> 5) * 8;
+#pragma unroll
+    for (int s2 = 0; s2 < 8; ++s2) {
+      const int c = c0 + s2;
+      va[p][s2] = (row < n && c < d) ? x[(int64_t)row * d + c] : 0.f;
+    }
+  }
+#pragma unroll
+  for (int p = 0; p < PB; ++p) {
+    const int task = p * 256 + t, kb2 = task / (NB * 64), rest = task % (NB * 64), lane = rest % 64;
+    const int c = (rest / 64) * 32 + (lane & 31);
+#pragma unroll
+    for (int s2 = 0; s2 < 8; ++s2) {
+      const int row = rb * 32 + acc_row(kb2 * 8 + s2, lane);
+      float v = 0.f;
+      if (row < n) {
+        if (c < DP) {
+          if (c < d) v = g[(int64_t)row * d + c];
+        } else if (c - DP < d) {
+          v = x[(int64_t)row * d + (c - DP)];
+        }
+      }
+      vb[p][s2] = v;
+    }
+  }
+  // the column means: the slices added in order, as one sum (loads in two batches of 32)
+  if (t < d) {
+    double s = 0.0;
+#pragma unroll 1
+    for (int u0 = 0; u0 < kMeanSlices; u0 += 32) {
+      double v[32];
+#pragma unroll
+      for (int u = 0; u < 32; ++u) v[u] = u0 + u < slices ? cpart[(int64_t)(u0 + u) * d + t] : 0.0;
+#pragma unroll
+      for (int u = 0; u < 32; ++u) s += v[u];
+    }
+    const float m = (float)(s / (double)n);
+    smean[t] = m;
+    if (rb == 0) mean[t] = m;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int p = 0; p < PA; ++p) {
+    const int task = p * 256 + t, kb = task / 64, lane = task % 64;
     const int rl = lane & 31, row = rb * 32 + rl, c0 = kb * 16 + (lane >> 5) * 8;
     bf16x8 h8, l8;
     float ss = 0.f;
 #pragma unroll
     for (int s2 = 0; s2 < 8; ++s2) {
       const int c = c0 + s2;
-      const float v = (row < n && c < d) ? x[(int64_t)row * d + c] - mean[c] : 0.f;
+      const float v = (row < n && c < d) ? va[p][s2] - smean[c] : 0.f;
       ss += v * v;
       __bf16 h, l;
       split_bf16(v, h, l);
@@ -123,21 +161,16 @@ __global__ __launch_bounds__(256) void svgd_pack_kernel(const float* __restrict_
   }
   // B layout [2][2][NB][64][8]: lane l holds column nb*32 + l%32 (grad for < DP, centred x above), rows
   // acc_row(kb2*8 + s, l)
-  for (int task = t; task < 2 * NB * 64; task += 256) {
-    const int kb2 = task / (NB * 64), rest = task % (NB * 64), nb = rest / 64, lane = rest % 64;
+#pragma unroll
+  for (int p = 0; p < PB; ++p) {
+    const int task = p * 256 + t, kb2 = task / (NB * 64), rest = task % (NB * 64), nb = rest / 64, lane = rest % 64;
     const int c = nb * 32 + (lane & 31);
+    const bool centred = c >= DP && c - DP < d;
     bf16x8 h8, l8;
 #pragma unroll
     for (int s2 = 0; s2 < 8; ++s2) {
       const int row = rb * 32 + acc_row(kb2 * 8 + s2, lane);
-      float v = 0.f;
-      if (row < n) {
-        if (c < DP) {
-          if (c < d) v = g[(int64_t)row * d + c];
-        } else if (c - DP < d) {
-          v = x[(int64_t)row * d + (c - DP)] - mean[c - DP];
-        }
-      }
+      const float v = (centred && row < n) ? vb[p][s2] - smean[c - DP] : vb[p][s2];
       __bf16 h, l;
       split_bf16(v, h, l);
       h8[s2] = h;
@@ -259,6 +292,10 @@ constexpr int kRowUnroll = 4;  // row elements loaded per thread before they are
 // histogram bins, which serialise across workgroups, so one workgroup per compute unit.
 constexpr int kSelectBlocks = 256;
 constexpr int kSelThreads = 1024;  // 16 waves per compute unit to cover the load latency
+// The bracketed passes run at 256 threads per workgroup: on the usual path (few candidates, selected by
+// the finishing kernel) they return at once, and their cost is the launch of the grid; a pass over all
+// pairs keeps 16 row loads in flight per thread instead.
+constexpr int kBracketThreads = 256, kBracketUnroll = 16;
 
 // Lanes of a wave that share an 8-bit digit (ballots of the digit's bits); the lowest lane of each
 // group adds the group's size to h[digit]: one LDS atomic per distinct digit of the wave, with no
@@ -928,8 +965,20 @@ MOCAT_DEV void radix_choose_block(int shift, const uint32_t* __restrict__ st_in,
 // remaining ranks) ends in st[0..3]. Every thread of the workgroup calls this.
 MOCAT_DEV void select_small_block(const uint32_t* __restrict__ cand, uint32_t m, uint32_t lo, uint32_t hi,
                                   uint32_t* sc, uint32_t* lh, uint32_t* st, uint32_t (*wtot)[4]) {
-  const int lane = threadIdx.x & (kWave - 1);
-  for (uint32_t k = threadIdx.x; k < m; k += blockDim.x) sc[k] = cand[k];
+  // staged with eight loads in flight per thread
+  for (uint32_t k0 = 0; k0 < m; k0 += 8 * blockDim.x) {
+    uint32_t u[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const uint32_t k = k0 + q * blockDim.x + threadIdx.x;
+      u[q] = k < m ? cand[k] : 0u;
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const uint32_t k = k0 + q * blockDim.x + threadIdx.x;
+      if (k < m) sc[k] = u[q];
+    }
+  }
   const uint32_t diff = lo ^ (hi - 1u);
   const int common = diff ? __clz(diff) : 32;  // leading bits shared by every candidate
   for (int pass = 0; pass < 4; ++pass) {
@@ -948,12 +997,12 @@ MOCAT_DEV void select_small_block(const uint32_t* __restrict__ cand, uint32_t m,
     __syncthreads();
     const uint32_t p0 = in[0] & mask, p1 = in[1] & mask;
     const bool shared_prefix = p0 == p1;
-    for (uint32_t k0 = 0; k0 < m; k0 += blockDim.x) {  // workgroup-uniform trip count
-      const uint32_t k = k0 + threadIdx.x;
-      const bool valid = k < m;
-      const uint32_t u = valid ? sc[k] : 0u, dig = (u >> shift) & 255u;
-      wave_multisplit_count(valid && (u & mask) == p0, dig, lh, lane);
-      if (!shared_prefix) wave_multisplit_count(valid && (u & mask) == p1, dig, lh + 256, lane);
+    // one LDS atomic per candidate: the whole selection runs on one compute unit, where the
+    // multisplit's ballots would cost more vector issue than the (rare) same-bin conflicts they avoid
+    for (uint32_t k = threadIdx.x; k < m; k += blockDim.x) {
+      const uint32_t u = sc[k], dig = (u >> shift) & 255u;
+      if ((u & mask) == p0) atomicAdd(&lh[dig], 1u);
+      if (!shared_prefix && (u & mask) == p1) atomicAdd(&lh[256 + dig], 1u);
     }
     __syncthreads();
     if (shared_prefix) {
@@ -970,7 +1019,7 @@ MOCAT_DEV void select_small_block(const uint32_t* __restrict__ cand, uint32_t m,
 // ranks reduced by the count below, when both ranks lie inside the bracket and the candidates fit, else
 // on all pairs; workgroup 0 records the decision and st[0]. Later passes first apply the digit choice
 // of the previous pass to their state (workgroup 0 records it as st[pass]).
-__global__ __launch_bounds__(kSelThreads) void radix_hist_bracket_kernel(const uint32_t* __restrict__ v, int n,
+__global__ __launch_bounds__(kBracketThreads) void radix_hist_bracket_kernel(const uint32_t* __restrict__ v, int n,
                                                                  const uint32_t* __restrict__ cand, Bracket* br,
                                                                  int pass, uint32_t* __restrict__ sel,
                                                                  uint32_t rank0, uint32_t rank1) {
@@ -1008,7 +1057,7 @@ __global__ __launch_bounds__(kSelThreads) void radix_hist_bracket_kernel(const u
   const uint32_t p0 = cur[0] & mask, p1 = cur[1] & mask;
   const bool shared_prefix = p0 == p1;
   const int lane = threadIdx.x & (kWave - 1);
-  for (int b = threadIdx.x; b < 512; b += kSelThreads) (&lh[0][0])[b] = 0u;
+  for (int b = threadIdx.x; b < 512; b += kBracketThreads) (&lh[0][0])[b] = 0u;
   __syncthreads();
   if (use_cand) {
     const int64_t m = br->inside;
@@ -1024,16 +1073,16 @@ __global__ __launch_bounds__(kSelThreads) void radix_hist_bracket_kernel(const u
   } else {
     for (int row = blockIdx.x; row < n - 1; row += gridDim.x) {
       const uint32_t* vr = v + (int64_t)row * n;
-      for (int c0 = row + 1; c0 < n; c0 += kRowUnroll * kSelThreads) {
-        uint32_t uu[kRowUnroll];
+      for (int c0 = row + 1; c0 < n; c0 += kBracketUnroll * kBracketThreads) {
+        uint32_t uu[kBracketUnroll];
 #pragma unroll
-        for (int k = 0; k < kRowUnroll; ++k) {
-          const int c = c0 + k * kSelThreads + (int)threadIdx.x;
+        for (int k = 0; k < kBracketUnroll; ++k) {
+          const int c = c0 + k * kBracketThreads + (int)threadIdx.x;
           uu[k] = c < n ? vr[c] : 0u;
         }
 #pragma unroll
-        for (int k = 0; k < kRowUnroll; ++k) {
-          const bool valid = c0 + k * kSelThreads + (int)threadIdx.x < n;
+        for (int k = 0; k < kBracketUnroll; ++k) {
+          const bool valid = c0 + k * kBracketThreads + (int)threadIdx.x < n;
           const uint32_t u = uu[k];
           const uint32_t dig = (u >> shift) & 255u;
           wave_multisplit_count(valid && (u & mask) == p0, dig, lh[0], lane);
@@ -1043,7 +1092,7 @@ __global__ __launch_bounds__(kSelThreads) void radix_hist_bracket_kernel(const u
     }
   }
   __syncthreads();
-  for (int b = threadIdx.x; b < 512; b += kSelThreads) {
+  for (int b = threadIdx.x; b < 512; b += kBracketThreads) {
     const uint32_t c = shared_prefix ? lh[0][b & 255] : (&lh[0][0])[b];
     if (c) atomicAdd(&hist[b], c);
   }
@@ -1060,11 +1109,14 @@ __global__ __launch_bounds__(kSelThreads) void bracket_finish_kernel(Bracket* br
   __shared__ uint32_t cur[8];
   __shared__ uint32_t lh[512];
   __shared__ uint32_t sc[kSmallCap];
-  if (br->done) {
+  __shared__ Bracket bs;
+  // the bracket state and the selection's starting state, read once (one round trip)
+  if (threadIdx.x == 0) bs = *br;
+  if (threadIdx.x < 4) cur[threadIdx.x] = sel[threadIdx.x];
+  __syncthreads();
+  if (bs.done) {
     // few candidates: all four digit passes here, in LDS, from st[0]
-    if (threadIdx.x < 4) cur[threadIdx.x] = sel[threadIdx.x];
-    __syncthreads();
-    select_small_block(cand, br->inside, br->lo, br->hi, sc, lh, cur, wtot);
+    select_small_block(cand, bs.inside, bs.lo, bs.hi, sc, lh, cur, wtot);
   } else {
     radix_choose_block(0, sel + 12, sel + kSelStateWords + 3 * 512, cur, wtot);
   }
@@ -1080,17 +1132,17 @@ __global__ __launch_bounds__(kSelThreads) void bracket_finish_kernel(Bracket* br
   // few (aiming at about kTargetCand); the bracket is centred on the linear extrapolation of the last
   // two values (the median drifts smoothly from one SVGD iteration to the next: measured relative
   // extrapolation errors of a few 1e-6, against drifts of about 1e-4 per iteration)
-  float delta = br->delta;
-  const uint32_t inside = br->inside;
-  if (br->overflow || inside > kBracketCap)
+  float delta = bs.delta;
+  const uint32_t inside = bs.inside;
+  if (bs.overflow || inside > kBracketCap)
     delta = fmaxf(delta * 0.5f, 1e-7f);
-  else if (!br->ok)
+  else if (!bs.ok)
     delta = fminf(delta * 4.f, 1.f);
   else if (inside > 2 * kTargetCand)
     delta = fmaxf(delta * 0.7f, 1e-7f);
   else if (inside < kTargetCand / 4)
     delta = fminf(delta * 1.25f, 1.f);
-  const float center = __uint_as_float(cur[1]), prev = br->prev;
+  const float center = __uint_as_float(cur[1]), prev = bs.prev;
   float pred = prev > 0.f ? center + (center - prev) : center;
   if (!(pred > 0.f)) pred = center;
   br->prev = center;
@@ -1136,11 +1188,45 @@ __global__ __launch_bounds__(256) void svgd_update_gmm_kernel(
     key[0] = k.a;
     key[1] = k.b;
   }
+  // the mixture parameters (K * d <= 1024 each), staged once per workgroup
+  __shared__ float smu[8 * 128], slam[8 * 128];
+  {
+    const int kd = K * d;
+    float a[4], b[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int e = u * 256 + (int)threadIdx.x;
+      a[u] = e < kd ? mu[e] : 0.f;
+      b[u] = e < kd ? lam[e] : 0.f;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int e = u * 256 + (int)threadIdx.x;
+      if (e < kd) {
+        smu[e] = a[u];
+        slam[e] = b[u];
+      }
+    }
+  }
+  __syncthreads();
   const int lane = threadIdx.x & (kWave - 1);
   const int64_t i = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
   if (i >= n) return;
   const float hb = *h_dev;
   const float inv_h2 = 1.f / (hb * hb);
+  // the row's coordinates, adagrad state and centring mean (columns lane and lane + 64, d <= 128),
+  // loaded ahead of the partial sums
+  float xv[2], gv[2], mv[2], cv[2];
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    const int k = lane + q * kWave;
+    const bool in = k < d;
+    const int64_t e = i * d + k;
+    xv[q] = in ? x[e] : 0.f;
+    gv[q] = in ? gsq[e] : 0.f;
+    mv[q] = in ? mom[e] : 0.f;
+    cv[q] = in ? mean[k] : 0.f;
+  }
   // the j-split partial sums, added in split order; loads issued 8 splits ahead of the additions
   constexpr int U = 8;
   float kg[2] = {0.f, 0.f}, kx[2] = {0.f, 0.f}, s = 0.f;
@@ -1175,9 +1261,8 @@ __global__ __launch_bounds__(256) void svgd_update_gmm_kernel(
     const int k = lane + q * kWave;
     if (k >= d) continue;
     const int64_t e = i * d + k;
-    const float xv = x[e];
     const AdagradOut u =
-        svgd_adagrad(xv, xv - mean[k], kg[q], kx[q], s, gsq[e], mom[e], inv_n, inv_h2, step, momentum);
+        svgd_adagrad(xv[q], xv[q] - cv[q], kg[q], kx[q], s, gv[q], mv[q], inv_n, inv_h2, step, momentum);
     gsq[e] = u.G;
     mom[e] = u.m;
     xn[q] = u.x;
@@ -1185,14 +1270,16 @@ __global__ __launch_bounds__(256) void svgd_update_gmm_kernel(
   }
   float qk[8];
   float qmax = -INFINITY;
-  for (int c = 0; c < K; ++c) {
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    if (c >= K) break;
     float a = 0.f;
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
       const int j = lane + q * kWave;
       if (j < d) {
-        const float df = xn[q] - mu[c * d + j];
-        a = fmaf(lam[c * d + j] * df, df, a);
+        const float df = xn[q] - smu[c * d + j];
+        a = fmaf(slam[c * d + j] * df, df, a);
       }
     }
     a = wave_sum(a);
@@ -1210,7 +1297,11 @@ __global__ __launch_bounds__(256) void svgd_update_gmm_kernel(
     const int j = lane + q * kWave;
     if (j >= d) continue;
     float gsum = 0.f;
-    for (int c = 0; c < K; ++c) gsum = fmaf(__expf(qk[c] - qmax) / ssum * lam[c * d + j], xn[q] - mu[c * d + j], gsum);
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      if (c >= K) break;
+      gsum = fmaf(__expf(qk[c] - qmax) / ssum * slam[c * d + j], xn[q] - smu[c * d + j], gsum);
+    }
     grad[i * d + j] = T != 1.f ? gsum * T : gsum;
   }
 }
@@ -1243,9 +1334,8 @@ void svgd_prepare_dp(const float* x, const float* grad, int n, int d, void* pack
   double* cpart = reinterpret_cast<double*>(reinterpret_cast<char*>(mean) + 512);
   hipLaunchKernelGGL(colmean_part_kernel, dim3((d + kWave - 1) / kWave, slices), dim3(256), 0, st, x, n, d, slices,
                      cpart);
-  hipLaunchKernelGGL(colmean_final_kernel, dim3((d + kWave - 1) / kWave), dim3(kWave), 0, st, (const double*)cpart, n,
-                     d, slices, mean);
-  hipLaunchKernelGGL(svgd_pack_kernel<DP>, dim3((unsigned)nrb), dim3(256), 0, st, x, grad, mean, n, d, (char*)packed);
+  hipLaunchKernelGGL(svgd_pack_kernel<DP>, dim3((unsigned)nrb), dim3(256), 0, st, x, grad, (const double*)cpart,
+                     slices, mean, n, d, (char*)packed);
 }
 
 // Stein direction from the prepared operands of x_in, then the adagrad step of x
@@ -1518,7 +1608,7 @@ int mocat_svgd_gmm_run(float* x, int64_t n, int64_t d, int K, const float* means
     if (bandwidth_mode == 2) {
       const unsigned rows = (unsigned)std::min<int64_t>(n - 1, kSelectBlocks);
       for (int pass = 0; pass < 4; ++pass)
-        hipLaunchKernelGGL(radix_hist_bracket_kernel, dim3(rows), dim3(kSelThreads), 0, stream, (const uint32_t*)d2,
+        hipLaunchKernelGGL(radix_hist_bracket_kernel, dim3(rows), dim3(kBracketThreads), 0, stream, (const uint32_t*)d2,
                            (int)n, (const uint32_t*)cand, br, pass, sel, (uint32_t)((r0 - n) / 2),
                            (uint32_t)((r1 - n) / 2));
       hipLaunchKernelGGL(bracket_finish_kernel, dim3(1), dim3(kSelThreads), 0, stream, br, (const uint32_t*)cand, sel,
