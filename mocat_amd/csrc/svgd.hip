@@ -16,6 +16,7 @@
 // sqrt(2 log N) (kernels.py:220-229). Squared distances are written once (N x N f32) and the two
 // middle order statistics found exactly by an 8-bit radix select over their bit patterns.
 #include <cmath>
+#include <cstdlib>
 
 #include "common.hpp"
 
@@ -684,6 +685,7 @@ struct Bracket {
   uint32_t overflow;   // some workgroup had more candidates than its LDS buffer
   uint32_t done;       // the selection was finished by one workgroup in LDS (few candidates)
   float prev;          // the previous selected value (0: none yet)
+  uint32_t bar[2];     // grid barrier of the digit passes: arrivals, abort flag
 };
 
 constexpr uint32_t kBracketCap = 1u << 20;  // candidate capacity (4 MB)
@@ -1014,87 +1016,126 @@ MOCAT_DEV void select_small_block(const uint32_t* __restrict__ cand, uint32_t m,
   // four passes: the result is back in st[0..3]
 }
 
-// Radix pass `pass` (digit shift 24 - 8 pass) over the candidates or over all pairs (upper triangle).
-// Pass 0 decides (every workgroup alike, from the finished count pass): on the candidates, with the
-// ranks reduced by the count below, when both ranks lie inside the bracket and the candidates fit, else
-// on all pairs; workgroup 0 records the decision and st[0]. Later passes first apply the digit choice
-// of the previous pass to their state (workgroup 0 records it as st[pass]).
-__global__ __launch_bounds__(kBracketThreads) void radix_hist_bracket_kernel(const uint32_t* __restrict__ v, int n,
-                                                                 const uint32_t* __restrict__ cand, Bracket* br,
-                                                                 int pass, uint32_t* __restrict__ sel,
-                                                                 uint32_t rank0, uint32_t rank1) {
+// Bounded grid-wide barrier of the bracketed passes: every workgroup adds one to the arrival counter
+// bar[0] after a release fence; barrier k is passed once the counter reaches k * gridDim.x, followed by
+// an acquire fence. A workgroup that polls max_polls times without seeing every arrival sets the abort
+// flag bar[1], and every workgroup leaves at its next barrier once the flag is set. The grid (at most
+// kSelectBlocks workgroups of kBracketThreads threads, 2 KB of LDS each) is resident at once, so a
+// barrier only ever waits on running workgroups.
+constexpr uint32_t kBracketBarrierPolls = 1u << 22;
+
+MOCAT_DEV bool bracket_grid_sync(uint32_t* bar, uint32_t k, uint32_t max_polls) {
+  __shared__ int pass_ok;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __hip_atomic_fetch_add(&bar[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint32_t target = k * gridDim.x;
+    int ok = 1;
+    for (uint32_t polls = 0;; ++polls) {
+      if (__hip_atomic_load(&bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) {
+        ok = 0;
+        break;
+      }
+      if (polls >= max_polls) {
+        __hip_atomic_store(&bar[1], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        ok = 0;
+        break;
+      }
+      if (__hip_atomic_load(&bar[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= target) break;
+      __builtin_amdgcn_s_sleep(1);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    pass_ok = ok;
+  }
+  __syncthreads();
+  return pass_ok != 0;
+}
+
+// The four digit passes (shift 24 - 8 pass) of the bracketed selection in one launch, over the
+// candidates or over all pairs (upper triangle), with a grid barrier before each later pass. The
+// decision (every workgroup alike, from the finished count pass): on the candidates, with the ranks
+// reduced by the count below, when both ranks lie inside the bracket and the candidates fit, else on
+// all pairs. With few candidates every workgroup returns at once and the finishing kernel selects them
+// in LDS (the usual case, where this launch costs only its start). Each workgroup applies a finished
+// pass's digit choice itself, from the pass's global histogram; workgroup 0 records the decision and
+// the state of every pass (sel[4 pass ..]) for the finishing kernel.
+__global__ __launch_bounds__(kBracketThreads) void radix_bracket_kernel(const uint32_t* __restrict__ v, int n,
+                                                                  const uint32_t* __restrict__ cand, Bracket* br,
+                                                                  uint32_t* __restrict__ sel, uint32_t rank0,
+                                                                  uint32_t rank1, uint32_t max_polls) {
   __shared__ uint32_t lh[2][256];
   __shared__ uint32_t wtot[2][4];
-  __shared__ uint32_t cur[4];
-  const int shift = 24 - 8 * pass;
-  uint32_t* hist = sel + kSelStateWords + pass * 512;
-  bool use_cand;
-  if (pass > 0 && br->done) return;  // few candidates: the finishing kernel selects them
-  if (pass == 0) {
-    const uint32_t below = br->below, inside = br->inside;
-    use_cand = !br->overflow && below <= rank0 && (uint64_t)rank1 < (uint64_t)below + inside &&
-               inside <= kBracketCap;
-    const bool small = use_cand && inside <= kSmallCap;
-    if (threadIdx.x == 0) {
-      cur[0] = 0u;
-      cur[1] = 0u;
-      cur[2] = use_cand ? rank0 - below : rank0;
-      cur[3] = use_cand ? rank1 - below : rank1;
-    }
-    __syncthreads();
-    if (blockIdx.x == 0 && threadIdx.x < 4) sel[threadIdx.x] = cur[threadIdx.x];
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-      br->ok = use_cand ? 1u : 0u;
-      br->done = small ? 1u : 0u;
-    }
-    if (small) return;
-  } else {
-    use_cand = br->ok != 0u;
-    radix_choose_block(shift + 8, sel + 4 * (pass - 1), hist - 512, cur, wtot);
-    if (blockIdx.x == 0 && threadIdx.x < 4) sel[4 * pass + threadIdx.x] = cur[threadIdx.x];
+  __shared__ uint32_t st[2][4];
+  const uint32_t below = br->below, inside = br->inside;
+  const bool use_cand = !br->overflow && below <= rank0 && (uint64_t)rank1 < (uint64_t)below + inside &&
+                        inside <= kBracketCap;
+  const bool small = use_cand && inside <= kSmallCap;
+  if (threadIdx.x == 0) {
+    st[0][0] = 0u;
+    st[0][1] = 0u;
+    st[0][2] = use_cand ? rank0 - below : rank0;
+    st[0][3] = use_cand ? rank1 - below : rank1;
   }
-  const uint32_t mask = shift == 24 ? 0u : (0xFFFFFFFFu << (shift + 8));
-  const uint32_t p0 = cur[0] & mask, p1 = cur[1] & mask;
-  const bool shared_prefix = p0 == p1;
-  const int lane = threadIdx.x & (kWave - 1);
-  for (int b = threadIdx.x; b < 512; b += kBracketThreads) (&lh[0][0])[b] = 0u;
   __syncthreads();
-  if (use_cand) {
-    const int64_t m = br->inside;
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t base = blockIdx.x * (int64_t)blockDim.x; base < m; base += stride) {
-      const int64_t e = base + threadIdx.x;
-      const bool valid = e < m;
-      const uint32_t u = valid ? cand[e] : 0u;
-      const uint32_t dig = (u >> shift) & 255u;
-      wave_multisplit_count(valid && (u & mask) == p0, dig, lh[0], lane);
-      if (!shared_prefix) wave_multisplit_count(valid && (u & mask) == p1, dig, lh[1], lane);
+  if (blockIdx.x == 0 && threadIdx.x < 4) sel[threadIdx.x] = st[0][threadIdx.x];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    br->ok = use_cand ? 1u : 0u;
+    br->done = small ? 1u : 0u;
+  }
+  if (small) return;
+  const int lane = threadIdx.x & (kWave - 1);
+  for (int pass = 0; pass < 4; ++pass) {
+    const int shift = 24 - 8 * pass;
+    uint32_t* hist = sel + kSelStateWords + pass * 512;
+    uint32_t* cur = st[pass & 1];
+    if (pass > 0) {
+      if (!bracket_grid_sync(br->bar, (uint32_t)pass, max_polls)) return;
+      radix_choose_block(shift + 8, st[(pass - 1) & 1], hist - 512, cur, wtot);
+      if (blockIdx.x == 0 && threadIdx.x < 4) sel[4 * pass + threadIdx.x] = cur[threadIdx.x];
     }
-  } else {
-    for (int row = blockIdx.x; row < n - 1; row += gridDim.x) {
-      const uint32_t* vr = v + (int64_t)row * n;
-      for (int c0 = row + 1; c0 < n; c0 += kBracketUnroll * kBracketThreads) {
-        uint32_t uu[kBracketUnroll];
+    const uint32_t mask = shift == 24 ? 0u : (0xFFFFFFFFu << (shift + 8));
+    const uint32_t p0 = cur[0] & mask, p1 = cur[1] & mask;
+    const bool shared_prefix = p0 == p1;
+    for (int b = threadIdx.x; b < 512; b += kBracketThreads) (&lh[0][0])[b] = 0u;
+    __syncthreads();
+    if (use_cand) {
+      const int64_t m = inside;
+      const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+      for (int64_t base = blockIdx.x * (int64_t)blockDim.x; base < m; base += stride) {
+        const int64_t e = base + threadIdx.x;
+        const bool valid = e < m;
+        const uint32_t u = valid ? cand[e] : 0u;
+        const uint32_t dig = (u >> shift) & 255u;
+        wave_multisplit_count(valid && (u & mask) == p0, dig, lh[0], lane);
+        if (!shared_prefix) wave_multisplit_count(valid && (u & mask) == p1, dig, lh[1], lane);
+      }
+    } else {
+      for (int row = blockIdx.x; row < n - 1; row += gridDim.x) {
+        const uint32_t* vr = v + (int64_t)row * n;
+        for (int c0 = row + 1; c0 < n; c0 += kBracketUnroll * kBracketThreads) {
+          uint32_t uu[kBracketUnroll];
 #pragma unroll
-        for (int k = 0; k < kBracketUnroll; ++k) {
-          const int c = c0 + k * kBracketThreads + (int)threadIdx.x;
-          uu[k] = c < n ? vr[c] : 0u;
-        }
+          for (int k = 0; k < kBracketUnroll; ++k) {
+            const int c = c0 + k * kBracketThreads + (int)threadIdx.x;
+            uu[k] = c < n ? vr[c] : 0u;
+          }
 #pragma unroll
-        for (int k = 0; k < kBracketUnroll; ++k) {
-          const bool valid = c0 + k * kBracketThreads + (int)threadIdx.x < n;
-          const uint32_t u = uu[k];
-          const uint32_t dig = (u >> shift) & 255u;
-          wave_multisplit_count(valid && (u & mask) == p0, dig, lh[0], lane);
-          if (!shared_prefix) wave_multisplit_count(valid && (u & mask) == p1, dig, lh[1], lane);
+          for (int k = 0; k < kBracketUnroll; ++k) {
+            const bool valid = c0 + k * kBracketThreads + (int)threadIdx.x < n;
+            const uint32_t u = uu[k];
+            const uint32_t dig = (u >> shift) & 255u;
+            wave_multisplit_count(valid && (u & mask) == p0, dig, lh[0], lane);
+            if (!shared_prefix) wave_multisplit_count(valid && (u & mask) == p1, dig, lh[1], lane);
+          }
         }
       }
     }
-  }
-  __syncthreads();
-  for (int b = threadIdx.x; b < 512; b += kBracketThreads) {
-    const uint32_t c = shared_prefix ? lh[0][b & 255] : (&lh[0][0])[b];
-    if (c) atomicAdd(&hist[b], c);
+    __syncthreads();
+    for (int b = threadIdx.x; b < 512; b += kBracketThreads) {
+      const uint32_t c = shared_prefix ? lh[0][b & 255] : (&lh[0][0])[b];
+      if (c) atomicAdd(&hist[b], c);
+    }
   }
 }
 
@@ -1125,7 +1166,10 @@ __global__ __launch_bounds__(kSelThreads) void bracket_finish_kernel(Bracket* br
   for (int t = 0; t < 4; ++t) sel[16 + t] = cur[t];
   br->done = 0u;
   const double v0 = (double)__uint_as_float(cur[0]), v1 = (double)__uint_as_float(cur[1]);
-  const float hv = (float)(0.5 * (sqrt(v0) + sqrt(v1)) / norm);
+  // a timed-out barrier of the digit passes leaves no selection: the bandwidth is NaN
+  const float hv = bs.bar[1] ? __builtin_nanf("") : (float)(0.5 * (sqrt(v0) + sqrt(v1)) / norm);
+  br->bar[0] = 0u;
+  br->bar[1] = 0u;
   *h = hv;
   if (trace) *trace = hv;
   // half-width: narrower when the candidates overflowed or are many, wider after a miss or when very
@@ -1168,6 +1212,8 @@ __global__ __launch_bounds__(256) void bracket_init_kernel(Bracket* br, uint32_t
     br->done = 0u;
     br->prev = 0.f;
     br->delta = 2e-3f;
+    br->bar[0] = 0u;
+    br->bar[1] = 0u;
   }
 }
 
@@ -1562,6 +1608,10 @@ int mocat_svgd_gmm_run(float* x, int64_t n, int64_t d, int K, const float* means
   Bracket* br = reinterpret_cast<Bracket*>(ws + o_br);
   uint32_t* cand = reinterpret_cast<uint32_t*>(ws + o_br + 256);
   if (bandwidth_mode == 2) hipLaunchKernelGGL(bracket_init_kernel, dim3(1), dim3(256), 0, stream, br, sel);
+  // the barrier wait limit of the digit passes (the MOCAT_SVGD_BARRIER_POLLS environment variable sets
+  // another; 0 makes every barrier time out, which tests use to check the NaN bandwidth)
+  uint32_t polls = kBracketBarrierPolls;
+  if (const char* e = std::getenv("MOCAT_SVGD_BARRIER_POLLS")) polls = (uint32_t)std::strtoul(e, nullptr, 10);
   const double norm = std::sqrt(2.0 * std::log((double)n));
   const int64_t m = n * n;
   const int64_t r0 = (m - 1) / 2, r1 = m / 2;  // both >= n for n >= 3
@@ -1607,10 +1657,9 @@ int mocat_svgd_gmm_run(float* x, int64_t n, int64_t d, int K, const float* means
       sqdist_packed_dp<128>(ws, (int)n, 1, d2, count_br, cand, stream);
     if (bandwidth_mode == 2) {
       const unsigned rows = (unsigned)std::min<int64_t>(n - 1, kSelectBlocks);
-      for (int pass = 0; pass < 4; ++pass)
-        hipLaunchKernelGGL(radix_hist_bracket_kernel, dim3(rows), dim3(kBracketThreads), 0, stream, (const uint32_t*)d2,
-                           (int)n, (const uint32_t*)cand, br, pass, sel, (uint32_t)((r0 - n) / 2),
-                           (uint32_t)((r1 - n) / 2));
+      hipLaunchKernelGGL(radix_bracket_kernel, dim3(rows), dim3(kBracketThreads), 0, stream, (const uint32_t*)d2,
+                         (int)n, (const uint32_t*)cand, br, sel, (uint32_t)((r0 - n) / 2), (uint32_t)((r1 - n) / 2),
+                         polls);
       hipLaunchKernelGGL(bracket_finish_kernel, dim3(1), dim3(kSelThreads), 0, stream, br, (const uint32_t*)cand, sel,
                          norm, bandwidth, tr);
     } else {

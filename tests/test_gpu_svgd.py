@@ -279,3 +279,21 @@ def test_fused_median_bracket_exact_over_many_iterations(m):
     ref = m.run(sc, m.SVGD(**kw), n, m.random.PRNGKey(1), stack=("value",))
     np.testing.assert_array_equal(fused.bandwidth.cpu().numpy(), ref.bandwidth.cpu().numpy())
     np.testing.assert_array_equal(fused.final_state.value.cpu().numpy(), ref.final_state.value.cpu().numpy())
+
+
+def test_fused_median_barrier_time_out_gives_nan_bandwidth(m, monkeypatch):
+    """The digit passes' grid barrier with a zero wait limit, forced from the host: the first selection
+    runs over all pairs, every barrier times out and the bandwidth is NaN. Without the limit the same
+    run gives the stepwise bandwidths again (the barrier state is reset by the finishing kernel)."""
+    rs = np.random.default_rng(11)
+    d, n, iters = 20, 3000, 3
+    sc = m.GaussianMixture(rs.normal(0, 1, (3, d)), rs.uniform(0.5, 2, (3, d)), rs.dirichlet(np.ones(3)))
+    kw = dict(stepsize=0.01, max_iter=iters, bandwidth_update="median")
+    monkeypatch.setenv("MOCAT_SVGD_BARRIER_POLLS", "0")
+    bad = m.run(sc, m.SVGD(**kw), n, m.random.PRNGKey(3))
+    bws = bad.bandwidth.cpu().numpy()  # the starting bandwidth, then one per iteration
+    assert np.isfinite(bws[0]) and np.isnan(bws[1])
+    monkeypatch.delenv("MOCAT_SVGD_BARRIER_POLLS")
+    fused = m.run(sc, m.SVGD(**kw), n, m.random.PRNGKey(3))
+    ref = m.run(sc, m.SVGD(**kw), n, m.random.PRNGKey(3), stack=("value",))
+    np.testing.assert_array_equal(fused.bandwidth.cpu().numpy(), ref.bandwidth.cpu().numpy())
