@@ -360,7 +360,10 @@ int mocat_sqrt_sum_upper(const float* d2, int64_t n, double* partial, int n_part
  * at the new particles (times temperature), then *bandwidth <- mean (bandwidth_mode 1) or median
  * (2) update, or unchanged (0). In/out: x, gsq, mom, potential, grad [n][d] / [n], key[2],
  * *bandwidth (device float). bandwidth_trace (optional) [n_iter]: the bandwidth after each
- * iteration. No host synchronisation. workspace: mocat_svgd_gmm_run_workspace_bytes. */
+ * iteration. No host synchronisation. workspace: mocat_svgd_gmm_run_workspace_bytes. The median's
+ * full digit passes (the first selection, and any whose bracket misses) run in one launch with a
+ * bounded grid barrier; a barrier that times out (the MOCAT_SVGD_BARRIER_POLLS environment variable
+ * lowers the wait limit; 0 forces it) makes that iteration's bandwidth NaN. */
 int mocat_svgd_gmm_run_workspace_bytes(int64_t n, int64_t d, int js_count, int64_t* bytes);
 int mocat_svgd_gmm_run(float* x, int64_t n, int64_t d, int K, const float* means, const float* lam, const float* logc,
                        float temperature, float step, float momentum, int bandwidth_mode, float* bandwidth,
