@@ -293,10 +293,10 @@ constexpr int kRowUnroll = 4;  // row elements loaded per thread before they are
 // histogram bins, which serialise across workgroups, so one workgroup per compute unit.
 constexpr int kSelectBlocks = 256;
 constexpr int kSelThreads = 1024;  // 16 waves per compute unit to cover the load latency
-// The bracketed passes run at 256 threads per workgroup: on the usual path (few candidates, selected by
-// the finishing kernel) they return at once, and their cost is the launch of the grid; a pass over all
-// pairs keeps 16 row loads in flight per thread instead.
-constexpr int kBracketThreads = 256, kBracketUnroll = 16;
+// The bracketed selection: 512 threads per workgroup, so that with its grid barriers the whole grid
+// (kSelectBlocks workgroups, three per compute unit) is resident with room to spare; eight row loads
+// in flight per thread on a pass over all pairs.
+constexpr int kBracketThreads = 512, kBracketUnroll = 8;
 
 // Lanes of a wave that share an 8-bit digit (ballots of the digit's bits); the lowest lane of each
 // group adds the group's size to h[digit]: one LDS atomic per distinct digit of the wave, with no
@@ -680,10 +680,8 @@ struct Bracket {
   uint32_t lo, hi;     // bit patterns of non-negative floats (ordered as unsigned ints)
   uint32_t below;      // pairs < lo
   uint32_t inside;     // pairs in [lo, hi) (the candidate count, possibly above the capacity)
-  uint32_t ok;         // the last selection ran on the candidates
   float delta;         // relative half-width of the next bracket
   uint32_t overflow;   // some workgroup had more candidates than its LDS buffer
-  uint32_t done;       // the selection was finished by one workgroup in LDS (few candidates)
   float prev;          // the previous selected value (0: none yet)
   uint32_t bar[2];     // grid barrier of the digit passes: arrivals, abort flag
 };
@@ -959,28 +957,16 @@ MOCAT_DEV void radix_choose_block(int shift, const uint32_t* __restrict__ st_in,
   __syncthreads();
 }
 
-// Both order statistics of the m <= kSmallCap bracket candidates by one workgroup: the candidates
-// staged in LDS, then the four 8-bit digit passes with LDS histograms (wave multisplit counts) and the
-// same digit choice as the multi-workgroup passes. Digits inside the common prefix of lo and hi - 1
-// are shared by every candidate (they lie in [lo, hi)) and are taken without a scan. st (LDS, 2 x 4
-// words) starts as {0, 0, rank0, rank1} in st[0..3]; the result (the two bit patterns, then the
-// remaining ranks) ends in st[0..3]. Every thread of the workgroup calls this.
+// Both order statistics of the m <= kSmallCap bracket candidates by one workgroup: the four 8-bit digit
+// passes over the candidates (read from global memory in every pass, eight loads in flight per thread;
+// they stay in L2) with LDS histograms and the same digit choice as the multi-workgroup passes. One LDS
+// atomic per candidate: on one compute unit a wave multisplit's ballots cost more vector issue than
+// the same-bin conflicts they avoid. Digits inside the common prefix of lo and hi - 1 are shared by
+// every candidate (they lie in [lo, hi)) and are taken without a scan. st (LDS, 2 x 4 words) starts as
+// {0, 0, rank0, rank1} in st[0..3]; the result (the two bit patterns, then the remaining ranks) ends in
+// st[0..3]. Every thread of the workgroup calls this.
 MOCAT_DEV void select_small_block(const uint32_t* __restrict__ cand, uint32_t m, uint32_t lo, uint32_t hi,
-                                  uint32_t* sc, uint32_t* lh, uint32_t* st, uint32_t (*wtot)[4]) {
-  // staged with eight loads in flight per thread
-  for (uint32_t k0 = 0; k0 < m; k0 += 8 * blockDim.x) {
-    uint32_t u[8];
-#pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      const uint32_t k = k0 + q * blockDim.x + threadIdx.x;
-      u[q] = k < m ? cand[k] : 0u;
-    }
-#pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      const uint32_t k = k0 + q * blockDim.x + threadIdx.x;
-      if (k < m) sc[k] = u[q];
-    }
-  }
+                                  uint32_t* lh, uint32_t* st, uint32_t (*wtot)[4]) {
   const uint32_t diff = lo ^ (hi - 1u);
   const int common = diff ? __clz(diff) : 32;  // leading bits shared by every candidate
   for (int pass = 0; pass < 4; ++pass) {
@@ -999,12 +985,22 @@ MOCAT_DEV void select_small_block(const uint32_t* __restrict__ cand, uint32_t m,
     __syncthreads();
     const uint32_t p0 = in[0] & mask, p1 = in[1] & mask;
     const bool shared_prefix = p0 == p1;
-    // one LDS atomic per candidate: the whole selection runs on one compute unit, where the
-    // multisplit's ballots would cost more vector issue than the (rare) same-bin conflicts they avoid
-    for (uint32_t k = threadIdx.x; k < m; k += blockDim.x) {
-      const uint32_t u = sc[k], dig = (u >> shift) & 255u;
-      if ((u & mask) == p0) atomicAdd(&lh[dig], 1u);
-      if (!shared_prefix && (u & mask) == p1) atomicAdd(&lh[256 + dig], 1u);
+    for (uint32_t k0 = 0; k0 < m; k0 += 8 * blockDim.x) {
+      uint32_t u[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const uint32_t k = k0 + q * blockDim.x + threadIdx.x;
+        u[q] = k < m ? cand[k] : 0u;
+      }
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const uint32_t k = k0 + q * blockDim.x + threadIdx.x;
+        if (k < m) {
+          const uint32_t dig = (u[q] >> shift) & 255u;
+          if ((u[q] & mask) == p0) atomicAdd(&lh[dig], 1u);
+          if (!shared_prefix && (u[q] & mask) == p1) atomicAdd(&lh[256 + dig], 1u);
+        }
+      }
     }
     __syncthreads();
     if (shared_prefix) {
@@ -1052,23 +1048,32 @@ MOCAT_DEV bool bracket_grid_sync(uint32_t* bar, uint32_t k, uint32_t max_polls) 
   return pass_ok != 0;
 }
 
-// The four digit passes (shift 24 - 8 pass) of the bracketed selection in one launch, over the
-// candidates or over all pairs (upper triangle), with a grid barrier before each later pass. The
-// decision (every workgroup alike, from the finished count pass): on the candidates, with the ranks
-// reduced by the count below, when both ranks lie inside the bracket and the candidates fit, else on
-// all pairs. With few candidates every workgroup returns at once and the finishing kernel selects them
-// in LDS (the usual case, where this launch costs only its start). Each workgroup applies a finished
-// pass's digit choice itself, from the pass's global histogram; workgroup 0 records the decision and
-// the state of every pass (sel[4 pass ..]) for the finishing kernel.
-__global__ __launch_bounds__(kBracketThreads) void radix_bracket_kernel(const uint32_t* __restrict__ v, int n,
-                                                                  const uint32_t* __restrict__ cand, Bracket* br,
-                                                                  uint32_t* __restrict__ sel, uint32_t rank0,
-                                                                  uint32_t rank1, uint32_t max_polls) {
+// The bracketed selection of the two middle order statistics in one launch, then the bandwidth and
+// the next bracket. The decision (every workgroup alike, from the finished count pass): on the
+// candidates, with the ranks reduced by the count below, when both ranks lie inside the bracket and the
+// candidates fit, else on all pairs (upper triangle). With at most kSmallCap candidates (the usual
+// case) workgroup 0 selects them alone and every other workgroup returns at once; otherwise the four
+// digit passes (shift 24 - 8 pass) run over the whole grid (at most kSelectBlocks workgroups, all
+// resident), with a grid barrier before each later pass and one after the last, each workgroup applying
+// a finished pass's digit choice itself from the pass's global histogram. Workgroup 0 then stores the
+// median bandwidth (FP64 square roots of the two order statistics, their mean, divided by
+// sqrt(2 log n) computed on the host, rounded to f32; NaN when a barrier timed out) and the next bracket.
+// The bracket counters, the barrier and the histograms are cleared by the next iteration's update
+// kernel (svgd_update_gmm_kernel), which runs after every workgroup of this launch has ended: a
+// workgroup of this launch may still be reading them when workgroup 0 finishes.
+__global__ __launch_bounds__(kBracketThreads) void bracket_select_kernel(const uint32_t* __restrict__ v, int n,
+                                                                     const uint32_t* __restrict__ cand, Bracket* br,
+                                                                     uint32_t* __restrict__ sel, uint32_t rank0,
+                                                                     uint32_t rank1, double norm, float* h,
+                                                                     float* trace, uint32_t max_polls) {
   __shared__ uint32_t lh[2][256];
   __shared__ uint32_t wtot[2][4];
   __shared__ uint32_t st[2][4];
-  const uint32_t below = br->below, inside = br->inside;
-  const bool use_cand = !br->overflow && below <= rank0 && (uint64_t)rank1 < (uint64_t)below + inside &&
+  __shared__ Bracket bs;
+  if (threadIdx.x == 0) bs = *br;
+  __syncthreads();
+  const uint32_t below = bs.below, inside = bs.inside;
+  const bool use_cand = !bs.overflow && below <= rank0 && (uint64_t)rank1 < (uint64_t)below + inside &&
                         inside <= kBracketCap;
   const bool small = use_cand && inside <= kSmallCap;
   if (threadIdx.x == 0) {
@@ -1078,98 +1083,75 @@ __global__ __launch_bounds__(kBracketThreads) void radix_bracket_kernel(const ui
     st[0][3] = use_cand ? rank1 - below : rank1;
   }
   __syncthreads();
-  if (blockIdx.x == 0 && threadIdx.x < 4) sel[threadIdx.x] = st[0][threadIdx.x];
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    br->ok = use_cand ? 1u : 0u;
-    br->done = small ? 1u : 0u;
-  }
-  if (small) return;
-  const int lane = threadIdx.x & (kWave - 1);
-  for (int pass = 0; pass < 4; ++pass) {
-    const int shift = 24 - 8 * pass;
-    uint32_t* hist = sel + kSelStateWords + pass * 512;
-    uint32_t* cur = st[pass & 1];
-    if (pass > 0) {
-      if (!bracket_grid_sync(br->bar, (uint32_t)pass, max_polls)) return;
-      radix_choose_block(shift + 8, st[(pass - 1) & 1], hist - 512, cur, wtot);
-      if (blockIdx.x == 0 && threadIdx.x < 4) sel[4 * pass + threadIdx.x] = cur[threadIdx.x];
-    }
-    const uint32_t mask = shift == 24 ? 0u : (0xFFFFFFFFu << (shift + 8));
-    const uint32_t p0 = cur[0] & mask, p1 = cur[1] & mask;
-    const bool shared_prefix = p0 == p1;
-    for (int b = threadIdx.x; b < 512; b += kBracketThreads) (&lh[0][0])[b] = 0u;
-    __syncthreads();
-    if (use_cand) {
-      const int64_t m = inside;
-      const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-      for (int64_t base = blockIdx.x * (int64_t)blockDim.x; base < m; base += stride) {
-        const int64_t e = base + threadIdx.x;
-        const bool valid = e < m;
-        const uint32_t u = valid ? cand[e] : 0u;
-        const uint32_t dig = (u >> shift) & 255u;
-        wave_multisplit_count(valid && (u & mask) == p0, dig, lh[0], lane);
-        if (!shared_prefix) wave_multisplit_count(valid && (u & mask) == p1, dig, lh[1], lane);
+  bool ok = true;
+  if (small) {
+    if (blockIdx.x != 0) return;
+    select_small_block(cand, inside, bs.lo, bs.hi, &lh[0][0], &st[0][0], wtot);  // result in st[0]
+  } else {
+    const int lane = threadIdx.x & (kWave - 1);
+    for (int pass = 0; pass < 4 && ok; ++pass) {
+      const int shift = 24 - 8 * pass;
+      uint32_t* hist = sel + kSelStateWords + pass * 512;
+      uint32_t* cur = st[pass & 1];
+      if (pass > 0) {
+        if (!bracket_grid_sync(br->bar, (uint32_t)pass, max_polls)) {
+          ok = false;
+          break;
+        }
+        radix_choose_block(shift + 8, st[(pass - 1) & 1], hist - 512, cur, wtot);
       }
-    } else {
-      for (int row = blockIdx.x; row < n - 1; row += gridDim.x) {
-        const uint32_t* vr = v + (int64_t)row * n;
-        for (int c0 = row + 1; c0 < n; c0 += kBracketUnroll * kBracketThreads) {
-          uint32_t uu[kBracketUnroll];
+      const uint32_t mask = shift == 24 ? 0u : (0xFFFFFFFFu << (shift + 8));
+      const uint32_t p0 = cur[0] & mask, p1 = cur[1] & mask;
+      const bool shared_prefix = p0 == p1;
+      for (int b = threadIdx.x; b < 512; b += kBracketThreads) (&lh[0][0])[b] = 0u;
+      __syncthreads();
+      if (use_cand) {
+        const int64_t m = inside;
+        const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+        for (int64_t base = blockIdx.x * (int64_t)blockDim.x; base < m; base += stride) {
+          const int64_t e = base + threadIdx.x;
+          const bool valid = e < m;
+          const uint32_t u = valid ? cand[e] : 0u;
+          const uint32_t dig = (u >> shift) & 255u;
+          wave_multisplit_count(valid && (u & mask) == p0, dig, lh[0], lane);
+          if (!shared_prefix) wave_multisplit_count(valid && (u & mask) == p1, dig, lh[1], lane);
+        }
+      } else {
+        for (int row = blockIdx.x; row < n - 1; row += gridDim.x) {
+          const uint32_t* vr = v + (int64_t)row * n;
+          for (int c0 = row + 1; c0 < n; c0 += kBracketUnroll * kBracketThreads) {
+            uint32_t uu[kBracketUnroll];
 #pragma unroll
-          for (int k = 0; k < kBracketUnroll; ++k) {
-            const int c = c0 + k * kBracketThreads + (int)threadIdx.x;
-            uu[k] = c < n ? vr[c] : 0u;
-          }
+            for (int k = 0; k < kBracketUnroll; ++k) {
+              const int c = c0 + k * kBracketThreads + (int)threadIdx.x;
+              uu[k] = c < n ? vr[c] : 0u;
+            }
 #pragma unroll
-          for (int k = 0; k < kBracketUnroll; ++k) {
-            const bool valid = c0 + k * kBracketThreads + (int)threadIdx.x < n;
-            const uint32_t u = uu[k];
-            const uint32_t dig = (u >> shift) & 255u;
-            wave_multisplit_count(valid && (u & mask) == p0, dig, lh[0], lane);
-            if (!shared_prefix) wave_multisplit_count(valid && (u & mask) == p1, dig, lh[1], lane);
+            for (int k = 0; k < kBracketUnroll; ++k) {
+              const bool valid = c0 + k * kBracketThreads + (int)threadIdx.x < n;
+              const uint32_t u = uu[k];
+              const uint32_t dig = (u >> shift) & 255u;
+              wave_multisplit_count(valid && (u & mask) == p0, dig, lh[0], lane);
+              if (!shared_prefix) wave_multisplit_count(valid && (u & mask) == p1, dig, lh[1], lane);
+            }
           }
         }
       }
+      __syncthreads();
+      for (int b = threadIdx.x; b < 512; b += kBracketThreads) {
+        const uint32_t c = shared_prefix ? lh[0][b & 255] : (&lh[0][0])[b];
+        if (c) atomicAdd(&hist[b], c);
+      }
     }
-    __syncthreads();
-    for (int b = threadIdx.x; b < 512; b += kBracketThreads) {
-      const uint32_t c = shared_prefix ? lh[0][b & 255] : (&lh[0][0])[b];
-      if (c) atomicAdd(&hist[b], c);
-    }
+    if (ok && !bracket_grid_sync(br->bar, 4u, max_polls)) ok = false;
+    if (blockIdx.x != 0) return;
+    // the last digit choice: the two order statistics in st[0]
+    if (ok) radix_choose_block(0, st[1], sel + kSelStateWords + 3 * 512, st[0], wtot);
   }
-}
-
-// End of a bracketed selection: the last digit choice (st[4] = the two order statistics), the median
-// bandwidth from them (FP64 square roots, their mean, divided by sqrt(2 log n) computed on the host,
-// rounded to f32), the next bracket around the upper one, and the counters and histograms cleared for
-// the next selection.
-__global__ __launch_bounds__(kSelThreads) void bracket_finish_kernel(Bracket* br, const uint32_t* __restrict__ cand,
-                                                                     uint32_t* __restrict__ sel, double norm, float* h,
-                                                                     float* trace) {
-  __shared__ uint32_t wtot[2][4];
-  __shared__ uint32_t cur[8];
-  __shared__ uint32_t lh[512];
-  __shared__ uint32_t sc[kSmallCap];
-  __shared__ Bracket bs;
-  // the bracket state and the selection's starting state, read once (one round trip)
-  if (threadIdx.x == 0) bs = *br;
-  if (threadIdx.x < 4) cur[threadIdx.x] = sel[threadIdx.x];
-  __syncthreads();
-  if (bs.done) {
-    // few candidates: all four digit passes here, in LDS, from st[0]
-    select_small_block(cand, bs.inside, bs.lo, bs.hi, sc, lh, cur, wtot);
-  } else {
-    radix_choose_block(0, sel + 12, sel + kSelStateWords + 3 * 512, cur, wtot);
-  }
-  for (int b = threadIdx.x; b < 4 * 512; b += blockDim.x) sel[kSelStateWords + b] = 0u;
   if (threadIdx.x != 0) return;
-  for (int t = 0; t < 4; ++t) sel[16 + t] = cur[t];
-  br->done = 0u;
-  const double v0 = (double)__uint_as_float(cur[0]), v1 = (double)__uint_as_float(cur[1]);
-  // a timed-out barrier of the digit passes leaves no selection: the bandwidth is NaN
-  const float hv = bs.bar[1] ? __builtin_nanf("") : (float)(0.5 * (sqrt(v0) + sqrt(v1)) / norm);
-  br->bar[0] = 0u;
-  br->bar[1] = 0u;
+  for (int t = 0; t < 4; ++t) sel[16 + t] = st[0][t];
+  const double v0 = (double)__uint_as_float(st[0][0]), v1 = (double)__uint_as_float(st[0][1]);
+  const float hv = ok ? (float)(0.5 * (sqrt(v0) + sqrt(v1)) / norm) : __builtin_nanf("");
   *h = hv;
   if (trace) *trace = hv;
   // half-width: narrower when the candidates overflowed or are many, wider after a miss or when very
@@ -1177,16 +1159,15 @@ __global__ __launch_bounds__(kSelThreads) void bracket_finish_kernel(Bracket* br
   // two values (the median drifts smoothly from one SVGD iteration to the next: measured relative
   // extrapolation errors of a few 1e-6, against drifts of about 1e-4 per iteration)
   float delta = bs.delta;
-  const uint32_t inside = bs.inside;
   if (bs.overflow || inside > kBracketCap)
     delta = fmaxf(delta * 0.5f, 1e-7f);
-  else if (!bs.ok)
+  else if (!use_cand)
     delta = fminf(delta * 4.f, 1.f);
   else if (inside > 2 * kTargetCand)
     delta = fmaxf(delta * 0.7f, 1e-7f);
   else if (inside < kTargetCand / 4)
     delta = fminf(delta * 1.25f, 1.f);
-  const float center = __uint_as_float(cur[1]), prev = bs.prev;
+  const float center = ok ? __uint_as_float(st[0][1]) : bs.prev, prev = bs.prev;
   float pred = prev > 0.f ? center + (center - prev) : center;
   if (!(pred > 0.f)) pred = center;
   br->prev = center;
@@ -1194,9 +1175,19 @@ __global__ __launch_bounds__(kSelThreads) void bracket_finish_kernel(Bracket* br
   br->lo = __float_as_uint(fmaxf(lo, 0.f));
   br->hi = hi > pred ? __float_as_uint(hi) : 0x7F800001u;  // above every finite value
   br->delta = delta;
-  br->below = 0u;
-  br->inside = 0u;
-  br->overflow = 0u;
+}
+
+// Clears the bracket counters, the barrier and the selection histograms for the next selection (by
+// workgroup 0 of the update kernel, which runs between one selection and the next count pass).
+MOCAT_DEV void bracket_clear_block(Bracket* br, uint32_t* sel) {
+  for (int b = threadIdx.x; b < kSelWords - kSelStateWords; b += blockDim.x) sel[kSelStateWords + b] = 0u;
+  if (threadIdx.x == 0) {
+    br->below = 0u;
+    br->inside = 0u;
+    br->overflow = 0u;
+    br->bar[0] = 0u;
+    br->bar[1] = 0u;
+  }
 }
 
 // First bracket: empty (no candidates: the first selection runs over all pairs); histograms cleared.
@@ -1207,9 +1198,7 @@ __global__ __launch_bounds__(256) void bracket_init_kernel(Bracket* br, uint32_t
     br->hi = 0u;
     br->below = 0u;
     br->inside = 0u;
-    br->ok = 0u;
     br->overflow = 0u;
-    br->done = 0u;
     br->prev = 0.f;
     br->delta = 2e-3f;
     br->bar[0] = 0u;
@@ -1221,19 +1210,20 @@ __global__ __launch_bounds__(256) void bracket_init_kernel(Bracket* br, uint32_t
 // The adagrad step of svgd_update_kernel and the mixture potential and gradient of gmm_kernel (times
 // the temperature T) in one pass, one wave per particle: the same operations in the same order, on
 // the updated row held in registers. Workgroup 0 also advances the key: key <- split(key, num)[idx]
-// (SVGD.update's key advance).
+// (SVGD.update's key advance), and clears the median's bracket state when br is given.
 __global__ __launch_bounds__(256) void svgd_update_gmm_kernel(
     const float* __restrict__ part_o, const float* __restrict__ part_s, int js_count, int n, int d, int DP,
     const float* __restrict__ mean, float inv_n, const float* __restrict__ h_dev, float step, float momentum,
     float* __restrict__ x, float* __restrict__ gsq, float* __restrict__ mom, int K, const float* __restrict__ mu,
     const float* __restrict__ lam, const float* __restrict__ logc, float T, float* __restrict__ pot,
-    float* __restrict__ grad, uint32_t* key, uint32_t knum, uint32_t kidx) {
+    float* __restrict__ grad, uint32_t* key, uint32_t knum, uint32_t kidx, Bracket* br, uint32_t* sel) {
 #pragma clang fp contract(off)
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     const Key k = split_at(Key{key[0], key[1]}, knum, kidx);
     key[0] = k.a;
     key[1] = k.b;
   }
+  if (blockIdx.x == 0 && br) bracket_clear_block(br, sel);
   // the mixture parameters (K * d <= 1024 each), staged once per workgroup
   __shared__ float smu[8 * 128], slam[8 * 128];
   {
@@ -1642,7 +1632,8 @@ int mocat_svgd_gmm_run(float* x, int64_t n, int64_t d, int K, const float* means
                          (const char*)ws, (int)n, 1.f, bandwidth, js_count, part_o, part_s);
     hipLaunchKernelGGL(svgd_update_gmm_kernel, dim3(upd_blocks), dim3(256), 0, stream, part_o, part_s, js_count,
                        (int)n, (int)d, DP, mean, 1.f / (float)n, bandwidth, step, momentum, x, gsq, mom, K, means, lam,
-                       logc, temperature, potential, grad, key, (uint32_t)(n + 2), (uint32_t)n);
+                       logc, temperature, potential, grad, key, (uint32_t)(n + 2), (uint32_t)n,
+                       bandwidth_mode == 2 ? br : nullptr, sel);
     prepare();
     float* tr = bandwidth_trace ? bandwidth_trace + it : nullptr;
     if (bandwidth_mode == 0) {
@@ -1657,11 +1648,9 @@ int mocat_svgd_gmm_run(float* x, int64_t n, int64_t d, int K, const float* means
       sqdist_packed_dp<128>(ws, (int)n, 1, d2, count_br, cand, stream);
     if (bandwidth_mode == 2) {
       const unsigned rows = (unsigned)std::min<int64_t>(n - 1, kSelectBlocks);
-      hipLaunchKernelGGL(radix_bracket_kernel, dim3(rows), dim3(kBracketThreads), 0, stream, (const uint32_t*)d2,
+      hipLaunchKernelGGL(bracket_select_kernel, dim3(rows), dim3(kBracketThreads), 0, stream, (const uint32_t*)d2,
                          (int)n, (const uint32_t*)cand, br, sel, (uint32_t)((r0 - n) / 2), (uint32_t)((r1 - n) / 2),
-                         polls);
-      hipLaunchKernelGGL(bracket_finish_kernel, dim3(1), dim3(kSelThreads), 0, stream, br, (const uint32_t*)cand, sel,
-                         norm, bandwidth, tr);
+                         norm, bandwidth, tr, polls);
     } else {
       const int rc = mocat_sqrt_sum_upper(d2, n, part, 1024, stream);
       if (rc != MOCAT_OK) return rc;
