@@ -676,15 +676,19 @@ __global__ __launch_bounds__(256) void gmm_full_kernel(const float* __restrict__
 // run over the candidates only, with the ranks shifted by the count below; otherwise over all pairs.
 // Either way the result is the exact order statistic. The bracket half-width (relative) shrinks
 // after hits and grows after misses.
+// The count below the bracket is striped over counters: every distance workgroup adds its count at
+// its end, and one shared counter would serialise those device-scope atomics.
+constexpr int kBelowSlots = 32;
 struct Bracket {
   uint32_t lo, hi;     // bit patterns of non-negative floats (ordered as unsigned ints)
-  uint32_t below;      // pairs < lo
+  uint32_t below[kBelowSlots];  // pairs < lo (workgroup k of the distance pass adds to slot k % kBelowSlots)
   uint32_t inside;     // pairs in [lo, hi) (the candidate count, possibly above the capacity)
   float delta;         // relative half-width of the next bracket
   uint32_t overflow;   // some workgroup had more candidates than its LDS buffer
   float prev;          // the previous selected value (0: none yet)
   uint32_t bar[2];     // grid barrier of the digit passes: arrivals, abort flag
 };
+static_assert(sizeof(Bracket) <= 256, "the bracket state has 256 bytes of the run workspace");
 
 constexpr uint32_t kBracketCap = 1u << 20;  // candidate capacity (4 MB)
 constexpr uint32_t kSmallCap = 32768;       // candidates selected by one workgroup in LDS (128 KB)
@@ -825,7 +829,7 @@ __global__ __launch_bounds__(kPhiThreads) void sqdist_kernel(const char* __restr
     __syncthreads();
     const uint32_t cnt = lcount;
     if (t == 0) {
-      if (lbelow) atomicAdd(&br->below, lbelow);
+      if (lbelow) atomicAdd(&br->below[blockIdx.x % kBelowSlots], lbelow);
       lbase = cnt ? atomicAdd(&br->inside, cnt) : 0u;
       if (cnt > (uint32_t)kLocal) atomicOr(&br->overflow, 1u);
     }
@@ -1072,7 +1076,9 @@ __global__ __launch_bounds__(kBracketThreads) void bracket_select_kernel(const u
   __shared__ Bracket bs;
   if (threadIdx.x == 0) bs = *br;
   __syncthreads();
-  const uint32_t below = bs.below, inside = bs.inside;
+  uint32_t below = 0u;
+  for (int k = 0; k < kBelowSlots; ++k) below += bs.below[k];
+  const uint32_t inside = bs.inside;
   const bool use_cand = !bs.overflow && below <= rank0 && (uint64_t)rank1 < (uint64_t)below + inside &&
                         inside <= kBracketCap;
   const bool small = use_cand && inside <= kSmallCap;
@@ -1182,7 +1188,7 @@ __global__ __launch_bounds__(kBracketThreads) void bracket_select_kernel(const u
 MOCAT_DEV void bracket_clear_block(Bracket* br, uint32_t* sel) {
   for (int b = threadIdx.x; b < kSelWords - kSelStateWords; b += blockDim.x) sel[kSelStateWords + b] = 0u;
   if (threadIdx.x == 0) {
-    br->below = 0u;
+    for (int k = 0; k < kBelowSlots; ++k) br->below[k] = 0u;
     br->inside = 0u;
     br->overflow = 0u;
     br->bar[0] = 0u;
@@ -1196,7 +1202,7 @@ __global__ __launch_bounds__(256) void bracket_init_kernel(Bracket* br, uint32_t
   if (threadIdx.x == 0) {
     br->lo = 0u;
     br->hi = 0u;
-    br->below = 0u;
+    for (int k = 0; k < kBelowSlots; ++k) br->below[k] = 0u;
     br->inside = 0u;
     br->overflow = 0u;
     br->prev = 0.f;
