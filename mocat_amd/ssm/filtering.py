@@ -405,6 +405,15 @@ def _key_np(k):
     return np.asarray(k, np.uint32).reshape(2)
 
 
+def _keys_np(keys):
+    """Every filter's key as a row of one (F, 2) uint32 host array. Device keys cross to the host in one
+    copy: a copy per key would synchronise with the device once per filter."""
+    if len(keys) > 1 and all(isinstance(k, torch.Tensor) for k in keys) and len({k.device for k in keys}) == 1:
+        rows = torch.stack([k.detach().reshape(2).view(torch.int32) for k in keys])
+        return rows.cpu().numpy().view(np.uint32)
+    return np.stack([_key_np(k) for k in keys]) if len(keys) else np.zeros((0, 2), np.uint32)
+
+
 def _large_flags(ns, Ts, large_n):
     """Which filters take the chunked path: those with at least large_n particles (default 4096:
     above it one workgroup per filter is slower than a few grid-wide launches per time step)."""
@@ -514,13 +523,14 @@ def _launch(models, ys, keys, ns, ess_threshold, resampling, store, grad=False, 
     order = sorted(range(F), key=lambda i: -(ns[i] * Ts[i]))  # longest first
     large = _large_flags(ns, Ts, large_n)
     desc = np.zeros(F, FILTER_DTYPE)
+    keys_host = _keys_np(keys)
     params, yflat, scales = [], [], []
     poff = yoff = soff = ooff = toff = 0
     for slot, i in enumerate(order):
         p = opt.kernel_params(models[i]) if optimal else models[i].kernel_params()
         yi = np.asarray(ys[i], np.float32).reshape(Ts[i], dy)
         resume = initial is not None and initial[i] is not None
-        desc[slot] = (ns[i], Ts[i], yoff, soff, ooff if store else -1, toff, _key_np(keys[i]), poff,
+        desc[slot] = (ns[i], Ts[i], yoff, soff, ooff if store else -1, toff, keys_host[i], poff,
                       (PF_LARGE if large[i] else 0) | (PF_RESUME if resume else 0) | extra_flags)
         params.append(p)
         yflat.append(yi.ravel())
