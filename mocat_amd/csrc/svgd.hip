@@ -293,9 +293,9 @@ constexpr int kRowUnroll = 4;  // row elements loaded per thread before they are
 // histogram bins, which serialise across workgroups, so one workgroup per compute unit.
 constexpr int kSelectBlocks = 256;
 constexpr int kSelThreads = 1024;  // 16 waves per compute unit to cover the load latency
-// The bracketed selection: 512 threads per workgroup, so that with its grid barriers the whole grid
-// (kSelectBlocks workgroups, three per compute unit) is resident with room to spare; eight row loads
-// in flight per thread on a pass over all pairs.
+// The bracketed selection: 512 threads per workgroup (three per compute unit, so kSelectBlocks of them
+// fit on a whole device with room to spare; the grid is also capped by what the device holds); eight
+// row loads in flight per thread on a pass over all pairs.
 constexpr int kBracketThreads = 512, kBracketUnroll = 8;
 
 // Lanes of a wave that share an 8-bit digit (ballots of the digit's bits); the lowest lane of each
@@ -1020,8 +1020,9 @@ MOCAT_DEV void select_small_block(const uint32_t* __restrict__ cand, uint32_t m,
 // bar[0] after a release fence; barrier k is passed once the counter reaches k * gridDim.x, followed by
 // an acquire fence. A workgroup that polls max_polls times without seeing every arrival sets the abort
 // flag bar[1], and every workgroup leaves at its next barrier once the flag is set. The grid (at most
-// kSelectBlocks workgroups of kBracketThreads threads, 2 KB of LDS each) is resident at once, so a
-// barrier only ever waits on running workgroups.
+// kSelectBlocks workgroups of kBracketThreads threads, 2 KB of LDS each, and no more than the device's
+// compute units times the kernel's occupancy) is resident at once, so a barrier only ever waits on
+// running workgroups.
 constexpr uint32_t kBracketBarrierPolls = 1u << 22;
 
 MOCAT_DEV bool bracket_grid_sync(uint32_t* bar, uint32_t k, uint32_t max_polls) {
@@ -1608,6 +1609,19 @@ int mocat_svgd_gmm_run(float* x, int64_t n, int64_t d, int K, const float* means
   // another; 0 makes every barrier time out, which tests use to check the NaN bandwidth)
   uint32_t polls = kBracketBarrierPolls;
   if (const char* e = std::getenv("MOCAT_SVGD_BARRIER_POLLS")) polls = (uint32_t)std::strtoul(e, nullptr, 10);
+  // the selection's grid: at most one workgroup per row, kSelectBlocks, and what the device holds at once
+  // (its grid barriers wait on every workgroup; a partitioned device has fewer compute units)
+  unsigned sel_rows = 1;
+  if (bandwidth_mode == 2) {
+    int dev = 0, cus = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(bracket_select_kernel),
+                                                     kBracketThreads, 0) != hipSuccess)
+      return MOCAT_ERR_HIP;
+    if (per_cu < 1 || cus < 1) return MOCAT_ERR_SIZE;
+    sel_rows = (unsigned)std::min<int64_t>(std::min<int64_t>(n - 1, kSelectBlocks), (int64_t)cus * per_cu);
+  }
   const double norm = std::sqrt(2.0 * std::log((double)n));
   const int64_t m = n * n;
   const int64_t r0 = (m - 1) / 2, r1 = m / 2;  // both >= n for n >= 3
@@ -1653,8 +1667,7 @@ int mocat_svgd_gmm_run(float* x, int64_t n, int64_t d, int K, const float* means
     else
       sqdist_packed_dp<128>(ws, (int)n, 1, d2, count_br, cand, stream);
     if (bandwidth_mode == 2) {
-      const unsigned rows = (unsigned)std::min<int64_t>(n - 1, kSelectBlocks);
-      hipLaunchKernelGGL(bracket_select_kernel, dim3(rows), dim3(kBracketThreads), 0, stream, (const uint32_t*)d2,
+      hipLaunchKernelGGL(bracket_select_kernel, dim3(sel_rows), dim3(kBracketThreads), 0, stream, (const uint32_t*)d2,
                          (int)n, (const uint32_t*)cand, br, sel, (uint32_t)((r0 - n) / 2), (uint32_t)((r1 - n) / 2),
                          norm, bandwidth, tr, polls);
     } else {
