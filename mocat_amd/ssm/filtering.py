@@ -408,7 +408,8 @@ def _key_np(k):
 def _keys_np(keys):
     """Every filter's key as a row of one (F, 2) uint32 host array. Device keys cross to the host in one
     copy: a copy per key would synchronise with the device once per filter."""
-    if len(keys) > 1 and all(isinstance(k, torch.Tensor) for k in keys) and len({k.device for k in keys}) == 1:
+    if (len(keys) > 1 and all(isinstance(k, torch.Tensor) and k.dtype in (torch.uint32, torch.int32) for k in keys)
+            and len({k.device for k in keys}) == 1):
         rows = torch.stack([k.detach().reshape(2).view(torch.int32) for k in keys])
         return rows.cpu().numpy().view(np.uint32)
     return np.stack([_key_np(k) for k in keys]) if len(keys) else np.zeros((0, 2), np.uint32)

@@ -56,3 +56,16 @@ def test_tuner_keeps_the_fastest_candidate_and_records_it(tmp_path, monkeypatch)
     assert sched.choose("sig", run) == rec and not calls
     sched._MEMO["old"] = {"large_n": 2048, "graph": False, "timings_ms": {}}
     assert sched.choose("old", run)["launch"] == "graph" and calls
+
+
+def test_filter_keys_cross_to_the_host_as_one_array():
+    # the batch launch copies every filter's key in one transfer; uint32 words above 2^31 keep their
+    # value, and keys of other integer types (or host arrays) take the per-key conversion
+    import torch
+    from mocat_amd.ssm.filtering import _key_np, _keys_np
+    k32 = [torch.tensor(np.array([1, 0xFFFFFFFF], np.uint32)), torch.tensor(np.array([0x80000000, 4], np.uint32))]
+    np.testing.assert_array_equal(_keys_np(k32), np.stack([_key_np(k) for k in k32]))
+    assert _keys_np(k32).dtype == np.uint32
+    k64 = [torch.tensor([1, 2], dtype=torch.int64), torch.tensor([3, 4], dtype=torch.int64)]
+    np.testing.assert_array_equal(_keys_np(k64), np.array([[1, 2], [3, 4]], np.uint32))
+    np.testing.assert_array_equal(_keys_np([np.array([5, 6], np.uint32)]), np.array([[5, 6]], np.uint32))
