@@ -297,3 +297,19 @@ def test_fused_median_barrier_time_out_gives_nan_bandwidth(m, monkeypatch):
     fused = m.run(sc, m.SVGD(**kw), n, m.random.PRNGKey(3))
     ref = m.run(sc, m.SVGD(**kw), n, m.random.PRNGKey(3), stack=("value",))
     np.testing.assert_array_equal(fused.bandwidth.cpu().numpy(), ref.bandwidth.cpu().numpy())
+
+
+@pytest.mark.parametrize("n", [3, 37, 130])
+def test_fused_median_small_ensembles(m, n):
+    """Few particles: the selection's grid is n - 1 workgroups (2 at n = 3), its first selection runs
+    over all pairs behind grid barriers and later ones on the bracket candidates; every bandwidth and
+    the final particles equal the stepwise full selection bit for bit."""
+    rs = np.random.default_rng(n)
+    d, iters = 7, 12
+    sc = m.GaussianMixture(rs.normal(0, 1, (2, d)), rs.uniform(0.5, 2, (2, d)), np.array([0.4, 0.6]))
+    kw = dict(stepsize=0.01, max_iter=iters, bandwidth_update="median")
+    fused = m.run(sc, m.SVGD(**kw), n, m.random.PRNGKey(2))
+    ref = m.run(sc, m.SVGD(**kw), n, m.random.PRNGKey(2), stack=("value",))
+    assert np.isfinite(fused.bandwidth.cpu().numpy()).all()
+    np.testing.assert_array_equal(fused.bandwidth.cpu().numpy(), ref.bandwidth.cpu().numpy())
+    np.testing.assert_array_equal(fused.final_state.value.cpu().numpy(), ref.final_state.value.cpu().numpy())
