@@ -313,3 +313,16 @@ def test_fused_median_small_ensembles(m, n):
     assert np.isfinite(fused.bandwidth.cpu().numpy()).all()
     np.testing.assert_array_equal(fused.bandwidth.cpu().numpy(), ref.bandwidth.cpu().numpy())
     np.testing.assert_array_equal(fused.final_state.value.cpu().numpy(), ref.final_state.value.cpu().numpy())
+
+
+def test_fused_median_large_ensemble(m):
+    """16384 particles: 8256 distance workgroups add to the striped count-below counters and append
+    candidates; the bandwidths and final particles equal the stepwise full selection bit for bit."""
+    rs = np.random.default_rng(5)
+    d, n, iters = 20, 16384, 4
+    sc = m.GaussianMixture(rs.normal(0, 1, (3, d)), rs.uniform(0.5, 2, (3, d)), rs.dirichlet(np.ones(3)))
+    kw = dict(stepsize=0.01, max_iter=iters, bandwidth_update="median")
+    fused = m.run(sc, m.SVGD(**kw), n, m.random.PRNGKey(6))
+    ref = m.run(sc, m.SVGD(**kw), n, m.random.PRNGKey(6), stack=("value",))
+    np.testing.assert_array_equal(fused.bandwidth.cpu().numpy(), ref.bandwidth.cpu().numpy())
+    np.testing.assert_array_equal(fused.final_state.value.cpu().numpy(), ref.final_state.value.cpu().numpy())
