@@ -764,20 +764,25 @@ __global__ __launch_bounds__(kPhiThreads) void sqdist_kernel(const char* __restr
     return c < ABYTES / 16 ? reinterpret_cast<const u32x4*>(blk) + c
                            : reinterpret_cast<const u32x4*>(blk + L::BLOCK_BF16 * 2) + (c - ABYTES / 16);
   };
-  u32x4 regs[PER];
-  const int nsteps = rb1 - rb0;
-  if (nsteps > 0) {
+  // blocks copied global -> LDS directly (16 bytes per lane: lane l of wave w copies chunk
+  // p * kPhiThreads + w * 64 + l to the same LDS chunk); the next block's copy overlaps the current
+  // block's products and is waited for before the step's closing barrier
+  auto stage_block = [&](int buf, int rb) {
 #pragma unroll
-    for (int p = 0; p < PER; ++p)
-      if (t + p * kPhiThreads < CHUNKS) stage[0][t + p * kPhiThreads] = *src_chunk(rb0, t + p * kPhiThreads);
-  }
+    for (int p = 0; p < PER; ++p) {
+      const int c = p * kPhiThreads + t;
+      if (c < CHUNKS)
+        __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)src_chunk(rb, c),
+                                         (__attribute__((address_space(3))) void*)(&stage[buf][c - lane]), 16, 0,
+                                         0);
+    }
+  };
+  const int nsteps = rb1 - rb0;
+  if (nsteps > 0) stage_block(0, rb0);
+  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
   __syncthreads();
   for (int st = 0; st < nsteps; ++st) {
-    if (st + 1 < nsteps) {
-#pragma unroll
-      for (int p = 0; p < PER; ++p)
-        if (t + p * kPhiThreads < CHUNKS) regs[p] = *src_chunk(rb0 + st + 1, t + p * kPhiThreads);
-    }
+    if (st + 1 < nsteps) stage_block((st + 1) & 1, rb0 + st + 1);
     const char* blk = reinterpret_cast<const char*>(stage[st & 1]);
     const bf16x8* A = reinterpret_cast<const bf16x8*>(blk);
     floatx16 z;
@@ -815,12 +820,7 @@ __global__ __launch_bounds__(kPhiThreads) void sqdist_kernel(const char* __restr
         }
       }
     }
-    if (st + 1 < nsteps) {
-      u32x4* dst = stage[(st + 1) & 1];
-#pragma unroll
-      for (int p = 0; p < PER; ++p)
-        if (t + p * kPhiThreads < CHUNKS) dst[t + p * kPhiThreads] = regs[p];
-    }
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
   }
   if constexpr (COUNT) {
